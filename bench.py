@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Benchmark of the B-spline MSE score+gradient (threading=hip) on one MI355X.
+
+One step = one score+gradient evaluation of the 256x256x128 MSE workload
+(1 x 1 x 2.5 mm, 10-voxel control grid, nearest-voxel moving gradient as in
+flavor g), with the images, moving gradient, coefficient vectors and
+outputs resident in HBM.  The coefficient vector cycles over 20 seeded
+N(0, 3 mm) draws.  Prints one JSON line.
+
+The dominant kernel (the fused tile kernel) is timed separately with HIP
+events on its launch stream; its roofline uses the algorithmic bytes of the
+flavor-g path: fixed 4 B + moving 4 B + moving gradient 12 B per voxel, plus
+the coefficient vector.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOAD = "mse_256x256x128_vpr10"
+HBM_PEAK_GBS = 8000.0
+N_COEFF_VECTORS = 20
+
+
+def alg_bytes(pair):
+    nvox = pair.fixed.size
+    return 20 * nvox + 4 * pair.true_coeff.size
+
+
+def cpu_baseline(pair, coeffs, gpu_results, budget_s=20.0):
+    """Time the oracle's flavor-g restatement (OpenMP) on a bounded sample of
+    the same workload, and report the GPU results' error against the FP64
+    oracle on the same coefficient vectors."""
+    import oracle
+    threads = min(16, len(os.sched_getaffinity(0)))
+    case = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn)
+    _ = case.m_grad
+    case.mse(coeffs[0], "g", nthreads=threads)          # warm-up
+    times = []
+    t_start = time.perf_counter()
+    for i in range(10):
+        t0 = time.perf_counter()
+        case.mse(coeffs[i % len(coeffs)], "g", nthreads=threads)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > budget_s:
+            break
+    med = float(np.median(times))
+    errs = []
+    for k, (score, grad) in gpu_results.items():
+        s64, g64, _ = case.mse_f64(coeffs[k], 0, nthreads=threads)
+        errs.append((abs(score - s64) / abs(s64), float(np.linalg.norm(grad - g64) / np.linalg.norm(g64))))
+    return {
+        "value": 1.0 / med,
+        "unit": "evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{len(times)} evals of the same workload after 1 warm-up, median "
+                  f"{med * 1e3:.1f} ms/eval, oracle flavor g (OpenMP, float as the reference)",
+        "gb_per_s": alg_bytes(pair) / med / 1e9,
+    }, errs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU split of this path is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from plastimatch_amd import synth
+    from plastimatch_amd.bspline import BsplineHip, Bspline_xform, Volume, volume_gradient
+
+    dev = torch.device("cuda:0")
+    pair = synth.make_config(WORKLOAD, seed=0)
+    fixed = Volume(torch.from_numpy(pair.fixed).to(dev), pair.spacing, pair.origin)
+    moving = Volume(torch.from_numpy(pair.moving).to(dev), pair.spacing, pair.origin)
+    mgrad = volume_gradient(moving)
+    bxf = Bspline_xform.for_volume(fixed, pair.vox_per_rgn)
+    st = BsplineHip(bxf, fixed, moving, mgrad)
+    coeffs = np.stack([synth.random_coeff(pair.dim, pair.vox_per_rgn, 3.0, seed=100 + i)
+                       for i in range(N_COEFF_VECTORS)])
+    d_coeffs = torch.from_numpy(coeffs).to(dev)
+    d_score = torch.zeros(N_COEFF_VECTORS, dtype=torch.float64, device=dev)
+    d_grad = torch.zeros(N_COEFF_VECTORS, bxf.num_coeff, dtype=torch.float64, device=dev)
+    d_nvox = torch.zeros(N_COEFF_VECTORS, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step(i):
+        k = i % N_COEFF_VECTORS
+        st.score_device(d_coeffs[k], d_score[k:k + 1], d_grad[k], d_nvox[k:k + 1], "mse", "ref_nn",
+                        stream=stream)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(i)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = args.steps / elapsed
+
+    # dominant-kernel timing with HIP events on the launch stream
+    st.set_timing(True)
+    for i in range(args.steps):
+        step(i)
+    ms, n = st.get_timing()
+    st.set_timing(False)
+    tile_ms = ms[0] / n
+    stage_ms = [float(x / n) for x in ms[:3]]
+    bytes_per_launch = alg_bytes(pair)
+    achieved = bytes_per_launch / (tile_ms * 1e-3) / 1e9
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"k_mse_tiles_{WORKLOAD}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "score+grad evals/s (single-pair HIP, MSE flavor g semantics)",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "dim": list(pair.dim), "spacing": list(pair.spacing),
+                   "vox_per_rgn": list(pair.vox_per_rgn), "grad_mode": "ref_nn",
+                   "coeff_vectors": N_COEFF_VECTORS, "num_tiles": st.num_tiles,
+                   "num_coeff": bxf.num_coeff},
+        "eval_gb_per_s": bytes_per_launch / (ms_per_step * 1e-3) / 1e9,
+        "stage_ms": {"k_mse_tiles": stage_ms[0], "k_mse_finalize": stage_ms[1],
+                     "k_grad_reduce": stage_ms[2]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_mse_tiles", "alg_bytes_per_launch": bytes_per_launch,
+                     "kernel_ms": tile_ms},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        torch.cuda.synchronize()
+        check = {}
+        for k in (0, 1):
+            check[k] = (float(d_score[k].item()), d_grad[k].cpu().numpy())
+        base, errs = cpu_baseline(pair, coeffs, check)
+        out["cpu_baseline"] = base
+        out["rel_err_vs_fp64"] = {"score": max(e[0] for e in errs), "grad_l2": max(e[1] for e in errs)}
+    st.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

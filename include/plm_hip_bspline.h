@@ -1,0 +1,191 @@
+/* plm_hip_bspline.h -- C ABI of libplmregisterhip.so, the MI355X (gfx950)
+ * B-spline score/gradient flavor ("threading=hip").
+ *
+ * The library replaces the compiled-out CUDA plugin ABI
+ * (src/register/cuda/bspline_cuda.h:123-336, loaded through
+ * LOAD_LIBRARY_SAFE / LOAD_SYMBOL, src/sys/cuda_delayload.h:26-59) with
+ * plain-old-data arguments and integer status codes.  Nothing in these
+ * signatures is a C++ class or a HIP type; streams travel as void*.
+ *
+ * Conventions, as in the reference:
+ *   - volumes are x-fastest (index i + dim0*(j + dim1*k),
+ *     src/base/volume_macros.h:13-23);
+ *   - coefficients and gradients are interleaved, coeff[3*knot+d], with
+ *     knot = (k*cdims1 + j)*cdims0 + i (src/base/bspline_interpolate.cxx:37-43);
+ *   - the moving-image gradient is float3 interleaved per voxel
+ *     (src/base/volume_grad.cxx:136-145).
+ *
+ * Every function returns PLM_HIP_OK (0) on success, a positive warning, or
+ * a negative error code; plm_hip_last_error() describes the last failure of
+ * the calling thread.  No function exits the process.
+ */
+#ifndef PLM_HIP_BSPLINE_H
+#define PLM_HIP_BSPLINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLM_HIP_ABI_VERSION 1
+
+/* status codes */
+#define PLM_HIP_OK                 0
+#define PLM_HIP_WARN_NO_OVERLAP    1   /* fewer than 1 voxel overlaps: score
+                                          FLT_MAX, gradient 0, as
+                                          bspline_score_normalize does
+                                          (src/register/bspline_mse.cxx:53-65) */
+#define PLM_HIP_ERR_ARG           -1
+#define PLM_HIP_ERR_HIP           -2
+#define PLM_HIP_ERR_GEOMETRY      -3
+#define PLM_HIP_ERR_UNSUPPORTED   -4
+#define PLM_HIP_ERR_NO_DEVICE     -5
+
+/* metric (src/register/similarity_metric_type.h) */
+#define PLM_METRIC_MSE 0
+#define PLM_METRIC_MI  1
+
+/* gradient semantics of the MSE dc/dv:
+ *   REF_NN -- diff * m_grad[round(mijk)], flavor g
+ *             (src/register/bspline_mse.cxx:679-681); needs moving_grad;
+ *   EXACT  -- diff * analytic trilinear derivative, flavor p
+ *             (src/register/bspline_mse.cxx:1316-1351); the exact adjoint. */
+#define PLM_GRAD_REF_NN 0
+#define PLM_GRAD_EXACT  1
+
+/* where a pointer lives */
+#define PLM_MEM_HOST   0
+#define PLM_MEM_DEVICE 1
+
+/* how the library keeps an image in HBM (arithmetic is always FP32) */
+#define PLM_STORE_F32  0
+#define PLM_STORE_BF16 1
+
+/* B-spline grid: the arguments of Bspline_header::set
+ * (src/base/bspline_header.cxx:28-62). rdims/cdims are derived. */
+typedef struct plm_bsp_geom {
+    float img_origin[3];
+    float img_spacing[3];
+    int64_t img_dim[3];
+    int64_t roi_offset[3];
+    int64_t roi_dim[3];
+    int64_t vox_per_rgn[3];
+    float direction_cosines[9];
+} plm_bsp_geom;
+
+/* One image: the Volume fields the path reads (src/base/volume.h:39-57).
+ * data is float32 (float3 for moving_grad).  step/proj are derived from
+ * direction_cosines and spacing as compute_direction_matrices does
+ * (src/base/direction_matrices.cxx:10-20). */
+typedef struct plm_img_desc {
+    const void* data;
+    int32_t location;          /* PLM_MEM_HOST or PLM_MEM_DEVICE */
+    int32_t storage;           /* PLM_STORE_F32 or PLM_STORE_BF16 */
+    int64_t dim[3];
+    float origin[3];
+    float spacing[3];
+    float direction_cosines[9];
+} plm_img_desc;
+
+/* Equal-spacing joint histogram (src/register/histogram.h:19-35,
+ * src/register/joint_histogram.cxx:115-139). */
+typedef struct plm_hist_desc {
+    int64_t fixed_bins;
+    int64_t moving_bins;
+    float fixed_offset;
+    float fixed_delta;
+    float moving_offset;
+    float moving_delta;
+} plm_hist_desc;
+
+typedef struct plm_hip_ctx plm_hip_ctx;
+
+int plm_hip_abi_version (void);
+const char* plm_hip_last_error (void);
+
+/* Replaces CUDA_bspline_mse_init_j / CUDA_bspline_mi_init_a
+ * (src/register/cuda/bspline_cuda.h:123-336, bspline_cuda.cu:388-682) and
+ * the per-eval bspline_cuda_state_create
+ * (src/register/bspline_state.cxx:165-237): uploads the images once per
+ * stage, builds the basis LUTs and the per-tile work buffers.
+ * moving_grad may be NULL (then only PLM_GRAD_EXACT and MI are
+ * available); hist may be NULL (then MI is unavailable). */
+int plm_hip_bspline_create (
+    const plm_bsp_geom* geom,
+    const plm_img_desc* fixed,
+    const plm_img_desc* moving,
+    const plm_img_desc* moving_grad,
+    const plm_hist_desc* hist,
+    int gpuid,
+    plm_hip_ctx** out);
+
+/* Replaces CUDA_bspline_mse_j / CUDA_bspline_mi_a
+ * (src/register/cuda/bspline_cuda.cxx:607-801), i.e. one evaluation of
+ * bspline_score_g_mse (src/register/bspline_mse.cxx:519-720),
+ * bspline_score_p_mse (:1149-1384) or bspline_score_g_mi
+ * (src/register/bspline_mi.cxx:1505-1737).  Host pointers; blocks until
+ * done.  grad has 3*num_knots doubles (already normalized: 2/N for MSE).
+ * hist_out, if not NULL, receives Bf*Bm joint bins (units of 2^-32 voxel)
+ * followed by Bf fixed-bin counts (MI only). */
+int plm_hip_bspline_score (
+    plm_hip_ctx* ctx,
+    int metric,
+    int grad_mode,
+    const float* coeff,
+    double* score,
+    double* grad,
+    int64_t* num_vox,
+    int64_t* hist_out);
+
+/* The same evaluation with every pointer in device memory, enqueued on
+ * stream (a hipStream_t, or NULL for the context's own stream); returns
+ * without waiting.  No counterpart in the reference: it is what an
+ * optimizer loop that keeps coefficients and gradient resident calls. */
+int plm_hip_bspline_score_device (
+    plm_hip_ctx* ctx,
+    int metric,
+    int grad_mode,
+    const float* d_coeff,
+    double* d_score,
+    double* d_grad,
+    int64_t* d_num_vox,
+    int64_t* d_hist,
+    void* stream);
+
+/* Replaces CUDA_bspline_mse_cleanup_j / CUDA_bspline_mi_cleanup_a and
+ * bspline_cuda_state_destroy (src/register/bspline_state.cxx:239-271). */
+int plm_hip_bspline_destroy (plm_hip_ctx* ctx);
+
+/* Moving-image gradient for PLM_GRAD_REF_NN, computed on the GPU:
+ * central differences with edge replicate, rotated by inv(dc), float3
+ * interleaved -- the volume_gradient the stage builds once per stage
+ * (src/base/volume_grad.cxx:9-76,136-145, called at
+ * src/register/registration_data.cxx:383).  vol->data and out may be host
+ * or device pointers (out_location says which); out holds 3*npix floats. */
+int plm_hip_volume_gradient (
+    const plm_img_desc* vol,
+    float* out,
+    int out_location,
+    int gpuid);
+
+/* Per-stage kernel timing with HIP events recorded on the launch stream,
+ * the counterpart of the CUDA flavor's PROFILE_MSE stage timers
+ * (src/register/cuda/bspline_cuda.h:28, bspline_cuda.cu:1371-1436).
+ * While enabled, every evaluation records its stage boundaries;
+ * get_timing waits for them and returns the summed milliseconds per stage
+ * (MSE: tiles, finalize, reduce; MI: zero, histogram, score, tiles,
+ * reduce) and the number of evaluations, then starts a new window.  One
+ * window should hold one metric. */
+int plm_hip_bspline_set_timing (plm_hip_ctx* ctx, int enable);
+int plm_hip_bspline_get_timing (plm_hip_ctx* ctx, double* stage_ms, int max_stages,
+    int64_t* evals);
+
+/* Geometry queries, for the host adapter. */
+int plm_hip_bspline_num_knots (const plm_hip_ctx* ctx, int64_t* num_knots);
+int plm_hip_bspline_num_tiles (const plm_hip_ctx* ctx, int64_t* num_tiles);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,248 @@
+"""ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+The oracle restates the reference's CPU flavors (plm_oracle.c) and is used
+by tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg as
+the checker.  The product package never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+c_float3 = ctypes.c_float * 3
+c_int64_3 = ctypes.c_int64 * 3
+c_float9 = ctypes.c_float * 9
+IDENTITY = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+
+
+class OrcVolHdr(ctypes.Structure):
+    _fields_ = [("dim", c_int64_3), ("origin", c_float3), ("spacing", c_float3), ("dc", c_float9)]
+
+
+class OrcGeom(ctypes.Structure):
+    _fields_ = [("img_origin", c_float3), ("img_spacing", c_float3), ("img_dim", c_int64_3),
+                ("roi_offset", c_int64_3), ("roi_dim", c_int64_3), ("vox_per_rgn", c_int64_3),
+                ("dc", c_float9)]
+
+
+class OrcHist(ctypes.Structure):
+    _fields_ = [("fixed_bins", ctypes.c_int64), ("moving_bins", ctypes.c_int64),
+                ("f_offset", ctypes.c_float), ("f_delta", ctypes.c_float),
+                ("m_offset", ctypes.c_float), ("m_delta", ctypes.c_float)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        L.orc_header.restype = ctypes.c_int64
+        L.orc_header.argtypes = [ctypes.POINTER(OrcGeom), P, P]
+        L.orc_basis_lut.argtypes = [ctypes.c_int64, P]
+        L.orc_direction_matrices.argtypes = [P, P, P, P, P]
+        L.orc_interp_vf.argtypes = [ctypes.POINTER(OrcGeom), P, P]
+        L.orc_volume_gradient.argtypes = [ctypes.POINTER(OrcVolHdr), P, P]
+        L.orc_hist_eqsp.argtypes = [P, ctypes.c_int64, ctypes.c_int64, P, P]
+        L.orc_pass1_position_mismatch.restype = ctypes.c_int64
+        L.orc_pass1_position_mismatch.argtypes = [ctypes.POINTER(OrcVolHdr), ctypes.POINTER(OrcGeom)]
+        L.orc_mse_score.restype = ctypes.c_int
+        L.orc_mse_score.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
+                                    ctypes.POINTER(OrcVolHdr), P, P, P, ctypes.c_int, ctypes.c_int,
+                                    P, P, P, P]
+        L.orc_mse_score_f64.restype = ctypes.c_int
+        L.orc_mse_score_f64.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
+                                        ctypes.POINTER(OrcVolHdr), P, P, P, ctypes.c_int, ctypes.c_int,
+                                        P, P, P]
+        L.orc_pv_quantize.argtypes = [P, P]
+        L.orc_mi_score.restype = ctypes.c_int
+        L.orc_mi_score.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
+                                   ctypes.POINTER(OrcVolHdr), P, P, ctypes.POINTER(OrcHist),
+                                   ctypes.c_int, P, P, P, P, P, P, P, P]
+        L.orc_mi_score_f64.restype = ctypes.c_int
+        L.orc_mi_score_f64.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
+                                       ctypes.POINTER(OrcVolHdr), P, P, ctypes.POINTER(OrcHist),
+                                       ctypes.c_int, P, P, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def vol_hdr(img, spacing, origin=(0.0, 0.0, 0.0), dc=IDENTITY):
+    nz, ny, nx = img.shape[:3]
+    return OrcVolHdr(c_int64_3(nx, ny, nz), c_float3(*origin), c_float3(*spacing), c_float9(*dc))
+
+
+def geom(dim, spacing, vox_per_rgn, origin=(0.0, 0.0, 0.0), roi_offset=(0, 0, 0), roi_dim=None,
+         dc=IDENTITY):
+    roi_dim = roi_dim or dim
+    return OrcGeom(c_float3(*origin), c_float3(*spacing), c_int64_3(*dim), c_int64_3(*roi_offset),
+                   c_int64_3(*roi_dim), c_int64_3(*vox_per_rgn), c_float9(*dc))
+
+
+def header(g):
+    r = np.zeros(3, np.int64)
+    c = np.zeros(3, np.int64)
+    nk = lib().orc_header(ctypes.byref(g), _ptr(r), _ptr(c))
+    return r, c, nk
+
+
+def basis_lut(vpr):
+    out = np.zeros(4 * vpr, np.float32)
+    lib().orc_basis_lut(vpr, _ptr(out))
+    return out.reshape(vpr, 4)
+
+
+def direction_matrices(dc, spacing):
+    dc = np.asarray(dc, np.float32)
+    sp = np.asarray(spacing, np.float32)
+    step = np.zeros(9, np.float32)
+    proj = np.zeros(9, np.float32)
+    inv = np.zeros(9, np.float32)
+    lib().orc_direction_matrices(_ptr(dc), _ptr(sp), _ptr(step), _ptr(proj), _ptr(inv))
+    return step, proj, inv
+
+
+def interp_vf(g, coeff):
+    nx, ny, nz = g.img_dim
+    vf = np.zeros((nz, ny, nx, 3), np.float32)
+    lib().orc_interp_vf(ctypes.byref(g), _ptr(np.ascontiguousarray(coeff, np.float32)), _ptr(vf))
+    return vf
+
+
+def volume_gradient(img, spacing, origin=(0.0, 0.0, 0.0), dc=IDENTITY):
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros(img.shape + (3,), np.float32)
+    h = vol_hdr(img, spacing, origin, dc)
+    lib().orc_volume_gradient(ctypes.byref(h), _ptr(img), _ptr(out))
+    return out
+
+
+def hist_eqsp(img, bins):
+    img = np.ascontiguousarray(img, np.float32)
+    off = np.zeros(1, np.float32)
+    dl = np.zeros(1, np.float32)
+    lib().orc_hist_eqsp(_ptr(img), img.size, bins, _ptr(off), _ptr(dl))
+    return float(off[0]), float(dl[0])
+
+
+def pass1_position_mismatch(fixed, spacing, g, origin=(0.0, 0.0, 0.0), dc=IDENTITY):
+    h = vol_hdr(fixed, spacing, origin, dc)
+    return lib().orc_pass1_position_mismatch(ctypes.byref(h), ctypes.byref(g))
+
+
+def pv_quantize(w):
+    w = np.ascontiguousarray(w, np.float32)
+    q = np.zeros(8, np.int64)
+    lib().orc_pv_quantize(_ptr(w), _ptr(q))
+    return q
+
+
+class Case:
+    """A fixed/moving pair on one B-spline grid, in the oracle's terms."""
+
+    def __init__(self, fixed, moving, spacing, vox_per_rgn, origin=(0.0, 0.0, 0.0),
+                 roi_offset=(0, 0, 0), roi_dim=None, dc=IDENTITY, m_spacing=None, m_origin=None,
+                 m_dc=None):
+        self.fixed = np.ascontiguousarray(fixed, np.float32)
+        self.moving = np.ascontiguousarray(moving, np.float32)
+        nz, ny, nx = self.fixed.shape
+        self.dim = (nx, ny, nz)
+        self.spacing = tuple(spacing)
+        self.origin = tuple(origin)
+        self.dc = tuple(dc)
+        self.m_spacing = tuple(m_spacing or spacing)
+        self.m_origin = tuple(m_origin or origin)
+        self.m_dc = tuple(m_dc or dc)
+        self.g = geom(self.dim, spacing, vox_per_rgn, origin, roi_offset, roi_dim, dc)
+        self.fh = vol_hdr(self.fixed, spacing, origin, dc)
+        self.mh = vol_hdr(self.moving, self.m_spacing, self.m_origin, self.m_dc)
+        self.rdims, self.cdims, self.num_knots = header(self.g)
+        self.num_coeff = 3 * self.num_knots
+        self._m_grad = None
+
+    @property
+    def m_grad(self):
+        if self._m_grad is None:
+            self._m_grad = volume_gradient(self.moving, self.m_spacing, self.m_origin, self.m_dc)
+        return self._m_grad
+
+    def mse(self, coeff, flavor="g", nthreads=0):
+        """Float-faithful flavor g/p/c: returns (score, grad float32, num_vox, raw)."""
+        coeff = np.ascontiguousarray(coeff, np.float32)
+        grad = np.zeros(self.num_coeff, np.float32)
+        raw = np.zeros(1, np.float64)
+        sc = np.zeros(1, np.float32)
+        nv = np.zeros(1, np.int64)
+        mg = self.m_grad if flavor in ("g", "c") else None
+        rc = lib().orc_mse_score(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
+                                 ctypes.byref(self.mh), _ptr(self.moving), _ptr(mg), _ptr(coeff),
+                                 ord(flavor), nthreads, _ptr(raw), _ptr(sc), _ptr(grad), _ptr(nv))
+        assert rc == 0, rc
+        return float(sc[0]), grad, int(nv[0]), float(raw[0])
+
+    def mse_f64(self, coeff, grad_mode=0, nthreads=0):
+        """FP64 yardstick: returns (score, grad float64, num_vox)."""
+        coeff = np.ascontiguousarray(coeff, np.float32)
+        grad = np.zeros(self.num_coeff, np.float64)
+        sc = np.zeros(1, np.float64)
+        nv = np.zeros(1, np.int64)
+        mg = self.m_grad if grad_mode == 0 else None
+        rc = lib().orc_mse_score_f64(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
+                                     ctypes.byref(self.mh), _ptr(self.moving), _ptr(mg), _ptr(coeff),
+                                     grad_mode, nthreads, _ptr(sc), _ptr(grad), _ptr(nv))
+        assert rc == 0, rc
+        return float(sc[0]), grad, int(nv[0])
+
+    def hist(self, bins=32, f_bins=None):
+        fb = f_bins or bins
+        fo, fd = hist_eqsp(self.fixed, fb)
+        mo, md = hist_eqsp(self.moving, bins)
+        return OrcHist(fb, bins, fo, fd, mo, md)
+
+    def mi(self, coeff, hp, nthreads=0):
+        """Float-faithful flavor g MI: returns dict with score, grad, num_vox,
+        double histograms and the fixed-point (2^-32) histograms."""
+        coeff = np.ascontiguousarray(coeff, np.float32)
+        fb, mb = hp.fixed_bins, hp.moving_bins
+        grad = np.zeros(self.num_coeff, np.float32)
+        sc = np.zeros(1, np.float32)
+        nv = np.zeros(1, np.int64)
+        fh = np.zeros(fb, np.float64)
+        mh = np.zeros(mb, np.float64)
+        jh = np.zeros(fb * mb, np.float64)
+        fxf = np.zeros(fb, np.int64)
+        fxj = np.zeros(fb * mb, np.int64)
+        rc = lib().orc_mi_score(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
+                                ctypes.byref(self.mh), _ptr(self.moving), _ptr(coeff), ctypes.byref(hp),
+                                nthreads, _ptr(sc), _ptr(grad), _ptr(nv), _ptr(fh), _ptr(mh), _ptr(jh),
+                                _ptr(fxf), _ptr(fxj))
+        assert rc == 0, rc
+        return dict(score=float(sc[0]), grad=grad, num_vox=int(nv[0]), f_hist=fh, m_hist=mh,
+                    j_hist=jh, fx_f_hist=fxf, fx_j_hist=fxj)
+
+    def mi_f64(self, coeff, hp, nthreads=0):
+        coeff = np.ascontiguousarray(coeff, np.float32)
+        grad = np.zeros(self.num_coeff, np.float64)
+        sc = np.zeros(1, np.float64)
+        nv = np.zeros(1, np.int64)
+        rc = lib().orc_mi_score_f64(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
+                                    ctypes.byref(self.mh), _ptr(self.moving), _ptr(coeff),
+                                    ctypes.byref(hp), nthreads, _ptr(sc), _ptr(grad), _ptr(nv))
+        assert rc == 0, rc
+        return float(sc[0]), grad, int(nv[0])

@@ -1,0 +1,96 @@
+/* CPU oracle for the B-spline score/gradient path (TEST INFRASTRUCTURE).
+ *
+ * This header and plm_oracle.c restate, in plain C99 + OpenMP, the CPU
+ * flavors of Plastimatch's B-spline MSE and Mattes-MI score and gradient.
+ * They exist only to check the HIP product path: the tests, the smoke
+ * check and the CPU-baseline leg of bench.py may load liboracle.so; the
+ * product library (libplmregisterhip.so) never does.
+ *
+ * Float semantics follow the reference's float operations one by one
+ * (compiled with -ffp-contract=off so no FMA is formed); the *_f64 entry
+ * points redo the same algorithm in double precision and are the FP64
+ * yardstick the relative errors are reported against.
+ */
+#ifndef PLM_ORACLE_H
+#define PLM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Volume header: mirrors the Volume fields the path reads
+   (src/base/volume.h:39-57). */
+typedef struct orc_vol_hdr {
+    int64_t dim[3];
+    float origin[3];
+    float spacing[3];
+    float dc[9];            /* direction cosines, row-major */
+} orc_vol_hdr;
+
+/* B-spline grid: the arguments of Bspline_header::set
+   (src/base/bspline_header.cxx:28-62). */
+typedef struct orc_geom {
+    float img_origin[3];
+    float img_spacing[3];
+    int64_t img_dim[3];
+    int64_t roi_offset[3];
+    int64_t roi_dim[3];
+    int64_t vox_per_rgn[3];
+    float dc[9];
+} orc_geom;
+
+/* Equal-spacing histogram parameters (src/register/histogram.h:19-35). */
+typedef struct orc_hist {
+    int64_t fixed_bins;
+    int64_t moving_bins;
+    float f_offset, f_delta;
+    float m_offset, m_delta;
+} orc_hist;
+
+int64_t orc_header (const orc_geom* g, int64_t rdims[3], int64_t cdims[3]);
+void orc_basis_lut (int64_t vpr, float* lut);
+void orc_direction_matrices (const float dc[9], const float spacing[3],
+    float step[9], float proj[9], float inv_dc[9]);
+void orc_interp_vf (const orc_geom* g, const float* coeff, float* vf);
+void orc_volume_gradient (const orc_vol_hdr* h, const float* img, float* out);
+void orc_hist_eqsp (const float* img, int64_t npix, int64_t bins,
+    float* offset, float* delta);
+int64_t orc_pass1_position_mismatch (const orc_vol_hdr* fh, const orc_geom* g);
+
+/* flavor: 'g' (tile condense, nearest-voxel m_grad), 'p' (tile condense,
+   analytic trilinear derivative), 'c' (voxel-order scatter, m_grad). */
+int orc_mse_score (const orc_geom* g,
+    const orc_vol_hdr* fh, const float* f_img,
+    const orc_vol_hdr* mh, const float* m_img, const float* m_grad,
+    const float* coeff, int flavor, int nthreads,
+    double* raw_score, float* score, float* grad, int64_t* num_vox);
+
+/* grad_mode 0 = nearest-voxel m_grad (flavor g), 1 = analytic (flavor p). */
+int orc_mse_score_f64 (const orc_geom* g,
+    const orc_vol_hdr* fh, const float* f_img,
+    const orc_vol_hdr* mh, const float* m_img, const float* m_grad,
+    const float* coeff, int grad_mode, int nthreads,
+    double* score, double* grad, int64_t* num_vox);
+
+void orc_pv_quantize (const float w[8], int64_t q[8]);
+
+int orc_mi_score (const orc_geom* g,
+    const orc_vol_hdr* fh, const float* f_img,
+    const orc_vol_hdr* mh, const float* m_img,
+    const float* coeff, const orc_hist* hp, int nthreads,
+    float* score, float* grad, int64_t* num_vox,
+    double* f_hist, double* m_hist, double* j_hist,
+    int64_t* fx_f_hist, int64_t* fx_j_hist);
+
+int orc_mi_score_f64 (const orc_geom* g,
+    const orc_vol_hdr* fh, const float* f_img,
+    const orc_vol_hdr* mh, const float* m_img,
+    const float* coeff, const orc_hist* hp, int nthreads,
+    double* score, double* grad, int64_t* num_vox);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,125 @@
+"""ctypes binding of libplmregisterhip.so (the C ABI in include/plm_hip_bspline.h).
+
+This is the Python-side counterpart of the ``LOAD_LIBRARY_SAFE`` /
+``LOAD_SYMBOL`` plugin loader of the reference (src/sys/cuda_delayload.h:26-59):
+it loads the HIP flavor library from the package directory and declares
+every exported symbol.  There is no fallback: if the library is missing,
+``load()`` raises.
+"""
+import ctypes
+import os
+
+LIB_NAME = "libplmregisterhip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+PLM_HIP_OK = 0
+PLM_HIP_WARN_NO_OVERLAP = 1
+PLM_HIP_ERR_ARG = -1
+PLM_HIP_ERR_HIP = -2
+PLM_HIP_ERR_GEOMETRY = -3
+PLM_HIP_ERR_UNSUPPORTED = -4
+PLM_HIP_ERR_NO_DEVICE = -5
+
+PLM_METRIC_MSE = 0
+PLM_METRIC_MI = 1
+PLM_GRAD_REF_NN = 0
+PLM_GRAD_EXACT = 1
+PLM_MEM_HOST = 0
+PLM_MEM_DEVICE = 1
+PLM_STORE_F32 = 0
+PLM_STORE_BF16 = 1
+
+c_float3 = ctypes.c_float * 3
+c_int64_3 = ctypes.c_int64 * 3
+c_float9 = ctypes.c_float * 9
+
+
+class PlmBspGeom(ctypes.Structure):
+    _fields_ = [
+        ("img_origin", c_float3),
+        ("img_spacing", c_float3),
+        ("img_dim", c_int64_3),
+        ("roi_offset", c_int64_3),
+        ("roi_dim", c_int64_3),
+        ("vox_per_rgn", c_int64_3),
+        ("direction_cosines", c_float9),
+    ]
+
+
+class PlmImgDesc(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("location", ctypes.c_int32),
+        ("storage", ctypes.c_int32),
+        ("dim", c_int64_3),
+        ("origin", c_float3),
+        ("spacing", c_float3),
+        ("direction_cosines", c_float9),
+    ]
+
+
+class PlmHistDesc(ctypes.Structure):
+    _fields_ = [
+        ("fixed_bins", ctypes.c_int64),
+        ("moving_bins", ctypes.c_int64),
+        ("fixed_offset", ctypes.c_float),
+        ("fixed_delta", ctypes.c_float),
+        ("moving_offset", ctypes.c_float),
+        ("moving_delta", ctypes.c_float),
+    ]
+
+
+# every symbol the header declares: name -> (restype, argtypes)
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "plm_hip_abi_version": (ctypes.c_int, []),
+    "plm_hip_last_error": (ctypes.c_char_p, []),
+    "plm_hip_bspline_create": (ctypes.c_int, [
+        ctypes.POINTER(PlmBspGeom), ctypes.POINTER(PlmImgDesc), ctypes.POINTER(PlmImgDesc),
+        ctypes.POINTER(PlmImgDesc), ctypes.POINTER(PlmHistDesc), ctypes.c_int,
+        ctypes.POINTER(_P)]),
+    "plm_hip_bspline_score": (ctypes.c_int, [
+        _P, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P]),
+    "plm_hip_bspline_score_device": (ctypes.c_int, [
+        _P, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P, _P]),
+    "plm_hip_bspline_destroy": (ctypes.c_int, [_P]),
+    "plm_hip_volume_gradient": (ctypes.c_int, [ctypes.POINTER(PlmImgDesc), _P, ctypes.c_int,
+                                               ctypes.c_int]),
+    "plm_hip_bspline_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "plm_hip_bspline_get_timing": (ctypes.c_int, [_P, _P, ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_int64)]),
+    "plm_hip_bspline_num_knots": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "plm_hip_bspline_num_tiles": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+}
+
+_lib = None
+
+
+class PlmHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"plm_hip error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load the HIP flavor library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc < 0:
+        msg = load().plm_hip_last_error()
+        raise PlmHipError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,257 @@
+"""Python host side of the ``threading=hip`` B-spline flavor.
+
+Mirrors the reference objects the score path reads and writes:
+
+* ``Volume``          -- src/base/volume.h:39-57 (dim, origin, spacing,
+                         direction cosines, voxels x-fastest);
+* ``Bspline_xform``   -- src/base/bspline_xform.h:22-96 / bspline_header.cxx:28-62
+                         (grid geometry and interleaved coefficients);
+* ``Joint_histogram`` -- src/register/joint_histogram.h:14-46 (EQSP bins);
+* ``Bspline_score``   -- src/register/bspline_score.h:33-95 (curr_smetric,
+                         curr_smetric_grad, curr_num_vox);
+* ``BsplineHip``      -- the device state the reference keeps behind
+                         ``Bspline_state::dev_ptrs`` (bspline_state.cxx:165-271),
+                         created once per stage, scored once per evaluation.
+
+Every evaluation runs on the GPU through libplmregisterhip.so; there is no
+CPU path here.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+
+IDENTITY = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+
+METRICS = {"mse": _abi.PLM_METRIC_MSE, "mi": _abi.PLM_METRIC_MI}
+GRAD_MODES = {"ref_nn": _abi.PLM_GRAD_REF_NN, "exact": _abi.PLM_GRAD_EXACT}
+STORAGE = {"f32": _abi.PLM_STORE_F32, "bf16": _abi.PLM_STORE_BF16}
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+class Volume:
+    """A float image (numpy array or CUDA torch tensor), shape (nz, ny, nx[, 3])."""
+
+    def __init__(self, img, spacing, origin=(0.0, 0.0, 0.0), direction_cosines=IDENTITY):
+        self.img = img if _is_torch(img) else np.ascontiguousarray(img, np.float32)
+        nz, ny, nx = img.shape[:3]
+        self.dim = (int(nx), int(ny), int(nz))
+        self.spacing = tuple(float(s) for s in spacing)
+        self.origin = tuple(float(o) for o in origin)
+        self.direction_cosines = tuple(float(d) for d in direction_cosines)
+
+    @property
+    def npix(self):
+        return self.dim[0] * self.dim[1] * self.dim[2]
+
+    def desc(self, storage="f32"):
+        if _is_torch(self.img):
+            if not self.img.is_cuda or not self.img.is_contiguous():
+                raise ValueError("device images must be contiguous CUDA tensors")
+            ptr, loc = self.img.data_ptr(), _abi.PLM_MEM_DEVICE
+        else:
+            ptr, loc = self.img.ctypes.data, _abi.PLM_MEM_HOST
+        return _abi.PlmImgDesc(ctypes.c_void_p(ptr), loc, STORAGE[storage], _abi.c_int64_3(*self.dim),
+                               _abi.c_float3(*self.origin), _abi.c_float3(*self.spacing),
+                               _abi.c_float9(*self.direction_cosines))
+
+
+class Bspline_xform:
+    """B-spline grid over a fixed image (Bspline_header::set semantics)."""
+
+    def __init__(self, img_origin, img_spacing, img_dim, vox_per_rgn, roi_offset=(0, 0, 0),
+                 roi_dim=None, direction_cosines=IDENTITY):
+        self.img_origin = tuple(float(v) for v in img_origin)
+        self.img_spacing = tuple(float(v) for v in img_spacing)
+        self.img_dim = tuple(int(v) for v in img_dim)
+        self.vox_per_rgn = tuple(int(v) for v in vox_per_rgn)
+        self.roi_offset = tuple(int(v) for v in roi_offset)
+        self.roi_dim = tuple(int(v) for v in (roi_dim or img_dim))
+        self.direction_cosines = tuple(direction_cosines)
+        self.grid_spac = tuple(v * abs(s) for v, s in zip(self.vox_per_rgn, self.img_spacing))
+        self.rdims = tuple(1 + (r - 1) // v for r, v in zip(self.roi_dim, self.vox_per_rgn))
+        self.cdims = tuple(r + 3 for r in self.rdims)
+        self.num_knots = self.cdims[0] * self.cdims[1] * self.cdims[2]
+        self.num_coeff = 3 * self.num_knots
+        self.coeff = np.zeros(self.num_coeff, np.float32)
+
+    @classmethod
+    def for_volume(cls, vol, vox_per_rgn, **kw):
+        return cls(vol.origin, vol.spacing, vol.dim, vox_per_rgn,
+                   direction_cosines=vol.direction_cosines, **kw)
+
+    def jitter_if_zero(self):
+        """All-zero coefficients become 0.01 (src/base/bspline_xform.cxx:630-644)."""
+        if not np.any(self.coeff != 0.0):
+            self.coeff[:] = np.float32(0.01)
+
+    def geom(self):
+        return _abi.PlmBspGeom(_abi.c_float3(*self.img_origin), _abi.c_float3(*self.img_spacing),
+                               _abi.c_int64_3(*self.img_dim), _abi.c_int64_3(*self.roi_offset),
+                               _abi.c_int64_3(*self.roi_dim), _abi.c_int64_3(*self.vox_per_rgn),
+                               _abi.c_float9(*self.direction_cosines))
+
+
+class Joint_histogram:
+    """Equal-spacing joint histogram parameters (src/register/joint_histogram.cxx:115-139)."""
+
+    def __init__(self, fixed_bins=32, moving_bins=32):
+        self.fixed_bins = int(fixed_bins)
+        self.moving_bins = int(moving_bins)
+        self.fixed_offset = self.fixed_delta = 0.0
+        self.moving_offset = self.moving_delta = 0.0
+
+    @staticmethod
+    def _eqsp(img, bins):
+        a = img.detach().cpu().numpy() if _is_torch(img) else img
+        mn = np.float32(a.min())
+        mx = np.float32(a.max())
+        delta = np.float32((mx - mn) / np.float32(bins - 1))
+        offset = np.float32(np.float64(mn) - 0.5 * np.float64(delta))
+        return float(offset), float(delta)
+
+    def initialize(self, fixed, moving):
+        self.fixed_offset, self.fixed_delta = self._eqsp(fixed.img, self.fixed_bins)
+        self.moving_offset, self.moving_delta = self._eqsp(moving.img, self.moving_bins)
+        return self
+
+    def desc(self):
+        return _abi.PlmHistDesc(self.fixed_bins, self.moving_bins, self.fixed_offset,
+                                self.fixed_delta, self.moving_offset, self.moving_delta)
+
+
+@dataclass
+class Bspline_score:
+    """One evaluation's result: curr_smetric / curr_smetric_grad / curr_num_vox."""
+    score: float
+    grad: np.ndarray          # float64, interleaved, already normalized
+    num_vox: int
+    hist: np.ndarray = None   # MI: Bf*Bm joint bins (2^-32 units) then Bf counts
+    status: int = 0
+
+    @property
+    def curr_smetric(self):
+        return np.float32(self.score)
+
+    @property
+    def curr_smetric_grad(self):
+        return self.grad.astype(np.float32)
+
+
+def volume_gradient(vol, gpuid=0):
+    """Moving-image gradient on the GPU (src/base/volume_grad.cxx:9-76)."""
+    lib = _abi.load()
+    d = vol.desc()
+    if _is_torch(vol.img):
+        import torch
+        out = torch.empty(vol.img.shape + (3,), dtype=torch.float32, device=vol.img.device)
+        _abi.check(lib.plm_hip_volume_gradient(ctypes.byref(d), ctypes.c_void_p(out.data_ptr()),
+                                               _abi.PLM_MEM_DEVICE, gpuid))
+        return Volume(out, vol.spacing, vol.origin, vol.direction_cosines)
+    out = np.empty(vol.img.shape + (3,), np.float32)
+    _abi.check(lib.plm_hip_volume_gradient(ctypes.byref(d), out.ctypes.data_as(ctypes.c_void_p),
+                                           _abi.PLM_MEM_HOST, gpuid))
+    return Volume(out, vol.spacing, vol.origin, vol.direction_cosines)
+
+
+class BsplineHip:
+    """Device state for one stage: images resident in HBM, scored per evaluation."""
+
+    def __init__(self, bxf, fixed, moving, moving_grad=None, hist=None, gpuid=0, storage="f32"):
+        self.lib = _abi.load()
+        self.bxf = bxf
+        self.hist = hist
+        self.storage = storage
+        self.gpuid = gpuid
+        fd = fixed.desc(storage)
+        md = moving.desc(storage)
+        gd = moving_grad.desc(storage) if moving_grad is not None else None
+        hd = hist.desc() if hist is not None else None
+        ctx = ctypes.c_void_p()
+        _abi.check(self.lib.plm_hip_bspline_create(
+            ctypes.byref(bxf.geom()), ctypes.byref(fd), ctypes.byref(md),
+            ctypes.byref(gd) if gd is not None else None,
+            ctypes.byref(hd) if hd is not None else None, gpuid, ctypes.byref(ctx)))
+        self.ctx = ctx
+        n = ctypes.c_int64()
+        _abi.check(self.lib.plm_hip_bspline_num_knots(ctx, ctypes.byref(n)))
+        if n.value != bxf.num_knots:
+            raise RuntimeError("library and host disagree on the knot count")
+        _abi.check(self.lib.plm_hip_bspline_num_tiles(ctx, ctypes.byref(n)))
+        self.num_tiles = n.value
+
+    @property
+    def hist_size(self):
+        if self.hist is None:
+            return 0
+        return self.hist.fixed_bins * self.hist.moving_bins + self.hist.fixed_bins
+
+    def score(self, coeff=None, metric="mse", grad_mode="ref_nn", want_hist=False):
+        """One evaluation with host buffers (the CUDA_bspline_mse_j contract)."""
+        coeff = self.bxf.coeff if coeff is None else coeff
+        coeff = np.ascontiguousarray(coeff, np.float32)
+        if coeff.size != self.bxf.num_coeff:
+            raise ValueError("coefficient vector has the wrong length")
+        grad = np.empty(self.bxf.num_coeff, np.float64)
+        sc = ctypes.c_double()
+        nv = ctypes.c_int64()
+        hist = np.empty(self.hist_size, np.int64) if want_hist else None
+        rc = _abi.check(self.lib.plm_hip_bspline_score(
+            self.ctx, METRICS[metric], GRAD_MODES[grad_mode], coeff.ctypes.data_as(ctypes.c_void_p),
+            ctypes.byref(sc), grad.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nv),
+            hist.ctypes.data_as(ctypes.c_void_p) if hist is not None else None))
+        return Bspline_score(sc.value, grad, nv.value, hist, rc)
+
+    def score_device(self, d_coeff, d_score, d_grad, d_num_vox, metric="mse", grad_mode="ref_nn",
+                     d_hist=None, stream=None):
+        """Enqueue one evaluation with device tensors; returns immediately."""
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+        st = ctypes.c_void_p(stream) if stream else None
+        return _abi.check(self.lib.plm_hip_bspline_score_device(
+            self.ctx, METRICS[metric], GRAD_MODES[grad_mode], ptr(d_coeff), ptr(d_score),
+            ptr(d_grad), ptr(d_num_vox), ptr(d_hist), st))
+
+    def set_timing(self, enable=True):
+        _abi.check(self.lib.plm_hip_bspline_set_timing(self.ctx, 1 if enable else 0))
+
+    def get_timing(self, max_stages=6):
+        """Summed per-stage milliseconds since the last call, and the eval count."""
+        ms = np.zeros(max_stages, np.float64)
+        n = ctypes.c_int64()
+        _abi.check(self.lib.plm_hip_bspline_get_timing(self.ctx, ms.ctypes.data_as(ctypes.c_void_p),
+                                                        max_stages, ctypes.byref(n)))
+        return ms, n.value
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.plm_hip_bspline_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stage_for_pair(pair, gpuid=0, storage="f32", with_grad=True, bins=None, moving_grad=None):
+    """Build Volumes, grid, optional histogram and the device state for a synth.Pair."""
+    fixed = Volume(pair.fixed, pair.spacing, pair.origin, pair.direction_cosines)
+    moving = Volume(pair.moving, pair.spacing, pair.origin, pair.direction_cosines)
+    bxf = Bspline_xform.for_volume(fixed, pair.vox_per_rgn)
+    mg = None
+    if with_grad:
+        mg = moving_grad if moving_grad is not None else volume_gradient(moving, gpuid)
+    hist = Joint_histogram(bins, bins).initialize(fixed, moving) if bins else None
+    return BsplineHip(bxf, fixed, moving, mg, hist, gpuid, storage)

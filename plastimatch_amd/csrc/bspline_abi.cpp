@@ -1,0 +1,545 @@
+// C ABI of libplmregisterhip.so: context lifetime, uploads, dispatch.
+// Declared in include/plm_hip_bspline.h.
+
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "plm_hip_bspline.h"
+#include "bspline_kernels.h"
+
+using plmhip::HGeom;
+using plmhip::KGeom;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int
+fail (int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int
+hip_fail (hipError_t e, const char* what)
+{
+    return fail (PLM_HIP_ERR_HIP, std::string (what) + ": " + hipGetErrorString (e));
+}
+
+#define HIP_TRY(expr)                                                   \
+    do {                                                                \
+        hipError_t e_ = (expr);                                         \
+        if (e_ != hipSuccess) return hip_fail (e_, #expr);              \
+    } while (0)
+
+// 3x3 inverse by cofactors, float, as volume_matrix3x3inverse
+// (src/base/direction_cosines.cxx:134-159).  Returns false if singular.
+bool
+mat3_inverse (float* out, const float* m)
+{
+    float det = m[0]*(m[4]*m[8]-m[5]*m[7])
+        - m[1]*(m[3]*m[8]-m[5]*m[6])
+        + m[2]*(m[3]*m[7]-m[4]*m[6]);
+    if (std::fabs (det) < 1e-8) return false;
+    out[0] =  (m[4]*m[8]-m[5]*m[7]) / det;
+    out[1] = -(m[1]*m[8]-m[2]*m[7]) / det;
+    out[2] =  (m[1]*m[5]-m[2]*m[4]) / det;
+    out[3] = -(m[3]*m[8]-m[5]*m[6]) / det;
+    out[4] =  (m[0]*m[8]-m[2]*m[6]) / det;
+    out[5] = -(m[0]*m[5]-m[2]*m[3]) / det;
+    out[6] =  (m[3]*m[7]-m[4]*m[6]) / det;
+    out[7] = -(m[0]*m[7]-m[1]*m[6]) / det;
+    out[8] =  (m[0]*m[4]-m[1]*m[3]) / det;
+    return true;
+}
+
+// Per-axis cubic B-spline basis B_t(i/vpr): float argument, double
+// polynomial, float result, as bspline_basis_eval
+// (src/base/bspline_xform.cxx:73-99).
+void
+basis_lut (int vpr, float* lut)
+{
+    for (int i = 0; i < vpr; i++) {
+        float u = (float) i / (float) vpr;
+        double du = u;
+        lut[i*4+0] = (float) ((1.0/6.0)
+            * (((-1.0 * du) * du) * du + (3.0 * du) * du - 3.0 * du + 1.0));
+        lut[i*4+1] = (float) ((1.0/6.0)
+            * (((3.0 * du) * du) * du - (6.0 * du) * du + 4.0));
+        lut[i*4+2] = (float) ((1.0/6.0)
+            * (((-3.0 * du) * du) * du + (3.0 * du) * du + 3.0 * du + 1.0));
+        lut[i*4+3] = (float) ((1.0/6.0) * (((1.0 * du) * du) * du));
+    }
+}
+
+} // namespace
+
+struct plm_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    KGeom g {};
+    HGeom h {};
+    bool have_hist = false;
+    bool have_mgrad = false;
+    bool mi_ok = false;
+    int storage = PLM_STORE_F32;
+    int hist_blocks = 512;
+    int64_t num_knots = 0;
+    int64_t num_tiles = 0;
+    size_t lds_limit = 0;
+
+    void* d_fixed = nullptr;
+    void* d_moving = nullptr;
+    void* d_mgrad = nullptr;
+    float* d_blut = nullptr;
+    float* d_cond = nullptr;
+    double* d_tile_score = nullptr;
+    int* d_tile_count = nullptr;
+    double* d_gscale = nullptr;
+    double* d_one = nullptr;
+    unsigned long long* d_hist = nullptr;   // joint bins then fixed bins
+    float* d_table = nullptr;
+    float* d_nvox_f = nullptr;
+    // staging for the host-pointer entry point
+    float* d_coeff = nullptr;
+    double* d_grad = nullptr;
+    double* d_score = nullptr;
+    int64_t* d_nvox = nullptr;
+    int64_t* d_hist_out = nullptr;
+
+    // stage timing (plm_hip_bspline_set_timing): one event set per evaluation
+    bool timing = false;
+    int last_stages = 0;
+    std::vector<std::vector<hipEvent_t>> ev_used;
+    std::vector<std::vector<hipEvent_t>> ev_free;
+};
+
+namespace {
+
+void
+destroy_events (std::vector<std::vector<hipEvent_t>>& sets)
+{
+    for (auto& s : sets)
+        for (hipEvent_t e : s) (void) hipEventDestroy (e);
+    sets.clear ();
+}
+
+void
+free_ctx (plm_hip_ctx* c)
+{
+    destroy_events (c->ev_used);
+    destroy_events (c->ev_free);
+    void* bufs[] = {c->d_fixed, c->d_moving, c->d_mgrad, c->d_blut, c->d_cond,
+        c->d_tile_score, c->d_tile_count, c->d_gscale, c->d_one, c->d_hist,
+        c->d_table, c->d_nvox_f, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
+        c->d_hist_out};
+    for (void* b : bufs) if (b) (void) hipFree (b);
+    if (c->stream) (void) hipStreamDestroy (c->stream);
+    delete c;
+}
+
+template <typename T>
+int
+alloc (T** p, size_t count, const char* what)
+{
+    hipError_t e = hipMalloc ((void**) p, sizeof (T) * (count ? count : 1));
+    if (e != hipSuccess) return hip_fail (e, what);
+    return PLM_HIP_OK;
+}
+
+// Upload one image into a library-owned buffer, converting to BF16 if asked.
+int
+upload_image (plm_hip_ctx* c, const plm_img_desc* d, int64_t nvals, void** out)
+{
+    const size_t fbytes = sizeof (float) * nvals;
+    float* tmp = nullptr;
+    int rc = alloc (&tmp, nvals, "hipMalloc(image)");
+    if (rc) return rc;
+    hipMemcpyKind kind = d->location == PLM_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                       : hipMemcpyHostToDevice;
+    hipError_t e = hipMemcpy (tmp, d->data, fbytes, kind);
+    if (e != hipSuccess) { (void) hipFree (tmp); return hip_fail (e, "hipMemcpy(image)"); }
+    if (c->storage == PLM_STORE_F32) {
+        *out = tmp;
+        return PLM_HIP_OK;
+    }
+    uint16_t* bf = nullptr;
+    rc = alloc (&bf, nvals, "hipMalloc(bf16 image)");
+    if (rc) { (void) hipFree (tmp); return rc; }
+    e = plmhip::launch_to_bf16 (tmp, bf, nvals, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize (c->stream);
+    (void) hipFree (tmp);
+    if (e != hipSuccess) { (void) hipFree (bf); return hip_fail (e, "bf16 conversion"); }
+    *out = bf;
+    return PLM_HIP_OK;
+}
+
+int
+check_img (const plm_img_desc* d, const char* name)
+{
+    if (!d || !d->data) return fail (PLM_HIP_ERR_ARG, std::string (name) + " is NULL");
+    if (d->location != PLM_MEM_HOST && d->location != PLM_MEM_DEVICE)
+        return fail (PLM_HIP_ERR_ARG, std::string (name) + ": bad location");
+    int64_t n = 1;
+    for (int a = 0; a < 3; a++) {
+        if (d->dim[a] < 2) return fail (PLM_HIP_ERR_GEOMETRY, std::string (name) + ": dim < 2");
+        if (!(d->spacing[a] != 0.f)) return fail (PLM_HIP_ERR_GEOMETRY, std::string (name) + ": zero spacing");
+        n *= d->dim[a];
+    }
+    if (3 * n >= ((int64_t) 1 << 31))
+        return fail (PLM_HIP_ERR_GEOMETRY, std::string (name) + ": too many voxels for 32-bit indexing");
+    return PLM_HIP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int
+plm_hip_abi_version (void)
+{
+    return PLM_HIP_ABI_VERSION;
+}
+
+const char*
+plm_hip_last_error (void)
+{
+    return g_last_error.c_str ();
+}
+
+int
+plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
+    const plm_img_desc* moving, const plm_img_desc* moving_grad,
+    const plm_hist_desc* hist, int gpuid, plm_hip_ctx** out)
+{
+    if (!out) return fail (PLM_HIP_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!geom) return fail (PLM_HIP_ERR_ARG, "geom is NULL");
+    int rc = check_img (fixed, "fixed");
+    if (rc) return rc;
+    rc = check_img (moving, "moving");
+    if (rc) return rc;
+    if (moving_grad) {
+        if (!moving_grad->data) return fail (PLM_HIP_ERR_ARG, "moving_grad data is NULL");
+        for (int a = 0; a < 3; a++)
+            if (moving_grad->dim[a] != moving->dim[a])
+                return fail (PLM_HIP_ERR_GEOMETRY, "moving_grad dims differ from moving");
+    }
+    if (fixed->storage != moving->storage
+        || (fixed->storage != PLM_STORE_F32 && fixed->storage != PLM_STORE_BF16))
+        return fail (PLM_HIP_ERR_ARG, "fixed and moving must use the same storage");
+
+    KGeom g {};
+    int64_t tiles = 1, knots = 1;
+    for (int a = 0; a < 3; a++) {
+        int64_t vpr = geom->vox_per_rgn[a];
+        if (vpr < 1 || vpr > plmhip::MAX_VPR)
+            return fail (PLM_HIP_ERR_GEOMETRY, "vox_per_rgn out of range [1, 32]");
+        if (geom->roi_dim[a] < 1 || geom->roi_offset[a] < 0
+            || geom->roi_offset[a] + geom->roi_dim[a] > fixed->dim[a])
+            return fail (PLM_HIP_ERR_GEOMETRY, "ROI outside the fixed image");
+        if (geom->img_dim[a] != fixed->dim[a])
+            return fail (PLM_HIP_ERR_GEOMETRY, "img_dim differs from the fixed image dims");
+        // Bspline_header::set (src/base/bspline_header.cxx:28-62)
+        int64_t rd = 1 + (geom->roi_dim[a] - 1) / vpr;
+        g.vpr[a] = (int) vpr;
+        g.rdims[a] = (int) rd;
+        g.cdims[a] = (int) (rd + 3);
+        g.roi_off[a] = (int) geom->roi_offset[a];
+        g.roi_end[a] = (int) (geom->roi_offset[a] + geom->roi_dim[a]);
+        g.fdim[a] = (int) fixed->dim[a];
+        g.mdim[a] = (int) moving->dim[a];
+        g.img_origin[a] = geom->img_origin[a];
+        g.morigin[a] = moving->origin[a];
+        g.inv_msp[a] = (float) (1.0 / moving->spacing[a]);
+        tiles *= rd;
+        knots *= rd + 3;
+    }
+    // step = dc*spacing, proj = inv(dc)/spacing
+    // (src/base/direction_matrices.cxx:10-20)
+    float finv[9], minv[9];
+    if (!mat3_inverse (finv, fixed->direction_cosines) || !mat3_inverse (minv, moving->direction_cosines))
+        return fail (PLM_HIP_ERR_GEOMETRY, "singular direction cosines");
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            g.fstep[3*i+j] = fixed->direction_cosines[3*i+j] * fixed->spacing[j];
+            g.mproj[3*i+j] = minv[3*i+j] / moving->spacing[i];
+        }
+    if (tiles >= (1 << 30) / 192 || knots >= (1 << 29))
+        return fail (PLM_HIP_ERR_GEOMETRY, "B-spline grid too large");
+    g.num_tiles = (int) tiles;
+    g.num_knots = (int) knots;
+    g.tiles_per_xcd = (int) ((tiles + 7) / 8);
+
+    HGeom h {};
+    bool mi_ok = false;
+    if (hist) {
+        if (hist->fixed_bins < 2 || hist->moving_bins < 2
+            || hist->fixed_bins > plmhip::MAX_BINS_1D || hist->moving_bins > plmhip::MAX_BINS_1D
+            || hist->fixed_bins * hist->moving_bins > plmhip::MAX_BINS_2D)
+            return fail (PLM_HIP_ERR_UNSUPPORTED, "histogram bins out of range (<=256 each, <=4096 joint)");
+        if (!(hist->fixed_delta > 0.f) || !(hist->moving_delta > 0.f))
+            return fail (PLM_HIP_ERR_ARG, "histogram delta must be positive");
+        h.fbins = (int) hist->fixed_bins;
+        h.mbins = (int) hist->moving_bins;
+        h.f_offset = hist->fixed_offset;
+        h.f_delta = hist->fixed_delta;
+        h.m_offset = hist->moving_offset;
+        h.m_delta = hist->moving_delta;
+        // flavor g MI walks the whole fixed image in pass 1
+        // (src/register/bspline_mi.cxx:1567-1615); a ROI selects flavor h.
+        mi_ok = true;
+        for (int a = 0; a < 3; a++)
+            if (geom->roi_offset[a] != 0 || geom->roi_dim[a] != fixed->dim[a]) mi_ok = false;
+    }
+
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount (&ndev);
+    if (e != hipSuccess || ndev < 1) return fail (PLM_HIP_ERR_NO_DEVICE, "no HIP device");
+    if (gpuid < 0 || gpuid >= ndev) return fail (PLM_HIP_ERR_ARG, "gpuid out of range");
+    HIP_TRY (hipSetDevice (gpuid));
+    hipDeviceProp_t prop;
+    HIP_TRY (hipGetDeviceProperties (&prop, gpuid));
+
+    plm_hip_ctx* c = new plm_hip_ctx;
+    c->device = gpuid;
+    c->g = g;
+    c->h = h;
+    c->have_hist = hist != nullptr;
+    c->mi_ok = mi_ok;
+    c->have_mgrad = moving_grad != nullptr;
+    c->storage = fixed->storage;
+    c->num_knots = knots;
+    c->num_tiles = tiles;
+    c->lds_limit = prop.sharedMemPerBlock;
+    c->hist_blocks = (int) std::min<int64_t> (tiles, (int64_t) prop.multiProcessorCount * 2);
+    if (plmhip::tile_lds_bytes (g) + sizeof (float) * h.fbins * h.mbins > c->lds_limit) {
+        free_ctx (c);
+        return fail (PLM_HIP_ERR_UNSUPPORTED, "tile does not fit in LDS; use a smaller vox_per_rgn");
+    }
+    e = hipStreamCreate (&c->stream);
+    if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "hipStreamCreate"); }
+
+#define CTX_TRY(expr) do { int r_ = (expr); if (r_) { free_ctx (c); return r_; } } while (0)
+    const int64_t nf = fixed->dim[0] * fixed->dim[1] * fixed->dim[2];
+    const int64_t nm = moving->dim[0] * moving->dim[1] * moving->dim[2];
+    CTX_TRY (upload_image (c, fixed, nf, &c->d_fixed));
+    CTX_TRY (upload_image (c, moving, nm, &c->d_moving));
+    if (moving_grad) CTX_TRY (upload_image (c, moving_grad, 3 * nm, &c->d_mgrad));
+
+    std::vector<float> blut (4 * (g.vpr[0] + g.vpr[1] + g.vpr[2]));
+    basis_lut (g.vpr[0], &blut[0]);
+    basis_lut (g.vpr[1], &blut[4 * g.vpr[0]]);
+    basis_lut (g.vpr[2], &blut[4 * (g.vpr[0] + g.vpr[1])]);
+    CTX_TRY (alloc (&c->d_blut, blut.size (), "hipMalloc(lut)"));
+    e = hipMemcpy (c->d_blut, blut.data (), sizeof (float) * blut.size (), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "hipMemcpy(lut)"); }
+
+    CTX_TRY (alloc (&c->d_cond, (size_t) tiles * 192, "hipMalloc(cond)"));
+    CTX_TRY (alloc (&c->d_tile_score, tiles, "hipMalloc(tile_score)"));
+    CTX_TRY (alloc (&c->d_tile_count, tiles, "hipMalloc(tile_count)"));
+    CTX_TRY (alloc (&c->d_gscale, 1, "hipMalloc(gscale)"));
+    CTX_TRY (alloc (&c->d_one, 1, "hipMalloc(one)"));
+    const size_t nbins = (size_t) h.fbins * h.mbins + h.fbins;
+    CTX_TRY (alloc (&c->d_hist, nbins, "hipMalloc(hist)"));
+    CTX_TRY (alloc (&c->d_table, (size_t) h.fbins * h.mbins, "hipMalloc(table)"));
+    CTX_TRY (alloc (&c->d_nvox_f, 1, "hipMalloc(nvox_f)"));
+    CTX_TRY (alloc (&c->d_coeff, 3 * knots, "hipMalloc(coeff)"));
+    CTX_TRY (alloc (&c->d_grad, 3 * knots, "hipMalloc(grad)"));
+    CTX_TRY (alloc (&c->d_score, 1, "hipMalloc(score)"));
+    CTX_TRY (alloc (&c->d_nvox, 1, "hipMalloc(nvox)"));
+    CTX_TRY (alloc (&c->d_hist_out, nbins, "hipMalloc(hist_out)"));
+#undef CTX_TRY
+    e = plmhip::launch_fill_one (c->d_one, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize (c->stream);
+    if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "init"); }
+    *out = c;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
+    const float* d_coeff, double* d_score, double* d_grad, int64_t* d_num_vox,
+    int64_t* d_hist, void* stream)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    if (!d_coeff || !d_score || !d_grad || !d_num_vox)
+        return fail (PLM_HIP_ERR_ARG, "NULL output or coefficient pointer");
+    hipStream_t st = stream ? (hipStream_t) stream : c->stream;
+    HIP_TRY (hipSetDevice (c->device));
+    if (metric != PLM_METRIC_MSE && metric != PLM_METRIC_MI)
+        return fail (PLM_HIP_ERR_ARG, "unknown metric");
+    if (metric == PLM_METRIC_MSE) {
+        if (grad_mode != PLM_GRAD_REF_NN && grad_mode != PLM_GRAD_EXACT)
+            return fail (PLM_HIP_ERR_ARG, "bad grad_mode");
+        if (grad_mode == PLM_GRAD_REF_NN && !c->have_mgrad)
+            return fail (PLM_HIP_ERR_ARG, "PLM_GRAD_REF_NN needs moving_grad");
+    } else {
+        if (!c->have_hist) return fail (PLM_HIP_ERR_ARG, "MI needs a histogram description");
+        if (!c->mi_ok) return fail (PLM_HIP_ERR_UNSUPPORTED, "MI flavor g does not take a ROI");
+    }
+    hipEvent_t* ev = nullptr;
+    if (c->timing) {
+        if (c->ev_free.empty ()) {
+            std::vector<hipEvent_t> s (plmhip::MAX_STAGE_EVENTS);
+            for (auto& x : s) HIP_TRY (hipEventCreate (&x));
+            c->ev_free.push_back (s);
+        }
+        c->ev_used.push_back (c->ev_free.back ());
+        c->ev_free.pop_back ();
+        ev = c->ev_used.back ().data ();
+    }
+    hipError_t e;
+    if (metric == PLM_METRIC_MSE) {
+        e = plmhip::launch_mse (c->g, grad_mode, c->storage == PLM_STORE_BF16, c->d_fixed,
+            c->d_moving, c->d_mgrad, d_coeff, c->d_blut, c->d_cond, c->d_tile_score,
+            c->d_tile_count, d_score, d_num_vox, c->d_gscale, d_grad, st, ev);
+        if (e != hipSuccess) return hip_fail (e, "MSE launch");
+    } else {
+        e = plmhip::launch_mi (c->g, c->h, c->storage == PLM_STORE_BF16, c->hist_blocks,
+            c->d_fixed, c->d_moving, d_coeff, c->d_blut, c->d_hist,
+            c->d_hist + (size_t) c->h.fbins * c->h.mbins, c->d_table, c->d_nvox_f, c->d_cond,
+            c->d_one, d_score, d_num_vox, d_grad, d_hist, st, ev);
+        if (e != hipSuccess) return hip_fail (e, "MI launch");
+    }
+    if (ev) c->last_stages = metric == PLM_METRIC_MSE ? plmhip::MSE_STAGES : plmhip::MI_STAGES;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_set_timing (plm_hip_ctx* c, int enable)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    c->timing = enable != 0;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_get_timing (plm_hip_ctx* c, double* stage_ms, int max_stages,
+    int64_t* evals)
+{
+    if (!c || !stage_ms || !evals) return fail (PLM_HIP_ERR_ARG, "NULL argument");
+    HIP_TRY (hipSetDevice (c->device));
+    for (int i = 0; i < max_stages; i++) stage_ms[i] = 0.0;
+    int64_t n = 0;
+    for (auto& s : c->ev_used) {
+        HIP_TRY (hipEventSynchronize (s[c->last_stages]));
+        for (int i = 0; i < c->last_stages && i < max_stages; i++) {
+            float ms = 0.f;
+            HIP_TRY (hipEventElapsedTime (&ms, s[i], s[i + 1]));
+            stage_ms[i] += ms;
+        }
+        n++;
+        c->ev_free.push_back (s);
+    }
+    c->ev_used.clear ();
+    *evals = n;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_score (plm_hip_ctx* c, int metric, int grad_mode, const float* coeff,
+    double* score, double* grad, int64_t* num_vox, int64_t* hist_out)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    if (!coeff || !score || !grad || !num_vox) return fail (PLM_HIP_ERR_ARG, "NULL argument");
+    if (hist_out && metric != PLM_METRIC_MI) return fail (PLM_HIP_ERR_ARG, "hist_out is MI only");
+    HIP_TRY (hipSetDevice (c->device));
+    const size_t nc = 3 * (size_t) c->num_knots;
+    HIP_TRY (hipMemcpyAsync (c->d_coeff, coeff, sizeof (float) * nc, hipMemcpyHostToDevice, c->stream));
+    int rc = plm_hip_bspline_score_device (c, metric, grad_mode, c->d_coeff, c->d_score,
+        c->d_grad, c->d_nvox, hist_out ? c->d_hist_out : nullptr, c->stream);
+    if (rc) return rc;
+    HIP_TRY (hipMemcpyAsync (grad, c->d_grad, sizeof (double) * nc, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY (hipMemcpyAsync (score, c->d_score, sizeof (double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY (hipMemcpyAsync (num_vox, c->d_nvox, sizeof (int64_t), hipMemcpyDeviceToHost, c->stream));
+    if (hist_out) {
+        const size_t nbins = (size_t) c->h.fbins * c->h.mbins + c->h.fbins;
+        HIP_TRY (hipMemcpyAsync (hist_out, c->d_hist_out, sizeof (int64_t) * nbins,
+            hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY (hipStreamSynchronize (c->stream));
+    if (*num_vox < 1) {
+        // bspline_score_normalize (src/register/bspline_mse.cxx:53-65); the
+        // MI flavor has no guard in the reference and gives the same here.
+        *score = (double) FLT_MAX;
+        for (size_t i = 0; i < nc; i++) grad[i] = 0.0;
+        return PLM_HIP_WARN_NO_OVERLAP;
+    }
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_destroy (plm_hip_ctx* c)
+{
+    if (!c) return PLM_HIP_OK;
+    (void) hipSetDevice (c->device);
+    (void) hipStreamSynchronize (c->stream);
+    free_ctx (c);
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_volume_gradient (const plm_img_desc* vol, float* out, int out_location, int gpuid)
+{
+    int rc = check_img (vol, "vol");
+    if (rc) return rc;
+    if (!out) return fail (PLM_HIP_ERR_ARG, "out is NULL");
+    if (vol->storage != PLM_STORE_F32) return fail (PLM_HIP_ERR_ARG, "vol must be float32");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount (&ndev);
+    if (e != hipSuccess || ndev < 1) return fail (PLM_HIP_ERR_NO_DEVICE, "no HIP device");
+    if (gpuid < 0 || gpuid >= ndev) return fail (PLM_HIP_ERR_ARG, "gpuid out of range");
+    HIP_TRY (hipSetDevice (gpuid));
+    plmhip::Mat3 inv;
+    if (!mat3_inverse (inv.m, vol->direction_cosines))
+        return fail (PLM_HIP_ERR_GEOMETRY, "singular direction cosines");
+    const int64_t n = vol->dim[0] * vol->dim[1] * vol->dim[2];
+    const int dim[3] = {(int) vol->dim[0], (int) vol->dim[1], (int) vol->dim[2]};
+    float* d_in = nullptr;
+    float* d_out = nullptr;
+    if (vol->location == PLM_MEM_HOST) {
+        rc = alloc (&d_in, n, "hipMalloc(gradient input)");
+        if (rc) return rc;
+        e = hipMemcpy (d_in, vol->data, sizeof (float) * n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { (void) hipFree (d_in); return hip_fail (e, "hipMemcpy(gradient input)"); }
+    }
+    if (out_location == PLM_MEM_HOST) {
+        rc = alloc (&d_out, 3 * n, "hipMalloc(gradient output)");
+        if (rc) { if (d_in) (void) hipFree (d_in); return rc; }
+    }
+    const float* src = d_in ? d_in : (const float*) vol->data;
+    float* dst = d_out ? d_out : out;
+    e = plmhip::launch_volume_gradient (src, dst, dim, vol->spacing, inv, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize ();
+    if (e == hipSuccess && d_out)
+        e = hipMemcpy (out, d_out, sizeof (float) * 3 * n, hipMemcpyDeviceToHost);
+    if (d_in) (void) hipFree (d_in);
+    if (d_out) (void) hipFree (d_out);
+    if (e != hipSuccess) return hip_fail (e, "volume gradient");
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_num_knots (const plm_hip_ctx* c, int64_t* n)
+{
+    if (!c || !n) return fail (PLM_HIP_ERR_ARG, "NULL argument");
+    *n = c->num_knots;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_num_tiles (const plm_hip_ctx* c, int64_t* n)
+{
+    if (!c || !n) return fail (PLM_HIP_ERR_ARG, "NULL argument");
+    *n = c->num_tiles;
+    return PLM_HIP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,72 @@
+// Kernel-side geometry and launchers of libplmregisterhip.so (internal).
+#ifndef PLM_BSPLINE_KERNELS_H
+#define PLM_BSPLINE_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace plmhip {
+
+constexpr int TILE_THREADS = 256;
+constexpr int MAX_BINS_1D = 256;     // per-axis histogram bins
+constexpr int MAX_BINS_2D = 4096;    // joint bins held in LDS (32 KB int64)
+constexpr int MAX_VPR = 32;          // tile edge bound (LDS sizing)
+
+// Everything a tile kernel needs, passed by value.
+struct KGeom {
+    int vpr[3];
+    int rdims[3];
+    int cdims[3];
+    int roi_off[3];
+    int roi_end[3];          // roi_offset + roi_dim
+    int fdim[3];
+    int mdim[3];
+    float img_origin[3];     // grid origin (Bspline_header::img_origin)
+    float fstep[9];          // fixed step = dc * spacing
+    float morigin[3];
+    float mproj[9];          // moving proj = inv(dc) / spacing
+    float inv_msp[3];        // (float)(1.0 / moving spacing)
+    int num_tiles;
+    int num_knots;
+    int tiles_per_xcd;
+};
+
+struct Mat3 {
+    float m[9];
+};
+
+struct HGeom {
+    int fbins;
+    int mbins;
+    float f_offset, f_delta;
+    float m_offset, m_delta;
+};
+
+size_t tile_lds_bytes (const KGeom& g);
+
+// Stage timing: when ev is not null, ev[0..n] are recorded on st between
+// the launches (MSE: tiles | finalize | reduce, 4 events; MI: zero | hist |
+// score | tiles | reduce, 6 events).
+constexpr int MSE_STAGES = 3;
+constexpr int MI_STAGES = 5;
+constexpr int MAX_STAGE_EVENTS = 6;
+
+hipError_t launch_mse (const KGeom& g, int mode, int storage, const void* f,
+    const void* m, const void* mg, const float* coeff, const float* blut,
+    float* cond, double* tile_score, int* tile_count, double* score,
+    int64_t* nvox, double* gscale, double* grad, hipStream_t st, hipEvent_t* ev);
+
+hipError_t launch_mi (const KGeom& g, const HGeom& h, int storage, int hist_blocks,
+    const void* f, const void* m, const float* coeff, const float* blut,
+    unsigned long long* jhist, unsigned long long* fhist, float* table,
+    float* nvox_f, float* cond, const double* one, double* score,
+    int64_t* nvox, double* grad, int64_t* hist_out, hipStream_t st, hipEvent_t* ev);
+
+hipError_t launch_fill_one (double* p, hipStream_t st);
+hipError_t launch_volume_gradient (const float* img, float* out, const int dim[3],
+    const float sp[3], const Mat3& inv, hipStream_t st);
+hipError_t launch_to_bf16 (const float* src, uint16_t* dst, int64_t n, hipStream_t st);
+
+} // namespace plmhip
+
+#endif
