@@ -28,7 +28,7 @@ def test_mi_matches_oracle(hip_lib, case):
     hp = oc.hist(bins)
     with stage_for_pair(pair, with_grad=False, bins=bins) as st:
         for k, coeff in enumerate([np.full(oc.num_coeff, 0.01, np.float32), pair.true_coeff,
-                                   synth.random_coeff(pair.dim, vpr, 2.0, seed=k)]):
+                                   synth.random_coeff(pair.dim, vpr, 2.0, seed=case + 7)]):
             r = st.score(coeff, "mi", want_hist=True)
             o = oc.mi(coeff, hp)
             s64, g64, n64 = oc.mi_f64(coeff, hp)
