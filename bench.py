@@ -120,12 +120,12 @@ def main():
     ms, n = st.get_timing()
     st.set_timing(False)
     tile_ms = ms[0] / n
-    stage_ms = [float(x / n) for x in ms[:3]]
+    stage_ms = [float(x / n) for x in ms[:2]]
     bytes_per_launch = alg_bytes(pair)
     achieved = bytes_per_launch / (tile_ms * 1e-3) / 1e9
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"k_mse_tiles_{WORKLOAD}_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"k_tiles_{WORKLOAD}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -148,11 +148,10 @@ def main():
                    "coeff_vectors": N_COEFF_VECTORS, "num_tiles": st.num_tiles,
                    "num_coeff": bxf.num_coeff},
         "eval_gb_per_s": bytes_per_launch / (ms_per_step * 1e-3) / 1e9,
-        "stage_ms": {"k_mse_tiles": stage_ms[0], "k_mse_finalize": stage_ms[1],
-                     "k_grad_reduce": stage_ms[2]},
+        "stage_ms": {"k_tiles": stage_ms[0], "k_reduce": stage_ms[1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_mse_tiles", "alg_bytes_per_launch": bytes_per_launch,
+                     "kernel": "k_tiles", "alg_bytes_per_launch": bytes_per_launch,
                      "kernel_ms": tile_ms},
         "cpu_baseline": None,
     }

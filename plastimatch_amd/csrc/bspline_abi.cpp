@@ -100,9 +100,7 @@ struct plm_hip_ctx {
     float* d_blut = nullptr;
     float* d_cond = nullptr;
     double* d_tile_score = nullptr;
-    int* d_tile_count = nullptr;
-    double* d_gscale = nullptr;
-    double* d_one = nullptr;
+    unsigned long long* d_ctr = nullptr;    // N accumulator, reduce ticket
     unsigned long long* d_hist = nullptr;   // joint bins then fixed bins
     float* d_table = nullptr;
     float* d_nvox_f = nullptr;
@@ -136,7 +134,7 @@ free_ctx (plm_hip_ctx* c)
     destroy_events (c->ev_used);
     destroy_events (c->ev_free);
     void* bufs[] = {c->d_fixed, c->d_moving, c->d_mgrad, c->d_blut, c->d_cond,
-        c->d_tile_score, c->d_tile_count, c->d_gscale, c->d_one, c->d_hist,
+        c->d_tile_score, c->d_ctr, c->d_hist,
         c->d_table, c->d_nvox_f, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
         c->d_hist_out};
     for (void* b : bufs) if (b) (void) hipFree (b);
@@ -276,6 +274,12 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     g.num_tiles = (int) tiles;
     g.num_knots = (int) knots;
     g.tiles_per_xcd = (int) ((tiles + 7) / 8);
+    // exact n/d via __umulhi(n, ceil(2^32/d)) for n*d < 2^32
+    g.mag_vx = (uint32_t) ((((uint64_t) 1 << 32) + g.vpr[0] - 1) / g.vpr[0]);
+    g.mag_vxy = (uint32_t) ((((uint64_t) 1 << 32) + g.vpr[0] * g.vpr[1] - 1) / (g.vpr[0] * g.vpr[1]));
+    g.vstep[0] = plmhip::TILE_THREADS % g.vpr[0];
+    g.vstep[1] = (plmhip::TILE_THREADS / g.vpr[0]) % g.vpr[1];
+    g.vstep[2] = plmhip::TILE_THREADS / (g.vpr[0] * g.vpr[1]);
 
     HGeom h {};
     bool mi_ok = false;
@@ -341,11 +345,10 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     e = hipMemcpy (c->d_blut, blut.data (), sizeof (float) * blut.size (), hipMemcpyHostToDevice);
     if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "hipMemcpy(lut)"); }
 
-    CTX_TRY (alloc (&c->d_cond, (size_t) tiles * 192, "hipMalloc(cond)"));
+    // knot-major condense slots; slots no tile owns stay zero for good
+    CTX_TRY (alloc (&c->d_cond, (size_t) knots * 192, "hipMalloc(cond)"));
     CTX_TRY (alloc (&c->d_tile_score, tiles, "hipMalloc(tile_score)"));
-    CTX_TRY (alloc (&c->d_tile_count, tiles, "hipMalloc(tile_count)"));
-    CTX_TRY (alloc (&c->d_gscale, 1, "hipMalloc(gscale)"));
-    CTX_TRY (alloc (&c->d_one, 1, "hipMalloc(one)"));
+    CTX_TRY (alloc (&c->d_ctr, plmhip::NUM_COUNTERS, "hipMalloc(counters)"));
     const size_t nbins = (size_t) h.fbins * h.mbins + h.fbins;
     CTX_TRY (alloc (&c->d_hist, nbins, "hipMalloc(hist)"));
     CTX_TRY (alloc (&c->d_table, (size_t) h.fbins * h.mbins, "hipMalloc(table)"));
@@ -356,7 +359,8 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     CTX_TRY (alloc (&c->d_nvox, 1, "hipMalloc(nvox)"));
     CTX_TRY (alloc (&c->d_hist_out, nbins, "hipMalloc(hist_out)"));
 #undef CTX_TRY
-    e = plmhip::launch_fill_one (c->d_one, c->stream);
+    e = plmhip::launch_zero (c->d_cond, (int64_t) knots * 192, c->d_ctr, plmhip::NUM_COUNTERS,
+        c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize (c->stream);
     if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "init"); }
     *out = c;
@@ -399,13 +403,13 @@ plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
     if (metric == PLM_METRIC_MSE) {
         e = plmhip::launch_mse (c->g, grad_mode, c->storage == PLM_STORE_BF16, c->d_fixed,
             c->d_moving, c->d_mgrad, d_coeff, c->d_blut, c->d_cond, c->d_tile_score,
-            c->d_tile_count, d_score, d_num_vox, c->d_gscale, d_grad, st, ev);
+            c->d_ctr, d_score, d_num_vox, d_grad, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MSE launch");
     } else {
         e = plmhip::launch_mi (c->g, c->h, c->storage == PLM_STORE_BF16, c->hist_blocks,
             c->d_fixed, c->d_moving, d_coeff, c->d_blut, c->d_hist,
             c->d_hist + (size_t) c->h.fbins * c->h.mbins, c->d_table, c->d_nvox_f, c->d_cond,
-            c->d_one, d_score, d_num_vox, d_grad, d_hist, st, ev);
+            c->d_ctr, d_score, d_num_vox, d_grad, d_hist, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MI launch");
     }
     if (ev) c->last_stages = metric == PLM_METRIC_MSE ? plmhip::MSE_STAGES : plmhip::MI_STAGES;

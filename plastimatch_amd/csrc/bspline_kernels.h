@@ -10,7 +10,8 @@ namespace plmhip {
 constexpr int TILE_THREADS = 256;
 constexpr int MAX_BINS_1D = 256;     // per-axis histogram bins
 constexpr int MAX_BINS_2D = 4096;    // joint bins held in LDS (32 KB int64)
-constexpr int MAX_VPR = 32;          // tile edge bound (LDS sizing)
+constexpr int MAX_VPR = 32;          // tile edge bound
+constexpr int NUM_COUNTERS = 4;      // [0] overlap count N, [1] reduce ticket
 
 // Everything a tile kernel needs, passed by value.
 struct KGeom {
@@ -29,6 +30,9 @@ struct KGeom {
     int num_tiles;
     int num_knots;
     int tiles_per_xcd;
+    uint32_t mag_vx;         // ceil(2^32 / vpr0), for n / vpr0 with __umulhi
+    uint32_t mag_vxy;        // ceil(2^32 / (vpr0*vpr1))
+    int vstep[3];            // TILE_THREADS in mixed radix (vpr0, vpr1, inf)
 };
 
 struct Mat3 {
@@ -45,24 +49,24 @@ struct HGeom {
 size_t tile_lds_bytes (const KGeom& g);
 
 // Stage timing: when ev is not null, ev[0..n] are recorded on st between
-// the launches (MSE: tiles | finalize | reduce, 4 events; MI: zero | hist |
-// score | tiles | reduce, 6 events).
-constexpr int MSE_STAGES = 3;
+// the launches (MSE: tiles | reduce, 3 events; MI: zero | hist | score |
+// tiles | reduce, 6 events).
+constexpr int MSE_STAGES = 2;
 constexpr int MI_STAGES = 5;
 constexpr int MAX_STAGE_EVENTS = 6;
 
 hipError_t launch_mse (const KGeom& g, int mode, int storage, const void* f,
     const void* m, const void* mg, const float* coeff, const float* blut,
-    float* cond, double* tile_score, int* tile_count, double* score,
-    int64_t* nvox, double* gscale, double* grad, hipStream_t st, hipEvent_t* ev);
+    float* cond, double* tile_score, unsigned long long* ctr, double* score,
+    int64_t* nvox, double* grad, hipStream_t st, hipEvent_t* ev);
 
 hipError_t launch_mi (const KGeom& g, const HGeom& h, int storage, int hist_blocks,
     const void* f, const void* m, const float* coeff, const float* blut,
     unsigned long long* jhist, unsigned long long* fhist, float* table,
-    float* nvox_f, float* cond, const double* one, double* score,
+    float* nvox_f, float* cond, unsigned long long* ctr, double* score,
     int64_t* nvox, double* grad, int64_t* hist_out, hipStream_t st, hipEvent_t* ev);
 
-hipError_t launch_fill_one (double* p, hipStream_t st);
+hipError_t launch_zero (float* p, int64_t n, unsigned long long* ctr, int nctr, hipStream_t st);
 hipError_t launch_volume_gradient (const float* img, float* out, const int dim[3],
     const float sp[3], const Mat3& inv, hipStream_t st);
 hipError_t launch_to_bf16 (const float* src, uint16_t* dst, int64_t n, hipStream_t st);

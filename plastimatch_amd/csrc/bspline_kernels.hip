@@ -5,13 +5,18 @@
 //   1. the 64x3 tile coefficients and the per-axis basis LUTs go to LDS;
 //   2. the displacement is contracted separably: T1 = sum_i bx*c, then
 //      T2 = sum_j by*T1, so each voxel only does the last 4-term sum in z;
-//   3. each voxel warps, samples the moving image trilinearly and forms
-//      dc/dv, which is parked in LDS;
-//   4. the condense into the tile's 64 knot slots is the transposed
-//      separable contraction (x, then y, then z), written once per tile to
-//      a tile-major buffer: no global atomics, a single writer per slot;
-//   5. a reduce kernel sums, per knot, its <= 64 slots in FP64 in a fixed
-//      order, so the gradient is deterministic run to run.
+//   3. each thread takes G voxels at a time and issues all of their global
+//      loads (fixed value, 8 moving corners, moving gradient) before using
+//      any, so a wavefront keeps G gathers per lane in flight;
+//   4. dc/dv is parked in LDS and condensed into the tile's 64 knot slots by
+//      the transposed separable contraction (x, then y, then z);
+//   5. the 192 slot values go to a knot-major buffer cond[knot][slot][c]
+//      (one writer per entry, no atomics on floats), and a reduce kernel
+//      gives each knot one wavefront: lane = slot, FP64 butterfly sum.
+// The overlap count N is an integer atomic (order independent); the score
+// is a fixed-order FP64 sum of per-tile partials, so results are bitwise
+// reproducible run to run.
+//
 // The MI histogram pass instead evaluates the displacement in the
 // reference's k-j-i order without FMA so its partial-volume weights are
 // bit-identical to the CPU flavor's, and accumulates them as 2^-32
@@ -29,13 +34,28 @@
 
 namespace plmhip {
 
+constexpr int G = 4;                    // voxels per thread in flight
+
 // ---------------------------------------------------------------------------
-// image loads (FP32 or BF16 storage, FP32 arithmetic)
+// small helpers
 
 __device__ __forceinline__ float ld (const float* __restrict__ p, int i) { return p[i]; }
 __device__ __forceinline__ float ld (const uint16_t* __restrict__ p, int i)
 {
     return __uint_as_float (((uint32_t) p[i]) << 16);
+}
+
+// n / d for n < 2^27, d <= 32, with m = ceil(2^32 / d).
+__device__ __forceinline__ int
+fdiv (int n, uint32_t m)
+{
+    return (int) __umulhi ((uint32_t) n, m);
+}
+
+__device__ __forceinline__ unsigned long long
+atomic_load_u64 (unsigned long long* p)
+{
+    return __hip_atomic_load (p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -52,6 +72,7 @@ struct Sample {
 // Follows bspline_find_correspondence_dcos
 // (src/util/bspline_correspond.cxx:80-109), Volume::is_inside
 // (src/base/volume.cxx:899-906) and li_clamp (src/base/interpolate.cxx:17-41).
+// Outside voxels get in-range indices so their loads can be issued blindly.
 __device__ __forceinline__ bool
 correspond (const KGeom& g, const float fxyz[3], const float d[3], Sample& s)
 {
@@ -66,11 +87,10 @@ correspond (const KGeom& g, const float fxyz[3], const float d[3], Sample& s)
 #pragma unroll
     for (int a = 0; a < 3; a++)
         inside = inside && (mijk[a] > -0.5f) && (mijk[a] < (float) g.mdim[a] - 0.5f);
-    if (!inside) return false;
     int f[3], r[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        float ma = mijk[a];
+        float ma = inside ? mijk[a] : 0.f;
         int dmax = g.mdim[a] - 1;
         float fa2;
         if (ma < 0.f) {
@@ -88,7 +108,7 @@ correspond (const KGeom& g, const float fxyz[3], const float d[3], Sample& s)
     }
     s.mvf = f[0] + g.mdim[0] * (f[1] + g.mdim[1] * f[2]);
     s.mvr = r[0] + g.mdim[0] * (r[1] + g.mdim[1] * r[2]);
-    return true;
+    return inside;
 }
 
 template <typename TI>
@@ -158,6 +178,15 @@ position (float fxyz[3], const KGeom& g, int fi, int fj, int fk)
         fxyz[a] = g.img_origin[a] + x * g.fstep[3*a+0] + y * g.fstep[3*a+1] + z * g.fstep[3*a+2];
 }
 
+__device__ __forceinline__ int
+hist_bin (float v, float offset, float delta, int bins)
+{
+    int b = (int) floorf ((v - offset) / delta);
+    // In-range intensities always give 0..bins-1; the clamp only keeps a
+    // corrupted input from indexing outside the histogram.
+    return b < 0 ? 0 : (b >= bins ? bins - 1 : b);
+}
+
 // ---------------------------------------------------------------------------
 // tile bookkeeping
 
@@ -165,7 +194,7 @@ __device__ __forceinline__ int
 xcd_tile (const KGeom& g, int b)
 {
     // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a
-    // contiguous run of tiles so x-neighbours share that XCD's L2.
+    // contiguous run of tiles so neighbouring tiles share that XCD's L2.
     return (b & 7) * g.tiles_per_xcd + (b >> 3);
 }
 
@@ -185,7 +214,8 @@ struct TileLds {
     float* t1;                         // [k][j][x][c] then S2 [z][j][i][c]
     float* t2;                         // [y][x][c][k] then S1 [z][y][i][c]
     float* dv;                         // [z][y][x][c]
-    double* red;                       // 2 * waves
+    double* red;                       // 16 doubles
+    float* tab;                        // MI: dS/dP table, Bf*Bm
 };
 
 __device__ __forceinline__ TileLds
@@ -193,16 +223,18 @@ carve (float* sm, const KGeom& g)
 {
     TileLds L;
     const int vx = g.vpr[0], vy = g.vpr[1], vz = g.vpr[2];
+    const int mx = vx > vz ? vx : vz;
     L.bx = sm;
     L.by = L.bx + 4 * vx;
     L.bz = L.by + 4 * vy;
     L.coef = L.bz + 4 * vz;
     L.t1 = L.coef + 192;
-    L.t2 = L.t1 + 48 * (vx > vz ? vx : vz);
-    L.dv = L.t2 + 12 * vy * (vx > vz ? vx : vz);
+    L.t2 = L.t1 + 48 * mx;
+    L.dv = L.t2 + 12 * vy * mx;
     float* end = L.dv + 3 * vx * vy * vz;
     uintptr_t e = ((uintptr_t) end + 7) & ~(uintptr_t) 7;
     L.red = (double*) e;
+    L.tab = (float*) (L.red + 16);
     return L;
 }
 
@@ -211,8 +243,7 @@ __device__ __forceinline__ void
 load_tile (const TileLds& L, const KGeom& g, const int p[3],
     const float* __restrict__ coeff, const float* __restrict__ blut)
 {
-    const int vx = g.vpr[0], vy = g.vpr[1], vz = g.vpr[2];
-    const int nl = 4 * (vx + vy + vz);
+    const int nl = 4 * (g.vpr[0] + g.vpr[1] + g.vpr[2]);
     for (int i = threadIdx.x; i < nl; i += blockDim.x) L.bx[i] = blut[i];
     for (int e = threadIdx.x; e < 192; e += blockDim.x) {
         int c = e % 3, m = e / 3;
@@ -230,7 +261,8 @@ separable_displacement (const TileLds& L, const KGeom& g)
     const int vx = g.vpr[0], vy = g.vpr[1];
     const int n1 = 48 * vx;
     for (int e = threadIdx.x; e < n1; e += blockDim.x) {
-        int c = e % 3, x = (e / 3) % vx, kj = e / (3 * vx);
+        int c = e % 3, e3 = e / 3;
+        int kj = fdiv (e3, g.mag_vx), x = e3 - kj * vx;
         const float* cf = &L.coef[(kj * 4) * 3 + c];
         const float* b = &L.bx[4 * x];
         float v = b[0] * cf[0];
@@ -243,7 +275,7 @@ separable_displacement (const TileLds& L, const KGeom& g)
     const int n2 = 12 * vx * vy;
     for (int e = threadIdx.x; e < n2; e += blockDim.x) {
         int k = e & 3, c = (e >> 2) % 3, xy = e / 12;
-        int x = xy % vx, y = xy / vx;
+        int y = fdiv (xy, g.mag_vx), x = xy - y * vx;
         const float* b = &L.by[4 * y];
         float v = 0.f;
 #pragma unroll
@@ -271,9 +303,11 @@ voxel_displacement (float d[3], const TileLds& L, int qx, int qy, int qz, int vx
 }
 
 // Condense the parked dc/dv of the tile into its 64 knot slots and write
-// them to cond[tile][slot][c].  S1 over x, S2 over y, S3 over z.
+// them knot-major: cond[(knot*64 + slot)*3 + c], slot = (k*4 + j)*4 + i,
+// the layout of bspline_sort_sets (src/register/bspline.cxx:377-398).
+// S1 over x, S2 over y, S3 over z.
 __device__ __forceinline__ void
-condense_tile (const TileLds& L, const KGeom& g, int tile, float* __restrict__ cond)
+condense_tile (const TileLds& L, const KGeom& g, const int p[3], float* __restrict__ cond)
 {
     const int vx = g.vpr[0], vy = g.vpr[1], vz = g.vpr[2];
     float* s1 = L.t2;   // [z][y][i][c]
@@ -312,22 +346,28 @@ condense_tile (const TileLds& L, const KGeom& g, int tile, float* __restrict__ c
     }
     __syncthreads ();
     if (threadIdx.x < 48) {
-        int jic = threadIdx.x;          // (j*4 + i)*3 + c
-        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+        const int jic = threadIdx.x;          // (j*4 + i)*3 + c
+        const int c = jic % 3, ji = jic / 3, i = ji & 3, j = ji >> 2;
+        float a[4] = {0.f, 0.f, 0.f, 0.f};
         for (int z = 0; z < vz; z++) {
             float v = s2[z * 48 + jic];
             const float* b = &L.bz[4 * z];
-            a0 = fmaf (b[0], v, a0);
-            a1 = fmaf (b[1], v, a1);
-            a2 = fmaf (b[2], v, a2);
-            a3 = fmaf (b[3], v, a3);
+#pragma unroll
+            for (int k = 0; k < 4; k++) a[k] = fmaf (b[k], v, a[k]);
         }
-        float* o = &cond[(size_t) tile * 192 + jic];
-        o[0] = a0; o[48] = a1; o[96] = a2; o[144] = a3;   // slot = k*16 + j*4 + i
+        const int cd0 = g.cdims[0], cd01 = g.cdims[0] * g.cdims[1];
+        const int knot0 = (p[0] + i) + cd0 * (p[1] + j) + cd01 * p[2];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int knot = knot0 + k * cd01;
+            const int slot = k * 16 + ji;
+            cond[((size_t) knot * 64 + slot) * 3 + c] = a[k];
+        }
     }
 }
 
-// Block sum of a double and an int in a fixed order (deterministic).
+// Block sum of a double and an int in a fixed order (deterministic);
+// the result is valid in thread 0.
 __device__ __forceinline__ void
 block_sum (const TileLds& L, double& s, int& n)
 {
@@ -352,173 +392,240 @@ block_sum (const TileLds& L, double& s, int& n)
 }
 
 // ---------------------------------------------------------------------------
-// MSE tile kernel (flavor g: MODE 0, flavor p: MODE 1)
+// Tile kernel.  KIND 0: MSE with the nearest-voxel moving gradient
+// (bspline_score_g_mse, src/register/bspline_mse.cxx:519-720); KIND 1: MSE
+// with the analytic trilinear derivative (bspline_score_p_mse,
+// src/register/bspline_mse.cxx:1149-1384); KIND 2: MI pass 2, the PV-8
+// derivative (pass 2 of bspline_score_g_mi,
+// src/register/bspline_mi.cxx:1627-1716, and bspline_mi_pvi_8_dc_dv_dcos,
+// src/register/bspline_mi.txx:9-120).
 
-template <int MODE, typename TI>
+struct MiArgs {
+    HGeom h;
+    const float* table;        // dS/dP per joint bin (0 where j <= 1e-4)
+    const float* nvox_f;       // (float) N
+};
+
+template <int KIND, typename TI>
 __global__ __launch_bounds__ (TILE_THREADS)
-void k_mse_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
+void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
     const TI* __restrict__ mgrad, const float* __restrict__ coeff,
     const float* __restrict__ blut, float* __restrict__ cond,
-    double* __restrict__ tile_score, int* __restrict__ tile_count)
+    double* __restrict__ tile_score, unsigned long long* __restrict__ ctr, MiArgs mi)
 {
     extern __shared__ float sm[];
     const int tile = xcd_tile (g, blockIdx.x);
     if (tile >= g.num_tiles) return;
-    const KGeom& G = g;
-    TileLds L = carve (sm, G);
+    const KGeom& G_ = g;
+    TileLds L = carve (sm, G_);
+    if (KIND == 2) {
+        const int nj = mi.h.fbins * mi.h.mbins;
+        for (int i = threadIdx.x; i < nj; i += blockDim.x) L.tab[i] = mi.table[i];
+    }
     int p[3];
-    tile_coords (G, tile, p);
-    load_tile (L, G, p, coeff, blut);
+    tile_coords (G_, tile, p);
+    load_tile (L, G_, p, coeff, blut);
     __syncthreads ();
-    separable_displacement (L, G);
+    separable_displacement (L, G_);
 
-    const int vx = G.vpr[0], vy = G.vpr[1], vz = G.vpr[2];
+    const int vx = G_.vpr[0], vy = G_.vpr[1], vz = G_.vpr[2];
     const int nvox = vx * vy * vz;
+    const int bx0 = G_.roi_off[0] + vx * p[0];
+    const int by0 = G_.roi_off[1] + vy * p[1];
+    const int bz0 = G_.roi_off[2] + vz * p[2];
+    float N = 1.f;
+    if (KIND == 2) N = *mi.nvox_f;
+
+    // local coordinates of this thread's first voxel, advanced by
+    // TILE_THREADS voxels per step in mixed radix
+    int cz = fdiv (threadIdx.x, G_.mag_vxy);
+    int rem = threadIdx.x - cz * vx * vy;
+    int cy = fdiv (rem, G_.mag_vx);
+    int cx = rem - cy * vx;
+
     double score = 0.0;
     int count = 0;
-    for (int v = threadIdx.x; v < nvox; v += blockDim.x) {
-        int qx = v % vx, t = v / vx, qy = t % vy, qz = t / vy;
-        int fi = G.roi_off[0] + vx * p[0] + qx;
-        int fj = G.roi_off[1] + vy * p[1] + qy;
-        int fk = G.roi_off[2] + vz * p[2] + qz;
-        float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f;
-        if (fi < G.roi_end[0] && fj < G.roi_end[1] && fk < G.roi_end[2]) {
+    for (int base = 0; base < nvox; base += TILE_THREADS * G) {
+        int qx[G], qy[G], qz[G], vv[G];
+        bool act[G], in[G];
+        int fidx[G];
+        float fval[G];
+#pragma unroll
+        for (int u = 0; u < G; u++) {
+            vv[u] = base + u * TILE_THREADS + threadIdx.x;
+            qx[u] = cx; qy[u] = cy; qz[u] = cz;
+            cx += G_.vstep[0]; cy += G_.vstep[1]; cz += G_.vstep[2];
+            if (cx >= vx) { cx -= vx; cy++; }
+            if (cy >= vy) { cy -= vy; cz++; }
+            const int fi = bx0 + qx[u], fj = by0 + qy[u], fk = bz0 + qz[u];
+            act[u] = vv[u] < nvox && fi < G_.roi_end[0] && fj < G_.roi_end[1] && fk < G_.roi_end[2];
+            fidx[u] = act[u] ? fi + G_.fdim[0] * (fj + G_.fdim[1] * fk) : 0;
+            fval[u] = ld (fimg, fidx[u]);
+        }
+        Sample s[G];
+#pragma unroll
+        for (int u = 0; u < G; u++) {
             float fxyz[3], d[3];
-            position (fxyz, G, fi, fj, fk);
-            voxel_displacement (d, L, qx, qy, qz, vx);
-            Sample s;
-            if (correspond (G, fxyz, d, s)) {
-                float a[8];
-                corners (a, mimg, G, s.mvf);
-                float m_val = li_value (s, a);
-                int fidx = fi + G.fdim[0] * (fj + G.fdim[1] * fk);
-                float diff = m_val - ld (fimg, fidx);
-                score += (double) (diff * diff);
-                count++;
-                if (MODE == 0) {
-                    dc0 = diff * ld (mgrad, 3 * s.mvr + 0);
-                    dc1 = diff * ld (mgrad, 3 * s.mvr + 1);
-                    dc2 = diff * ld (mgrad, 3 * s.mvr + 2);
-                } else {
-                    float md[3];
-                    li_value_d (md, s, a, G.inv_msp);
-                    dc0 = diff * md[0];
-                    dc1 = diff * md[1];
-                    dc2 = diff * md[2];
+            position (fxyz, G_, bx0 + qx[u], by0 + qy[u], bz0 + qz[u]);
+            int qyc = qy[u] < vy ? qy[u] : 0, qzc = qz[u] < vz ? qz[u] : 0;
+            voxel_displacement (d, L, qx[u], qyc, qzc, vx);
+            in[u] = correspond (G_, fxyz, d, s[u]) && act[u];
+        }
+        float a[G][8];
+        float mg[G][3];
+#pragma unroll
+        for (int u = 0; u < G; u++) {
+            corners (a[u], mimg, G_, s[u].mvf);
+            if (KIND == 0) {
+                mg[u][0] = ld (mgrad, 3 * s[u].mvr + 0);
+                mg[u][1] = ld (mgrad, 3 * s[u].mvr + 1);
+                mg[u][2] = ld (mgrad, 3 * s[u].mvr + 2);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < G; u++) {
+            float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f;
+            if (KIND < 2) {
+                float m_val = li_value (s[u], a[u]);
+                float diff = m_val - fval[u];
+                if (in[u]) {
+                    score += (double) (diff * diff);
+                    count++;
+                    if (KIND == 0) {
+                        dc0 = diff * mg[u][0];
+                        dc1 = diff * mg[u][1];
+                        dc2 = diff * mg[u][2];
+                    } else {
+                        float md[3];
+                        li_value_d (md, s[u], a[u], G_.inv_msp);
+                        dc0 = diff * md[0];
+                        dc1 = diff * md[1];
+                        dc2 = diff * md[2];
+                    }
                 }
+            } else if (in[u]) {
+                const float* l1 = s[u].l1;
+                const float* l2 = s[u].l2;
+                float dw[24];
+                dw[0] = -l1[1] * l1[2];  dw[1] = -l1[0] * l1[2];  dw[2] = -(l1[0] * l1[1]);
+                dw[3] = l1[1] * l1[2];   dw[4] = -l2[0] * l1[2];  dw[5] = -(l2[0] * l1[1]);
+                dw[6] = -l2[1] * l1[2];  dw[7] = l1[0] * l1[2];   dw[8] = -(l1[0] * l2[1]);
+                dw[9] = l2[1] * l1[2];   dw[10] = l2[0] * l1[2];  dw[11] = -(l2[0] * l2[1]);
+                dw[12] = -l1[1] * l2[2]; dw[13] = -l1[0] * l2[2]; dw[14] = l1[0] * l1[1];
+                dw[15] = l1[1] * l2[2];  dw[16] = -l2[0] * l2[2]; dw[17] = l2[0] * l1[1];
+                dw[18] = -l2[1] * l2[2]; dw[19] = l1[0] * l2[2];  dw[20] = l1[0] * l2[1];
+                dw[21] = l2[1] * l2[2];  dw[22] = l2[0] * l2[2];  dw[23] = l2[0] * l2[1];
+                const int fb = hist_bin (fval[u], mi.h.f_offset, mi.h.f_delta, mi.h.fbins);
+                const float* row = &L.tab[fb * mi.h.mbins];
+#pragma unroll
+                for (int n = 0; n < 8; n++) {
+                    const int mb = hist_bin (a[u][n], mi.h.m_offset, mi.h.m_delta, mi.h.mbins);
+                    const float ds = row[mb];
+                    dc0 -= dw[3 * n + 0] * ds;
+                    dc1 -= dw[3 * n + 1] * ds;
+                    dc2 -= dw[3 * n + 2] * ds;
+                }
+                dc0 = dc0 / N;
+                dc1 = dc1 / N;
+                dc2 = dc2 / N;
+                // projected in place, component by component, as the
+                // reference does (src/register/bspline_mi.txx:107-109)
+                const float* P = G_.mproj;
+                dc0 = dc0 * P[0] + dc1 * P[1] + dc2 * P[2];
+                dc1 = dc0 * P[3] + dc1 * P[4] + dc2 * P[5];
+                dc2 = dc0 * P[6] + dc1 * P[7] + dc2 * P[8];
             }
-        }
-        L.dv[3 * v + 0] = dc0;
-        L.dv[3 * v + 1] = dc1;
-        L.dv[3 * v + 2] = dc2;
-    }
-    __syncthreads ();
-    condense_tile (L, G, tile, cond);
-    block_sum (L, score, count);
-    if (threadIdx.x == 0) {
-        tile_score[tile] = score;
-        tile_count[tile] = count;
-    }
-}
-
-// Sum the per-tile partial scores in a fixed order; score = raw/N,
-// gradient scale 2/N, or FLT_MAX and 0 when N < 1
-// (src/register/bspline_mse.cxx:43-73).
-__global__ __launch_bounds__ (1024)
-void k_mse_finalize (const double* __restrict__ tile_score,
-    const int* __restrict__ tile_count, int num_tiles,
-    double* __restrict__ out_score, int64_t* __restrict__ out_nvox,
-    double* __restrict__ out_gscale)
-{
-    __shared__ double s_s[1024];
-    __shared__ long long s_n[1024];
-    double s = 0.0;
-    long long n = 0;
-    for (int i = threadIdx.x; i < num_tiles; i += blockDim.x) {
-        s += tile_score[i];
-        n += tile_count[i];
-    }
-    s_s[threadIdx.x] = s;
-    s_n[threadIdx.x] = n;
-    __syncthreads ();
-    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-        if ((int) threadIdx.x < h) {
-            s_s[threadIdx.x] += s_s[threadIdx.x + h];
-            s_n[threadIdx.x] += s_n[threadIdx.x + h];
-        }
-        __syncthreads ();
-    }
-    if (threadIdx.x == 0) {
-        long long N = s_n[0];
-        if (N < 1) {
-            *out_score = (double) FLT_MAX;
-            *out_gscale = 0.0;
-        } else {
-            *out_score = s_s[0] / (double) N;
-            *out_gscale = 2.0 / (double) N;
-        }
-        *out_nvox = N;
-    }
-}
-
-// Per knot, sum its <= 64 tile slots in slot order in FP64 and scale.
-// Same result as bspline_condense_smetric_grad
-// (src/register/bspline.cxx:400-413) followed by the normalization.
-__global__ __launch_bounds__ (256)
-void k_grad_reduce (KGeom g, const float* __restrict__ cond,
-    const double* __restrict__ gscale, double* __restrict__ grad)
-{
-    const int knot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (knot >= g.num_knots) return;
-    const int kx = knot % g.cdims[0];
-    const int kt = knot / g.cdims[0];
-    const int ky = kt % g.cdims[1];
-    const int kz = kt / g.cdims[1];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int k = 0; k < 4; k++) {
-        int tz = kz - k;
-        if (tz < 0 || tz >= g.rdims[2]) continue;
-        for (int j = 0; j < 4; j++) {
-            int ty = ky - j;
-            if (ty < 0 || ty >= g.rdims[1]) continue;
-            for (int i = 0; i < 4; i++) {
-                int tx = kx - i;
-                if (tx < 0 || tx >= g.rdims[0]) continue;
-                int tile = tx + g.rdims[0] * (ty + g.rdims[1] * tz);
-                const float* c = &cond[(size_t) tile * 192 + ((k * 4 + j) * 4 + i) * 3];
-                a0 += (double) c[0];
-                a1 += (double) c[1];
-                a2 += (double) c[2];
+            if (vv[u] < nvox) {
+                L.dv[3 * vv[u] + 0] = dc0;
+                L.dv[3 * vv[u] + 1] = dc1;
+                L.dv[3 * vv[u] + 2] = dc2;
             }
         }
     }
-    const double s = *gscale;
-    grad[3 * knot + 0] = a0 * s;
-    grad[3 * knot + 1] = a1 * s;
-    grad[3 * knot + 2] = a2 * s;
+    __syncthreads ();
+    condense_tile (L, G_, p, cond);
+    if (KIND < 2) {
+        block_sum (L, score, count);
+        if (threadIdx.x == 0) {
+            tile_score[tile] = score;
+            atomicAdd (&ctr[0], (unsigned long long) count);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Mattes MI
+// Reduce: one wavefront per knot, lane = slot, FP64 butterfly (the same
+// bits on every lane, and run to run).  Same quantity as
+// bspline_condense_smetric_grad (src/register/bspline.cxx:400-413), scaled
+// by 2/N for MSE (src/register/bspline_mse.cxx:43-73) or 1 for MI.
+// MSE: block 0 also sums the per-tile scores in tile order and writes
+// score = raw/N (FLT_MAX if N < 1) and N.  The last block to finish clears
+// the counters for the next evaluation.
 
-__device__ __forceinline__ int
-hist_bin (float v, float offset, float delta, int bins)
+__device__ __forceinline__ double
+wave_sum (double v)
 {
-    int b = (int) floorf ((v - offset) / delta);
-    // In-range intensities always give 0..bins-1; the clamp only keeps a
-    // corrupted input from writing outside the histogram.
-    return b < 0 ? 0 : (b >= bins ? bins - 1 : b);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor (v, off);
+    return v;
 }
+
+template <bool MSE>
+__global__ __launch_bounds__ (256)
+void k_reduce (KGeom g, const float* __restrict__ cond, const double* __restrict__ tile_score,
+    unsigned long long* __restrict__ ctr, double* __restrict__ grad,
+    double* __restrict__ out_score, int64_t* __restrict__ out_nvox)
+{
+    __shared__ double s_red[256];
+    const long long N = MSE ? (long long) atomic_load_u64 (&ctr[0]) : 1;
+    const double scale = MSE ? (N >= 1 ? 2.0 / (double) N : 0.0) : 1.0;
+    const int lane = threadIdx.x & 63;
+    const int knot = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (knot < g.num_knots) {
+        const float* c = &cond[((size_t) knot * 64 + lane) * 3];
+        double a0 = wave_sum ((double) c[0]);
+        double a1 = wave_sum ((double) c[1]);
+        double a2 = wave_sum ((double) c[2]);
+        if (lane < 3) {
+            double v = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
+            grad[3 * knot + lane] = v * scale;
+        }
+    }
+    if (MSE && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < g.num_tiles; i += blockDim.x) s += tile_score[i];
+        s_red[threadIdx.x] = s;
+        __syncthreads ();
+        for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+            if ((int) threadIdx.x < h) s_red[threadIdx.x] += s_red[threadIdx.x + h];
+            __syncthreads ();
+        }
+        if (threadIdx.x == 0) {
+            *out_score = N >= 1 ? s_red[0] / (double) N : (double) FLT_MAX;
+            *out_nvox = N;
+        }
+    }
+    __syncthreads ();
+    if (threadIdx.x == 0) {
+        __threadfence ();
+        const unsigned long long t = atomicAdd (&ctr[1], 1ull);
+        if (t == gridDim.x - 1) {
+            atomicExch (&ctr[0], 0ull);
+            atomicExch (&ctr[1], 0ull);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mattes MI pass 1 and score
 
 // Displacement in the reference's k-j-i order, bx[i]*(by[j]*bz[k]) weights,
 // no FMA: bit-identical to bspline_interp_pix_c
 // (src/base/bspline_interpolate.cxx:78-110).
 __device__ __forceinline__ void
-exact_displacement (float out[3], const TileLds& L, int qx, int qy, int qz)
+exact_displacement (float out[3], const float* bx, const float* by, const float* bz,
+    const float* coef)
 {
-    const float* bx = &L.bx[4 * qx];
-    const float* by = &L.by[4 * qy];
-    const float* bz = &L.bz[4 * qz];
     float o0 = 0.f, o1 = 0.f, o2 = 0.f;
     int m = 0;
 #pragma unroll
@@ -530,7 +637,7 @@ exact_displacement (float out[3], const TileLds& L, int qx, int qy, int qz)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 float A = bx[i] * B;
-                const float* cf = &L.coef[3 * m];
+                const float* cf = &coef[3 * m];
                 o0 += A * cf[0];
                 o1 += A * cf[1];
                 o2 += A * cf[2];
@@ -572,15 +679,15 @@ void k_mi_hist (KGeom g, HGeom h, const TI* __restrict__ fimg,
     unsigned long long* __restrict__ fhist)
 {
     extern __shared__ float sm[];
-    const KGeom& G = g;
-    const int vx = G.vpr[0], vy = G.vpr[1], vz = G.vpr[2];
+    const KGeom& G_ = g;
+    const int vx = G_.vpr[0], vy = G_.vpr[1], vz = G_.vpr[2];
     float* s_bx = sm;
-    float* s_coef = s_bx + 4 * (vx + vy + vz);
+    float* s_by = s_bx + 4 * vx;
+    float* s_bz = s_by + 4 * vy;
+    float* s_coef = s_bz + 4 * vz;
     uintptr_t hp = ((uintptr_t) (s_coef + 192) + 7) & ~(uintptr_t) 7;
     unsigned long long* s_j = (unsigned long long*) hp;
     unsigned long long* s_f = s_j + h.fbins * h.mbins;
-    TileLds L;
-    L.bx = s_bx; L.by = s_bx + 4 * vx; L.bz = L.by + 4 * vy; L.coef = s_coef;
 
     const int nbins = h.fbins * h.mbins + h.fbins;
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) s_j[i] = 0ull;
@@ -588,28 +695,31 @@ void k_mi_hist (KGeom g, HGeom h, const TI* __restrict__ fimg,
     for (int i = threadIdx.x; i < nl; i += blockDim.x) s_bx[i] = blut[i];
 
     const int nvox = vx * vy * vz;
-    for (int tile = blockIdx.x; tile < G.num_tiles; tile += gridDim.x) {
+    for (int tile = blockIdx.x; tile < G_.num_tiles; tile += gridDim.x) {
         int p[3];
-        tile_coords (G, tile, p);
+        tile_coords (G_, tile, p);
         __syncthreads ();
         for (int e = threadIdx.x; e < 192; e += blockDim.x) {
             int c = e % 3, m = e / 3;
             int i = m & 3, j = (m >> 2) & 3, k = m >> 4;
-            int knot = (p[0] + i) + G.cdims[0] * ((p[1] + j) + G.cdims[1] * (p[2] + k));
+            int knot = (p[0] + i) + G_.cdims[0] * ((p[1] + j) + G_.cdims[1] * (p[2] + k));
             s_coef[e] = coeff[3 * knot + c];
         }
         __syncthreads ();
         for (int v = threadIdx.x; v < nvox; v += blockDim.x) {
-            int qx = v % vx, t = v / vx, qy = t % vy, qz = t / vy;
+            int qz = fdiv (v, G_.mag_vxy);
+            int r = v - qz * vx * vy;
+            int qy = fdiv (r, G_.mag_vx);
+            int qx = r - qy * vx;
             int fi = vx * p[0] + qx, fj = vy * p[1] + qy, fk = vz * p[2] + qz;
-            if (fi >= G.fdim[0] || fj >= G.fdim[1] || fk >= G.fdim[2]) continue;
+            if (fi >= G_.fdim[0] || fj >= G_.fdim[1] || fk >= G_.fdim[2]) continue;
             float fxyz[3], d[3];
-            position (fxyz, G, fi, fj, fk);
-            exact_displacement (d, L, qx, qy, qz);
+            position (fxyz, G_, fi, fj, fk);
+            exact_displacement (d, &s_bx[4 * qx], &s_by[4 * qy], &s_bz[4 * qz], s_coef);
             Sample s;
-            if (!correspond (G, fxyz, d, s)) continue;
+            if (!correspond (G_, fxyz, d, s)) continue;
             float a[8];
-            corners (a, mimg, G, s.mvf);
+            corners (a, mimg, G_, s.mvf);
             const float* l1 = s.l1;
             const float* l2 = s.l2;
             float w[8];
@@ -623,7 +733,7 @@ void k_mi_hist (KGeom g, HGeom h, const TI* __restrict__ fimg,
             w[7] = l2[0] * l2[1] * l2[2];
             long long q[8];
             pv_quantize (w, q);
-            int fidx = fi + G.fdim[0] * (fj + G.fdim[1] * fk);
+            int fidx = fi + G_.fdim[0] * (fj + G_.fdim[1] * fk);
             int fb = hist_bin (ld (fimg, fidx), h.f_offset, h.f_delta, h.fbins);
             atomicAdd (&s_f[fb], 1ull);
             unsigned long long* row = &s_j[fb * h.mbins];
@@ -704,87 +814,8 @@ void k_mi_score (HGeom h, const unsigned long long* __restrict__ jhist,
     }
 }
 
-// Pass 2: PV-8 derivative per voxel, condensed per tile.  Follows the
-// pass-2 loop of bspline_score_g_mi (src/register/bspline_mi.cxx:1627-1716)
-// and bspline_mi_pvi_8_dc_dv_dcos (src/register/bspline_mi.txx:9-120).
-template <typename TI>
-__global__ __launch_bounds__ (TILE_THREADS)
-void k_mi_tiles (KGeom g, HGeom h, const TI* __restrict__ fimg,
-    const TI* __restrict__ mimg, const float* __restrict__ coeff,
-    const float* __restrict__ blut, const float* __restrict__ table,
-    const float* __restrict__ nvox_f, float* __restrict__ cond)
-{
-    extern __shared__ float sm[];
-    const int tile = xcd_tile (g, blockIdx.x);
-    if (tile >= g.num_tiles) return;
-    const KGeom& G = g;
-    TileLds L = carve (sm, G);
-    float* s_tab = (float*) (L.red + 16);
-    const int nj = h.fbins * h.mbins;
-    for (int i = threadIdx.x; i < nj; i += blockDim.x) s_tab[i] = table[i];
-    int p[3];
-    tile_coords (G, tile, p);
-    load_tile (L, G, p, coeff, blut);
-    __syncthreads ();
-    separable_displacement (L, G);
-
-    const float N = *nvox_f;
-    const int vx = G.vpr[0], vy = G.vpr[1], vz = G.vpr[2];
-    const int nvox = vx * vy * vz;
-    const float* P = G.mproj;
-    for (int v = threadIdx.x; v < nvox; v += blockDim.x) {
-        int qx = v % vx, t = v / vx, qy = t % vy, qz = t / vy;
-        int fi = G.roi_off[0] + vx * p[0] + qx;
-        int fj = G.roi_off[1] + vy * p[1] + qy;
-        int fk = G.roi_off[2] + vz * p[2] + qz;
-        float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f;
-        if (fi < G.roi_end[0] && fj < G.roi_end[1] && fk < G.roi_end[2]) {
-            float fxyz[3], d[3];
-            position (fxyz, G, fi, fj, fk);
-            voxel_displacement (d, L, qx, qy, qz, vx);
-            Sample s;
-            if (correspond (G, fxyz, d, s)) {
-                float a[8];
-                corners (a, mimg, G, s.mvf);
-                const float* l1 = s.l1;
-                const float* l2 = s.l2;
-                float dw[24];
-                dw[0] = -l1[1] * l1[2];  dw[1] = -l1[0] * l1[2];  dw[2] = -(l1[0] * l1[1]);
-                dw[3] = l1[1] * l1[2];   dw[4] = -l2[0] * l1[2];  dw[5] = -(l2[0] * l1[1]);
-                dw[6] = -l2[1] * l1[2];  dw[7] = l1[0] * l1[2];   dw[8] = -(l1[0] * l2[1]);
-                dw[9] = l2[1] * l1[2];   dw[10] = l2[0] * l1[2];  dw[11] = -(l2[0] * l2[1]);
-                dw[12] = -l1[1] * l2[2]; dw[13] = -l1[0] * l2[2]; dw[14] = l1[0] * l1[1];
-                dw[15] = l1[1] * l2[2];  dw[16] = -l2[0] * l2[2]; dw[17] = l2[0] * l1[1];
-                dw[18] = -l2[1] * l2[2]; dw[19] = l1[0] * l2[2];  dw[20] = l1[0] * l2[1];
-                dw[21] = l2[1] * l2[2];  dw[22] = l2[0] * l2[2];  dw[23] = l2[0] * l2[1];
-                int fidx = fi + G.fdim[0] * (fj + G.fdim[1] * fk);
-                int fb = hist_bin (ld (fimg, fidx), h.f_offset, h.f_delta, h.fbins);
-                const float* row = &s_tab[fb * h.mbins];
-#pragma unroll
-                for (int n = 0; n < 8; n++) {
-                    int mb = hist_bin (a[n], h.m_offset, h.m_delta, h.mbins);
-                    float ds = row[mb];
-                    dc0 -= dw[3 * n + 0] * ds;
-                    dc1 -= dw[3 * n + 1] * ds;
-                    dc2 -= dw[3 * n + 2] * ds;
-                }
-                dc0 = dc0 / N;
-                dc1 = dc1 / N;
-                dc2 = dc2 / N;
-                // projected in place, component by component, as the
-                // reference does (src/register/bspline_mi.txx:107-109)
-                dc0 = dc0 * P[0] + dc1 * P[1] + dc2 * P[2];
-                dc1 = dc0 * P[3] + dc1 * P[4] + dc2 * P[5];
-                dc2 = dc0 * P[6] + dc1 * P[7] + dc2 * P[8];
-            }
-        }
-        L.dv[3 * v + 0] = dc0;
-        L.dv[3 * v + 1] = dc1;
-        L.dv[3 * v + 2] = dc2;
-    }
-    __syncthreads ();
-    condense_tile (L, G, tile, cond);
-}
+// ---------------------------------------------------------------------------
+// small kernels
 
 // Central-difference gradient with edge replicate, rotated by inv(dc),
 // float3 interleaved.  The difference is formed in float, divided by 2 and
@@ -821,13 +852,19 @@ void k_volume_gradient (const float* __restrict__ img, float* __restrict__ out,
     }
 }
 
-__global__ void k_fill_one (double* p) { *p = 1.0; }
-
 __global__ void
 k_zero_u64 (unsigned long long* __restrict__ p, int n)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         p[i] = 0ull;
+}
+
+__global__ void
+k_zero_f32 (float* __restrict__ p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
+         i += (int64_t) gridDim.x * blockDim.x)
+        p[i] = 0.f;
 }
 
 // float32 -> bfloat16, round to nearest even (NaN kept quiet).
@@ -872,51 +909,47 @@ mark (hipEvent_t* ev, int i, hipStream_t st)
         if (e_ != hipSuccess) return e_;                                \
     } while (0)
 
+template <int KIND, typename TI>
+static void
+launch_tiles (const KGeom& g, size_t lds, hipStream_t st, const void* f, const void* m,
+    const void* mg, const float* coeff, const float* blut, float* cond, double* tile_score,
+    unsigned long long* ctr, const MiArgs& mi)
+{
+    hipLaunchKernelGGL ((k_tiles<KIND, TI>), dim3 (8 * g.tiles_per_xcd), dim3 (TILE_THREADS),
+        lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff, blut, cond, tile_score,
+        ctr, mi);
+}
+
 hipError_t
 launch_mse (const KGeom& g, int mode, int storage, const void* f, const void* m,
     const void* mg, const float* coeff, const float* blut, float* cond,
-    double* tile_score, int* tile_count, double* score, int64_t* nvox,
-    double* gscale, double* grad, hipStream_t st, hipEvent_t* ev)
+    double* tile_score, unsigned long long* ctr, double* score, int64_t* nvox,
+    double* grad, hipStream_t st, hipEvent_t* ev)
 {
-    dim3 grid (8 * g.tiles_per_xcd), block (TILE_THREADS);
-    size_t lds = tile_lds_bytes (g);
+    const size_t lds = tile_lds_bytes (g);
+    MiArgs mi {};
     hipError_t e = mark (ev, 0, st);
     if (e != hipSuccess) return e;
     if (storage == 0) {
-        const float *F = (const float*) f, *M = (const float*) m, *MG = (const float*) mg;
-        if (mode == 0)
-            hipLaunchKernelGGL ((k_mse_tiles<0, float>), grid, block, lds, st, g, F, M, MG,
-                coeff, blut, cond, tile_score, tile_count);
-        else
-            hipLaunchKernelGGL ((k_mse_tiles<1, float>), grid, block, lds, st, g, F, M, MG,
-                coeff, blut, cond, tile_score, tile_count);
+        if (mode == 0) launch_tiles<0, float> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, ctr, mi);
+        else launch_tiles<1, float> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, ctr, mi);
     } else {
-        const uint16_t *F = (const uint16_t*) f, *M = (const uint16_t*) m,
-            *MG = (const uint16_t*) mg;
-        if (mode == 0)
-            hipLaunchKernelGGL ((k_mse_tiles<0, uint16_t>), grid, block, lds, st, g, F, M, MG,
-                coeff, blut, cond, tile_score, tile_count);
-        else
-            hipLaunchKernelGGL ((k_mse_tiles<1, uint16_t>), grid, block, lds, st, g, F, M, MG,
-                coeff, blut, cond, tile_score, tile_count);
+        if (mode == 0) launch_tiles<0, uint16_t> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, ctr, mi);
+        else launch_tiles<1, uint16_t> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, ctr, mi);
     }
     LAUNCH_CHECK ();
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL (k_mse_finalize, dim3 (1), dim3 (1024), 0, st, tile_score,
-        tile_count, g.num_tiles, score, nvox, gscale);
+    hipLaunchKernelGGL (k_reduce<true>, dim3 ((g.num_knots + 3) / 4), dim3 (256), 0, st, g,
+        cond, tile_score, ctr, grad, score, nvox);
     LAUNCH_CHECK ();
-    if ((e = mark (ev, 2, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL (k_grad_reduce, dim3 ((g.num_knots + 255) / 256), dim3 (256), 0, st,
-        g, cond, gscale, grad);
-    LAUNCH_CHECK ();
-    return mark (ev, 3, st);
+    return mark (ev, 2, st);
 }
 
 hipError_t
 launch_mi (const KGeom& g, const HGeom& h, int storage, int hist_blocks,
     const void* f, const void* m, const float* coeff, const float* blut,
     unsigned long long* jhist, unsigned long long* fhist, float* table,
-    float* nvox_f, float* cond, const double* one, double* score, int64_t* nvox,
+    float* nvox_f, float* cond, unsigned long long* ctr, double* score, int64_t* nvox,
     double* grad, int64_t* hist_out, hipStream_t st, hipEvent_t* ev)
 {
     const int nbins = h.fbins * h.mbins + h.fbins;
@@ -928,38 +961,38 @@ launch_mi (const KGeom& g, const HGeom& h, int storage, int hist_blocks,
     const size_t lds1 = ((4 * (g.vpr[0] + g.vpr[1] + g.vpr[2]) + 192) * 4 + 8)
         + sizeof (unsigned long long) * nbins;
     const size_t lds2 = tile_lds_bytes (g) + sizeof (float) * h.fbins * h.mbins;
-    dim3 grid (8 * g.tiles_per_xcd), block (TILE_THREADS);
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
     if (storage == 0)
-        hipLaunchKernelGGL ((k_mi_hist<float>), dim3 (hist_blocks), block, lds1, st, g, h,
-            (const float*) f, (const float*) m, coeff, blut, jhist, fhist);
+        hipLaunchKernelGGL ((k_mi_hist<float>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
+            g, h, (const float*) f, (const float*) m, coeff, blut, jhist, fhist);
     else
-        hipLaunchKernelGGL ((k_mi_hist<uint16_t>), dim3 (hist_blocks), block, lds1, st, g, h,
-            (const uint16_t*) f, (const uint16_t*) m, coeff, blut, jhist, fhist);
+        hipLaunchKernelGGL ((k_mi_hist<uint16_t>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1,
+            st, g, h, (const uint16_t*) f, (const uint16_t*) m, coeff, blut, jhist, fhist);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 2, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_mi_score, dim3 (1), dim3 (1024), 0, st, h, jhist, fhist, table,
         score, nvox, nvox_f, hist_out);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 3, st)) != hipSuccess) return e;
+    MiArgs mi {h, table, nvox_f};
     if (storage == 0)
-        hipLaunchKernelGGL ((k_mi_tiles<float>), grid, block, lds2, st, g, h, (const float*) f,
-            (const float*) m, coeff, blut, table, nvox_f, cond);
+        launch_tiles<2, float> (g, lds2, st, f, m, nullptr, coeff, blut, cond, nullptr, ctr, mi);
     else
-        hipLaunchKernelGGL ((k_mi_tiles<uint16_t>), grid, block, lds2, st, g, h,
-            (const uint16_t*) f, (const uint16_t*) m, coeff, blut, table, nvox_f, cond);
+        launch_tiles<2, uint16_t> (g, lds2, st, f, m, nullptr, coeff, blut, cond, nullptr, ctr, mi);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 4, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL (k_grad_reduce, dim3 ((g.num_knots + 255) / 256), dim3 (256), 0, st,
-        g, cond, one, grad);
+    hipLaunchKernelGGL (k_reduce<false>, dim3 ((g.num_knots + 3) / 4), dim3 (256), 0, st, g,
+        cond, nullptr, ctr, grad, nullptr, nullptr);
     LAUNCH_CHECK ();
     return mark (ev, 5, st);
 }
 
 hipError_t
-launch_fill_one (double* p, hipStream_t st)
+launch_zero (float* p, int64_t n, unsigned long long* ctr, int nctr, hipStream_t st)
 {
-    hipLaunchKernelGGL (k_fill_one, dim3 (1), dim3 (1), 0, st, p);
+    hipLaunchKernelGGL (k_zero_f32, dim3 (1024), dim3 (256), 0, st, p, n);
+    LAUNCH_CHECK ();
+    hipLaunchKernelGGL (k_zero_u64, dim3 (1), dim3 (64), 0, st, ctr, nctr);
     return hipGetLastError ();
 }
 

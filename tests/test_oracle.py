@@ -1,0 +1,310 @@
+"""Pins of the CPU oracle (no GPU).
+
+Each check fixes a part of the oracle independently of its own code: closed
+forms of the cubic B-spline basis, reproduction of constant and linear
+fields, exactness of central differences on quadratics, equality of the tile
+condense with the voxel-order scatter, finite differences of the FP64 score,
+histogram mass and the S = -H(F) special case for MI, and golden vectors from
+an independent float64 brute force (tests/golden/bruteforce.py).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from plastimatch_amd import synth
+from tests.helpers import oracle_case, rel, rel_scalar, small_pair
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bspline_tiny.json")
+
+
+# ---------------------------------------------------------------- basis / grid
+
+@pytest.mark.parametrize("vpr", [2, 4, 5, 8, 10, 12, 15])
+def test_basis_closed_forms(vpr):
+    lut = oracle.basis_lut(vpr).astype(np.float64)
+    u = np.arange(vpr) / vpr
+    exact = np.stack([(1 - u) ** 3, 3 * u ** 3 - 6 * u ** 2 + 4, -3 * u ** 3 + 3 * u ** 2 + 3 * u + 1,
+                      u ** 3], axis=1) / 6.0
+    assert np.max(np.abs(lut - exact)) < 2e-7
+    assert np.max(np.abs(lut.sum(axis=1) - 1.0)) < 3e-7
+    assert np.allclose(lut[0], [1 / 6, 4 / 6, 1 / 6, 0], atol=1e-7)
+    if vpr % 2 == 0:
+        assert np.allclose(lut[vpr // 2], [1 / 48, 23 / 48, 23 / 48, 1 / 48], atol=1e-7)
+    # linear reproduction: sum_t B_t(u) (t - 1) = u
+    assert np.max(np.abs(lut @ np.array([-1.0, 0.0, 1.0, 2.0]) - u)) < 5e-7
+
+
+@pytest.mark.parametrize("name,rdims,cdims,nk", [
+    ("mse_128cube_vpr15", (9, 9, 9), (12, 12, 12), 1728),
+    ("mse_256x256x128_vpr10", (26, 26, 13), (29, 29, 16), 13456),
+    ("mi_512x512x160_vpr12", (43, 43, 14), (46, 46, 17), 35972),
+    ("adjoint_192x192x96_vpr8", (24, 24, 12), (27, 27, 15), 10935),
+])
+def test_grid_geometry(name, rdims, cdims, nk):
+    cfg = synth.CONFIGS[name]
+    g = oracle.geom(cfg["dim"], cfg["spacing"], cfg["vox_per_rgn"])
+    r, c, n = oracle.header(g)
+    assert tuple(r) == rdims and tuple(c) == cdims and n == nk
+    assert synth.grid_dims(cfg["dim"], cfg["vox_per_rgn"]) == (rdims, cdims)
+
+
+def test_direction_matrices_rotation():
+    th = 0.3
+    dc = (np.cos(th), -np.sin(th), 0.0, np.sin(th), np.cos(th), 0.0, 0.0, 0.0, 1.0)
+    sp = (0.5, 2.0, 3.0)
+    step, proj, inv = oracle.direction_matrices(dc, sp)
+    S = np.array(step, np.float64).reshape(3, 3)
+    P = np.array(proj, np.float64).reshape(3, 3)
+    assert np.allclose(P @ S, np.eye(3), atol=1e-6)
+    assert np.allclose(S, np.array(dc).reshape(3, 3) * np.array(sp)[None, :], atol=1e-6)
+
+
+# ---------------------------------------------------------------- displacement
+
+def test_interp_zero_constant_linear():
+    dim, vpr, sp = (23, 17, 13), (5, 4, 3), (1.0, 1.0, 2.0)
+    g = oracle.geom(dim, sp, vpr)
+    _, cd, nk = oracle.header(g)
+    assert np.all(oracle.interp_vf(g, np.zeros(3 * nk, np.float32)) == 0)
+    const = np.tile(np.array([1.5, -2.0, 0.25], np.float32), nk)
+    vf = oracle.interp_vf(g, const)
+    assert np.max(np.abs(vf - const[:3])) < 1e-5
+    # linear field: knot (ki, kj, kk) at voxel ((ki-1)*vpr0, ...); coefficient
+    # = a . knot position reproduces d(i,j,k) = a . (i,j,k)
+    a = np.array([[0.1, -0.2, 0.05], [0.03, 0.07, -0.04], [-0.02, 0.01, 0.09]])   # rows: x,y,z
+    kk, kj, ki = np.meshgrid(np.arange(cd[2]), np.arange(cd[1]), np.arange(cd[0]), indexing="ij")
+    pos = np.stack([(ki - 1) * vpr[0], (kj - 1) * vpr[1], (kk - 1) * vpr[2]], axis=-1).astype(np.float64)
+    coeff = (pos @ a).astype(np.float32).reshape(-1)
+    vf = oracle.interp_vf(g, coeff)
+    z, y, x = np.meshgrid(np.arange(dim[2]), np.arange(dim[1]), np.arange(dim[0]), indexing="ij")
+    expect = np.stack([x, y, z], axis=-1) @ a
+    assert np.max(np.abs(vf - expect)) < 1e-4
+
+
+# ---------------------------------------------------------------- moving gradient
+
+def test_volume_gradient_quadratic_and_edges():
+    nz, ny, nx = 9, 8, 7
+    sp = (0.5, 2.0, 1.25)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    img = (0.5 * x ** 2 - 3.0 * y + 0.25 * z ** 2 + x * 0.0).astype(np.float32)
+    g = oracle.volume_gradient(img, sp)
+    inner = (slice(1, -1), slice(1, -1), slice(1, -1))
+    assert np.allclose(g[inner][..., 0], (x[inner]) / sp[0], atol=1e-5)
+    assert np.allclose(g[inner][..., 1], -3.0 / sp[1], atol=1e-5)
+    assert np.allclose(g[inner][..., 2], 0.5 * z[inner] / sp[2], atol=1e-5)
+    # edge replicate: (m[1] - m[0]) / 2 / sp
+    assert np.allclose(g[:, :, 0, 0], (img[:, :, 1] - img[:, :, 0]) / 2 / sp[0], atol=1e-6)
+
+
+def test_volume_gradient_rotated():
+    th = 0.4
+    dc = (np.cos(th), -np.sin(th), 0.0, np.sin(th), np.cos(th), 0.0, 0.0, 0.0, 1.0)
+    img = np.random.default_rng(0).normal(size=(6, 7, 8)).astype(np.float32)
+    g0 = oracle.volume_gradient(img, (1.0, 1.0, 1.0))
+    g1 = oracle.volume_gradient(img, (1.0, 1.0, 1.0), dc=dc)
+    _, _, inv = oracle.direction_matrices(dc, (1.0, 1.0, 1.0))
+    expect = g0.astype(np.float64) @ np.array(inv, np.float64).reshape(3, 3).T
+    assert np.allclose(g1, expect, atol=1e-5)
+
+
+# ---------------------------------------------------------------- MSE
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_tile_condense_equals_voxel_scatter(seed):
+    """Flavor g (tile condense) and flavor c (voxel-order scatter) compute the
+    same quantity with different summation orders."""
+    pair = small_pair(dim=(31, 27, 19), spacing=(1.0, 0.5, 2.0), vpr=(6, 5, 4), seed=seed,
+                      origin=(-7.0, 2.5, 10.0))
+    oc = oracle_case(pair)
+    coeff = synth.random_coeff(pair.dim, pair.vox_per_rgn, 1.5, seed=seed)
+    sg, gg, ng, _ = oc.mse(coeff, "g")
+    sc, gc, nc, _ = oc.mse(coeff, "c")
+    s64, g64, n64 = oc.mse_f64(coeff, 0)
+    assert ng == nc == n64
+    assert rel_scalar(sg, sc) < 1e-6 and rel_scalar(sg, s64) < 1e-6
+    assert rel(gg, gc) < 2e-5 and rel(gg, g64) < 2e-5
+
+
+def test_identical_images_and_constant_offset():
+    pair = small_pair()
+    oc = oracle.Case(pair.fixed, pair.fixed, pair.spacing, pair.vox_per_rgn)
+    z = np.zeros(oc.num_coeff, np.float32)
+    s, g, n, _ = oc.mse(z, "g")
+    assert s == 0.0 and np.all(g == 0) and n == pair.fixed.size
+    oc2 = oracle.Case(pair.fixed, pair.fixed + np.float32(3.0), pair.spacing, pair.vox_per_rgn)
+    s, g, n, _ = oc2.mse(z, "p")
+    assert abs(s - 9.0) < 1e-4
+
+
+def test_no_overlap_flt_max():
+    pair = small_pair()
+    oc = oracle_case(pair)
+    s, g, n, _ = oc.mse(np.full(oc.num_coeff, 1e4, np.float32), "g")
+    assert n == 0 and s == np.finfo(np.float32).max and np.all(g == 0)
+
+
+def test_linear_ramp_nn_gradient_is_exact():
+    """On a linear moving image the central difference is the exact slope, so
+    flavors g and p agree away from the border, and both match central finite
+    differences of the FP64 score."""
+    nz, ny, nx = 20, 22, 24
+    sp = (1.0, 1.0, 2.0)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    moving = (3.0 * x * sp[0] - 2.0 * y * sp[1] + 1.5 * z * sp[2]).astype(np.float32)
+    fixed = (moving + np.random.default_rng(3).normal(0, 2.0, moving.shape)).astype(np.float32)
+    oc = oracle.Case(fixed, moving, sp, (5, 5, 5), roi_offset=(3, 3, 3), roi_dim=(18, 16, 14))
+    coeff = synth.random_coeff((18, 16, 14), (5, 5, 5), 0.3, seed=4)
+    _, gg, _, _ = oc.mse(coeff, "g")
+    _, gp, _, _ = oc.mse(coeff, "p")
+    assert rel(gg, gp) < 1e-5
+    _, g64, _ = oc.mse_f64(coeff, 1)
+    idx = np.argsort(-np.abs(g64))[:12]
+    h = 1e-3
+    fd = []
+    for i in idx:
+        a = coeff.copy(); a[i] += h
+        b = coeff.copy(); b[i] -= h
+        fd.append((oc.mse_f64(a, 1)[0] - oc.mse_f64(b, 1)[0]) / (2 * h))
+    assert rel(g64[idx], np.array(fd)) < 1e-3
+
+
+def test_exact_mode_matches_finite_differences():
+    """Flavor p's derivative is the exact adjoint wherever the trilinear sample
+    is not clamped; an interior ROI keeps every warped voxel off the border
+    (at the border li_value_dx keeps its slope while the clamped value does
+    not move, src/base/interpolate.cxx:17-41,162-244)."""
+    pair = small_pair(dim=(32, 30, 24), spacing=(1.0, 1.0, 1.0), vpr=(6, 6, 6), ct=False, seed=5)
+    oc = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn, roi_offset=(4, 4, 4),
+                     roi_dim=(24, 22, 16))
+    coeff = synth.random_coeff((24, 22, 16), pair.vox_per_rgn, 1.0, seed=9)
+    _, g64, _ = oc.mse_f64(coeff, 1)
+    _, g32, _, _ = oc.mse(coeff, "p")
+    assert rel(g32, g64) < 1e-5
+    idx = np.argsort(-np.abs(g64))[:16]
+    h = 1e-3
+    fd = []
+    for i in idx:
+        a = coeff.copy(); a[i] += h
+        b = coeff.copy(); b[i] -= h
+        fd.append((oc.mse_f64(a, 1)[0] - oc.mse_f64(b, 1)[0]) / (2 * h))
+    assert rel(g64[idx], np.array(fd)) < 2e-4
+
+
+# ---------------------------------------------------------------- MI
+
+def test_pv_quantize_exact_mass():
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        l2 = rng.uniform(0, 1, 3).astype(np.float32)
+        l1 = (np.float32(1) - l2).astype(np.float32)
+        w = np.array([l1[0] * l1[1] * l1[2], l2[0] * l1[1] * l1[2], l1[0] * l2[1] * l1[2],
+                      l2[0] * l2[1] * l1[2], l1[0] * l1[1] * l2[2], l2[0] * l1[1] * l2[2],
+                      l1[0] * l2[1] * l2[2], l2[0] * l2[1] * l2[2]], np.float32)
+        q = oracle.pv_quantize(w)
+        assert q.sum() == 1 << 32
+        assert np.all(q >= 0)
+        exact = w.astype(np.float64) * 2.0 ** 32
+        amax = int(np.argmax(w))
+        others = [n for n in range(8) if n != amax]
+        assert np.max(np.abs(q[others] - exact[others])) <= 0.5
+        # the largest weight absorbs the float weights' departure from 1
+        slack = abs(1.0 - w.astype(np.float64).sum()) * 2.0 ** 32 + 4.0
+        assert abs(q[amax] - exact[amax]) <= slack
+
+
+def test_mi_histogram_mass_and_quantization():
+    pair = small_pair(dim=(30, 28, 20), vpr=(5, 5, 5), seed=6)
+    oc = oracle_case(pair)
+    hp = oc.hist(32)
+    r = oc.mi(pair.true_coeff, hp)
+    n = r["num_vox"]
+    assert r["fx_f_hist"].sum() == n and r["f_hist"].sum() == n
+    assert r["fx_j_hist"].sum() == n << 32
+    assert abs(r["j_hist"].sum() - n) < 1e-6 * n
+    assert abs(r["m_hist"].sum() - n) < 1e-6 * n
+    # the fixed-point joint histogram is the double one to within the rounding
+    assert np.max(np.abs(r["fx_j_hist"] / 2.0 ** 32 - r["j_hist"])) < 1e-6 * n
+
+
+def test_mi_identical_images_is_minus_entropy():
+    pair = small_pair(dim=(24, 22, 18), vpr=(5, 5, 5), seed=7)
+    oc = oracle.Case(pair.fixed, pair.fixed, pair.spacing, pair.vox_per_rgn)
+    hp = oc.hist(16)
+    r = oc.mi(np.zeros(oc.num_coeff, np.float32), hp)
+    j = r["j_hist"].reshape(16, 16)
+    assert np.all(j == np.diag(np.diag(j)))
+    p = r["f_hist"][r["f_hist"] > 0] / r["num_vox"]
+    h = -np.sum(p * np.log(p))
+    assert abs(r["score"] + h) < 1e-5 * h
+    s64, _, _ = oc.mi_f64(np.zeros(oc.num_coeff, np.float32), hp)
+    assert abs(s64 + h) < 1e-9 * h
+
+
+def test_pass1_positions_exact_for_dyadic_geometry():
+    for name in ("mi_512x512x160_vpr12",):
+        cfg = synth.CONFIGS[name]
+        dim = cfg["dim"]
+        fixed = np.zeros((dim[2], dim[1], dim[0]), np.float32)
+        g = oracle.geom(dim, cfg["spacing"], cfg["vox_per_rgn"])
+        assert oracle.pass1_position_mismatch(fixed, cfg["spacing"], g) == 0
+
+
+def test_mi_f64_gradient_vs_finite_differences():
+    pair = small_pair(dim=(30, 28, 22), spacing=(1.0, 1.0, 1.0), vpr=(6, 6, 6), seed=8, ct=False,
+                      n_blobs=12)
+    oc = oracle_case(pair)
+    hp = oc.hist(16)
+    coeff = synth.random_coeff(pair.dim, pair.vox_per_rgn, 1.0, seed=3)
+    _, g64, _ = oc.mi_f64(coeff, hp)
+    idx = np.argsort(-np.abs(g64))[:12]
+    h = 1e-3
+    fd = []
+    for i in idx:
+        a = coeff.copy(); a[i] += h
+        b = coeff.copy(); b[i] -= h
+        fd.append((oc.mi_f64(a, hp)[0] - oc.mi_f64(b, hp)[0]) / (2 * h))
+    fd = np.array(fd)
+    assert np.all(np.sign(fd) == np.sign(g64[idx]))
+    assert rel(g64[idx], fd) < 0.1
+
+
+# ---------------------------------------------------------------- golden vectors
+
+def _golden_cases():
+    with open(GOLDEN) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("case", _golden_cases(), ids=lambda c: c["name"])
+def test_golden_bruteforce(case):
+    dim, vpr, sp = case["dim"], case["vox_per_rgn"], case["spacing"]
+    shape = (dim[2], dim[1], dim[0])
+    fixed = np.array(case["fixed"], np.float32).reshape(shape)
+    moving = np.array(case["moving"], np.float32).reshape(shape)
+    coeff = np.array(case["coeff"], np.float32)
+    oc = oracle.Case(fixed, moving, sp, vpr)
+    assert np.array_equal(oc.m_grad.reshape(-1), np.array(case["m_grad"], np.float32))
+    for mode, key, flavor in ((0, "mse_nn", "g"), (1, "mse_exact", "p")):
+        exp = case[key]
+        s64, g64, n64 = oc.mse_f64(coeff, mode)
+        assert n64 == exp["num_vox"]
+        assert rel_scalar(s64, exp["score"]) < 1e-12
+        assert rel(g64, exp["grad"]) < 1e-10
+        s, g, n, _ = oc.mse(coeff, flavor)
+        assert n == exp["num_vox"]
+        assert rel_scalar(s, exp["score"]) < 1e-5
+        assert rel(g, exp["grad"]) < 1e-4
+    m = case["mi"]
+    hp = oracle.OrcHist(m["bins"], m["bins"], m["f_offset"], m["f_delta"], m["m_offset"], m["m_delta"])
+    s64, g64, n64 = oc.mi_f64(coeff, hp)
+    assert n64 == m["num_vox"]
+    assert rel_scalar(s64, m["score"]) < 1e-10
+    assert rel(g64, m["grad"]) < 1e-8
+    r = oc.mi(coeff, hp)
+    assert r["num_vox"] == m["num_vox"]
+    assert rel_scalar(r["score"], m["score"]) < 1e-5
+    assert rel(r["grad"], m["grad"]) < 1e-4
