@@ -1,0 +1,467 @@
+/* C++ host side of the threading=hip B-spline flavor; see bspline_hip_host.h. */
+#include "bspline_hip_host.h"
+
+#include <dlfcn.h>
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <chrono>
+#include <string>
+
+/* ---------------------------------------------------------------------
+   Plugin loading (the LOAD_LIBRARY_SAFE / LOAD_SYMBOL pattern of
+   src/sys/cuda_delayload.h:26-59, without the exit(0) on failure)
+   --------------------------------------------------------------------- */
+
+namespace {
+
+struct Hip_plugin {
+    void* handle = nullptr;
+#define PLM_SYM(name) decltype (&::name) name = nullptr;
+    PLM_SYM (plm_hip_abi_version)
+    PLM_SYM (plm_hip_last_error)
+    PLM_SYM (plm_hip_bspline_create)
+    PLM_SYM (plm_hip_bspline_score)
+    PLM_SYM (plm_hip_bspline_score_device)
+    PLM_SYM (plm_hip_bspline_destroy)
+    PLM_SYM (plm_hip_volume_gradient)
+    PLM_SYM (plm_hip_bspline_num_knots)
+#undef PLM_SYM
+};
+
+Hip_plugin g_plugin;
+
+std::string
+plugin_dir ()
+{
+    Dl_info info;
+    if (dladdr ((void*) &plugin_dir, &info) && info.dli_fname) {
+        std::string p (info.dli_fname);
+        size_t s = p.rfind ('/');
+        if (s != std::string::npos) return p.substr (0, s);
+    }
+    return ".";
+}
+
+std::string
+hip_error ()
+{
+    const char* m = g_plugin.plm_hip_last_error ? g_plugin.plm_hip_last_error () : nullptr;
+    return m ? m : "";
+}
+
+} // namespace
+
+bool
+hip_plugin_load (std::string* err)
+{
+    if (g_plugin.handle) return true;
+    std::vector<std::string> cands;
+    if (const char* env = getenv ("PLM_HIP_LIBRARY")) cands.push_back (env);
+    cands.push_back (plugin_dir () + "/../libplmregisterhip.so");
+    cands.push_back (plugin_dir () + "/libplmregisterhip.so");
+    cands.push_back ("libplmregisterhip.so");
+    void* h = nullptr;
+    std::string tried;
+    for (const std::string& c : cands) {
+        h = dlopen (c.c_str (), RTLD_NOW | RTLD_LOCAL);
+        if (h) break;
+        tried += " " + c;
+    }
+    if (!h) {
+        if (err) *err = "cannot load libplmregisterhip.so (tried" + tried + ")";
+        return false;
+    }
+#define PLM_LOAD(name)                                                      \
+    g_plugin.name = (decltype (g_plugin.name)) dlsym (h, #name);            \
+    if (!g_plugin.name) {                                                   \
+        if (err) *err = "missing symbol " #name;                            \
+        dlclose (h);                                                        \
+        return false;                                                       \
+    }
+    PLM_LOAD (plm_hip_abi_version)
+    PLM_LOAD (plm_hip_last_error)
+    PLM_LOAD (plm_hip_bspline_create)
+    PLM_LOAD (plm_hip_bspline_score)
+    PLM_LOAD (plm_hip_bspline_score_device)
+    PLM_LOAD (plm_hip_bspline_destroy)
+    PLM_LOAD (plm_hip_volume_gradient)
+    PLM_LOAD (plm_hip_bspline_num_knots)
+#undef PLM_LOAD
+    if (g_plugin.plm_hip_abi_version () != PLM_HIP_ABI_VERSION) {
+        if (err) *err = "libplmregisterhip.so ABI version mismatch";
+        dlclose (h);
+        return false;
+    }
+    g_plugin.handle = h;
+    return true;
+}
+
+/* ---------------------------------------------------------------------
+   Volume, Bspline_xform, Joint_histogram
+   --------------------------------------------------------------------- */
+
+static const float IDENTITY[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+
+Volume::Volume (const plm_long d[3], const float o[3], const float sp[3], const float* dc,
+    int planes)
+{
+    for (int a = 0; a < 3; a++) {
+        dim[a] = d[a];
+        origin[a] = o[a];
+        spacing[a] = sp[a];
+    }
+    memcpy (direction_cosines, dc ? dc : IDENTITY, sizeof direction_cosines);
+    vox_planes = planes;
+    npix = dim[0] * dim[1] * dim[2];
+    img.assign ((size_t) npix * planes, 0.f);
+}
+
+plm_img_desc
+Volume::desc (int storage) const
+{
+    plm_img_desc d;
+    d.data = img.data ();
+    d.location = PLM_MEM_HOST;
+    d.storage = storage;
+    for (int a = 0; a < 3; a++) {
+        d.dim[a] = dim[a];
+        d.origin[a] = origin[a];
+        d.spacing[a] = spacing[a];
+    }
+    memcpy (d.direction_cosines, direction_cosines, sizeof d.direction_cosines);
+    return d;
+}
+
+/* rdims = 1 + (roi_dim-1)/vpr, cdims = rdims + 3, as Bspline_header::set
+   (src/base/bspline_header.cxx:28-62); coefficients start at zero as in
+   Bspline_xform::allocate (src/base/bspline_xform.cxx:366-368). */
+void
+Bspline_xform::initialize (const float o[3], const float sp[3], const plm_long d[3],
+    const plm_long roff[3], const plm_long rdim[3], const plm_long vpr[3], const float* dcos)
+{
+    for (int a = 0; a < 3; a++) {
+        img_origin[a] = o[a];
+        img_spacing[a] = sp[a];
+        img_dim[a] = d[a];
+        roi_offset[a] = roff[a];
+        roi_dim[a] = rdim[a];
+        vox_per_rgn[a] = vpr[a];
+        grid_spac[a] = vpr[a] * fabsf (sp[a]);
+        rdims[a] = 1 + (rdim[a] - 1) / vpr[a];
+        cdims[a] = 3 + rdims[a];
+    }
+    memcpy (dc, dcos ? dcos : IDENTITY, sizeof dc);
+    num_knots = cdims[0] * cdims[1] * cdims[2];
+    num_coeff = 3 * num_knots;
+    coeff.assign ((size_t) num_coeff, 0.f);
+}
+
+void
+Bspline_xform::fill_coefficients (float val)
+{
+    for (auto& c : coeff) c = val;
+}
+
+/* All-zero coefficients become 0.01 (src/base/bspline_xform.cxx:630-644). */
+void
+Bspline_xform::jitter_if_zero ()
+{
+    for (float c : coeff)
+        if (c != 0.0f) return;
+    fill_coefficients (0.01f);
+}
+
+plm_bsp_geom
+Bspline_xform::geom () const
+{
+    plm_bsp_geom g;
+    for (int a = 0; a < 3; a++) {
+        g.img_origin[a] = img_origin[a];
+        g.img_spacing[a] = img_spacing[a];
+        g.img_dim[a] = img_dim[a];
+        g.roi_offset[a] = roi_offset[a];
+        g.roi_dim[a] = roi_dim[a];
+        g.vox_per_rgn[a] = vox_per_rgn[a];
+    }
+    memcpy (g.direction_cosines, dc, sizeof g.direction_cosines);
+    return g;
+}
+
+Joint_histogram::Joint_histogram (plm_long fb, plm_long mb)
+    : moving (mb), fixed (fb), f_hist (fb, 0.0), m_hist (mb, 0.0), j_hist (fb * mb, 0.0)
+{
+}
+
+/* delta = (max-min)/(bins-1), offset = min - 0.5*delta, float, as
+   bspline_initialize_mi_hist_eqsp (src/register/joint_histogram.cxx:115-139). */
+static void
+hist_eqsp (Histogram* h, const Volume* v)
+{
+    float mn = v->img[0], mx = v->img[0];
+    for (plm_long i = 0; i < v->npix; i++) {
+        if (v->img[i] < mn) mn = v->img[i];
+        else if (v->img[i] > mx) mx = v->img[i];
+    }
+    h->delta = (mx - mn) / (float) (h->bins - 1);
+    h->offset = (float) (mn - 0.5 * h->delta);
+}
+
+void
+Joint_histogram::initialize (const Volume* f, const Volume* m)
+{
+    hist_eqsp (&fixed, f);
+    hist_eqsp (&moving, m);
+}
+
+plm_hist_desc
+Joint_histogram::desc () const
+{
+    plm_hist_desc d;
+    d.fixed_bins = fixed.bins;
+    d.moving_bins = moving.bins;
+    d.fixed_offset = fixed.offset;
+    d.fixed_delta = fixed.delta;
+    d.moving_offset = moving.offset;
+    d.moving_delta = moving.delta;
+    return d;
+}
+
+/* ---------------------------------------------------------------------
+   Parms, score, state
+   --------------------------------------------------------------------- */
+
+Bspline_parms::Bspline_parms ()
+    : threading (BTHR_HIP), implementation ('g'), gpuid (0), storage (PLM_STORE_F32),
+      grad_mode (PLM_GRAD_REF_NN), mi_hist_fixed_bins (32), mi_hist_moving_bins (32),
+      max_its (10), max_feval (10)
+{
+}
+
+Bspline_score::Bspline_score ()
+    : total_score (0), num_coeff (0), curr_smetric (0), curr_num_vox (0), curr_smetric_f64 (0)
+{
+}
+
+void
+Bspline_score::set_num_coeff (plm_long n)
+{
+    num_coeff = n;
+    total_grad.assign ((size_t) n, 0.f);
+    curr_smetric_grad.assign ((size_t) n, 0.f);
+    curr_smetric_grad_f64.assign ((size_t) n, 0.0);
+}
+
+/* src/register/bspline_score.cxx:52-64 */
+void
+Bspline_score::reset_score ()
+{
+    total_score = 0;
+    for (auto& g : total_grad) g = 0.f;
+    metric_record.clear ();
+    curr_num_vox = 0;
+    curr_smetric = 0;
+    for (auto& g : curr_smetric_grad) g = 0.f;
+}
+
+/* total += lambda * curr, then curr is cleared
+   (src/register/bspline_score.cxx:66-77). */
+void
+Bspline_score::accumulate (float lambda)
+{
+    total_score += lambda * curr_smetric;
+    for (plm_long i = 0; i < num_coeff; i++)
+        total_grad[i] += lambda * curr_smetric_grad[i];
+    curr_smetric = 0;
+    curr_num_vox = 0;
+    for (auto& g : curr_smetric_grad) g = 0.f;
+}
+
+Bspline_state::Bspline_state ()
+    : it (0), feval (0), dev_ptrs (nullptr), metric_type (SIMILARITY_METRIC_MSE),
+      metric_lambda (1.0f), fixed (nullptr), moving (nullptr), moving_grad (nullptr),
+      mi_hist (nullptr)
+{
+}
+
+Bspline_state::~Bspline_state ()
+{
+    if (dev_ptrs && g_plugin.plm_hip_bspline_destroy)
+        g_plugin.plm_hip_bspline_destroy ((plm_hip_ctx*) dev_ptrs);
+}
+
+int
+bspline_state_initialize_hip (Bspline_state* bst, Bspline_xform* bxf, Bspline_parms* parms)
+{
+    std::string err;
+    if (!hip_plugin_load (&err)) {
+        bst->last_error = err;
+        return PLM_HIP_ERR_NO_DEVICE;
+    }
+    bst->ssd.set_num_coeff (bxf->num_coeff);
+    if (bst->dev_ptrs) {
+        g_plugin.plm_hip_bspline_destroy ((plm_hip_ctx*) bst->dev_ptrs);
+        bst->dev_ptrs = nullptr;
+    }
+    plm_bsp_geom g = bxf->geom ();
+    plm_img_desc fd = bst->fixed->desc (parms->storage);
+    plm_img_desc md = bst->moving->desc (parms->storage);
+    plm_img_desc gd;
+    if (bst->moving_grad) gd = bst->moving_grad->desc (parms->storage);
+    plm_hist_desc hd;
+    if (bst->mi_hist) hd = bst->mi_hist->desc ();
+    plm_hip_ctx* ctx = nullptr;
+    int rc = g_plugin.plm_hip_bspline_create (&g, &fd, &md, bst->moving_grad ? &gd : nullptr,
+        bst->mi_hist ? &hd : nullptr, parms->gpuid, &ctx);
+    if (rc < 0) {
+        bst->last_error = hip_error ();
+        return rc;
+    }
+    bst->dev_ptrs = ctx;
+    return PLM_HIP_OK;
+}
+
+int
+volume_gradient_hip (const Volume* vol, Volume* out, int gpuid)
+{
+    std::string err;
+    if (!hip_plugin_load (&err)) return PLM_HIP_ERR_NO_DEVICE;
+    plm_img_desc d = vol->desc (PLM_STORE_F32);
+    return g_plugin.plm_hip_volume_gradient (&d, out->img.data (), PLM_MEM_HOST, gpuid);
+}
+
+/* ---------------------------------------------------------------------
+   Score dispatch
+   --------------------------------------------------------------------- */
+
+/* The BTHR_HIP branch the reference's bspline_score_mse / bspline_score_mi
+   dispatchers gain (src/register/bspline_mse.cxx:1877-1963,
+   src/register/bspline_mi.cxx:3035-3141). */
+static int
+bspline_score_hip (Bspline_optimize* bod, int metric)
+{
+    Bspline_parms* parms = bod->get_bspline_parms ();
+    Bspline_state* bst = bod->get_bspline_state ();
+    Bspline_xform* bxf = bod->get_bspline_xform ();
+    Bspline_score* ssd = &bst->ssd;
+    if (!bst->dev_ptrs) {
+        int rc = bspline_state_initialize_hip (bst, bxf, parms);
+        if (rc < 0) return rc;
+    }
+    std::vector<int64_t> hist;
+    if (metric == PLM_METRIC_MI && bst->mi_hist)
+        hist.resize ((size_t) (bst->mi_hist->fixed.bins * bst->mi_hist->moving.bins
+                + bst->mi_hist->fixed.bins));
+    double score = 0.0;
+    int64_t nvox = 0;
+    int rc = g_plugin.plm_hip_bspline_score ((plm_hip_ctx*) bst->dev_ptrs, metric,
+        metric == PLM_METRIC_MSE ? parms->grad_mode : PLM_GRAD_REF_NN, bxf->coeff.data (),
+        &score, ssd->curr_smetric_grad_f64.data (), &nvox, hist.empty () ? nullptr : hist.data ());
+    if (rc < 0) {
+        bst->last_error = hip_error ();
+        return rc;
+    }
+    ssd->curr_smetric_f64 = score;
+    ssd->curr_smetric = (float) score;
+    ssd->curr_num_vox = nvox;
+    for (plm_long i = 0; i < bxf->num_coeff; i++)
+        ssd->curr_smetric_grad[i] = (float) ssd->curr_smetric_grad_f64[i];
+    if (!hist.empty ()) {
+        Joint_histogram* jh = bst->mi_hist;
+        const plm_long fb = jh->fixed.bins, mb = jh->moving.bins;
+        for (auto& v : jh->m_hist) v = 0.0;
+        for (plm_long a = 0; a < fb; a++) {
+            jh->f_hist[a] = (double) hist[fb * mb + a];
+            for (plm_long b = 0; b < mb; b++) {
+                double j = (double) hist[a * mb + b] / 4294967296.0;
+                jh->j_hist[a * mb + b] = j;
+                jh->m_hist[b] += j;
+            }
+        }
+    }
+    return rc;
+}
+
+/* Per evaluation: reset, one metric, record, accumulate, report
+   (src/register/bspline.cxx:558-632). */
+int
+bspline_score (Bspline_optimize* bod)
+{
+    Bspline_parms* parms = bod->get_bspline_parms ();
+    Bspline_state* bst = bod->get_bspline_state ();
+    bst->ssd.reset_score ();
+    if (parms->threading != BTHR_HIP) {
+        bst->last_error = "only threading=hip is built in this library";
+        return PLM_HIP_ERR_UNSUPPORTED;
+    }
+    auto t0 = std::chrono::steady_clock::now ();
+    int rc = bspline_score_hip (bod, bst->metric_type == SIMILARITY_METRIC_MI_MATTES
+        ? PLM_METRIC_MI : PLM_METRIC_MSE);
+    if (rc < 0) return rc;
+    double t = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
+    Metric_score ms;
+    ms.score = bst->ssd.curr_smetric;
+    ms.time = t;
+    ms.num_vox = bst->ssd.curr_num_vox;
+    bst->ssd.metric_record.push_back (ms);
+    bst->ssd.accumulate (bst->metric_lambda);
+    return rc;
+}
+
+/* ---------------------------------------------------------------------
+   Optimizer: steepest descent along -grad/|grad| with a trust region
+   updated from the gain ratio against the linear model (the rules of
+   src/register/bspline_optimize_steepest.cxx): shrink on increase, shrink
+   and accept below 0.25, grow x3 above 0.75.
+   --------------------------------------------------------------------- */
+
+int
+Bspline_optimize::optimize ()
+{
+    Bspline_score* ssd = &bst->ssd;
+    const plm_long n = bxf->num_coeff;
+    std::vector<float> x (bxf->coeff), h (n);
+    float alpha = 1.0f;
+    bst->it = 0;
+    bst->feval = 0;
+    int rc = bspline_score (this);
+    if (rc < 0) return rc;
+    float old_score = ssd->total_score;
+    for (;;) {
+        double gnorm = 0.0;
+        for (plm_long i = 0; i < n; i++) gnorm += (double) ssd->total_grad[i] * ssd->total_grad[i];
+        gnorm = sqrt (gnorm);
+        if (gnorm == 0.0) break;
+        double htg = 0.0;
+        for (plm_long i = 0; i < n; i++) {
+            h[i] = (float) (-ssd->total_grad[i] / gnorm);
+            htg -= h[i] * ssd->total_grad[i];
+        }
+        bool accepted = false;
+        while (!accepted && bst->feval < parms->max_feval) {
+            bst->feval++;
+            for (plm_long i = 0; i < n; i++) bxf->coeff[i] = x[i] + alpha * h[i];
+            rc = bspline_score (this);
+            if (rc < 0) return rc;
+            double gr = (old_score - ssd->total_score) / htg;
+            if (gr < 0) {
+                alpha *= 0.5f;
+            } else {
+                accepted = true;
+                if (gr < 0.25) alpha *= 0.5f;
+                else if (gr > 0.75) alpha *= 3.0f;
+            }
+        }
+        if (!accepted) {
+            bxf->coeff = x;
+            break;
+        }
+        bst->it++;
+        x = bxf->coeff;
+        old_score = ssd->total_score;
+        if (bst->it >= parms->max_its || bst->feval >= parms->max_feval) break;
+    }
+    return PLM_HIP_OK;
+}

@@ -1,0 +1,91 @@
+"""The C++ host side (plastimatch_amd/host): Bspline_state / bspline_score()
+with threading=hip, driven through bspline_hip_check like the reference's
+check_grad tool."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from plastimatch_amd import synth
+from tests.helpers import oracle_case, rel, rel_scalar, small_pair
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "plastimatch_amd", "host")
+TOOL = os.path.join(HOST, "bspline_hip_check")
+
+
+def run_tool(args, tmp_path, pair):
+    f = os.path.join(tmp_path, "fixed.mha")
+    m = os.path.join(tmp_path, "moving.mha")
+    synth.write_mha(f, pair.fixed, pair.spacing, pair.origin)
+    synth.write_mha(m, pair.moving, pair.spacing, pair.origin)
+    r = subprocess.run([TOOL] + args + [f, m], capture_output=True, text=True, timeout=240)
+    return r.returncode, json.loads(r.stdout.strip().splitlines()[-1]), r.stderr
+
+
+def test_host_builds_and_reports_missing_device(tmp_path):
+    subprocess.run(["make", "-s", "-C", HOST], check=True)
+    assert os.path.exists(TOOL)
+    pytest.importorskip("torch")
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present; the GPU tests cover the tool")
+    pair = small_pair(dim=(12, 10, 8), vpr=(4, 4, 4))
+    rc, out, _ = run_tool(["-g", "4"], str(tmp_path), pair)
+    assert rc == 1 and out["status"] < 0 and out["error"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flavor", ["g", "p"])
+def test_host_mse_matches_oracle(tmp_path, flavor):
+    pair = small_pair(dim=(40, 36, 28), spacing=(1.0, 1.0, 2.5), vpr=(5, 5, 5), seed=31)
+    oc = oracle_case(pair)
+    gfile = os.path.join(str(tmp_path), "grad.raw")
+    xfile = os.path.join(str(tmp_path), "coeff.raw")
+    coeff = synth.random_coeff(pair.dim, pair.vox_per_rgn, 1.5, seed=11)
+    coeff.tofile(xfile)
+    rc, out, err = run_tool(["-g", "5", "-f", flavor, "-X", xfile, "-O", gfile], str(tmp_path), pair)
+    assert rc == 0, (out, err)
+    grad = np.fromfile(gfile, np.float64)
+    s, g, n, _ = oc.mse(coeff, flavor)
+    assert out["num_vox"] == n
+    assert rel_scalar(out["score"], s) < 1e-5
+    assert rel(grad, g) < 1e-4
+
+
+@pytest.mark.gpu
+def test_host_mse_zero_coeff_parity(tmp_path):
+    pair = small_pair(dim=(40, 36, 28), spacing=(1.0, 1.0, 2.5), vpr=(5, 5, 5), seed=32)
+    oc = oracle_case(pair)
+    gfile = os.path.join(str(tmp_path), "grad.raw")
+    rc, out, err = run_tool(["-g", "5", "-O", gfile], str(tmp_path), pair)
+    assert rc == 0, (out, err)
+    s, g, n, _ = oc.mse(np.zeros(oc.num_coeff, np.float32), "g")
+    grad = np.fromfile(gfile, np.float64)
+    assert out["num_vox"] == n
+    assert rel_scalar(out["score"], s) < 1e-5
+    assert rel(grad, g) < 1e-4
+
+
+@pytest.mark.gpu
+def test_host_central_difference(tmp_path):
+    pair = small_pair(dim=(40, 36, 28), spacing=(1.0, 1.0, 1.0), vpr=(5, 5, 5), seed=33, ct=False)
+    rc, out, err = run_tool(["-g", "5", "-f", "p", "-p", "ctr", "-e", "1e-2", "-i", "700",
+                             "-R", "-0.5", "0.5"], str(tmp_path), pair)
+    assert rc == 0, (out, err)
+    assert abs(out["analytic"] - out["finite_difference"]) <= 0.05 * abs(out["analytic"]) + 1e-9
+
+
+@pytest.mark.gpu
+def test_host_mi_and_optimizer(tmp_path):
+    pair = small_pair(dim=(40, 36, 28), spacing=(1.0, 1.0, 2.5), vpr=(5, 5, 5), seed=34)
+    rc, out, err = run_tool(["-g", "5", "-M", "mi", "--bins", "32"], str(tmp_path), pair)
+    assert rc == 0, (out, err)
+    oc = oracle_case(pair)
+    o = oc.mi(np.full(oc.num_coeff, 0.01, np.float32), oc.hist(32))
+    assert rel_scalar(out["score"], o["score"]) < 1e-5
+    rc, out, err = run_tool(["-g", "5", "-p", "optimize", "--iterations", "5"], str(tmp_path), pair)
+    assert rc == 0, (out, err)
+    assert out["final_score"] < out["initial_score"]
