@@ -108,6 +108,7 @@ def main():
     d_grad = torch.zeros(N_COEFF_VECTORS, bxf.num_coeff, dtype=torch.float64, device=dev)
     d_nvox = torch.zeros(N_COEFF_VECTORS, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
+    vpt, tuned_ms = st.autotune("mse", "ref_nn", d_coeffs[0], reps=10)
 
     def step(i):
         k = i % N_COEFF_VECTORS
@@ -158,7 +159,7 @@ def main():
         "config": {"workload": WORKLOAD, "dim": list(pair.dim), "spacing": list(pair.spacing),
                    "vox_per_rgn": list(pair.vox_per_rgn), "grad_mode": "ref_nn",
                    "coeff_vectors": N_COEFF_VECTORS, "num_tiles": st.num_tiles,
-                   "num_coeff": bxf.num_coeff},
+                   "num_coeff": bxf.num_coeff, "vox_per_thread": vpt},
         "eval_gb_per_s": bytes_per_launch / (ms_per_step * 1e-3) / 1e9,
         "stage_ms": {"k_tiles": stage_ms[0], "k_reduce": stage_ms[1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -174,12 +174,24 @@ int plm_hip_volume_gradient (
  * (src/register/cuda/bspline_cuda.h:28, bspline_cuda.cu:1371-1436).
  * While enabled, every evaluation records its stage boundaries;
  * get_timing waits for them and returns the summed milliseconds per stage
- * (MSE: tiles, finalize, reduce; MI: zero, histogram, score, tiles,
- * reduce) and the number of evaluations, then starts a new window.  One
+ * (MSE: tiles, reduce; MI: zero, histogram, score, tiles, reduce) and the
+ * number of evaluations, then starts a new window.  One
  * window should hold one metric. */
 int plm_hip_bspline_set_timing (plm_hip_ctx* ctx, int enable);
 int plm_hip_bspline_get_timing (plm_hip_ctx* ctx, double* stage_ms, int max_stages,
     int64_t* evals);
+
+/* Launch-shape tuning of the tile kernel: how many voxels each thread keeps
+ * in flight (1, 2 or 4).  autotune times every variant on this context's
+ * own images (reps evaluations each, after one warm-up, HIP events on the
+ * context stream), keeps the fastest whole evaluation and reports it.
+ * d_coeff is a device coefficient vector or NULL (zeros).  The counterpart
+ * of the reference's launch-configuration search (CUDA_exec_conf_1tpe /
+ * _1bpe, src/cuda/cuda_kernel_util.cu:48-188; divideWork,
+ * src/opencl/autotune_opencl.cxx:7-53). */
+int plm_hip_bspline_set_tuning (plm_hip_ctx* ctx, int vox_per_thread);
+int plm_hip_bspline_autotune (plm_hip_ctx* ctx, int metric, int grad_mode,
+    const float* d_coeff, int reps, int* chosen_vox_per_thread, double* eval_ms);
 
 /* Geometry queries, for the host adapter. */
 int plm_hip_bspline_num_knots (const plm_hip_ctx* ctx, int64_t* num_knots);

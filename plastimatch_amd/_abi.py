@@ -86,6 +86,10 @@ SIGNATURES = {
     "plm_hip_volume_gradient": (ctypes.c_int, [ctypes.POINTER(PlmImgDesc), _P, ctypes.c_int,
                                                ctypes.c_int]),
     "plm_hip_bspline_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "plm_hip_bspline_set_tuning": (ctypes.c_int, [_P, ctypes.c_int]),
+    "plm_hip_bspline_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int,
+                                                ctypes.POINTER(ctypes.c_int),
+                                                ctypes.POINTER(ctypes.c_double)]),
     "plm_hip_bspline_get_timing": (ctypes.c_int, [_P, _P, ctypes.c_int,
                                                   ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_bspline_num_knots": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),

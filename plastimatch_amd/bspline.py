@@ -216,6 +216,18 @@ class BsplineHip:
             self.ctx, METRICS[metric], GRAD_MODES[grad_mode], ptr(d_coeff), ptr(d_score),
             ptr(d_grad), ptr(d_num_vox), ptr(d_hist), st))
 
+    def set_tuning(self, vox_per_thread):
+        _abi.check(self.lib.plm_hip_bspline_set_tuning(self.ctx, int(vox_per_thread)))
+
+    def autotune(self, metric="mse", grad_mode="ref_nn", d_coeff=None, reps=5):
+        """Pick the fastest launch shape on this stage's data; returns (choice, ms/eval)."""
+        ch = ctypes.c_int()
+        ms = ctypes.c_double()
+        ptr = ctypes.c_void_p(d_coeff.data_ptr()) if d_coeff is not None else None
+        _abi.check(self.lib.plm_hip_bspline_autotune(self.ctx, METRICS[metric], GRAD_MODES[grad_mode],
+                                                      ptr, reps, ctypes.byref(ch), ctypes.byref(ms)))
+        return ch.value, ms.value
+
     def set_timing(self, enable=True):
         _abi.check(self.lib.plm_hip_bspline_set_timing(self.ctx, 1 if enable else 0))
 

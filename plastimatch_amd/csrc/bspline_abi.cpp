@@ -90,6 +90,7 @@ struct plm_hip_ctx {
     bool mi_ok = false;
     int storage = PLM_STORE_F32;
     int hist_blocks = 512;
+    plmhip::Tune tune {2};
     int64_t num_knots = 0;
     int64_t num_tiles = 0;
     size_t lds_limit = 0;
@@ -100,7 +101,7 @@ struct plm_hip_ctx {
     float* d_blut = nullptr;
     float* d_cond = nullptr;
     double* d_tile_score = nullptr;
-    unsigned long long* d_ctr = nullptr;    // N accumulator, reduce ticket
+    int* d_tile_count = nullptr;
     unsigned long long* d_hist = nullptr;   // joint bins then fixed bins
     float* d_table = nullptr;
     float* d_nvox_f = nullptr;
@@ -134,7 +135,7 @@ free_ctx (plm_hip_ctx* c)
     destroy_events (c->ev_used);
     destroy_events (c->ev_free);
     void* bufs[] = {c->d_fixed, c->d_moving, c->d_mgrad, c->d_blut, c->d_cond,
-        c->d_tile_score, c->d_ctr, c->d_hist,
+        c->d_tile_score, c->d_tile_count, c->d_hist,
         c->d_table, c->d_nvox_f, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
         c->d_hist_out};
     for (void* b : bufs) if (b) (void) hipFree (b);
@@ -348,19 +349,20 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     // knot-major condense slots; slots no tile owns stay zero for good
     CTX_TRY (alloc (&c->d_cond, (size_t) knots * 192, "hipMalloc(cond)"));
     CTX_TRY (alloc (&c->d_tile_score, tiles, "hipMalloc(tile_score)"));
-    CTX_TRY (alloc (&c->d_ctr, plmhip::NUM_COUNTERS, "hipMalloc(counters)"));
+    CTX_TRY (alloc (&c->d_tile_count, tiles, "hipMalloc(tile_count)"));
     const size_t nbins = (size_t) h.fbins * h.mbins + h.fbins;
     CTX_TRY (alloc (&c->d_hist, nbins, "hipMalloc(hist)"));
     CTX_TRY (alloc (&c->d_table, (size_t) h.fbins * h.mbins, "hipMalloc(table)"));
     CTX_TRY (alloc (&c->d_nvox_f, 1, "hipMalloc(nvox_f)"));
     CTX_TRY (alloc (&c->d_coeff, 3 * knots, "hipMalloc(coeff)"));
+    e = hipMemset (c->d_coeff, 0, sizeof (float) * 3 * knots);
+    if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "hipMemset(coeff)"); }
     CTX_TRY (alloc (&c->d_grad, 3 * knots, "hipMalloc(grad)"));
     CTX_TRY (alloc (&c->d_score, 1, "hipMalloc(score)"));
     CTX_TRY (alloc (&c->d_nvox, 1, "hipMalloc(nvox)"));
     CTX_TRY (alloc (&c->d_hist_out, nbins, "hipMalloc(hist_out)"));
 #undef CTX_TRY
-    e = plmhip::launch_zero (c->d_cond, (int64_t) knots * 192, c->d_ctr, plmhip::NUM_COUNTERS,
-        c->stream);
+    e = plmhip::launch_zero (c->d_cond, (int64_t) knots * 192, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize (c->stream);
     if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "init"); }
     *out = c;
@@ -401,15 +403,15 @@ plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
     }
     hipError_t e;
     if (metric == PLM_METRIC_MSE) {
-        e = plmhip::launch_mse (c->g, grad_mode, c->storage == PLM_STORE_BF16, c->d_fixed,
-            c->d_moving, c->d_mgrad, d_coeff, c->d_blut, c->d_cond, c->d_tile_score,
-            c->d_ctr, d_score, d_num_vox, d_grad, st, ev);
+        e = plmhip::launch_mse (c->g, c->tune, grad_mode, c->storage == PLM_STORE_BF16,
+            c->d_fixed, c->d_moving, c->d_mgrad, d_coeff, c->d_blut, c->d_cond, c->d_tile_score,
+            c->d_tile_count, d_score, d_num_vox, d_grad, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MSE launch");
     } else {
-        e = plmhip::launch_mi (c->g, c->h, c->storage == PLM_STORE_BF16, c->hist_blocks,
+        e = plmhip::launch_mi (c->g, c->tune, c->h, c->storage == PLM_STORE_BF16, c->hist_blocks,
             c->d_fixed, c->d_moving, d_coeff, c->d_blut, c->d_hist,
             c->d_hist + (size_t) c->h.fbins * c->h.mbins, c->d_table, c->d_nvox_f, c->d_cond,
-            c->d_ctr, d_score, d_num_vox, d_grad, d_hist, st, ev);
+            d_score, d_num_vox, d_grad, d_hist, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MI launch");
     }
     if (ev) c->last_stages = metric == PLM_METRIC_MSE ? plmhip::MSE_STAGES : plmhip::MI_STAGES;
@@ -476,6 +478,58 @@ plm_hip_bspline_score (plm_hip_ctx* c, int metric, int grad_mode, const float* c
         for (size_t i = 0; i < nc; i++) grad[i] = 0.0;
         return PLM_HIP_WARN_NO_OVERLAP;
     }
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_set_tuning (plm_hip_ctx* c, int vox_per_thread)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    if (vox_per_thread != 1 && vox_per_thread != 2 && vox_per_thread != 4)
+        return fail (PLM_HIP_ERR_ARG, "vox_per_thread must be 1, 2 or 4");
+    c->tune.vox_per_thread = vox_per_thread;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_autotune (plm_hip_ctx* c, int metric, int grad_mode, const float* d_coeff,
+    int reps, int* chosen, double* eval_ms)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    if (reps < 1) reps = 1;
+    const float* coeff = d_coeff ? d_coeff : c->d_coeff;
+    const bool was_timing = c->timing;
+    double stage[plmhip::MAX_STAGE_EVENTS];
+    int64_t n = 0;
+    int rc = plm_hip_bspline_get_timing (c, stage, plmhip::MAX_STAGE_EVENTS, &n);
+    if (rc) return rc;
+    c->timing = true;
+    const int cands[3] = {1, 2, 4};
+    int best = c->tune.vox_per_thread;
+    double best_ms = 1e300;
+    for (int k = 0; k < 3; k++) {
+        c->tune.vox_per_thread = cands[k];
+        for (int r = 0; r < reps + 1; r++) {
+            rc = plm_hip_bspline_score_device (c, metric, grad_mode, coeff, c->d_score, c->d_grad,
+                c->d_nvox, nullptr, c->stream);
+            if (rc) { c->timing = was_timing; return rc; }
+            if (r == 0) {
+                // the first call of each variant is a warm-up
+                rc = plm_hip_bspline_get_timing (c, stage, plmhip::MAX_STAGE_EVENTS, &n);
+                if (rc) { c->timing = was_timing; return rc; }
+            }
+        }
+        rc = plm_hip_bspline_get_timing (c, stage, plmhip::MAX_STAGE_EVENTS, &n);
+        if (rc) { c->timing = was_timing; return rc; }
+        double total = 0.0;
+        for (int i = 0; i < plmhip::MAX_STAGE_EVENTS; i++) total += stage[i];
+        total /= (double) (n ? n : 1);
+        if (total < best_ms) { best_ms = total; best = cands[k]; }
+    }
+    c->tune.vox_per_thread = best;
+    c->timing = was_timing;
+    if (chosen) *chosen = best;
+    if (eval_ms) *eval_ms = best_ms;
     return PLM_HIP_OK;
 }
 

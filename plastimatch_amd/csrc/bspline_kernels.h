@@ -11,7 +11,6 @@ constexpr int TILE_THREADS = 256;
 constexpr int MAX_BINS_1D = 256;     // per-axis histogram bins
 constexpr int MAX_BINS_2D = 4096;    // joint bins held in LDS (32 KB int64)
 constexpr int MAX_VPR = 32;          // tile edge bound
-constexpr int NUM_COUNTERS = 4;      // [0] overlap count N, [1] reduce ticket
 
 // Everything a tile kernel needs, passed by value.
 struct KGeom {
@@ -46,6 +45,11 @@ struct HGeom {
     float m_offset, m_delta;
 };
 
+// Launch-shape choices made by the autotuner.
+struct Tune {
+    int vox_per_thread;      // voxels each tile-kernel thread keeps in flight: 1, 2 or 4
+};
+
 size_t tile_lds_bytes (const KGeom& g);
 
 // Stage timing: when ev is not null, ev[0..n] are recorded on st between
@@ -55,18 +59,18 @@ constexpr int MSE_STAGES = 2;
 constexpr int MI_STAGES = 5;
 constexpr int MAX_STAGE_EVENTS = 6;
 
-hipError_t launch_mse (const KGeom& g, int mode, int storage, const void* f,
-    const void* m, const void* mg, const float* coeff, const float* blut,
-    float* cond, double* tile_score, unsigned long long* ctr, double* score,
+hipError_t launch_mse (const KGeom& g, const Tune& tune, int mode, int storage,
+    const void* f, const void* m, const void* mg, const float* coeff, const float* blut,
+    float* cond, double* tile_score, int* tile_count, double* score,
     int64_t* nvox, double* grad, hipStream_t st, hipEvent_t* ev);
 
-hipError_t launch_mi (const KGeom& g, const HGeom& h, int storage, int hist_blocks,
-    const void* f, const void* m, const float* coeff, const float* blut,
+hipError_t launch_mi (const KGeom& g, const Tune& tune, const HGeom& h, int storage,
+    int hist_blocks, const void* f, const void* m, const float* coeff, const float* blut,
     unsigned long long* jhist, unsigned long long* fhist, float* table,
-    float* nvox_f, float* cond, unsigned long long* ctr, double* score,
+    float* nvox_f, float* cond, double* score,
     int64_t* nvox, double* grad, int64_t* hist_out, hipStream_t st, hipEvent_t* ev);
 
-hipError_t launch_zero (float* p, int64_t n, unsigned long long* ctr, int nctr, hipStream_t st);
+hipError_t launch_zero (float* p, int64_t n, hipStream_t st);
 hipError_t launch_volume_gradient (const float* img, float* out, const int dim[3],
     const float sp[3], const Mat3& inv, hipStream_t st);
 hipError_t launch_to_bf16 (const float* src, uint16_t* dst, int64_t n, hipStream_t st);

@@ -5,17 +5,18 @@
 //   1. the 64x3 tile coefficients and the per-axis basis LUTs go to LDS;
 //   2. the displacement is contracted separably: T1 = sum_i bx*c, then
 //      T2 = sum_j by*T1, so each voxel only does the last 4-term sum in z;
-//   3. each thread takes G voxels at a time and issues all of their global
-//      loads (fixed value, 8 moving corners, moving gradient) before using
-//      any, so a wavefront keeps G gathers per lane in flight;
+//   3. each thread takes G voxels at a time (G in {1, 2, 4}, chosen by the
+//      autotuner) and issues all of their global loads (fixed value,
+//      8 moving corners, moving gradient) before using any, so a wavefront
+//      keeps G gathers per lane in flight;
 //   4. dc/dv is parked in LDS and condensed into the tile's 64 knot slots by
 //      the transposed separable contraction (x, then y, then z);
 //   5. the 192 slot values go to a knot-major buffer cond[knot][slot][c]
 //      (one writer per entry, no atomics on floats), and a reduce kernel
 //      gives each knot one wavefront: lane = slot, FP64 butterfly sum.
-// The overlap count N is an integer atomic (order independent); the score
-// is a fixed-order FP64 sum of per-tile partials, so results are bitwise
-// reproducible run to run.
+// The overlap count N and the score are fixed-order sums of per-tile
+// partials (integer and FP64); there are no atomics on the MSE path, so the
+// results are bitwise reproducible run to run.
 //
 // The MI histogram pass instead evaluates the displacement in the
 // reference's k-j-i order without FMA so its partial-volume weights are
@@ -34,7 +35,6 @@
 
 namespace plmhip {
 
-constexpr int G = 4;                    // voxels per thread in flight
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -50,12 +50,6 @@ __device__ __forceinline__ int
 fdiv (int n, uint32_t m)
 {
     return (int) __umulhi ((uint32_t) n, m);
-}
-
-__device__ __forceinline__ unsigned long long
-atomic_load_u64 (unsigned long long* p)
-{
-    return __hip_atomic_load (p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -406,12 +400,12 @@ struct MiArgs {
     const float* nvox_f;       // (float) N
 };
 
-template <int KIND, typename TI>
+template <int KIND, typename TI, int G>
 __global__ __launch_bounds__ (TILE_THREADS)
 void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
     const TI* __restrict__ mgrad, const float* __restrict__ coeff,
     const float* __restrict__ blut, float* __restrict__ cond,
-    double* __restrict__ tile_score, unsigned long long* __restrict__ ctr, MiArgs mi)
+    double* __restrict__ tile_score, int* __restrict__ tile_count, MiArgs mi)
 {
     extern __shared__ float sm[];
     const int tile = xcd_tile (g, blockIdx.x);
@@ -548,7 +542,7 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
         block_sum (L, score, count);
         if (threadIdx.x == 0) {
             tile_score[tile] = score;
-            atomicAdd (&ctr[0], (unsigned long long) count);
+            tile_count[tile] = count;
         }
     }
 }
@@ -558,9 +552,12 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
 // bits on every lane, and run to run).  Same quantity as
 // bspline_condense_smetric_grad (src/register/bspline.cxx:400-413), scaled
 // by 2/N for MSE (src/register/bspline_mse.cxx:43-73) or 1 for MI.
-// MSE: block 0 also sums the per-tile scores in tile order and writes
-// score = raw/N (FLT_MAX if N < 1) and N.  The last block to finish clears
-// the counters for the next evaluation.
+// MSE: every workgroup first sums the per-tile overlap counts (integer, so
+// every workgroup gets the same N without any cross-workgroup traffic);
+// workgroup 0 also sums the per-tile scores in tile order and writes
+// score = raw/N (FLT_MAX if N < 1) and N.
+
+constexpr int REDUCE_THREADS = 1024;
 
 __device__ __forceinline__ double
 wave_sum (double v)
@@ -570,48 +567,57 @@ wave_sum (double v)
     return v;
 }
 
+__device__ __forceinline__ long long
+wave_sum_ll (long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor (v, off);
+    return v;
+}
+
 template <bool MSE>
-__global__ __launch_bounds__ (256)
+__global__ __launch_bounds__ (REDUCE_THREADS)
 void k_reduce (KGeom g, const float* __restrict__ cond, const double* __restrict__ tile_score,
-    unsigned long long* __restrict__ ctr, double* __restrict__ grad,
+    const int* __restrict__ tile_count, double* __restrict__ grad,
     double* __restrict__ out_score, int64_t* __restrict__ out_nvox)
 {
-    __shared__ double s_red[256];
-    const long long N = MSE ? (long long) atomic_load_u64 (&ctr[0]) : 1;
+    __shared__ long long s_n[REDUCE_THREADS / 64];
+    __shared__ double s_s[REDUCE_THREADS / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nwaves = REDUCE_THREADS / 64;
+    long long N = 1;
+    if (MSE) {
+        long long n = 0;
+        for (int i = threadIdx.x; i < g.num_tiles; i += REDUCE_THREADS) n += tile_count[i];
+        n = wave_sum_ll (n);
+        if (lane == 0) s_n[wave] = n;
+        __syncthreads ();
+        N = 0;
+        for (int w = 0; w < nwaves; w++) N += s_n[w];
+    }
     const double scale = MSE ? (N >= 1 ? 2.0 / (double) N : 0.0) : 1.0;
-    const int lane = threadIdx.x & 63;
-    const int knot = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (knot < g.num_knots) {
+    for (int knot = blockIdx.x * nwaves + wave; knot < g.num_knots; knot += gridDim.x * nwaves) {
         const float* c = &cond[((size_t) knot * 64 + lane) * 3];
-        double a0 = wave_sum ((double) c[0]);
-        double a1 = wave_sum ((double) c[1]);
-        double a2 = wave_sum ((double) c[2]);
+        const float c0 = c[0], c1 = c[1], c2 = c[2];
+        const double a0 = wave_sum ((double) c0);
+        const double a1 = wave_sum ((double) c1);
+        const double a2 = wave_sum ((double) c2);
         if (lane < 3) {
-            double v = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
+            const double v = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
             grad[3 * knot + lane] = v * scale;
         }
     }
     if (MSE && blockIdx.x == 0) {
         double s = 0.0;
-        for (int i = threadIdx.x; i < g.num_tiles; i += blockDim.x) s += tile_score[i];
-        s_red[threadIdx.x] = s;
+        for (int i = threadIdx.x; i < g.num_tiles; i += REDUCE_THREADS) s += tile_score[i];
+        s = wave_sum (s);
+        if (lane == 0) s_s[wave] = s;
         __syncthreads ();
-        for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-            if ((int) threadIdx.x < h) s_red[threadIdx.x] += s_red[threadIdx.x + h];
-            __syncthreads ();
-        }
         if (threadIdx.x == 0) {
-            *out_score = N >= 1 ? s_red[0] / (double) N : (double) FLT_MAX;
+            double t = 0.0;
+            for (int w = 0; w < nwaves; w++) t += s_s[w];
+            *out_score = N >= 1 ? t / (double) N : (double) FLT_MAX;
             *out_nvox = N;
-        }
-    }
-    __syncthreads ();
-    if (threadIdx.x == 0) {
-        __threadfence ();
-        const unsigned long long t = atomicAdd (&ctr[1], 1ull);
-        if (t == gridDim.x - 1) {
-            atomicExch (&ctr[0], 0ull);
-            atomicExch (&ctr[1], 0ull);
         }
     }
 }
@@ -909,47 +915,70 @@ mark (hipEvent_t* ev, int i, hipStream_t st)
         if (e_ != hipSuccess) return e_;                                \
     } while (0)
 
+template <int KIND, typename TI, int G>
+static void
+launch_tiles_g (const KGeom& g, size_t lds, hipStream_t st, const void* f, const void* m,
+    const void* mg, const float* coeff, const float* blut, float* cond, double* tile_score,
+    int* tile_count, const MiArgs& mi)
+{
+    hipLaunchKernelGGL ((k_tiles<KIND, TI, G>), dim3 (8 * g.tiles_per_xcd), dim3 (TILE_THREADS),
+        lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff, blut, cond, tile_score,
+        tile_count, mi);
+}
+
 template <int KIND, typename TI>
 static void
-launch_tiles (const KGeom& g, size_t lds, hipStream_t st, const void* f, const void* m,
+launch_tiles (const KGeom& g, int vpt, size_t lds, hipStream_t st, const void* f, const void* m,
     const void* mg, const float* coeff, const float* blut, float* cond, double* tile_score,
-    unsigned long long* ctr, const MiArgs& mi)
+    int* tile_count, const MiArgs& mi)
 {
-    hipLaunchKernelGGL ((k_tiles<KIND, TI>), dim3 (8 * g.tiles_per_xcd), dim3 (TILE_THREADS),
-        lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff, blut, cond, tile_score,
-        ctr, mi);
+    if (vpt == 1)
+        launch_tiles_g<KIND, TI, 1> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+    else if (vpt == 2)
+        launch_tiles_g<KIND, TI, 2> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+    else
+        launch_tiles_g<KIND, TI, 4> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+}
+
+static int
+reduce_blocks (const KGeom& g)
+{
+    const int per = REDUCE_THREADS / 64;
+    int b = (g.num_knots + per - 1) / per;
+    return b < 1 ? 1 : b;
 }
 
 hipError_t
-launch_mse (const KGeom& g, int mode, int storage, const void* f, const void* m,
-    const void* mg, const float* coeff, const float* blut, float* cond,
-    double* tile_score, unsigned long long* ctr, double* score, int64_t* nvox,
-    double* grad, hipStream_t st, hipEvent_t* ev)
+launch_mse (const KGeom& g, const Tune& tune, int mode, int storage, const void* f,
+    const void* m, const void* mg, const float* coeff, const float* blut, float* cond,
+    double* tile_score, int* tile_count, double* score, int64_t* nvox, double* grad,
+    hipStream_t st, hipEvent_t* ev)
 {
     const size_t lds = tile_lds_bytes (g);
     MiArgs mi {};
     hipError_t e = mark (ev, 0, st);
     if (e != hipSuccess) return e;
+    const int v = tune.vox_per_thread;
     if (storage == 0) {
-        if (mode == 0) launch_tiles<0, float> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, ctr, mi);
-        else launch_tiles<1, float> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, ctr, mi);
+        if (mode == 0) launch_tiles<0, float> (g, v, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        else launch_tiles<1, float> (g, v, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
     } else {
-        if (mode == 0) launch_tiles<0, uint16_t> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, ctr, mi);
-        else launch_tiles<1, uint16_t> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, ctr, mi);
+        if (mode == 0) launch_tiles<0, uint16_t> (g, v, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        else launch_tiles<1, uint16_t> (g, v, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
     }
     LAUNCH_CHECK ();
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL (k_reduce<true>, dim3 ((g.num_knots + 3) / 4), dim3 (256), 0, st, g,
-        cond, tile_score, ctr, grad, score, nvox);
+    hipLaunchKernelGGL (k_reduce<true>, dim3 (reduce_blocks (g)), dim3 (REDUCE_THREADS), 0, st, g,
+        cond, tile_score, tile_count, grad, score, nvox);
     LAUNCH_CHECK ();
     return mark (ev, 2, st);
 }
 
 hipError_t
-launch_mi (const KGeom& g, const HGeom& h, int storage, int hist_blocks,
+launch_mi (const KGeom& g, const Tune& tune, const HGeom& h, int storage, int hist_blocks,
     const void* f, const void* m, const float* coeff, const float* blut,
     unsigned long long* jhist, unsigned long long* fhist, float* table,
-    float* nvox_f, float* cond, unsigned long long* ctr, double* score, int64_t* nvox,
+    float* nvox_f, float* cond, double* score, int64_t* nvox,
     double* grad, int64_t* hist_out, hipStream_t st, hipEvent_t* ev)
 {
     const int nbins = h.fbins * h.mbins + h.fbins;
@@ -976,23 +1005,23 @@ launch_mi (const KGeom& g, const HGeom& h, int storage, int hist_blocks,
     if ((e = mark (ev, 3, st)) != hipSuccess) return e;
     MiArgs mi {h, table, nvox_f};
     if (storage == 0)
-        launch_tiles<2, float> (g, lds2, st, f, m, nullptr, coeff, blut, cond, nullptr, ctr, mi);
+        launch_tiles<2, float> (g, tune.vox_per_thread, lds2, st, f, m, nullptr, coeff, blut, cond,
+            nullptr, nullptr, mi);
     else
-        launch_tiles<2, uint16_t> (g, lds2, st, f, m, nullptr, coeff, blut, cond, nullptr, ctr, mi);
+        launch_tiles<2, uint16_t> (g, tune.vox_per_thread, lds2, st, f, m, nullptr, coeff, blut,
+            cond, nullptr, nullptr, mi);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 4, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL (k_reduce<false>, dim3 ((g.num_knots + 3) / 4), dim3 (256), 0, st, g,
-        cond, nullptr, ctr, grad, nullptr, nullptr);
+    hipLaunchKernelGGL (k_reduce<false>, dim3 (reduce_blocks (g)), dim3 (REDUCE_THREADS), 0, st, g,
+        cond, nullptr, nullptr, grad, nullptr, nullptr);
     LAUNCH_CHECK ();
     return mark (ev, 5, st);
 }
 
 hipError_t
-launch_zero (float* p, int64_t n, unsigned long long* ctr, int nctr, hipStream_t st)
+launch_zero (float* p, int64_t n, hipStream_t st)
 {
     hipLaunchKernelGGL (k_zero_f32, dim3 (1024), dim3 (256), 0, st, p, n);
-    LAUNCH_CHECK ();
-    hipLaunchKernelGGL (k_zero_u64, dim3 (1), dim3 (64), 0, st, ctr, nctr);
     return hipGetLastError ();
 }
 

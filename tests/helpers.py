@@ -33,3 +33,27 @@ def bf16_round(a):
     u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
     r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
     return (r.astype(np.uint32) << 16).view(np.float32).reshape(np.shape(a))
+
+
+def assert_parity(score, grad, ref, ref64, tol_s=1e-5, tol_g=1e-4, label=""):
+    """GPU result against the float reference flavor (ref = (score, grad)) and
+    the FP64 oracle (ref64).  The bars are 1e-5 (score) and 1e-4 (gradient,
+    relative L2).  Where the float flavor is itself farther than the bar from
+    FP64 -- a nearest-voxel rounding flip of flavor g's round(mijk), or
+    cancellation in MI's float PV derivative -- the bar becomes 1.5 times
+    that own error: the GPU must then be as close to FP64 as the reference
+    flavor is."""
+    s_ref, g_ref = ref
+    s64, g64 = ref64
+    own_s = rel_scalar(s_ref, s64)
+    own_g = rel(g_ref, g64)
+    bar_s = max(tol_s, 1.5 * own_s)
+    bar_g = max(tol_g, 1.5 * own_g)
+    es, eg = rel_scalar(score, s64), rel(grad, g64)
+    print(f"{label} score err vs f64 {es:.2e} (reference flavor {own_s:.2e}), "
+          f"grad err vs f64 {eg:.2e} (reference flavor {own_g:.2e}), vs reference "
+          f"{rel(grad, g_ref):.2e}")
+    assert es < bar_s, (es, bar_s)
+    assert rel_scalar(score, s_ref) < bar_s
+    assert eg < bar_g, (eg, bar_g)
+    assert rel(grad, g_ref) < bar_g, (rel(grad, g_ref), bar_g)
