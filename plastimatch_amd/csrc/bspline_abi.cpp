@@ -105,6 +105,8 @@ struct plm_hip_ctx {
     unsigned long long* d_hist = nullptr;   // joint bins then fixed bins
     float* d_table = nullptr;
     float* d_nvox_f = nullptr;
+    uint8_t* d_fbin = nullptr;              // MI bin images (once per stage)
+    uint8_t* d_mbin = nullptr;
     // staging for the host-pointer entry point
     float* d_coeff = nullptr;
     double* d_grad = nullptr;
@@ -136,7 +138,7 @@ free_ctx (plm_hip_ctx* c)
     destroy_events (c->ev_free);
     void* bufs[] = {c->d_fixed, c->d_moving, c->d_mgrad, c->d_blut, c->d_cond,
         c->d_tile_score, c->d_tile_count, c->d_hist,
-        c->d_table, c->d_nvox_f, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
+        c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
         c->d_hist_out};
     for (void* b : bufs) if (b) (void) hipFree (b);
     if (c->stream) (void) hipStreamDestroy (c->stream);
@@ -281,6 +283,10 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     g.vstep[0] = plmhip::TILE_THREADS % g.vpr[0];
     g.vstep[1] = (plmhip::TILE_THREADS / g.vpr[0]) % g.vpr[1];
     g.vstep[2] = plmhip::TILE_THREADS / (g.vpr[0] * g.vpr[1]);
+    g.aligned = 1;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (i != j && (g.fstep[3*i+j] != 0.f || g.mproj[3*i+j] != 0.f)) g.aligned = 0;
 
     HGeom h {};
     bool mi_ok = false;
@@ -354,6 +360,17 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     CTX_TRY (alloc (&c->d_hist, nbins, "hipMalloc(hist)"));
     CTX_TRY (alloc (&c->d_table, (size_t) h.fbins * h.mbins, "hipMalloc(table)"));
     CTX_TRY (alloc (&c->d_nvox_f, 1, "hipMalloc(nvox_f)"));
+    if (c->have_hist) {
+        // bin images: the bin of every voxel is fixed for the stage
+        CTX_TRY (alloc (&c->d_fbin, nf, "hipMalloc(fixed bins)"));
+        CTX_TRY (alloc (&c->d_mbin, nm, "hipMalloc(moving bins)"));
+        e = plmhip::launch_bin_image (c->d_fixed, c->storage == PLM_STORE_BF16, c->d_fbin, nf,
+            h.f_offset, h.f_delta, h.fbins, c->stream);
+        if (e == hipSuccess)
+            e = plmhip::launch_bin_image (c->d_moving, c->storage == PLM_STORE_BF16, c->d_mbin, nm,
+                h.m_offset, h.m_delta, h.mbins, c->stream);
+        if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "bin images"); }
+    }
     CTX_TRY (alloc (&c->d_coeff, 3 * knots, "hipMalloc(coeff)"));
     e = hipMemset (c->d_coeff, 0, sizeof (float) * 3 * knots);
     if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "hipMemset(coeff)"); }
@@ -409,7 +426,7 @@ plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
         if (e != hipSuccess) return hip_fail (e, "MSE launch");
     } else {
         e = plmhip::launch_mi (c->g, c->tune, c->h, c->storage == PLM_STORE_BF16, c->hist_blocks,
-            c->d_fixed, c->d_moving, d_coeff, c->d_blut, c->d_hist,
+            c->d_fixed, c->d_moving, c->d_fbin, c->d_mbin, d_coeff, c->d_blut, c->d_hist,
             c->d_hist + (size_t) c->h.fbins * c->h.mbins, c->d_table, c->d_nvox_f, c->d_cond,
             d_score, d_num_vox, d_grad, d_hist, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MI launch");

@@ -32,6 +32,7 @@ struct KGeom {
     uint32_t mag_vx;         // ceil(2^32 / vpr0), for n / vpr0 with __umulhi
     uint32_t mag_vxy;        // ceil(2^32 / (vpr0*vpr1))
     int vstep[3];            // TILE_THREADS in mixed radix (vpr0, vpr1, inf)
+    int aligned;             // fixed step and moving proj are diagonal
 };
 
 struct Mat3 {
@@ -65,10 +66,13 @@ hipError_t launch_mse (const KGeom& g, const Tune& tune, int mode, int storage,
     int64_t* nvox, double* grad, hipStream_t st, hipEvent_t* ev);
 
 hipError_t launch_mi (const KGeom& g, const Tune& tune, const HGeom& h, int storage,
-    int hist_blocks, const void* f, const void* m, const float* coeff, const float* blut,
-    unsigned long long* jhist, unsigned long long* fhist, float* table,
-    float* nvox_f, float* cond, double* score,
+    int hist_blocks, const void* f, const void* m, const uint8_t* fbin, const uint8_t* mbin,
+    const float* coeff, const float* blut, unsigned long long* jhist,
+    unsigned long long* fhist, float* table, float* nvox_f, float* cond, double* score,
     int64_t* nvox, double* grad, int64_t* hist_out, hipStream_t st, hipEvent_t* ev);
+
+hipError_t launch_bin_image (const void* img, int storage, uint8_t* out, int64_t n,
+    float offset, float delta, int bins, hipStream_t st);
 
 hipError_t launch_zero (float* p, int64_t n, hipStream_t st);
 hipError_t launch_volume_gradient (const float* img, float* out, const int dim[3],

@@ -66,7 +66,13 @@ struct Sample {
 // Follows bspline_find_correspondence_dcos
 // (src/util/bspline_correspond.cxx:80-109), Volume::is_inside
 // (src/base/volume.cxx:899-906) and li_clamp (src/base/interpolate.cxx:17-41).
+// ALIGNED (direction cosines diagonal): the off-diagonal terms of proj are
+// exact zeros, t*0 adds +-0 to a float, so dropping them leaves every bit
+// unchanged.  The clamp is branch free: f = clamp(floor(ma), 0, dmax-1),
+// frac = clamp(ma - f, 0, 1) and round = f + (frac >= 0.5) give exactly the
+// three branches of li_clamp (and trunc(ma + 0.5) of ROUND_PLM_LONG).
 // Outside voxels get in-range indices so their loads can be issued blindly.
+template <bool ALIGNED>
 __device__ __forceinline__ bool
 correspond (const KGeom& g, const float fxyz[3], const float d[3], Sample& s)
 {
@@ -75,8 +81,12 @@ correspond (const KGeom& g, const float fxyz[3], const float d[3], Sample& s)
     t[1] = (fxyz[1] + d[1]) - g.morigin[1];
     t[2] = (fxyz[2] + d[2]) - g.morigin[2];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
-        mijk[a] = t[0] * g.mproj[3*a+0] + t[1] * g.mproj[3*a+1] + t[2] * g.mproj[3*a+2];
+    for (int a = 0; a < 3; a++) {
+        if (ALIGNED)
+            mijk[a] = t[a] * g.mproj[4*a];
+        else
+            mijk[a] = t[0] * g.mproj[3*a+0] + t[1] * g.mproj[3*a+1] + t[2] * g.mproj[3*a+2];
+    }
     bool inside = true;
 #pragma unroll
     for (int a = 0; a < 3; a++)
@@ -84,19 +94,11 @@ correspond (const KGeom& g, const float fxyz[3], const float d[3], Sample& s)
     int f[3], r[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        float ma = inside ? mijk[a] : 0.f;
-        int dmax = g.mdim[a] - 1;
-        float fa2;
-        if (ma < 0.f) {
-            f[a] = 0; r[a] = 0; fa2 = 0.f;
-        } else if (ma >= (float) dmax) {
-            f[a] = dmax - 1; r[a] = dmax; fa2 = 1.f;
-        } else {
-            float fl = floorf (ma);
-            f[a] = (int) fl;
-            fa2 = ma - fl;                              // exact
-            r[a] = f[a] + (fa2 >= 0.5f ? 1 : 0);        // == trunc(ma + 0.5) in double
-        }
+        const float ma = inside ? mijk[a] : 0.f;
+        const float fl = fminf (fmaxf (floorf (ma), 0.f), (float) (g.mdim[a] - 2));
+        const float fa2 = fminf (fmaxf (ma - fl, 0.f), 1.f);
+        f[a] = (int) fl;
+        r[a] = f[a] + (fa2 >= 0.5f ? 1 : 0);
         s.l2[a] = fa2;
         s.l1[a] = 1.0f - fa2;
     }
@@ -121,55 +123,72 @@ corners (float a[8], const TI* __restrict__ m, const KGeom& g, int mvf)
     a[7] = ld (m, mvf + sxy + sx + 1);
 }
 
-// Trilinear value, products and sum in the reference's order.
-// Follows li_value (src/base/interpolate.cxx:135-160).
-__device__ __forceinline__ float
-li_value (const Sample& s, const float a[8])
-{
-    const float* f1 = s.l1;
-    const float* f2 = s.l2;
-    float v0 = f1[0] * f1[1] * f1[2] * a[0];
-    float v1 = f2[0] * f1[1] * f1[2] * a[1];
-    float v2 = f1[0] * f2[1] * f1[2] * a[2];
-    float v3 = f2[0] * f2[1] * f1[2] * a[3];
-    float v4 = f1[0] * f1[1] * f2[2] * a[4];
-    float v5 = f2[0] * f1[1] * f2[2] * a[5];
-    float v6 = f1[0] * f2[1] * f2[2] * a[6];
-    float v7 = f2[0] * f2[1] * f2[2] * a[7];
-    return v0 + v1 + v2 + v3 + v4 + v5 + v6 + v7;
-}
-
-// Analytic derivative of the trilinear value times 1/spacing.
-// Follows li_value_dx/dy/dz (src/base/interpolate.cxx:162-244).
 __device__ __forceinline__ void
-li_value_d (float out[3], const Sample& s, const float a[8], const float inv[3])
+corner_bins (int b[8], const uint8_t* __restrict__ m, const KGeom& g, int mvf)
 {
-    const float* f1 = s.l1;
-    const float* f2 = s.l2;
-    const float rx = inv[0], ry = inv[1], rz = inv[2];
-    out[0] = -rx * f1[1] * f1[2] * a[0] + rx * f1[1] * f1[2] * a[1]
-        + -rx * f2[1] * f1[2] * a[2] + rx * f2[1] * f1[2] * a[3]
-        + -rx * f1[1] * f2[2] * a[4] + rx * f1[1] * f2[2] * a[5]
-        + -rx * f2[1] * f2[2] * a[6] + rx * f2[1] * f2[2] * a[7];
-    out[1] = -f1[0] * ry * f1[2] * a[0] + -f2[0] * ry * f1[2] * a[1]
-        + f1[0] * ry * f1[2] * a[2] + f2[0] * ry * f1[2] * a[3]
-        + -f1[0] * ry * f2[2] * a[4] + -f2[0] * ry * f2[2] * a[5]
-        + f1[0] * ry * f2[2] * a[6] + f2[0] * ry * f2[2] * a[7];
-    out[2] = -f1[0] * f1[1] * rz * a[0] + -f2[0] * f1[1] * rz * a[1]
-        + -f1[0] * f2[1] * rz * a[2] + -f2[0] * f2[1] * rz * a[3]
-        + f1[0] * f1[1] * rz * a[4] + f2[0] * f1[1] * rz * a[5]
-        + f1[0] * f2[1] * rz * a[6] + f2[0] * f2[1] * rz * a[7];
+    const int sx = g.mdim[0];
+    const int sxy = g.mdim[0] * g.mdim[1];
+    b[0] = m[mvf];
+    b[1] = m[mvf + 1];
+    b[2] = m[mvf + sx];
+    b[3] = m[mvf + sx + 1];
+    b[4] = m[mvf + sxy];
+    b[5] = m[mvf + sxy + 1];
+    b[6] = m[mvf + sxy + sx];
+    b[7] = m[mvf + sxy + sx + 1];
 }
 
-// Fixed-image position, origin + step*ijk in the reference's order.
-// Follows POSITION_FROM_COORDS (src/base/volume_macros.h:82-96).
+// Linear interpolation t*b + (1-t)*a as two FMAs; exact at t = 0 and
+// t = 1, which the clamped border samples of li_clamp rely on.
+__device__ __forceinline__ float
+lerp (float a, float b, float t)
+{
+    return fmaf (t, b, fmaf (-t, a, a));
+}
+
+// Trilinear value and, optionally, its analytic derivative times
+// 1/spacing (li_value and li_value_dx/dy/dz, src/base/interpolate.cxx:135-244)
+// as nested lerps: the same polynomial in 14 FMAs (+ 17 operations for the
+// derivative), rounding within a few ulp of the reference's
+// product-and-sum form and exact on voxel centres and clamped borders.
+template <bool DERIV>
+__device__ __forceinline__ float
+trilinear (const Sample& s, const float a[8], const float inv[3], float dm[3])
+{
+    const float fx = s.l2[0], fy = s.l2[1], fz = s.l2[2];
+    const float x00 = lerp (a[0], a[1], fx);
+    const float x10 = lerp (a[2], a[3], fx);
+    const float x01 = lerp (a[4], a[5], fx);
+    const float x11 = lerp (a[6], a[7], fx);
+    const float y0 = lerp (x00, x10, fy);
+    const float y1 = lerp (x01, x11, fy);
+    if (DERIV) {
+        const float gx0 = lerp (a[1] - a[0], a[3] - a[2], fy);
+        const float gx1 = lerp (a[5] - a[4], a[7] - a[6], fy);
+        dm[0] = inv[0] * lerp (gx0, gx1, fz);
+        dm[1] = inv[1] * lerp (x10 - x00, x11 - x01, fz);
+        dm[2] = inv[2] * (y1 - y0);
+    }
+    return lerp (y0, y1, fz);
+}
+
+// Fixed-image position, origin + step*ijk in the reference's order
+// (POSITION_FROM_COORDS, src/base/volume_macros.h:82-96); ALIGNED drops the
+// exact-zero off-diagonal products, bit for bit the same.
+template <bool ALIGNED>
 __device__ __forceinline__ void
 position (float fxyz[3], const KGeom& g, int fi, int fj, int fk)
 {
     const float x = (float) fi, y = (float) fj, z = (float) fk;
+    if (ALIGNED) {
+        fxyz[0] = g.img_origin[0] + x * g.fstep[0];
+        fxyz[1] = g.img_origin[1] + y * g.fstep[4];
+        fxyz[2] = g.img_origin[2] + z * g.fstep[8];
+    } else {
 #pragma unroll
-    for (int a = 0; a < 3; a++)
-        fxyz[a] = g.img_origin[a] + x * g.fstep[3*a+0] + y * g.fstep[3*a+1] + z * g.fstep[3*a+2];
+        for (int a = 0; a < 3; a++)
+            fxyz[a] = g.img_origin[a] + x * g.fstep[3*a+0] + y * g.fstep[3*a+1] + z * g.fstep[3*a+2];
+    }
 }
 
 __device__ __forceinline__ int
@@ -297,9 +316,10 @@ voxel_displacement (float d[3], const TileLds& L, int qx, int qy, int qz, int vx
 }
 
 // Condense the parked dc/dv of the tile into its 64 knot slots and write
-// them knot-major: cond[(knot*64 + slot)*3 + c], slot = (k*4 + j)*4 + i,
-// the layout of bspline_sort_sets (src/register/bspline.cxx:377-398).
-// S1 over x, S2 over y, S3 over z.
+// them slot-major: cond[slot][knot][c], slot = (k*4 + j)*4 + i.  Each
+// (knot, slot) entry has exactly one writer, as in bspline_sort_sets
+// (src/register/bspline.cxx:377-398); slot-major keeps the reduce's reads
+// coalesced.  S1 over x, S2 over y, S3 over z.
 __device__ __forceinline__ void
 condense_tile (const TileLds& L, const KGeom& g, const int p[3], float* __restrict__ cond)
 {
@@ -351,11 +371,12 @@ condense_tile (const TileLds& L, const KGeom& g, const int p[3], float* __restri
         }
         const int cd0 = g.cdims[0], cd01 = g.cdims[0] * g.cdims[1];
         const int knot0 = (p[0] + i) + cd0 * (p[1] + j) + cd01 * p[2];
+        const size_t plane = (size_t) 3 * g.num_knots;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int knot = knot0 + k * cd01;
             const int slot = k * 16 + ji;
-            cond[((size_t) knot * 64 + slot) * 3 + c] = a[k];
+            cond[slot * plane + 3 * knot + c] = a[k];
         }
     }
 }
@@ -398,9 +419,11 @@ struct MiArgs {
     HGeom h;
     const float* table;        // dS/dP per joint bin (0 where j <= 1e-4)
     const float* nvox_f;       // (float) N
+    const uint8_t* fbin;       // per-voxel fixed bin (bin image of the stage)
+    const uint8_t* mbin;       // per-voxel moving bin
 };
 
-template <int KIND, typename TI, int G>
+template <int KIND, typename TI, int G, bool ALIGNED>
 __global__ __launch_bounds__ (TILE_THREADS)
 void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
     const TI* __restrict__ mgrad, const float* __restrict__ coeff,
@@ -444,6 +467,7 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
         bool act[G], in[G];
         int fidx[G];
         float fval[G];
+        int fbv[G];
 #pragma unroll
         for (int u = 0; u < G; u++) {
             vv[u] = base + u * TILE_THREADS + threadIdx.x;
@@ -454,22 +478,25 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
             const int fi = bx0 + qx[u], fj = by0 + qy[u], fk = bz0 + qz[u];
             act[u] = vv[u] < nvox && fi < G_.roi_end[0] && fj < G_.roi_end[1] && fk < G_.roi_end[2];
             fidx[u] = act[u] ? fi + G_.fdim[0] * (fj + G_.fdim[1] * fk) : 0;
-            fval[u] = ld (fimg, fidx[u]);
+            if (KIND < 2) fval[u] = ld (fimg, fidx[u]);
+            else fbv[u] = mi.fbin[fidx[u]];
         }
         Sample s[G];
 #pragma unroll
         for (int u = 0; u < G; u++) {
             float fxyz[3], d[3];
-            position (fxyz, G_, bx0 + qx[u], by0 + qy[u], bz0 + qz[u]);
+            position<ALIGNED> (fxyz, G_, bx0 + qx[u], by0 + qy[u], bz0 + qz[u]);
             int qyc = qy[u] < vy ? qy[u] : 0, qzc = qz[u] < vz ? qz[u] : 0;
             voxel_displacement (d, L, qx[u], qyc, qzc, vx);
-            in[u] = correspond (G_, fxyz, d, s[u]) && act[u];
+            in[u] = correspond<ALIGNED> (G_, fxyz, d, s[u]) && act[u];
         }
         float a[G][8];
         float mg[G][3];
+        int bn[G][8];
 #pragma unroll
         for (int u = 0; u < G; u++) {
-            corners (a[u], mimg, G_, s[u].mvf);
+            if (KIND < 2) corners (a[u], mimg, G_, s[u].mvf);
+            else corner_bins (bn[u], mi.mbin, G_, s[u].mvf);
             if (KIND == 0) {
                 mg[u][0] = ld (mgrad, 3 * s[u].mvr + 0);
                 mg[u][1] = ld (mgrad, 3 * s[u].mvr + 1);
@@ -480,8 +507,9 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
         for (int u = 0; u < G; u++) {
             float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f;
             if (KIND < 2) {
-                float m_val = li_value (s[u], a[u]);
-                float diff = m_val - fval[u];
+                float md[3];
+                const float m_val = trilinear<KIND == 1> (s[u], a[u], G_.inv_msp, md);
+                const float diff = m_val - fval[u];
                 if (in[u]) {
                     score += (double) (diff * diff);
                     count++;
@@ -490,8 +518,6 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
                         dc1 = diff * mg[u][1];
                         dc2 = diff * mg[u][2];
                     } else {
-                        float md[3];
-                        li_value_d (md, s[u], a[u], G_.inv_msp);
                         dc0 = diff * md[0];
                         dc1 = diff * md[1];
                         dc2 = diff * md[2];
@@ -509,12 +535,10 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
                 dw[15] = l1[1] * l2[2];  dw[16] = -l2[0] * l2[2]; dw[17] = l2[0] * l1[1];
                 dw[18] = -l2[1] * l2[2]; dw[19] = l1[0] * l2[2];  dw[20] = l1[0] * l2[1];
                 dw[21] = l2[1] * l2[2];  dw[22] = l2[0] * l2[2];  dw[23] = l2[0] * l2[1];
-                const int fb = hist_bin (fval[u], mi.h.f_offset, mi.h.f_delta, mi.h.fbins);
-                const float* row = &L.tab[fb * mi.h.mbins];
+                const float* row = &L.tab[fbv[u] * mi.h.mbins];
 #pragma unroll
                 for (int n = 0; n < 8; n++) {
-                    const int mb = hist_bin (a[u][n], mi.h.m_offset, mi.h.m_delta, mi.h.mbins);
-                    const float ds = row[mb];
+                    const float ds = row[bn[u][n]];
                     dc0 -= dw[3 * n + 0] * ds;
                     dc1 -= dw[3 * n + 1] * ds;
                     dc2 -= dw[3 * n + 2] * ds;
@@ -548,16 +572,16 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
 }
 
 // ---------------------------------------------------------------------------
-// Reduce: one wavefront per knot, lane = slot, FP64 butterfly (the same
-// bits on every lane, and run to run).  Same quantity as
-// bspline_condense_smetric_grad (src/register/bspline.cxx:400-413), scaled
-// by 2/N for MSE (src/register/bspline_mse.cxx:43-73) or 1 for MI.
-// MSE: every workgroup first sums the per-tile overlap counts (integer, so
-// every workgroup gets the same N without any cross-workgroup traffic);
-// workgroup 0 also sums the per-tile scores in tile order and writes
-// score = raw/N (FLT_MAX if N < 1) and N.
+// Reduce: one thread per gradient component, summing its 64 slots in slot
+// order in FP64 -- the order of bspline_condense_smetric_grad
+// (src/register/bspline.cxx:400-413), in double -- then scaled by 2/N for
+// MSE (src/register/bspline_mse.cxx:43-73) or 1 for MI.  For MSE every
+// workgroup also sums the per-tile overlap counts (integers, so every
+// workgroup gets the same N with no cross-workgroup traffic), interleaved
+// with the slot loads; workgroup 0 also sums the per-tile scores in tile
+// order and writes score = raw/N (FLT_MAX if N < 1) and N.
 
-constexpr int REDUCE_THREADS = 1024;
+constexpr int REDUCE_THREADS = 256;
 
 __device__ __forceinline__ double
 wave_sum (double v)
@@ -585,41 +609,42 @@ void k_reduce (KGeom g, const float* __restrict__ cond, const double* __restrict
     __shared__ double s_s[REDUCE_THREADS / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nwaves = REDUCE_THREADS / 64;
-    long long N = 1;
-    if (MSE) {
-        long long n = 0;
-        for (int i = threadIdx.x; i < g.num_tiles; i += REDUCE_THREADS) n += tile_count[i];
-        n = wave_sum_ll (n);
-        if (lane == 0) s_n[wave] = n;
-        __syncthreads ();
-        N = 0;
-        for (int w = 0; w < nwaves; w++) N += s_n[w];
-    }
-    const double scale = MSE ? (N >= 1 ? 2.0 / (double) N : 0.0) : 1.0;
-    for (int knot = blockIdx.x * nwaves + wave; knot < g.num_knots; knot += gridDim.x * nwaves) {
-        const float* c = &cond[((size_t) knot * 64 + lane) * 3];
-        const float c0 = c[0], c1 = c[1], c2 = c[2];
-        const double a0 = wave_sum ((double) c0);
-        const double a1 = wave_sum ((double) c1);
-        const double a2 = wave_sum ((double) c2);
-        if (lane < 3) {
-            const double v = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
-            grad[3 * knot + lane] = v * scale;
+    const int nc = 3 * g.num_knots;
+    const int i = blockIdx.x * REDUCE_THREADS + threadIdx.x;
+    const bool own = i < nc;
+    const float* c = cond + (own ? i : 0);
+    double acc = 0.0;
+    long long n = 0;
+    double sc = 0.0;
+    const int ntiles = MSE ? g.num_tiles : 0;
+    const int iters = max (64, (ntiles + REDUCE_THREADS - 1) / REDUCE_THREADS);
+#pragma unroll 8
+    for (int m = 0; m < iters; m++) {
+        if (m < 64) acc += (double) c[(size_t) m * nc];
+        if (MSE) {
+            const int t = m * REDUCE_THREADS + threadIdx.x;
+            if (t < ntiles) {
+                n += tile_count[t];
+                if (blockIdx.x == 0) sc += tile_score[t];
+            }
         }
     }
-    if (MSE && blockIdx.x == 0) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < g.num_tiles; i += REDUCE_THREADS) s += tile_score[i];
-        s = wave_sum (s);
-        if (lane == 0) s_s[wave] = s;
+    double scale = 1.0;
+    if (MSE) {
+        n = wave_sum_ll (n);
+        sc = wave_sum (sc);
+        if (lane == 0) { s_n[wave] = n; s_s[wave] = sc; }
         __syncthreads ();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int w = 0; w < nwaves; w++) t += s_s[w];
-            *out_score = N >= 1 ? t / (double) N : (double) FLT_MAX;
+        long long N = 0;
+        double S = 0.0;
+        for (int w = 0; w < nwaves; w++) { N += s_n[w]; S += s_s[w]; }
+        scale = N >= 1 ? 2.0 / (double) N : 0.0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            *out_score = N >= 1 ? S / (double) N : (double) FLT_MAX;
             *out_nvox = N;
         }
     }
+    if (own) grad[i] = acc * scale;
 }
 
 // ---------------------------------------------------------------------------
@@ -676,11 +701,46 @@ pv_quantize (const float w[8], long long q[8])
 // bspline_score_g_mi (src/register/bspline_mi.cxx:1567-1615) with
 // Joint_histogram::add_pvi_8 (src/register/joint_histogram.cxx:423-487).
 // Persistent: each workgroup walks tiles with a grid stride and keeps a
-// private int64 histogram in LDS, flushed once with integer atomics.
-template <typename TI>
+// private int64 histogram in LDS, flushed once with integer atomics.  Bins
+// come from the stage's bin images.  Because integer addition is
+// associative, contributions to one bin may be merged before the atomic:
+// equal corner bins of a voxel are merged in registers, and lanes of a
+// wavefront that share a joint (or fixed) bin are summed with a wave
+// reduction and added by one lane -- uniform image regions, where every
+// lane hits the same bin, then cost one LDS atomic per wavefront instead
+// of 64 serialized ones.
+
+__device__ __forceinline__ unsigned long long
+wave_sum_u64 (unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor (v, off);
+    return v;
+}
+
+// Adds val at s_hist[key] for every lane with act set, merging lanes that
+// share a key (up to 4 distinct keys per wavefront; the rest go directly).
+__device__ __forceinline__ void
+wave_hist_add (unsigned long long* s_hist, bool act, int key, unsigned long long val)
+{
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot (act);
+    for (int it = 0; it < 4 && pending; it++) {
+        const int leader = __ffsll ((long long) pending) - 1;
+        const int lkey = __shfl (key, leader);
+        const bool match = act && key == lkey;
+        const unsigned long long tot = wave_sum_u64 (match ? val : 0ull);
+        if (lane == leader) atomicAdd (&s_hist[lkey], tot);
+        pending &= ~__ballot (match);
+        act = act && !match;
+    }
+    if (act) atomicAdd (&s_hist[key], val);
+}
+
+template <bool ALIGNED>
 __global__ __launch_bounds__ (TILE_THREADS)
-void k_mi_hist (KGeom g, HGeom h, const TI* __restrict__ fimg,
-    const TI* __restrict__ mimg, const float* __restrict__ coeff,
+void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
+    const uint8_t* __restrict__ mbin, const float* __restrict__ coeff,
     const float* __restrict__ blut, unsigned long long* __restrict__ jhist,
     unsigned long long* __restrict__ fhist)
 {
@@ -701,6 +761,7 @@ void k_mi_hist (KGeom g, HGeom h, const TI* __restrict__ fimg,
     for (int i = threadIdx.x; i < nl; i += blockDim.x) s_bx[i] = blut[i];
 
     const int nvox = vx * vy * vz;
+    const int rounds = (nvox + TILE_THREADS - 1) / TILE_THREADS;
     for (int tile = blockIdx.x; tile < G_.num_tiles; tile += gridDim.x) {
         int p[3];
         tile_coords (G_, tile, p);
@@ -712,20 +773,27 @@ void k_mi_hist (KGeom g, HGeom h, const TI* __restrict__ fimg,
             s_coef[e] = coeff[3 * knot + c];
         }
         __syncthreads ();
-        for (int v = threadIdx.x; v < nvox; v += blockDim.x) {
-            int qz = fdiv (v, G_.mag_vxy);
-            int r = v - qz * vx * vy;
-            int qy = fdiv (r, G_.mag_vx);
-            int qx = r - qy * vx;
-            int fi = vx * p[0] + qx, fj = vy * p[1] + qy, fk = vz * p[2] + qz;
-            if (fi >= G_.fdim[0] || fj >= G_.fdim[1] || fk >= G_.fdim[2]) continue;
-            float fxyz[3], d[3];
-            position (fxyz, G_, fi, fj, fk);
-            exact_displacement (d, &s_bx[4 * qx], &s_by[4 * qy], &s_bz[4 * qz], s_coef);
+        // every lane runs the same number of rounds (the wave-level merges
+        // need the whole wavefront), inactive lanes contribute nothing
+        for (int r = 0; r < rounds; r++) {
+            const int v = r * TILE_THREADS + threadIdx.x;
+            const int qz = fdiv (v, G_.mag_vxy);
+            const int rr = v - qz * vx * vy;
+            const int qy = fdiv (rr, G_.mag_vx);
+            const int qx = rr - qy * vx;
+            const int fi = vx * p[0] + qx, fj = vy * p[1] + qy, fk = vz * p[2] + qz;
+            bool act = v < nvox && fi < G_.fdim[0] && fj < G_.fdim[1] && fk < G_.fdim[2];
             Sample s;
-            if (!correspond (G_, fxyz, d, s)) continue;
-            float a[8];
-            corners (a, mimg, G_, s.mvf);
+            float d[3] = {0.f, 0.f, 0.f};
+            float fxyz[3];
+            position<ALIGNED> (fxyz, G_, fi, fj, fk);
+            if (act)
+                exact_displacement (d, &s_bx[4 * qx], &s_by[4 * qy], &s_bz[4 * qz], s_coef);
+            act = correspond<ALIGNED> (G_, fxyz, d, s) && act;
+            int b[8];
+            corner_bins (b, mbin, G_, s.mvf);
+            const int fidx = act ? fi + G_.fdim[0] * (fj + G_.fdim[1] * fk) : 0;
+            const int fb = fbin[fidx];
             const float* l1 = s.l1;
             const float* l2 = s.l2;
             float w[8];
@@ -739,15 +807,16 @@ void k_mi_hist (KGeom g, HGeom h, const TI* __restrict__ fimg,
             w[7] = l2[0] * l2[1] * l2[2];
             long long q[8];
             pv_quantize (w, q);
-            int fidx = fi + G_.fdim[0] * (fj + G_.fdim[1] * fk);
-            int fb = hist_bin (ld (fimg, fidx), h.f_offset, h.f_delta, h.fbins);
-            atomicAdd (&s_f[fb], 1ull);
-            unsigned long long* row = &s_j[fb * h.mbins];
+            // merge corners that fall in the bin of corner 0
 #pragma unroll
-            for (int n = 0; n < 8; n++) {
-                int mb = hist_bin (a[n], h.m_offset, h.m_delta, h.mbins);
-                atomicAdd (&row[mb], (unsigned long long) q[n]);
-            }
+            for (int n = 1; n < 8; n++)
+                if (b[n] == b[0]) { q[0] += q[n]; q[n] = 0; }
+            const int row = fb * h.mbins;
+            wave_hist_add (s_f, act, fb, 1ull);
+            wave_hist_add (s_j, act, row + b[0], (unsigned long long) q[0]);
+#pragma unroll
+            for (int n = 1; n < 8; n++)
+                if (act && q[n] != 0) atomicAdd (&s_j[row + b[n]], (unsigned long long) q[n]);
         }
     }
     __syncthreads ();
@@ -858,6 +927,19 @@ void k_volume_gradient (const float* __restrict__ img, float* __restrict__ out,
     }
 }
 
+// Bin image: floor((v - offset) / delta) with the IEEE division, as
+// Joint_histogram::add_pvi_8 (src/register/joint_histogram.cxx:458-482)
+// computes it per use; done once per stage.
+template <typename TI>
+__global__ void
+k_bin_image (const TI* __restrict__ img, uint8_t* __restrict__ out, int64_t n, float offset,
+    float delta, int bins)
+{
+    for (int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
+         i += (int64_t) gridDim.x * blockDim.x)
+        out[i] = (uint8_t) hist_bin (ld (img, (int) i), offset, delta, bins);
+}
+
 __global__ void
 k_zero_u64 (unsigned long long* __restrict__ p, int n)
 {
@@ -921,9 +1003,14 @@ launch_tiles_g (const KGeom& g, size_t lds, hipStream_t st, const void* f, const
     const void* mg, const float* coeff, const float* blut, float* cond, double* tile_score,
     int* tile_count, const MiArgs& mi)
 {
-    hipLaunchKernelGGL ((k_tiles<KIND, TI, G>), dim3 (8 * g.tiles_per_xcd), dim3 (TILE_THREADS),
-        lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff, blut, cond, tile_score,
-        tile_count, mi);
+    if (g.aligned)
+        hipLaunchKernelGGL ((k_tiles<KIND, TI, G, true>), dim3 (8 * g.tiles_per_xcd),
+            dim3 (TILE_THREADS), lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff,
+            blut, cond, tile_score, tile_count, mi);
+    else
+        hipLaunchKernelGGL ((k_tiles<KIND, TI, G, false>), dim3 (8 * g.tiles_per_xcd),
+            dim3 (TILE_THREADS), lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff,
+            blut, cond, tile_score, tile_count, mi);
 }
 
 template <int KIND, typename TI>
@@ -943,9 +1030,7 @@ launch_tiles (const KGeom& g, int vpt, size_t lds, hipStream_t st, const void* f
 static int
 reduce_blocks (const KGeom& g)
 {
-    const int per = REDUCE_THREADS / 64;
-    int b = (g.num_knots + per - 1) / per;
-    return b < 1 ? 1 : b;
+    return (3 * g.num_knots + REDUCE_THREADS - 1) / REDUCE_THREADS;
 }
 
 hipError_t
@@ -976,8 +1061,8 @@ launch_mse (const KGeom& g, const Tune& tune, int mode, int storage, const void*
 
 hipError_t
 launch_mi (const KGeom& g, const Tune& tune, const HGeom& h, int storage, int hist_blocks,
-    const void* f, const void* m, const float* coeff, const float* blut,
-    unsigned long long* jhist, unsigned long long* fhist, float* table,
+    const void* f, const void* m, const uint8_t* fbin, const uint8_t* mbin, const float* coeff,
+    const float* blut, unsigned long long* jhist, unsigned long long* fhist, float* table,
     float* nvox_f, float* cond, double* score, int64_t* nvox,
     double* grad, int64_t* hist_out, hipStream_t st, hipEvent_t* ev)
 {
@@ -991,31 +1076,41 @@ launch_mi (const KGeom& g, const Tune& tune, const HGeom& h, int storage, int hi
         + sizeof (unsigned long long) * nbins;
     const size_t lds2 = tile_lds_bytes (g) + sizeof (float) * h.fbins * h.mbins;
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
-    if (storage == 0)
-        hipLaunchKernelGGL ((k_mi_hist<float>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
-            g, h, (const float*) f, (const float*) m, coeff, blut, jhist, fhist);
+    if (g.aligned)
+        hipLaunchKernelGGL ((k_mi_hist<true>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
+            g, h, fbin, mbin, coeff, blut, jhist, fhist);
     else
-        hipLaunchKernelGGL ((k_mi_hist<uint16_t>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1,
-            st, g, h, (const uint16_t*) f, (const uint16_t*) m, coeff, blut, jhist, fhist);
+        hipLaunchKernelGGL ((k_mi_hist<false>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
+            g, h, fbin, mbin, coeff, blut, jhist, fhist);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 2, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_mi_score, dim3 (1), dim3 (1024), 0, st, h, jhist, fhist, table,
         score, nvox, nvox_f, hist_out);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 3, st)) != hipSuccess) return e;
-    MiArgs mi {h, table, nvox_f};
-    if (storage == 0)
-        launch_tiles<2, float> (g, tune.vox_per_thread, lds2, st, f, m, nullptr, coeff, blut, cond,
-            nullptr, nullptr, mi);
-    else
-        launch_tiles<2, uint16_t> (g, tune.vox_per_thread, lds2, st, f, m, nullptr, coeff, blut,
-            cond, nullptr, nullptr, mi);
+    MiArgs mi {h, table, nvox_f, fbin, mbin};
+    // pass 2 reads only bin images; the image type parameter is unused
+    launch_tiles<2, float> (g, tune.vox_per_thread, lds2, st, f, m, nullptr, coeff, blut, cond,
+        nullptr, nullptr, mi);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 4, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_reduce<false>, dim3 (reduce_blocks (g)), dim3 (REDUCE_THREADS), 0, st, g,
         cond, nullptr, nullptr, grad, nullptr, nullptr);
     LAUNCH_CHECK ();
     return mark (ev, 5, st);
+}
+
+hipError_t
+launch_bin_image (const void* img, int storage, uint8_t* out, int64_t n, float offset, float delta,
+    int bins, hipStream_t st)
+{
+    if (storage == 0)
+        hipLaunchKernelGGL ((k_bin_image<float>), dim3 (1024), dim3 (256), 0, st,
+            (const float*) img, out, n, offset, delta, bins);
+    else
+        hipLaunchKernelGGL ((k_bin_image<uint16_t>), dim3 (1024), dim3 (256), 0, st,
+            (const uint16_t*) img, out, n, offset, delta, bins);
+    return hipGetLastError ();
 }
 
 hipError_t
