@@ -613,22 +613,24 @@ void k_reduce (KGeom g, const float* __restrict__ cond, const double* __restrict
     const int i = blockIdx.x * REDUCE_THREADS + threadIdx.x;
     const bool own = i < nc;
     const float* c = cond + (own ? i : 0);
-    double acc = 0.0;
+    // all 64 slot loads are issued before any is consumed (a deep enough
+    // queue to cover the memory latency with few waves), then summed in slot
+    // order; the per-tile counts and scores are fetched meanwhile
+    float v[64];
+#pragma unroll
+    for (int m = 0; m < 64; m++) v[m] = c[(size_t) m * nc];
     long long n = 0;
     double sc = 0.0;
-    const int ntiles = MSE ? g.num_tiles : 0;
-    const int iters = max (64, (ntiles + REDUCE_THREADS - 1) / REDUCE_THREADS);
-#pragma unroll 8
-    for (int m = 0; m < iters; m++) {
-        if (m < 64) acc += (double) c[(size_t) m * nc];
-        if (MSE) {
-            const int t = m * REDUCE_THREADS + threadIdx.x;
-            if (t < ntiles) {
-                n += tile_count[t];
-                if (blockIdx.x == 0) sc += tile_score[t];
-            }
+    if (MSE) {
+#pragma unroll 4
+        for (int t = threadIdx.x; t < g.num_tiles; t += REDUCE_THREADS) {
+            n += tile_count[t];
+            if (blockIdx.x == 0) sc += tile_score[t];
         }
     }
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < 64; m++) acc += (double) v[m];
     double scale = 1.0;
     if (MSE) {
         n = wave_sum_ll (n);
@@ -652,27 +654,28 @@ void k_reduce (KGeom g, const float* __restrict__ cond, const double* __restrict
 
 // Displacement in the reference's k-j-i order, bx[i]*(by[j]*bz[k]) weights,
 // no FMA: bit-identical to bspline_interp_pix_c
-// (src/base/bspline_interpolate.cxx:78-110).
+// (src/base/bspline_interpolate.cxx:78-110).  The tile's coefficients are
+// read straight from global memory at wave-uniform addresses, so they come
+// through the scalar cache into SGPRs and every multiply takes its
+// coefficient as a scalar operand.
 __device__ __forceinline__ void
 exact_displacement (float out[3], const float* bx, const float* by, const float* bz,
-    const float* coef)
+    const float* __restrict__ coeff, int knot0, int cd0, int cd01)
 {
     float o0 = 0.f, o1 = 0.f, o2 = 0.f;
-    int m = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        float C = bz[k];
+        const float C = bz[k];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            float B = by[j] * C;
+            const float B = by[j] * C;
+            const float* cf = &coeff[3 * (knot0 + k * cd01 + j * cd0)];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                float A = bx[i] * B;
-                const float* cf = &coef[3 * m];
-                o0 += A * cf[0];
-                o1 += A * cf[1];
-                o2 += A * cf[2];
-                m++;
+                const float A = bx[i] * B;
+                o0 += A * cf[3 * i + 0];
+                o1 += A * cf[3 * i + 1];
+                o2 += A * cf[3 * i + 2];
             }
         }
     }
@@ -765,13 +768,7 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
     for (int tile = blockIdx.x; tile < G_.num_tiles; tile += gridDim.x) {
         int p[3];
         tile_coords (G_, tile, p);
-        __syncthreads ();
-        for (int e = threadIdx.x; e < 192; e += blockDim.x) {
-            int c = e % 3, m = e / 3;
-            int i = m & 3, j = (m >> 2) & 3, k = m >> 4;
-            int knot = (p[0] + i) + G_.cdims[0] * ((p[1] + j) + G_.cdims[1] * (p[2] + k));
-            s_coef[e] = coeff[3 * knot + c];
-        }
+        const int knot0 = p[0] + G_.cdims[0] * (p[1] + G_.cdims[1] * p[2]);
         __syncthreads ();
         // every lane runs the same number of rounds (the wave-level merges
         // need the whole wavefront), inactive lanes contribute nothing
@@ -788,7 +785,8 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
             float fxyz[3];
             position<ALIGNED> (fxyz, G_, fi, fj, fk);
             if (act)
-                exact_displacement (d, &s_bx[4 * qx], &s_by[4 * qy], &s_bz[4 * qz], s_coef);
+                exact_displacement (d, &s_bx[4 * qx], &s_by[4 * qy], &s_bz[4 * qz], coeff, knot0,
+                    G_.cdims[0], G_.cdims[0] * G_.cdims[1]);
             act = correspond<ALIGNED> (G_, fxyz, d, s) && act;
             int b[8];
             corner_bins (b, mbin, G_, s.mvf);
