@@ -59,6 +59,28 @@ mat3_inverse (float* out, const float* m)
     return true;
 }
 
+// The same cofactor inverse in double.
+bool
+mat3_inverse_d (double* out, const float* mf)
+{
+    double m[9];
+    for (int i = 0; i < 9; i++) m[i] = mf[i];
+    double det = m[0]*(m[4]*m[8]-m[5]*m[7])
+        - m[1]*(m[3]*m[8]-m[5]*m[6])
+        + m[2]*(m[3]*m[7]-m[4]*m[6]);
+    if (std::fabs (det) < 1e-8) return false;
+    out[0] =  (m[4]*m[8]-m[5]*m[7]) / det;
+    out[1] = -(m[1]*m[8]-m[2]*m[7]) / det;
+    out[2] =  (m[1]*m[5]-m[2]*m[4]) / det;
+    out[3] = -(m[3]*m[8]-m[5]*m[6]) / det;
+    out[4] =  (m[0]*m[8]-m[2]*m[6]) / det;
+    out[5] = -(m[0]*m[5]-m[2]*m[3]) / det;
+    out[6] =  (m[3]*m[7]-m[4]*m[6]) / det;
+    out[7] = -(m[0]*m[7]-m[1]*m[6]) / det;
+    out[8] =  (m[0]*m[4]-m[1]*m[3]) / det;
+    return true;
+}
+
 // Per-axis cubic B-spline basis B_t(i/vpr): float argument, double
 // polynomial, float result, as bspline_basis_eval
 // (src/base/bspline_xform.cxx:73-99).
@@ -90,7 +112,7 @@ struct plm_hip_ctx {
     bool mi_ok = false;
     int storage = PLM_STORE_F32;
     int hist_blocks = 512;
-    plmhip::Tune tune {2};
+    plmhip::Tune tune {2, 0};
     int64_t num_knots = 0;
     int64_t num_tiles = 0;
     size_t lds_limit = 0;
@@ -277,12 +299,49 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     g.num_tiles = (int) tiles;
     g.num_knots = (int) knots;
     g.tiles_per_xcd = (int) ((tiles + 7) / 8);
+    // The tile kernel maps a fixed voxel to moving index coordinates as
+    // mijk = P d + moff + M ijk with M = P step and moff = P (origin - morigin),
+    // P the moving proj; M and moff are formed in double from the float
+    // inputs, so an identity map is exactly the identity.
+    {
+        double pd[9];
+        if (!mat3_inverse_d (pd, moving->direction_cosines))
+            return fail (PLM_HIP_ERR_GEOMETRY, "singular direction cosines");
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) pd[3*i+j] /= (double) moving->spacing[i];
+        for (int a = 0; a < 3; a++) {
+            double o = 0.0;
+            for (int c = 0; c < 3; c++)
+                o += pd[3*a+c] * ((double) geom->img_origin[c] - (double) moving->origin[c]);
+            g.moff[a] = (float) o;
+            for (int b = 0; b < 3; b++) {
+                double s = 0.0;
+                for (int c = 0; c < 3; c++)
+                    s += pd[3*a+c] * (double) fixed->direction_cosines[3*c+b] * (double) fixed->spacing[b];
+                g.mstep[3*a+b] = (float) s;
+            }
+        }
+    }
+    // 32-bit byte offsets and 24-bit products of the tile kernel: every
+    // image (and the moving gradient, 12 bytes a voxel), plus the fixed
+    // voxels a border tile addresses past the image, stays below 2 GiB; a
+    // fixed slice holds < 2^24 voxels and a moving slice's gradient bytes
+    // stay below 2^24 (1398101 voxels, e.g. 1024 x 1024).
+    {
+        const int64_t nf = fixed->dim[0] * fixed->dim[1] * fixed->dim[2];
+        const int64_t nm = moving->dim[0] * moving->dim[1] * moving->dim[2];
+        const int64_t f01 = fixed->dim[0] * fixed->dim[1], m01 = moving->dim[0] * moving->dim[1];
+        if (f01 >= ((int64_t) 1 << 24) || 12 * m01 >= ((int64_t) 1 << 24)
+            || 4 * (nf + plmhip::MAX_VPR * f01) >= ((int64_t) 1 << 31)
+            || 12 * nm >= ((int64_t) 1 << 31))
+            return fail (PLM_HIP_ERR_UNSUPPORTED,
+                "image too large: fixed slices must hold < 2^24 voxels, moving slices "
+                "< 1398101 voxels, and images < 2 GiB (moving gradient at 12 bytes a voxel)");
+    }
     // exact n/d via __umulhi(n, ceil(2^32/d)) for n*d < 2^32
     g.mag_vx = (uint32_t) ((((uint64_t) 1 << 32) + g.vpr[0] - 1) / g.vpr[0]);
+    g.mag_vy = (uint32_t) ((((uint64_t) 1 << 32) + g.vpr[1] - 1) / g.vpr[1]);
     g.mag_vxy = (uint32_t) ((((uint64_t) 1 << 32) + g.vpr[0] * g.vpr[1] - 1) / (g.vpr[0] * g.vpr[1]));
-    g.vstep[0] = plmhip::TILE_THREADS % g.vpr[0];
-    g.vstep[1] = (plmhip::TILE_THREADS / g.vpr[0]) % g.vpr[1];
-    g.vstep[2] = plmhip::TILE_THREADS / (g.vpr[0] * g.vpr[1]);
     g.aligned = 1;
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
@@ -329,8 +388,9 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     c->num_knots = knots;
     c->num_tiles = tiles;
     c->lds_limit = prop.sharedMemPerBlock;
+    c->tune.lds_limit = c->lds_limit;
     c->hist_blocks = (int) std::min<int64_t> (tiles, (int64_t) prop.multiProcessorCount * 2);
-    if (plmhip::tile_lds_bytes (g) + sizeof (float) * h.fbins * h.mbins > c->lds_limit) {
+    if (plmhip::tile_lds_bytes (g, 1) + sizeof (float) * h.fbins * h.mbins > c->lds_limit) {
         free_ctx (c);
         return fail (PLM_HIP_ERR_UNSUPPORTED, "tile does not fit in LDS; use a smaller vox_per_rgn");
     }

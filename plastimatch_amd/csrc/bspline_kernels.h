@@ -26,14 +26,35 @@ struct KGeom {
     float morigin[3];
     float mproj[9];          // moving proj = inv(dc) / spacing
     float inv_msp[3];        // (float)(1.0 / moving spacing)
+    float mstep[9];          // M = moving proj * fixed step (moving index per fixed index)
+    float moff[3];           // moving proj * (img_origin - moving origin)
     int num_tiles;
     int num_knots;
     int tiles_per_xcd;
     uint32_t mag_vx;         // ceil(2^32 / vpr0), for n / vpr0 with __umulhi
+    uint32_t mag_vy;         // ceil(2^32 / vpr1)
     uint32_t mag_vxy;        // ceil(2^32 / (vpr0*vpr1))
-    int vstep[3];            // TILE_THREADS in mixed radix (vpr0, vpr1, inf)
     int aligned;             // fixed step and moving proj are diagonal
 };
+
+// dc/dv slots of a tile: its voxel count rounded up to whole batches of
+// TILE_THREADS * vox_per_thread.
+__host__ __device__ inline int
+tile_slots (int nvox, int vox_per_thread)
+{
+    const int s = TILE_THREADS * vox_per_thread;
+    return (nvox + s - 1) / s * s;
+}
+
+// Floats of the tile-kernel LDS region shared by the displacement stages
+// (T1, column terms) and the condense stages (S1, S2).
+__host__ __device__ inline int
+tile_region1_floats (int vx, int vy, int vz)
+{
+    const int a = 64 * vx + 16 * vx * vy;
+    const int b = 12 * vy * vz + 48 * vz;
+    return a > b ? a : b;
+}
 
 struct Mat3 {
     float m[9];
@@ -49,9 +70,12 @@ struct HGeom {
 // Launch-shape choices made by the autotuner.
 struct Tune {
     int vox_per_thread;      // voxels each tile-kernel thread keeps in flight: 1, 2 or 4
+    size_t lds_limit;        // LDS bytes a workgroup may use
 };
 
-size_t tile_lds_bytes (const KGeom& g);
+// LDS bytes of the tile kernel at vox_per_thread voxels per batch (the MI
+// table comes on top).
+size_t tile_lds_bytes (const KGeom& g, int vox_per_thread);
 
 // Stage timing: when ev is not null, ev[0..n] are recorded on st between
 // the launches (MSE: tiles | reduce, 3 events; MI: zero | hist | score |

@@ -220,164 +220,339 @@ tile_coords (const KGeom& g, int tile, int p[3])
     p[2] = t / g.rdims[1];
 }
 
-// LDS carve-up shared by the tile kernels (sizes in floats).
+// ---------------------------------------------------------------------------
+// raw buffer loads: 32-bit byte offsets from a wave-uniform base, so the
+// per-voxel address arithmetic stays in 32-bit VALU; a load at or past the
+// resource's byte size returns 0 without touching memory.
+
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+__device__ __forceinline__ Rsrc
+make_rsrc (const void* p, uint32_t bytes)
+{
+    // word 3 = 0x00020000: untyped raw buffer on gfx9 (no swizzle, no format)
+    return __builtin_amdgcn_make_buffer_rsrc (const_cast<void*> (p), (short) 0, (int) bytes,
+        0x00020000);
+}
+
+__device__ __forceinline__ int
+load_u8 (Rsrc r, int off, int soff)
+{
+    return (int) __builtin_amdgcn_raw_buffer_load_b8 (r, off, soff, 0);
+}
+
+// Image element loads by storage type: one value, two x-neighbours, and an
+// interleaved float3 (the moving gradient).
+template <typename TI> struct Img;
+
+template <> struct Img<float> {
+    static constexpr int ES = 4;
+    static __device__ __forceinline__ float one (Rsrc r, int off, int soff)
+    {
+        return __uint_as_float (__builtin_amdgcn_raw_buffer_load_b32 (r, off, soff, 0));
+    }
+    static __device__ __forceinline__ void pair (float& a, float& b, Rsrc r, int off, int soff)
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64 (r, off, soff, 0);
+        a = __uint_as_float (v[0]);
+        b = __uint_as_float (v[1]);
+    }
+    static __device__ __forceinline__ void vec3 (float o[3], Rsrc r, int off, int soff)
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b96 (r, off, soff, 0);
+        o[0] = __uint_as_float (v[0]);
+        o[1] = __uint_as_float (v[1]);
+        o[2] = __uint_as_float (v[2]);
+    }
+};
+
+template <> struct Img<uint16_t> {
+    static constexpr int ES = 2;
+    static __device__ __forceinline__ float one (Rsrc r, int off, int soff)
+    {
+        return __uint_as_float (((uint32_t) __builtin_amdgcn_raw_buffer_load_b16 (r, off, soff, 0)) << 16);
+    }
+    static __device__ __forceinline__ void pair (float& a, float& b, Rsrc r, int off, int soff)
+    {
+        a = one (r, off, soff);
+        b = one (r, off + 2, soff);
+    }
+    static __device__ __forceinline__ void vec3 (float o[3], Rsrc r, int off, int soff)
+    {
+        o[0] = one (r, off, soff);
+        o[1] = one (r, off + 2, soff);
+        o[2] = one (r, off + 4, soff);
+    }
+};
+
+// n / D for 0 <= n < 4096 and D < 256 with one 24-bit multiply:
+// floor(n * ceil(2^20 / D) / 2^20) is exact there (the excess is below
+// n / 2^20 < 1/256 < 1/D).
+template <int D>
+__device__ __forceinline__ int
+cdiv (int n)
+{
+    static_assert (D > 0 && D < 256, "cdiv divisor out of range");
+    constexpr uint32_t M = ((1u << 20) + D - 1) / D;
+    return (int) (__umul24 ((uint32_t) n, M) >> 20);
+}
+
+// Tile edge lengths: compile-time for the isotropic edges the kernels are
+// specialized for (V > 0), else the runtime vox_per_rgn.
+template <int V>
+struct Shape {
+    int vx, vy, vz;
+    uint32_t mx, my, mxy;
+    __device__ __forceinline__ explicit Shape (const KGeom& g)
+        : vx (V ? V : g.vpr[0]), vy (V ? V : g.vpr[1]), vz (V ? V : g.vpr[2]),
+          mx (g.mag_vx), my (g.mag_vy), mxy (g.mag_vxy) {}
+    __device__ __forceinline__ int div_x (int n) const
+    {
+        if constexpr (V > 0) return cdiv<V> (n); else return fdiv (n, mx);
+    }
+    __device__ __forceinline__ int div_y (int n) const
+    {
+        if constexpr (V > 0) return cdiv<V> (n); else return fdiv (n, my);
+    }
+    __device__ __forceinline__ int div_xy (int n) const
+    {
+        if constexpr (V > 0) return cdiv<V * V> (n); else return fdiv (n, mxy);
+    }
+};
+
+// LDS carve-up of the tile kernel, in floats; every array read as float4
+// starts on a 16-byte boundary (all offsets are multiples of 4 floats).
 struct TileLds {
-    float* bx; float* by; float* bz;   // 4*vpr each
-    float* coef;                       // [k][j][i][c], 192
-    float* t1;                         // [k][j][x][c] then S2 [z][j][i][c]
-    float* t2;                         // [y][x][c][k] then S1 [z][y][i][c]
-    float* dv;                         // [z][y][x][c]
-    double* red;                       // 16 doubles
-    float* tab;                        // MI: dS/dP table, Bf*Bm
+    float* bx;      // [x][4] basis LUT along x
+    float* by;      // [y][4]
+    float* bzz;     // [z][8]: bz[0..3], Z[z][0..2], pad
+    float* coef;    // [k][j][i][c]: the tile's 64 knots
+    float* t1;      // [k][j][x][4]: sum_i bx coef
+    float* col;     // [4][y][x][4]: planes A[0][k], A[1][k], A[2][k], (C[a], pad)
+    float* s1;      // [z][y][i][c]: dc/dv condensed over x (reuses t1/col)
+    float* s2;      // [z][j][i][c]: then over y (reuses t1/col)
+    float* dv;      // [c][slot]: dc/dv of every voxel slot, one plane per component
+    int dvp;        // plane stride (slots)
+    double* red;    // block-sum scratch, 16 doubles
+    float* tab;     // MI: dS/dP table, Bf*Bm
 };
 
 __device__ __forceinline__ TileLds
-carve (float* sm, const KGeom& g)
+carve (float* sm, int vx, int vy, int vz, int slots)
 {
     TileLds L;
-    const int vx = g.vpr[0], vy = g.vpr[1], vz = g.vpr[2];
-    const int mx = vx > vz ? vx : vz;
     L.bx = sm;
     L.by = L.bx + 4 * vx;
-    L.bz = L.by + 4 * vy;
-    L.coef = L.bz + 4 * vz;
+    L.bzz = L.by + 4 * vy;
+    L.coef = L.bzz + 8 * vz;
     L.t1 = L.coef + 192;
-    L.t2 = L.t1 + 48 * mx;
-    L.dv = L.t2 + 12 * vy * mx;
-    float* end = L.dv + 3 * vx * vy * vz;
-    uintptr_t e = ((uintptr_t) end + 7) & ~(uintptr_t) 7;
-    L.red = (double*) e;
+    L.col = L.t1 + 64 * vx;
+    L.s1 = L.t1;
+    L.s2 = L.s1 + 12 * vy * vz;
+    L.dv = L.t1 + tile_region1_floats (vx, vy, vz);
+    L.dvp = slots;
+    // slots is a multiple of 256, so red stays 16-byte aligned
+    L.red = (double*) (L.dv + 3 * slots);
     L.tab = (float*) (L.red + 16);
     return L;
 }
 
-// Stage LUTs and the tile's 64 knot coefficients.
+// Stage 1: basis LUTs, the per-slice moving-index term Z and the tile's 64
+// knot coefficients.  Z[z][a] = M[a][2] * (bz0 + z), M = P_moving * step_fixed.
+template <int V>
 __device__ __forceinline__ void
-load_tile (const TileLds& L, const KGeom& g, const int p[3],
+tile_load (const TileLds& L, const Shape<V>& S, const KGeom& g, int knot0, int bz0,
     const float* __restrict__ coeff, const float* __restrict__ blut)
 {
-    const int nl = 4 * (g.vpr[0] + g.vpr[1] + g.vpr[2]);
-    for (int i = threadIdx.x; i < nl; i += blockDim.x) L.bx[i] = blut[i];
-    for (int e = threadIdx.x; e < 192; e += blockDim.x) {
-        int c = e % 3, m = e / 3;
-        int i = m & 3, j = (m >> 2) & 3, k = m >> 4;
-        int knot = (p[0] + i) + g.cdims[0] * ((p[1] + j) + g.cdims[1] * (p[2] + k));
-        L.coef[e] = coeff[3 * knot + c];
+    const int t = threadIdx.x;
+    const int nxy = 4 * (S.vx + S.vy);
+    for (int e = t; e < nxy + 4 * S.vz; e += TILE_THREADS) {
+        const float b = blut[e];
+        if (e < nxy) {
+            L.bx[e] = b;                  // bx and by are contiguous in both
+        } else {
+            const int q = e - nxy;        // z*4 + k -> z*8 + k
+            L.bzz[2 * q - (q & 3)] = b;
+        }
+    }
+    for (int e = t; e < 3 * S.vz; e += TILE_THREADS) {
+        const int z = e / 3, a = e - 3 * z;
+        L.bzz[8 * z + 4 + a] = g.mstep[3 * a + 2] * (float) (bz0 + z);
+    }
+    if (t < 64) {
+        // t = (k*4 + j)*4 + i, the slot order of coef
+        const int cd0 = g.cdims[0], cd01 = g.cdims[0] * g.cdims[1];
+        const int knot = knot0 + (t & 3) + cd0 * ((t >> 2) & 3) + cd01 * (t >> 4);
+        const float* c = &coeff[3 * knot];
+        L.coef[3 * t + 0] = c[0];
+        L.coef[3 * t + 1] = c[1];
+        L.coef[3 * t + 2] = c[2];
     }
 }
 
-// T1[k][j][x][c] = sum_i bx[x][i] coef[k][j][i][c];
-// T2[y][x][c][k] = sum_j by[y][j] T1[k][j][x][c].
+// Stage 2: T1[k][j][x][c] = sum_i bx[x][i] coef[k][j][i][c], one (k, j, x)
+// per thread.  A coefficient row (k, j) is 12 floats: c fastest, then i.
+template <int V>
 __device__ __forceinline__ void
-separable_displacement (const TileLds& L, const KGeom& g)
+tile_t1 (const TileLds& L, const Shape<V>& S)
 {
-    const int vx = g.vpr[0], vy = g.vpr[1];
-    const int n1 = 48 * vx;
-    for (int e = threadIdx.x; e < n1; e += blockDim.x) {
-        int c = e % 3, e3 = e / 3;
-        int kj = fdiv (e3, g.mag_vx), x = e3 - kj * vx;
-        const float* cf = &L.coef[(kj * 4) * 3 + c];
-        const float* b = &L.bx[4 * x];
-        float v = b[0] * cf[0];
-        v = fmaf (b[1], cf[3], v);
-        v = fmaf (b[2], cf[6], v);
-        v = fmaf (b[3], cf[9], v);
-        L.t1[e] = v;   // index ((kj*vx + x)*3 + c)
+    for (int e = threadIdx.x; e < 16 * S.vx; e += TILE_THREADS) {
+        const int kj = S.div_x (e), x = e - kj * S.vx;
+        const float4* q = reinterpret_cast<const float4*> (&L.coef[12 * kj]);
+        const float4 q0 = q[0], q1 = q[1], q2 = q[2];
+        const float4 b = reinterpret_cast<const float4*> (L.bx)[x];
+        float4 o;
+        o.x = fmaf (b.w, q2.y, fmaf (b.z, q1.z, fmaf (b.y, q0.w, b.x * q0.x)));
+        o.y = fmaf (b.w, q2.z, fmaf (b.z, q1.w, fmaf (b.y, q1.x, b.x * q0.y)));
+        o.z = fmaf (b.w, q2.w, fmaf (b.z, q2.x, fmaf (b.y, q1.y, b.x * q0.z)));
+        o.w = 0.f;
+        reinterpret_cast<float4*> (L.t1)[e] = o;     // e = kj*vx + x
     }
-    __syncthreads ();
-    const int n2 = 12 * vx * vy;
-    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
-        int k = e & 3, c = (e >> 2) % 3, xy = e / 12;
-        int y = fdiv (xy, g.mag_vx), x = xy - y * vx;
-        const float* b = &L.by[4 * y];
-        float v = 0.f;
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            v = fmaf (b[j], L.t1[(((k * 4 + j) * vx) + x) * 3 + c], v);
-        L.t2[e] = v;   // index ((y*vx + x)*3 + c)*4 + k
-    }
-    __syncthreads ();
 }
 
+// Stage 3: the per-column terms of the moving-voxel coordinate.  With
+// T2[c][k] = sum_j by[y][j] T1[k][j][x][c] (the displacement still to be
+// contracted along z), A[a][k] = sum_c P[a][c] T2[c][k] and
+// C[a] = moff[a] + M[a][0] fi + M[a][1] fj, a voxel (x, y, z) of the tile
+// maps to moving index coordinates
+//   mijk[a] = sum_k bz[z][k] A[a][k] + C[a] + Z[z][a],
+// which is P * (fixed position + displacement - moving origin) regrouped
+// (bspline_find_correspondence_dcos, src/util/bspline_correspond.cxx:80-109,
+// with the position of POSITION_FROM_COORDS, src/base/volume_macros.h:82-96).
+template <int V>
 __device__ __forceinline__ void
-voxel_displacement (float d[3], const TileLds& L, int qx, int qy, int qz, int vx)
+tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int by0)
 {
-    const float4* t = reinterpret_cast<const float4*> (&L.t2[((qy * vx + qx) * 3) * 4]);
-    const float* b = &L.bz[4 * qz];
+    const int vx = S.vx;
+    const float4* t1 = reinterpret_cast<const float4*> (L.t1);
+    for (int e = threadIdx.x; e < vx * S.vy; e += TILE_THREADS) {
+        const int y = S.div_x (e), x = e - y * vx;
+        const float4 b = reinterpret_cast<const float4*> (L.by)[y];
+        float T[3][4];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float4 v = t[c];
-        float s = b[0] * v.x;
-        s = fmaf (b[1], v.y, s);
-        s = fmaf (b[2], v.z, s);
-        s = fmaf (b[3], v.w, s);
-        d[c] = s;
+        for (int k = 0; k < 4; k++) {
+            const float4 r0 = t1[(4 * k + 0) * vx + x];
+            const float4 r1 = t1[(4 * k + 1) * vx + x];
+            const float4 r2 = t1[(4 * k + 2) * vx + x];
+            const float4 r3 = t1[(4 * k + 3) * vx + x];
+            T[0][k] = fmaf (b.w, r3.x, fmaf (b.z, r2.x, fmaf (b.y, r1.x, b.x * r0.x)));
+            T[1][k] = fmaf (b.w, r3.y, fmaf (b.z, r2.y, fmaf (b.y, r1.y, b.x * r0.y)));
+            T[2][k] = fmaf (b.w, r3.z, fmaf (b.z, r2.z, fmaf (b.y, r1.z, b.x * r0.z)));
+        }
+        const float* P = g.mproj;
+        // one float4 plane per term, so the voxel pass reads consecutive
+        // 16-byte records (conflict-free) for consecutive columns
+        float4* o = reinterpret_cast<float4*> (L.col) + e;
+        const int np = vx * S.vy;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float A[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                A[k] = fmaf (P[3 * a + 2], T[2][k], fmaf (P[3 * a + 1], T[1][k], P[3 * a] * T[0][k]));
+            o[a * np] = make_float4 (A[0], A[1], A[2], A[3]);
+        }
+        const float fi = (float) (bx0 + x), fj = (float) (by0 + y);
+        float C[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) C[a] = (g.moff[a] + g.mstep[3 * a] * fi) + g.mstep[3 * a + 1] * fj;
+        o[3 * np] = make_float4 (C[0], C[1], C[2], 0.f);
     }
 }
 
-// Condense the parked dc/dv of the tile into its 64 knot slots and write
-// them slot-major: cond[slot][knot][c], slot = (k*4 + j)*4 + i.  Each
-// (knot, slot) entry has exactly one writer, as in bspline_sort_sets
+// Stages 5-7: condense the tile's parked dc/dv into its 64 knot slots, the
+// transposed separable contraction (x, then y, then z), and write them
+// slot-major: cond[slot][knot][c], slot = (k*4 + j)*4 + i.  Each (knot,
+// slot) entry has exactly one writer, as in bspline_sort_sets
 // (src/register/bspline.cxx:377-398); slot-major keeps the reduce's reads
-// coalesced.  S1 over x, S2 over y, S3 over z.
+// coalesced.
+template <int V>
 __device__ __forceinline__ void
-condense_tile (const TileLds& L, const KGeom& g, const int p[3], float* __restrict__ cond)
+tile_condense (const TileLds& L, const Shape<V>& S, const KGeom& g, int knot0,
+    float* __restrict__ cond)
 {
-    const int vx = g.vpr[0], vy = g.vpr[1], vz = g.vpr[2];
-    float* s1 = L.t2;   // [z][y][i][c]
-    float* s2 = L.t1;   // [z][j][i][c]
-    const int n1 = 3 * vy * vz;
-    for (int e = threadIdx.x; e < n1; e += blockDim.x) {
-        int c = e % 3, zy = e / 3;
-        const float* dv = &L.dv[zy * vx * 3 + c];
-        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-        for (int x = 0; x < vx; x++) {
-            float v = dv[3 * x];
-            const float* b = &L.bx[4 * x];
-            a0 = fmaf (b[0], v, a0);
-            a1 = fmaf (b[1], v, a1);
-            a2 = fmaf (b[2], v, a2);
-            a3 = fmaf (b[3], v, a3);
+    const int vx = S.vx, vy = S.vy, vz = S.vz;
+    const int t = threadIdx.x;
+    // S1[z][y][i][c] = sum_x bx[x][i] dv[z][y][x][c], one (z, y) row per thread
+    for (int e = t; e < vy * vz; e += TILE_THREADS) {
+        const float* d0 = L.dv + e * vx;
+        const float* d1 = d0 + L.dvp;
+        const float* d2 = d1 + L.dvp;
+        const float4* bx = reinterpret_cast<const float4*> (L.bx);
+        float acc[4][3];
+        {
+            const float v0 = d0[0], v1 = d1[0], v2 = d2[0];
+            const float4 b = bx[0];
+            acc[0][0] = b.x * v0; acc[0][1] = b.x * v1; acc[0][2] = b.x * v2;
+            acc[1][0] = b.y * v0; acc[1][1] = b.y * v1; acc[1][2] = b.y * v2;
+            acc[2][0] = b.z * v0; acc[2][1] = b.z * v1; acc[2][2] = b.z * v2;
+            acc[3][0] = b.w * v0; acc[3][1] = b.w * v1; acc[3][2] = b.w * v2;
         }
-        float* o = &s1[zy * 12 + c];
-        o[0] = a0; o[3] = a1; o[6] = a2; o[9] = a3;
+#pragma unroll
+        for (int x = 1; x < vx; x++) {
+            const float v0 = d0[x], v1 = d1[x], v2 = d2[x];
+            const float4 b = bx[x];
+            const float bb[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                acc[i][0] = fmaf (bb[i], v0, acc[i][0]);
+                acc[i][1] = fmaf (bb[i], v1, acc[i][1]);
+                acc[i][2] = fmaf (bb[i], v2, acc[i][2]);
+            }
+        }
+        float4* o = reinterpret_cast<float4*> (&L.s1[12 * e]);
+        o[0] = make_float4 (acc[0][0], acc[0][1], acc[0][2], acc[1][0]);
+        o[1] = make_float4 (acc[1][1], acc[1][2], acc[2][0], acc[2][1]);
+        o[2] = make_float4 (acc[2][2], acc[3][0], acc[3][1], acc[3][2]);
     }
     __syncthreads ();
-    const int n2 = 12 * vz;
-    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
-        int ic = e % 12, z = e / 12;
-        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-        for (int y = 0; y < vy; y++) {
-            float v = s1[(z * vy + y) * 12 + ic];
-            const float* b = &L.by[4 * y];
-            a0 = fmaf (b[0], v, a0);
-            a1 = fmaf (b[1], v, a1);
-            a2 = fmaf (b[2], v, a2);
-            a3 = fmaf (b[3], v, a3);
+    // S2[z][j][i][c] = sum_y by[y][j] S1[z][y][i][c], one (z, i, c) per thread
+    for (int e = t; e < 12 * vz; e += TILE_THREADS) {
+        const int z = e / 12, ic = e - 12 * z;
+        const float* s = &L.s1[z * vy * 12 + ic];
+        const float4* by = reinterpret_cast<const float4*> (L.by);
+        float a0, a1, a2, a3;
+        {
+            const float v = s[0];
+            const float4 b = by[0];
+            a0 = b.x * v; a1 = b.y * v; a2 = b.z * v; a3 = b.w * v;
         }
-        float* o = &s2[z * 48 + ic];
+        for (int y = 1; y < vy; y++) {
+            const float v = s[12 * y];
+            const float4 b = by[y];
+            a0 = fmaf (b.x, v, a0);
+            a1 = fmaf (b.y, v, a1);
+            a2 = fmaf (b.z, v, a2);
+            a3 = fmaf (b.w, v, a3);
+        }
+        float* o = &L.s2[z * 48 + ic];
         o[0] = a0; o[12] = a1; o[24] = a2; o[36] = a3;
     }
     __syncthreads ();
-    if (threadIdx.x < 48) {
-        const int jic = threadIdx.x;          // (j*4 + i)*3 + c
-        const int c = jic % 3, ji = jic / 3, i = ji & 3, j = ji >> 2;
-        float a[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int z = 0; z < vz; z++) {
-            float v = s2[z * 48 + jic];
-            const float* b = &L.bz[4 * z];
-#pragma unroll
-            for (int k = 0; k < 4; k++) a[k] = fmaf (b[k], v, a[k]);
+    // S3: sum over z into the 4 slots along k, one (j, i, c) per thread
+    if (t < 48) {
+        const int jic = t;                      // (j*4 + i)*3 + c
+        const int ji = jic / 3, c = jic - 3 * ji, i = ji & 3, j = ji >> 2;
+        float a[4];
+        {
+            const float v = L.s2[jic];
+            const float4 b = reinterpret_cast<const float4*> (L.bzz)[0];
+            a[0] = b.x * v; a[1] = b.y * v; a[2] = b.z * v; a[3] = b.w * v;
+        }
+        for (int z = 1; z < vz; z++) {
+            const float v = L.s2[z * 48 + jic];
+            const float4 b = reinterpret_cast<const float4*> (L.bzz)[2 * z];
+            a[0] = fmaf (b.x, v, a[0]);
+            a[1] = fmaf (b.y, v, a[1]);
+            a[2] = fmaf (b.z, v, a[2]);
+            a[3] = fmaf (b.w, v, a[3]);
         }
         const int cd0 = g.cdims[0], cd01 = g.cdims[0] * g.cdims[1];
-        const int knot0 = (p[0] + i) + cd0 * (p[1] + j) + cd01 * p[2];
+        const int knot = knot0 + i + cd0 * j;
         const size_t plane = (size_t) 3 * g.num_knots;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int knot = knot0 + k * cd01;
-            const int slot = k * 16 + ji;
-            cond[slot * plane + 3 * knot + c] = a[k];
-        }
+        for (int k = 0; k < 4; k++)
+            cond[(size_t) (k * 16 + ji) * plane + 3 * (knot + k * cd01) + c] = a[k];
     }
 }
 
@@ -386,23 +561,26 @@ condense_tile (const TileLds& L, const KGeom& g, const int p[3], float* __restri
 __device__ __forceinline__ void
 block_sum (const TileLds& L, double& s, int& n)
 {
-    double nd = (double) n;
     for (int off = 32; off > 0; off >>= 1) {
         s += __shfl_down (s, off);
-        nd += __shfl_down (nd, off);
+        n += __shfl_down (n, off);
     }
     const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     __syncthreads ();
     if ((threadIdx.x & 63) == 0) {
         L.red[2 * w] = s;
-        L.red[2 * w + 1] = nd;
+        reinterpret_cast<int*> (&L.red[2 * w + 1])[0] = n;
     }
     __syncthreads ();
     if (threadIdx.x == 0) {
-        double a = 0, b = 0;
-        for (int i = 0; i < nw; i++) { a += L.red[2 * i]; b += L.red[2 * i + 1]; }
+        double a = 0;
+        int b = 0;
+        for (int i = 0; i < nw; i++) {
+            a += L.red[2 * i];
+            b += reinterpret_cast<const int*> (&L.red[2 * i + 1])[0];
+        }
         s = a;
-        n = (int) b;
+        n = b;
     }
 }
 
@@ -423,109 +601,173 @@ struct MiArgs {
     const uint8_t* mbin;       // per-voxel moving bin
 };
 
-template <int KIND, typename TI, int G, bool ALIGNED>
+template <int KIND, typename TI, int V, int G>
 __global__ __launch_bounds__ (TILE_THREADS)
 void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
     const TI* __restrict__ mgrad, const float* __restrict__ coeff,
     const float* __restrict__ blut, float* __restrict__ cond,
     double* __restrict__ tile_score, int* __restrict__ tile_count, MiArgs mi)
 {
-    extern __shared__ float sm[];
+    extern __shared__ float4 sm4[];
     const int tile = xcd_tile (g, blockIdx.x);
     if (tile >= g.num_tiles) return;
-    const KGeom& G_ = g;
-    TileLds L = carve (sm, G_);
+    const Shape<V> S (g);
+    const int vx = S.vx, vy = S.vy, vz = S.vz;
+    const int nvox = vx * vy * vz;
+    const int slots = tile_slots (nvox, G);
+    const TileLds L = carve (reinterpret_cast<float*> (sm4), vx, vy, vz, slots);
+    const int t = threadIdx.x;
     if (KIND == 2) {
         const int nj = mi.h.fbins * mi.h.mbins;
-        for (int i = threadIdx.x; i < nj; i += blockDim.x) L.tab[i] = mi.table[i];
+        for (int i = t; i < nj; i += TILE_THREADS) L.tab[i] = mi.table[i];
     }
     int p[3];
-    tile_coords (G_, tile, p);
-    load_tile (L, G_, p, coeff, blut);
-    __syncthreads ();
-    separable_displacement (L, G_);
+    tile_coords (g, tile, p);
+    const int bx0 = g.roi_off[0] + vx * p[0];
+    const int by0 = g.roi_off[1] + vy * p[1];
+    const int bz0 = g.roi_off[2] + vz * p[2];
+    const int knot0 = p[0] + g.cdims[0] * (p[1] + g.cdims[1] * p[2]);
 
-    const int vx = G_.vpr[0], vy = G_.vpr[1], vz = G_.vpr[2];
-    const int nvox = vx * vy * vz;
-    const int bx0 = G_.roi_off[0] + vx * p[0];
-    const int by0 = G_.roi_off[1] + vy * p[1];
-    const int bz0 = G_.roi_off[2] + vz * p[2];
+    tile_load (L, S, g, knot0, bz0, coeff, blut);
+    __syncthreads ();
+    tile_t1 (L, S);
+    __syncthreads ();
+    tile_columns (L, S, g, bx0, by0);
+    __syncthreads ();
+
+    // Stage 4: the voxels.  Thread t takes voxels t, t + 256, ... of the
+    // tile (x fastest), G at a time, issuing all their loads before using
+    // any.  Voxels outside the ROI or the tile still run (their fixed load
+    // may fall past the image and read 0) and are masked at the end.
+    const int ex = min (vx, g.roi_end[0] - bx0);
+    const int ey = min (vy, g.roi_end[1] - by0);
+    const int ez = min (vz, g.roi_end[2] - bz0);
+    const int fd0 = g.fdim[0], fd01 = g.fdim[0] * g.fdim[1];
+    const int fbase = bx0 + fd0 * by0 + fd01 * bz0;
+    const int md0 = g.mdim[0], md01 = g.mdim[0] * g.mdim[1];
+    constexpr int ES = KIND == 2 ? 1 : Img<TI>::ES;   // bytes per voxel of what is loaded
+    const Rsrc fr = make_rsrc (KIND == 2 ? (const void*) mi.fbin : (const void*) fimg,
+        (uint32_t) (fd01 * g.fdim[2]) * ES);
+    const Rsrc mr = make_rsrc (KIND == 2 ? (const void*) mi.mbin : (const void*) mimg,
+        (uint32_t) (md01 * g.mdim[2]) * ES);
+    const Rsrc gr = make_rsrc (mgrad, KIND == 0 ? (uint32_t) (md01 * g.mdim[2]) * 3 * ES : 0u);
+    const int SX = md0 * ES, SXY = md01 * ES;
+    float mhi[3], mmax[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        mhi[a] = (float) g.mdim[a] - 0.5f;
+        mmax[a] = (float) (g.mdim[a] - 2);
+    }
     float N = 1.f;
     if (KIND == 2) N = *mi.nvox_f;
 
-    // local coordinates of this thread's first voxel, advanced by
-    // TILE_THREADS voxels per step in mixed radix
-    int cz = fdiv (threadIdx.x, G_.mag_vxy);
-    int rem = threadIdx.x - cz * vx * vy;
-    int cy = fdiv (rem, G_.mag_vx);
-    int cx = rem - cy * vx;
-
     double score = 0.0;
     int count = 0;
+#pragma unroll
     for (int base = 0; base < nvox; base += TILE_THREADS * G) {
-        int qx[G], qy[G], qz[G], vv[G];
-        bool act[G], in[G];
-        int fidx[G];
+        bool in[G];
+        float fr_[G][3];
         float fval[G];
         int fbv[G];
-#pragma unroll
-        for (int u = 0; u < G; u++) {
-            vv[u] = base + u * TILE_THREADS + threadIdx.x;
-            qx[u] = cx; qy[u] = cy; qz[u] = cz;
-            cx += G_.vstep[0]; cy += G_.vstep[1]; cz += G_.vstep[2];
-            if (cx >= vx) { cx -= vx; cy++; }
-            if (cy >= vy) { cy -= vy; cz++; }
-            const int fi = bx0 + qx[u], fj = by0 + qy[u], fk = bz0 + qz[u];
-            act[u] = vv[u] < nvox && fi < G_.roi_end[0] && fj < G_.roi_end[1] && fk < G_.roi_end[2];
-            fidx[u] = act[u] ? fi + G_.fdim[0] * (fj + G_.fdim[1] * fk) : 0;
-            if (KIND < 2) fval[u] = ld (fimg, fidx[u]);
-            else fbv[u] = mi.fbin[fidx[u]];
-        }
-        Sample s[G];
-#pragma unroll
-        for (int u = 0; u < G; u++) {
-            float fxyz[3], d[3];
-            position<ALIGNED> (fxyz, G_, bx0 + qx[u], by0 + qy[u], bz0 + qz[u]);
-            int qyc = qy[u] < vy ? qy[u] : 0, qzc = qz[u] < vz ? qz[u] : 0;
-            voxel_displacement (d, L, qx[u], qyc, qzc, vx);
-            in[u] = correspond<ALIGNED> (G_, fxyz, d, s[u]) && act[u];
-        }
         float a[G][8];
         float mg[G][3];
         int bn[G][8];
 #pragma unroll
         for (int u = 0; u < G; u++) {
-            if (KIND < 2) corners (a[u], mimg, G_, s[u].mvf);
-            else corner_bins (bn[u], mi.mbin, G_, s[u].mvf);
+            const int v = base + u * TILE_THREADS + t;
+            const int z = S.div_xy (v), r = v - z * (vx * vy);
+            const int y = S.div_x (r), x = r - y * vx;
+            const bool act = v < nvox && x < ex && y < ey && z < ez;
+            const int zc = min (z, vz - 1);
+            const int fofs = fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
+            if (KIND < 2) fval[u] = Img<TI>::one (fr, fofs * ES, 0);
+            else fbv[u] = load_u8 (fr, fofs, 0);
+
+            const float4* cp = reinterpret_cast<const float4*> (L.col) + r;
+            const float4* zp = reinterpret_cast<const float4*> (&L.bzz[8 * zc]);
+            const float4 b = zp[0], Z = zp[1];
+            const int np = vx * vy;
+            const float4 A0 = cp[0], A1 = cp[np], A2 = cp[2 * np], C = cp[3 * np];
+            float m[3];
+            m[0] = fmaf (b.w, A0.w, fmaf (b.z, A0.z, fmaf (b.y, A0.y, fmaf (b.x, A0.x, C.x)))) + Z.x;
+            m[1] = fmaf (b.w, A1.w, fmaf (b.z, A1.z, fmaf (b.y, A1.y, fmaf (b.x, A1.x, C.y)))) + Z.y;
+            m[2] = fmaf (b.w, A2.w, fmaf (b.z, A2.z, fmaf (b.y, A2.y, fmaf (b.x, A2.x, C.z)))) + Z.z;
+
+            // Volume::is_inside (src/base/volume.cxx:899-906) and li_clamp
+            // (src/base/interpolate.cxx:17-41), branch free:
+            // f = clamp(floor(m), 0, dim-2), frac = clamp(m - f, 0, 1) and
+            // round = f + (frac >= 0.5) give exactly its three branches (and
+            // the rounding of ROUND_PLM_LONG).  Any m, NaN included, yields
+            // in-range indices, so the loads are issued unconditionally.
+            bool inside = true;
+            int f[3], c[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                inside = inside && m[q] > -0.5f && m[q] < mhi[q];
+                const float fl = __builtin_amdgcn_fmed3f (floorf (m[q]), 0.f, mmax[q]);
+                const float fa = fminf (fmaxf (m[q] - fl, 0.f), 1.f);
+                f[q] = (int) fl;
+                c[q] = fa >= 0.5f ? 1 : 0;
+                fr_[u][q] = fa;
+            }
+            in[u] = inside && act;
+            // byte offsets as 24-bit products (slices hold < 2^24 / 12
+            // voxels, checked at context creation)
+            const int idx = f[0] + (int) __umul24 (md0, f[1]) + (int) __umul24 (md01, f[2]);
+            if (KIND < 2) {
+                const int off = (int) __umul24 (f[0], ES) + (int) __umul24 (f[1], SX)
+                    + (int) __umul24 (f[2], SXY);
+                Img<TI>::pair (a[u][0], a[u][1], mr, off, 0);
+                Img<TI>::pair (a[u][2], a[u][3], mr, off, SX);
+                Img<TI>::pair (a[u][4], a[u][5], mr, off, SXY);
+                Img<TI>::pair (a[u][6], a[u][7], mr, off, SX + SXY);
+            } else {
+                bn[u][0] = load_u8 (mr, idx, 0);
+                bn[u][1] = load_u8 (mr, idx + 1, 0);
+                bn[u][2] = load_u8 (mr, idx, md0);
+                bn[u][3] = load_u8 (mr, idx + 1, md0);
+                bn[u][4] = load_u8 (mr, idx, md01);
+                bn[u][5] = load_u8 (mr, idx + 1, md01);
+                bn[u][6] = load_u8 (mr, idx, md0 + md01);
+                bn[u][7] = load_u8 (mr, idx + 1, md0 + md01);
+            }
             if (KIND == 0) {
-                mg[u][0] = ld (mgrad, 3 * s[u].mvr + 0);
-                mg[u][1] = ld (mgrad, 3 * s[u].mvr + 1);
-                mg[u][2] = ld (mgrad, 3 * s[u].mvr + 2);
+                // nearest voxel: the gradient sample of flavor g
+                // (src/register/bspline_mse.cxx:679-681)
+                const int goff = (int) __umul24 (f[0] + c[0], 3 * ES)
+                    + (int) __umul24 (f[1] + c[1], 3 * SX) + (int) __umul24 (f[2] + c[2], 3 * SXY);
+                Img<TI>::vec3 (mg[u], gr, goff, 0);
             }
         }
 #pragma unroll
         for (int u = 0; u < G; u++) {
-            float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f;
+            const int v = base + u * TILE_THREADS + t;
+            float dc0, dc1, dc2;
             if (KIND < 2) {
+                Sample s;
+                s.l2[0] = fr_[u][0]; s.l2[1] = fr_[u][1]; s.l2[2] = fr_[u][2];
                 float md[3];
-                const float m_val = trilinear<KIND == 1> (s[u], a[u], G_.inv_msp, md);
-                const float diff = m_val - fval[u];
-                if (in[u]) {
-                    score += (double) (diff * diff);
-                    count++;
-                    if (KIND == 0) {
-                        dc0 = diff * mg[u][0];
-                        dc1 = diff * mg[u][1];
-                        dc2 = diff * mg[u][2];
-                    } else {
-                        dc0 = diff * md[0];
-                        dc1 = diff * md[1];
-                        dc2 = diff * md[2];
-                    }
+                const float m_val = trilinear<KIND == 1> (s, a[u], g.inv_msp, md);
+                const float diff = in[u] ? m_val - fval[u] : 0.f;
+                score += (double) (diff * diff);
+                count += in[u] ? 1 : 0;
+                if (KIND == 0) {
+                    dc0 = diff * mg[u][0];
+                    dc1 = diff * mg[u][1];
+                    dc2 = diff * mg[u][2];
+                } else {
+                    dc0 = diff * md[0];
+                    dc1 = diff * md[1];
+                    dc2 = diff * md[2];
                 }
-            } else if (in[u]) {
-                const float* l1 = s[u].l1;
-                const float* l2 = s[u].l2;
+            } else {
+                float l1[3], l2[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    l2[q] = fr_[u][q];
+                    l1[q] = 1.0f - l2[q];
+                }
+                dc0 = 0.f; dc1 = 0.f; dc2 = 0.f;
                 float dw[24];
                 dw[0] = -l1[1] * l1[2];  dw[1] = -l1[0] * l1[2];  dw[2] = -(l1[0] * l1[1]);
                 dw[3] = l1[1] * l1[2];   dw[4] = -l2[0] * l1[2];  dw[5] = -(l2[0] * l1[1]);
@@ -548,20 +790,21 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
                 dc2 = dc2 / N;
                 // projected in place, component by component, as the
                 // reference does (src/register/bspline_mi.txx:107-109)
-                const float* P = G_.mproj;
+                const float* P = g.mproj;
                 dc0 = dc0 * P[0] + dc1 * P[1] + dc2 * P[2];
                 dc1 = dc0 * P[3] + dc1 * P[4] + dc2 * P[5];
                 dc2 = dc0 * P[6] + dc1 * P[7] + dc2 * P[8];
+                if (!in[u]) { dc0 = 0.f; dc1 = 0.f; dc2 = 0.f; }
             }
-            if (vv[u] < nvox) {
-                L.dv[3 * vv[u] + 0] = dc0;
-                L.dv[3 * vv[u] + 1] = dc1;
-                L.dv[3 * vv[u] + 2] = dc2;
-            }
+            // every slot below tile_slots is in LDS; slots past the tile
+            // hold zeros and are never read
+            L.dv[v] = dc0;
+            L.dv[L.dvp + v] = dc1;
+            L.dv[2 * L.dvp + v] = dc2;
         }
     }
     __syncthreads ();
-    condense_tile (L, G_, p, cond);
+    tile_condense (L, S, g, knot0, cond);
     if (KIND < 2) {
         block_sum (L, score, count);
         if (threadIdx.x == 0) {
@@ -622,10 +865,31 @@ void k_reduce (KGeom g, const float* __restrict__ cond, const double* __restrict
     long long n = 0;
     double sc = 0.0;
     if (MSE) {
-#pragma unroll 4
-        for (int t = threadIdx.x; t < g.num_tiles; t += REDUCE_THREADS) {
-            n += tile_count[t];
-            if (blockIdx.x == 0) sc += tile_score[t];
+        // the per-tile partials in chunks of 32 per thread, each chunk's
+        // loads all in flight before they are summed (in tile order per
+        // thread, so the result does not depend on timing)
+        constexpr int CH = 32;
+        for (int t0 = 0; t0 < g.num_tiles; t0 += CH * REDUCE_THREADS) {
+            int cnt[CH];
+#pragma unroll
+            for (int q = 0; q < CH; q++) {
+                const int t = t0 + q * REDUCE_THREADS + threadIdx.x;
+                cnt[q] = t < g.num_tiles ? tile_count[t] : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < CH; q++) n += cnt[q];
+        }
+        if (blockIdx.x == 0) {
+            for (int t0 = 0; t0 < g.num_tiles; t0 += CH * REDUCE_THREADS) {
+                double s[CH];
+#pragma unroll
+                for (int q = 0; q < CH; q++) {
+                    const int t = t0 + q * REDUCE_THREADS + threadIdx.x;
+                    s[q] = t < g.num_tiles ? tile_score[t] : 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < CH; q++) sc += s[q];
+            }
         }
     }
     double acc = 0.0;
@@ -973,13 +1237,13 @@ k_to_bf16 (const float* __restrict__ src, uint16_t* __restrict__ dst, int64_t n)
 // launchers
 
 size_t
-tile_lds_bytes (const KGeom& g)
+tile_lds_bytes (const KGeom& g, int vox_per_thread)
 {
-    const size_t vx = g.vpr[0], vy = g.vpr[1], vz = g.vpr[2];
-    const size_t mx = vx > vz ? vx : vz;
-    size_t f = 4 * (vx + vy + vz) + 192 + 48 * mx + 12 * vy * mx + 3 * vx * vy * vz;
-    size_t bytes = ((f * 4 + 7) & ~(size_t) 7) + 16 * sizeof (double);
-    return bytes;
+    // the carve-up of the tile kernel
+    const int vx = g.vpr[0], vy = g.vpr[1], vz = g.vpr[2];
+    const size_t f = 4 * vx + 4 * vy + 8 * vz + 192 + tile_region1_floats (vx, vy, vz)
+        + 3 * (size_t) tile_slots (vx * vy * vz, vox_per_thread);
+    return f * sizeof (float) + 16 * sizeof (double);
 }
 
 // Records ev[i] on st when stage timing is on (ev != nullptr).
@@ -995,34 +1259,56 @@ mark (hipEvent_t* ev, int i, hipStream_t st)
         if (e_ != hipSuccess) return e_;                                \
     } while (0)
 
-template <int KIND, typename TI, int G>
+template <int KIND, typename TI, int V, int G>
 static void
-launch_tiles_g (const KGeom& g, size_t lds, hipStream_t st, const void* f, const void* m,
+launch_tiles_vg (const KGeom& g, size_t lds, hipStream_t st, const void* f, const void* m,
     const void* mg, const float* coeff, const float* blut, float* cond, double* tile_score,
     int* tile_count, const MiArgs& mi)
 {
-    if (g.aligned)
-        hipLaunchKernelGGL ((k_tiles<KIND, TI, G, true>), dim3 (8 * g.tiles_per_xcd),
-            dim3 (TILE_THREADS), lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff,
+    hipLaunchKernelGGL ((k_tiles<KIND, TI, V, G>), dim3 (8 * g.tiles_per_xcd), dim3 (TILE_THREADS),
+        lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff, blut, cond, tile_score,
+        tile_count, mi);
+}
+
+template <int KIND, typename TI, int V>
+static void
+launch_tiles_v (const KGeom& g, int vpt, size_t extra, hipStream_t st, const void* f,
+    const void* m, const void* mg, const float* coeff, const float* blut, float* cond,
+    double* tile_score, int* tile_count, const MiArgs& mi)
+{
+    if (vpt == 1)
+        launch_tiles_vg<KIND, TI, V, 1> (g, tile_lds_bytes (g, 1) + extra, st, f, m, mg, coeff,
+            blut, cond, tile_score, tile_count, mi);
+    else if (vpt == 2)
+        launch_tiles_vg<KIND, TI, V, 2> (g, tile_lds_bytes (g, 2) + extra, st, f, m, mg, coeff,
             blut, cond, tile_score, tile_count, mi);
     else
-        hipLaunchKernelGGL ((k_tiles<KIND, TI, G, false>), dim3 (8 * g.tiles_per_xcd),
-            dim3 (TILE_THREADS), lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff,
+        launch_tiles_vg<KIND, TI, V, 4> (g, tile_lds_bytes (g, 4) + extra, st, f, m, mg, coeff,
             blut, cond, tile_score, tile_count, mi);
 }
 
+// The tile kernel is specialized at compile time for the isotropic tile
+// edges 8, 10, 12 and 15 voxels; other vox_per_rgn run the generic build.
+// vox_per_thread falls back to the largest value whose dc/dv slots fit in
+// LDS (1 always fits: the context checked it at creation).
 template <int KIND, typename TI>
 static void
-launch_tiles (const KGeom& g, int vpt, size_t lds, hipStream_t st, const void* f, const void* m,
-    const void* mg, const float* coeff, const float* blut, float* cond, double* tile_score,
-    int* tile_count, const MiArgs& mi)
+launch_tiles (const KGeom& g, const Tune& tune, size_t extra, hipStream_t st, const void* f,
+    const void* m, const void* mg, const float* coeff, const float* blut, float* cond,
+    double* tile_score, int* tile_count, const MiArgs& mi)
 {
-    if (vpt == 1)
-        launch_tiles_g<KIND, TI, 1> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
-    else if (vpt == 2)
-        launch_tiles_g<KIND, TI, 2> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
-    else
-        launch_tiles_g<KIND, TI, 4> (g, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+    int vpt = tune.vox_per_thread;
+    while (vpt > 1 && tile_lds_bytes (g, vpt) + extra > tune.lds_limit) vpt >>= 1;
+    const int v = g.vpr[0];
+    const bool iso = g.vpr[1] == v && g.vpr[2] == v;
+#define PLM_TILES(VV) launch_tiles_v<KIND, TI, VV> (g, vpt, extra, st, f, m, mg, coeff, blut, \
+        cond, tile_score, tile_count, mi)
+    if (iso && v == 8) PLM_TILES (8);
+    else if (iso && v == 10) PLM_TILES (10);
+    else if (iso && v == 12) PLM_TILES (12);
+    else if (iso && v == 15) PLM_TILES (15);
+    else PLM_TILES (0);
+#undef PLM_TILES
 }
 
 static int
@@ -1037,17 +1323,15 @@ launch_mse (const KGeom& g, const Tune& tune, int mode, int storage, const void*
     double* tile_score, int* tile_count, double* score, int64_t* nvox, double* grad,
     hipStream_t st, hipEvent_t* ev)
 {
-    const size_t lds = tile_lds_bytes (g);
     MiArgs mi {};
     hipError_t e = mark (ev, 0, st);
     if (e != hipSuccess) return e;
-    const int v = tune.vox_per_thread;
     if (storage == 0) {
-        if (mode == 0) launch_tiles<0, float> (g, v, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
-        else launch_tiles<1, float> (g, v, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        if (mode == 0) launch_tiles<0, float> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        else launch_tiles<1, float> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
     } else {
-        if (mode == 0) launch_tiles<0, uint16_t> (g, v, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
-        else launch_tiles<1, uint16_t> (g, v, lds, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        if (mode == 0) launch_tiles<0, uint16_t> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        else launch_tiles<1, uint16_t> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
     }
     LAUNCH_CHECK ();
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
@@ -1072,7 +1356,7 @@ launch_mi (const KGeom& g, const Tune& tune, const HGeom& h, int storage, int hi
     LAUNCH_CHECK ();
     const size_t lds1 = ((4 * (g.vpr[0] + g.vpr[1] + g.vpr[2]) + 192) * 4 + 8)
         + sizeof (unsigned long long) * nbins;
-    const size_t lds2 = tile_lds_bytes (g) + sizeof (float) * h.fbins * h.mbins;
+    const size_t table_bytes = sizeof (float) * h.fbins * h.mbins;
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
     if (g.aligned)
         hipLaunchKernelGGL ((k_mi_hist<true>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
@@ -1088,7 +1372,7 @@ launch_mi (const KGeom& g, const Tune& tune, const HGeom& h, int storage, int hi
     if ((e = mark (ev, 3, st)) != hipSuccess) return e;
     MiArgs mi {h, table, nvox_f, fbin, mbin};
     // pass 2 reads only bin images; the image type parameter is unused
-    launch_tiles<2, float> (g, tune.vox_per_thread, lds2, st, f, m, nullptr, coeff, blut, cond,
+    launch_tiles<2, float> (g, tune, table_bytes, st, f, m, nullptr, coeff, blut, cond,
         nullptr, nullptr, mi);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 4, st)) != hipSuccess) return e;
