@@ -112,7 +112,7 @@ struct plm_hip_ctx {
     bool mi_ok = false;
     int storage = PLM_STORE_F32;
     int hist_blocks = 512;
-    plmhip::Tune tune {2, 0};
+    plmhip::Tune tune {2, 0, 0};
     int64_t num_knots = 0;
     int64_t num_tiles = 0;
     size_t lds_limit = 0;
@@ -281,6 +281,8 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
         g.img_origin[a] = geom->img_origin[a];
         g.morigin[a] = moving->origin[a];
         g.inv_msp[a] = (float) (1.0 / moving->spacing[a]);
+        g.mhi[a] = (float) moving->dim[a] - 0.5f;
+        g.mmax[a] = (float) (moving->dim[a] - 2);
         tiles *= rd;
         knots *= rd + 3;
     }
@@ -389,6 +391,7 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     c->num_tiles = tiles;
     c->lds_limit = prop.sharedMemPerBlock;
     c->tune.lds_limit = c->lds_limit;
+    c->tune.num_cus = prop.multiProcessorCount;
     c->hist_blocks = (int) std::min<int64_t> (tiles, (int64_t) prop.multiProcessorCount * 2);
     if (plmhip::tile_lds_bytes (g, 1) + sizeof (float) * h.fbins * h.mbins > c->lds_limit) {
         free_ctx (c);
@@ -414,8 +417,10 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
 
     // knot-major condense slots; slots no tile owns stay zero for good
     CTX_TRY (alloc (&c->d_cond, (size_t) knots * 192, "hipMalloc(cond)"));
-    CTX_TRY (alloc (&c->d_tile_score, tiles, "hipMalloc(tile_score)"));
-    CTX_TRY (alloc (&c->d_tile_count, tiles, "hipMalloc(tile_count)"));
+    // per-workgroup partials of the persistent tile kernel: at most one
+    // workgroup per tile, rounded up to a multiple of 8
+    CTX_TRY (alloc (&c->d_tile_score, tiles + 8, "hipMalloc(tile_score)"));
+    CTX_TRY (alloc (&c->d_tile_count, tiles + 8, "hipMalloc(tile_count)"));
     const size_t nbins = (size_t) h.fbins * h.mbins + h.fbins;
     CTX_TRY (alloc (&c->d_hist, nbins, "hipMalloc(hist)"));
     CTX_TRY (alloc (&c->d_table, (size_t) h.fbins * h.mbins, "hipMalloc(table)"));

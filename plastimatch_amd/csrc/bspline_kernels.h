@@ -28,6 +28,8 @@ struct KGeom {
     float inv_msp[3];        // (float)(1.0 / moving spacing)
     float mstep[9];          // M = moving proj * fixed step (moving index per fixed index)
     float moff[3];           // moving proj * (img_origin - moving origin)
+    float mhi[3];            // mdim - 0.5: moving index coordinates inside the image are below it
+    float mmax[3];           // mdim - 2: the largest floor corner of the trilinear clamp
     int num_tiles;
     int num_knots;
     int tiles_per_xcd;
@@ -71,6 +73,7 @@ struct HGeom {
 struct Tune {
     int vox_per_thread;      // voxels each tile-kernel thread keeps in flight: 1, 2 or 4
     size_t lds_limit;        // LDS bytes a workgroup may use
+    int num_cus;             // compute units of the device (persistent grid size)
 };
 
 // LDS bytes of the tile kernel at vox_per_thread voxels per batch (the MI

@@ -30,6 +30,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <float.h>
+#include <algorithm>
+#include <mutex>
 
 #include "bspline_kernels.h"
 
@@ -357,16 +359,14 @@ carve (float* sm, int vx, int vy, int vz, int slots)
     return L;
 }
 
-// Stage 1: basis LUTs, the per-slice moving-index term Z and the tile's 64
-// knot coefficients.  Z[z][a] = M[a][2] * (bz0 + z), M = P_moving * step_fixed.
+// Basis LUTs, once per workgroup: bx, by, and bz into the first half of
+// each bzz row.
 template <int V>
 __device__ __forceinline__ void
-tile_load (const TileLds& L, const Shape<V>& S, const KGeom& g, int knot0, int bz0,
-    const float* __restrict__ coeff, const float* __restrict__ blut)
+tile_luts (const TileLds& L, const Shape<V>& S, const float* __restrict__ blut)
 {
-    const int t = threadIdx.x;
     const int nxy = 4 * (S.vx + S.vy);
-    for (int e = t; e < nxy + 4 * S.vz; e += TILE_THREADS) {
+    for (int e = threadIdx.x; e < nxy + 4 * S.vz; e += TILE_THREADS) {
         const float b = blut[e];
         if (e < nxy) {
             L.bx[e] = b;                  // bx and by are contiguous in both
@@ -375,18 +375,55 @@ tile_load (const TileLds& L, const Shape<V>& S, const KGeom& g, int knot0, int b
             L.bzz[2 * q - (q & 3)] = b;
         }
     }
-    for (int e = t; e < 3 * S.vz; e += TILE_THREADS) {
-        const int z = e / 3, a = e - 3 * z;
-        L.bzz[8 * z + 4 + a] = g.mstep[3 * a + 2] * (float) (bz0 + z);
-    }
+}
+
+// Tile position and the knot coordinates of its first knot.
+struct TilePos {
+    int bx0, by0, bz0;      // first fixed voxel of the tile
+    int knot0;              // knot index of its (0, 0, 0) control point
+};
+
+__device__ __forceinline__ TilePos
+tile_pos (const KGeom& g, int tile)
+{
+    int p[3];
+    tile_coords (g, tile, p);
+    TilePos q;
+    q.bx0 = g.roi_off[0] + g.vpr[0] * p[0];
+    q.by0 = g.roi_off[1] + g.vpr[1] * p[1];
+    q.bz0 = g.roi_off[2] + g.vpr[2] * p[2];
+    q.knot0 = p[0] + g.cdims[0] * (p[1] + g.cdims[1] * p[2]);
+    return q;
+}
+
+// The 3 coefficients of knot slot t < 64 of a tile, t = (k*4 + j)*4 + i,
+// the slot order of coef.
+__device__ __forceinline__ void
+tile_coef_fetch (float c[3], const KGeom& g, int knot0, int t, const float* __restrict__ coeff)
+{
+    const int cd0 = g.cdims[0], cd01 = g.cdims[0] * g.cdims[1];
+    const int knot = knot0 + (t & 3) + cd0 * ((t >> 2) & 3) + cd01 * (t >> 4);
+    c[0] = coeff[3 * knot + 0];
+    c[1] = coeff[3 * knot + 1];
+    c[2] = coeff[3 * knot + 2];
+}
+
+// Stage 1 of a tile: its 64 knot coefficients (fetched ahead into
+// registers) and the per-slice moving-index term
+// Z[z][a] = M[a][2] * (bz0 + z), M = P_moving * step_fixed.
+template <int V>
+__device__ __forceinline__ void
+tile_stage1 (const TileLds& L, const Shape<V>& S, const KGeom& g, int bz0, const float c[3],
+    int t)
+{
     if (t < 64) {
-        // t = (k*4 + j)*4 + i, the slot order of coef
-        const int cd0 = g.cdims[0], cd01 = g.cdims[0] * g.cdims[1];
-        const int knot = knot0 + (t & 3) + cd0 * ((t >> 2) & 3) + cd01 * (t >> 4);
-        const float* c = &coeff[3 * knot];
         L.coef[3 * t + 0] = c[0];
         L.coef[3 * t + 1] = c[1];
         L.coef[3 * t + 2] = c[2];
+    }
+    for (int e = t; e < 3 * S.vz; e += TILE_THREADS) {
+        const int z = e / 3, a = e - 3 * z;
+        L.bzz[8 * z + 4 + a] = g.mstep[3 * a + 2] * (float) (bz0 + z);
     }
 }
 
@@ -394,9 +431,9 @@ tile_load (const TileLds& L, const Shape<V>& S, const KGeom& g, int knot0, int b
 // per thread.  A coefficient row (k, j) is 12 floats: c fastest, then i.
 template <int V>
 __device__ __forceinline__ void
-tile_t1 (const TileLds& L, const Shape<V>& S)
+tile_t1 (const TileLds& L, const Shape<V>& S, int t)
 {
-    for (int e = threadIdx.x; e < 16 * S.vx; e += TILE_THREADS) {
+    for (int e = t; e < 16 * S.vx; e += TILE_THREADS) {
         const int kj = S.div_x (e), x = e - kj * S.vx;
         const float4* q = reinterpret_cast<const float4*> (&L.coef[12 * kj]);
         const float4 q0 = q[0], q1 = q[1], q2 = q[2];
@@ -421,11 +458,11 @@ tile_t1 (const TileLds& L, const Shape<V>& S)
 // with the position of POSITION_FROM_COORDS, src/base/volume_macros.h:82-96).
 template <int V>
 __device__ __forceinline__ void
-tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int by0)
+tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int by0, int t)
 {
     const int vx = S.vx;
     const float4* t1 = reinterpret_cast<const float4*> (L.t1);
-    for (int e = threadIdx.x; e < vx * S.vy; e += TILE_THREADS) {
+    for (int e = t; e < vx * S.vy; e += TILE_THREADS) {
         const int y = S.div_x (e), x = e - y * vx;
         const float4 b = reinterpret_cast<const float4*> (L.by)[y];
         float T[3][4];
@@ -469,10 +506,9 @@ tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int 
 template <int V>
 __device__ __forceinline__ void
 tile_condense (const TileLds& L, const Shape<V>& S, const KGeom& g, int knot0,
-    float* __restrict__ cond)
+    float* __restrict__ cond, int t)
 {
     const int vx = S.vx, vy = S.vy, vz = S.vz;
-    const int t = threadIdx.x;
     // S1[z][y][i][c] = sum_x bx[x][i] dv[z][y][x][c], one (z, y) row per thread
     for (int e = t; e < vy * vz; e += TILE_THREADS) {
         const float* d0 = L.dv + e * vx;
@@ -488,7 +524,7 @@ tile_condense (const TileLds& L, const Shape<V>& S, const KGeom& g, int knot0,
             acc[2][0] = b.z * v0; acc[2][1] = b.z * v1; acc[2][2] = b.z * v2;
             acc[3][0] = b.w * v0; acc[3][1] = b.w * v1; acc[3][2] = b.w * v2;
         }
-#pragma unroll
+#pragma unroll 3
         for (int x = 1; x < vx; x++) {
             const float v0 = d0[x], v1 = d1[x], v2 = d2[x];
             const float4 b = bx[x];
@@ -609,41 +645,35 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
     double* __restrict__ tile_score, int* __restrict__ tile_count, MiArgs mi)
 {
     extern __shared__ float4 sm4[];
-    const int tile = xcd_tile (g, blockIdx.x);
-    if (tile >= g.num_tiles) return;
+    // Persistent workgroups: those on XCD x = blockIdx % 8 walk that XCD's
+    // contiguous run of tiles together, so neighbouring tiles (which share
+    // image lines and knots) meet in the same L2.
+    const int wpx = gridDim.x >> 3;
+    const int tbeg = (blockIdx.x & 7) * g.tiles_per_xcd;
+    const int tend = min (tbeg + g.tiles_per_xcd, g.num_tiles);
+    int tile = tbeg + (blockIdx.x >> 3);
+    if (tile >= tend) {
+        // a workgroup without tiles still owns its (empty) partial
+        if (KIND < 2 && threadIdx.x == 0) {
+            tile_score[blockIdx.x] = 0.0;
+            tile_count[blockIdx.x] = 0;
+        }
+        return;
+    }
+
     const Shape<V> S (g);
     const int vx = S.vx, vy = S.vy, vz = S.vz;
     const int nvox = vx * vy * vz;
     const int slots = tile_slots (nvox, G);
     const TileLds L = carve (reinterpret_cast<float*> (sm4), vx, vy, vz, slots);
     const int t = threadIdx.x;
+    tile_luts (L, S, blut);
     if (KIND == 2) {
         const int nj = mi.h.fbins * mi.h.mbins;
         for (int i = t; i < nj; i += TILE_THREADS) L.tab[i] = mi.table[i];
     }
-    int p[3];
-    tile_coords (g, tile, p);
-    const int bx0 = g.roi_off[0] + vx * p[0];
-    const int by0 = g.roi_off[1] + vy * p[1];
-    const int bz0 = g.roi_off[2] + vz * p[2];
-    const int knot0 = p[0] + g.cdims[0] * (p[1] + g.cdims[1] * p[2]);
 
-    tile_load (L, S, g, knot0, bz0, coeff, blut);
-    __syncthreads ();
-    tile_t1 (L, S);
-    __syncthreads ();
-    tile_columns (L, S, g, bx0, by0);
-    __syncthreads ();
-
-    // Stage 4: the voxels.  Thread t takes voxels t, t + 256, ... of the
-    // tile (x fastest), G at a time, issuing all their loads before using
-    // any.  Voxels outside the ROI or the tile still run (their fixed load
-    // may fall past the image and read 0) and are masked at the end.
-    const int ex = min (vx, g.roi_end[0] - bx0);
-    const int ey = min (vy, g.roi_end[1] - by0);
-    const int ez = min (vz, g.roi_end[2] - bz0);
     const int fd0 = g.fdim[0], fd01 = g.fdim[0] * g.fdim[1];
-    const int fbase = bx0 + fd0 * by0 + fd01 * bz0;
     const int md0 = g.mdim[0], md01 = g.mdim[0] * g.mdim[1];
     constexpr int ES = KIND == 2 ? 1 : Img<TI>::ES;   // bytes per voxel of what is loaded
     const Rsrc fr = make_rsrc (KIND == 2 ? (const void*) mi.fbin : (const void*) fimg,
@@ -652,19 +682,80 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
         (uint32_t) (md01 * g.mdim[2]) * ES);
     const Rsrc gr = make_rsrc (mgrad, KIND == 0 ? (uint32_t) (md01 * g.mdim[2]) * 3 * ES : 0u);
     const int SX = md0 * ES, SXY = md01 * ES;
-    float mhi[3], mmax[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        mhi[a] = (float) g.mdim[a] - 0.5f;
-        mmax[a] = (float) (g.mdim[a] - 2);
-    }
+    const float* mhi = g.mhi;
+    const float* mmax = g.mmax;
     float N = 1.f;
     if (KIND == 2) N = *mi.nvox_f;
 
+    // This thread's first-batch voxel slots t + 256u (x fastest), the same
+    // in every tile, and their offsets from the tile's first fixed voxel.
+    // Packed as x | y << 10 | z << 20 (each below 1024).
+    int sxyz[G], sfo[G];
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+        const int v = u * TILE_THREADS + t;
+        const int z = S.div_xy (v);
+        const int r = v - z * (vx * vy);
+        const int y = S.div_x (r);
+        const int x = r - y * vx;
+        sxyz[u] = x | (y << 10) | (z << 20);
+        sfo[u] = x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
+    }
+    // Fetched one tile ahead: the knot coefficients (threads < 64) and the
+    // first batch's fixed voxels (or fixed bins).  Slots past the image read 0.
+    float cpre[3] = {0.f, 0.f, 0.f};
+    float fpre[G];
+    int fbpre[G];
+    TilePos nq = tile_pos (g, tile);
+    if (t < 64) tile_coef_fetch (cpre, g, nq.knot0, t, coeff);
+    {
+        const int fb = nq.bx0 + fd0 * nq.by0 + fd01 * nq.bz0;
+#pragma unroll
+        for (int u = 0; u < G; u++) {
+            if (KIND < 2) fpre[u] = Img<TI>::one (fr, (fb + sfo[u]) * ES, 0);
+            else fbpre[u] = load_u8 (fr, fb + sfo[u], 0);
+        }
+    }
+
+    // this thread's squared differences and overlap count over all the
+    // tiles of the workgroup, summed over the workgroup once at the end
     double score = 0.0;
     int count = 0;
+    for (; tile < tend; tile += wpx) {
+    const TilePos P = nq;
+    const int next = tile + wpx;
+    const bool has_next = next < tend;
+    if (has_next) nq = tile_pos (g, next);
+    // The thread index is re-read opaquely each tile, so the compiler
+    // recomputes the per-stage LDS addresses from it instead of keeping
+    // them all live across the tile loop (which costs occupancy).
+    int tt = t;
+    asm volatile ("" : "+v" (tt));
+#pragma unroll
+    for (int u = 0; u < G; u++) asm volatile ("" : "+v" (sxyz[u]), "+v" (sfo[u]));
+    __syncthreads ();       // the previous tile is done with coef, t1, s1, s2, red
+    tile_stage1 (L, S, g, P.bz0, cpre, tt);
+    __syncthreads ();
+    if (tt < 64 && has_next) tile_coef_fetch (cpre, g, nq.knot0, tt, coeff);
+    tile_t1 (L, S, tt);
+    __syncthreads ();
+    tile_columns (L, S, g, P.bx0, P.by0, tt);
+    __syncthreads ();
+
+    // Stage 4: the voxels.  Thread t takes voxels t, t + 256, ... of the
+    // tile, G at a time, issuing all their loads before using any.  Voxels
+    // outside the ROI or the tile still run (their fixed load may fall past
+    // the image and read 0) and are masked at the end; z < ez also rules
+    // out the slots past the tile.
+    const int ex = min (vx, g.roi_end[0] - P.bx0);
+    const int ey = min (vy, g.roi_end[1] - P.by0);
+    const int ez = min (vz, g.roi_end[2] - P.bz0);
+    const int fbase = P.bx0 + fd0 * P.by0 + fd01 * P.bz0;
+    const int fbase_next = nq.bx0 + fd0 * nq.by0 + fd01 * nq.bz0;
+
 #pragma unroll
     for (int base = 0; base < nvox; base += TILE_THREADS * G) {
+        const bool first = base == 0;
         bool in[G];
         float fr_[G][3];
         float fval[G];
@@ -674,14 +765,26 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
         int bn[G][8];
 #pragma unroll
         for (int u = 0; u < G; u++) {
-            const int v = base + u * TILE_THREADS + t;
-            const int z = S.div_xy (v), r = v - z * (vx * vy);
-            const int y = S.div_x (r), x = r - y * vx;
-            const bool act = v < nvox && x < ex && y < ey && z < ez;
+            int x, y, z;
+            if (first) {
+                x = sxyz[u] & 1023;
+                y = (sxyz[u] >> 10) & 1023;
+                z = sxyz[u] >> 20;
+                if (KIND < 2) fval[u] = fpre[u];
+                else fbv[u] = fbpre[u];
+            } else {
+                const int v = base + u * TILE_THREADS + tt;
+                z = S.div_xy (v);
+                const int r = v - z * (vx * vy);
+                y = S.div_x (r);
+                x = r - y * vx;
+                const int fofs = fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
+                if (KIND < 2) fval[u] = Img<TI>::one (fr, fofs * ES, 0);
+                else fbv[u] = load_u8 (fr, fofs, 0);
+            }
+            const bool act = x < ex && y < ey && z < ez;
+            const int r = x + (int) __umul24 (vx, y);
             const int zc = min (z, vz - 1);
-            const int fofs = fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
-            if (KIND < 2) fval[u] = Img<TI>::one (fr, fofs * ES, 0);
-            else fbv[u] = load_u8 (fr, fofs, 0);
 
             const float4* cp = reinterpret_cast<const float4*> (L.col) + r;
             const float4* zp = reinterpret_cast<const float4*> (&L.bzz[8 * zc]);
@@ -739,9 +842,10 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
                 Img<TI>::vec3 (mg[u], gr, goff, 0);
             }
         }
+        float bsum = 0.f;       // this batch's squared differences (<= G terms)
 #pragma unroll
         for (int u = 0; u < G; u++) {
-            const int v = base + u * TILE_THREADS + t;
+            const int v = base + u * TILE_THREADS + tt;
             float dc0, dc1, dc2;
             if (KIND < 2) {
                 Sample s;
@@ -749,7 +853,7 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
                 float md[3];
                 const float m_val = trilinear<KIND == 1> (s, a[u], g.inv_msp, md);
                 const float diff = in[u] ? m_val - fval[u] : 0.f;
-                score += (double) (diff * diff);
+                bsum += diff * diff;
                 count += in[u] ? 1 : 0;
                 if (KIND == 0) {
                     dc0 = diff * mg[u][0];
@@ -802,14 +906,28 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
             L.dv[L.dvp + v] = dc1;
             L.dv[2 * L.dvp + v] = dc2;
         }
+        score += (double) bsum;
+        if (first && has_next) {
+            // the next tile's first-batch fixed voxels, now that this
+            // batch's are consumed
+#pragma unroll
+            for (int u = 0; u < G; u++) {
+                if (KIND < 2) fpre[u] = Img<TI>::one (fr, (fbase_next + sfo[u]) * ES, 0);
+                else fbpre[u] = load_u8 (fr, fbase_next + sfo[u], 0);
+            }
+        }
     }
     __syncthreads ();
-    tile_condense (L, S, g, knot0, cond);
+    tile_condense (L, S, g, P.knot0, cond, tt);
+    }
+    // one partial per workgroup: each thread's tiles in their fixed order,
+    // then the threads in a fixed order
     if (KIND < 2) {
+        __syncthreads ();
         block_sum (L, score, count);
         if (threadIdx.x == 0) {
-            tile_score[tile] = score;
-            tile_count[tile] = count;
+            tile_score[blockIdx.x] = score;
+            tile_count[blockIdx.x] = count;
         }
     }
 }
@@ -845,7 +963,7 @@ wave_sum_ll (long long v)
 template <bool MSE>
 __global__ __launch_bounds__ (REDUCE_THREADS)
 void k_reduce (KGeom g, const float* __restrict__ cond, const double* __restrict__ tile_score,
-    const int* __restrict__ tile_count, double* __restrict__ grad,
+    const int* __restrict__ tile_count, int nparts, double* __restrict__ grad,
     double* __restrict__ out_score, int64_t* __restrict__ out_nvox)
 {
     __shared__ long long s_n[REDUCE_THREADS / 64];
@@ -865,30 +983,24 @@ void k_reduce (KGeom g, const float* __restrict__ cond, const double* __restrict
     long long n = 0;
     double sc = 0.0;
     if (MSE) {
-        // the per-tile partials in chunks of 32 per thread, each chunk's
-        // loads all in flight before they are summed (in tile order per
-        // thread, so the result does not depend on timing)
-        constexpr int CH = 32;
-        for (int t0 = 0; t0 < g.num_tiles; t0 += CH * REDUCE_THREADS) {
+        // the per-workgroup partials of the tile kernel, CH per thread with
+        // all loads in flight together (one round for up to 2048
+        // partials), summed in a fixed order
+        constexpr int CH = 8;
+        const bool want_score = blockIdx.x == 0;
+        for (int t0 = 0; t0 < nparts; t0 += CH * REDUCE_THREADS) {
             int cnt[CH];
+            double s[CH];
 #pragma unroll
             for (int q = 0; q < CH; q++) {
                 const int t = t0 + q * REDUCE_THREADS + threadIdx.x;
-                cnt[q] = t < g.num_tiles ? tile_count[t] : 0;
+                cnt[q] = t < nparts ? tile_count[t] : 0;
+                s[q] = want_score && t < nparts ? tile_score[t] : 0.0;
             }
 #pragma unroll
-            for (int q = 0; q < CH; q++) n += cnt[q];
-        }
-        if (blockIdx.x == 0) {
-            for (int t0 = 0; t0 < g.num_tiles; t0 += CH * REDUCE_THREADS) {
-                double s[CH];
-#pragma unroll
-                for (int q = 0; q < CH; q++) {
-                    const int t = t0 + q * REDUCE_THREADS + threadIdx.x;
-                    s[q] = t < g.num_tiles ? tile_score[t] : 0.0;
-                }
-#pragma unroll
-                for (int q = 0; q < CH; q++) sc += s[q];
+            for (int q = 0; q < CH; q++) {
+                n += cnt[q];
+                sc += s[q];
             }
         }
     }
@@ -1259,40 +1371,63 @@ mark (hipEvent_t* ev, int i, hipStream_t st)
         if (e_ != hipSuccess) return e_;                                \
     } while (0)
 
+// Persistent grid: as many workgroups as fit on the device at once
+// (occupancy from the kernel's registers and this launch's LDS), a
+// multiple of 8 so every XCD gets the same count, and no more than an XCD
+// has tiles.
 template <int KIND, typename TI, int V, int G>
-static void
-launch_tiles_vg (const KGeom& g, size_t lds, hipStream_t st, const void* f, const void* m,
-    const void* mg, const float* coeff, const float* blut, float* cond, double* tile_score,
-    int* tile_count, const MiArgs& mi)
-{
-    hipLaunchKernelGGL ((k_tiles<KIND, TI, V, G>), dim3 (8 * g.tiles_per_xcd), dim3 (TILE_THREADS),
-        lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff, blut, cond, tile_score,
-        tile_count, mi);
-}
-
-template <int KIND, typename TI, int V>
-static void
-launch_tiles_v (const KGeom& g, int vpt, size_t extra, hipStream_t st, const void* f,
+static int
+launch_tiles_vg (const KGeom& g, const Tune& tune, size_t lds, hipStream_t st, const void* f,
     const void* m, const void* mg, const float* coeff, const float* blut, float* cond,
     double* tile_score, int* tile_count, const MiArgs& mi)
 {
+    static std::mutex mu;
+    static size_t cached_lds = 0;
+    static int cached_occ = 0;
+    int occ;
+    {
+        std::lock_guard<std::mutex> lock (mu);
+        if (cached_lds != lds) {
+            int n = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor (&n, k_tiles<KIND, TI, V, G>,
+                    TILE_THREADS, lds) != hipSuccess || n < 1)
+                n = 1;
+            cached_lds = lds;
+            cached_occ = n;
+        }
+        occ = cached_occ;
+    }
+    const int cus = tune.num_cus > 0 ? tune.num_cus : 8;
+    const int per_xcd = std::max (1, std::min (g.tiles_per_xcd, occ * cus / 8));
+    hipLaunchKernelGGL ((k_tiles<KIND, TI, V, G>), dim3 (8 * per_xcd), dim3 (TILE_THREADS),
+        lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff, blut, cond, tile_score,
+        tile_count, mi);
+    return 8 * per_xcd;
+}
+
+template <int KIND, typename TI, int V>
+static int
+launch_tiles_v (const KGeom& g, const Tune& tune, int vpt, size_t extra, hipStream_t st,
+    const void* f, const void* m, const void* mg, const float* coeff, const float* blut,
+    float* cond, double* tile_score, int* tile_count, const MiArgs& mi)
+{
     if (vpt == 1)
-        launch_tiles_vg<KIND, TI, V, 1> (g, tile_lds_bytes (g, 1) + extra, st, f, m, mg, coeff,
-            blut, cond, tile_score, tile_count, mi);
-    else if (vpt == 2)
-        launch_tiles_vg<KIND, TI, V, 2> (g, tile_lds_bytes (g, 2) + extra, st, f, m, mg, coeff,
-            blut, cond, tile_score, tile_count, mi);
-    else
-        launch_tiles_vg<KIND, TI, V, 4> (g, tile_lds_bytes (g, 4) + extra, st, f, m, mg, coeff,
-            blut, cond, tile_score, tile_count, mi);
+        return launch_tiles_vg<KIND, TI, V, 1> (g, tune, tile_lds_bytes (g, 1) + extra, st, f, m,
+            mg, coeff, blut, cond, tile_score, tile_count, mi);
+    if (vpt == 2)
+        return launch_tiles_vg<KIND, TI, V, 2> (g, tune, tile_lds_bytes (g, 2) + extra, st, f, m,
+            mg, coeff, blut, cond, tile_score, tile_count, mi);
+    return launch_tiles_vg<KIND, TI, V, 4> (g, tune, tile_lds_bytes (g, 4) + extra, st, f, m, mg,
+        coeff, blut, cond, tile_score, tile_count, mi);
 }
 
 // The tile kernel is specialized at compile time for the isotropic tile
 // edges 8, 10, 12 and 15 voxels; other vox_per_rgn run the generic build.
 // vox_per_thread falls back to the largest value whose dc/dv slots fit in
-// LDS (1 always fits: the context checked it at creation).
+// LDS (1 always fits: the context checked it at creation).  Returns the
+// grid size, the number of per-workgroup score/count partials.
 template <int KIND, typename TI>
-static void
+static int
 launch_tiles (const KGeom& g, const Tune& tune, size_t extra, hipStream_t st, const void* f,
     const void* m, const void* mg, const float* coeff, const float* blut, float* cond,
     double* tile_score, int* tile_count, const MiArgs& mi)
@@ -1301,13 +1436,13 @@ launch_tiles (const KGeom& g, const Tune& tune, size_t extra, hipStream_t st, co
     while (vpt > 1 && tile_lds_bytes (g, vpt) + extra > tune.lds_limit) vpt >>= 1;
     const int v = g.vpr[0];
     const bool iso = g.vpr[1] == v && g.vpr[2] == v;
-#define PLM_TILES(VV) launch_tiles_v<KIND, TI, VV> (g, vpt, extra, st, f, m, mg, coeff, blut, \
-        cond, tile_score, tile_count, mi)
+#define PLM_TILES(VV) return launch_tiles_v<KIND, TI, VV> (g, tune, vpt, extra, st, f, m, mg, \
+        coeff, blut, cond, tile_score, tile_count, mi)
     if (iso && v == 8) PLM_TILES (8);
-    else if (iso && v == 10) PLM_TILES (10);
-    else if (iso && v == 12) PLM_TILES (12);
-    else if (iso && v == 15) PLM_TILES (15);
-    else PLM_TILES (0);
+    if (iso && v == 10) PLM_TILES (10);
+    if (iso && v == 12) PLM_TILES (12);
+    if (iso && v == 15) PLM_TILES (15);
+    PLM_TILES (0);
 #undef PLM_TILES
 }
 
@@ -1326,17 +1461,18 @@ launch_mse (const KGeom& g, const Tune& tune, int mode, int storage, const void*
     MiArgs mi {};
     hipError_t e = mark (ev, 0, st);
     if (e != hipSuccess) return e;
+    int nparts;
     if (storage == 0) {
-        if (mode == 0) launch_tiles<0, float> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
-        else launch_tiles<1, float> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        if (mode == 0) nparts = launch_tiles<0, float> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        else nparts = launch_tiles<1, float> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
     } else {
-        if (mode == 0) launch_tiles<0, uint16_t> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
-        else launch_tiles<1, uint16_t> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        if (mode == 0) nparts = launch_tiles<0, uint16_t> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        else nparts = launch_tiles<1, uint16_t> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
     }
     LAUNCH_CHECK ();
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_reduce<true>, dim3 (reduce_blocks (g)), dim3 (REDUCE_THREADS), 0, st, g,
-        cond, tile_score, tile_count, grad, score, nvox);
+        cond, tile_score, tile_count, nparts, grad, score, nvox);
     LAUNCH_CHECK ();
     return mark (ev, 2, st);
 }
@@ -1377,7 +1513,7 @@ launch_mi (const KGeom& g, const Tune& tune, const HGeom& h, int storage, int hi
     LAUNCH_CHECK ();
     if ((e = mark (ev, 4, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_reduce<false>, dim3 (reduce_blocks (g)), dim3 (REDUCE_THREADS), 0, st, g,
-        cond, nullptr, nullptr, grad, nullptr, nullptr);
+        cond, nullptr, nullptr, 0, grad, nullptr, nullptr);
     LAUNCH_CHECK ();
     return mark (ev, 5, st);
 }
