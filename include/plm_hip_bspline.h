@@ -197,6 +197,57 @@ int plm_hip_bspline_autotune (plm_hip_ctx* ctx, int metric, int grad_mode,
 int plm_hip_bspline_num_knots (const plm_hip_ctx* ctx, int64_t* num_knots);
 int plm_hip_bspline_num_tiles (const plm_hip_ctx* ctx, int64_t* num_tiles);
 
+/* ------------------------------------------------------------------------
+ * Batched MSE evaluation of many stages at once (the multi-atlas case).
+ *
+ * A batch strings together up to 64 contexts (same device, same storage,
+ * equal vox_per_rgn) and evaluates all of them with one persistent
+ * tile-kernel launch over the flattened queue of their tiles, plus one
+ * reduce launch; the evaluation can be captured once in a hipGraph and
+ * replayed every optimizer iteration.  The reference has no batched form:
+ * its flavors score one pair per call (bspline_score,
+ * src/register/bspline.cxx:583-601), so these entry points have no
+ * counterpart to replace; per pair the results are those of
+ * plm_hip_bspline_score_device (the same gradient bits, the score to
+ * summation order).
+ *
+ * The batch borrows each context's images and condense buffer: the
+ * contexts must outlive the batch, and a context must not be evaluated on
+ * its own while a batched evaluation that includes it is in flight. */
+typedef struct plm_hip_batch plm_hip_batch;
+
+int plm_hip_batch_create (plm_hip_ctx* const* ctxs, int n, plm_hip_batch** out);
+
+/* Device buffers of the evaluation: d_coeff[i] and d_grad[i] are host
+ * arrays of n device pointers (pair i's 3*num_knots coefficients and FP64
+ * gradient), d_score and d_num_vox device arrays of n.  Kept in the batch's
+ * device table (this call synchronizes); a captured graph refers to them,
+ * so setting new buffers drops the graph. */
+int plm_hip_batch_set_buffers (plm_hip_batch* b, const float* const* d_coeff,
+    double* const* d_grad, double* d_score, int64_t* d_num_vox);
+
+/* One batched evaluation (grad_mode as plm_hip_bspline_score), enqueued
+ * on stream (NULL: the batch's own stream). */
+int plm_hip_batch_score_device (plm_hip_batch* b, int grad_mode, void* stream);
+
+/* Capture one evaluation into a hipGraph (on the batch's stream) and
+ * instantiate it; plm_hip_batch_launch replays it on stream.  Changing the
+ * buffers or the tuning drops the graph (launch then returns
+ * PLM_HIP_ERR_ARG until the next capture). */
+int plm_hip_batch_capture (plm_hip_batch* b, int grad_mode);
+int plm_hip_batch_launch (plm_hip_batch* b, void* stream);
+
+/* Tuning and stage timing of the batched tile kernel, as
+ * plm_hip_bspline_set_tuning / _autotune / _set_timing / _get_timing. */
+int plm_hip_batch_set_tuning (plm_hip_batch* b, int vox_per_thread);
+int plm_hip_batch_autotune (plm_hip_batch* b, int grad_mode, int reps,
+    int* chosen_vox_per_thread, double* eval_ms);
+int plm_hip_batch_set_timing (plm_hip_batch* b, int enable);
+int plm_hip_batch_get_timing (plm_hip_batch* b, double* stage_ms, int max_stages,
+    int64_t* evals);
+
+int plm_hip_batch_destroy (plm_hip_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,19 @@ SIGNATURES = {
                                                   ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_bspline_num_knots": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_bspline_num_tiles": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "plm_hip_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
+    "plm_hip_batch_set_buffers": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), _P, _P]),
+    "plm_hip_batch_score_device": (ctypes.c_int, [_P, ctypes.c_int, _P]),
+    "plm_hip_batch_capture": (ctypes.c_int, [_P, ctypes.c_int]),
+    "plm_hip_batch_launch": (ctypes.c_int, [_P, _P]),
+    "plm_hip_batch_set_tuning": (ctypes.c_int, [_P, ctypes.c_int]),
+    "plm_hip_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_double)]),
+    "plm_hip_batch_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "plm_hip_batch_get_timing": (ctypes.c_int, [_P, _P, ctypes.c_int,
+                                                ctypes.POINTER(ctypes.c_int64)]),
+    "plm_hip_batch_destroy": (ctypes.c_int, [_P]),
 }
 
 _lib = None
@@ -113,6 +126,15 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise FileNotFoundError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    # A ROCm torch wheel carries its own libamdhip64.so.7, which it loads
+    # under a name the dynamic loader does not match against an already
+    # loaded copy.  Loading torch first makes this library bind to torch's
+    # runtime (same soname), so the process holds one HIP runtime; the
+    # other order would leave torch with a second runtime that sees no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -267,3 +267,83 @@ def stage_for_pair(pair, gpuid=0, storage="f32", with_grad=True, bins=None, movi
         mg = moving_grad if moving_grad is not None else volume_gradient(moving, gpuid)
     hist = Joint_histogram(bins, bins).initialize(fixed, moving) if bins else None
     return BsplineHip(bxf, fixed, moving, mg, hist, gpuid, storage)
+
+
+class BsplineBatch:
+    """Many stages evaluated together: one persistent tile-kernel launch over
+    the flattened tiles of every pair plus one reduce launch, optionally
+    captured once in a hipGraph and replayed (the multi-atlas case; the
+    C ABI's plm_hip_batch_*).  The stages must share the device, the storage
+    type and vox_per_rgn, and must outlive the batch."""
+
+    def __init__(self, stages):
+        self.lib = _abi.load()
+        self.stages = list(stages)
+        arr = (ctypes.c_void_p * len(self.stages))(*[s.ctx.value for s in self.stages])
+        b = ctypes.c_void_p()
+        _abi.check(self.lib.plm_hip_batch_create(arr, len(self.stages), ctypes.byref(b)))
+        self.b = b
+        self._keep = None
+
+    def set_buffers(self, d_coeffs, d_grads, d_score, d_num_vox):
+        """Device tensors: per-pair coefficient and FP64 gradient vectors, and
+        score (float64) and voxel-count (int64) vectors of one entry per pair."""
+        n = len(self.stages)
+        if len(d_coeffs) != n or len(d_grads) != n or d_score.numel() != n or d_num_vox.numel() != n:
+            raise ValueError("one buffer per pair")
+        for s, c, g in zip(self.stages, d_coeffs, d_grads):
+            if c.numel() != s.bxf.num_coeff or g.numel() != s.bxf.num_coeff:
+                raise ValueError("buffer length differs from the pair's coefficient count")
+        ca = (ctypes.c_void_p * n)(*[c.data_ptr() for c in d_coeffs])
+        ga = (ctypes.c_void_p * n)(*[g.data_ptr() for g in d_grads])
+        _abi.check(self.lib.plm_hip_batch_set_buffers(
+            self.b, ca, ga, ctypes.c_void_p(d_score.data_ptr()), ctypes.c_void_p(d_num_vox.data_ptr())))
+        self._keep = (list(d_coeffs), list(d_grads), d_score, d_num_vox)
+
+    def score_device(self, grad_mode="ref_nn", stream=None):
+        st = ctypes.c_void_p(stream) if stream else None
+        return _abi.check(self.lib.plm_hip_batch_score_device(self.b, GRAD_MODES[grad_mode], st))
+
+    def capture(self, grad_mode="ref_nn"):
+        _abi.check(self.lib.plm_hip_batch_capture(self.b, GRAD_MODES[grad_mode]))
+
+    def launch(self, stream=None):
+        st = ctypes.c_void_p(stream) if stream else None
+        return _abi.check(self.lib.plm_hip_batch_launch(self.b, st))
+
+    def set_tuning(self, vox_per_thread):
+        _abi.check(self.lib.plm_hip_batch_set_tuning(self.b, int(vox_per_thread)))
+
+    def autotune(self, grad_mode="ref_nn", reps=5):
+        ch = ctypes.c_int()
+        ms = ctypes.c_double()
+        _abi.check(self.lib.plm_hip_batch_autotune(self.b, GRAD_MODES[grad_mode], reps,
+                                                    ctypes.byref(ch), ctypes.byref(ms)))
+        return ch.value, ms.value
+
+    def set_timing(self, enable=True):
+        _abi.check(self.lib.plm_hip_batch_set_timing(self.b, 1 if enable else 0))
+
+    def get_timing(self, max_stages=6):
+        ms = np.zeros(max_stages, np.float64)
+        n = ctypes.c_int64()
+        _abi.check(self.lib.plm_hip_batch_get_timing(self.b, ms.ctypes.data_as(ctypes.c_void_p),
+                                                      max_stages, ctypes.byref(n)))
+        return ms, n.value
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.plm_hip_batch_destroy(self.b)
+            self.b = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

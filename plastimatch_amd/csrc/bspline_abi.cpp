@@ -100,6 +100,66 @@ basis_lut (int vpr, float* lut)
     }
 }
 
+// Per-stage kernel timing with HIP events recorded on the launch stream:
+// one event set per evaluation while enabled, summed and recycled by
+// collect().
+struct StageTimer {
+    bool on = false;
+    int last_stages = 0;
+    std::vector<std::vector<hipEvent_t>> used;
+    std::vector<std::vector<hipEvent_t>> free_sets;
+
+    // The event set for the next evaluation, or nullptr when timing is off.
+    hipError_t next (hipEvent_t** ev)
+    {
+        *ev = nullptr;
+        if (!on) return hipSuccess;
+        if (free_sets.empty ()) {
+            std::vector<hipEvent_t> s (plmhip::MAX_STAGE_EVENTS);
+            for (auto& x : s) {
+                hipError_t e = hipEventCreate (&x);
+                if (e != hipSuccess) return e;
+            }
+            free_sets.push_back (s);
+        }
+        used.push_back (free_sets.back ());
+        free_sets.pop_back ();
+        *ev = used.back ().data ();
+        return hipSuccess;
+    }
+
+    // Waits for the recorded evaluations and sums their stage times.
+    hipError_t collect (double* stage_ms, int max_stages, int64_t* evals)
+    {
+        for (int i = 0; i < max_stages; i++) stage_ms[i] = 0.0;
+        int64_t n = 0;
+        for (auto& s : used) {
+            hipError_t e = hipEventSynchronize (s[last_stages]);
+            if (e != hipSuccess) return e;
+            for (int i = 0; i < last_stages && i < max_stages; i++) {
+                float ms = 0.f;
+                e = hipEventElapsedTime (&ms, s[i], s[i + 1]);
+                if (e != hipSuccess) return e;
+                stage_ms[i] += ms;
+            }
+            n++;
+            free_sets.push_back (s);
+        }
+        used.clear ();
+        *evals = n;
+        return hipSuccess;
+    }
+
+    void destroy ()
+    {
+        for (auto* sets : {&used, &free_sets})
+            for (auto& s : *sets)
+                for (hipEvent_t e : s) (void) hipEventDestroy (e);
+        used.clear ();
+        free_sets.clear ();
+    }
+};
+
 } // namespace
 
 struct plm_hip_ctx {
@@ -124,6 +184,8 @@ struct plm_hip_ctx {
     float* d_cond = nullptr;
     double* d_tile_score = nullptr;
     int* d_tile_count = nullptr;
+    plmhip::PairDev* d_pair = nullptr;      // this context as a one-pair table
+    plmhip::Plan plan {};
     unsigned long long* d_hist = nullptr;   // joint bins then fixed bins
     float* d_table = nullptr;
     float* d_nvox_f = nullptr;
@@ -136,30 +198,36 @@ struct plm_hip_ctx {
     int64_t* d_nvox = nullptr;
     int64_t* d_hist_out = nullptr;
 
-    // stage timing (plm_hip_bspline_set_timing): one event set per evaluation
-    bool timing = false;
-    int last_stages = 0;
-    std::vector<std::vector<hipEvent_t>> ev_used;
-    std::vector<std::vector<hipEvent_t>> ev_free;
+    StageTimer timer;       // plm_hip_bspline_set_timing
+};
+
+// A batch of contexts evaluated together (plm_hip_batch_*).
+struct plm_hip_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<plm_hip_ctx*> ctxs;
+    int storage = PLM_STORE_F32;
+    bool all_mgrad = true;
+    std::vector<plmhip::PairDev> h_pairs;   // host copy of the device table
+    plmhip::PairDev* d_pairs = nullptr;
+    plmhip::Plan plan {};
+    plmhip::Tune tune {2, 0, 0};
+    double* d_part_score = nullptr;
+    int* d_part_count = nullptr;
+    bool buffers_set = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    StageTimer timer;       // plm_hip_batch_set_timing
 };
 
 namespace {
 
 void
-destroy_events (std::vector<std::vector<hipEvent_t>>& sets)
-{
-    for (auto& s : sets)
-        for (hipEvent_t e : s) (void) hipEventDestroy (e);
-    sets.clear ();
-}
-
-void
 free_ctx (plm_hip_ctx* c)
 {
-    destroy_events (c->ev_used);
-    destroy_events (c->ev_free);
+    c->timer.destroy ();
     void* bufs[] = {c->d_fixed, c->d_moving, c->d_mgrad, c->d_blut, c->d_cond,
-        c->d_tile_score, c->d_tile_count, c->d_hist,
+        c->d_tile_score, c->d_tile_count, c->d_pair, c->d_hist,
         c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
         c->d_hist_out};
     for (void* b : bufs) if (b) (void) hipFree (b);
@@ -415,7 +483,7 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     e = hipMemcpy (c->d_blut, blut.data (), sizeof (float) * blut.size (), hipMemcpyHostToDevice);
     if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "hipMemcpy(lut)"); }
 
-    // knot-major condense slots; slots no tile owns stay zero for good
+    // slot-major condense buffer; slots no tile owns stay zero for good
     CTX_TRY (alloc (&c->d_cond, (size_t) knots * 192, "hipMalloc(cond)"));
     // per-workgroup partials of the persistent tile kernel: at most one
     // workgroup per tile, rounded up to a multiple of 8
@@ -443,6 +511,28 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     CTX_TRY (alloc (&c->d_score, 1, "hipMalloc(score)"));
     CTX_TRY (alloc (&c->d_nvox, 1, "hipMalloc(nvox)"));
     CTX_TRY (alloc (&c->d_hist_out, nbins, "hipMalloc(hist_out)"));
+    {
+        // the context as a table of one pair; evaluation buffers come with
+        // every call
+        plmhip::PairDev pd {};
+        pd.g = g;
+        pd.fimg = c->d_fixed;
+        pd.mimg = c->d_moving;
+        pd.mgrad = c->d_mgrad;
+        pd.cond = c->d_cond;
+        CTX_TRY (alloc (&c->d_pair, 1, "hipMalloc(pair)"));
+        e = hipMemcpy (c->d_pair, &pd, sizeof (pd), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "hipMemcpy(pair)"); }
+        plmhip::Plan& pl = c->plan;
+        pl.d_pairs = c->d_pair;
+        pl.npairs = 1;
+        pl.g0 = g;
+        pl.total_tiles = (int) tiles;
+        pl.reduce_blocks = (int) ((3 * knots + plmhip::REDUCE_BLOCK - 1) / plmhip::REDUCE_BLOCK);
+        pl.blut = c->d_blut;
+        pl.part_score = c->d_tile_score;
+        pl.part_count = c->d_tile_count;
+    }
 #undef CTX_TRY
     e = plmhip::launch_zero (c->d_cond, (int64_t) knots * 192, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize (c->stream);
@@ -473,30 +563,21 @@ plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
         if (!c->mi_ok) return fail (PLM_HIP_ERR_UNSUPPORTED, "MI flavor g does not take a ROI");
     }
     hipEvent_t* ev = nullptr;
-    if (c->timing) {
-        if (c->ev_free.empty ()) {
-            std::vector<hipEvent_t> s (plmhip::MAX_STAGE_EVENTS);
-            for (auto& x : s) HIP_TRY (hipEventCreate (&x));
-            c->ev_free.push_back (s);
-        }
-        c->ev_used.push_back (c->ev_free.back ());
-        c->ev_free.pop_back ();
-        ev = c->ev_used.back ().data ();
-    }
+    HIP_TRY (c->timer.next (&ev));
     hipError_t e;
     if (metric == PLM_METRIC_MSE) {
-        e = plmhip::launch_mse (c->g, c->tune, grad_mode, c->storage == PLM_STORE_BF16,
-            c->d_fixed, c->d_moving, c->d_mgrad, d_coeff, c->d_blut, c->d_cond, c->d_tile_score,
-            c->d_tile_count, d_score, d_num_vox, d_grad, st, ev);
+        const plmhip::EvalOut out {d_grad, d_score, d_num_vox};
+        e = plmhip::launch_mse (c->plan, c->tune, grad_mode, c->storage == PLM_STORE_BF16,
+            d_coeff, out, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MSE launch");
     } else {
-        e = plmhip::launch_mi (c->g, c->tune, c->h, c->storage == PLM_STORE_BF16, c->hist_blocks,
-            c->d_fixed, c->d_moving, c->d_fbin, c->d_mbin, d_coeff, c->d_blut, c->d_hist,
-            c->d_hist + (size_t) c->h.fbins * c->h.mbins, c->d_table, c->d_nvox_f, c->d_cond,
-            d_score, d_num_vox, d_grad, d_hist, st, ev);
+        const plmhip::EvalOut out {d_grad, d_score, d_num_vox};
+        e = plmhip::launch_mi (c->plan, c->tune, c->h, c->hist_blocks, c->d_fbin, c->d_mbin,
+            d_coeff, c->d_hist, c->d_hist + (size_t) c->h.fbins * c->h.mbins, c->d_table,
+            c->d_nvox_f, out, d_hist, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MI launch");
     }
-    if (ev) c->last_stages = metric == PLM_METRIC_MSE ? plmhip::MSE_STAGES : plmhip::MI_STAGES;
+    if (ev) c->timer.last_stages = metric == PLM_METRIC_MSE ? plmhip::MSE_STAGES : plmhip::MI_STAGES;
     return PLM_HIP_OK;
 }
 
@@ -504,7 +585,7 @@ int
 plm_hip_bspline_set_timing (plm_hip_ctx* c, int enable)
 {
     if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
-    c->timing = enable != 0;
+    c->timer.on = enable != 0;
     return PLM_HIP_OK;
 }
 
@@ -514,20 +595,7 @@ plm_hip_bspline_get_timing (plm_hip_ctx* c, double* stage_ms, int max_stages,
 {
     if (!c || !stage_ms || !evals) return fail (PLM_HIP_ERR_ARG, "NULL argument");
     HIP_TRY (hipSetDevice (c->device));
-    for (int i = 0; i < max_stages; i++) stage_ms[i] = 0.0;
-    int64_t n = 0;
-    for (auto& s : c->ev_used) {
-        HIP_TRY (hipEventSynchronize (s[c->last_stages]));
-        for (int i = 0; i < c->last_stages && i < max_stages; i++) {
-            float ms = 0.f;
-            HIP_TRY (hipEventElapsedTime (&ms, s[i], s[i + 1]));
-            stage_ms[i] += ms;
-        }
-        n++;
-        c->ev_free.push_back (s);
-    }
-    c->ev_used.clear ();
-    *evals = n;
+    HIP_TRY (c->timer.collect (stage_ms, max_stages, evals));
     return PLM_HIP_OK;
 }
 
@@ -580,12 +648,12 @@ plm_hip_bspline_autotune (plm_hip_ctx* c, int metric, int grad_mode, const float
     if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
     if (reps < 1) reps = 1;
     const float* coeff = d_coeff ? d_coeff : c->d_coeff;
-    const bool was_timing = c->timing;
+    const bool was_timing = c->timer.on;
     double stage[plmhip::MAX_STAGE_EVENTS];
     int64_t n = 0;
     int rc = plm_hip_bspline_get_timing (c, stage, plmhip::MAX_STAGE_EVENTS, &n);
     if (rc) return rc;
-    c->timing = true;
+    c->timer.on = true;
     const int cands[3] = {1, 2, 4};
     int best = c->tune.vox_per_thread;
     double best_ms = 1e300;
@@ -594,22 +662,22 @@ plm_hip_bspline_autotune (plm_hip_ctx* c, int metric, int grad_mode, const float
         for (int r = 0; r < reps + 1; r++) {
             rc = plm_hip_bspline_score_device (c, metric, grad_mode, coeff, c->d_score, c->d_grad,
                 c->d_nvox, nullptr, c->stream);
-            if (rc) { c->timing = was_timing; return rc; }
+            if (rc) { c->timer.on = was_timing; return rc; }
             if (r == 0) {
                 // the first call of each variant is a warm-up
                 rc = plm_hip_bspline_get_timing (c, stage, plmhip::MAX_STAGE_EVENTS, &n);
-                if (rc) { c->timing = was_timing; return rc; }
+                if (rc) { c->timer.on = was_timing; return rc; }
             }
         }
         rc = plm_hip_bspline_get_timing (c, stage, plmhip::MAX_STAGE_EVENTS, &n);
-        if (rc) { c->timing = was_timing; return rc; }
+        if (rc) { c->timer.on = was_timing; return rc; }
         double total = 0.0;
         for (int i = 0; i < plmhip::MAX_STAGE_EVENTS; i++) total += stage[i];
         total /= (double) (n ? n : 1);
         if (total < best_ms) { best_ms = total; best = cands[k]; }
     }
     c->tune.vox_per_thread = best;
-    c->timing = was_timing;
+    c->timer.on = was_timing;
     if (chosen) *chosen = best;
     if (eval_ms) *eval_ms = best_ms;
     return PLM_HIP_OK;
@@ -679,6 +747,276 @@ plm_hip_bspline_num_tiles (const plm_hip_ctx* c, int64_t* n)
 {
     if (!c || !n) return fail (PLM_HIP_ERR_ARG, "NULL argument");
     *n = c->num_tiles;
+    return PLM_HIP_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+void
+drop_graph (plm_hip_batch* b)
+{
+    if (b->exec) (void) hipGraphExecDestroy (b->exec);
+    if (b->graph) (void) hipGraphDestroy (b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void
+free_batch (plm_hip_batch* b)
+{
+    drop_graph (b);
+    b->timer.destroy ();
+    void* bufs[] = {b->d_pairs, b->d_part_score, b->d_part_count};
+    for (void* p : bufs) if (p) (void) hipFree (p);
+    if (b->stream) (void) hipStreamDestroy (b->stream);
+    delete b;
+}
+
+int
+check_batch_mode (const plm_hip_batch* b, int grad_mode)
+{
+    if (grad_mode != PLM_GRAD_REF_NN && grad_mode != PLM_GRAD_EXACT)
+        return fail (PLM_HIP_ERR_ARG, "bad grad_mode");
+    if (grad_mode == PLM_GRAD_REF_NN && !b->all_mgrad)
+        return fail (PLM_HIP_ERR_ARG, "PLM_GRAD_REF_NN needs moving_grad in every context");
+    return PLM_HIP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int
+plm_hip_batch_create (plm_hip_ctx* const* ctxs, int n, plm_hip_batch** out)
+{
+    if (!out) return fail (PLM_HIP_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!ctxs || n < 1 || n > plmhip::MAX_BATCH)
+        return fail (PLM_HIP_ERR_ARG, "a batch holds 1 to 64 contexts");
+    for (int i = 0; i < n; i++) {
+        const plm_hip_ctx* c = ctxs[i];
+        if (!c) return fail (PLM_HIP_ERR_ARG, "NULL context in the batch");
+        if (c->device != ctxs[0]->device)
+            return fail (PLM_HIP_ERR_ARG, "batched contexts must share a device");
+        if (c->storage != ctxs[0]->storage)
+            return fail (PLM_HIP_ERR_ARG, "batched contexts must share the storage type");
+        for (int a = 0; a < 3; a++)
+            if (c->g.vpr[a] != ctxs[0]->g.vpr[a])
+                return fail (PLM_HIP_ERR_ARG, "batched contexts must share vox_per_rgn");
+    }
+    HIP_TRY (hipSetDevice (ctxs[0]->device));
+    plm_hip_batch* b = new plm_hip_batch;
+    b->device = ctxs[0]->device;
+    b->storage = ctxs[0]->storage;
+    b->ctxs.assign (ctxs, ctxs + n);
+    b->tune = ctxs[0]->tune;
+    hipError_t e = hipStreamCreate (&b->stream);
+    if (e != hipSuccess) { free_batch (b); return hip_fail (e, "hipStreamCreate"); }
+    // the flattened tile queue and the reduce workgroups, pair after pair;
+    // until plm_hip_batch_set_buffers, each pair reads and writes its
+    // context's own staging buffers
+    int64_t tiles = 0, blocks = 0;
+    b->h_pairs.resize (n);
+    for (int i = 0; i < n; i++) {
+        plm_hip_ctx* c = ctxs[i];
+        plmhip::PairDev& pd = b->h_pairs[i];
+        pd = plmhip::PairDev {};
+        pd.g = c->g;
+        pd.fimg = c->d_fixed;
+        pd.mimg = c->d_moving;
+        pd.mgrad = c->d_mgrad;
+        pd.coeff = c->d_coeff;
+        pd.cond = c->d_cond;
+        pd.grad = c->d_grad;
+        pd.score = c->d_score;
+        pd.nvox = c->d_nvox;
+        pd.tile0 = (int) tiles;
+        pd.blk0 = (int) blocks;
+        tiles += c->num_tiles;
+        blocks += (3 * c->num_knots + plmhip::REDUCE_BLOCK - 1) / plmhip::REDUCE_BLOCK;
+        b->all_mgrad = b->all_mgrad && c->have_mgrad;
+    }
+    if (tiles >= (1 << 30) / plmhip::MAX_BATCH || blocks >= (1 << 30)) {
+        free_batch (b);
+        return fail (PLM_HIP_ERR_GEOMETRY, "batch too large");
+    }
+#define BATCH_TRY(expr) do { int r_ = (expr); if (r_) { free_batch (b); return r_; } } while (0)
+    BATCH_TRY (alloc (&b->d_pairs, (size_t) n, "hipMalloc(pairs)"));
+    BATCH_TRY (alloc (&b->d_part_score, (size_t) n * (tiles + 8), "hipMalloc(partials)"));
+    BATCH_TRY (alloc (&b->d_part_count, (size_t) n * (tiles + 8), "hipMalloc(partials)"));
+#undef BATCH_TRY
+    e = hipMemcpy (b->d_pairs, b->h_pairs.data (), sizeof (plmhip::PairDev) * n,
+        hipMemcpyHostToDevice);
+    if (e != hipSuccess) { free_batch (b); return hip_fail (e, "hipMemcpy(pairs)"); }
+    plmhip::Plan& pl = b->plan;
+    pl.d_pairs = b->d_pairs;
+    pl.npairs = n;
+    pl.g0 = ctxs[0]->g;
+    pl.total_tiles = (int) tiles;
+    pl.reduce_blocks = (int) blocks;
+    pl.blut = ctxs[0]->d_blut;      // equal tile edges, equal LUTs
+    pl.part_score = b->d_part_score;
+    pl.part_count = b->d_part_count;
+    *out = b;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_batch_set_buffers (plm_hip_batch* b, const float* const* d_coeff,
+    double* const* d_grad, double* d_score, int64_t* d_num_vox)
+{
+    if (!b) return fail (PLM_HIP_ERR_ARG, "batch is NULL");
+    if (!d_coeff || !d_grad || !d_score || !d_num_vox) return fail (PLM_HIP_ERR_ARG, "NULL argument");
+    const int n = (int) b->ctxs.size ();
+    for (int i = 0; i < n; i++)
+        if (!d_coeff[i] || !d_grad[i]) return fail (PLM_HIP_ERR_ARG, "NULL buffer pointer");
+    HIP_TRY (hipSetDevice (b->device));
+    HIP_TRY (hipStreamSynchronize (b->stream));
+    for (int i = 0; i < n; i++) {
+        plmhip::PairDev& pd = b->h_pairs[i];
+        pd.coeff = d_coeff[i];
+        pd.grad = d_grad[i];
+        pd.score = d_score + i;
+        pd.nvox = d_num_vox + i;
+    }
+    HIP_TRY (hipMemcpy (b->d_pairs, b->h_pairs.data (), sizeof (plmhip::PairDev) * n,
+        hipMemcpyHostToDevice));
+    drop_graph (b);
+    b->buffers_set = true;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_batch_score_device (plm_hip_batch* b, int grad_mode, void* stream)
+{
+    if (!b) return fail (PLM_HIP_ERR_ARG, "batch is NULL");
+    int rc = check_batch_mode (b, grad_mode);
+    if (rc) return rc;
+    HIP_TRY (hipSetDevice (b->device));
+    hipStream_t st = stream ? (hipStream_t) stream : b->stream;
+    hipEvent_t* ev = nullptr;
+    HIP_TRY (b->timer.next (&ev));
+    const plmhip::EvalOut none {nullptr, nullptr, nullptr};
+    hipError_t e = plmhip::launch_mse (b->plan, b->tune, grad_mode, b->storage == PLM_STORE_BF16,
+        nullptr, none, st, ev);
+    if (e != hipSuccess) return hip_fail (e, "batched MSE launch");
+    if (ev) b->timer.last_stages = plmhip::MSE_STAGES;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_batch_capture (plm_hip_batch* b, int grad_mode)
+{
+    if (!b) return fail (PLM_HIP_ERR_ARG, "batch is NULL");
+    int rc = check_batch_mode (b, grad_mode);
+    if (rc) return rc;
+    HIP_TRY (hipSetDevice (b->device));
+    HIP_TRY (hipStreamSynchronize (b->stream));
+    drop_graph (b);
+    // the evaluation has no memset and no host work: two kernel nodes
+    HIP_TRY (hipStreamBeginCapture (b->stream, hipStreamCaptureModeThreadLocal));
+    const plmhip::EvalOut none {nullptr, nullptr, nullptr};
+    hipError_t e = plmhip::launch_mse (b->plan, b->tune, grad_mode, b->storage == PLM_STORE_BF16,
+        nullptr, none, b->stream, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture (b->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void) hipGraphDestroy (g);
+        return hip_fail (e != hipSuccess ? e : e2, "graph capture");
+    }
+    b->graph = g;
+    e = hipGraphInstantiate (&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) { drop_graph (b); return hip_fail (e, "hipGraphInstantiate"); }
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_batch_launch (plm_hip_batch* b, void* stream)
+{
+    if (!b) return fail (PLM_HIP_ERR_ARG, "batch is NULL");
+    if (!b->exec) return fail (PLM_HIP_ERR_ARG, "no captured evaluation (plm_hip_batch_capture)");
+    HIP_TRY (hipSetDevice (b->device));
+    HIP_TRY (hipGraphLaunch (b->exec, stream ? (hipStream_t) stream : b->stream));
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_batch_set_tuning (plm_hip_batch* b, int vox_per_thread)
+{
+    if (!b) return fail (PLM_HIP_ERR_ARG, "batch is NULL");
+    if (vox_per_thread != 1 && vox_per_thread != 2 && vox_per_thread != 4)
+        return fail (PLM_HIP_ERR_ARG, "vox_per_thread must be 1, 2 or 4");
+    if (b->tune.vox_per_thread != vox_per_thread) drop_graph (b);
+    b->tune.vox_per_thread = vox_per_thread;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_batch_set_timing (plm_hip_batch* b, int enable)
+{
+    if (!b) return fail (PLM_HIP_ERR_ARG, "batch is NULL");
+    b->timer.on = enable != 0;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_batch_get_timing (plm_hip_batch* b, double* stage_ms, int max_stages, int64_t* evals)
+{
+    if (!b || !stage_ms || !evals) return fail (PLM_HIP_ERR_ARG, "NULL argument");
+    HIP_TRY (hipSetDevice (b->device));
+    HIP_TRY (b->timer.collect (stage_ms, max_stages, evals));
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_batch_autotune (plm_hip_batch* b, int grad_mode, int reps, int* chosen, double* eval_ms)
+{
+    if (!b) return fail (PLM_HIP_ERR_ARG, "batch is NULL");
+    int rc = check_batch_mode (b, grad_mode);
+    if (rc) return rc;
+    if (reps < 1) reps = 1;
+    const bool was_timing = b->timer.on;
+    double stage[plmhip::MAX_STAGE_EVENTS];
+    int64_t n = 0;
+    rc = plm_hip_batch_get_timing (b, stage, plmhip::MAX_STAGE_EVENTS, &n);
+    if (rc) return rc;
+    b->timer.on = true;
+    const int cands[3] = {1, 2, 4};
+    const int before = b->tune.vox_per_thread;
+    int best = before;
+    double best_ms = 1e300;
+    for (int k = 0; k < 3; k++) {
+        b->tune.vox_per_thread = cands[k];
+        for (int r = 0; r < reps + 1; r++) {
+            rc = plm_hip_batch_score_device (b, grad_mode, b->stream);
+            if (rc == PLM_HIP_OK && r == 0)     // the first call of each variant is a warm-up
+                rc = plm_hip_batch_get_timing (b, stage, plmhip::MAX_STAGE_EVENTS, &n);
+            if (rc) { b->timer.on = was_timing; b->tune.vox_per_thread = before; return rc; }
+        }
+        rc = plm_hip_batch_get_timing (b, stage, plmhip::MAX_STAGE_EVENTS, &n);
+        if (rc) { b->timer.on = was_timing; b->tune.vox_per_thread = before; return rc; }
+        double total = 0.0;
+        for (int i = 0; i < plmhip::MAX_STAGE_EVENTS; i++) total += stage[i];
+        total /= (double) (n ? n : 1);
+        if (total < best_ms) { best_ms = total; best = cands[k]; }
+    }
+    b->tune.vox_per_thread = best;
+    if (best != before) drop_graph (b);
+    b->timer.on = was_timing;
+    if (chosen) *chosen = best;
+    if (eval_ms) *eval_ms = best_ms;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_batch_destroy (plm_hip_batch* b)
+{
+    if (!b) return PLM_HIP_OK;
+    (void) hipSetDevice (b->device);
+    (void) hipStreamSynchronize (b->stream);
+    free_batch (b);
     return PLM_HIP_OK;
 }
 

@@ -11,6 +11,7 @@ constexpr int TILE_THREADS = 256;
 constexpr int MAX_BINS_1D = 256;     // per-axis histogram bins
 constexpr int MAX_BINS_2D = 4096;    // joint bins held in LDS (32 KB int64)
 constexpr int MAX_VPR = 32;          // tile edge bound
+constexpr int MAX_BATCH = 64;        // pairs in one batched evaluation
 
 // Everything a tile kernel needs, passed by value.
 struct KGeom {
@@ -76,9 +77,55 @@ struct Tune {
     int num_cus;             // compute units of the device (persistent grid size)
 };
 
-// LDS bytes of the tile kernel at vox_per_thread voxels per batch (the MI
-// table comes on top).
+// LDS bytes of the tile kernel at vox_per_thread voxels per batch, static
+// arrays included (the MI table comes on top).
 size_t tile_lds_bytes (const KGeom& g, int vox_per_thread);
+
+// One fixed/moving pair as the tile and reduce kernels see it (a device
+// table of these drives a launch; a single context is a table of one).
+struct PairDev {
+    KGeom g;
+    const void* fimg;        // fixed, moving, moving gradient (the batch's storage type)
+    const void* mimg;
+    const void* mgrad;
+    const float* coeff;      // coefficients of the evaluation (batched launches)
+    float* cond;             // slot-major condense buffer, 64 * 3 * num_knots
+    double* grad;            // outputs (batched launches): 3 * num_knots
+    double* score;
+    int64_t* nvox;
+    int tile0;               // first tile of the pair in the flattened tile queue
+    int blk0;                // first reduce workgroup of the pair
+};
+
+struct TileQueue {
+    int total_tiles;
+    int tiles_per_xcd;       // ceil(total_tiles / 8)
+    int npairs;
+    int pad;
+};
+
+// Per-evaluation outputs of pair 0 that override its table entry (the
+// single-pair entry points pass them with every call); null fields fall
+// back to the table.
+struct EvalOut {
+    double* grad;
+    double* score;
+    int64_t* nvox;
+};
+
+// Everything a launch needs about its set of pairs.
+struct Plan {
+    const PairDev* d_pairs;  // device table
+    int npairs;
+    KGeom g0;                // pair 0's geometry (every pair shares the tile edges)
+    int total_tiles;
+    int reduce_blocks;       // sum over pairs of ceil(3 * num_knots / 256)
+    const float* blut;       // basis LUTs of the shared tile edges
+    double* part_score;      // per (pair, workgroup) partials, npairs * (total_tiles + 8)
+    int* part_count;
+};
+
+constexpr int REDUCE_BLOCK = 256;    // gradient components per reduce workgroup
 
 // Stage timing: when ev is not null, ev[0..n] are recorded on st between
 // the launches (MSE: tiles | reduce, 3 events; MI: zero | hist | score |
@@ -87,16 +134,16 @@ constexpr int MSE_STAGES = 2;
 constexpr int MI_STAGES = 5;
 constexpr int MAX_STAGE_EVENTS = 6;
 
-hipError_t launch_mse (const KGeom& g, const Tune& tune, int mode, int storage,
-    const void* f, const void* m, const void* mg, const float* coeff, const float* blut,
-    float* cond, double* tile_score, int* tile_count, double* score,
-    int64_t* nvox, double* grad, hipStream_t st, hipEvent_t* ev);
+// MSE over every pair of the plan (coeff0/out0 override pair 0's table
+// entry when not null).
+hipError_t launch_mse (const Plan& plan, const Tune& tune, int mode, int storage,
+    const float* coeff0, const EvalOut& out0, hipStream_t st, hipEvent_t* ev);
 
-hipError_t launch_mi (const KGeom& g, const Tune& tune, const HGeom& h, int storage,
-    int hist_blocks, const void* f, const void* m, const uint8_t* fbin, const uint8_t* mbin,
-    const float* coeff, const float* blut, unsigned long long* jhist,
-    unsigned long long* fhist, float* table, float* nvox_f, float* cond, double* score,
-    int64_t* nvox, double* grad, int64_t* hist_out, hipStream_t st, hipEvent_t* ev);
+// Mattes MI of a single-pair plan.
+hipError_t launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
+    const uint8_t* fbin, const uint8_t* mbin, const float* coeff, unsigned long long* jhist,
+    unsigned long long* fhist, float* table, float* nvox_f, const EvalOut& out0,
+    int64_t* hist_out, hipStream_t st, hipEvent_t* ev);
 
 hipError_t launch_bin_image (const void* img, int storage, uint8_t* out, int64_t n,
     float offset, float delta, int bins, hipStream_t st);

@@ -205,14 +205,6 @@ hist_bin (float v, float offset, float delta, int bins)
 // ---------------------------------------------------------------------------
 // tile bookkeeping
 
-__device__ __forceinline__ int
-xcd_tile (const KGeom& g, int b)
-{
-    // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a
-    // contiguous run of tiles so neighbouring tiles share that XCD's L2.
-    return (b & 7) * g.tiles_per_xcd + (b >> 3);
-}
-
 __device__ __forceinline__ void
 tile_coords (const KGeom& g, int tile, int p[3])
 {
@@ -637,297 +629,350 @@ struct MiArgs {
     const uint8_t* mbin;       // per-voxel moving bin
 };
 
-template <int KIND, typename TI, int V, int G>
-__global__ __launch_bounds__ (TILE_THREADS)
-void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
-    const TI* __restrict__ mgrad, const float* __restrict__ coeff,
-    const float* __restrict__ blut, float* __restrict__ cond,
-    double* __restrict__ tile_score, int* __restrict__ tile_count, MiArgs mi)
+// The pair a tile of the flattened queue belongs to; p is the pair of an
+// earlier tile of the same workgroup (tiles only move forward).
+__device__ __forceinline__ int
+advance_pair (const PairDev* __restrict__ pairs, int npairs, int p, int tile)
 {
-    extern __shared__ float4 sm4[];
-    // Persistent workgroups: those on XCD x = blockIdx % 8 walk that XCD's
-    // contiguous run of tiles together, so neighbouring tiles (which share
-    // image lines and knots) meet in the same L2.
-    const int wpx = gridDim.x >> 3;
-    const int tbeg = (blockIdx.x & 7) * g.tiles_per_xcd;
-    const int tend = min (tbeg + g.tiles_per_xcd, g.num_tiles);
-    int tile = tbeg + (blockIdx.x >> 3);
-    if (tile >= tend) {
-        // a workgroup without tiles still owns its (empty) partial
-        if (KIND < 2 && threadIdx.x == 0) {
-            tile_score[blockIdx.x] = 0.0;
-            tile_count[blockIdx.x] = 0;
-        }
-        return;
-    }
+    while (p + 1 < npairs && pairs[p + 1].tile0 <= tile) p++;
+    return p;
+}
 
-    const Shape<V> S (g);
+// BATCH false: a single pair, whose geometry the compiler may then keep in
+// registers for the whole launch.
+template <int KIND, typename TI, int V, int G, bool BATCH>
+__global__ __launch_bounds__ (TILE_THREADS)
+void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __restrict__ coeff0,
+    const float* __restrict__ blut, double* __restrict__ part_score,
+    int* __restrict__ part_count, MiArgs mi)
+{
+    const int npairs = BATCH ? q.npairs : 1;
+    extern __shared__ float4 sm4[];
+    __shared__ double s_ps[MAX_BATCH];      // per-pair sums of this workgroup
+    __shared__ int s_pn[MAX_BATCH];
+    const int t = threadIdx.x;
+    if (KIND < 2 && t < npairs) {
+        s_ps[t] = 0.0;
+        s_pn[t] = 0;
+    }
+    // Persistent workgroups: those on XCD x = blockIdx % 8 walk that XCD's
+    // contiguous run of the (flattened) tile queue together, so
+    // neighbouring tiles, which share image lines and knots, meet in the
+    // same L2.
+    const int wpx = gridDim.x >> 3;
+    const int tbeg = (blockIdx.x & 7) * q.tiles_per_xcd;
+    const int tend = min (tbeg + q.tiles_per_xcd, q.total_tiles);
+    int tile = tbeg + (blockIdx.x >> 3);
+
+    // every pair shares the tile edges, so the LDS layout and the LUTs
+    // are those of pair 0
+    const Shape<V> S (pairs[0].g);
     const int vx = S.vx, vy = S.vy, vz = S.vz;
     const int nvox = vx * vy * vz;
     const int slots = tile_slots (nvox, G);
     const TileLds L = carve (reinterpret_cast<float*> (sm4), vx, vy, vz, slots);
-    const int t = threadIdx.x;
+    if (tile < tend) {
     tile_luts (L, S, blut);
     if (KIND == 2) {
         const int nj = mi.h.fbins * mi.h.mbins;
         for (int i = t; i < nj; i += TILE_THREADS) L.tab[i] = mi.table[i];
     }
-
-    const int fd0 = g.fdim[0], fd01 = g.fdim[0] * g.fdim[1];
-    const int md0 = g.mdim[0], md01 = g.mdim[0] * g.mdim[1];
     constexpr int ES = KIND == 2 ? 1 : Img<TI>::ES;   // bytes per voxel of what is loaded
-    const Rsrc fr = make_rsrc (KIND == 2 ? (const void*) mi.fbin : (const void*) fimg,
-        (uint32_t) (fd01 * g.fdim[2]) * ES);
-    const Rsrc mr = make_rsrc (KIND == 2 ? (const void*) mi.mbin : (const void*) mimg,
-        (uint32_t) (md01 * g.mdim[2]) * ES);
-    const Rsrc gr = make_rsrc (mgrad, KIND == 0 ? (uint32_t) (md01 * g.mdim[2]) * 3 * ES : 0u);
-    const int SX = md0 * ES, SXY = md01 * ES;
-    const float* mhi = g.mhi;
-    const float* mmax = g.mmax;
     float N = 1.f;
     if (KIND == 2) N = *mi.nvox_f;
 
     // This thread's first-batch voxel slots t + 256u (x fastest), the same
-    // in every tile, and their offsets from the tile's first fixed voxel.
-    // Packed as x | y << 10 | z << 20 (each below 1024).
-    int sxyz[G], sfo[G];
+    // in every tile, packed as x | y << 10 | z << 20 (each below 1024).
+    int sxyz[G];
 #pragma unroll
     for (int u = 0; u < G; u++) {
         const int v = u * TILE_THREADS + t;
         const int z = S.div_xy (v);
         const int r = v - z * (vx * vy);
         const int y = S.div_x (r);
-        const int x = r - y * vx;
-        sxyz[u] = x | (y << 10) | (z << 20);
-        sfo[u] = x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
+        sxyz[u] = (r - y * vx) | (y << 10) | (z << 20);
     }
+
     // Fetched one tile ahead: the knot coefficients (threads < 64) and the
-    // first batch's fixed voxels (or fixed bins).  Slots past the image read 0.
+    // first batch's fixed voxels (or fixed bins); voxels past the image read 0.
+    int p = advance_pair (pairs, npairs, 0, tile);
+    int np = p;                             // pair of the tile fetched ahead
+    TilePos nq = tile_pos (pairs[p].g, tile - pairs[p].tile0);
     float cpre[3] = {0.f, 0.f, 0.f};
     float fpre[G];
     int fbpre[G];
-    TilePos nq = tile_pos (g, tile);
-    if (t < 64) tile_coef_fetch (cpre, g, nq.knot0, t, coeff);
     {
+        const KGeom& g = pairs[p].g;
+        const float* cf = (coeff0 && p == 0) ? coeff0 : pairs[p].coeff;
+        if (t < 64) tile_coef_fetch (cpre, g, nq.knot0, t, cf);
+        const int fd0 = g.fdim[0], fd01 = g.fdim[0] * g.fdim[1];
         const int fb = nq.bx0 + fd0 * nq.by0 + fd01 * nq.bz0;
+        const Rsrc fr = make_rsrc (KIND == 2 ? (const void*) mi.fbin : pairs[p].fimg,
+            (uint32_t) (fd01 * g.fdim[2]) * ES);
 #pragma unroll
         for (int u = 0; u < G; u++) {
-            if (KIND < 2) fpre[u] = Img<TI>::one (fr, (fb + sfo[u]) * ES, 0);
-            else fbpre[u] = load_u8 (fr, fb + sfo[u], 0);
+            const int x = sxyz[u] & 1023, y = (sxyz[u] >> 10) & 1023, z = sxyz[u] >> 20;
+            const int fo = fb + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
+            if (KIND < 2) fpre[u] = Img<TI>::one (fr, fo * ES, 0);
+            else fbpre[u] = load_u8 (fr, fo, 0);
         }
     }
 
-    // this thread's squared differences and overlap count over all the
-    // tiles of the workgroup, summed over the workgroup once at the end
+    // this thread's squared differences and overlap count over the tiles
+    // of the current pair, folded into s_ps/s_pn when the pair changes
     double score = 0.0;
     int count = 0;
     for (; tile < tend; tile += wpx) {
-    const TilePos P = nq;
-    const int next = tile + wpx;
-    const bool has_next = next < tend;
-    if (has_next) nq = tile_pos (g, next);
-    // The thread index is re-read opaquely each tile, so the compiler
-    // recomputes the per-stage LDS addresses from it instead of keeping
-    // them all live across the tile loop (which costs occupancy).
-    int tt = t;
-    asm volatile ("" : "+v" (tt));
-#pragma unroll
-    for (int u = 0; u < G; u++) asm volatile ("" : "+v" (sxyz[u]), "+v" (sfo[u]));
-    __syncthreads ();       // the previous tile is done with coef, t1, s1, s2, red
-    tile_stage1 (L, S, g, P.bz0, cpre, tt);
-    __syncthreads ();
-    if (tt < 64 && has_next) tile_coef_fetch (cpre, g, nq.knot0, tt, coeff);
-    tile_t1 (L, S, tt);
-    __syncthreads ();
-    tile_columns (L, S, g, P.bx0, P.by0, tt);
-    __syncthreads ();
-
-    // Stage 4: the voxels.  Thread t takes voxels t, t + 256, ... of the
-    // tile, G at a time, issuing all their loads before using any.  Voxels
-    // outside the ROI or the tile still run (their fixed load may fall past
-    // the image and read 0) and are masked at the end; z < ez also rules
-    // out the slots past the tile.
-    const int ex = min (vx, g.roi_end[0] - P.bx0);
-    const int ey = min (vy, g.roi_end[1] - P.by0);
-    const int ez = min (vz, g.roi_end[2] - P.bz0);
-    const int fbase = P.bx0 + fd0 * P.by0 + fd01 * P.bz0;
-    const int fbase_next = nq.bx0 + fd0 * nq.by0 + fd01 * nq.bz0;
-
-#pragma unroll
-    for (int base = 0; base < nvox; base += TILE_THREADS * G) {
-        const bool first = base == 0;
-        bool in[G];
-        float fr_[G][3];
-        float fval[G];
-        int fbv[G];
-        float a[G][8];
-        float mg[G][3];
-        int bn[G][8];
-#pragma unroll
-        for (int u = 0; u < G; u++) {
-            int x, y, z;
-            if (first) {
-                x = sxyz[u] & 1023;
-                y = (sxyz[u] >> 10) & 1023;
-                z = sxyz[u] >> 20;
-                if (KIND < 2) fval[u] = fpre[u];
-                else fbv[u] = fbpre[u];
-            } else {
-                const int v = base + u * TILE_THREADS + tt;
-                z = S.div_xy (v);
-                const int r = v - z * (vx * vy);
-                y = S.div_x (r);
-                x = r - y * vx;
-                const int fofs = fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
-                if (KIND < 2) fval[u] = Img<TI>::one (fr, fofs * ES, 0);
-                else fbv[u] = load_u8 (fr, fofs, 0);
+        const TilePos P = nq;
+        if (KIND < 2 && np != p) {
+            // the workgroup moved on to the next pair: fold the last one
+            block_sum (L, score, count);
+            if (t == 0) {
+                s_ps[p] += score;
+                s_pn[p] += count;
             }
-            const bool act = x < ex && y < ey && z < ez;
-            const int r = x + (int) __umul24 (vx, y);
-            const int zc = min (z, vz - 1);
-
-            const float4* cp = reinterpret_cast<const float4*> (L.col) + r;
-            const float4* zp = reinterpret_cast<const float4*> (&L.bzz[8 * zc]);
-            const float4 b = zp[0], Z = zp[1];
-            const int np = vx * vy;
-            const float4 A0 = cp[0], A1 = cp[np], A2 = cp[2 * np], C = cp[3 * np];
-            float m[3];
-            m[0] = fmaf (b.w, A0.w, fmaf (b.z, A0.z, fmaf (b.y, A0.y, fmaf (b.x, A0.x, C.x)))) + Z.x;
-            m[1] = fmaf (b.w, A1.w, fmaf (b.z, A1.z, fmaf (b.y, A1.y, fmaf (b.x, A1.x, C.y)))) + Z.y;
-            m[2] = fmaf (b.w, A2.w, fmaf (b.z, A2.z, fmaf (b.y, A2.y, fmaf (b.x, A2.x, C.z)))) + Z.z;
-
-            // Volume::is_inside (src/base/volume.cxx:899-906) and li_clamp
-            // (src/base/interpolate.cxx:17-41), branch free:
-            // f = clamp(floor(m), 0, dim-2), frac = clamp(m - f, 0, 1) and
-            // round = f + (frac >= 0.5) give exactly its three branches (and
-            // the rounding of ROUND_PLM_LONG).  Any m, NaN included, yields
-            // in-range indices, so the loads are issued unconditionally.
-            bool inside = true;
-            int f[3], c[3];
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                inside = inside && m[q] > -0.5f && m[q] < mhi[q];
-                const float fl = __builtin_amdgcn_fmed3f (floorf (m[q]), 0.f, mmax[q]);
-                const float fa = fminf (fmaxf (m[q] - fl, 0.f), 1.f);
-                f[q] = (int) fl;
-                c[q] = fa >= 0.5f ? 1 : 0;
-                fr_[u][q] = fa;
-            }
-            in[u] = inside && act;
-            // byte offsets as 24-bit products (slices hold < 2^24 / 12
-            // voxels, checked at context creation)
-            const int idx = f[0] + (int) __umul24 (md0, f[1]) + (int) __umul24 (md01, f[2]);
-            if (KIND < 2) {
-                const int off = (int) __umul24 (f[0], ES) + (int) __umul24 (f[1], SX)
-                    + (int) __umul24 (f[2], SXY);
-                Img<TI>::pair (a[u][0], a[u][1], mr, off, 0);
-                Img<TI>::pair (a[u][2], a[u][3], mr, off, SX);
-                Img<TI>::pair (a[u][4], a[u][5], mr, off, SXY);
-                Img<TI>::pair (a[u][6], a[u][7], mr, off, SX + SXY);
-            } else {
-                bn[u][0] = load_u8 (mr, idx, 0);
-                bn[u][1] = load_u8 (mr, idx + 1, 0);
-                bn[u][2] = load_u8 (mr, idx, md0);
-                bn[u][3] = load_u8 (mr, idx + 1, md0);
-                bn[u][4] = load_u8 (mr, idx, md01);
-                bn[u][5] = load_u8 (mr, idx + 1, md01);
-                bn[u][6] = load_u8 (mr, idx, md0 + md01);
-                bn[u][7] = load_u8 (mr, idx + 1, md0 + md01);
-            }
-            if (KIND == 0) {
-                // nearest voxel: the gradient sample of flavor g
-                // (src/register/bspline_mse.cxx:679-681)
-                const int goff = (int) __umul24 (f[0] + c[0], 3 * ES)
-                    + (int) __umul24 (f[1] + c[1], 3 * SX) + (int) __umul24 (f[2] + c[2], 3 * SXY);
-                Img<TI>::vec3 (mg[u], gr, goff, 0);
-            }
+            score = 0.0;
+            count = 0;
         }
-        float bsum = 0.f;       // this batch's squared differences (<= G terms)
-#pragma unroll
-        for (int u = 0; u < G; u++) {
-            const int v = base + u * TILE_THREADS + tt;
-            float dc0, dc1, dc2;
-            if (KIND < 2) {
-                Sample s;
-                s.l2[0] = fr_[u][0]; s.l2[1] = fr_[u][1]; s.l2[2] = fr_[u][2];
-                float md[3];
-                const float m_val = trilinear<KIND == 1> (s, a[u], g.inv_msp, md);
-                const float diff = in[u] ? m_val - fval[u] : 0.f;
-                bsum += diff * diff;
-                count += in[u] ? 1 : 0;
-                if (KIND == 0) {
-                    dc0 = diff * mg[u][0];
-                    dc1 = diff * mg[u][1];
-                    dc2 = diff * mg[u][2];
-                } else {
-                    dc0 = diff * md[0];
-                    dc1 = diff * md[1];
-                    dc2 = diff * md[2];
-                }
-            } else {
-                float l1[3], l2[3];
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    l2[q] = fr_[u][q];
-                    l1[q] = 1.0f - l2[q];
-                }
-                dc0 = 0.f; dc1 = 0.f; dc2 = 0.f;
-                float dw[24];
-                dw[0] = -l1[1] * l1[2];  dw[1] = -l1[0] * l1[2];  dw[2] = -(l1[0] * l1[1]);
-                dw[3] = l1[1] * l1[2];   dw[4] = -l2[0] * l1[2];  dw[5] = -(l2[0] * l1[1]);
-                dw[6] = -l2[1] * l1[2];  dw[7] = l1[0] * l1[2];   dw[8] = -(l1[0] * l2[1]);
-                dw[9] = l2[1] * l1[2];   dw[10] = l2[0] * l1[2];  dw[11] = -(l2[0] * l2[1]);
-                dw[12] = -l1[1] * l2[2]; dw[13] = -l1[0] * l2[2]; dw[14] = l1[0] * l1[1];
-                dw[15] = l1[1] * l2[2];  dw[16] = -l2[0] * l2[2]; dw[17] = l2[0] * l1[1];
-                dw[18] = -l2[1] * l2[2]; dw[19] = l1[0] * l2[2];  dw[20] = l1[0] * l2[1];
-                dw[21] = l2[1] * l2[2];  dw[22] = l2[0] * l2[2];  dw[23] = l2[0] * l2[1];
-                const float* row = &L.tab[fbv[u] * mi.h.mbins];
-#pragma unroll
-                for (int n = 0; n < 8; n++) {
-                    const float ds = row[bn[u][n]];
-                    dc0 -= dw[3 * n + 0] * ds;
-                    dc1 -= dw[3 * n + 1] * ds;
-                    dc2 -= dw[3 * n + 2] * ds;
-                }
-                dc0 = dc0 / N;
-                dc1 = dc1 / N;
-                dc2 = dc2 / N;
-                // projected in place, component by component, as the
-                // reference does (src/register/bspline_mi.txx:107-109)
-                const float* P = g.mproj;
-                dc0 = dc0 * P[0] + dc1 * P[1] + dc2 * P[2];
-                dc1 = dc0 * P[3] + dc1 * P[4] + dc2 * P[5];
-                dc2 = dc0 * P[6] + dc1 * P[7] + dc2 * P[8];
-                if (!in[u]) { dc0 = 0.f; dc1 = 0.f; dc2 = 0.f; }
-            }
-            // every slot below tile_slots is in LDS; slots past the tile
-            // hold zeros and are never read
-            L.dv[v] = dc0;
-            L.dv[L.dvp + v] = dc1;
-            L.dv[2 * L.dvp + v] = dc2;
+        p = np;
+        const KGeom& g = pairs[p].g;
+        const PairDev& pd = pairs[p];
+        const int next = tile + wpx;
+        const bool has_next = next < tend;
+        if (has_next) {
+            np = advance_pair (pairs, npairs, p, next);
+            nq = tile_pos (pairs[np].g, next - pairs[np].tile0);
         }
-        score += (double) bsum;
-        if (first && has_next) {
-            // the next tile's first-batch fixed voxels, now that this
-            // batch's are consumed
+        // The thread index and the slot coordinates are re-read opaquely
+        // each tile, so the compiler recomputes what derives from them
+        // instead of keeping it all live across the loop (which costs
+        // occupancy).
+        int tt = t;
+        asm volatile ("" : "+v" (tt));
+#pragma unroll
+        for (int u = 0; u < G; u++) asm volatile ("" : "+v" (sxyz[u]));
+        __syncthreads ();       // the previous tile is done with coef, t1, s1, s2, red
+        tile_stage1 (L, S, g, P.bz0, cpre, tt);
+        __syncthreads ();
+        if (tt < 64 && has_next) {
+            const float* cf = (coeff0 && np == 0) ? coeff0 : pairs[np].coeff;
+            tile_coef_fetch (cpre, pairs[np].g, nq.knot0, tt, cf);
+        }
+        tile_t1 (L, S, tt);
+        __syncthreads ();
+        tile_columns (L, S, g, P.bx0, P.by0, tt);
+        __syncthreads ();
+
+        // Stage 4: the voxels.  Thread t takes voxels t, t + 256, ... of
+        // the tile, G at a time, issuing all their loads before using any.
+        // Voxels outside the ROI or the tile still run (their fixed load may
+        // fall past the image and read 0) and are masked at the end; z < ez
+        // also rules out the slots past the tile.
+        const int ex = min (vx, g.roi_end[0] - P.bx0);
+        const int ey = min (vy, g.roi_end[1] - P.by0);
+        const int ez = min (vz, g.roi_end[2] - P.bz0);
+        const int fd0 = g.fdim[0], fd01 = g.fdim[0] * g.fdim[1];
+        const int md0 = g.mdim[0], md01 = g.mdim[0] * g.mdim[1];
+        const int fbase = P.bx0 + fd0 * P.by0 + fd01 * P.bz0;
+        const Rsrc fr = make_rsrc (KIND == 2 ? (const void*) mi.fbin : pd.fimg,
+            (uint32_t) (fd01 * g.fdim[2]) * ES);
+        const Rsrc mr = make_rsrc (KIND == 2 ? (const void*) mi.mbin : pd.mimg,
+            (uint32_t) (md01 * g.mdim[2]) * ES);
+        const Rsrc gr = make_rsrc (pd.mgrad, KIND == 0 ? (uint32_t) (md01 * g.mdim[2]) * 3 * ES : 0u);
+        const int SX = md0 * ES, SXY = md01 * ES;
+
+#pragma unroll
+        for (int base = 0; base < nvox; base += TILE_THREADS * G) {
+            const bool first = base == 0;
+            bool in[G];
+            float fr_[G][3];
+            float fval[G];
+            int fbv[G];
+            float a[G][8];
+            float mg[G][3];
+            int bn[G][8];
 #pragma unroll
             for (int u = 0; u < G; u++) {
-                if (KIND < 2) fpre[u] = Img<TI>::one (fr, (fbase_next + sfo[u]) * ES, 0);
-                else fbpre[u] = load_u8 (fr, fbase_next + sfo[u], 0);
+                int x, y, z;
+                if (first) {
+                    x = sxyz[u] & 1023;
+                    y = (sxyz[u] >> 10) & 1023;
+                    z = sxyz[u] >> 20;
+                    if (KIND < 2) fval[u] = fpre[u];
+                    else fbv[u] = fbpre[u];
+                } else {
+                    const int v = base + u * TILE_THREADS + tt;
+                    z = S.div_xy (v);
+                    const int r = v - z * (vx * vy);
+                    y = S.div_x (r);
+                    x = r - y * vx;
+                    const int fofs = fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
+                    if (KIND < 2) fval[u] = Img<TI>::one (fr, fofs * ES, 0);
+                    else fbv[u] = load_u8 (fr, fofs, 0);
+                }
+                const bool act = x < ex && y < ey && z < ez;
+                const int r = x + (int) __umul24 (vx, y);
+                const int zc = min (z, vz - 1);
+
+                const float4* cp = reinterpret_cast<const float4*> (L.col) + r;
+                const float4* zp = reinterpret_cast<const float4*> (&L.bzz[8 * zc]);
+                const float4 b = zp[0], Z = zp[1];
+                const int ncol = vx * vy;
+                const float4 A0 = cp[0], A1 = cp[ncol], A2 = cp[2 * ncol], C = cp[3 * ncol];
+                float m[3];
+                m[0] = fmaf (b.w, A0.w, fmaf (b.z, A0.z, fmaf (b.y, A0.y, fmaf (b.x, A0.x, C.x)))) + Z.x;
+                m[1] = fmaf (b.w, A1.w, fmaf (b.z, A1.z, fmaf (b.y, A1.y, fmaf (b.x, A1.x, C.y)))) + Z.y;
+                m[2] = fmaf (b.w, A2.w, fmaf (b.z, A2.z, fmaf (b.y, A2.y, fmaf (b.x, A2.x, C.z)))) + Z.z;
+
+                // Volume::is_inside (src/base/volume.cxx:899-906) and
+                // li_clamp (src/base/interpolate.cxx:17-41), branch free:
+                // f = clamp(floor(m), 0, dim-2), frac = clamp(m - f, 0, 1)
+                // and round = f + (frac >= 0.5) give exactly its three
+                // branches (and the rounding of ROUND_PLM_LONG).  Any m, NaN
+                // included, yields in-range indices, so the loads are issued
+                // unconditionally.
+                bool inside = true;
+                int f[3], c[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    inside = inside && m[k] > -0.5f && m[k] < g.mhi[k];
+                    const float fl = __builtin_amdgcn_fmed3f (floorf (m[k]), 0.f, g.mmax[k]);
+                    const float fa = fminf (fmaxf (m[k] - fl, 0.f), 1.f);
+                    f[k] = (int) fl;
+                    c[k] = fa >= 0.5f ? 1 : 0;
+                    fr_[u][k] = fa;
+                }
+                in[u] = inside && act;
+                // byte offsets as 24-bit products (slices hold < 2^24 / 12
+                // voxels, checked at context creation)
+                if (KIND < 2) {
+                    const int off = (int) __umul24 (f[0], ES) + (int) __umul24 (f[1], SX)
+                        + (int) __umul24 (f[2], SXY);
+                    Img<TI>::pair (a[u][0], a[u][1], mr, off, 0);
+                    Img<TI>::pair (a[u][2], a[u][3], mr, off, SX);
+                    Img<TI>::pair (a[u][4], a[u][5], mr, off, SXY);
+                    Img<TI>::pair (a[u][6], a[u][7], mr, off, SX + SXY);
+                } else {
+                    const int idx = f[0] + (int) __umul24 (md0, f[1]) + (int) __umul24 (md01, f[2]);
+                    bn[u][0] = load_u8 (mr, idx, 0);
+                    bn[u][1] = load_u8 (mr, idx + 1, 0);
+                    bn[u][2] = load_u8 (mr, idx, md0);
+                    bn[u][3] = load_u8 (mr, idx + 1, md0);
+                    bn[u][4] = load_u8 (mr, idx, md01);
+                    bn[u][5] = load_u8 (mr, idx + 1, md01);
+                    bn[u][6] = load_u8 (mr, idx, md0 + md01);
+                    bn[u][7] = load_u8 (mr, idx + 1, md0 + md01);
+                }
+                if (KIND == 0) {
+                    // nearest voxel: the gradient sample of flavor g
+                    // (src/register/bspline_mse.cxx:679-681)
+                    const int goff = (int) __umul24 (f[0] + c[0], 3 * ES)
+                        + (int) __umul24 (f[1] + c[1], 3 * SX) + (int) __umul24 (f[2] + c[2], 3 * SXY);
+                    Img<TI>::vec3 (mg[u], gr, goff, 0);
+                }
+            }
+            float bsum = 0.f;       // this batch's squared differences (<= G terms)
+#pragma unroll
+            for (int u = 0; u < G; u++) {
+                const int v = base + u * TILE_THREADS + tt;
+                float dc0, dc1, dc2;
+                if (KIND < 2) {
+                    Sample s;
+                    s.l2[0] = fr_[u][0]; s.l2[1] = fr_[u][1]; s.l2[2] = fr_[u][2];
+                    float md[3];
+                    const float m_val = trilinear<KIND == 1> (s, a[u], g.inv_msp, md);
+                    const float diff = in[u] ? m_val - fval[u] : 0.f;
+                    bsum += diff * diff;
+                    count += in[u] ? 1 : 0;
+                    if (KIND == 0) {
+                        dc0 = diff * mg[u][0];
+                        dc1 = diff * mg[u][1];
+                        dc2 = diff * mg[u][2];
+                    } else {
+                        dc0 = diff * md[0];
+                        dc1 = diff * md[1];
+                        dc2 = diff * md[2];
+                    }
+                } else {
+                    float l1[3], l2[3];
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        l2[k] = fr_[u][k];
+                        l1[k] = 1.0f - l2[k];
+                    }
+                    dc0 = 0.f; dc1 = 0.f; dc2 = 0.f;
+                    float dw[24];
+                    dw[0] = -l1[1] * l1[2];  dw[1] = -l1[0] * l1[2];  dw[2] = -(l1[0] * l1[1]);
+                    dw[3] = l1[1] * l1[2];   dw[4] = -l2[0] * l1[2];  dw[5] = -(l2[0] * l1[1]);
+                    dw[6] = -l2[1] * l1[2];  dw[7] = l1[0] * l1[2];   dw[8] = -(l1[0] * l2[1]);
+                    dw[9] = l2[1] * l1[2];   dw[10] = l2[0] * l1[2];  dw[11] = -(l2[0] * l2[1]);
+                    dw[12] = -l1[1] * l2[2]; dw[13] = -l1[0] * l2[2]; dw[14] = l1[0] * l1[1];
+                    dw[15] = l1[1] * l2[2];  dw[16] = -l2[0] * l2[2]; dw[17] = l2[0] * l1[1];
+                    dw[18] = -l2[1] * l2[2]; dw[19] = l1[0] * l2[2];  dw[20] = l1[0] * l2[1];
+                    dw[21] = l2[1] * l2[2];  dw[22] = l2[0] * l2[2];  dw[23] = l2[0] * l2[1];
+                    const float* row = &L.tab[fbv[u] * mi.h.mbins];
+#pragma unroll
+                    for (int n = 0; n < 8; n++) {
+                        const float ds = row[bn[u][n]];
+                        dc0 -= dw[3 * n + 0] * ds;
+                        dc1 -= dw[3 * n + 1] * ds;
+                        dc2 -= dw[3 * n + 2] * ds;
+                    }
+                    dc0 = dc0 / N;
+                    dc1 = dc1 / N;
+                    dc2 = dc2 / N;
+                    // projected in place, component by component, as the
+                    // reference does (src/register/bspline_mi.txx:107-109)
+                    const float* Pm = g.mproj;
+                    dc0 = dc0 * Pm[0] + dc1 * Pm[1] + dc2 * Pm[2];
+                    dc1 = dc0 * Pm[3] + dc1 * Pm[4] + dc2 * Pm[5];
+                    dc2 = dc0 * Pm[6] + dc1 * Pm[7] + dc2 * Pm[8];
+                    if (!in[u]) { dc0 = 0.f; dc1 = 0.f; dc2 = 0.f; }
+                }
+                // every slot below tile_slots is in LDS; slots past the tile
+                // hold zeros and are never read
+                L.dv[v] = dc0;
+                L.dv[L.dvp + v] = dc1;
+                L.dv[2 * L.dvp + v] = dc2;
+            }
+            score += (double) bsum;
+            if (first && has_next) {
+                // the next tile's first-batch fixed voxels, now that this
+                // batch's are consumed
+                const KGeom& gn = pairs[np].g;
+                const int nd0 = gn.fdim[0], nd01 = gn.fdim[0] * gn.fdim[1];
+                const int fbn = nq.bx0 + nd0 * nq.by0 + nd01 * nq.bz0;
+                const Rsrc frn = make_rsrc (KIND == 2 ? (const void*) mi.fbin : pairs[np].fimg,
+                    (uint32_t) (nd01 * gn.fdim[2]) * ES);
+#pragma unroll
+                for (int u = 0; u < G; u++) {
+                    const int x = sxyz[u] & 1023, y = (sxyz[u] >> 10) & 1023, z = sxyz[u] >> 20;
+                    const int fo = fbn + x + (int) __umul24 (nd0, y) + (int) __umul24 (nd01, z);
+                    if (KIND < 2) fpre[u] = Img<TI>::one (frn, fo * ES, 0);
+                    else fbpre[u] = load_u8 (frn, fo, 0);
+                }
             }
         }
+        __syncthreads ();
+        tile_condense (L, S, g, P.knot0, pd.cond, tt);
     }
-    __syncthreads ();
-    tile_condense (L, S, g, P.knot0, cond, tt);
+    if (KIND < 2) {
+        // fold the last pair: each thread's tiles in their fixed order, then
+        // the threads in a fixed order
+        block_sum (L, score, count);
+        if (t == 0) {
+            s_ps[p] += score;
+            s_pn[p] += count;
+        }
     }
-    // one partial per workgroup: each thread's tiles in their fixed order,
-    // then the threads in a fixed order
+    }
+    // one partial per (pair, workgroup); pairs this workgroup did not touch
+    // get zeros
     if (KIND < 2) {
         __syncthreads ();
-        block_sum (L, score, count);
-        if (threadIdx.x == 0) {
-            tile_score[blockIdx.x] = score;
-            tile_count[blockIdx.x] = count;
+        if (t < npairs) {
+            part_score[t * gridDim.x + blockIdx.x] = s_ps[t];
+            part_count[t * gridDim.x + blockIdx.x] = s_pn[t];
         }
     }
 }
@@ -936,13 +981,14 @@ void k_tiles (KGeom g, const TI* __restrict__ fimg, const TI* __restrict__ mimg,
 // Reduce: one thread per gradient component, summing its 64 slots in slot
 // order in FP64 -- the order of bspline_condense_smetric_grad
 // (src/register/bspline.cxx:400-413), in double -- then scaled by 2/N for
-// MSE (src/register/bspline_mse.cxx:43-73) or 1 for MI.  For MSE every
-// workgroup also sums the per-tile overlap counts (integers, so every
-// workgroup gets the same N with no cross-workgroup traffic), interleaved
-// with the slot loads; workgroup 0 also sums the per-tile scores in tile
-// order and writes score = raw/N (FLT_MAX if N < 1) and N.
+// MSE (src/register/bspline_mse.cxx:43-73) or 1 for MI.  Workgroups are
+// dealt to the pairs of the launch in order.  For MSE every workgroup also
+// sums its pair's per-workgroup overlap counts (integers, so every
+// workgroup gets the same N with no cross-workgroup traffic), fetched with
+// the slot loads; the pair's first workgroup also sums the score partials
+// in a fixed order and writes score = raw/N (FLT_MAX if N < 1) and N.
 
-constexpr int REDUCE_THREADS = 256;
+constexpr int REDUCE_THREADS = REDUCE_BLOCK;
 
 __device__ __forceinline__ double
 wave_sum (double v)
@@ -962,45 +1008,52 @@ wave_sum_ll (long long v)
 
 template <bool MSE>
 __global__ __launch_bounds__ (REDUCE_THREADS)
-void k_reduce (KGeom g, const float* __restrict__ cond, const double* __restrict__ tile_score,
-    const int* __restrict__ tile_count, int nparts, double* __restrict__ grad,
-    double* __restrict__ out_score, int64_t* __restrict__ out_nvox)
+void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __restrict__ part_score,
+    const int* __restrict__ part_count, int nparts, EvalOut out0)
 {
     __shared__ long long s_n[REDUCE_THREADS / 64];
     __shared__ double s_s[REDUCE_THREADS / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nwaves = REDUCE_THREADS / 64;
-    const int nc = 3 * g.num_knots;
-    const int i = blockIdx.x * REDUCE_THREADS + threadIdx.x;
+    // the pair this workgroup works for (workgroups are dealt to pairs in
+    // order, pairs[p].blk0 being the first of pair p)
+    int p = 0;
+    while (p + 1 < npairs && pairs[p + 1].blk0 <= (int) blockIdx.x) p++;
+    const PairDev& pd = pairs[p];
+    const int blk = blockIdx.x - pd.blk0;
+    const bool lead = blk == 0;
+    const int nc = 3 * pd.g.num_knots;
+    const int i = blk * REDUCE_THREADS + threadIdx.x;
     const bool own = i < nc;
-    const float* c = cond + (own ? i : 0);
+    const float* c = pd.cond + (own ? i : 0);
+    double* grad = (out0.grad && p == 0) ? out0.grad : pd.grad;
     // all 64 slot loads are issued before any is consumed (a deep enough
     // queue to cover the memory latency with few waves), then summed in slot
-    // order; the per-tile counts and scores are fetched meanwhile
+    // order; the pair's per-workgroup partials are fetched meanwhile
     float v[64];
 #pragma unroll
     for (int m = 0; m < 64; m++) v[m] = c[(size_t) m * nc];
     long long n = 0;
     double sc = 0.0;
     if (MSE) {
-        // the per-workgroup partials of the tile kernel, CH per thread with
-        // all loads in flight together (one round for up to 2048
-        // partials), summed in a fixed order
+        // CH partials per thread with all loads in flight together (one
+        // round for up to 2048 partials), summed in a fixed order
         constexpr int CH = 8;
-        const bool want_score = blockIdx.x == 0;
+        const int* pc = part_count + (size_t) p * nparts;
+        const double* ps = part_score + (size_t) p * nparts;
         for (int t0 = 0; t0 < nparts; t0 += CH * REDUCE_THREADS) {
             int cnt[CH];
             double s[CH];
 #pragma unroll
-            for (int q = 0; q < CH; q++) {
-                const int t = t0 + q * REDUCE_THREADS + threadIdx.x;
-                cnt[q] = t < nparts ? tile_count[t] : 0;
-                s[q] = want_score && t < nparts ? tile_score[t] : 0.0;
+            for (int k = 0; k < CH; k++) {
+                const int t = t0 + k * REDUCE_THREADS + threadIdx.x;
+                cnt[k] = t < nparts ? pc[t] : 0;
+                s[k] = lead && t < nparts ? ps[t] : 0.0;
             }
 #pragma unroll
-            for (int q = 0; q < CH; q++) {
-                n += cnt[q];
-                sc += s[q];
+            for (int k = 0; k < CH; k++) {
+                n += cnt[k];
+                sc += s[k];
             }
         }
     }
@@ -1013,13 +1066,15 @@ void k_reduce (KGeom g, const float* __restrict__ cond, const double* __restrict
         sc = wave_sum (sc);
         if (lane == 0) { s_n[wave] = n; s_s[wave] = sc; }
         __syncthreads ();
-        long long N = 0;
-        double S = 0.0;
-        for (int w = 0; w < nwaves; w++) { N += s_n[w]; S += s_s[w]; }
-        scale = N >= 1 ? 2.0 / (double) N : 0.0;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            *out_score = N >= 1 ? S / (double) N : (double) FLT_MAX;
-            *out_nvox = N;
+        long long Nv = 0;
+        double Sv = 0.0;
+        for (int w = 0; w < nwaves; w++) { Nv += s_n[w]; Sv += s_s[w]; }
+        scale = Nv >= 1 ? 2.0 / (double) Nv : 0.0;
+        if (lead && threadIdx.x == 0) {
+            double* score = (out0.score && p == 0) ? out0.score : pd.score;
+            int64_t* nvox = (out0.nvox && p == 0) ? out0.nvox : pd.nvox;
+            *score = Nv >= 1 ? Sv / (double) Nv : (double) FLT_MAX;
+            *nvox = Nv;
         }
     }
     if (own) grad[i] = acc * scale;
@@ -1355,7 +1410,8 @@ tile_lds_bytes (const KGeom& g, int vox_per_thread)
     const int vx = g.vpr[0], vy = g.vpr[1], vz = g.vpr[2];
     const size_t f = 4 * vx + 4 * vy + 8 * vz + 192 + tile_region1_floats (vx, vy, vz)
         + 3 * (size_t) tile_slots (vx * vy * vz, vox_per_thread);
-    return f * sizeof (float) + 16 * sizeof (double);
+    // + the per-pair partial sums, static in the kernel
+    return f * sizeof (float) + 16 * sizeof (double) + MAX_BATCH * (sizeof (double) + sizeof (int));
 }
 
 // Records ev[i] on st when stage timing is on (ev != nullptr).
@@ -1375,11 +1431,10 @@ mark (hipEvent_t* ev, int i, hipStream_t st)
 // (occupancy from the kernel's registers and this launch's LDS), a
 // multiple of 8 so every XCD gets the same count, and no more than an XCD
 // has tiles.
-template <int KIND, typename TI, int V, int G>
+template <int KIND, typename TI, int V, int G, bool BATCH>
 static int
-launch_tiles_vg (const KGeom& g, const Tune& tune, size_t lds, hipStream_t st, const void* f,
-    const void* m, const void* mg, const float* coeff, const float* blut, float* cond,
-    double* tile_score, int* tile_count, const MiArgs& mi)
+launch_tiles_vgb (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st,
+    const float* coeff0, const MiArgs& mi)
 {
     static std::mutex mu;
     static size_t cached_lds = 0;
@@ -1389,7 +1444,7 @@ launch_tiles_vg (const KGeom& g, const Tune& tune, size_t lds, hipStream_t st, c
         std::lock_guard<std::mutex> lock (mu);
         if (cached_lds != lds) {
             int n = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor (&n, k_tiles<KIND, TI, V, G>,
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor (&n, k_tiles<KIND, TI, V, G, BATCH>,
                     TILE_THREADS, lds) != hipSuccess || n < 1)
                 n = 1;
             cached_lds = lds;
@@ -1397,47 +1452,65 @@ launch_tiles_vg (const KGeom& g, const Tune& tune, size_t lds, hipStream_t st, c
         }
         occ = cached_occ;
     }
+    TileQueue q;
+    q.total_tiles = plan.total_tiles;
+    q.tiles_per_xcd = (plan.total_tiles + 7) / 8;
+    q.npairs = plan.npairs;
+    q.pad = 0;
     const int cus = tune.num_cus > 0 ? tune.num_cus : 8;
-    const int per_xcd = std::max (1, std::min (g.tiles_per_xcd, occ * cus / 8));
-    hipLaunchKernelGGL ((k_tiles<KIND, TI, V, G>), dim3 (8 * per_xcd), dim3 (TILE_THREADS),
-        lds, st, g, (const TI*) f, (const TI*) m, (const TI*) mg, coeff, blut, cond, tile_score,
-        tile_count, mi);
+    const int per_xcd = std::max (1, std::min (q.tiles_per_xcd, occ * cus / 8));
+    hipLaunchKernelGGL ((k_tiles<KIND, TI, V, G, BATCH>), dim3 (8 * per_xcd), dim3 (TILE_THREADS),
+        lds, st, plan.d_pairs, q, coeff0, plan.blut, plan.part_score, plan.part_count, mi);
     return 8 * per_xcd;
+}
+
+template <int KIND, typename TI, int V, int G>
+static int
+launch_tiles_vg (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st,
+    const float* coeff0, const MiArgs& mi)
+{
+    // MI is single-pair only
+    if (KIND < 2 && plan.npairs > 1)
+        return launch_tiles_vgb<KIND, TI, V, G, (KIND < 2)> (plan, tune, lds, st, coeff0, mi);
+    return launch_tiles_vgb<KIND, TI, V, G, false> (plan, tune, lds, st, coeff0, mi);
 }
 
 template <int KIND, typename TI, int V>
 static int
-launch_tiles_v (const KGeom& g, const Tune& tune, int vpt, size_t extra, hipStream_t st,
-    const void* f, const void* m, const void* mg, const float* coeff, const float* blut,
-    float* cond, double* tile_score, int* tile_count, const MiArgs& mi)
+launch_tiles_v (const Plan& plan, const Tune& tune, int vpt, size_t extra, hipStream_t st,
+    const float* coeff0, const MiArgs& mi)
 {
+    const KGeom& g = plan.g0;
+    // dynamic LDS: the carve-up (+ the MI table), without the static arrays
+    constexpr size_t stat = MAX_BATCH * (sizeof (double) + sizeof (int));
     if (vpt == 1)
-        return launch_tiles_vg<KIND, TI, V, 1> (g, tune, tile_lds_bytes (g, 1) + extra, st, f, m,
-            mg, coeff, blut, cond, tile_score, tile_count, mi);
+        return launch_tiles_vg<KIND, TI, V, 1> (plan, tune, tile_lds_bytes (g, 1) - stat + extra,
+            st, coeff0, mi);
     if (vpt == 2)
-        return launch_tiles_vg<KIND, TI, V, 2> (g, tune, tile_lds_bytes (g, 2) + extra, st, f, m,
-            mg, coeff, blut, cond, tile_score, tile_count, mi);
-    return launch_tiles_vg<KIND, TI, V, 4> (g, tune, tile_lds_bytes (g, 4) + extra, st, f, m, mg,
-        coeff, blut, cond, tile_score, tile_count, mi);
+        return launch_tiles_vg<KIND, TI, V, 2> (plan, tune, tile_lds_bytes (g, 2) - stat + extra,
+            st, coeff0, mi);
+    return launch_tiles_vg<KIND, TI, V, 4> (plan, tune, tile_lds_bytes (g, 4) - stat + extra,
+        st, coeff0, mi);
 }
 
 // The tile kernel is specialized at compile time for the isotropic tile
 // edges 8, 10, 12 and 15 voxels; other vox_per_rgn run the generic build.
 // vox_per_thread falls back to the largest value whose dc/dv slots fit in
 // LDS (1 always fits: the context checked it at creation).  Returns the
-// grid size, the number of per-workgroup score/count partials.
+// grid size, the number of per-workgroup score/count partials per pair.
+// (The dynamic LDS passed at launch excludes the kernel's static arrays,
+// which tile_lds_bytes counts for the fit.)
 template <int KIND, typename TI>
 static int
-launch_tiles (const KGeom& g, const Tune& tune, size_t extra, hipStream_t st, const void* f,
-    const void* m, const void* mg, const float* coeff, const float* blut, float* cond,
-    double* tile_score, int* tile_count, const MiArgs& mi)
+launch_tiles (const Plan& plan, const Tune& tune, size_t extra, hipStream_t st,
+    const float* coeff0, const MiArgs& mi)
 {
+    const KGeom& g = plan.g0;
     int vpt = tune.vox_per_thread;
     while (vpt > 1 && tile_lds_bytes (g, vpt) + extra > tune.lds_limit) vpt >>= 1;
     const int v = g.vpr[0];
     const bool iso = g.vpr[1] == v && g.vpr[2] == v;
-#define PLM_TILES(VV) return launch_tiles_v<KIND, TI, VV> (g, tune, vpt, extra, st, f, m, mg, \
-        coeff, blut, cond, tile_score, tile_count, mi)
+#define PLM_TILES(VV) return launch_tiles_v<KIND, TI, VV> (plan, tune, vpt, extra, st, coeff0, mi)
     if (iso && v == 8) PLM_TILES (8);
     if (iso && v == 10) PLM_TILES (10);
     if (iso && v == 12) PLM_TILES (12);
@@ -1446,44 +1519,36 @@ launch_tiles (const KGeom& g, const Tune& tune, size_t extra, hipStream_t st, co
 #undef PLM_TILES
 }
 
-static int
-reduce_blocks (const KGeom& g)
-{
-    return (3 * g.num_knots + REDUCE_THREADS - 1) / REDUCE_THREADS;
-}
-
 hipError_t
-launch_mse (const KGeom& g, const Tune& tune, int mode, int storage, const void* f,
-    const void* m, const void* mg, const float* coeff, const float* blut, float* cond,
-    double* tile_score, int* tile_count, double* score, int64_t* nvox, double* grad,
-    hipStream_t st, hipEvent_t* ev)
+launch_mse (const Plan& plan, const Tune& tune, int mode, int storage, const float* coeff0,
+    const EvalOut& out0, hipStream_t st, hipEvent_t* ev)
 {
     MiArgs mi {};
     hipError_t e = mark (ev, 0, st);
     if (e != hipSuccess) return e;
     int nparts;
     if (storage == 0) {
-        if (mode == 0) nparts = launch_tiles<0, float> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
-        else nparts = launch_tiles<1, float> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        if (mode == 0) nparts = launch_tiles<0, float> (plan, tune, 0, st, coeff0, mi);
+        else nparts = launch_tiles<1, float> (plan, tune, 0, st, coeff0, mi);
     } else {
-        if (mode == 0) nparts = launch_tiles<0, uint16_t> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
-        else nparts = launch_tiles<1, uint16_t> (g, tune, 0, st, f, m, mg, coeff, blut, cond, tile_score, tile_count, mi);
+        if (mode == 0) nparts = launch_tiles<0, uint16_t> (plan, tune, 0, st, coeff0, mi);
+        else nparts = launch_tiles<1, uint16_t> (plan, tune, 0, st, coeff0, mi);
     }
     LAUNCH_CHECK ();
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL (k_reduce<true>, dim3 (reduce_blocks (g)), dim3 (REDUCE_THREADS), 0, st, g,
-        cond, tile_score, tile_count, nparts, grad, score, nvox);
+    hipLaunchKernelGGL (k_reduce<true>, dim3 (plan.reduce_blocks), dim3 (REDUCE_THREADS), 0, st,
+        plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts, out0);
     LAUNCH_CHECK ();
     return mark (ev, 2, st);
 }
 
 hipError_t
-launch_mi (const KGeom& g, const Tune& tune, const HGeom& h, int storage, int hist_blocks,
-    const void* f, const void* m, const uint8_t* fbin, const uint8_t* mbin, const float* coeff,
-    const float* blut, unsigned long long* jhist, unsigned long long* fhist, float* table,
-    float* nvox_f, float* cond, double* score, int64_t* nvox,
-    double* grad, int64_t* hist_out, hipStream_t st, hipEvent_t* ev)
+launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
+    const uint8_t* fbin, const uint8_t* mbin, const float* coeff, unsigned long long* jhist,
+    unsigned long long* fhist, float* table, float* nvox_f, const EvalOut& out0,
+    int64_t* hist_out, hipStream_t st, hipEvent_t* ev)
 {
+    const KGeom& g = plan.g0;
     const int nbins = h.fbins * h.mbins + h.fbins;
     hipError_t e = mark (ev, 0, st);
     if (e != hipSuccess) return e;
@@ -1496,24 +1561,23 @@ launch_mi (const KGeom& g, const Tune& tune, const HGeom& h, int storage, int hi
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
     if (g.aligned)
         hipLaunchKernelGGL ((k_mi_hist<true>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
-            g, h, fbin, mbin, coeff, blut, jhist, fhist);
+            g, h, fbin, mbin, coeff, plan.blut, jhist, fhist);
     else
         hipLaunchKernelGGL ((k_mi_hist<false>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
-            g, h, fbin, mbin, coeff, blut, jhist, fhist);
+            g, h, fbin, mbin, coeff, plan.blut, jhist, fhist);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 2, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_mi_score, dim3 (1), dim3 (1024), 0, st, h, jhist, fhist, table,
-        score, nvox, nvox_f, hist_out);
+        out0.score, out0.nvox, nvox_f, hist_out);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 3, st)) != hipSuccess) return e;
     MiArgs mi {h, table, nvox_f, fbin, mbin};
     // pass 2 reads only bin images; the image type parameter is unused
-    launch_tiles<2, float> (g, tune, table_bytes, st, f, m, nullptr, coeff, blut, cond,
-        nullptr, nullptr, mi);
+    launch_tiles<2, float> (plan, tune, table_bytes, st, coeff, mi);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 4, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL (k_reduce<false>, dim3 (reduce_blocks (g)), dim3 (REDUCE_THREADS), 0, st, g,
-        cond, nullptr, nullptr, 0, grad, nullptr, nullptr);
+    hipLaunchKernelGGL (k_reduce<false>, dim3 (plan.reduce_blocks), dim3 (REDUCE_THREADS), 0, st,
+        plan.d_pairs, plan.npairs, nullptr, nullptr, 0, out0);
     LAUNCH_CHECK ();
     return mark (ev, 5, st);
 }

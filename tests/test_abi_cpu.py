@@ -41,6 +41,17 @@ def test_argument_errors_without_device():
     assert b"NULL" in lib.plm_hip_last_error()
     assert lib.plm_hip_bspline_score(None, 0, 0, None, None, None, None, None) == _abi.PLM_HIP_ERR_ARG
     assert lib.plm_hip_bspline_destroy(None) == _abi.PLM_HIP_OK
+    # the batch entry points check their arguments before any HIP call
+    b = ctypes.c_void_p()
+    assert lib.plm_hip_batch_create(None, 1, ctypes.byref(b)) == _abi.PLM_HIP_ERR_ARG
+    arr = (ctypes.c_void_p * 1)(None)
+    assert lib.plm_hip_batch_create(arr, 1, ctypes.byref(b)) == _abi.PLM_HIP_ERR_ARG
+    assert lib.plm_hip_batch_create(arr, 65, ctypes.byref(b)) == _abi.PLM_HIP_ERR_ARG
+    assert b.value is None
+    assert lib.plm_hip_batch_score_device(None, 0, None) == _abi.PLM_HIP_ERR_ARG
+    assert lib.plm_hip_batch_launch(None, None) == _abi.PLM_HIP_ERR_ARG
+    assert lib.plm_hip_batch_set_tuning(None, 2) == _abi.PLM_HIP_ERR_ARG
+    assert lib.plm_hip_batch_destroy(None) == _abi.PLM_HIP_OK
 
 
 def test_bad_geometry_rejected_before_device():
