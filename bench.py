@@ -11,6 +11,13 @@ The dominant kernel (the fused tile kernel) is timed separately with HIP
 events on its launch stream; its roofline uses the algorithmic bytes of the
 flavor-g path: fixed 4 B + moving 4 B + moving gradient 12 B per voxel, plus
 the coefficient vector.
+
+The same line carries the batched multi-atlas comparison ("batched"): 16
+pairs with edges drawn from 96..256 per axis (10-voxel grid, N(0, 3 mm)
+fields), images and moving gradient stored as BF16, evaluated (a) as one
+batched evaluation replayed from a hipGraph and (b) by looping the
+single-pair entry point over the 16 pairs on one stream; both report
+aggregate pair-evaluations per second.
 """
 import argparse
 import json
@@ -24,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 WORKLOAD = "mse_256x256x128_vpr10"
+METRIC = "score+grad evals/s and HBM GB/s: batched vs single-pair HIP at \u22641e-4 rel grad err"
 HBM_PEAK_GBS = 8000.0
 N_COEFF_VECTORS = 20
 
@@ -66,12 +74,129 @@ def cpu_baseline(pair, coeffs, gpu_results, budget_s=20.0):
     }, errs
 
 
+BATCH_PAIRS = 16
+BATCH_SEED = 0
+
+
+def batch_oracle_errors(pairs, coeffs, results):
+    """Per-pair relative score and gradient error of the batched GPU results
+    against the FP64 oracle on the same (BF16-rounded) images and BF16-stored
+    moving gradient."""
+    import oracle
+    from plastimatch_amd.bspline import bf16_round
+    threads = min(16, len(os.sched_getaffinity(0)))
+    errs = []
+    for p, c, (score, grad) in zip(pairs, coeffs, results):
+        case = oracle.Case(p.fixed, p.moving, p.spacing, p.vox_per_rgn, origin=p.origin)
+        case._m_grad = bf16_round(case.m_grad)
+        s64, g64, _ = case.mse_f64(c, 0, nthreads=threads)
+        errs.append((abs(score - s64) / abs(s64), float(np.linalg.norm(grad - g64) / np.linalg.norm(g64))))
+    return errs
+
+
+def batched_leg(args, dev, want_errors):
+    """Batched (hipGraph replay) vs looped single-pair evaluation of the
+    16-pair BF16 multi-atlas workload; aggregate pair-evaluations per second."""
+    import torch
+    from plastimatch_amd import synth
+    from plastimatch_amd.bspline import (BsplineBatch, BsplineHip, Bspline_xform, Volume, bf16_round,
+                                         volume_gradient)
+
+    pairs = synth.make_batch(BATCH_PAIRS, seed=BATCH_SEED)
+    stages = []
+    for p in pairs:
+        p.fixed = bf16_round(p.fixed)
+        p.moving = bf16_round(p.moving)
+        fixed = Volume(torch.from_numpy(p.fixed).to(dev), p.spacing, p.origin)
+        moving = Volume(torch.from_numpy(p.moving).to(dev), p.spacing, p.origin)
+        mg = volume_gradient(moving)
+        bxf = Bspline_xform.for_volume(fixed, p.vox_per_rgn)
+        stages.append(BsplineHip(bxf, fixed, moving, mg, storage="bf16"))
+        del fixed, moving, mg
+    nvox = sum(p.fixed.size for p in pairs)
+    coeffs = [synth.random_coeff(p.dim, p.vox_per_rgn, 3.0, seed=200 + i) for i, p in enumerate(pairs)]
+    d_coeffs = [torch.from_numpy(c).to(dev) for c in coeffs]
+    d_grads = [torch.zeros(c.size, dtype=torch.float64, device=dev) for c in coeffs]
+    d_score = torch.zeros(BATCH_PAIRS, dtype=torch.float64, device=dev)
+    d_nvox = torch.zeros(BATCH_PAIRS, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    steps = max(10, args.steps // 10)
+    warm = max(2, args.warmup // 10)
+
+    # (b) looping the single-pair entry point, each stage autotuned on its own
+    for st, c in zip(stages, d_coeffs):
+        st.autotune("mse", "ref_nn", c, reps=3)
+
+    def loop_once():
+        for i, st in enumerate(stages):
+            st.score_device(d_coeffs[i], d_score[i:i + 1], d_grads[i], d_nvox[i:i + 1], "mse", "ref_nn",
+                            stream=stream)
+
+    for _ in range(warm):
+        loop_once()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loop_once()
+    torch.cuda.synchronize()
+    t_loop = (time.perf_counter() - t0) / steps
+
+    # (a) one batched evaluation, autotuned, captured once and replayed
+    batch = BsplineBatch(stages)
+    batch.set_buffers(d_coeffs, d_grads, d_score, d_nvox)
+    vpt, _ = batch.autotune("ref_nn", reps=3)
+    batch.capture("ref_nn")
+    for _ in range(warm):
+        batch.launch(stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.launch(stream)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    # the batched tile kernel alone, event-timed on its stream (eager launches)
+    batch.set_timing(True)
+    for _ in range(steps):
+        batch.score_device("ref_nn", stream)
+    ms, n = batch.get_timing()
+    batch.set_timing(False)
+    tile_ms = ms[0] / n
+    torch.cuda.synchronize()
+    results = [(float(d_score[i].item()), d_grads[i].cpu().numpy()) for i in range(BATCH_PAIRS)]
+    bytes_per_eval = 10 * nvox + sum(4 * c.size + 8 * c.size for c in coeffs)   # BF16: 2 + 2 + 6 B/voxel
+    out = {
+        "workload": f"batch{BATCH_PAIRS}_edges96-256_vpr10_bf16",
+        "pairs": BATCH_PAIRS,
+        "voxels": int(nvox),
+        "dims": [list(p.dim) for p in pairs],
+        "storage": "bf16 (FP32 interpolation, FP64 reductions)",
+        "vox_per_thread": vpt,
+        "batched_graph_ms_per_eval": t_batch * 1e3,
+        "looped_single_ms_per_eval": t_loop * 1e3,
+        "batched_pair_evals_per_s": BATCH_PAIRS / t_batch,
+        "looped_pair_evals_per_s": BATCH_PAIRS / t_loop,
+        "speedup_batched_vs_looped": t_loop / t_batch,
+        "batched_tile_kernel_ms": tile_ms,
+        "batched_gb_per_s": bytes_per_eval / t_batch / 1e9,
+        "tile_kernel_gb_per_s": 10 * nvox / (tile_ms * 1e-3) / 1e9,
+    }
+    if want_errors:
+        errs = batch_oracle_errors(pairs, coeffs, results)
+        out["rel_err_vs_fp64_per_pair"] = [{"score": e[0], "grad_l2": e[1]} for e in errs]
+        out["max_rel_err_vs_fp64"] = {"score": max(e[0] for e in errs), "grad_l2": max(e[1] for e in errs)}
+    batch.close()
+    for st in stages:
+        st.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-atlas leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy kernel 10 times "
                          "(reference for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -144,7 +269,7 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
     out = {
-        "metric": "score+grad evals/s (single-pair HIP, MSE flavor g semantics)",
+        "metric": METRIC,
         "value": value,
         "unit": "evals/s",
         "n_gpus": 1,
@@ -168,15 +293,18 @@ def main():
                      "kernel_ms": tile_ms},
         "cpu_baseline": None,
     }
+    torch.cuda.synchronize()
+    check = {}
+    for k in (0, 1):
+        check[k] = (float(d_score[k].item()), d_grad[k].cpu().numpy())
+    st.close()
+    del fixed, moving, mgrad
     if not args.no_cpu:
-        torch.cuda.synchronize()
-        check = {}
-        for k in (0, 1):
-            check[k] = (float(d_score[k].item()), d_grad[k].cpu().numpy())
         base, errs = cpu_baseline(pair, coeffs, check)
         out["cpu_baseline"] = base
         out["rel_err_vs_fp64"] = {"score": max(e[0] for e in errs), "grad_l2": max(e[1] for e in errs)}
-    st.close()
+    if not args.no_batch:
+        out["batched"] = batched_leg(args, dev, want_errors=not args.no_cpu)
     print(json.dumps(out))
 
 

@@ -143,6 +143,14 @@ class Bspline_score:
         return self.grad.astype(np.float32)
 
 
+def bf16_round(a):
+    """float32 -> bfloat16 -> float32 with round to nearest even: what
+    PLM_STORE_BF16 keeps of an image (NaN payloads aside)."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (r.astype(np.uint32) << 16).view(np.float32).reshape(np.shape(a))
+
+
 def volume_gradient(vol, gpuid=0):
     """Moving-image gradient on the GPU (src/base/volume_grad.cxx:9-76)."""
     lib = _abi.load()
