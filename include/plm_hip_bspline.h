@@ -53,6 +53,9 @@ extern "C" {
  *             (src/register/bspline_mse.cxx:1316-1351); the exact adjoint. */
 #define PLM_GRAD_REF_NN 0
 #define PLM_GRAD_EXACT  1
+/* MSE score and overlap count only, no gradient (single contexts): the
+ * forward pass of the differentiable binding. */
+#define PLM_GRAD_NONE   2
 
 /* where a pointer lives */
 #define PLM_MEM_HOST   0
@@ -192,6 +195,22 @@ int plm_hip_bspline_get_timing (plm_hip_ctx* ctx, double* stage_ms, int max_stag
 int plm_hip_bspline_set_tuning (plm_hip_ctx* ctx, int vox_per_thread);
 int plm_hip_bspline_autotune (plm_hip_ctx* ctx, int metric, int grad_mode,
     const float* d_coeff, int reps, int* chosen_vox_per_thread, double* eval_ms);
+
+/* Dense displacement field of coefficients d_coeff, float3 (mm)
+ * interleaved per fixed voxel: the ROI voxels are filled, the rest set to
+ * zero -- bspline_interpolate_vf (src/base/bspline_interpolate.cxx:112-142),
+ * with device pointers, enqueued on stream (NULL: the context's). */
+int plm_hip_bspline_interpolate_vf (plm_hip_ctx* ctx, const float* d_coeff, float* d_vf,
+    void* stream);
+
+/* MSE score and exact-derivative gradient (as PLM_GRAD_EXACT) taking the
+ * displacement from a stored field d_vf (as plm_hip_bspline_interpolate_vf
+ * writes it) instead of interpolating the coefficients: the stored-field
+ * form of the backward pass.  No counterpart in the reference, whose
+ * flavors always interpolate (bspline_interp_pix_c,
+ * src/base/bspline_interpolate.cxx:78-110). */
+int plm_hip_bspline_score_vf_device (plm_hip_ctx* ctx, const float* d_vf, double* d_score,
+    double* d_grad, int64_t* d_num_vox, void* stream);
 
 /* Geometry queries, for the host adapter. */
 int plm_hip_bspline_num_knots (const plm_hip_ctx* ctx, int64_t* num_knots);

@@ -24,6 +24,7 @@ PLM_METRIC_MSE = 0
 PLM_METRIC_MI = 1
 PLM_GRAD_REF_NN = 0
 PLM_GRAD_EXACT = 1
+PLM_GRAD_NONE = 2
 PLM_MEM_HOST = 0
 PLM_MEM_DEVICE = 1
 PLM_STORE_F32 = 0
@@ -92,6 +93,8 @@ SIGNATURES = {
                                                 ctypes.POINTER(ctypes.c_double)]),
     "plm_hip_bspline_get_timing": (ctypes.c_int, [_P, _P, ctypes.c_int,
                                                   ctypes.POINTER(ctypes.c_int64)]),
+    "plm_hip_bspline_interpolate_vf": (ctypes.c_int, [_P, _P, _P, _P]),
+    "plm_hip_bspline_score_vf_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "plm_hip_bspline_num_knots": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_bspline_num_tiles": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),

@@ -26,7 +26,7 @@ from . import _abi
 IDENTITY = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
 
 METRICS = {"mse": _abi.PLM_METRIC_MSE, "mi": _abi.PLM_METRIC_MI}
-GRAD_MODES = {"ref_nn": _abi.PLM_GRAD_REF_NN, "exact": _abi.PLM_GRAD_EXACT}
+GRAD_MODES = {"ref_nn": _abi.PLM_GRAD_REF_NN, "exact": _abi.PLM_GRAD_EXACT, "none": _abi.PLM_GRAD_NONE}
 STORAGE = {"f32": _abi.PLM_STORE_F32, "bf16": _abi.PLM_STORE_BF16}
 
 
@@ -194,6 +194,12 @@ class BsplineHip:
         self.num_tiles = n.value
 
     @property
+    def fixed_voxels(self):
+        """Voxels of the fixed image (a displacement field holds 3 floats each)."""
+        d = self.bxf.img_dim
+        return d[0] * d[1] * d[2]
+
+    @property
     def hist_size(self):
         if self.hist is None:
             return 0
@@ -223,6 +229,19 @@ class BsplineHip:
         return _abi.check(self.lib.plm_hip_bspline_score_device(
             self.ctx, METRICS[metric], GRAD_MODES[grad_mode], ptr(d_coeff), ptr(d_score),
             ptr(d_grad), ptr(d_num_vox), ptr(d_hist), st))
+
+    def interpolate_vf(self, d_coeff, d_vf, stream=None):
+        """Dense displacement field (mm, float3 per fixed voxel) into d_vf."""
+        st = ctypes.c_void_p(stream) if stream else None
+        return _abi.check(self.lib.plm_hip_bspline_interpolate_vf(
+            self.ctx, ctypes.c_void_p(d_coeff.data_ptr()), ctypes.c_void_p(d_vf.data_ptr()), st))
+
+    def score_vf_device(self, d_vf, d_score, d_grad, d_num_vox, stream=None):
+        """MSE score and exact-derivative gradient from a stored displacement field."""
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+        st = ctypes.c_void_p(stream) if stream else None
+        return _abi.check(self.lib.plm_hip_bspline_score_vf_device(
+            self.ctx, ptr(d_vf), ptr(d_score), ptr(d_grad), ptr(d_num_vox), st))
 
     def set_tuning(self, vox_per_thread):
         _abi.check(self.lib.plm_hip_bspline_set_tuning(self.ctx, int(vox_per_thread)))

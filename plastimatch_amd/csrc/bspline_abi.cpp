@@ -547,18 +547,19 @@ plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
     int64_t* d_hist, void* stream)
 {
     if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
-    if (!d_coeff || !d_score || !d_grad || !d_num_vox)
+    if (!d_coeff || !d_score || !d_num_vox || (!d_grad && grad_mode != PLM_GRAD_NONE))
         return fail (PLM_HIP_ERR_ARG, "NULL output or coefficient pointer");
     hipStream_t st = stream ? (hipStream_t) stream : c->stream;
     HIP_TRY (hipSetDevice (c->device));
     if (metric != PLM_METRIC_MSE && metric != PLM_METRIC_MI)
         return fail (PLM_HIP_ERR_ARG, "unknown metric");
     if (metric == PLM_METRIC_MSE) {
-        if (grad_mode != PLM_GRAD_REF_NN && grad_mode != PLM_GRAD_EXACT)
+        if (grad_mode != PLM_GRAD_REF_NN && grad_mode != PLM_GRAD_EXACT && grad_mode != PLM_GRAD_NONE)
             return fail (PLM_HIP_ERR_ARG, "bad grad_mode");
         if (grad_mode == PLM_GRAD_REF_NN && !c->have_mgrad)
             return fail (PLM_HIP_ERR_ARG, "PLM_GRAD_REF_NN needs moving_grad");
     } else {
+        if (grad_mode == PLM_GRAD_NONE) return fail (PLM_HIP_ERR_ARG, "PLM_GRAD_NONE is MSE only");
         if (!c->have_hist) return fail (PLM_HIP_ERR_ARG, "MI needs a histogram description");
         if (!c->mi_ok) return fail (PLM_HIP_ERR_UNSUPPORTED, "MI flavor g does not take a ROI");
     }
@@ -567,7 +568,9 @@ plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
     hipError_t e;
     if (metric == PLM_METRIC_MSE) {
         const plmhip::EvalOut out {d_grad, d_score, d_num_vox};
-        e = plmhip::launch_mse (c->plan, c->tune, grad_mode, c->storage == PLM_STORE_BF16,
+        const int mode = grad_mode == PLM_GRAD_REF_NN ? plmhip::MSE_REF_NN
+            : grad_mode == PLM_GRAD_EXACT ? plmhip::MSE_EXACT : plmhip::MSE_SCORE_ONLY;
+        e = plmhip::launch_mse (c->plan, c->tune, mode, c->storage == PLM_STORE_BF16,
             d_coeff, out, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MSE launch");
     } else {
@@ -731,6 +734,36 @@ plm_hip_volume_gradient (const plm_img_desc* vol, float* out, int out_location, 
     if (d_in) (void) hipFree (d_in);
     if (d_out) (void) hipFree (d_out);
     if (e != hipSuccess) return hip_fail (e, "volume gradient");
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_interpolate_vf (plm_hip_ctx* c, const float* d_coeff, float* d_vf, void* stream)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    if (!d_coeff || !d_vf) return fail (PLM_HIP_ERR_ARG, "NULL argument");
+    HIP_TRY (hipSetDevice (c->device));
+    hipStream_t st = stream ? (hipStream_t) stream : c->stream;
+    hipError_t e = plmhip::launch_vf (c->plan, c->tune, d_coeff, d_vf, st);
+    if (e != hipSuccess) return hip_fail (e, "displacement field launch");
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_score_vf_device (plm_hip_ctx* c, const float* d_vf, double* d_score,
+    double* d_grad, int64_t* d_num_vox, void* stream)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    if (!d_vf || !d_score || !d_grad || !d_num_vox) return fail (PLM_HIP_ERR_ARG, "NULL argument");
+    HIP_TRY (hipSetDevice (c->device));
+    hipStream_t st = stream ? (hipStream_t) stream : c->stream;
+    hipEvent_t* ev = nullptr;
+    HIP_TRY (c->timer.next (&ev));
+    const plmhip::EvalOut out {d_grad, d_score, d_num_vox};
+    hipError_t e = plmhip::launch_mse_from_vf (c->plan, c->tune, c->storage == PLM_STORE_BF16,
+        d_vf, out, st, ev);
+    if (e != hipSuccess) return hip_fail (e, "MSE from field launch");
+    if (ev) c->timer.last_stages = plmhip::MSE_STAGES;
     return PLM_HIP_OK;
 }
 

@@ -134,10 +134,25 @@ constexpr int MSE_STAGES = 2;
 constexpr int MI_STAGES = 5;
 constexpr int MAX_STAGE_EVENTS = 6;
 
+// What an MSE evaluation computes.
+constexpr int MSE_REF_NN = 0;        // score + gradient, nearest-voxel moving gradient
+constexpr int MSE_EXACT = 1;         // score + gradient, analytic trilinear derivative
+constexpr int MSE_SCORE_ONLY = 2;    // score and overlap count (single-pair plans)
+
 // MSE over every pair of the plan (coeff0/out0 override pair 0's table
 // entry when not null).
 hipError_t launch_mse (const Plan& plan, const Tune& tune, int mode, int storage,
     const float* coeff0, const EvalOut& out0, hipStream_t st, hipEvent_t* ev);
+
+// Dense displacement field of a single-pair plan: float3 (mm) per fixed
+// voxel, the ROI filled and the rest zero.
+hipError_t launch_vf (const Plan& plan, const Tune& tune, const float* coeff0, float* vf,
+    hipStream_t st);
+
+// MSE score and exact-derivative gradient of a single-pair plan from a
+// stored displacement field instead of the coefficients.
+hipError_t launch_mse_from_vf (const Plan& plan, const Tune& tune, int storage, const float* vf,
+    const EvalOut& out0, hipStream_t st, hipEvent_t* ev);
 
 // Mattes MI of a single-pair plan.
 hipError_t launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,

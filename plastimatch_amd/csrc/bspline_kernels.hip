@@ -400,22 +400,29 @@ tile_coef_fetch (float c[3], const KGeom& g, int knot0, int t, const float* __re
     c[2] = coeff[3 * knot + 2];
 }
 
+// How a tile kernel forms the per-voxel coordinate m from the column
+// terms: PROJECTED, the moving index coordinate (A = P T2, C and Z the
+// position terms); DISPLACEMENT, the displacement itself in mm (A = T2,
+// C = Z = 0); OFFSETS, the position terms only (A unused, the displacement
+// comes from elsewhere).
+enum ColumnMode { PROJECTED = 0, DISPLACEMENT = 1, OFFSETS = 2 };
+
 // Stage 1 of a tile: its 64 knot coefficients (fetched ahead into
 // registers) and the per-slice moving-index term
 // Z[z][a] = M[a][2] * (bz0 + z), M = P_moving * step_fixed.
-template <int V>
+template <int V, int MODE>
 __device__ __forceinline__ void
 tile_stage1 (const TileLds& L, const Shape<V>& S, const KGeom& g, int bz0, const float c[3],
     int t)
 {
-    if (t < 64) {
+    if (MODE != OFFSETS && t < 64) {
         L.coef[3 * t + 0] = c[0];
         L.coef[3 * t + 1] = c[1];
         L.coef[3 * t + 2] = c[2];
     }
     for (int e = t; e < 3 * S.vz; e += TILE_THREADS) {
         const int z = e / 3, a = e - 3 * z;
-        L.bzz[8 * z + 4 + a] = g.mstep[3 * a + 2] * (float) (bz0 + z);
+        L.bzz[8 * z + 4 + a] = MODE == DISPLACEMENT ? 0.f : g.mstep[3 * a + 2] * (float) (bz0 + z);
     }
 }
 
@@ -448,7 +455,7 @@ tile_t1 (const TileLds& L, const Shape<V>& S, int t)
 // which is P * (fixed position + displacement - moving origin) regrouped
 // (bspline_find_correspondence_dcos, src/util/bspline_correspond.cxx:80-109,
 // with the position of POSITION_FROM_COORDS, src/base/volume_macros.h:82-96).
-template <int V>
+template <int V, int MODE>
 __device__ __forceinline__ void
 tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int by0, int t)
 {
@@ -456,6 +463,20 @@ tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int 
     const float4* t1 = reinterpret_cast<const float4*> (L.t1);
     for (int e = t; e < vx * S.vy; e += TILE_THREADS) {
         const int y = S.div_x (e), x = e - y * vx;
+        // one float4 plane per term, so the voxel pass reads consecutive
+        // 16-byte records (conflict-free) for consecutive columns
+        float4* o = reinterpret_cast<float4*> (L.col) + e;
+        const int np = vx * S.vy;
+        if (MODE != DISPLACEMENT) {
+            const float fi = (float) (bx0 + x), fj = (float) (by0 + y);
+            float C[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) C[a] = (g.moff[a] + g.mstep[3 * a] * fi) + g.mstep[3 * a + 1] * fj;
+            o[3 * np] = make_float4 (C[0], C[1], C[2], 0.f);
+        } else {
+            o[3 * np] = make_float4 (0.f, 0.f, 0.f, 0.f);
+        }
+        if (MODE == OFFSETS) continue;
         const float4 b = reinterpret_cast<const float4*> (L.by)[y];
         float T[3][4];
 #pragma unroll
@@ -469,23 +490,15 @@ tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int 
             T[2][k] = fmaf (b.w, r3.z, fmaf (b.z, r2.z, fmaf (b.y, r1.z, b.x * r0.z)));
         }
         const float* P = g.mproj;
-        // one float4 plane per term, so the voxel pass reads consecutive
-        // 16-byte records (conflict-free) for consecutive columns
-        float4* o = reinterpret_cast<float4*> (L.col) + e;
-        const int np = vx * S.vy;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             float A[4];
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                A[k] = fmaf (P[3 * a + 2], T[2][k], fmaf (P[3 * a + 1], T[1][k], P[3 * a] * T[0][k]));
+                A[k] = MODE == DISPLACEMENT ? T[a][k]
+                    : fmaf (P[3 * a + 2], T[2][k], fmaf (P[3 * a + 1], T[1][k], P[3 * a] * T[0][k]));
             o[a * np] = make_float4 (A[0], A[1], A[2], A[3]);
         }
-        const float fi = (float) (bx0 + x), fj = (float) (by0 + y);
-        float C[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) C[a] = (g.moff[a] + g.mstep[3 * a] * fi) + g.mstep[3 * a + 1] * fj;
-        o[3 * np] = make_float4 (C[0], C[1], C[2], 0.f);
     }
 }
 
@@ -639,19 +652,30 @@ advance_pair (const PairDev* __restrict__ pairs, int npairs, int p, int tile)
 }
 
 // BATCH false: a single pair, whose geometry the compiler may then keep in
-// registers for the whole launch.
+// registers for the whole launch.  vf is the dense displacement field of
+// pair 0 (KIND 4 writes it, KIND 5 reads it), float3 interleaved per fixed
+// voxel.
 template <int KIND, typename TI, int V, int G, bool BATCH>
 __global__ __launch_bounds__ (TILE_THREADS)
 void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __restrict__ coeff0,
     const float* __restrict__ blut, double* __restrict__ part_score,
-    int* __restrict__ part_count, MiArgs mi)
+    int* __restrict__ part_count, MiArgs mi, float* __restrict__ vf)
 {
+    // what this kind computes
+    constexpr bool MI = KIND == 2;
+    constexpr bool VF_OUT = KIND == 4;          // the displacement field only
+    constexpr bool VF_IN = KIND == 5;           // the displacement read from vf
+    constexpr bool DERIV = KIND == 1 || KIND == 5;  // analytic trilinear derivative
+    constexpr bool GRAD = KIND <= 2 || KIND == 5;   // dc/dv and the condense
+    constexpr bool SCORE = !MI && !VF_OUT;          // squared differences, overlap count
+    constexpr int MODE = VF_OUT ? DISPLACEMENT : (VF_IN ? OFFSETS : PROJECTED);
     const int npairs = BATCH ? q.npairs : 1;
+
     extern __shared__ float4 sm4[];
     __shared__ double s_ps[MAX_BATCH];      // per-pair sums of this workgroup
     __shared__ int s_pn[MAX_BATCH];
     const int t = threadIdx.x;
-    if (KIND < 2 && t < npairs) {
+    if (SCORE && t < npairs) {
         s_ps[t] = 0.0;
         s_pn[t] = 0;
     }
@@ -673,13 +697,13 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
     const TileLds L = carve (reinterpret_cast<float*> (sm4), vx, vy, vz, slots);
     if (tile < tend) {
     tile_luts (L, S, blut);
-    if (KIND == 2) {
+    if (MI) {
         const int nj = mi.h.fbins * mi.h.mbins;
         for (int i = t; i < nj; i += TILE_THREADS) L.tab[i] = mi.table[i];
     }
-    constexpr int ES = KIND == 2 ? 1 : Img<TI>::ES;   // bytes per voxel of what is loaded
+    constexpr int ES = MI ? 1 : Img<TI>::ES;    // bytes per voxel of what is loaded
     float N = 1.f;
-    if (KIND == 2) N = *mi.nvox_f;
+    if (MI) N = *mi.nvox_f;
 
     // This thread's first-batch voxel slots t + 256u (x fastest), the same
     // in every tile, packed as x | y << 10 | z << 20 (each below 1024).
@@ -694,29 +718,35 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
     }
 
     // Fetched one tile ahead: the knot coefficients (threads < 64) and the
-    // first batch's fixed voxels (or fixed bins); voxels past the image read 0.
+    // first batch's fixed voxels (fixed bins for MI, stored displacements
+    // for VF_IN); voxels past the image read 0.
     int p = advance_pair (pairs, npairs, 0, tile);
     int np = p;                             // pair of the tile fetched ahead
     TilePos nq = tile_pos (pairs[p].g, tile - pairs[p].tile0);
     float cpre[3] = {0.f, 0.f, 0.f};
     float fpre[G];
     int fbpre[G];
-    {
-        const KGeom& g = pairs[p].g;
-        const float* cf = (coeff0 && p == 0) ? coeff0 : pairs[p].coeff;
-        if (t < 64) tile_coef_fetch (cpre, g, nq.knot0, t, cf);
-        const int fd0 = g.fdim[0], fd01 = g.fdim[0] * g.fdim[1];
-        const int fb = nq.bx0 + fd0 * nq.by0 + fd01 * nq.bz0;
-        const Rsrc fr = make_rsrc (KIND == 2 ? (const void*) mi.fbin : pairs[p].fimg,
-            (uint32_t) (fd01 * g.fdim[2]) * ES);
+    float dpre[G][3];
+    // the voxel data of the first batch of tile position q of pair pp
+    auto fetch_first = [&] (int pp, const TilePos& qq) {
+        const KGeom& gg = pairs[pp].g;
+        const int d0 = gg.fdim[0], d01 = gg.fdim[0] * gg.fdim[1];
+        const int fb = qq.bx0 + d0 * qq.by0 + d01 * qq.bz0;
+        const uint32_t nf = (uint32_t) (d01 * gg.fdim[2]);
+        const Rsrc fr = make_rsrc (MI ? (const void*) mi.fbin : pairs[pp].fimg, nf * ES);
+        const Rsrc vr = make_rsrc (vf, VF_IN ? nf * 12u : 0u);
 #pragma unroll
         for (int u = 0; u < G; u++) {
             const int x = sxyz[u] & 1023, y = (sxyz[u] >> 10) & 1023, z = sxyz[u] >> 20;
-            const int fo = fb + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
-            if (KIND < 2) fpre[u] = Img<TI>::one (fr, fo * ES, 0);
-            else fbpre[u] = load_u8 (fr, fo, 0);
+            const int fo = fb + x + (int) __umul24 (d0, y) + (int) __umul24 (d01, z);
+            if (SCORE) fpre[u] = Img<TI>::one (fr, fo * ES, 0);
+            if (MI) fbpre[u] = load_u8 (fr, fo, 0);
+            if (VF_IN) Img<float>::vec3 (dpre[u], vr, fo * 12, 0);
         }
-    }
+    };
+    if (MODE != OFFSETS && t < 64)
+        tile_coef_fetch (cpre, pairs[p].g, nq.knot0, t, (coeff0 && p == 0) ? coeff0 : pairs[p].coeff);
+    fetch_first (p, nq);
 
     // this thread's squared differences and overlap count over the tiles
     // of the current pair, folded into s_ps/s_pn when the pair changes
@@ -724,7 +754,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
     int count = 0;
     for (; tile < tend; tile += wpx) {
         const TilePos P = nq;
-        if (KIND < 2 && np != p) {
+        if (SCORE && np != p) {
             // the workgroup moved on to the next pair: fold the last one
             block_sum (L, score, count);
             if (t == 0) {
@@ -752,15 +782,16 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
 #pragma unroll
         for (int u = 0; u < G; u++) asm volatile ("" : "+v" (sxyz[u]));
         __syncthreads ();       // the previous tile is done with coef, t1, s1, s2, red
-        tile_stage1 (L, S, g, P.bz0, cpre, tt);
+        tile_stage1<V, MODE> (L, S, g, P.bz0, cpre, tt);
         __syncthreads ();
-        if (tt < 64 && has_next) {
-            const float* cf = (coeff0 && np == 0) ? coeff0 : pairs[np].coeff;
-            tile_coef_fetch (cpre, pairs[np].g, nq.knot0, tt, cf);
+        if (MODE != OFFSETS && tt < 64 && has_next)
+            tile_coef_fetch (cpre, pairs[np].g, nq.knot0, tt,
+                (coeff0 && np == 0) ? coeff0 : pairs[np].coeff);
+        if (MODE != OFFSETS) {
+            tile_t1 (L, S, tt);
+            __syncthreads ();
         }
-        tile_t1 (L, S, tt);
-        __syncthreads ();
-        tile_columns (L, S, g, P.bx0, P.by0, tt);
+        tile_columns<V, MODE> (L, S, g, P.bx0, P.by0, tt);
         __syncthreads ();
 
         // Stage 4: the voxels.  Thread t takes voxels t, t + 256, ... of
@@ -774,11 +805,12 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         const int fd0 = g.fdim[0], fd01 = g.fdim[0] * g.fdim[1];
         const int md0 = g.mdim[0], md01 = g.mdim[0] * g.mdim[1];
         const int fbase = P.bx0 + fd0 * P.by0 + fd01 * P.bz0;
-        const Rsrc fr = make_rsrc (KIND == 2 ? (const void*) mi.fbin : pd.fimg,
-            (uint32_t) (fd01 * g.fdim[2]) * ES);
-        const Rsrc mr = make_rsrc (KIND == 2 ? (const void*) mi.mbin : pd.mimg,
+        const uint32_t nfix = (uint32_t) (fd01 * g.fdim[2]);
+        const Rsrc fr = make_rsrc (MI ? (const void*) mi.fbin : pd.fimg, nfix * ES);
+        const Rsrc mr = make_rsrc (MI ? (const void*) mi.mbin : pd.mimg,
             (uint32_t) (md01 * g.mdim[2]) * ES);
         const Rsrc gr = make_rsrc (pd.mgrad, KIND == 0 ? (uint32_t) (md01 * g.mdim[2]) * 3 * ES : 0u);
+        const Rsrc vr = make_rsrc (vf, VF_IN ? nfix * 12u : 0u);
         const int SX = md0 * ES, SXY = md01 * ES;
 
 #pragma unroll
@@ -794,12 +826,14 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
 #pragma unroll
             for (int u = 0; u < G; u++) {
                 int x, y, z;
+                float dd[3] = {0.f, 0.f, 0.f};
                 if (first) {
                     x = sxyz[u] & 1023;
                     y = (sxyz[u] >> 10) & 1023;
                     z = sxyz[u] >> 20;
-                    if (KIND < 2) fval[u] = fpre[u];
-                    else fbv[u] = fbpre[u];
+                    if (SCORE) fval[u] = fpre[u];
+                    if (MI) fbv[u] = fbpre[u];
+                    if (VF_IN) { dd[0] = dpre[u][0]; dd[1] = dpre[u][1]; dd[2] = dpre[u][2]; }
                 } else {
                     const int v = base + u * TILE_THREADS + tt;
                     z = S.div_xy (v);
@@ -807,8 +841,9 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     y = S.div_x (r);
                     x = r - y * vx;
                     const int fofs = fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
-                    if (KIND < 2) fval[u] = Img<TI>::one (fr, fofs * ES, 0);
-                    else fbv[u] = load_u8 (fr, fofs, 0);
+                    if (SCORE) fval[u] = Img<TI>::one (fr, fofs * ES, 0);
+                    if (MI) fbv[u] = load_u8 (fr, fofs, 0);
+                    if (VF_IN) Img<float>::vec3 (dd, vr, fofs * 12, 0);
                 }
                 const bool act = x < ex && y < ey && z < ez;
                 const int r = x + (int) __umul24 (vx, y);
@@ -818,11 +853,34 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                 const float4* zp = reinterpret_cast<const float4*> (&L.bzz[8 * zc]);
                 const float4 b = zp[0], Z = zp[1];
                 const int ncol = vx * vy;
-                const float4 A0 = cp[0], A1 = cp[ncol], A2 = cp[2 * ncol], C = cp[3 * ncol];
+                const float4 C = cp[3 * ncol];
                 float m[3];
-                m[0] = fmaf (b.w, A0.w, fmaf (b.z, A0.z, fmaf (b.y, A0.y, fmaf (b.x, A0.x, C.x)))) + Z.x;
-                m[1] = fmaf (b.w, A1.w, fmaf (b.z, A1.z, fmaf (b.y, A1.y, fmaf (b.x, A1.x, C.y)))) + Z.y;
-                m[2] = fmaf (b.w, A2.w, fmaf (b.z, A2.z, fmaf (b.y, A2.y, fmaf (b.x, A2.x, C.z)))) + Z.z;
+                if (VF_IN) {
+                    // P * (position + d - moving origin), regrouped as for
+                    // the interpolated displacement
+                    const float* Pm = g.mproj;
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        m[k] = ((k == 0 ? C.x : k == 1 ? C.y : C.z) + (k == 0 ? Z.x : k == 1 ? Z.y : Z.z))
+                            + fmaf (Pm[3 * k + 2], dd[2], fmaf (Pm[3 * k + 1], dd[1], Pm[3 * k] * dd[0]));
+                } else {
+                    const float4 A0 = cp[0], A1 = cp[ncol], A2 = cp[2 * ncol];
+                    m[0] = fmaf (b.w, A0.w, fmaf (b.z, A0.z, fmaf (b.y, A0.y, fmaf (b.x, A0.x, C.x)))) + Z.x;
+                    m[1] = fmaf (b.w, A1.w, fmaf (b.z, A1.z, fmaf (b.y, A1.y, fmaf (b.x, A1.x, C.y)))) + Z.y;
+                    m[2] = fmaf (b.w, A2.w, fmaf (b.z, A2.z, fmaf (b.y, A2.y, fmaf (b.x, A2.x, C.z)))) + Z.z;
+                }
+                if (VF_OUT) {
+                    // bspline_interpolate_vf (src/base/bspline_interpolate.cxx:112-142):
+                    // the ROI voxels of the field get the displacement in mm
+                    if (act) {
+                        const int fofs = fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
+                        float* o = vf + 3 * (size_t) fofs;
+                        o[0] = m[0];
+                        o[1] = m[1];
+                        o[2] = m[2];
+                    }
+                    continue;
+                }
 
                 // Volume::is_inside (src/base/volume.cxx:899-906) and
                 // li_clamp (src/base/interpolate.cxx:17-41), branch free:
@@ -845,7 +903,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                 in[u] = inside && act;
                 // byte offsets as 24-bit products (slices hold < 2^24 / 12
                 // voxels, checked at context creation)
-                if (KIND < 2) {
+                if (!MI) {
                     const int off = (int) __umul24 (f[0], ES) + (int) __umul24 (f[1], SX)
                         + (int) __umul24 (f[2], SXY);
                     Img<TI>::pair (a[u][0], a[u][1], mr, off, 0);
@@ -871,16 +929,17 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     Img<TI>::vec3 (mg[u], gr, goff, 0);
                 }
             }
+            if (!VF_OUT) {
             float bsum = 0.f;       // this batch's squared differences (<= G terms)
 #pragma unroll
             for (int u = 0; u < G; u++) {
                 const int v = base + u * TILE_THREADS + tt;
-                float dc0, dc1, dc2;
-                if (KIND < 2) {
+                float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f;
+                if (!MI) {
                     Sample s;
                     s.l2[0] = fr_[u][0]; s.l2[1] = fr_[u][1]; s.l2[2] = fr_[u][2];
                     float md[3];
-                    const float m_val = trilinear<KIND == 1> (s, a[u], g.inv_msp, md);
+                    const float m_val = trilinear<DERIV> (s, a[u], g.inv_msp, md);
                     const float diff = in[u] ? m_val - fval[u] : 0.f;
                     bsum += diff * diff;
                     count += in[u] ? 1 : 0;
@@ -888,7 +947,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                         dc0 = diff * mg[u][0];
                         dc1 = diff * mg[u][1];
                         dc2 = diff * mg[u][2];
-                    } else {
+                    } else if (DERIV) {
                         dc0 = diff * md[0];
                         dc1 = diff * md[1];
                         dc2 = diff * md[2];
@@ -929,34 +988,24 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     dc2 = dc0 * Pm[6] + dc1 * Pm[7] + dc2 * Pm[8];
                     if (!in[u]) { dc0 = 0.f; dc1 = 0.f; dc2 = 0.f; }
                 }
-                // every slot below tile_slots is in LDS; slots past the tile
-                // hold zeros and are never read
-                L.dv[v] = dc0;
-                L.dv[L.dvp + v] = dc1;
-                L.dv[2 * L.dvp + v] = dc2;
-            }
-            score += (double) bsum;
-            if (first && has_next) {
-                // the next tile's first-batch fixed voxels, now that this
-                // batch's are consumed
-                const KGeom& gn = pairs[np].g;
-                const int nd0 = gn.fdim[0], nd01 = gn.fdim[0] * gn.fdim[1];
-                const int fbn = nq.bx0 + nd0 * nq.by0 + nd01 * nq.bz0;
-                const Rsrc frn = make_rsrc (KIND == 2 ? (const void*) mi.fbin : pairs[np].fimg,
-                    (uint32_t) (nd01 * gn.fdim[2]) * ES);
-#pragma unroll
-                for (int u = 0; u < G; u++) {
-                    const int x = sxyz[u] & 1023, y = (sxyz[u] >> 10) & 1023, z = sxyz[u] >> 20;
-                    const int fo = fbn + x + (int) __umul24 (nd0, y) + (int) __umul24 (nd01, z);
-                    if (KIND < 2) fpre[u] = Img<TI>::one (frn, fo * ES, 0);
-                    else fbpre[u] = load_u8 (frn, fo, 0);
+                if (GRAD) {
+                    // every slot below tile_slots is in LDS; slots past the
+                    // tile hold zeros and are never read
+                    L.dv[v] = dc0;
+                    L.dv[L.dvp + v] = dc1;
+                    L.dv[2 * L.dvp + v] = dc2;
                 }
             }
+            score += (double) bsum;
+            }
+            if (first && has_next) fetch_first (np, nq);
         }
-        __syncthreads ();
-        tile_condense (L, S, g, P.knot0, pd.cond, tt);
+        if (GRAD) {
+            __syncthreads ();
+            tile_condense (L, S, g, P.knot0, pd.cond, tt);
+        }
     }
-    if (KIND < 2) {
+    if (SCORE) {
         // fold the last pair: each thread's tiles in their fixed order, then
         // the threads in a fixed order
         block_sum (L, score, count);
@@ -968,7 +1017,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
     }
     // one partial per (pair, workgroup); pairs this workgroup did not touch
     // get zeros
-    if (KIND < 2) {
+    if (SCORE) {
         __syncthreads ();
         if (t < npairs) {
             part_score[t * gridDim.x + blockIdx.x] = s_ps[t];
@@ -1009,7 +1058,7 @@ wave_sum_ll (long long v)
 template <bool MSE>
 __global__ __launch_bounds__ (REDUCE_THREADS)
 void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __restrict__ part_score,
-    const int* __restrict__ part_count, int nparts, EvalOut out0)
+    const int* __restrict__ part_count, int nparts, EvalOut out0, int with_grad)
 {
     __shared__ long long s_n[REDUCE_THREADS / 64];
     __shared__ double s_s[REDUCE_THREADS / 64];
@@ -1024,7 +1073,7 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
     const bool lead = blk == 0;
     const int nc = 3 * pd.g.num_knots;
     const int i = blk * REDUCE_THREADS + threadIdx.x;
-    const bool own = i < nc;
+    const bool own = with_grad && i < nc;
     const float* c = pd.cond + (own ? i : 0);
     double* grad = (out0.grad && p == 0) ? out0.grad : pd.grad;
     // all 64 slot loads are issued before any is consumed (a deep enough
@@ -1032,7 +1081,7 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
     // order; the pair's per-workgroup partials are fetched meanwhile
     float v[64];
 #pragma unroll
-    for (int m = 0; m < 64; m++) v[m] = c[(size_t) m * nc];
+    for (int m = 0; m < 64; m++) v[m] = own ? c[(size_t) m * nc] : 0.f;
     long long n = 0;
     double sc = 0.0;
     if (MSE) {
@@ -1427,6 +1476,13 @@ mark (hipEvent_t* ev, int i, hipStream_t st)
         if (e_ != hipSuccess) return e_;                                \
     } while (0)
 
+// What a tile-kernel launch reads besides the plan.
+struct TileArgs {
+    const float* coeff0;    // pair 0's coefficients (single-pair calls), or null
+    MiArgs mi;              // MI pass 2
+    float* vf;              // pair 0's displacement field (KIND 4 out, KIND 5 in)
+};
+
 // Persistent grid: as many workgroups as fit on the device at once
 // (occupancy from the kernel's registers and this launch's LDS), a
 // multiple of 8 so every XCD gets the same count, and no more than an XCD
@@ -1434,7 +1490,7 @@ mark (hipEvent_t* ev, int i, hipStream_t st)
 template <int KIND, typename TI, int V, int G, bool BATCH>
 static int
 launch_tiles_vgb (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st,
-    const float* coeff0, const MiArgs& mi)
+    const TileArgs& ta)
 {
     static std::mutex mu;
     static size_t cached_lds = 0;
@@ -1460,37 +1516,39 @@ launch_tiles_vgb (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st
     const int cus = tune.num_cus > 0 ? tune.num_cus : 8;
     const int per_xcd = std::max (1, std::min (q.tiles_per_xcd, occ * cus / 8));
     hipLaunchKernelGGL ((k_tiles<KIND, TI, V, G, BATCH>), dim3 (8 * per_xcd), dim3 (TILE_THREADS),
-        lds, st, plan.d_pairs, q, coeff0, plan.blut, plan.part_score, plan.part_count, mi);
+        lds, st, plan.d_pairs, q, ta.coeff0, plan.blut, plan.part_score, plan.part_count, ta.mi,
+        ta.vf);
     return 8 * per_xcd;
 }
 
+// Batched launches exist for the two MSE gradient kinds; everything else
+// runs on single-pair plans.
 template <int KIND, typename TI, int V, int G>
 static int
 launch_tiles_vg (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st,
-    const float* coeff0, const MiArgs& mi)
+    const TileArgs& ta)
 {
-    // MI is single-pair only
     if (KIND < 2 && plan.npairs > 1)
-        return launch_tiles_vgb<KIND, TI, V, G, (KIND < 2)> (plan, tune, lds, st, coeff0, mi);
-    return launch_tiles_vgb<KIND, TI, V, G, false> (plan, tune, lds, st, coeff0, mi);
+        return launch_tiles_vgb<KIND, TI, V, G, (KIND < 2)> (plan, tune, lds, st, ta);
+    return launch_tiles_vgb<KIND, TI, V, G, false> (plan, tune, lds, st, ta);
 }
 
 template <int KIND, typename TI, int V>
 static int
 launch_tiles_v (const Plan& plan, const Tune& tune, int vpt, size_t extra, hipStream_t st,
-    const float* coeff0, const MiArgs& mi)
+    const TileArgs& ta)
 {
     const KGeom& g = plan.g0;
     // dynamic LDS: the carve-up (+ the MI table), without the static arrays
     constexpr size_t stat = MAX_BATCH * (sizeof (double) + sizeof (int));
     if (vpt == 1)
         return launch_tiles_vg<KIND, TI, V, 1> (plan, tune, tile_lds_bytes (g, 1) - stat + extra,
-            st, coeff0, mi);
+            st, ta);
     if (vpt == 2)
         return launch_tiles_vg<KIND, TI, V, 2> (plan, tune, tile_lds_bytes (g, 2) - stat + extra,
-            st, coeff0, mi);
+            st, ta);
     return launch_tiles_vg<KIND, TI, V, 4> (plan, tune, tile_lds_bytes (g, 4) - stat + extra,
-        st, coeff0, mi);
+        st, ta);
 }
 
 // The tile kernel is specialized at compile time for the isotropic tile
@@ -1503,14 +1561,14 @@ launch_tiles_v (const Plan& plan, const Tune& tune, int vpt, size_t extra, hipSt
 template <int KIND, typename TI>
 static int
 launch_tiles (const Plan& plan, const Tune& tune, size_t extra, hipStream_t st,
-    const float* coeff0, const MiArgs& mi)
+    const TileArgs& ta)
 {
     const KGeom& g = plan.g0;
     int vpt = tune.vox_per_thread;
     while (vpt > 1 && tile_lds_bytes (g, vpt) + extra > tune.lds_limit) vpt >>= 1;
     const int v = g.vpr[0];
     const bool iso = g.vpr[1] == v && g.vpr[2] == v;
-#define PLM_TILES(VV) return launch_tiles_v<KIND, TI, VV> (plan, tune, vpt, extra, st, coeff0, mi)
+#define PLM_TILES(VV) return launch_tiles_v<KIND, TI, VV> (plan, tune, vpt, extra, st, ta)
     if (iso && v == 8) PLM_TILES (8);
     if (iso && v == 10) PLM_TILES (10);
     if (iso && v == 12) PLM_TILES (12);
@@ -1519,25 +1577,61 @@ launch_tiles (const Plan& plan, const Tune& tune, size_t extra, hipStream_t st,
 #undef PLM_TILES
 }
 
+template <int KIND>
+static int
+launch_tiles_storage (const Plan& plan, const Tune& tune, int storage, hipStream_t st,
+    const TileArgs& ta)
+{
+    if (storage == 0) return launch_tiles<KIND, float> (plan, tune, 0, st, ta);
+    return launch_tiles<KIND, uint16_t> (plan, tune, 0, st, ta);
+}
+
 hipError_t
 launch_mse (const Plan& plan, const Tune& tune, int mode, int storage, const float* coeff0,
     const EvalOut& out0, hipStream_t st, hipEvent_t* ev)
 {
-    MiArgs mi {};
+    const TileArgs ta {coeff0, MiArgs {}, nullptr};
     hipError_t e = mark (ev, 0, st);
     if (e != hipSuccess) return e;
     int nparts;
-    if (storage == 0) {
-        if (mode == 0) nparts = launch_tiles<0, float> (plan, tune, 0, st, coeff0, mi);
-        else nparts = launch_tiles<1, float> (plan, tune, 0, st, coeff0, mi);
-    } else {
-        if (mode == 0) nparts = launch_tiles<0, uint16_t> (plan, tune, 0, st, coeff0, mi);
-        else nparts = launch_tiles<1, uint16_t> (plan, tune, 0, st, coeff0, mi);
-    }
+    if (mode == MSE_REF_NN) nparts = launch_tiles_storage<0> (plan, tune, storage, st, ta);
+    else if (mode == MSE_EXACT) nparts = launch_tiles_storage<1> (plan, tune, storage, st, ta);
+    else nparts = launch_tiles_storage<3> (plan, tune, storage, st, ta);
+    LAUNCH_CHECK ();
+    if ((e = mark (ev, 1, st)) != hipSuccess) return e;
+    // the score alone needs one reduce workgroup per pair (single-pair plans)
+    const bool grad = mode != MSE_SCORE_ONLY;
+    hipLaunchKernelGGL (k_reduce<true>, dim3 (grad ? plan.reduce_blocks : 1), dim3 (REDUCE_THREADS),
+        0, st, plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts, out0,
+        grad ? 1 : 0);
+    LAUNCH_CHECK ();
+    return mark (ev, 2, st);
+}
+
+hipError_t
+launch_vf (const Plan& plan, const Tune& tune, const float* coeff0, float* vf, hipStream_t st)
+{
+    const TileArgs ta {coeff0, MiArgs {}, vf};
+    // the field covers the whole fixed image; voxels outside the ROI stay 0
+    const KGeom& g = plan.g0;
+    hipError_t e = launch_zero (vf, 3 * (int64_t) g.fdim[0] * g.fdim[1] * g.fdim[2], st);
+    if (e != hipSuccess) return e;
+    launch_tiles<4, float> (plan, tune, 0, st, ta);
+    return hipGetLastError ();
+}
+
+hipError_t
+launch_mse_from_vf (const Plan& plan, const Tune& tune, int storage, const float* vf,
+    const EvalOut& out0, hipStream_t st, hipEvent_t* ev)
+{
+    const TileArgs ta {nullptr, MiArgs {}, const_cast<float*> (vf)};
+    hipError_t e = mark (ev, 0, st);
+    if (e != hipSuccess) return e;
+    const int nparts = launch_tiles_storage<5> (plan, tune, storage, st, ta);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_reduce<true>, dim3 (plan.reduce_blocks), dim3 (REDUCE_THREADS), 0, st,
-        plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts, out0);
+        plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts, out0, 1);
     LAUNCH_CHECK ();
     return mark (ev, 2, st);
 }
@@ -1571,13 +1665,13 @@ launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
         out0.score, out0.nvox, nvox_f, hist_out);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 3, st)) != hipSuccess) return e;
-    MiArgs mi {h, table, nvox_f, fbin, mbin};
+    const TileArgs ta {coeff, MiArgs {h, table, nvox_f, fbin, mbin}, nullptr};
     // pass 2 reads only bin images; the image type parameter is unused
-    launch_tiles<2, float> (plan, tune, table_bytes, st, coeff, mi);
+    launch_tiles<2, float> (plan, tune, table_bytes, st, ta);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 4, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_reduce<false>, dim3 (plan.reduce_blocks), dim3 (REDUCE_THREADS), 0, st,
-        plan.d_pairs, plan.npairs, nullptr, nullptr, 0, out0);
+        plan.d_pairs, plan.npairs, nullptr, nullptr, 0, out0, 1);
     LAUNCH_CHECK ();
     return mark (ev, 5, st);
 }
