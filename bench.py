@@ -17,7 +17,10 @@ pairs with edges drawn from 96..256 per axis (10-voxel grid, N(0, 3 mm)
 fields), images and moving gradient stored as BF16, evaluated (a) as one
 batched evaluation replayed from a hipGraph and (b) by looping the
 single-pair entry point over the 16 pairs on one stream; both report
-aggregate pair-evaluations per second.
+aggregate pair-evaluations per second.  The "adjoint" object covers the
+differentiable path (torch.autograd.Function) on 192x192x96 with an 8-voxel
+grid: forward+backward time and memory of its two modes and a central
+finite-difference check of the adjoint.
 """
 import argparse
 import json
@@ -190,6 +193,88 @@ def batched_leg(args, dev, want_errors):
     return out
 
 
+ADJOINT_WORKLOAD = "adjoint_192x192x96_vpr8"
+FD_COMPONENTS = 200
+FD_STEP_MM = 1e-3
+
+
+def adjoint_leg(args, dev, want_errors):
+    """The differentiable MSE (torch.autograd.Function) on 192x192x96, vpr 8:
+    forward+backward time and extra memory of the recompute-in-backward and
+    stored-field modes, and the adjoint against central finite differences
+    of the GPU score on 200 random coefficients (and, in the CPU leg, against
+    the reference flavor's gradient)."""
+    import torch
+    from plastimatch_amd import synth
+    from plastimatch_amd.autograd import mse
+    from plastimatch_amd.bspline import BsplineHip, Bspline_xform, Volume
+
+    pair = synth.make_config(ADJOINT_WORKLOAD, seed=0)
+    fixed = Volume(torch.from_numpy(pair.fixed).to(dev), pair.spacing, pair.origin)
+    moving = Volume(torch.from_numpy(pair.moving).to(dev), pair.spacing, pair.origin)
+    bxf = Bspline_xform.for_volume(fixed, pair.vox_per_rgn)
+    st = BsplineHip(bxf, fixed, moving)
+    c = synth.random_coeff(pair.dim, pair.vox_per_rgn, 2.0, seed=300)
+    steps = max(10, args.steps // 10)
+    out = {"workload": ADJOINT_WORKLOAD, "dim": list(pair.dim), "vox_per_rgn": list(pair.vox_per_rgn),
+           "grad_semantics": "exact trilinear derivative (flavor p)"}
+    grads = {}
+    for mode in ("recompute", "stored"):
+        coeff = torch.from_numpy(c).to(dev).requires_grad_(True)
+
+        def fwd_bwd():
+            coeff.grad = None
+            s = mse(coeff, st, mode)
+            s.backward()
+            return s
+
+        for _ in range(3):
+            fwd_bwd()
+        torch.cuda.synchronize()
+        base_mem = torch.cuda.memory_allocated(dev)
+        torch.cuda.reset_peak_memory_stats(dev)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s = fwd_bwd()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        peak = torch.cuda.max_memory_allocated(dev) - base_mem
+        grads[mode] = (float(s.item()), coeff.grad.detach().cpu().numpy().astype(np.float64))
+        out[mode] = {"ms_per_forward_backward": dt * 1e3, "extra_device_bytes": int(peak)}
+    out["stored_field_bytes"] = 12 * pair.fixed.size
+    g = grads["recompute"][1]
+    out["stored_vs_recompute_grad_rel_l2"] = float(np.linalg.norm(grads["stored"][1] - g) / np.linalg.norm(g))
+    # central differences of the GPU score on random coefficients
+    rng = np.random.default_rng(301)
+    idx = rng.choice(c.size, FD_COMPONENTS, replace=False)
+    base = torch.from_numpy(c).to(dev)
+    fd = np.empty(FD_COMPONENTS)
+    with torch.no_grad():
+        for n, i in enumerate(idx):
+            a = base.clone()
+            a[i] += FD_STEP_MM
+            b = base.clone()
+            b[i] -= FD_STEP_MM
+            fd[n] = (mse(a, st).item() - mse(b, st).item()) / (2 * FD_STEP_MM)
+    out["fd_check"] = {"components": FD_COMPONENTS, "h_mm": FD_STEP_MM,
+                       "rel_l2": float(np.linalg.norm(g[idx] - fd) / np.linalg.norm(fd)),
+                       "median_abs_rel": float(np.median(np.abs(g[idx] - fd) / np.maximum(np.abs(fd), 1e-30)))}
+    if want_errors:
+        import oracle
+        threads = min(16, len(os.sched_getaffinity(0)))
+        case = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn)
+        s_ref, g_ref, _, _ = case.mse(c, "p", nthreads=threads)
+        s64, g64, _ = case.mse_f64(c, 1, nthreads=threads)
+        sc = grads["recompute"][0]
+        out["rel_err_vs_reference_flavor_p"] = {
+            "score": abs(sc - s_ref) / abs(s_ref),
+            "grad_l2": float(np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref))}
+        out["rel_err_vs_fp64"] = {"score": abs(sc - s64) / abs(s64),
+                                  "grad_l2": float(np.linalg.norm(g - g64) / np.linalg.norm(g64))}
+    st.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +282,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-atlas leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-path leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy kernel 10 times "
                          "(reference for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -305,6 +391,8 @@ def main():
         out["rel_err_vs_fp64"] = {"score": max(e[0] for e in errs), "grad_l2": max(e[1] for e in errs)}
     if not args.no_batch:
         out["batched"] = batched_leg(args, dev, want_errors=not args.no_cpu)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(args, dev, want_errors=not args.no_cpu)
     print(json.dumps(out))
 
 
