@@ -260,10 +260,13 @@ upload_image (plm_hip_ctx* c, const plm_img_desc* d, int64_t nvals, void** out)
         *out = tmp;
         return PLM_HIP_OK;
     }
+    // 16 bytes of tail padding: BF16 pairs and gradient samples are read as
+    // 64-bit words from the dword at or below them
     uint16_t* bf = nullptr;
-    rc = alloc (&bf, nvals, "hipMalloc(bf16 image)");
+    rc = alloc (&bf, nvals + 8, "hipMalloc(bf16 image)");
     if (rc) { (void) hipFree (tmp); return rc; }
     e = plmhip::launch_to_bf16 (tmp, bf, nvals, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync (bf + nvals, 0, 16, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize (c->stream);
     (void) hipFree (tmp);
     if (e != hipSuccess) { (void) hipFree (bf); return hip_fail (e, "bf16 conversion"); }

@@ -260,24 +260,53 @@ template <> struct Img<float> {
     }
 };
 
+// BF16: a corner pair (4 bytes) or a gradient sample (6 bytes) starts on an
+// even byte; one 64-bit load from the dword boundary at or below it always
+// covers it, and a funnel shift brings it down.  The 64-bit load may reach
+// up to 4 bytes past the last element, so BF16 images carry 16 bytes of
+// tail padding and their resources count 8 of them (img_bytes).
 template <> struct Img<uint16_t> {
     static constexpr int ES = 2;
     static __device__ __forceinline__ float one (Rsrc r, int off, int soff)
     {
         return __uint_as_float (((uint32_t) __builtin_amdgcn_raw_buffer_load_b16 (r, off, soff, 0)) << 16);
     }
+    // the 64 bits at byte o - (o & 2), o = off + soff, shifted down by
+    // (o & 2) bytes (the alignment is that of the whole offset, so soff is
+    // folded in)
+    static __device__ __forceinline__ void load_shifted (uint32_t& lo, uint32_t& hi, Rsrc r,
+        int off, int soff)
+    {
+        const int o = off + soff;
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64 (r, o & ~3, 0, 0);
+        const uint32_t sh = (uint32_t) (o & 2) << 3;
+        lo = __builtin_amdgcn_alignbit (v[1], v[0], sh);
+        hi = v[1] >> sh;
+    }
     static __device__ __forceinline__ void pair (float& a, float& b, Rsrc r, int off, int soff)
     {
-        a = one (r, off, soff);
-        b = one (r, off + 2, soff);
+        uint32_t lo, hi;
+        load_shifted (lo, hi, r, off, soff);
+        a = __uint_as_float (lo << 16);
+        b = __uint_as_float (lo & 0xffff0000u);
     }
     static __device__ __forceinline__ void vec3 (float o[3], Rsrc r, int off, int soff)
     {
-        o[0] = one (r, off, soff);
-        o[1] = one (r, off + 2, soff);
-        o[2] = one (r, off + 4, soff);
+        uint32_t lo, hi;
+        load_shifted (lo, hi, r, off, soff);
+        o[0] = __uint_as_float (lo << 16);
+        o[1] = __uint_as_float (lo & 0xffff0000u);
+        o[2] = __uint_as_float (hi << 16);
     }
 };
+
+// Byte size of a resource over n elements of es bytes (BF16 images count
+// part of their tail padding, see Img<uint16_t>).
+__device__ __forceinline__ uint32_t
+img_bytes (uint32_t n, int es)
+{
+    return n * (uint32_t) es + (es == 2 ? 8u : 0u);
+}
 
 // n / D for 0 <= n < 4096 and D < 256 with one 24-bit multiply:
 // floor(n * ceil(2^20 / D) / 2^20) is exact there (the excess is below
@@ -733,7 +762,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         const int d0 = gg.fdim[0], d01 = gg.fdim[0] * gg.fdim[1];
         const int fb = qq.bx0 + d0 * qq.by0 + d01 * qq.bz0;
         const uint32_t nf = (uint32_t) (d01 * gg.fdim[2]);
-        const Rsrc fr = make_rsrc (MI ? (const void*) mi.fbin : pairs[pp].fimg, nf * ES);
+        const Rsrc fr = make_rsrc (MI ? (const void*) mi.fbin : pairs[pp].fimg, img_bytes (nf, ES));
         const Rsrc vr = make_rsrc (vf, VF_IN ? nf * 12u : 0u);
 #pragma unroll
         for (int u = 0; u < G; u++) {
@@ -806,10 +835,11 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         const int md0 = g.mdim[0], md01 = g.mdim[0] * g.mdim[1];
         const int fbase = P.bx0 + fd0 * P.by0 + fd01 * P.bz0;
         const uint32_t nfix = (uint32_t) (fd01 * g.fdim[2]);
-        const Rsrc fr = make_rsrc (MI ? (const void*) mi.fbin : pd.fimg, nfix * ES);
+        const Rsrc fr = make_rsrc (MI ? (const void*) mi.fbin : pd.fimg, img_bytes (nfix, ES));
         const Rsrc mr = make_rsrc (MI ? (const void*) mi.mbin : pd.mimg,
-            (uint32_t) (md01 * g.mdim[2]) * ES);
-        const Rsrc gr = make_rsrc (pd.mgrad, KIND == 0 ? (uint32_t) (md01 * g.mdim[2]) * 3 * ES : 0u);
+            img_bytes ((uint32_t) (md01 * g.mdim[2]), ES));
+        const Rsrc gr = make_rsrc (pd.mgrad,
+            KIND == 0 ? img_bytes ((uint32_t) (md01 * g.mdim[2]) * 3, ES) : 0u);
         const Rsrc vr = make_rsrc (vf, VF_IN ? nfix * 12u : 0u);
         const int SX = md0 * ES, SXY = md01 * ES;
 
@@ -1074,14 +1104,20 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
     const int nc = 3 * pd.g.num_knots;
     const int i = blk * REDUCE_THREADS + threadIdx.x;
     const bool own = with_grad && i < nc;
-    const float* c = pd.cond + (own ? i : 0);
+    const float* c = pd.cond + (i < nc ? i : 0);
     double* grad = (out0.grad && p == 0) ? out0.grad : pd.grad;
     // all 64 slot loads are issued before any is consumed (a deep enough
     // queue to cover the memory latency with few waves), then summed in slot
-    // order; the pair's per-workgroup partials are fetched meanwhile
+    // order; the pair's per-workgroup partials are fetched meanwhile.  The
+    // loads are unconditional (c stays in range) so nothing serializes them.
     float v[64];
+    if (with_grad) {
 #pragma unroll
-    for (int m = 0; m < 64; m++) v[m] = own ? c[(size_t) m * nc] : 0.f;
+        for (int m = 0; m < 64; m++) v[m] = c[(size_t) m * nc];
+    } else {
+#pragma unroll
+        for (int m = 0; m < 64; m++) v[m] = 0.f;
+    }
     long long n = 0;
     double sc = 0.0;
     if (MSE) {
