@@ -32,6 +32,7 @@
 #include <float.h>
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "bspline_kernels.h"
 
@@ -148,16 +149,41 @@ lerp (float a, float b, float t)
     return fmaf (t, b, fmaf (-t, a, a));
 }
 
-// Trilinear value and, optionally, its analytic derivative times
-// 1/spacing (li_value and li_value_dx/dy/dz, src/base/interpolate.cxx:135-244)
-// as nested lerps: the same polynomial in 14 FMAs (+ 17 operations for the
+// Trilinear value and, optionally, its analytic derivative in moving index
+// units (li_value and li_value_dx/dy/dz, src/base/interpolate.cxx:135-244,
+// without their 1/spacing factor, which the condense applies per component)
+// as nested lerps: the same polynomial in 14 FMAs (+ 15 operations for the
 // derivative), rounding within a few ulp of the reference's
 // product-and-sum form and exact on voxel centres and clamped borders.
-template <bool DERIV>
+// SHARED (fractions below 1: no clamped upper border) forms each lerp as
+// a + t (b - a), still exact at t = 0; the differences are those the
+// derivative needs, so value and derivative take 25 operations.  Every MSE
+// kind uses the same form for the same voxels, so their scores agree bit
+// for bit.
+template <bool DERIV, bool SHARED = false>
 __device__ __forceinline__ float
-trilinear (const Sample& s, const float a[8], const float inv[3], float dm[3])
+trilinear (const Sample& s, const float a[8], float dm[3])
 {
     const float fx = s.l2[0], fy = s.l2[1], fz = s.l2[2];
+    if (SHARED) {
+        const float d0 = a[1] - a[0], d1 = a[3] - a[2], d2 = a[5] - a[4], d3 = a[7] - a[6];
+        const float x00 = fmaf (fx, d0, a[0]);
+        const float x10 = fmaf (fx, d1, a[2]);
+        const float x01 = fmaf (fx, d2, a[4]);
+        const float x11 = fmaf (fx, d3, a[6]);
+        const float e0 = x10 - x00, e1 = x11 - x01;
+        const float y0 = fmaf (fy, e0, x00);
+        const float y1 = fmaf (fy, e1, x01);
+        const float f0 = y1 - y0;
+        if (DERIV) {
+            const float gx0 = fmaf (fy, d1 - d0, d0);
+            const float gx1 = fmaf (fy, d3 - d2, d2);
+            dm[0] = fmaf (fz, gx1 - gx0, gx0);
+            dm[1] = fmaf (fz, e1 - e0, e0);
+            dm[2] = f0;
+        }
+        return fmaf (fz, f0, y0);
+    }
     const float x00 = lerp (a[0], a[1], fx);
     const float x10 = lerp (a[2], a[3], fx);
     const float x01 = lerp (a[4], a[5], fx);
@@ -167,9 +193,9 @@ trilinear (const Sample& s, const float a[8], const float inv[3], float dm[3])
     if (DERIV) {
         const float gx0 = lerp (a[1] - a[0], a[3] - a[2], fy);
         const float gx1 = lerp (a[5] - a[4], a[7] - a[6], fy);
-        dm[0] = inv[0] * lerp (gx0, gx1, fz);
-        dm[1] = inv[1] * lerp (x10 - x00, x11 - x01, fz);
-        dm[2] = inv[2] * (y1 - y0);
+        dm[0] = lerp (gx0, gx1, fz);
+        dm[1] = lerp (x10 - x00, x11 - x01, fz);
+        dm[2] = y1 - y0;
     }
     return lerp (y0, y1, fz);
 }
@@ -484,12 +510,20 @@ tile_t1 (const TileLds& L, const Shape<V>& S, int t)
 // which is P * (fixed position + displacement - moving origin) regrouped
 // (bspline_find_correspondence_dcos, src/util/bspline_correspond.cxx:80-109,
 // with the position of POSITION_FROM_COORDS, src/base/volume_macros.h:82-96).
+//
+// Returns whether every column this thread formed stays in the interior of
+// the moving image over all the tile's slices: with bz >= 0, sum_k bz = 1
+// and Z linear in z, mijk lies in C + [min_k A, max_k A] + [Z(first),
+// Z(last)]; 0.01 voxel of margin covers the rounding of mijk.  Inside that
+// range li_clamp's clamps are identities and Volume::is_inside holds.
 template <int V, int MODE>
-__device__ __forceinline__ void
-tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int by0, int t)
+__device__ __forceinline__ bool
+tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int by0, int bz0,
+    int t)
 {
     const int vx = S.vx;
     const float4* t1 = reinterpret_cast<const float4*> (L.t1);
+    bool ok = true;
     for (int e = t; e < vx * S.vy; e += TILE_THREADS) {
         const int y = S.div_x (e), x = e - y * vx;
         // one float4 plane per term, so the voxel pass reads consecutive
@@ -527,8 +561,18 @@ tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int 
                 A[k] = MODE == DISPLACEMENT ? T[a][k]
                     : fmaf (P[3 * a + 2], T[2][k], fmaf (P[3 * a + 1], T[1][k], P[3 * a] * T[0][k]));
             o[a * np] = make_float4 (A[0], A[1], A[2], A[3]);
+            if (MODE == PROJECTED) {
+                const float4 Cv = o[3 * np];
+                const float c = a == 0 ? Cv.x : (a == 1 ? Cv.y : Cv.z);
+                const float za = g.mstep[3 * a + 2] * (float) bz0;
+                const float zb = g.mstep[3 * a + 2] * (float) (bz0 + S.vz - 1);
+                const float lo = c + fminf (za, zb) + fminf (fminf (A[0], A[1]), fminf (A[2], A[3]));
+                const float hi = c + fmaxf (za, zb) + fmaxf (fmaxf (A[0], A[1]), fmaxf (A[2], A[3]));
+                ok = ok & (lo >= 0.01f) & (hi <= g.mmax[a] + 0.99f);
+            }
         }
     }
+    return ok;
 }
 
 // Stages 5-7: condense the tile's parked dc/dv into its 64 knot slots, the
@@ -537,7 +581,10 @@ tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int 
 // slot) entry has exactly one writer, as in bspline_sort_sets
 // (src/register/bspline.cxx:377-398); slot-major keeps the reduce's reads
 // coalesced.
-template <int V>
+//
+// SCALE: multiply component c by g.inv_msp[c] on the way out (the analytic
+// derivative's 1/spacing, left out per voxel).
+template <int V, bool SCALE>
 __device__ __forceinline__ void
 tile_condense (const TileLds& L, const Shape<V>& S, const KGeom& g, int knot0,
     float* __restrict__ cond, int t)
@@ -620,9 +667,10 @@ tile_condense (const TileLds& L, const Shape<V>& S, const KGeom& g, int knot0,
         const int cd0 = g.cdims[0], cd01 = g.cdims[0] * g.cdims[1];
         const int knot = knot0 + i + cd0 * j;
         const size_t plane = (size_t) 3 * g.num_knots;
+        const float sc = SCALE ? g.inv_msp[c] : 1.f;
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            cond[(size_t) (k * 16 + ji) * plane + 3 * (knot + k * cd01) + c] = a[k];
+            cond[(size_t) (k * 16 + ji) * plane + 3 * (knot + k * cd01) + c] = SCALE ? a[k] * sc : a[k];
     }
 }
 
@@ -777,18 +825,21 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         tile_coef_fetch (cpre, pairs[p].g, nq.knot0, t, (coeff0 && p == 0) ? coeff0 : pairs[p].coeff);
     fetch_first (p, nq);
 
-    // this thread's squared differences and overlap count over the tiles
-    // of the current pair, folded into s_ps/s_pn when the pair changes
+    // this thread's squared differences and its wave's overlap count over
+    // the tiles of the current pair, folded into s_ps/s_pn when the pair
+    // changes
     double score = 0.0;
     int count = 0;
+    int rot = 0;
     for (; tile < tend; tile += wpx) {
         const TilePos P = nq;
         if (SCORE && np != p) {
             // the workgroup moved on to the next pair: fold the last one
-            block_sum (L, score, count);
+            int wc = (t & 63) == 0 ? count : 0;
+            block_sum (L, score, wc);
             if (t == 0) {
                 s_ps[p] += score;
-                s_pn[p] += count;
+                s_pn[p] += wc;
             }
             score = 0.0;
             count = 0;
@@ -810,18 +861,41 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         asm volatile ("" : "+v" (tt));
 #pragma unroll
         for (int u = 0; u < G; u++) asm volatile ("" : "+v" (sxyz[u]));
+        // The per-tile stages keep fewer than 256 threads busy, so which
+        // wave leads them rotates from tile to tile (stage index tr = t +
+        // 64 * tile count): over a few tiles every wave, and so every SIMD,
+        // takes the same share.  tn is the next tile's stage index, under
+        // which its coefficients are fetched ahead.
+        const int tr = (tt + rot) & (TILE_THREADS - 1);
+        rot += 64;
+        const int tn = (tt + rot) & (TILE_THREADS - 1);
         __syncthreads ();       // the previous tile is done with coef, t1, s1, s2, red
-        tile_stage1<V, MODE> (L, S, g, P.bz0, cpre, tt);
+        tile_stage1<V, MODE> (L, S, g, P.bz0, cpre, tr);
         __syncthreads ();
-        if (MODE != OFFSETS && tt < 64 && has_next)
-            tile_coef_fetch (cpre, pairs[np].g, nq.knot0, tt,
+        if (MODE != OFFSETS && tn < 64 && has_next)
+            tile_coef_fetch (cpre, pairs[np].g, nq.knot0, tn,
                 (coeff0 && np == 0) ? coeff0 : pairs[np].coeff);
         if (MODE != OFFSETS) {
-            tile_t1 (L, S, tt);
+            tile_t1 (L, S, tr);
             __syncthreads ();
         }
-        tile_columns<V, MODE> (L, S, g, P.bx0, P.by0, tt);
+        // MSE and MI tiles whose every column stays in the moving image's
+        // interior take the voxel pass without clamps or inside tests (the
+        // same results bit for bit): one flag per wave, ANDed after the
+        // barrier (red is free here).
+        constexpr bool FASTABLE = MODE == PROJECTED && !VF_OUT;
+        const bool col_ok = tile_columns<V, MODE> (L, S, g, P.bx0, P.by0, P.bz0, tr);
+        int* wflag = reinterpret_cast<int*> (L.red + 12);
+        if (FASTABLE) {
+            const bool w = __all (col_ok);
+            if ((t & 63) == 0) wflag[t >> 6] = w ? 1 : 0;
+        }
         __syncthreads ();
+        bool fast = false;
+        if (FASTABLE) {
+            const int4 f4 = *reinterpret_cast<const int4*> (wflag);
+            fast = (f4.x & f4.y & f4.z & f4.w) != 0;
+        }
 
         // Stage 4: the voxels.  Thread t takes voxels t, t + 256, ... of
         // the tile, G at a time, issuing all their loads before using any.
@@ -842,7 +916,19 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
             KIND == 0 ? img_bytes ((uint32_t) (md01 * g.mdim[2]) * 3, ES) : 0u);
         const Rsrc vr = make_rsrc (vf, VF_IN ? nfix * 12u : 0u);
         const int SX = md0 * ES, SXY = md01 * ES;
+        // The moving-index bounds, read once per tile into registers: left
+        // in memory, they are reloaded per voxel (each reload a scalar-load
+        // round trip).
+        float mhi[3], mmax[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            mhi[k] = g.mhi[k];
+            mmax[k] = g.mmax[k];
+            asm volatile ("" : "+s" (mhi[k]), "+s" (mmax[k]));
+        }
 
+        auto voxels = [&] (auto fast_c) {
+        constexpr bool FAST = decltype (fast_c)::value;
 #pragma unroll
         for (int base = 0; base < nvox; base += TILE_THREADS * G) {
             const bool first = base == 0;
@@ -923,14 +1009,22 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                 int f[3], c[3];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    inside = inside && m[k] > -0.5f && m[k] < g.mhi[k];
-                    const float fl = __builtin_amdgcn_fmed3f (floorf (m[k]), 0.f, g.mmax[k]);
-                    const float fa = fminf (fmaxf (m[k] - fl, 0.f), 1.f);
+                    float fl, fa;
+                    if (FAST) {
+                        // interior tile: the clamps are identities
+                        fl = floorf (m[k]);
+                        fa = m[k] - fl;
+                    } else {
+                        // bitwise, not short-circuit: no branch per component
+                        inside = inside & (m[k] > -0.5f) & (m[k] < mhi[k]);
+                        fl = __builtin_amdgcn_fmed3f (floorf (m[k]), 0.f, mmax[k]);
+                        fa = fminf (fmaxf (m[k] - fl, 0.f), 1.f);
+                    }
                     f[k] = (int) fl;
                     c[k] = fa >= 0.5f ? 1 : 0;
                     fr_[u][k] = fa;
                 }
-                in[u] = inside && act;
+                in[u] = inside & act;
                 // byte offsets as 24-bit products (slices hold < 2^24 / 12
                 // voxels, checked at context creation)
                 if (!MI) {
@@ -969,10 +1063,16 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     Sample s;
                     s.l2[0] = fr_[u][0]; s.l2[1] = fr_[u][1]; s.l2[2] = fr_[u][2];
                     float md[3];
-                    const float m_val = trilinear<DERIV> (s, a[u], g.inv_msp, md);
-                    const float diff = in[u] ? m_val - fval[u] : 0.f;
+                    float m_val = trilinear<DERIV, FAST> (s, a[u], md);
+                    // Opaque here, so the select below cannot become a
+                    // branch that sinks this voxel's loads (and a wait for
+                    // them) into it.
+                    float fv = fval[u];
+                    asm volatile ("" : "+v" (m_val), "+v" (fv));
+                    const float diff = in[u] ? m_val - fv : 0.f;
                     bsum += diff * diff;
-                    count += in[u] ? 1 : 0;
+                    // the wave's overlap count, on the scalar unit
+                    count += (int) __popcll (__ballot (in[u]));
                     if (KIND == 0) {
                         dc0 = diff * mg[u][0];
                         dc1 = diff * mg[u][1];
@@ -1030,18 +1130,26 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
             }
             if (first && has_next) fetch_first (np, nq);
         }
+        };
+        if constexpr (FASTABLE) {
+            if (fast) voxels (std::true_type {});
+            else voxels (std::false_type {});
+        } else {
+            voxels (std::false_type {});
+        }
         if (GRAD) {
             __syncthreads ();
-            tile_condense (L, S, g, P.knot0, pd.cond, tt);
+            tile_condense<V, DERIV> (L, S, g, P.knot0, pd.cond, tr);
         }
     }
     if (SCORE) {
         // fold the last pair: each thread's tiles in their fixed order, then
         // the threads in a fixed order
-        block_sum (L, score, count);
+        int wc = (t & 63) == 0 ? count : 0;
+        block_sum (L, score, wc);
         if (t == 0) {
             s_ps[p] += score;
-            s_pn[p] += count;
+            s_pn[p] += wc;
         }
     }
     }
