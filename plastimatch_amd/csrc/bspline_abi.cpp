@@ -367,7 +367,9 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
             g.fstep[3*i+j] = fixed->direction_cosines[3*i+j] * fixed->spacing[j];
             g.mproj[3*i+j] = minv[3*i+j] / moving->spacing[i];
         }
-    if (tiles >= (1 << 30) / 192 || knots >= (1 << 29))
+    // (the slot-major condense buffer, 192 floats per knot, is addressed
+    // with 32-bit byte offsets)
+    if (tiles >= (1 << 30) / 192 || knots >= ((int64_t) 1 << 31) / 768)
         return fail (PLM_HIP_ERR_GEOMETRY, "B-spline grid too large");
     g.num_tiles = (int) tiles;
     g.num_knots = (int) knots;

@@ -463,7 +463,9 @@ tile_coef_fetch (float c[3], const KGeom& g, int knot0, int t, const float* __re
 enum ColumnMode { PROJECTED = 0, DISPLACEMENT = 1, OFFSETS = 2 };
 
 // Stage 1 of a tile: its 64 knot coefficients (fetched ahead into
-// registers) and the per-slice moving-index term
+// registers; PROJECTED: projected onto the moving index axes, P c, so the
+// contractions that follow carry moving-index units and the column terms
+// need no projection) and the per-slice moving-index term
 // Z[z][a] = M[a][2] * (bz0 + z), M = P_moving * step_fixed.
 template <int V, int MODE>
 __device__ __forceinline__ void
@@ -471,9 +473,19 @@ tile_stage1 (const TileLds& L, const Shape<V>& S, const KGeom& g, int bz0, const
     int t)
 {
     if (MODE != OFFSETS && t < 64) {
-        L.coef[3 * t + 0] = c[0];
-        L.coef[3 * t + 1] = c[1];
-        L.coef[3 * t + 2] = c[2];
+        float c0 = c[0], c1 = c[1], c2 = c[2];
+        if (MODE == PROJECTED) {
+            const float* P = g.mproj;
+            const float a0 = fmaf (P[2], c2, fmaf (P[1], c1, P[0] * c0));
+            const float a1 = fmaf (P[5], c2, fmaf (P[4], c1, P[3] * c0));
+            const float a2 = fmaf (P[8], c2, fmaf (P[7], c1, P[6] * c0));
+            c0 = a0;
+            c1 = a1;
+            c2 = a2;
+        }
+        L.coef[3 * t + 0] = c0;
+        L.coef[3 * t + 1] = c1;
+        L.coef[3 * t + 2] = c2;
     }
     for (int e = t; e < 3 * S.vz; e += TILE_THREADS) {
         const int z = e / 3, a = e - 3 * z;
@@ -502,10 +514,9 @@ tile_t1 (const TileLds& L, const Shape<V>& S, int t)
 }
 
 // Stage 3: the per-column terms of the moving-voxel coordinate.  With
-// T2[c][k] = sum_j by[y][j] T1[k][j][x][c] (the displacement still to be
-// contracted along z), A[a][k] = sum_c P[a][c] T2[c][k] and
-// C[a] = moff[a] + M[a][0] fi + M[a][1] fj, a voxel (x, y, z) of the tile
-// maps to moving index coordinates
+// A[a][k] = sum_j by[y][j] T1[k][j][x][a] (the projected displacement still
+// to be contracted along z) and C[a] = moff[a] + M[a][0] fi + M[a][1] fj, a
+// voxel (x, y, z) of the tile maps to moving index coordinates
 //   mijk[a] = sum_k bz[z][k] A[a][k] + C[a] + Z[z][a],
 // which is P * (fixed position + displacement - moving origin) regrouped
 // (bspline_find_correspondence_dcos, src/util/bspline_correspond.cxx:80-109,
@@ -552,14 +563,9 @@ tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int 
             T[1][k] = fmaf (b.w, r3.y, fmaf (b.z, r2.y, fmaf (b.y, r1.y, b.x * r0.y)));
             T[2][k] = fmaf (b.w, r3.z, fmaf (b.z, r2.z, fmaf (b.y, r1.z, b.x * r0.z)));
         }
-        const float* P = g.mproj;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            float A[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                A[k] = MODE == DISPLACEMENT ? T[a][k]
-                    : fmaf (P[3 * a + 2], T[2][k], fmaf (P[3 * a + 1], T[1][k], P[3 * a] * T[0][k]));
+            const float A[4] = {T[a][0], T[a][1], T[a][2], T[a][3]};
             o[a * np] = make_float4 (A[0], A[1], A[2], A[3]);
             if (MODE == PROJECTED) {
                 const float4 Cv = o[3 * np];
@@ -666,11 +672,16 @@ tile_condense (const TileLds& L, const Shape<V>& S, const KGeom& g, int knot0,
         }
         const int cd0 = g.cdims[0], cd01 = g.cdims[0] * g.cdims[1];
         const int knot = knot0 + i + cd0 * j;
-        const size_t plane = (size_t) 3 * g.num_knots;
+        // 32-bit byte offsets: the buffer holds 192 * num_knots floats,
+        // below 2^31 bytes (checked at context creation)
+        const uint32_t plane = 12u * (uint32_t) g.num_knots;
+        const Rsrc cr = make_rsrc (cond, 64u * plane);
         const float sc = SCALE ? g.inv_msp[c] : 1.f;
+        const uint32_t off = (uint32_t) ji * plane + 4u * (uint32_t) (3 * knot + c);
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            cond[(size_t) (k * 16 + ji) * plane + 3 * (knot + k * cd01) + c] = SCALE ? a[k] * sc : a[k];
+            __builtin_amdgcn_raw_buffer_store_b32 (__float_as_uint (SCALE ? a[k] * sc : a[k]), cr,
+                (int) (off + (uint32_t) k * (16u * plane + 12u * (uint32_t) cd01)), 0, 0);
     }
 }
 
