@@ -203,6 +203,17 @@ int plm_hip_bspline_autotune (plm_hip_ctx* ctx, int metric, int grad_mode,
 int plm_hip_bspline_interpolate_vf (plm_hip_ctx* ctx, const float* d_coeff, float* d_vf,
     void* stream);
 
+/* Warped moving image of coefficients d_coeff: one float per fixed voxel
+ * (fixed-image dims, x fastest) into d_out, the moving image sampled where
+ * the fixed voxel lands -- trilinear (linear_interp = 1) or the nearest
+ * voxel (0) -- and default_val where that position leaves the moving image
+ * and on voxels outside the ROI.  Replaces bspline_warp
+ * (src/base/bspline_warp.cxx:171-214; bspline_warp_dcos :27-167) for float
+ * images; the moving image is read at the context's storage precision.
+ * Device pointers, enqueued on stream (NULL: the context's). */
+int plm_hip_bspline_warp (plm_hip_ctx* ctx, const float* d_coeff, float* d_out,
+    float default_val, int linear_interp, void* stream);
+
 /* MSE score and exact-derivative gradient (as PLM_GRAD_EXACT) taking the
  * displacement from a stored field d_vf (as plm_hip_bspline_interpolate_vf
  * writes it) instead of interpolating the coefficients: the stored-field

@@ -54,6 +54,9 @@ def lib():
         L.orc_basis_lut.argtypes = [ctypes.c_int64, P]
         L.orc_direction_matrices.argtypes = [P, P, P, P, P]
         L.orc_interp_vf.argtypes = [ctypes.POINTER(OrcGeom), P, P]
+        L.orc_warp.restype = ctypes.c_int
+        L.orc_warp.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P, P, ctypes.c_int,
+                               ctypes.c_float, ctypes.c_int, P]
         L.orc_volume_gradient.argtypes = [ctypes.POINTER(OrcVolHdr), P, P]
         L.orc_hist_eqsp.argtypes = [P, ctypes.c_int64, ctypes.c_int64, P, P]
         L.orc_pass1_position_mismatch.restype = ctypes.c_int64
@@ -123,6 +126,21 @@ def interp_vf(g, coeff):
     vf = np.zeros((nz, ny, nx, 3), np.float32)
     lib().orc_interp_vf(ctypes.byref(g), _ptr(np.ascontiguousarray(coeff, np.float32)), _ptr(vf))
     return vf
+
+
+def warp(g, moving, m_spacing, coeff, linear=True, default_val=0.0, m_origin=(0.0, 0.0, 0.0),
+         m_dc=IDENTITY, nthreads=0):
+    """Warped moving image on the fixed grid of g (orc_warp)."""
+    moving = np.ascontiguousarray(moving, np.float32)
+    nx, ny, nz = g.img_dim
+    out = np.zeros((nz, ny, nx), np.float32)
+    mh = vol_hdr(moving, m_spacing, m_origin, m_dc)
+    rc = lib().orc_warp(ctypes.byref(g), ctypes.byref(mh), _ptr(moving),
+                        _ptr(np.ascontiguousarray(coeff, np.float32)), 1 if linear else 0,
+                        float(default_val), int(nthreads), _ptr(out))
+    if rc:
+        raise ValueError(f"orc_warp failed ({rc})")
+    return out
 
 
 def volume_gradient(img, spacing, origin=(0.0, 0.0, 0.0), dc=IDENTITY):

@@ -531,6 +531,86 @@ orc_interp_vf (const orc_geom* g, const float* coeff, float* vf)
 }
 
 /* ---------------------------------------------------------------------
+   Warped moving image
+   --------------------------------------------------------------------- */
+
+/* out = the moving image at each fixed voxel moved by the B-spline
+   displacement, trilinear (linear_interp) or nearest voxel; default_val
+   where the moving position falls outside the moving image, and on the
+   voxels outside the ROI.  Follows bspline_warp_dcos,
+   src/base/bspline_warp.cxx:27-167, for float images: fixed positions as
+   the LOOP_Z_OMP / LOOP_Y / LOOP_X macros form them
+   (src/base/volume_macros.h:46-73: x accumulated step by step), the
+   displacement of bspline_interp_pix_b, the correspondence and inside
+   test, li_clamp_3d and LI_VALUE (li_value here).  The reference indexes
+   regions from the image's first voxel (REGION_INDEX_*,
+   src/base/bspline_macros.h:33-80), i.e. it assumes the ROI is the whole
+   image; regions count from the ROI's first voxel here, the same thing in
+   that case. */
+int
+orc_warp (const orc_geom* g, const orc_vol_hdr* mh, const float* m_img,
+    const float* coeff, int linear_interp, float default_val, int nthreads, float* out)
+{
+    bxf_t b;
+    vol_t mv;
+    if (bxf_init (&b, g, coeff)) return -1;
+    vol_init (&mv, mh, m_img);
+    set_threads (nthreads);
+    float step[9], proj[9], inv_dc[9];
+    orc_direction_matrices (g->dc, g->img_spacing, step, proj, inv_dc);
+    const int64_t* dim = g->img_dim;
+    const float* origin = g->img_origin;
+    const int64_t npix = dim[0] * dim[1] * dim[2];
+    for (int64_t v = 0; v < npix; v++) out[v] = default_val;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < dim[2]; k++) {
+        int64_t fijk[3];
+        float fxyz[3];
+        fijk[2] = k;
+        fxyz[2] = origin[2] + fijk[2] * step[2*3+2];
+        for (fijk[1] = 0, fxyz[1] = origin[1] + fijk[2] * step[1*3+2];
+             fijk[1] < dim[1];
+             ++fijk[1],
+             fxyz[2] = origin[2] + fijk[2] * step[2*3+2] + fijk[1] * step[2*3+1],
+             fxyz[1] = origin[1] + fijk[2] * step[1*3+2] + fijk[1] * step[1*3+1]) {
+            for (fijk[0] = 0, fxyz[0] = origin[0] + fijk[2] * step[0*3+2] + fijk[1] * step[0*3+1];
+                 fijk[0] < dim[0];
+                 ++fijk[0], fxyz[0] += step[0*3+0], fxyz[1] += step[1*3+0],
+                 fxyz[2] += step[2*3+0]) {
+                int64_t r[3], p[3], q[3];
+                int in_roi = 1;
+                for (int d = 0; d < 3; d++) {
+                    r[d] = fijk[d] - b.roi_offset[d];
+                    if (r[d] < 0 || r[d] >= b.roi_dim[d]) in_roi = 0;
+                }
+                if (!in_roi) continue;
+                for (int d = 0; d < 3; d++) {
+                    p[d] = r[d] / b.vpr[d];
+                    q[d] = r[d] % b.vpr[d];
+                }
+                const int64_t pidx = p[0] + b.rdims[0] * (p[1] + b.rdims[1] * p[2]);
+                const int64_t qidx = q[0] + b.vpr[0] * (q[1] + b.vpr[1] * q[2]);
+                float d[3], mijk[3], l1[3], l2[3];
+                int64_t mf[3], mr[3];
+                interp_pix_b (d, &b, pidx, qidx);
+                const int64_t fv = fijk[0] + dim[0] * (fijk[1] + dim[1] * fijk[2]);
+                if (!find_correspondence (mijk, fxyz, d, &mv)) continue;
+                li_clamp_3d (mijk, mf, mr, l1, l2, &mv);
+                if (linear_interp) {
+                    const int64_t mvf = mf[0] + mv.dim[0] * (mf[1] + mv.dim[1] * mf[2]);
+                    out[fv] = li_value (l1, l2, mvf, m_img, mv.dim);
+                } else {
+                    const int64_t mvr = mr[0] + mv.dim[0] * (mr[1] + mv.dim[1] * mr[2]);
+                    out[fv] = m_img[mvr];
+                }
+            }
+        }
+    }
+    bxf_free (&b);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------
    MSE
    --------------------------------------------------------------------- */
 

@@ -54,6 +54,10 @@ void orc_basis_lut (int64_t vpr, float* lut);
 void orc_direction_matrices (const float dc[9], const float spacing[3],
     float step[9], float proj[9], float inv_dc[9]);
 void orc_interp_vf (const orc_geom* g, const float* coeff, float* vf);
+/* Warped moving image over the fixed grid (bspline_warp_dcos, float):
+   linear_interp 1 = trilinear, 0 = nearest voxel; default_val outside. */
+int orc_warp (const orc_geom* g, const orc_vol_hdr* mh, const float* m_img,
+    const float* coeff, int linear_interp, float default_val, int nthreads, float* out);
 void orc_volume_gradient (const orc_vol_hdr* h, const float* img, float* out);
 void orc_hist_eqsp (const float* img, int64_t npix, int64_t bins,
     float* offset, float* delta);

@@ -95,6 +95,7 @@ SIGNATURES = {
                                                   ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_bspline_interpolate_vf": (ctypes.c_int, [_P, _P, _P, _P]),
     "plm_hip_bspline_score_vf_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "plm_hip_bspline_warp": (ctypes.c_int, [_P, _P, _P, ctypes.c_float, ctypes.c_int, _P]),
     "plm_hip_bspline_num_knots": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_bspline_num_tiles": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),

@@ -236,6 +236,15 @@ class BsplineHip:
         return _abi.check(self.lib.plm_hip_bspline_interpolate_vf(
             self.ctx, ctypes.c_void_p(d_coeff.data_ptr()), ctypes.c_void_p(d_vf.data_ptr()), st))
 
+    def warp(self, d_coeff, d_out, default_val=0.0, linear=True, stream=None):
+        """Warped moving image (one float per fixed voxel) into d_out: trilinear
+        or nearest voxel, default_val outside the moving image and the ROI
+        (bspline_warp)."""
+        st = ctypes.c_void_p(stream) if stream else None
+        return _abi.check(self.lib.plm_hip_bspline_warp(
+            self.ctx, ctypes.c_void_p(d_coeff.data_ptr()), ctypes.c_void_p(d_out.data_ptr()),
+            float(default_val), 1 if linear else 0, st))
+
     def score_vf_device(self, d_vf, d_score, d_grad, d_num_vox, stream=None):
         """MSE score and exact-derivative gradient from a stored displacement field."""
         ptr = lambda t: ctypes.c_void_p(t.data_ptr())

@@ -755,6 +755,22 @@ plm_hip_bspline_interpolate_vf (plm_hip_ctx* c, const float* d_coeff, float* d_v
 }
 
 int
+plm_hip_bspline_warp (plm_hip_ctx* c, const float* d_coeff, float* d_out, float default_val,
+    int linear_interp, void* stream)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    if (!d_coeff || !d_out) return fail (PLM_HIP_ERR_ARG, "NULL argument");
+    if (linear_interp != 0 && linear_interp != 1)
+        return fail (PLM_HIP_ERR_ARG, "linear_interp must be 0 or 1");
+    HIP_TRY (hipSetDevice (c->device));
+    hipStream_t st = stream ? (hipStream_t) stream : c->stream;
+    hipError_t e = plmhip::launch_warp (c->plan, c->tune, c->storage == PLM_STORE_BF16, d_coeff,
+        d_out, default_val, linear_interp, st);
+    if (e != hipSuccess) return hip_fail (e, "warp launch");
+    return PLM_HIP_OK;
+}
+
+int
 plm_hip_bspline_score_vf_device (plm_hip_ctx* c, const float* d_vf, double* d_score,
     double* d_grad, int64_t* d_num_vox, void* stream)
 {

@@ -119,13 +119,13 @@ struct Plan {
     int npairs;
     KGeom g0;                // pair 0's geometry (every pair shares the tile edges)
     int total_tiles;
-    int reduce_blocks;       // sum over pairs of ceil(3 * num_knots / 256)
+    int reduce_blocks;       // sum over pairs of ceil(3 * num_knots / REDUCE_BLOCK)
     const float* blut;       // basis LUTs of the shared tile edges
     double* part_score;      // per (pair, workgroup) partials, npairs * (total_tiles + 8)
     int* part_count;
 };
 
-constexpr int REDUCE_BLOCK = 256;    // gradient components per reduce workgroup
+constexpr int REDUCE_BLOCK = 64;     // gradient components per reduce workgroup
 
 // Stage timing: when ev is not null, ev[0..n] are recorded on st between
 // the launches (MSE: tiles | reduce, 3 events; MI: zero | hist | score |
@@ -154,6 +154,11 @@ hipError_t launch_vf (const Plan& plan, const Tune& tune, const float* coeff0, f
 hipError_t launch_mse_from_vf (const Plan& plan, const Tune& tune, int storage, const float* vf,
     const EvalOut& out0, hipStream_t st, hipEvent_t* ev);
 
+// Warped moving image of a single-pair plan (trilinear if linear, else the
+// nearest voxel): one float per fixed voxel, fill where the moving position
+// leaves the moving image and outside the ROI.
+hipError_t launch_warp (const Plan& plan, const Tune& tune, int storage, const float* coeff0,
+    float* out, float fill, int linear, hipStream_t st);
 // Mattes MI of a single-pair plan.
 hipError_t launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     const uint8_t* fbin, const uint8_t* mbin, const float* coeff, unsigned long long* jhist,

@@ -747,15 +747,17 @@ template <int KIND, typename TI, int V, int G, bool BATCH>
 __global__ __launch_bounds__ (TILE_THREADS)
 void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __restrict__ coeff0,
     const float* __restrict__ blut, double* __restrict__ part_score,
-    int* __restrict__ part_count, MiArgs mi, float* __restrict__ vf)
+    int* __restrict__ part_count, MiArgs mi, float* __restrict__ vf, float fill)
 {
     // what this kind computes
     constexpr bool MI = KIND == 2;
     constexpr bool VF_OUT = KIND == 4;          // the displacement field only
     constexpr bool VF_IN = KIND == 5;           // the displacement read from vf
+    constexpr bool WARP = KIND == 6 || KIND == 7;   // warped moving image into vf
+    constexpr bool NEAREST = KIND == 7;             // (nearest voxel instead of trilinear)
     constexpr bool DERIV = KIND == 1 || KIND == 5;  // analytic trilinear derivative
     constexpr bool GRAD = KIND <= 2 || KIND == 5;   // dc/dv and the condense
-    constexpr bool SCORE = !MI && !VF_OUT;          // squared differences, overlap count
+    constexpr bool SCORE = !MI && !VF_OUT && !WARP; // squared differences, overlap count
     constexpr int MODE = VF_OUT ? DISPLACEMENT : (VF_IN ? OFFSETS : PROJECTED);
     const int npairs = BATCH ? q.npairs : 1;
 
@@ -950,6 +952,8 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
             float a[G][8];
             float mg[G][3];
             int bn[G][8];
+            float nv[G];            // WARP, NEAREST: the nearest moving voxel
+            int wofs[G];            // WARP: the fixed voxel written, or -1
 #pragma unroll
             for (int u = 0; u < G; u++) {
                 int x, y, z;
@@ -973,6 +977,8 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     if (VF_IN) Img<float>::vec3 (dd, vr, fofs * 12, 0);
                 }
                 const bool act = x < ex && y < ey && z < ez;
+                if (WARP)
+                    wofs[u] = act ? fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z) : -1;
                 const int r = x + (int) __umul24 (vx, y);
                 const int zc = min (z, vz - 1);
 
@@ -1038,7 +1044,12 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                 in[u] = inside & act;
                 // byte offsets as 24-bit products (slices hold < 2^24 / 12
                 // voxels, checked at context creation)
-                if (!MI) {
+                if (NEAREST) {
+                    // bspline_warp_dcos's nearest voxel: the rounded index
+                    const int roff = (int) __umul24 (f[0] + c[0], ES) + (int) __umul24 (f[1] + c[1], SX)
+                        + (int) __umul24 (f[2] + c[2], SXY);
+                    nv[u] = Img<TI>::one (mr, roff, 0);
+                } else if (!MI) {
                     const int off = (int) __umul24 (f[0], ES) + (int) __umul24 (f[1], SX)
                         + (int) __umul24 (f[2], SXY);
                     Img<TI>::pair (a[u][0], a[u][1], mr, off, 0);
@@ -1069,6 +1080,23 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
 #pragma unroll
             for (int u = 0; u < G; u++) {
                 const int v = base + u * TILE_THREADS + tt;
+                if (WARP) {
+                    // bspline_warp_dcos (src/base/bspline_warp.cxx:27-167):
+                    // the moving value where the voxel lands, the fill
+                    // value where it leaves the moving image
+                    float val;
+                    if (NEAREST) {
+                        val = nv[u];
+                    } else {
+                        Sample s;
+                        s.l2[0] = fr_[u][0]; s.l2[1] = fr_[u][1]; s.l2[2] = fr_[u][2];
+                        float md[3];
+                        val = trilinear<false, FAST> (s, a[u], md);
+                    }
+                    asm volatile ("" : "+v" (val));
+                    if (wofs[u] >= 0) vf[wofs[u]] = in[u] ? val : fill;
+                    continue;
+                }
                 float dc0 = 0.f, dc1 = 0.f, dc2 = 0.f;
                 if (!MI) {
                     Sample s;
@@ -1176,17 +1204,20 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
 }
 
 // ---------------------------------------------------------------------------
-// Reduce: one thread per gradient component, summing its 64 slots in slot
-// order in FP64 -- the order of bspline_condense_smetric_grad
-// (src/register/bspline.cxx:400-413), in double -- then scaled by 2/N for
-// MSE (src/register/bspline_mse.cxx:43-73) or 1 for MI.  Workgroups are
-// dealt to the pairs of the launch in order.  For MSE every workgroup also
-// sums its pair's per-workgroup overlap counts (integers, so every
-// workgroup gets the same N with no cross-workgroup traffic), fetched with
-// the slot loads; the pair's first workgroup also sums the score partials
-// in a fixed order and writes score = raw/N (FLT_MAX if N < 1) and N.
+// Reduce: REDUCE_BLOCK (64) gradient components per workgroup of 256
+// threads, four per component: thread (q, c) sums slots 16q .. 16q + 15 of
+// component c in FP64, and the four partial sums are added in q order --
+// a fixed order, so the result is bitwise reproducible (the slot sum of
+// bspline_condense_smetric_grad, src/register/bspline.cxx:400-413, in
+// double) -- then scaled by 2/N for MSE (src/register/bspline_mse.cxx:43-73)
+// or 1 for MI.  Workgroups are dealt to the pairs of the launch in order.
+// For MSE every workgroup also sums its pair's per-workgroup overlap counts
+// (integers, so every workgroup gets the same N with no cross-workgroup
+// traffic), fetched with the slot loads; the pair's first workgroup also
+// sums the score partials in a fixed order and writes score = raw/N
+// (FLT_MAX if N < 1) and N.
 
-constexpr int REDUCE_THREADS = REDUCE_BLOCK;
+constexpr int REDUCE_THREADS = 4 * REDUCE_BLOCK;
 
 __device__ __forceinline__ double
 wave_sum (double v)
@@ -1211,6 +1242,7 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
 {
     __shared__ long long s_n[REDUCE_THREADS / 64];
     __shared__ double s_s[REDUCE_THREADS / 64];
+    __shared__ double s_q[4][REDUCE_BLOCK];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nwaves = REDUCE_THREADS / 64;
     // the pair this workgroup works for (workgroups are dealt to pairs in
@@ -1221,21 +1253,21 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
     const int blk = blockIdx.x - pd.blk0;
     const bool lead = blk == 0;
     const int nc = 3 * pd.g.num_knots;
-    const int i = blk * REDUCE_THREADS + threadIdx.x;
-    const bool own = with_grad && i < nc;
-    const float* c = pd.cond + (i < nc ? i : 0);
+    const int i = blk * REDUCE_BLOCK + lane;        // this thread's component
+    const int q = wave;                              // and its quarter of the slots
+    const float* c = pd.cond + (size_t) (16 * q) * nc + (i < nc ? i : 0);
     double* grad = (out0.grad && p == 0) ? out0.grad : pd.grad;
-    // all 64 slot loads are issued before any is consumed (a deep enough
-    // queue to cover the memory latency with few waves), then summed in slot
-    // order; the pair's per-workgroup partials are fetched meanwhile.  The
-    // loads are unconditional (c stays in range) so nothing serializes them.
-    float v[64];
+    // all 16 slot loads are issued before any is consumed, then summed in
+    // slot order; the pair's per-workgroup partials are fetched meanwhile.
+    // The loads are unconditional (c stays in range) so nothing serializes
+    // them.
+    float v[16];
     if (with_grad) {
 #pragma unroll
-        for (int m = 0; m < 64; m++) v[m] = c[(size_t) m * nc];
+        for (int m = 0; m < 16; m++) v[m] = c[(size_t) m * nc];
     } else {
 #pragma unroll
-        for (int m = 0; m < 64; m++) v[m] = 0.f;
+        for (int m = 0; m < 16; m++) v[m] = 0.f;
     }
     long long n = 0;
     double sc = 0.0;
@@ -1263,13 +1295,16 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
     }
     double acc = 0.0;
 #pragma unroll
-    for (int m = 0; m < 64; m++) acc += (double) v[m];
+    for (int m = 0; m < 16; m++) acc += (double) v[m];
+    s_q[q][lane] = acc;
     double scale = 1.0;
     if (MSE) {
         n = wave_sum_ll (n);
         sc = wave_sum (sc);
         if (lane == 0) { s_n[wave] = n; s_s[wave] = sc; }
-        __syncthreads ();
+    }
+    __syncthreads ();
+    if (MSE) {
         long long Nv = 0;
         double Sv = 0.0;
         for (int w = 0; w < nwaves; w++) { Nv += s_n[w]; Sv += s_s[w]; }
@@ -1281,7 +1316,8 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
             *nvox = Nv;
         }
     }
-    if (own) grad[i] = acc * scale;
+    if (q == 0 && with_grad && i < nc)
+        grad[i] = (((s_q[0][lane] + s_q[1][lane]) + s_q[2][lane]) + s_q[3][lane]) * scale;
 }
 
 // ---------------------------------------------------------------------------
@@ -1581,11 +1617,11 @@ k_zero_u64 (unsigned long long* __restrict__ p, int n)
 }
 
 __global__ void
-k_zero_f32 (float* __restrict__ p, int64_t n)
+k_fill_f32 (float* __restrict__ p, int64_t n, float v)
 {
     for (int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
          i += (int64_t) gridDim.x * blockDim.x)
-        p[i] = 0.f;
+        p[i] = v;
 }
 
 // float32 -> bfloat16, round to nearest even (NaN kept quiet).
@@ -1635,7 +1671,9 @@ mark (hipEvent_t* ev, int i, hipStream_t st)
 struct TileArgs {
     const float* coeff0;    // pair 0's coefficients (single-pair calls), or null
     MiArgs mi;              // MI pass 2
-    float* vf;              // pair 0's displacement field (KIND 4 out, KIND 5 in)
+    float* vf;              // pair 0's displacement field (KIND 4 out, KIND 5 in), or
+                            // the warped image (KIND 6, 7 out)
+    float fill = 0.f;       // KIND 6, 7: the value where a voxel leaves the moving image
 };
 
 // Persistent grid: as many workgroups as fit on the device at once
@@ -1672,7 +1710,7 @@ launch_tiles_vgb (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st
     const int per_xcd = std::max (1, std::min (q.tiles_per_xcd, occ * cus / 8));
     hipLaunchKernelGGL ((k_tiles<KIND, TI, V, G, BATCH>), dim3 (8 * per_xcd), dim3 (TILE_THREADS),
         lds, st, plan.d_pairs, q, ta.coeff0, plan.blut, plan.part_score, plan.part_count, ta.mi,
-        ta.vf);
+        ta.vf, ta.fill);
     return 8 * per_xcd;
 }
 
@@ -1847,7 +1885,28 @@ launch_bin_image (const void* img, int storage, uint8_t* out, int64_t n, float o
 hipError_t
 launch_zero (float* p, int64_t n, hipStream_t st)
 {
-    hipLaunchKernelGGL (k_zero_f32, dim3 (1024), dim3 (256), 0, st, p, n);
+    hipLaunchKernelGGL (k_fill_f32, dim3 (1024), dim3 (256), 0, st, p, n, 0.f);
+    return hipGetLastError ();
+}
+
+hipError_t
+launch_warp (const Plan& plan, const Tune& tune, int storage, const float* coeff0, float* out,
+    float fill, int linear, hipStream_t st)
+{
+    const KGeom& g = plan.g0;
+    // voxels outside the ROI keep the fill value; the tile kernel writes
+    // every ROI voxel
+    bool whole = true;
+    for (int a = 0; a < 3; a++)
+        whole = whole && g.roi_off[a] == 0 && g.roi_end[a] == g.fdim[a];
+    if (!whole) {
+        hipLaunchKernelGGL (k_fill_f32, dim3 (1024), dim3 (256), 0, st, out,
+            (int64_t) g.fdim[0] * g.fdim[1] * g.fdim[2], fill);
+        LAUNCH_CHECK ();
+    }
+    const TileArgs ta {coeff0, MiArgs {}, out, fill};
+    if (linear) launch_tiles_storage<6> (plan, tune, storage, st, ta);
+    else launch_tiles_storage<7> (plan, tune, storage, st, ta);
     return hipGetLastError ();
 }
 

@@ -308,3 +308,60 @@ def test_golden_bruteforce(case):
     assert r["num_vox"] == m["num_vox"]
     assert rel_scalar(r["score"], m["score"]) < 1e-5
     assert rel(r["grad"], m["grad"]) < 1e-4
+
+
+# ---------------------------------------------------------------- warp
+
+def _warp_case(dim=(20, 18, 16), spacing=(1.0, 1.0, 1.0), vpr=(5, 5, 4), seed=3):
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dim
+    moving = rng.normal(100.0, 30.0, size=(nz, ny, nx)).astype(np.float32)
+    g = oracle.geom(dim, spacing, vpr)
+    _, cdims, nk = oracle.header(g)
+    return moving, g, nk
+
+
+@pytest.mark.parametrize("linear", [True, False])
+def test_warp_zero_field_is_a_copy(linear):
+    """Zero coefficients on a shared grid land every fixed voxel on a moving
+    voxel centre (fraction 0, or 1 at the clamped upper border): the warp is
+    the moving image bit for bit."""
+    moving, g, nk = _warp_case()
+    out = oracle.warp(g, moving, (1.0, 1.0, 1.0), np.zeros(3 * nk, np.float32), linear, -7.0)
+    assert np.array_equal(out, moving)
+
+
+def test_warp_integer_shift_and_default():
+    """A constant field of s voxels along x moves the image by s voxels (the
+    basis sums to one to within a few ulp, so the linear value is within
+    1e-5); fixed voxels whose moving position leaves the image (x + s beyond
+    dim - 0.5) keep the default value."""
+    s = 3
+    sp = (1.25, 1.0, 2.0)
+    moving, g, nk = _warp_case(spacing=sp)
+    coeff = np.zeros((nk, 3), np.float32)
+    coeff[:, 0] = s * sp[0]
+    out = oracle.warp(g, moving, sp, coeff.ravel(), True, -7.0)
+    nx = moving.shape[2]
+    np.testing.assert_allclose(out[:, :, :nx - s], moving[:, :, s:], rtol=1e-5, atol=1e-4)
+    assert np.all(out[:, :, nx - s:] == -7.0)
+    near = oracle.warp(g, moving, sp, coeff.ravel(), False, -7.0)
+    assert np.array_equal(near[:, :, :nx - s], moving[:, :, s:])
+
+
+def test_warp_nearest_rounds_half_away_and_roi_default():
+    """Shifts of 0.3 and 0.7 voxel pick the lower and the upper neighbour
+    (ROUND_PLM_LONG); voxels outside the ROI keep the default value."""
+    moving, _, _ = _warp_case()
+    dim = (20, 18, 16)
+    g = oracle.geom(dim, (1.0, 1.0, 1.0), (5, 5, 4), roi_offset=(2, 3, 1), roi_dim=(15, 12, 14))
+    _, _, nk = oracle.header(g)
+    for frac, pick in ((0.3, 0), (0.7, 1)):
+        coeff = np.zeros((nk, 3), np.float32)
+        coeff[:, 1] = frac
+        out = oracle.warp(g, moving, (1.0, 1.0, 1.0), coeff.ravel(), False, -7.0)
+        # the ROI's last row reads moving row 15 at most: always inside
+        assert np.array_equal(out[1:15, 3:15, 2:17], moving[1:15, 3 + pick:15 + pick, 2:17])
+        mask = np.ones(out.shape, bool)
+        mask[1:15, 3:15, 2:17] = False
+        assert np.all(out[mask] == -7.0)
