@@ -102,6 +102,19 @@ typedef struct plm_hist_desc {
     float moving_delta;
 } plm_hist_desc;
 
+/* Regularization weights: the fields of Regularization_parms
+ * (src/register/regularization_parms.h:20-28) the analytic flavors read. */
+typedef struct plm_reg_parms {
+    float total_displacement_penalty;
+    float diffusion_penalty;
+    float curvature_penalty;
+    float lame_coefficient_1;
+    float lame_coefficient_2;
+    float linear_elastic_multiplier;
+    float third_order_penalty;
+    float curvature_mixed_weight;
+} plm_reg_parms;
+
 typedef struct plm_hip_ctx plm_hip_ctx;
 
 int plm_hip_abi_version (void);
@@ -202,6 +215,19 @@ int plm_hip_bspline_autotune (plm_hip_ctx* ctx, int metric, int grad_mode,
  * with device pointers, enqueued on stream (NULL: the context's). */
 int plm_hip_bspline_interpolate_vf (plm_hip_ctx* ctx, const float* d_coeff, float* d_vf,
     void* stream);
+
+/* Analytic smoothness regularization of coefficients d_coeff over the
+ * context's B-spline grid: *d_score = the regularization score (double)
+ * and d_grad (3 * num_knots doubles) = its gradient, added to what d_grad
+ * holds when accumulate is 1 (the reference adds it to the similarity
+ * gradient).  Replaces Bspline_regularize::compute_score_analytic_omp,
+ * flavor c (src/register/bspline_regularize_analytic.cxx:943-1007, with the
+ * matrices of analytic_init :638-940), called from bspline_score
+ * (src/register/bspline.cxx:619-622).  Device pointers, enqueued on stream
+ * (NULL: the context's); the first call allocates the context's
+ * regularization scratch. */
+int plm_hip_bspline_regularize (plm_hip_ctx* ctx, const plm_reg_parms* parms,
+    const float* d_coeff, double* d_score, double* d_grad, int accumulate, void* stream);
 
 /* Warped moving image of coefficients d_coeff: one float per fixed voxel
  * (fixed-image dims, x fastest) into d_out, the moving image sampled where

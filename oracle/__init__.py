@@ -29,6 +29,26 @@ class OrcGeom(ctypes.Structure):
                 ("dc", c_float9)]
 
 
+class OrcRegParms(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in (
+        "total_displacement_penalty", "diffusion_penalty", "curvature_penalty", "lame_coefficient_1",
+        "lame_coefficient_2", "linear_elastic_multiplier", "third_order_penalty",
+        "curvature_mixed_weight")]
+
+
+REG_DEFAULTS = {"total_displacement_penalty": 0.0, "diffusion_penalty": 0.0, "curvature_penalty": 0.0,
+                "lame_coefficient_1": 0.0, "lame_coefficient_2": 0.0, "linear_elastic_multiplier": 1.0,
+                "third_order_penalty": 0.0, "curvature_mixed_weight": 1.0}
+
+
+def reg_parms(**kw):
+    """Regularization weights with the reference's defaults
+    (src/register/regularization_parms.h:31-42)."""
+    d = dict(REG_DEFAULTS)
+    d.update(kw)
+    return OrcRegParms(**d)
+
+
 class OrcHist(ctypes.Structure):
     _fields_ = [("fixed_bins", ctypes.c_int64), ("moving_bins", ctypes.c_int64),
                 ("f_offset", ctypes.c_float), ("f_delta", ctypes.c_float),
@@ -70,6 +90,11 @@ def lib():
                                         ctypes.POINTER(OrcVolHdr), P, P, P, ctypes.c_int, ctypes.c_int,
                                         P, P, P]
         L.orc_pv_quantize.argtypes = [P, P]
+        L.orc_reg_analytic.restype = ctypes.c_int
+        L.orc_reg_analytic.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcRegParms), P, ctypes.c_int,
+                                       P, P]
+        L.orc_reg_matrices.restype = ctypes.c_int
+        L.orc_reg_matrices.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcRegParms), P]
         L.orc_mi_score.restype = ctypes.c_int
         L.orc_mi_score.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
                                    ctypes.POINTER(OrcVolHdr), P, P, ctypes.POINTER(OrcHist),
@@ -141,6 +166,26 @@ def warp(g, moving, m_spacing, coeff, linear=True, default_val=0.0, m_origin=(0.
     if rc:
         raise ValueError(f"orc_warp failed ({rc})")
     return out
+
+
+def reg_analytic(g, rp, coeff, nthreads=0):
+    """Analytic regularization, flavor c: (score, gradient) in double."""
+    _, _, nk = header(g)
+    grad = np.zeros(3 * nk, np.float64)
+    sc = np.zeros(1, np.float64)
+    rc = lib().orc_reg_analytic(ctypes.byref(g), ctypes.byref(rp),
+                                _ptr(np.ascontiguousarray(coeff, np.float32)), int(nthreads), _ptr(sc),
+                                _ptr(grad))
+    if rc:
+        raise ValueError(f"orc_reg_analytic failed ({rc})")
+    return float(sc[0]), grad
+
+
+def reg_matrices(g, rp):
+    """The 32 V matrices, shape (32, 64, 64)."""
+    V = np.zeros((32, 64, 64), np.float64)
+    lib().orc_reg_matrices(ctypes.byref(g), ctypes.byref(rp), _ptr(V))
+    return V
 
 
 def volume_gradient(img, spacing, origin=(0.0, 0.0, 0.0), dc=IDENTITY):

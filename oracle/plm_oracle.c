@@ -1507,3 +1507,266 @@ orc_mi_score_f64 (const orc_geom* g,
     dgeo_free (&d);
     return 0;
 }
+
+/* ---------------------------------------------------------------------
+   Analytic regularization (flavor c)
+   --------------------------------------------------------------------- */
+
+/* V[4j+i] = sum_{a,b} I[a+b] Qn1[4j+a] Qn2[4i+b], I[n] = gs^(n+1)/(n+1):
+   the integral over one region of the product of two basis polynomials.
+   Follows eval_integral, src/register/bspline_regularize_analytic.cxx:167-199
+   (the outer product of vec_outer, src/sys/plm_math.h:204-212). */
+static void
+reg_eval_integral (double* V, const double* Qn1, const double* Qn2, double gs)
+{
+    double I[7];
+    I[0] = gs;
+    I[1] = (1.0/2.0) * (gs * gs);
+    I[2] = (1.0/3.0) * (gs * gs * gs);
+    I[3] = (1.0/4.0) * (gs * gs * gs * gs);
+    I[4] = (1.0/5.0) * (gs * gs * gs * gs * gs);
+    I[5] = (1.0/6.0) * (gs * gs * gs * gs * gs * gs);
+    I[6] = (1.0/7.0) * (gs * gs * gs * gs * gs * gs * gs);
+    for (int j = 0; j < 4; j++) {
+        for (int i = 0; i < 4; i++) {
+            double S[16];
+            for (int a = 0; a < 4; a++)
+                for (int b = 0; b < 4; b++)
+                    S[4*a + b] = Qn1[4*j + a] * Qn2[4*i + b];
+            V[4*j + i] = (I[0] * S[0])
+                + (I[1] * (S[1] + S[4]))
+                + (I[2] * (S[2] + S[5] + S[8]))
+                + (I[3] * (S[3] + S[6] + S[9] + S[12]))
+                + (I[4] * (S[7] + S[10] + S[13]))
+                + (I[5] * (S[11] + S[14]))
+                + (I[6] * (S[15]));
+        }
+    }
+}
+
+static void
+reg_mat_mult (double* m1, const double* m2, const double* m3)
+{
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            double acc = 0.0;
+            for (int k = 0; k < 4; k++) acc += m2[4*i + k] * m3[4*k + j];
+            m1[4*i + j] = acc;
+        }
+}
+
+/* Q[d][n] = B R_d delta_n: the basis polynomial coefficients scaled by the
+   reciprocal grid spacing, and their n-th derivatives.  Follows
+   init_analytic, src/register/bspline_regularize_analytic.cxx:201-281. */
+static void
+reg_init_q (double Q[3][4][16], const double gs[3])
+{
+    static const double B[16] = {
+        1.0/6.0, -1.0/2.0,  1.0/2.0, -1.0/6.0,
+        2.0/3.0,  0.0    , -1.0    ,  1.0/2.0,
+        1.0/6.0,  1.0/2.0,  1.0/2.0, -1.0/2.0,
+        0.0    ,  0.0    ,  0.0    ,  1.0/6.0
+    };
+    static const double delta[3][16] = {
+        {0,0,0,0, 1,0,0,0, 0,2,0,0, 0,0,3,0},
+        {0,0,0,0, 0,0,0,0, 2,0,0,0, 0,6,0,0},
+        {0,0,0,0, 0,0,0,0, 0,0,0,0, 6,0,0,0}
+    };
+    for (int d = 0; d < 3; d++) {
+        const double r = 1.0 / gs[d];
+        double R[16] = {1.0, 0, 0, 0, 0, r, 0, 0, 0, 0, r*r, 0, 0, 0, 0, r*r*r};
+        reg_mat_mult (Q[d][0], B, R);
+        for (int n = 1; n < 4; n++) reg_mat_mult (Q[d][n], Q[d][0], delta[n-1]);
+    }
+}
+
+/* V = X (x) Y (x) Z as get_Vmatrix lays it out: row 16a + 4c + j takes X's
+   row a, Y's row c and Z's row j (src/register/bspline_regularize_analytic.cxx:283-338),
+   scaled by lambda (scale_Vmatrix, :340-346). */
+static void
+reg_vmatrix (double* V, const double* X, const double* Y, const double* Z, double lambda)
+{
+    for (int a = 0; a < 4; a++)
+        for (int c = 0; c < 4; c++)
+            for (int j = 0; j < 4; j++)
+                for (int b = 0; b < 4; b++)
+                    for (int d = 0; d < 4; d++)
+                        for (int i = 0; i < 4; i++)
+                            V[64*(16*a + 4*c + j) + (16*b + 4*d + i)]
+                                = lambda * (X[4*a + b] * (Y[4*c + d] * Z[4*j + i]));
+}
+
+/* The 32 V matrices of Bspline_regularize::analytic_init
+   (src/register/bspline_regularize_analytic.cxx:638-940): (x order,
+   y order, z order) of each factor's two Q matrices and the weight. */
+static void
+reg_analytic_init (double* V, const double gs[3], const orc_reg_parms* rp)
+{
+    double Q[3][4][16];
+    reg_init_q (Q, gs);
+    /* the weights as the reference forms them, in float */
+    const float cp = rp->curvature_penalty, mix = rp->curvature_mixed_weight;
+    const float dp = rp->diffusion_penalty, m = rp->linear_elastic_multiplier;
+    const float l1 = rp->lame_coefficient_1, l2 = rp->lame_coefficient_2;
+    const float tdp = rp->total_displacement_penalty, top = rp->third_order_penalty;
+    const float cpm = cp * mix, e1 = l1 * m, e2 = l2 * m;
+    const float ed = (m) * (2*l1 + l2)/2, eh = (m) * (l1)/2;
+    /* {x1, x2, y1, y2, z1, z2} derivative orders, weight */
+    const struct { int o[6]; float w; } T[32] = {
+        {{2,2, 0,0, 0,0}, cp}, {{0,0, 2,2, 0,0}, cp}, {{0,0, 0,0, 2,2}, cp},
+        {{1,1, 1,1, 0,0}, cpm}, {{1,1, 0,0, 1,1}, cpm}, {{0,0, 1,1, 1,1}, cpm},
+        {{1,1, 0,0, 0,0}, dp}, {{0,0, 1,1, 0,0}, dp}, {{0,0, 0,0, 1,1}, dp},
+        {{1,0, 0,1, 0,0}, e1}, {{0,1, 1,0, 0,0}, e2},
+        {{1,0, 0,0, 0,1}, e1}, {{0,1, 0,0, 1,0}, e2},
+        {{0,0, 1,0, 0,1}, e1}, {{0,0, 0,1, 1,0}, e2},
+        {{1,1, 0,0, 0,0}, ed}, {{0,0, 1,1, 0,0}, ed}, {{0,0, 0,0, 1,1}, ed},
+        {{1,1, 0,0, 0,0}, eh}, {{0,0, 1,1, 0,0}, eh}, {{0,0, 0,0, 1,1}, eh},
+        {{0,0, 0,0, 0,0}, tdp},
+        {{3,3, 0,0, 0,0}, top}, {{0,0, 3,3, 0,0}, top}, {{0,0, 0,0, 3,3}, top},
+        {{2,2, 1,1, 0,0}, top}, {{2,2, 0,0, 1,1}, top}, {{0,0, 2,2, 1,1}, top},
+        {{1,1, 2,2, 0,0}, top}, {{1,1, 0,0, 2,2}, top}, {{0,0, 1,1, 2,2}, top},
+        {{1,1, 1,1, 1,1}, top}
+    };
+    for (int n = 0; n < 32; n++) {
+        double X[16], Y[16], Z[16];
+        reg_eval_integral (X, Q[0][T[n].o[0]], Q[0][T[n].o[1]], gs[0]);
+        reg_eval_integral (Y, Q[1][T[n].o[2]], Q[1][T[n].o[3]], gs[1]);
+        reg_eval_integral (Z, Q[2][T[n].o[4]], Q[2][T[n].o[5]], gs[2]);
+        reg_vmatrix (&V[4096 * n], X, Y, Z, (double) T[n].w);
+    }
+}
+
+/* The 64 knots of region r, z outer then y then x.  Follows find_knots_3,
+   src/register/bspline_regularize_analytic.cxx:129-165. */
+static void
+reg_find_knots (int64_t knots[64], int64_t r, const int64_t rdims[3], const int64_t cdims[3])
+{
+    int64_t t0 = r % rdims[0], t1 = (r / rdims[0]) % rdims[1], t2 = r / (rdims[0] * rdims[1]);
+    int idx = 0;
+    for (int k = 0; k < 4; k++)
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 4; i++)
+                knots[idx++] = cdims[0] * cdims[1] * (t2 + k) + cdims[0] * (t1 + j) + (t0 + i);
+}
+
+/* S = sum over the symmetric V of p'Vp per component, sets += 2 V p.
+   Follows region_smoothness_omp, :363-399. */
+static double
+reg_region_smoothness (double* sets, const float* coeff, const double* V, const int64_t knots[64])
+{
+    double S = 0.0;
+    for (int j = 0; j < 64; j++) {
+        double X = 0.0, Y = 0.0, Z = 0.0;
+        for (int i = 0; i < 64; i++) {
+            X += coeff[3*knots[i]+0] * V[64*j + i];
+            Y += coeff[3*knots[i]+1] * V[64*j + i];
+            Z += coeff[3*knots[i]+2] * V[64*j + i];
+        }
+        S += X * coeff[3*knots[j]+0];
+        S += Y * coeff[3*knots[j]+1];
+        S += Z * coeff[3*knots[j]+2];
+        sets[3*j+0] += 2 * X;
+        sets[3*j+1] += 2 * Y;
+        sets[3*j+2] += 2 * Z;
+    }
+    return S;
+}
+
+/* The linear-elastic terms V9..V20 with their component couplings and the
+   reference's gradient expression.  Follows region_smoothness_elastic_omp,
+   :501-601. */
+static double
+reg_region_elastic (double* sets, const float* coeff, const double* V0, const int64_t knots[64])
+{
+    const double* V9 = V0 + 9*4096;  const double* V10 = V0 + 10*4096;
+    const double* V11 = V0 + 11*4096; const double* V12 = V0 + 12*4096;
+    const double* V13 = V0 + 13*4096; const double* V14 = V0 + 14*4096;
+    const double* V15 = V0 + 15*4096; const double* V16 = V0 + 16*4096;
+    const double* V17 = V0 + 17*4096; const double* V18 = V0 + 18*4096;
+    const double* V19 = V0 + 19*4096; const double* V20 = V0 + 20*4096;
+    double S = 0.0;
+    for (int j = 0; j < 64; j++) {
+        double X1 = 0, Y1 = 0, X2 = 0, Y2 = 0, X3 = 0, Z3 = 0, X4 = 0, Z4 = 0, Y5 = 0, Z5 = 0;
+        double Y6 = 0, Z6 = 0, X7 = 0, Y7 = 0, Z7 = 0, X8 = 0, Y8 = 0, Z8 = 0, X9 = 0, Y9 = 0, Z9 = 0;
+        for (int i = 0; i < 64; i++) {
+            const double cx = coeff[3*knots[i]+0], cy = coeff[3*knots[i]+1], cz = coeff[3*knots[i]+2];
+            const int e = 64*j + i;
+            X1 += cx * V9[e];  X2 += cx * V10[e]; X3 += cx * V11[e]; X4 += cx * V12[e];
+            Y5 += cy * V13[e]; Y6 += cy * V14[e]; Y1 += cy * V9[e];  Y2 += cy * V10[e];
+            Z3 += cz * V11[e]; Z4 += cz * V12[e]; Z5 += cz * V13[e]; Z6 += cz * V14[e];
+            X7 += cx * V15[e]; Y8 += cy * V16[e]; Z9 += cz * V17[e]; Y7 += cy * V18[e];
+            Z7 += cz * V18[e]; X8 += cx * V19[e]; Z8 += cz * V19[e]; X9 += cx * V20[e];
+            Y9 += cy * V20[e];
+        }
+        const double px = coeff[3*knots[j]+0], py = coeff[3*knots[j]+1], pz = coeff[3*knots[j]+2];
+        S += X1 * py; S += X2 * py; S += X3 * pz; S += X4 * pz; S += Y5 * pz; S += Y6 * pz;
+        S += X7 * px; S += Y8 * py; S += Z9 * pz; S += Y7 * py; S += Z7 * pz; S += X8 * px;
+        S += Z8 * pz; S += X9 * px; S += Y9 * py;
+        sets[3*j+0] += Y1 + Y2 + Z3 + Z4 + 2 * X7 + 2 * X8 + 2 * X9;
+        sets[3*j+1] += X1 + X2 + Z5 + Z6 + 2 * Y8 + 2 * Y7 + 2 * Y9;
+        sets[3*j+2] += X3 + X4 + Y5 + Y6 + 2 * Z9 + 2 * Z7 + 2 * Z8;
+    }
+    return S;
+}
+
+/* Regularization score and gradient, flavor c: per region the 21
+   symmetric terms and the elastic terms, slots placed per knot and summed
+   in slot order.  Follows Bspline_regularize::compute_score_analytic_omp,
+   src/register/bspline_regularize_analytic.cxx:943-1007 (with
+   reg_sort_sets :87-105 and reg_update_grad :110-127); the region sums are
+   added in region order here (the reference's OpenMP reduction has no fixed
+   order).  grid_spac = vox_per_rgn * |img_spacing| in float
+   (src/base/bspline_header.cxx:50). */
+int
+orc_reg_analytic (const orc_geom* g, const orc_reg_parms* rp, const float* coeff, int nthreads,
+    double* score, double* grad)
+{
+    int64_t rdims[3], cdims[3];
+    const int64_t nk = orc_header (g, rdims, cdims);
+    const int64_t nr = rdims[0] * rdims[1] * rdims[2];
+    double gs[3];
+    for (int d = 0; d < 3; d++)
+        gs[d] = (double) ((float) g->vox_per_rgn[d] * fabsf (g->img_spacing[d]));
+    double* V = (double*) malloc (sizeof (double) * 32 * 4096);
+    double* cond = (double*) calloc ((size_t) 3 * 64 * nk, sizeof (double));
+    double* Sr = (double*) malloc (sizeof (double) * nr);
+    if (!V || !cond || !Sr) { free (V); free (cond); free (Sr); return -1; }
+    reg_analytic_init (V, gs, rp);
+    set_threads (nthreads);
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < nr; r++) {
+        int64_t knots[64];
+        double sets[3*64];
+        memset (sets, 0, sizeof (sets));
+        reg_find_knots (knots, r, rdims, cdims);
+        double S = 0.0;
+        for (int n = 0; n < 9; n++) S += reg_region_smoothness (sets, coeff, &V[4096*n], knots);
+        S += reg_region_elastic (sets, coeff, V, knots);
+        for (int n = 21; n < 32; n++) S += reg_region_smoothness (sets, coeff, &V[4096*n], knots);
+        for (int s = 0; s < 64; s++)
+            for (int d = 0; d < 3; d++) cond[3 * (64 * knots[s] + s) + d] = sets[3*s + d];
+        Sr[r] = S;
+    }
+    double S = 0.0;
+    for (int64_t r = 0; r < nr; r++) S += Sr[r];
+    for (int64_t k = 0; k < nk; k++)
+        for (int d = 0; d < 3; d++) {
+            double a = 0.0;
+            for (int s = 0; s < 64; s++) a += cond[3 * (64 * k + s) + d];
+            grad[3*k + d] = a;
+        }
+    *score = S;
+    free (V); free (cond); free (Sr);
+    return 0;
+}
+
+/* The 32 V matrices, for tests. */
+int
+orc_reg_matrices (const orc_geom* g, const orc_reg_parms* rp, double* V)
+{
+    double gs[3];
+    for (int d = 0; d < 3; d++)
+        gs[d] = (double) ((float) g->vox_per_rgn[d] * fabsf (g->img_spacing[d]));
+    reg_analytic_init (V, gs, rp);
+    return 0;
+}

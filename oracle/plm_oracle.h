@@ -80,6 +80,25 @@ int orc_mse_score_f64 (const orc_geom* g,
 
 void orc_pv_quantize (const float w[8], int64_t q[8]);
 
+/* Regularization weights (src/register/regularization_parms.h:20-28). */
+typedef struct orc_reg_parms {
+    float total_displacement_penalty;
+    float diffusion_penalty;
+    float curvature_penalty;
+    float lame_coefficient_1;
+    float lame_coefficient_2;
+    float linear_elastic_multiplier;
+    float third_order_penalty;
+    float curvature_mixed_weight;
+} orc_reg_parms;
+
+/* Analytic regularization (flavor c): score and the gradient it adds,
+   both double, for the coefficients of g's grid. */
+int orc_reg_analytic (const orc_geom* g, const orc_reg_parms* rp, const float* coeff,
+    int nthreads, double* score, double* grad);
+/* Its 32 64x64 V matrices (32 * 4096 doubles). */
+int orc_reg_matrices (const orc_geom* g, const orc_reg_parms* rp, double* V);
+
 int orc_mi_score (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,

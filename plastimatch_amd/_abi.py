@@ -59,6 +59,14 @@ class PlmImgDesc(ctypes.Structure):
     ]
 
 
+class PlmRegParms(ctypes.Structure):
+    """Regularization weights (Regularization_parms)."""
+    _fields_ = [(n, ctypes.c_float) for n in (
+        "total_displacement_penalty", "diffusion_penalty", "curvature_penalty", "lame_coefficient_1",
+        "lame_coefficient_2", "linear_elastic_multiplier", "third_order_penalty",
+        "curvature_mixed_weight")]
+
+
 class PlmHistDesc(ctypes.Structure):
     _fields_ = [
         ("fixed_bins", ctypes.c_int64),
@@ -96,6 +104,8 @@ SIGNATURES = {
     "plm_hip_bspline_interpolate_vf": (ctypes.c_int, [_P, _P, _P, _P]),
     "plm_hip_bspline_score_vf_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "plm_hip_bspline_warp": (ctypes.c_int, [_P, _P, _P, ctypes.c_float, ctypes.c_int, _P]),
+    "plm_hip_bspline_regularize": (ctypes.c_int, [_P, ctypes.POINTER(PlmRegParms), _P, _P, _P, ctypes.c_int,
+                                                  _P]),
     "plm_hip_bspline_num_knots": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_bspline_num_tiles": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),

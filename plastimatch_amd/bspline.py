@@ -167,6 +167,12 @@ def volume_gradient(vol, gpuid=0):
     return Volume(out, vol.spacing, vol.origin, vol.direction_cosines)
 
 
+# Regularization_parms defaults (src/register/regularization_parms.h:31-42)
+REG_DEFAULTS = {"total_displacement_penalty": 0.0, "diffusion_penalty": 0.0, "curvature_penalty": 0.0,
+                "lame_coefficient_1": 0.0, "lame_coefficient_2": 0.0, "linear_elastic_multiplier": 1.0,
+                "third_order_penalty": 0.0, "curvature_mixed_weight": 1.0}
+
+
 class BsplineHip:
     """Device state for one stage: images resident in HBM, scored per evaluation."""
 
@@ -235,6 +241,22 @@ class BsplineHip:
         st = ctypes.c_void_p(stream) if stream else None
         return _abi.check(self.lib.plm_hip_bspline_interpolate_vf(
             self.ctx, ctypes.c_void_p(d_coeff.data_ptr()), ctypes.c_void_p(d_vf.data_ptr()), st))
+
+    def regularize(self, d_coeff, d_score, d_grad, accumulate=False, stream=None, **weights):
+        """Analytic smoothness regularization (flavor c): score into d_score,
+        gradient into (or added to) d_grad.  Weights by their
+        Regularization_parms names; unset ones take the reference defaults
+        (0, except linear_elastic_multiplier = curvature_mixed_weight = 1)."""
+        w = dict(REG_DEFAULTS)
+        unknown = set(weights) - set(w)
+        if unknown:
+            raise ValueError(f"unknown regularization weights: {sorted(unknown)}")
+        w.update(weights)
+        rp = _abi.PlmRegParms(**w)
+        st = ctypes.c_void_p(stream) if stream else None
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+        return _abi.check(self.lib.plm_hip_bspline_regularize(
+            self.ctx, ctypes.byref(rp), ptr(d_coeff), ptr(d_score), ptr(d_grad), 1 if accumulate else 0, st))
 
     def warp(self, d_coeff, d_out, default_val=0.0, linear=True, stream=None):
         """Warped moving image (one float per fixed voxel) into d_out: trilinear

@@ -176,6 +176,7 @@ struct plm_hip_ctx {
     int64_t num_knots = 0;
     int64_t num_tiles = 0;
     size_t lds_limit = 0;
+    float grid_spac[3] = {0.f, 0.f, 0.f};  // vox_per_rgn * |spacing| (Bspline_header::set)
 
     void* d_fixed = nullptr;
     void* d_moving = nullptr;
@@ -183,6 +184,8 @@ struct plm_hip_ctx {
     float* d_blut = nullptr;
     float* d_cond = nullptr;
     double* d_tile_score = nullptr;
+    double* d_reg_part = nullptr;           // regularization scratch (first use)
+    double* d_reg_cond = nullptr;
     int* d_tile_count = nullptr;
     plmhip::PairDev* d_pair = nullptr;      // this context as a one-pair table
     plmhip::Plan plan {};
@@ -229,7 +232,7 @@ free_ctx (plm_hip_ctx* c)
     void* bufs[] = {c->d_fixed, c->d_moving, c->d_mgrad, c->d_blut, c->d_cond,
         c->d_tile_score, c->d_tile_count, c->d_pair, c->d_hist,
         c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
-        c->d_hist_out};
+        c->d_hist_out, c->d_reg_part, c->d_reg_cond};
     for (void* b : bufs) if (b) (void) hipFree (b);
     if (c->stream) (void) hipStreamDestroy (c->stream);
     delete c;
@@ -462,6 +465,8 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     c->storage = fixed->storage;
     c->num_knots = knots;
     c->num_tiles = tiles;
+    for (int a = 0; a < 3; a++)
+        c->grid_spac[a] = (float) geom->vox_per_rgn[a] * std::fabs (geom->img_spacing[a]);
     c->lds_limit = prop.sharedMemPerBlock;
     c->tune.lds_limit = c->lds_limit;
     c->tune.num_cus = prop.multiProcessorCount;
@@ -751,6 +756,109 @@ plm_hip_bspline_interpolate_vf (plm_hip_ctx* c, const float* d_coeff, float* d_v
     hipStream_t st = stream ? (hipStream_t) stream : c->stream;
     hipError_t e = plmhip::launch_vf (c->plan, c->tune, d_coeff, d_vf, st);
     if (e != hipSuccess) return hip_fail (e, "displacement field launch");
+    return PLM_HIP_OK;
+}
+
+namespace {
+
+// V[4j+i] = sum_{a,b} I[a+b] Qn1[4j+a] Qn2[4i+b], I[n] = gs^(n+1)/(n+1):
+// one region's integral of a product of two basis polynomials (eval_integral,
+// src/register/bspline_regularize_analytic.cxx:167-199).
+void
+reg_integral (double* V, const double* Qn1, const double* Qn2, double gs)
+{
+    double I[7];
+    double pw = gs;
+    for (int n = 0; n < 7; n++) {
+        I[n] = pw / (double) (n + 1);
+        pw *= gs;
+    }
+    for (int j = 0; j < 4; j++)
+        for (int i = 0; i < 4; i++) {
+            double acc = 0.0;
+            for (int a = 0; a < 4; a++)
+                for (int b = 0; b < 4; b++) acc += I[a + b] * (Qn1[4 * j + a] * Qn2[4 * i + b]);
+            V[4 * j + i] = acc;
+        }
+}
+
+// The per-axis factors and weights of the analytic regularizer
+// (init_analytic :201-281 and analytic_init :638-940): Q_n = B R d^n, the
+// cubic basis coefficients scaled by the reciprocal grid spacing and
+// differentiated n times; factor k < 4 pairs Q_k with Q_k, 4 pairs (Q_1,
+// Q_0), 5 pairs (Q_0, Q_1).  The weights are formed in float as the
+// reference forms them.
+plmhip::RegOp
+reg_operator (const float grid_spac[3], const plm_reg_parms& rp)
+{
+    static const double B[16] = {
+        1.0 / 6.0, -1.0 / 2.0, 1.0 / 2.0, -1.0 / 6.0,
+        2.0 / 3.0, 0.0, -1.0, 1.0 / 2.0,
+        1.0 / 6.0, 1.0 / 2.0, 1.0 / 2.0, -1.0 / 2.0,
+        0.0, 0.0, 0.0, 1.0 / 6.0};
+    // d^n on polynomial coefficients (rows: power of u)
+    static const double D[3][16] = {
+        {0, 0, 0, 0, 1, 0, 0, 0, 0, 2, 0, 0, 0, 0, 3, 0},
+        {0, 0, 0, 0, 0, 0, 0, 0, 2, 0, 0, 0, 0, 6, 0, 0},
+        {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 6, 0, 0, 0}};
+    auto mul = [] (double* m1, const double* m2, const double* m3) {
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {
+                double acc = 0.0;
+                for (int k = 0; k < 4; k++) acc += m2[4 * i + k] * m3[4 * k + j];
+                m1[4 * i + j] = acc;
+            }
+    };
+    plmhip::RegOp op {};
+    double (*F[3])[16] = {op.X, op.Y, op.Z};
+    for (int a = 0; a < 3; a++) {
+        const double gs = (double) grid_spac[a];
+        const double r = 1.0 / gs;
+        const double R[16] = {1.0, 0, 0, 0, 0, r, 0, 0, 0, 0, r * r, 0, 0, 0, 0, r * r * r};
+        double Q[4][16];
+        mul (Q[0], B, R);
+        for (int n = 1; n < 4; n++) mul (Q[n], Q[0], D[n - 1]);
+        for (int n = 0; n < 4; n++) reg_integral (F[a][n], Q[n], Q[n], gs);
+        reg_integral (F[a][4], Q[1], Q[0], gs);
+        reg_integral (F[a][5], Q[0], Q[1], gs);
+    }
+    const float cp = rp.curvature_penalty, mix = rp.curvature_mixed_weight;
+    const float m = rp.linear_elastic_multiplier, l1 = rp.lame_coefficient_1, l2 = rp.lame_coefficient_2;
+    op.cp = cp;
+    op.cpm = (float) (cp * mix);
+    op.dp = rp.diffusion_penalty;
+    op.tdp = rp.total_displacement_penalty;
+    op.top = rp.third_order_penalty;
+    op.e1 = (float) (l1 * m);
+    op.e2 = (float) (l2 * m);
+    op.ed = (float) ((m) * (2 * l1 + l2) / 2);
+    op.eh = (float) ((m) * (l1) / 2);
+    return op;
+}
+
+} // namespace
+
+int
+plm_hip_bspline_regularize (plm_hip_ctx* c, const plm_reg_parms* parms, const float* d_coeff,
+    double* d_score, double* d_grad, int accumulate, void* stream)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    if (!parms || !d_coeff || !d_score || !d_grad) return fail (PLM_HIP_ERR_ARG, "NULL argument");
+    if (accumulate != 0 && accumulate != 1) return fail (PLM_HIP_ERR_ARG, "accumulate must be 0 or 1");
+    HIP_TRY (hipSetDevice (c->device));
+    hipStream_t st = stream ? (hipStream_t) stream : c->stream;
+    if (!c->d_reg_part) {
+        int rc = alloc (&c->d_reg_part, (size_t) c->num_tiles, "hipMalloc(reg partials)");
+        if (rc) return rc;
+        rc = alloc (&c->d_reg_cond, (size_t) 192 * c->num_knots, "hipMalloc(reg slots)");
+        if (rc) return rc;
+        // slots no region owns (knots near the grid's edge) stay zero for good
+        HIP_TRY (hipMemset (c->d_reg_cond, 0, sizeof (double) * 192 * (size_t) c->num_knots));
+    }
+    const plmhip::RegOp op = reg_operator (c->grid_spac, *parms);
+    hipError_t e = plmhip::launch_regularize (c->g, op, d_coeff, c->d_reg_part, c->d_reg_cond, d_score,
+        d_grad, accumulate, st);
+    if (e != hipSuccess) return hip_fail (e, "regularization launch");
     return PLM_HIP_OK;
 }
 

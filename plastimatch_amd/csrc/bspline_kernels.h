@@ -159,6 +159,23 @@ hipError_t launch_mse_from_vf (const Plan& plan, const Tune& tune, int storage, 
 // leaves the moving image and outside the ROI.
 hipError_t launch_warp (const Plan& plan, const Tune& tune, int storage, const float* coeff0,
     float* out, float fill, int linear, hipStream_t st);
+// Analytic regularization operator: the 4x4 region-integral matrices of
+// each axis (index 0..3: both factors of derivative order n; 4: orders
+// (1, 0); 5: orders (0, 1)) and the term weights, formed on the host as
+// Bspline_regularize::analytic_init does.
+struct RegOp {
+    double X[6][16];         // grid spacing 0: acts on the knot index's z digit
+    double Y[6][16];
+    double Z[6][16];         // grid spacing 2: acts on the x digit
+    double cp, cpm, dp, tdp, top;   // curvature, mixed curvature, diffusion, displacement, 3rd order
+    double e1, e2;           // elastic cross terms: lame_1 * m, lame_2 * m
+    double ed, eh;           // elastic diagonal: m (2 lame_1 + lame_2) / 2, m lame_1 / 2
+};
+// Regularization score and gradient of coefficients coeff over g's grid:
+// part (num_tiles doubles) and cond (64 * 3 * num_knots doubles) are
+// scratch; score = the score, grad = (accumulate ? grad : 0) + gradient.
+hipError_t launch_regularize (const KGeom& g, const RegOp& op, const float* coeff, double* part,
+    double* cond, double* score, double* grad, int accumulate, hipStream_t st);
 // Mattes MI of a single-pair plan.
 hipError_t launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     const uint8_t* fbin, const uint8_t* mbin, const float* coeff, unsigned long long* jhist,

@@ -1559,6 +1559,187 @@ void k_mi_score (HGeom h, const unsigned long long* __restrict__ jhist,
 }
 
 // ---------------------------------------------------------------------------
+// Analytic regularization (flavor c of Bspline_regularize,
+// src/register/bspline_regularize_analytic.cxx:943-1007).
+//
+// Per region, with p_a the 64 knot coefficients of component a indexed
+// 16 kz + 4 ky + kx (find_knots_3, :129-165), the reference sums p'Vp over
+// 32 matrices V = w X (x) Y (x) Z (get_Vmatrix, :283-338: X acts on the
+// z digit, Z on the x digit) and the elastic couplings.  Grouping equal
+// Kronecker monomials, that is
+//   S = sum_a p_a' O_a p_a + p_y' Exy p_x + p_z' Exz p_x + p_z' Eyz p_y,
+//   g_x = 2 O_x p_x + Exy p_y + Exz p_z   (and cyclically, :501-601),
+// and each operator is applied separably: a 4x4 contraction along the x
+// digit, then y, then z -- 276 FMAs per knot for the O_a instead of the
+// 64-term rows of 21 dense matrices.  One 64-thread workgroup per region,
+// thread t = knot slot t; all in double, as the reference.  The slot values
+// go slot-major to cond (as reg_sort_sets, :87-105) and a reduce sums each
+// knot's 64 slots in slot order (reg_update_grad, :110-127).
+
+// out(t) = sum_c M[digit(t)][c] in(t with that digit = c); digit 0 (x)
+// has stride 1, 1 (y) stride 4, 2 (z) stride 16.
+template <int D>
+__device__ __forceinline__ double
+reg_contract (const double* M, const double* in, int t)
+{
+    constexpr int s = D == 0 ? 1 : (D == 1 ? 4 : 16);
+    const int r = (t / s) & 3;
+    const double* b = in + (t - r * s);
+    return M[4 * r + 0] * b[0] + M[4 * r + 1] * b[s] + M[4 * r + 2] * b[2 * s] + M[4 * r + 3] * b[3 * s];
+}
+
+__global__ __launch_bounds__ (64)
+void k_regularize (KGeom g, RegOp op, const float* __restrict__ coeff, double* __restrict__ part,
+    double* __restrict__ cond)
+{
+    // per component a: p, the x-digit contractions Z0..Z3, Z01, Z10, and
+    // the staged sums of the y step
+    __shared__ double s_p[3][64];
+    __shared__ double s_z[3][6][64];
+    __shared__ double s_w[3][4][64];    // W_i: what X_i contracts
+    __shared__ double s_e[3][4][64];    // elastic: Y01 T0, Y10 T0, Y0 Z01p, Y0 Z10p (by component)
+    __shared__ double s_f[3][2][64];    // elastic: Y10 Z01p, Y01 Z10p
+    __shared__ double s_red[64];
+    const int t = threadIdx.x;
+    const int r = blockIdx.x;
+    const int r0 = r % g.rdims[0], r1 = (r / g.rdims[0]) % g.rdims[1], r2 = r / (g.rdims[0] * g.rdims[1]);
+    const int kx = t & 3, ky = (t >> 2) & 3, kz = t >> 4;
+    const int knot = (r0 + kx) + g.cdims[0] * ((r1 + ky) + g.cdims[1] * (r2 + kz));
+    double p[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        p[a] = (double) coeff[3 * knot + a];
+        s_p[a][t] = p[a];
+    }
+    const bool elastic = op.e1 != 0.0 || op.e2 != 0.0;
+    __syncthreads ();
+    // x digit: Z_l, l = 0..3, and the elastic Z01, Z10
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+#pragma unroll
+        for (int l = 0; l < 4; l++) s_z[a][l][t] = reg_contract<0> (op.Z[l], s_p[a], t);
+        if (elastic) {
+            s_z[a][4][t] = reg_contract<0> (op.Z[5], s_p[a], t);     // Z01: orders (0, 1)
+            s_z[a][5][t] = reg_contract<0> (op.Z[4], s_p[a], t);     // Z10: orders (1, 0)
+        }
+    }
+    __syncthreads ();
+    // y digit: U_jl = Y_j Z_l p, weighted into W_i (the monomials grouped by
+    // their X order i; component a's diagonal elastic weights folded in)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double dX = a == 0 ? op.ed : op.eh;
+        const double dY = a == 1 ? op.ed : op.eh;
+        const double dZ = a == 2 ? op.ed : op.eh;
+        const double* T0 = s_z[a][0];
+        const double* T1 = s_z[a][1];
+        const double* T2 = s_z[a][2];
+        const double* T3 = s_z[a][3];
+        const double u00 = reg_contract<1> (op.Y[0], T0, t);
+        const double u10 = reg_contract<1> (op.Y[1], T0, t);
+        const double u01 = reg_contract<1> (op.Y[0], T1, t);
+        const double u20 = reg_contract<1> (op.Y[2], T0, t);
+        const double u02 = reg_contract<1> (op.Y[0], T2, t);
+        const double u11 = reg_contract<1> (op.Y[1], T1, t);
+        const double u30 = reg_contract<1> (op.Y[3], T0, t);
+        const double u03 = reg_contract<1> (op.Y[0], T3, t);
+        const double u21 = reg_contract<1> (op.Y[2], T1, t);
+        const double u12 = reg_contract<1> (op.Y[1], T2, t);
+        s_w[a][0][t] = op.cp * (u20 + u02) + op.cpm * u11 + (op.dp + dY) * u10 + (op.dp + dZ) * u01
+            + op.tdp * u00 + op.top * (u30 + u03 + u21 + u12);
+        s_w[a][1][t] = op.cpm * (u10 + u01) + (op.dp + dX) * u00 + op.top * (u20 + u02 + u11);
+        s_w[a][2][t] = op.cp * u00 + op.top * (u10 + u01);
+        s_w[a][3][t] = op.top * u00;
+        if (elastic) {
+            s_e[a][0][t] = reg_contract<1> (op.Y[5], T0, t);          // Y01 Z0 p
+            s_e[a][1][t] = reg_contract<1> (op.Y[4], T0, t);          // Y10 Z0 p
+            s_e[a][2][t] = reg_contract<1> (op.Y[0], s_z[a][4], t);   // Y0 Z01 p
+            s_e[a][3][t] = reg_contract<1> (op.Y[0], s_z[a][5], t);   // Y0 Z10 p
+            s_f[a][0][t] = reg_contract<1> (op.Y[4], s_z[a][4], t);   // Y10 Z01 p
+            s_f[a][1][t] = reg_contract<1> (op.Y[5], s_z[a][5], t);   // Y01 Z10 p
+        }
+    }
+    __syncthreads ();
+    // z digit: O_a p_a = sum_i X_i W_i, and the couplings
+    //   Exy = e1 X10 Y01 Z0 + e2 X01 Y10 Z0,  Exz = e1 X10 Y0 Z01 + e2 X01 Y0 Z10,
+    //   Eyz = e1 X0 Y10 Z01 + e2 X0 Y01 Z10
+    double R[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        R[a] = reg_contract<2> (op.X[0], s_w[a][0], t) + reg_contract<2> (op.X[1], s_w[a][1], t)
+            + reg_contract<2> (op.X[2], s_w[a][2], t) + reg_contract<2> (op.X[3], s_w[a][3], t);
+    double Exy_x = 0, Exy_y = 0, Exz_x = 0, Exz_z = 0, Eyz_y = 0, Eyz_z = 0;
+    if (elastic) {
+        auto exy = [&] (int a) {
+            return op.e1 * reg_contract<2> (op.X[4], s_e[a][0], t) + op.e2 * reg_contract<2> (op.X[5], s_e[a][1], t);
+        };
+        auto exz = [&] (int a) {
+            return op.e1 * reg_contract<2> (op.X[4], s_e[a][2], t) + op.e2 * reg_contract<2> (op.X[5], s_e[a][3], t);
+        };
+        auto eyz = [&] (int a) {
+            return op.e1 * reg_contract<2> (op.X[0], s_f[a][0], t) + op.e2 * reg_contract<2> (op.X[0], s_f[a][1], t);
+        };
+        Exy_x = exy (0); Exy_y = exy (1);
+        Exz_x = exz (0); Exz_z = exz (2);
+        Eyz_y = eyz (1); Eyz_z = eyz (2);
+    }
+    const double gx = 2.0 * R[0] + Exy_y + Exz_z;
+    const double gy = 2.0 * R[1] + Exy_x + Eyz_z;
+    const double gz = 2.0 * R[2] + Exz_x + Eyz_y;
+    // slot t of this knot, slot-major: cond[t][3 knot + c]
+    const size_t plane = (size_t) 3 * g.num_knots;
+    cond[(size_t) t * plane + 3 * knot + 0] = gx;
+    cond[(size_t) t * plane + 3 * knot + 1] = gy;
+    cond[(size_t) t * plane + 3 * knot + 2] = gz;
+    // the region's score, summed over the wave in a fixed order
+    double sr = p[0] * R[0] + p[1] * R[1] + p[2] * R[2] + p[1] * Exy_x + p[2] * Exz_x + p[2] * Eyz_y;
+    s_red[t] = sr;
+    __syncthreads ();
+    if (t == 0) {
+        double a = 0.0;
+        for (int i = 0; i < 64; i++) a += s_red[i];
+        part[r] = a;
+    }
+}
+
+// grad[i] = (accumulate ? grad[i] : 0) + sum over the 64 slots in slot
+// order; workgroup 0 also sums the region scores in region order.
+__global__ __launch_bounds__ (256)
+void k_reg_reduce (const double* __restrict__ cond, const double* __restrict__ part, int nregions,
+    int nc, double* __restrict__ score, double* __restrict__ grad, int accumulate)
+{
+    __shared__ double s_q[4][64];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    const double* c = cond + (size_t) (16 * q) * nc + (i < nc ? i : 0);
+    double v[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) v[m] = c[(size_t) m * nc];
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < 16; m++) acc += v[m];
+    s_q[q][lane] = acc;
+    __syncthreads ();
+    if (q == 0 && i < nc) {
+        const double sum = ((s_q[0][lane] + s_q[1][lane]) + s_q[2][lane]) + s_q[3][lane];
+        grad[i] = accumulate ? grad[i] + sum : sum;
+    }
+    if (blockIdx.x == 0) {
+        __syncthreads ();
+        // 256 strided partial sums, then those in thread order
+        double a = 0.0;
+        for (int r = threadIdx.x; r < nregions; r += 256) a += part[r];
+        s_q[q][lane] = a;
+        __syncthreads ();
+        if (threadIdx.x == 0) {
+            double sum = 0.0;
+            for (int k = 0; k < 256; k++) sum += s_q[k >> 6][k & 63];
+            *score = sum;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // small kernels
 
 // Central-difference gradient with edge replicate, rotated by inv(dc),
@@ -1886,6 +2067,18 @@ hipError_t
 launch_zero (float* p, int64_t n, hipStream_t st)
 {
     hipLaunchKernelGGL (k_fill_f32, dim3 (1024), dim3 (256), 0, st, p, n, 0.f);
+    return hipGetLastError ();
+}
+
+hipError_t
+launch_regularize (const KGeom& g, const RegOp& op, const float* coeff, double* part, double* cond,
+    double* score, double* grad, int accumulate, hipStream_t st)
+{
+    hipLaunchKernelGGL (k_regularize, dim3 (g.num_tiles), dim3 (64), 0, st, g, op, coeff, part, cond);
+    LAUNCH_CHECK ();
+    const int nc = 3 * g.num_knots;
+    hipLaunchKernelGGL (k_reg_reduce, dim3 ((nc + 63) / 64), dim3 (256), 0, st, cond, part,
+        g.num_tiles, nc, score, grad, accumulate);
     return hipGetLastError ();
 }
 

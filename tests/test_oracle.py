@@ -365,3 +365,101 @@ def test_warp_nearest_rounds_half_away_and_roi_default():
         mask = np.ones(out.shape, bool)
         mask[1:15, 3:15, 2:17] = False
         assert np.all(out[mask] == -7.0)
+
+
+# ---------------------------------------------------------------- regularization
+
+def _reg_grid(dim=(30, 24, 20), spacing=(1.0, 1.5, 2.5), vpr=(5, 4, 4)):
+    g = oracle.geom(dim, spacing, vpr)
+    rdims, cdims, nk = oracle.header(g)
+    gs = tuple(float(np.float32(v) * np.float32(abs(s))) for v, s in zip(vpr, spacing))
+    return g, rdims, cdims, nk, gs
+
+
+def test_reg_constant_field_closed_form():
+    """A constant field has no derivatives: only the total-displacement
+    term is left, tdp * |c|^2 * (region volume) per region."""
+    g, rdims, cdims, nk, gs = _reg_grid()
+    rp = oracle.reg_parms(total_displacement_penalty=0.3, diffusion_penalty=0.7, curvature_penalty=1.1,
+                          lame_coefficient_1=0.4, lame_coefficient_2=0.9, third_order_penalty=0.2,
+                          curvature_mixed_weight=0.6)
+    c = np.array([1.5, -2.0, 0.25])
+    coeff = np.tile(c.astype(np.float32), nk)
+    S, _ = oracle.reg_analytic(g, rp, coeff)
+    nr = int(np.prod(rdims))
+    want = float(np.float32(0.3)) * float(c @ c) * gs[0] * gs[1] * gs[2] * nr
+    assert abs(S - want) < 1e-10 * want
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_reg_linear_field_axis_convention(axis):
+    """A field linear along knot axis `axis` (slope alpha per knot) has a
+    constant first derivative and no higher ones: only the diffusion term
+    remains, dp * alpha^2 / gs_u * (the other two spacings) per region,
+    where -- as get_Vmatrix lays out the Kronecker product -- the factor
+    built from grid spacing gs[0] acts on the knot index's z digit and
+    gs[2]'s on its x digit (u = 2 - axis)."""
+    g, rdims, cdims, nk, gs = _reg_grid()
+    rp = oracle.reg_parms(diffusion_penalty=0.7, curvature_penalty=1.1, third_order_penalty=0.2)
+    alpha = 1.75
+    idx = np.indices((cdims[2], cdims[1], cdims[0]))   # z, y, x knot indices
+    ramp = alpha * idx[2 - axis].ravel()
+    coeff = np.zeros((nk, 3), np.float32)
+    coeff[:, 1] = ramp
+    S, _ = oracle.reg_analytic(g, rp, coeff.ravel())
+    u = 2 - axis
+    other = [gs[d] for d in range(3) if d != u]
+    want = float(np.float32(0.7)) * alpha ** 2 / gs[u] * other[0] * other[1] * int(np.prod(rdims))
+    assert abs(S - want) < 1e-9 * want, (S, want)
+
+
+def test_reg_gradient_is_exact_derivative_when_lames_equal():
+    """With lame_1 = lame_2 every term is a symmetric quadratic form, so the
+    gradient is the exact derivative: central differences of a quadratic
+    are exact up to rounding."""
+    g, rdims, cdims, nk, gs = _reg_grid(dim=(20, 16, 12), vpr=(5, 4, 4))
+    rp = oracle.reg_parms(total_displacement_penalty=0.3, diffusion_penalty=0.7, curvature_penalty=1.1,
+                          lame_coefficient_1=0.4, lame_coefficient_2=0.4, linear_elastic_multiplier=1.3,
+                          third_order_penalty=0.2, curvature_mixed_weight=0.6)
+    rng = np.random.default_rng(4)
+    coeff = rng.normal(0, 2.0, 3 * nk).astype(np.float32)
+    S, grad = oracle.reg_analytic(g, rp, coeff)
+    h = 0.25          # a power of two: coeff +- h is exact in float
+    for i in rng.choice(3 * nk, 25, replace=False):
+        a = coeff.copy(); a[i] += h
+        b = coeff.copy(); b[i] -= h
+        fd = (oracle.reg_analytic(g, rp, a)[0] - oracle.reg_analytic(g, rp, b)[0]) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-8 * max(1.0, np.abs(grad).max())
+
+
+def test_reg_assembly_matches_dense_forms():
+    """The per-region assembly (knot lists, slot placement, slot sums, the
+    elastic couplings and their gradient expression, lame_1 != lame_2)
+    against a numpy evaluation of the same 64x64 forms over whole-grid
+    knot vectors."""
+    g, rdims, cdims, nk, gs = _reg_grid(dim=(15, 12, 12), vpr=(5, 4, 4))
+    rp = oracle.reg_parms(total_displacement_penalty=0.3, diffusion_penalty=0.7, curvature_penalty=1.1,
+                          lame_coefficient_1=0.4, lame_coefficient_2=0.9, linear_elastic_multiplier=1.3,
+                          third_order_penalty=0.2, curvature_mixed_weight=0.6)
+    V = oracle.reg_matrices(g, rp)
+    rng = np.random.default_rng(5)
+    coeff = rng.normal(0, 2.0, 3 * nk).astype(np.float32)
+    S, grad = oracle.reg_analytic(g, rp, coeff)
+    P = coeff.astype(np.float64).reshape(nk, 3)
+    sym = sum(V[n] for n in list(range(9)) + list(range(21, 32)))
+    Exy, Exz, Eyz = V[9] + V[10], V[11] + V[12], V[13] + V[14]
+    D = [V[15] + V[19] + V[20], V[16] + V[18] + V[20], V[17] + V[18] + V[19]]
+    S_ref = 0.0
+    G = np.zeros((nk, 3))
+    for r in range(int(np.prod(rdims))):
+        t0, t1, t2 = r % rdims[0], (r // rdims[0]) % rdims[1], r // (rdims[0] * rdims[1])
+        kn = np.array([cdims[0] * cdims[1] * (t2 + k) + cdims[0] * (t1 + j) + (t0 + i)
+                       for k in range(4) for j in range(4) for i in range(4)])
+        px, py, pz = P[kn, 0], P[kn, 1], P[kn, 2]
+        S_ref += px @ (sym + D[0]) @ px + py @ (sym + D[1]) @ py + pz @ (sym + D[2]) @ pz
+        S_ref += py @ (Exy @ px) + pz @ (Exz @ px) + pz @ (Eyz @ py)
+        G[kn, 0] += 2 * (sym + D[0]) @ px + Exy @ py + Exz @ pz
+        G[kn, 1] += 2 * (sym + D[1]) @ py + Exy @ px + Eyz @ pz
+        G[kn, 2] += 2 * (sym + D[2]) @ pz + Exz @ px + Eyz @ py
+    assert abs(S - S_ref) < 1e-10 * abs(S_ref)
+    assert np.abs(grad - G.ravel()).max() < 1e-10 * np.abs(G).max()
