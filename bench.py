@@ -20,7 +20,9 @@ single-pair entry point over the 16 pairs on one stream; both report
 aggregate pair-evaluations per second.  The "adjoint" object covers the
 differentiable path (torch.autograd.Function) on 192x192x96 with an 8-voxel
 grid: forward+backward time and memory of its two modes and a central
-finite-difference check of the adjoint.
+finite-difference check of the adjoint.  "warp_regularize" times the
+warped moving image and the analytic regularizer on the headline workload,
+with their rooflines and the oracle's versions timed beside them.
 """
 import argparse
 import json
@@ -275,6 +277,104 @@ def adjoint_leg(args, dev, want_errors):
     return out
 
 
+# The deformation-output and regularization rows, on the headline workload
+REG_WEIGHTS = dict(total_displacement_penalty=0.3, diffusion_penalty=0.7, curvature_penalty=1.1,
+                   lame_coefficient_1=0.4, lame_coefficient_2=0.9, linear_elastic_multiplier=1.3,
+                   third_order_penalty=0.2, curvature_mixed_weight=0.6)
+# FP64 operations per knot slot of k_regularize with every term on (FMA = 2):
+# per component the x-digit step 6 contractions x 7, the y-digit step 16
+# contractions x 7 + 31 for the weighted sums, the z-digit step 4
+# contractions x 7 + 3; the elastic couplings 6 x (2 x 7 + 3); the gradient 9
+# and the score 11 -- 3 x (42 + 143 + 31) + 102 + 20.
+REG_FLOPS_PER_KNOT = 3 * (42 + 143 + 31) + 6 * 17 + 20
+FP64_PEAK_TFLOPS = 78.6
+
+
+def warp_reg_leg(args, dev, want_cpu):
+    """The warped moving image (trilinear) and the analytic regularizer (all
+    terms on) on 256x256x128, vpr 10: kernel time with HIP events on the
+    stream they run on, rooflines (the warp reads a moving value and writes
+    a float per voxel, 8 B; the regularizer is FP64 arithmetic), and the
+    oracle's warp and regularizer timed beside them with their errors."""
+    import torch
+    from plastimatch_amd import synth
+    from plastimatch_amd.bspline import BsplineHip, Bspline_xform, Volume
+
+    pair = synth.make_config(WORKLOAD, seed=0)
+    fixed = Volume(torch.from_numpy(pair.fixed).to(dev), pair.spacing, pair.origin)
+    moving = Volume(torch.from_numpy(pair.moving).to(dev), pair.spacing, pair.origin)
+    bxf = Bspline_xform.for_volume(fixed, pair.vox_per_rgn)
+    st = BsplineHip(bxf, fixed, moving)
+    c = synth.random_coeff(pair.dim, pair.vox_per_rgn, 3.0, seed=100)
+    d_c = torch.from_numpy(c).to(dev)
+    npix = pair.fixed.size
+    out_img = torch.empty(npix, dtype=torch.float32, device=dev)
+    sc = torch.zeros(1, dtype=torch.float64, device=dev)
+    gr = torch.zeros(bxf.num_coeff, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+    steps = max(20, args.steps // 10)
+
+    def timed(fn):
+        for _ in range(5):
+            fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    warp_ms = timed(lambda: st.warp(d_c, out_img, -1000.0, True, stream=s))
+    reg_ms = timed(lambda: st.regularize(d_c, sc, gr, stream=s, **REG_WEIGHTS))
+    warp_bytes = 8 * npix + 4 * bxf.num_coeff
+    warp_gbs = warp_bytes / (warp_ms * 1e-3) / 1e9
+    reg_flops = REG_FLOPS_PER_KNOT * 64 * st.num_tiles
+    reg_tflops = reg_flops / (reg_ms * 1e-3) / 1e12
+    out = {
+        "workload": WORKLOAD,
+        "warp": {"ms": warp_ms, "voxels_per_s": npix / (warp_ms * 1e-3),
+                 "roofline": {"bound": "hbm", "achieved": warp_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": warp_gbs / HBM_PEAK_GBS, "alg_bytes_per_launch": warp_bytes,
+                              "kernel": "k_tiles<6> (+ no fill: the ROI is the image)"}},
+        "regularize": {"ms": reg_ms, "weights": REG_WEIGHTS, "regions": st.num_tiles,
+                       "roofline": {"bound": "mfma", "achieved": reg_tflops, "peak": FP64_PEAK_TFLOPS,
+                                    "unit": "TFLOP/s", "frac": reg_tflops / FP64_PEAK_TFLOPS,
+                                    "flops_per_launch": reg_flops, "dtype": "f64",
+                                    "kernel": "k_regularize + k_reg_reduce"}},
+    }
+    got_img = out_img.cpu().numpy().reshape(pair.fixed.shape)
+    got_s, got_g = sc.item(), gr.cpu().numpy()
+    st.close()
+    if want_cpu:
+        import oracle
+        threads = min(16, len(os.sched_getaffinity(0)))
+        g = oracle.geom(pair.dim, pair.spacing, pair.vox_per_rgn, origin=pair.origin)
+        t0 = time.perf_counter()
+        ref_img = oracle.warp(g, pair.moving, pair.spacing, c, True, -1000.0, m_origin=pair.origin,
+                              nthreads=threads)
+        t_warp = time.perf_counter() - t0
+        both = (got_img != -1000.0) & (ref_img != -1000.0)
+        span = float(pair.moving.max() - pair.moving.min())
+        t0 = time.perf_counter()
+        s_ref, g_ref = oracle.reg_analytic(g, oracle.reg_parms(**REG_WEIGHTS), c, nthreads=threads)
+        t_reg = time.perf_counter() - t0
+        out["warp"]["cpu_baseline"] = {"value": npix / t_warp, "unit": "voxels/s", "cores": threads,
+                                       "kind": "port", "sample": "one full warp (orc_warp, OpenMP)"}
+        out["warp"]["rms_err_vs_oracle_rel_range"] = float(
+            np.sqrt(((got_img[both].astype(np.float64) - ref_img[both]) ** 2).mean()) / span)
+        out["warp"]["default_mismatches"] = int(np.count_nonzero((got_img == -1000.0) != (ref_img == -1000.0)))
+        out["regularize"]["cpu_baseline"] = {"value": 1.0 / t_reg, "unit": "evals/s", "cores": threads,
+                                             "kind": "port",
+                                             "sample": "one evaluation, dense 64x64 forms (orc_reg_analytic)"}
+        out["regularize"]["rel_err_vs_oracle"] = {
+            "score": abs(got_s - s_ref) / abs(s_ref),
+            "grad_max_rel": float(np.abs(got_g - g_ref).max() / np.abs(g_ref).max())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +383,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-atlas leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-path leg")
+    ap.add_argument("--no-warp-reg", action="store_true", help="skip the warp and regularization leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy kernel 10 times "
                          "(reference for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -393,6 +494,8 @@ def main():
         out["batched"] = batched_leg(args, dev, want_errors=not args.no_cpu)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args, dev, want_errors=not args.no_cpu)
+    if not args.no_warp_reg:
+        out["warp_regularize"] = warp_reg_leg(args, dev, want_cpu=not args.no_cpu)
     print(json.dumps(out))
 
 
