@@ -23,6 +23,9 @@ grid: forward+backward time and memory of its two modes and a central
 finite-difference check of the adjoint.  "warp_regularize" times the
 warped moving image and the analytic regularizer on the headline workload,
 with their rooflines and the oracle's versions timed beside them.
+"optimizer_loop" runs the device L-BFGS (liblbfgs's algorithm) for 20
+iterations on the headline workload, with and without regularization, and
+the reference's liblbfgs on the oracle's score for 3 iterations beside it.
 """
 import argparse
 import json
@@ -375,6 +378,82 @@ def warp_reg_leg(args, dev, want_cpu):
     return out
 
 
+OPT_ITS = 20
+OPT_CPU_ITS = 3
+# regularization of the optimizer leg: the analytic penalties integrate over
+# the grid in mm^3, so weights of this size keep them about a tenth of the
+# MSE at the synthetic field
+OPT_REG_WEIGHTS = {"curvature_penalty": 1e-4, "diffusion_penalty": 1e-6}
+
+
+def optimizer_leg(args, dev, eval_ms, want_cpu):
+    """The optimizer loop on the device: liblbfgs's L-BFGS (More-Thuente,
+    defaults) over the headline workload's flavor-g MSE from the identity,
+    OPT_ITS iterations (max_feval unbounded), everything resident; wall time
+    of the whole call against the bare evaluation.  CPU beside it: the
+    reference's own liblbfgs (oracle/_ref) driving the oracle's flavor-g
+    score for OPT_CPU_ITS iterations, whose progress reports the device
+    run's first iterations are compared with."""
+    import torch
+    from plastimatch_amd import synth
+    from plastimatch_amd.bspline import BsplineHip, Bspline_xform, Volume, volume_gradient
+
+    pair = synth.make_config(WORKLOAD, seed=0)
+    fixed = Volume(torch.from_numpy(pair.fixed).to(dev), pair.spacing, pair.origin)
+    moving = Volume(torch.from_numpy(pair.moving).to(dev), pair.spacing, pair.origin)
+    bxf = Bspline_xform.for_volume(fixed, pair.vox_per_rgn)
+    st = BsplineHip(bxf, fixed, moving, volume_gradient(moving))
+    st.autotune("mse", "ref_nn", None, reps=5)
+    d_x = torch.zeros(bxf.num_coeff, dtype=torch.float64, device=dev)
+    d_c = torch.zeros(bxf.num_coeff, dtype=torch.float32, device=dev)
+    st.optimize_lbfgs(d_x, d_c, max_its=2, max_feval=-1)          # warm-up
+    out = {"workload": WORKLOAD, "optimizer": "L-BFGS, More-Thuente (liblbfgs 1.9 defaults)",
+           "start": "identity (zero coefficients)"}
+    runs = {}
+    for label, reg in (("mse", None), ("mse_plus_regularization", OPT_REG_WEIGHTS)):
+        d_x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = st.optimize_lbfgs(d_x, d_c, "mse", "ref_nn", reg, max_its=OPT_ITS, max_feval=-1)
+        dt = time.perf_counter() - t0
+        runs[label] = r
+        out[label] = {"status": r.status, "iterations": r.iterations, "fevals": r.fevals,
+                      "ms_per_iteration": dt * 1e3 / max(r.iterations, 1),
+                      "ms_per_evaluation_in_loop": dt * 1e3 / r.fevals,
+                      "fx_start": r.reports[0][0] if r.reports else None, "fx_end": r.fx}
+    out["mse_plus_regularization"]["weights"] = OPT_REG_WEIGHTS
+    out["mse"]["bare_evaluation_ms"] = eval_ms
+    out["mse"]["loop_overhead_ms_per_evaluation"] = out["mse"]["ms_per_evaluation_in_loop"] - eval_ms
+    st.close()
+    if want_cpu:
+        import oracle
+        threads = min(16, len(os.sched_getaffinity(0)))
+        case = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn)
+        _ = case.m_grad
+
+        def fun(x):
+            sc, g, _, _ = case.mse(x.astype(np.float32), "g", nthreads=threads)
+            return float(sc), g.astype(np.float64)
+
+        impl = "ref" if oracle.ref_lbfgs_lib() is not None else "oracle"
+        t0 = time.perf_counter()
+        ref = oracle.lbfgs(fun, np.zeros(bxf.num_coeff), impl=impl, max_its=OPT_CPU_ITS, max_feval=1000)
+        dt = time.perf_counter() - t0
+        dev_rep = runs["mse"].reports[:len(ref.reports)]
+        diffs = [abs(a[0] - b[0]) / abs(b[0]) for a, b in zip(dev_rep, ref.reports)]
+        out["cpu_baseline"] = {
+            "value": ref.feval / dt, "unit": "evaluations/s in the loop", "cores": threads,
+            "kind": "reference" if impl == "ref" else "port",
+            "ms_per_iteration": dt * 1e3 / max(ref.it, 1),
+            "sample": f"{ref.feval} evaluations ({OPT_CPU_ITS} iterations + the stopping trial) of the "
+                      "reference's liblbfgs driving the oracle's flavor-g score (OpenMP)"}
+        out["vs_cpu_reference_run"] = {
+            "reports_compared": len(diffs),
+            "same_iteration_record": [a[4:] for a in dev_rep] == [b[4:] for b in ref.reports],
+            "max_rel_diff_fx": max(diffs) if diffs else None}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,6 +463,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-atlas leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-path leg")
     ap.add_argument("--no-warp-reg", action="store_true", help="skip the warp and regularization leg")
+    ap.add_argument("--no-optimizer", action="store_true", help="skip the optimizer-loop leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy kernel 10 times "
                          "(reference for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -496,6 +576,8 @@ def main():
         out["adjoint"] = adjoint_leg(args, dev, want_errors=not args.no_cpu)
     if not args.no_warp_reg:
         out["warp_regularize"] = warp_reg_leg(args, dev, want_cpu=not args.no_cpu)
+    if not args.no_optimizer:
+        out["optimizer_loop"] = optimizer_leg(args, dev, ms_per_step, want_cpu=not args.no_cpu)
     print(json.dumps(out))
 
 

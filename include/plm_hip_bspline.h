@@ -304,6 +304,86 @@ int plm_hip_batch_get_timing (plm_hip_batch* b, double* stage_ms, int max_stages
 
 int plm_hip_batch_destroy (plm_hip_batch* b);
 
+/* ------------------------------------------------------------------------
+ * The optimizer loop on the device: L-BFGS with the More-Thuente line
+ * search, the algorithm of the liblbfgs 1.9 the reference vendors
+ * (libs/liblbfgs-1.9/lib/lbfgs.c:256-653, 823-1330) as
+ * bspline_optimize_liblbfgs runs it
+ * (src/register/bspline_optimize_liblbfgs.cxx:20-109): default parameters,
+ * orthantwise_c = 0, every line-search evaluation reported to the progress
+ * rule.  Coefficients, gradients, the search direction and the m history
+ * pairs stay in device memory (double); per evaluation the host reads back
+ * only the score and a few dot products. */
+
+/* Return codes of the optimizer (plm_lbfgs_result::status): liblbfgs's
+ * values (libs/liblbfgs-1.9/include/lbfgs.h:72-145). */
+#define PLM_LBFGS_SUCCESS                   0
+#define PLM_LBFGS_STOP                      1
+#define PLM_LBFGS_ALREADY_MINIMIZED         2
+#define PLM_LBFGSERR_ROUNDING_ERROR     (-1001)
+#define PLM_LBFGSERR_MINIMUMSTEP        (-1000)
+#define PLM_LBFGSERR_MAXIMUMSTEP         (-999)
+#define PLM_LBFGSERR_MAXIMUMLINESEARCH   (-998)
+#define PLM_LBFGSERR_MAXIMUMITERATION    (-997)
+#define PLM_LBFGSERR_WIDTHTOOSMALL       (-996)
+#define PLM_LBFGSERR_INVALIDPARAMETERS   (-995)
+#define PLM_LBFGSERR_INCREASEGRADIENT    (-994)
+#define PLM_LBFGSERR_OUTOFINTERVAL      (-1003)
+#define PLM_LBFGSERR_INCORRECT_TMINMAX  (-1002)
+
+/* The lbfgs_parameter_t fields the More-Thuente path reads
+ * (libs/liblbfgs-1.9/include/lbfgs.h:200-330), and the reference's
+ * progress limits Bspline_parms::max_its / max_feval
+ * (src/register/bspline_parms.cxx:16-17): the run stops once the iteration
+ * number exceeds max_its or the evaluations exceed max_feval
+ * (progress, src/register/bspline_optimize_liblbfgs.cxx:55-80); a negative
+ * limit is no limit. */
+typedef struct plm_lbfgs_parms {
+    int m;                 /* history pairs, 1..32 */
+    double epsilon;
+    int past;              /* 0, or the period of the delta test, <= 64 */
+    double delta;
+    int max_iterations;
+    int max_linesearch;
+    double min_step;
+    double max_step;
+    double ftol;
+    double gtol;
+    double xtol;
+    int max_its;
+    int max_feval;
+} plm_lbfgs_parms;
+
+/* liblbfgs's defaults (lbfgs_parameter_init, lbfgs.c:113-118, 240-243)
+ * and the reference's max_its = max_feval = 10. */
+void plm_hip_lbfgs_parms_init (plm_lbfgs_parms* p);
+
+/* Called after every evaluation inside the line search (ls = 0, fx the
+ * line search's best value so far) and after every accepted step (ls =
+ * the evaluations it took), as liblbfgs's progress callback; d_x and d_g
+ * are device pointers valid during the call.  Nonzero stops the run. */
+typedef int (*plm_lbfgs_progress_fn) (void* instance, const double* d_x, const double* d_g,
+    double fx, double xnorm, double gnorm, double step, int n, int k, int ls);
+
+typedef struct plm_lbfgs_result {
+    int status;            /* PLM_LBFGS_* / PLM_LBFGSERR_* */
+    int iterations;        /* k of the last progress report (Bspline_state::it) */
+    int fevals;            /* evaluations (Bspline_state::feval) */
+    double fx;             /* objective at exit, as lbfgs()'s *ptr_fx */
+} plm_lbfgs_result;
+
+/* Minimize score + regularization over the context's coefficients.  The
+ * objective is plm_hip_bspline_score_device's score (metric, grad_mode)
+ * plus, when reg is not NULL, plm_hip_bspline_regularize's score, with
+ * the sum of their FP64 gradients.  d_x: 3 * num_knots doubles, the start
+ * point in, lbfgs()'s x at exit out; d_coeff (may be NULL): the
+ * coefficients of the last evaluation, as Bspline_xform::coeff holds them
+ * when the reference's run ends.  parms NULL: the defaults.  Runs on the
+ * context's stream and returns when the run has ended. */
+int plm_hip_bspline_optimize_lbfgs (plm_hip_ctx* ctx, int metric, int grad_mode,
+    const plm_reg_parms* reg, const plm_lbfgs_parms* parms, double* d_x, float* d_coeff,
+    plm_lbfgs_progress_fn progress, void* instance, plm_lbfgs_result* result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,7 +55,37 @@ class OrcHist(ctypes.Structure):
                 ("m_offset", ctypes.c_float), ("m_delta", ctypes.c_float)]
 
 
+class OrcLbfgsParms(ctypes.Structure):
+    """The lbfgs_parameter_t fields the More-Thuente path reads."""
+    _fields_ = [("m", ctypes.c_int), ("epsilon", ctypes.c_double), ("past", ctypes.c_int),
+                ("delta", ctypes.c_double), ("max_iterations", ctypes.c_int),
+                ("max_linesearch", ctypes.c_int), ("min_step", ctypes.c_double),
+                ("max_step", ctypes.c_double), ("ftol", ctypes.c_double), ("gtol", ctypes.c_double),
+                ("xtol", ctypes.c_double)]
+
+
+class RefLbfgsParameter(ctypes.Structure):
+    """lbfgs_parameter_t of the reference's liblbfgs 1.9
+    (libs/liblbfgs-1.9/include/lbfgs.h:200-360), for the build in _ref/."""
+    _fields_ = [("m", ctypes.c_int), ("epsilon", ctypes.c_double), ("past", ctypes.c_int),
+                ("delta", ctypes.c_double), ("max_iterations", ctypes.c_int),
+                ("linesearch", ctypes.c_int), ("max_linesearch", ctypes.c_int),
+                ("min_step", ctypes.c_double), ("max_step", ctypes.c_double), ("ftol", ctypes.c_double),
+                ("wolfe", ctypes.c_double), ("gtol", ctypes.c_double), ("xtol", ctypes.c_double),
+                ("orthantwise_c", ctypes.c_double), ("orthantwise_start", ctypes.c_int),
+                ("orthantwise_end", ctypes.c_int)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+LBFGS_EVAL = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double)
+LBFGS_PROGRESS = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int)
+
+REF_DIR = os.path.join(HERE, "_ref")
+REF_LBFGS_PATH = os.path.join(REF_DIR, "liblbfgs_ref.so")
+
 _lib = None
+_ref_lbfgs = None
 
 
 def build():
@@ -103,6 +133,10 @@ def lib():
         L.orc_mi_score_f64.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
                                        ctypes.POINTER(OrcVolHdr), P, P, ctypes.POINTER(OrcHist),
                                        ctypes.c_int, P, P, P]
+        L.orc_lbfgs_parms_init.argtypes = [ctypes.POINTER(OrcLbfgsParms)]
+        L.orc_lbfgs.restype = ctypes.c_int
+        L.orc_lbfgs.argtypes = [ctypes.c_int, P, P, LBFGS_EVAL, LBFGS_PROGRESS, P,
+                                ctypes.POINTER(OrcLbfgsParms)]
         _lib = L
     return _lib
 
@@ -309,3 +343,91 @@ class Case:
                                     ctypes.byref(hp), nthreads, _ptr(sc), _ptr(grad), _ptr(nv))
         assert rc == 0, rc
         return float(sc[0]), grad, int(nv[0])
+
+
+def ref_lbfgs_lib():
+    """The reference's liblbfgs built from its source (oracle/_ref), or None
+    when it was not built (no reference sources where it was compiled)."""
+    global _ref_lbfgs
+    if _ref_lbfgs is None and os.path.exists(REF_LBFGS_PATH):
+        L = ctypes.CDLL(REF_LBFGS_PATH)
+        L.lbfgs_parameter_init.argtypes = [ctypes.POINTER(RefLbfgsParameter)]
+        L.lbfgs.restype = ctypes.c_int
+        L.lbfgs.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, LBFGS_EVAL, LBFGS_PROGRESS,
+                            ctypes.c_void_p, ctypes.POINTER(RefLbfgsParameter)]
+        _ref_lbfgs = L
+    return _ref_lbfgs
+
+
+ORC_LBFGS_SUCCESS = 0
+ORC_LBFGS_STOP = 1
+ORC_LBFGS_ALREADY_MINIMIZED = 2
+ORC_LBFGSERR_INVALID_EPSILON = -1017
+ORC_LBFGSERR_MAXIMUMLINESEARCH = -998
+ORC_LBFGSERR_INCREASEGRADIENT = -994
+
+LBFGS_DEFAULTS = dict(m=6, epsilon=1e-5, past=0, delta=1e-5, max_iterations=0, max_linesearch=40,
+                      min_step=1e-20, max_step=1e20, ftol=1e-4, gtol=0.9, xtol=1e-16)
+
+
+class LbfgsRun:
+    """Outcome of one L-BFGS run: liblbfgs status, x, fx at exit, and the
+    trace -- one entry per evaluation (f, step) and per progress report
+    (fx, xnorm, gnorm, step, k, ls)."""
+
+    def __init__(self):
+        self.status = None
+        self.x = None
+        self.fx = None
+        self.evals = []
+        self.reports = []
+        self.it = 0
+        self.feval = 0
+
+
+def lbfgs(fun, x0, impl="oracle", max_its=None, max_feval=None, **parms):
+    """Minimize fun (x -> (f, g), float64) from x0 with the oracle's L-BFGS
+    (impl="oracle") or the reference's liblbfgs build (impl="ref").
+
+    max_its / max_feval add the reference's progress rule
+    (src/register/bspline_optimize_liblbfgs.cxx:55-80): stop once the
+    iteration number exceeds max_its or the evaluations exceed max_feval."""
+    p = dict(LBFGS_DEFAULTS)
+    p.update(parms)
+    n = len(x0)
+    x = np.array(x0, dtype=np.float64, copy=True)
+    fx = np.zeros(1, np.float64)
+    run = LbfgsRun()
+
+    def _eval(inst, xp, gp, nn, step):
+        xa = np.ctypeslib.as_array(xp, shape=(nn,))
+        f, g = fun(xa.copy())
+        np.ctypeslib.as_array(gp, shape=(nn,))[:] = g
+        run.feval += 1
+        run.evals.append((float(f), float(step)))
+        return float(f)
+
+    def _progress(inst, xp, gp, f, xnorm, gnorm, step, nn, k, ls):
+        run.reports.append((float(f), float(xnorm), float(gnorm), float(step), int(k), int(ls)))
+        run.it = k
+        if (max_its is not None and run.it > max_its) or (max_feval is not None and run.feval > max_feval):
+            return 1
+        return 0
+
+    ev, pr = LBFGS_EVAL(_eval), LBFGS_PROGRESS(_progress)
+    if impl == "oracle":
+        op = OrcLbfgsParms(**p)
+        run.status = lib().orc_lbfgs(n, _ptr(x), _ptr(fx), ev, pr, None, ctypes.byref(op))
+    elif impl == "ref":
+        L = ref_lbfgs_lib()
+        if L is None:
+            raise FileNotFoundError(REF_LBFGS_PATH)
+        rp = RefLbfgsParameter()
+        L.lbfgs_parameter_init(ctypes.byref(rp))
+        for k, v in p.items():
+            setattr(rp, k, v)
+        run.status = L.lbfgs(n, _ptr(x), _ptr(fx), ev, pr, None, ctypes.byref(rp))
+    else:
+        raise ValueError(impl)
+    run.x, run.fx = x, float(fx[0])
+    return run

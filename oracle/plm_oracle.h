@@ -113,6 +113,60 @@ int orc_mi_score_f64 (const orc_geom* g,
     const float* coeff, const orc_hist* hp, int nthreads,
     double* score, double* grad, int64_t* num_vox);
 
+/* L-BFGS (lbfgs_oracle.c): the reference's vendored liblbfgs 1.9 on its
+   More-Thuente path.  Return codes keep liblbfgs's values
+   (libs/liblbfgs-1.9/include/lbfgs.h:72-145). */
+enum {
+    ORC_LBFGS_SUCCESS = 0,
+    ORC_LBFGS_STOP = 1,
+    ORC_LBFGS_ALREADY_MINIMIZED = 2,
+    ORC_LBFGSERR_OUTOFMEMORY = -1022,
+    ORC_LBFGSERR_INVALID_N = -1020,
+    ORC_LBFGSERR_INVALID_EPSILON = -1017,
+    ORC_LBFGSERR_INVALID_TESTPERIOD = -1016,
+    ORC_LBFGSERR_INVALID_DELTA = -1015,
+    ORC_LBFGSERR_INVALID_MINSTEP = -1013,
+    ORC_LBFGSERR_INVALID_MAXSTEP = -1012,
+    ORC_LBFGSERR_INVALID_FTOL = -1011,
+    ORC_LBFGSERR_INVALID_GTOL = -1009,
+    ORC_LBFGSERR_INVALID_XTOL = -1008,
+    ORC_LBFGSERR_INVALID_MAXLINESEARCH = -1007,
+    ORC_LBFGSERR_OUTOFINTERVAL = -1003,
+    ORC_LBFGSERR_INCORRECT_TMINMAX = -1002,
+    ORC_LBFGSERR_ROUNDING_ERROR = -1001,
+    ORC_LBFGSERR_MINIMUMSTEP = -1000,
+    ORC_LBFGSERR_MAXIMUMSTEP = -999,
+    ORC_LBFGSERR_MAXIMUMLINESEARCH = -998,
+    ORC_LBFGSERR_MAXIMUMITERATION = -997,
+    ORC_LBFGSERR_WIDTHTOOSMALL = -996,
+    ORC_LBFGSERR_INVALIDPARAMETERS = -995,
+    ORC_LBFGSERR_INCREASEGRADIENT = -994
+};
+
+typedef double (*orc_lbfgs_eval_fn) (void* inst, const double* x, double* g, int n,
+    double step);
+typedef int (*orc_lbfgs_progress_fn) (void* inst, const double* x, const double* g,
+    double fx, double xnorm, double gnorm, double step, int n, int k, int ls);
+
+/* The fields of lbfgs_parameter_t that the More-Thuente path reads. */
+typedef struct orc_lbfgs_parms {
+    int m;
+    double epsilon;
+    int past;
+    double delta;
+    int max_iterations;
+    int max_linesearch;
+    double min_step;
+    double max_step;
+    double ftol;
+    double gtol;
+    double xtol;
+} orc_lbfgs_parms;
+
+void orc_lbfgs_parms_init (orc_lbfgs_parms* p);
+int orc_lbfgs (int n, double* x, double* fx, orc_lbfgs_eval_fn eval,
+    orc_lbfgs_progress_fn progress, void* inst, const orc_lbfgs_parms* p);
+
 #ifdef __cplusplus
 }
 #endif

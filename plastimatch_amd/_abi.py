@@ -67,6 +67,33 @@ class PlmRegParms(ctypes.Structure):
         "curvature_mixed_weight")]
 
 
+class PlmLbfgsParms(ctypes.Structure):
+    """L-BFGS parameters (plm_lbfgs_parms: liblbfgs's lbfgs_parameter_t
+    fields of the More-Thuente path plus the reference's max_its and
+    max_feval)."""
+    _fields_ = [("m", ctypes.c_int), ("epsilon", ctypes.c_double), ("past", ctypes.c_int),
+                ("delta", ctypes.c_double), ("max_iterations", ctypes.c_int),
+                ("max_linesearch", ctypes.c_int), ("min_step", ctypes.c_double),
+                ("max_step", ctypes.c_double), ("ftol", ctypes.c_double), ("gtol", ctypes.c_double),
+                ("xtol", ctypes.c_double), ("max_its", ctypes.c_int), ("max_feval", ctypes.c_int)]
+
+
+class PlmLbfgsResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int), ("fevals", ctypes.c_int),
+                ("fx", ctypes.c_double)]
+
+
+# progress callback: (instance, d_x, d_g, fx, xnorm, gnorm, step, n, k, ls) -> int
+PLM_LBFGS_PROGRESS = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int)
+
+PLM_LBFGS_SUCCESS = 0
+PLM_LBFGS_STOP = 1
+PLM_LBFGS_ALREADY_MINIMIZED = 2
+PLM_LBFGSERR_MAXIMUMLINESEARCH = -998
+
+
 class PlmHistDesc(ctypes.Structure):
     _fields_ = [
         ("fixed_bins", ctypes.c_int64),
@@ -121,6 +148,10 @@ SIGNATURES = {
     "plm_hip_batch_get_timing": (ctypes.c_int, [_P, _P, ctypes.c_int,
                                                 ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_batch_destroy": (ctypes.c_int, [_P]),
+    "plm_hip_lbfgs_parms_init": (None, [ctypes.POINTER(PlmLbfgsParms)]),
+    "plm_hip_bspline_optimize_lbfgs": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(PlmRegParms),
+                                                      ctypes.POINTER(PlmLbfgsParms), _P, _P, PLM_LBFGS_PROGRESS,
+                                                      _P, ctypes.POINTER(PlmLbfgsResult)]),
 }
 
 _lib = None

@@ -173,6 +173,19 @@ REG_DEFAULTS = {"total_displacement_penalty": 0.0, "diffusion_penalty": 0.0, "cu
                 "third_order_penalty": 0.0, "curvature_mixed_weight": 1.0}
 
 
+@dataclass
+class LbfgsResult:
+    """Outcome of plm_hip_bspline_optimize_lbfgs: liblbfgs's status, the
+    iteration number and evaluation count at exit (Bspline_state::it and
+    ::feval), the objective lbfgs() returns, and every progress report
+    (fx, xnorm, gnorm, step, k, ls)."""
+    status: int
+    iterations: int
+    fevals: int
+    fx: float
+    reports: list
+
+
 class BsplineHip:
     """Device state for one stage: images resident in HBM, scored per evaluation."""
 
@@ -266,6 +279,47 @@ class BsplineHip:
         return _abi.check(self.lib.plm_hip_bspline_warp(
             self.ctx, ctypes.c_void_p(d_coeff.data_ptr()), ctypes.c_void_p(d_out.data_ptr()),
             float(default_val), 1 if linear else 0, st))
+
+    def optimize_lbfgs(self, d_x, d_coeff=None, metric="mse", grad_mode="ref_nn", reg=None, progress=None,
+                       **parms):
+        """L-BFGS on the device (plm_hip_bspline_optimize_lbfgs): minimize the
+        score (+ the analytic regularization with weights ``reg``, a dict of
+        Regularization_parms names) from the float64 device tensor d_x, which
+        holds the x liblbfgs returns at exit; d_coeff (float32 device tensor,
+        optional) gets the coefficients of the last evaluation.  ``parms``:
+        plm_lbfgs_parms fields (liblbfgs names, max_its, max_feval; unset ones
+        take liblbfgs's defaults and max_its = max_feval = 10).  ``progress``:
+        callable (d_x_ptr, d_g_ptr, fx, xnorm, gnorm, step, n, k, ls) -> int,
+        nonzero stops.  Returns an LbfgsResult."""
+        if d_x.numel() != self.bxf.num_coeff:
+            raise ValueError("d_x has the wrong length")
+        p = _abi.PlmLbfgsParms()
+        self.lib.plm_hip_lbfgs_parms_init(ctypes.byref(p))
+        for k, v in parms.items():
+            if not hasattr(p, k) or k.startswith("_"):
+                raise ValueError(f"unknown L-BFGS parameter {k}")
+            setattr(p, k, v)
+        rp = None
+        if reg is not None:
+            w = dict(REG_DEFAULTS)
+            unknown = set(reg) - set(w)
+            if unknown:
+                raise ValueError(f"unknown regularization weights: {sorted(unknown)}")
+            w.update(reg)
+            rp = _abi.PlmRegParms(**w)
+        reports = []
+
+        def _progress(inst, px, pg, fx, xnorm, gnorm, step, n, k, ls):
+            reports.append((fx, xnorm, gnorm, step, k, ls))
+            return int(progress(px, pg, fx, xnorm, gnorm, step, n, k, ls)) if progress else 0
+
+        cb = _abi.PLM_LBFGS_PROGRESS(_progress)
+        res = _abi.PlmLbfgsResult()
+        _abi.check(self.lib.plm_hip_bspline_optimize_lbfgs(
+            self.ctx, METRICS[metric], GRAD_MODES[grad_mode], ctypes.byref(rp) if rp is not None else None,
+            ctypes.byref(p), ctypes.c_void_p(d_x.data_ptr()),
+            ctypes.c_void_p(d_coeff.data_ptr()) if d_coeff is not None else None, cb, None, ctypes.byref(res)))
+        return LbfgsResult(res.status, res.iterations, res.fevals, res.fx, reports)
 
     def score_vf_device(self, d_vf, d_score, d_grad, d_num_vox, stream=None):
         """MSE score and exact-derivative gradient from a stored displacement field."""

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "plm_hip_bspline.h"
+#include "bspline_internal.h"
 #include "bspline_kernels.h"
 
 using plmhip::HGeom;
@@ -913,6 +914,32 @@ plm_hip_bspline_num_tiles (const plm_hip_ctx* c, int64_t* n)
 }
 
 } // extern "C"
+
+namespace plmhip {
+
+int
+abi_fail (int code, const std::string& msg)
+{
+    return fail (code, msg);
+}
+
+int
+abi_hip_fail (hipError_t e, const char* what)
+{
+    return hip_fail (e, what);
+}
+
+int
+ctx_info (const plm_hip_ctx* c, int* device, hipStream_t* stream, int64_t* num_knots)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    *device = c->device;
+    *stream = c->stream;
+    *num_knots = c->num_knots;
+    return PLM_HIP_OK;
+}
+
+} // namespace plmhip
 
 namespace {
 

@@ -1,0 +1,535 @@
+// The optimizer loop on the device: plm_hip_bspline_optimize_lbfgs.
+//
+// The algorithm is the one liblbfgs 1.9 runs for the reference
+// (libs/liblbfgs-1.9/lib/lbfgs.c:256-653, 823-1330; driven by
+// src/register/bspline_optimize_liblbfgs.cxx:83-109): L-BFGS two-loop
+// recursion over the last m (s, y) pairs with H0 = (ys/yy) I, and the
+// More-Thuente line search with every evaluation reported to the progress
+// rule.  Its vectors live in device memory (lbfgs_kernels.hip); this file
+// holds the scalar logic, which needs per evaluation only the objective
+// and three dot products (g.d, x.x, g.g), read back in one copy.
+//
+// The first trial of each line search is enqueued right behind the
+// direction kernels, before the host knows the initial slope g.d: the
+// slope's partials come back with that trial's results.  Should the
+// slope be positive (liblbfgs returns LBFGSERR_INCREASEGRADIENT before
+// evaluating anything), the trial is dropped uncounted -- its coefficients
+// went to the spare buffer -- so the run's record matches liblbfgs's.
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "plm_hip_bspline.h"
+#include "bspline_internal.h"
+#include "lbfgs_kernels.h"
+
+using plmhip::LbScalars;
+using plmhip::lb_reduce_parts_host;
+
+namespace {
+
+constexpr int MAX_PAST = 64;
+
+// One step of the line search: step, value, slope along the direction.
+struct Pt {
+    double st, f, g;
+};
+
+double
+lmax (double a, double b)
+{
+    return a >= b ? a : b;
+}
+
+double
+lmin (double a, double b)
+{
+    return a <= b ? a : b;
+}
+
+// Minimizer of the cubic interpolating (u, fu, du) and (v, fv, dv)
+// (More and Thuente, ACM TOMS 20(3), 1994, section 4; lbfgs.c:1043-1057).
+double
+cubic_min (double u, double fu, double du, double v, double fv, double dv)
+{
+    const double span = v - u;
+    const double theta = (fu - fv) * 3 / span + du + dv;
+    const double s = lmax (lmax (std::fabs (theta), std::fabs (du)), std::fabs (dv));
+    const double a = theta / s;
+    double gamma = s * std::sqrt (a * a - (du / s) * (dv / s));
+    if (v < u) gamma = -gamma;
+    const double p = gamma - du + theta;
+    const double q = gamma - du + gamma + dv;
+    return u + p / q * span;
+}
+
+// The cubic step that falls back to tmin / tmax when the cubic has no
+// minimizer beyond v (lbfgs.c:1071-1091).
+double
+cubic_min_safe (double u, double fu, double du, double v, double fv, double dv, double tmin,
+    double tmax)
+{
+    const double span = v - u;
+    const double theta = (fu - fv) * 3 / span + du + dv;
+    const double s = lmax (lmax (std::fabs (theta), std::fabs (du)), std::fabs (dv));
+    const double a = theta / s;
+    double gamma = s * std::sqrt (lmax (0, a * a - (du / s) * (dv / s)));
+    if (u < v) gamma = -gamma;
+    const double p = gamma - dv + theta;
+    const double q = gamma - dv + gamma + du;
+    const double r = p / q;
+    if (r < 0. && gamma != 0.) return v - r * span;
+    return a < 0 ? tmax : tmin;
+}
+
+// Quadratic through (u, fu, du) and (v, fv); secant through (u, du), (v, dv)
+// (lbfgs.c:1103-1118).
+double
+quad_min (double u, double fu, double du, double v, double fv)
+{
+    const double a = v - u;
+    return u + du / ((fu - fv) / a + du) / 2 * a;
+}
+
+double
+secant_min (double u, double du, double v, double dv)
+{
+    const double a = u - v;
+    return v + dv / (dv - du) * a;
+}
+
+// Next trial step and the new interval of uncertainty [bx, by]
+// (update_trial_interval, lbfgs.c:1159-1326).
+int
+next_trial (Pt& bx, Pt& by, Pt& t, double tmin, double tmax, int& brackt)
+{
+    const bool opposite = t.g * (bx.g / std::fabs (bx.g)) < 0.;
+    bool bound;
+    double next;
+    if (brackt) {
+        if (t.st <= lmin (bx.st, by.st) || lmax (bx.st, by.st) <= t.st)
+            return PLM_LBFGSERR_OUTOFINTERVAL;
+        if (0. <= bx.g * (t.st - bx.st)) return PLM_LBFGSERR_INCREASEGRADIENT;
+        if (tmax < tmin) return PLM_LBFGSERR_INCORRECT_TMINMAX;
+    }
+    if (bx.f < t.f) {
+        brackt = 1;
+        bound = true;
+        const double mc = cubic_min (bx.st, bx.f, bx.g, t.st, t.f, t.g);
+        const double mq = quad_min (bx.st, bx.f, bx.g, t.st, t.f);
+        next = std::fabs (mc - bx.st) < std::fabs (mq - bx.st) ? mc : mc + 0.5 * (mq - mc);
+    } else if (opposite) {
+        brackt = 1;
+        bound = false;
+        const double mc = cubic_min (bx.st, bx.f, bx.g, t.st, t.f, t.g);
+        const double mq = secant_min (bx.st, bx.g, t.st, t.g);
+        next = std::fabs (mc - t.st) > std::fabs (mq - t.st) ? mc : mq;
+    } else if (std::fabs (t.g) < std::fabs (bx.g)) {
+        bound = true;
+        const double mc = cubic_min_safe (bx.st, bx.f, bx.g, t.st, t.f, t.g, tmin, tmax);
+        const double mq = secant_min (bx.st, bx.g, t.st, t.g);
+        if (brackt) next = std::fabs (t.st - mc) < std::fabs (t.st - mq) ? mc : mq;
+        else next = std::fabs (t.st - mc) > std::fabs (t.st - mq) ? mc : mq;
+    } else {
+        bound = false;
+        if (brackt) next = cubic_min (t.st, t.f, t.g, by.st, by.f, by.g);
+        else next = bx.st < t.st ? tmax : tmin;
+    }
+    if (bx.f < t.f) {
+        by = t;
+    } else {
+        if (opposite) by = bx;
+        bx = t;
+    }
+    if (tmax < next) next = tmax;
+    if (next < tmin) next = tmin;
+    if (brackt && bound) {
+        const double mq = bx.st + 0.66 * (by.st - bx.st);
+        if (bx.st < by.st) {
+            if (mq < next) next = mq;
+        } else {
+            if (next < mq) next = mq;
+        }
+    }
+    t.st = next;
+    return 0;
+}
+
+// Device and pinned-host state of one run.
+struct Run {
+    plm_hip_ctx* c = nullptr;
+    hipStream_t st = nullptr;
+    int64_t n = 0;
+    int nb = 0;
+    int metric = 0, grad_mode = 0;
+    const plm_reg_parms* reg = nullptr;
+    plm_lbfgs_parms p {};
+    plm_lbfgs_progress_fn progress = nullptr;
+    void* inst = nullptr;
+
+    double* X[2] = {nullptr, nullptr};      // x and xp, by index xi / xpi
+    double* G[2] = {nullptr, nullptr};      // g and gp, by index gi / gpi
+    float* C[2] = {nullptr, nullptr};       // coefficients of the evaluations
+    double* d = nullptr;
+    double* S = nullptr;                    // history slot j at S + j n
+    double* Y = nullptr;
+    LbScalars* sc = nullptr;
+    double* part[2] = {nullptr, nullptr};   // partials between the direction kernels
+    double* hb = nullptr;   // device: score, reg score, num_vox, dots [3 nb], slope [nb]
+    double* hh = nullptr;   // its pinned host copy
+    int xi = 0, xpi = 1, gi = 0, gpi = 1;
+    int c_last = -1;        // C buffer of the last counted evaluation
+    int c_trial = 0;        // C buffer of the trial in flight
+    int feval = 0, it = 0;
+    int err = PLM_HIP_OK;   // first C-ABI error; the run stops on it
+    double f = 0., dg = 0., xx = 0., gg = 0.;   // the last trial's values
+
+    ~Run ()
+    {
+        void* bufs[] = {X[0], X[1], G[0], G[1], C[0], C[1], d, S, Y, sc, part[0], part[1], hb};
+        for (void* b : bufs) if (b) (void) hipFree (b);
+        if (hh) (void) hipHostFree (hh);
+    }
+
+    size_t hb_doubles () const { return 3 + 4 * (size_t) nb; }
+
+    int hip (hipError_t e, const char* what)
+    {
+        if (e != hipSuccess && err == PLM_HIP_OK) err = plmhip::abi_hip_fail (e, what);
+        return err;
+    }
+
+    template <typename T>
+    int alloc (T** ptr, size_t count, const char* what)
+    {
+        return hip (hipMalloc ((void**) ptr, sizeof (T) * (count ? count : 1)), what);
+    }
+
+    // x = xp + stp d (d null: x = xp), the objective and its gradient at
+    // x, the dot products; wait and read them back.  Not yet counted.
+    int trial (const double* xp, const double* dir, double stp)
+    {
+        c_trial = c_last == 0 ? 1 : 0;
+        float* cf = C[c_trial];
+        if (hip (plmhip::lb_launch_trial (xp, dir, stp, X[xi], cf, n, st), "L-BFGS trial")) return err;
+        int rc = plm_hip_bspline_score_device (c, metric, grad_mode, cf, hb, G[gi],
+            reinterpret_cast<int64_t*> (hb + 2), nullptr, st);
+        if (rc < 0) return err = rc;
+        if (reg) {
+            rc = plm_hip_bspline_regularize (c, reg, cf, hb + 1, G[gi], 1, st);
+            if (rc < 0) return err = rc;
+        }
+        if (hip (plmhip::lb_launch_dots (G[gi], dir ? dir : G[gi], X[xi], hb + 3, n, st), "L-BFGS dots"))
+            return err;
+        if (hip (hipMemcpyAsync (hh, hb, sizeof (double) * hb_doubles (), hipMemcpyDeviceToHost, st),
+                 "L-BFGS readback"))
+            return err;
+        if (hip (hipStreamSynchronize (st), "L-BFGS evaluation")) return err;
+        f = reg ? hh[0] + hh[1] : hh[0];
+        dg = lb_reduce_parts_host (hh + 3, nb);
+        xx = lb_reduce_parts_host (hh + 3 + nb, nb);
+        gg = lb_reduce_parts_host (hh + 3 + 2 * nb, nb);
+        return PLM_HIP_OK;
+    }
+
+    void count ()
+    {
+        c_last = c_trial;
+        ++feval;
+    }
+
+    // The progress report: the caller's callback, then the reference's
+    // max_its / max_feval rule.
+    int report (double fx, double xnorm, double gnorm, double step, int k, int ls)
+    {
+        it = k;
+        int ret = 0;
+        if (progress)
+            ret = progress (inst, X[xi], G[gi], fx, xnorm, gnorm, step, (int) n, k, ls);
+        if (!ret && ((p.max_its >= 0 && it > p.max_its) || (p.max_feval >= 0 && feval > p.max_feval)))
+            ret = 1;
+        return ret;
+    }
+
+    // The More-Thuente search along d from (xp, gp), the current point
+    // being already moved to the xp / gp buffers.  Returns the evaluations
+    // it took or a negative liblbfgs code; err set on a C-ABI failure.
+    // slope_known: dginit is given (the first iteration: -g.g); else it
+    // arrives with the first trial.
+    int search (double* fx, double* stp, bool slope_known, double dginit, int k)
+    {
+        if (*stp <= 0.) return PLM_LBFGSERR_INVALIDPARAMETERS;
+        if (slope_known && 0 < dginit) return PLM_LBFGSERR_INCREASEGRADIENT;
+        int count_ = 0, brackt = 0, stage1 = 1, uinfo = 0;
+        const double finit = *fx;
+        double dgtest = p.ftol * dginit;
+        double width = p.max_step - p.min_step;
+        double prev_width = 2.0 * width;
+        Pt bx = {0., finit, dginit}, by = {0., finit, dginit};
+        double stmin, stmax;
+        for (;;) {
+            if (brackt) {
+                stmin = lmin (bx.st, by.st);
+                stmax = lmax (bx.st, by.st);
+            } else {
+                stmin = bx.st;
+                stmax = *stp + 4.0 * (*stp - bx.st);
+            }
+            if (*stp < p.min_step) *stp = p.min_step;
+            if (p.max_step < *stp) *stp = p.max_step;
+            if ((brackt && ((*stp <= stmin || stmax <= *stp) || p.max_linesearch <= count_ + 1
+                     || uinfo != 0))
+                || (brackt && (stmax - stmin <= p.xtol * stmax)))
+                *stp = bx.st;
+
+            if (trial (X[xpi], d, *stp)) return PLM_LBFGSERR_INVALIDPARAMETERS;
+            if (!slope_known) {
+                slope_known = true;
+                dginit = lb_reduce_parts_host (hh + 3 + 3 * nb, nb);
+                if (0 < dginit) return PLM_LBFGSERR_INCREASEGRADIENT;    // trial dropped
+                dgtest = p.ftol * dginit;
+                bx.g = by.g = dginit;
+            }
+            count ();
+            *fx = f;
+            if (report (bx.f, std::sqrt (xx), std::sqrt (gg), *stp, k, 0))
+                return PLM_LBFGSERR_MAXIMUMLINESEARCH;
+            const double ftest1 = finit + *stp * dgtest;
+            ++count_;
+
+            if (brackt && ((*stp <= stmin || stmax <= *stp) || uinfo != 0))
+                return PLM_LBFGSERR_ROUNDING_ERROR;
+            if (*stp == p.max_step && f <= ftest1 && dg <= dgtest) return PLM_LBFGSERR_MAXIMUMSTEP;
+            if (*stp == p.min_step && (ftest1 < f || dgtest <= dg)) return PLM_LBFGSERR_MINIMUMSTEP;
+            if (brackt && (stmax - stmin) <= p.xtol * stmax) return PLM_LBFGSERR_WIDTHTOOSMALL;
+            if (p.max_linesearch <= count_) return PLM_LBFGSERR_MAXIMUMLINESEARCH;
+            if (f <= ftest1 && std::fabs (dg) <= p.gtol * (-dginit)) return count_;
+
+            if (stage1 && f <= ftest1 && lmin (p.ftol, p.gtol) * dginit <= dg) stage1 = 0;
+
+            Pt t = {*stp, f, dg};
+            if (stage1 && ftest1 < f && f <= bx.f) {
+                Pt mx = {bx.st, bx.f - bx.st * dgtest, bx.g - dgtest};
+                Pt my = {by.st, by.f - by.st * dgtest, by.g - dgtest};
+                Pt mt = {t.st, t.f - t.st * dgtest, t.g - dgtest};
+                uinfo = next_trial (mx, my, mt, stmin, stmax, brackt);
+                bx = {mx.st, mx.f + mx.st * dgtest, mx.g + dgtest};
+                by = {my.st, my.f + my.st * dgtest, my.g + dgtest};
+                *stp = mt.st;
+            } else {
+                uinfo = next_trial (bx, by, t, stmin, stmax, brackt);
+                *stp = t.st;
+            }
+            if (brackt) {
+                if (0.66 * prev_width <= std::fabs (by.st - bx.st)) *stp = bx.st + 0.5 * (by.st - bx.st);
+                prev_width = width;
+                width = std::fabs (by.st - bx.st);
+            }
+        }
+    }
+
+    // d = -H g from the newest pair (slot jn, just formed) and the bound-1
+    // before it; the last kernel leaves the partials of g.d (the next
+    // search's initial slope) in hb.
+    int direction (int jn, int end, int bound)
+    {
+        const int m = p.m;
+        double* sj_new = S + (size_t) jn * n;
+        double* yj_new = Y + (size_t) jn * n;
+        if (hip (plmhip::lb_launch_pair (X[xi], X[xpi], G[gi], G[gpi], sj_new, yj_new, d, part[0], n, st),
+                 "L-BFGS pair"))
+            return err;
+        int in = 0;
+        int j = end;
+        for (int i = 0; i < bound; ++i) {
+            j = (j + m - 1) % m;
+            const bool last = i == bound - 1;
+            const double* next = last ? Y + (size_t) j * n : S + (size_t) ((j + m - 1) % m) * n;
+            if (hip (plmhip::lb_launch_loop1 (part[in], i == 0, last, j, jn, Y + (size_t) j * n, next, d,
+                         sc, part[1 - in], n, st),
+                     "L-BFGS first loop"))
+                return err;
+            in = 1 - in;
+        }
+        for (int i = 0; i < bound; ++i) {
+            const bool last = i == bound - 1;
+            const double* next = last ? G[gi] : Y + (size_t) ((j + 1) % m) * n;
+            double* out = last ? hb + 3 + 3 * (size_t) nb : part[1 - in];
+            if (hip (plmhip::lb_launch_loop2 (part[in], j, S + (size_t) j * n, next, d, sc, out, n, st),
+                     "L-BFGS second loop"))
+                return err;
+            in = 1 - in;
+            j = (j + 1) % m;
+        }
+        return PLM_HIP_OK;
+    }
+};
+
+int
+check_parms (const plm_lbfgs_parms& p)
+{
+    if (p.m < 1 || p.m > plmhip::LB_MAX_M)
+        return plmhip::abi_fail (PLM_HIP_ERR_ARG, "L-BFGS m must be 1..32");
+    if (p.past < 0 || p.past > MAX_PAST)
+        return plmhip::abi_fail (PLM_HIP_ERR_ARG, "L-BFGS past must be 0..64");
+    if (!(p.epsilon >= 0.) || !(p.delta >= 0.) || !(p.min_step >= 0.) || !(p.max_step >= p.min_step)
+        || !(p.ftol >= 0.) || !(p.gtol >= 0.) || !(p.xtol >= 0.) || p.max_linesearch <= 0)
+        return plmhip::abi_fail (PLM_HIP_ERR_ARG, "invalid L-BFGS parameter (as lbfgs() rejects it)");
+    return PLM_HIP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+void
+plm_hip_lbfgs_parms_init (plm_lbfgs_parms* p)
+{
+    if (!p) return;
+    p->m = 6;
+    p->epsilon = 1e-5;
+    p->past = 0;
+    p->delta = 1e-5;
+    p->max_iterations = 0;
+    p->max_linesearch = 40;
+    p->min_step = 1e-20;
+    p->max_step = 1e20;
+    p->ftol = 1e-4;
+    p->gtol = 0.9;
+    p->xtol = 1.0e-16;
+    p->max_its = 10;
+    p->max_feval = 10;
+}
+
+int
+plm_hip_bspline_optimize_lbfgs (plm_hip_ctx* c, int metric, int grad_mode,
+    const plm_reg_parms* reg, const plm_lbfgs_parms* parms, double* d_x, float* d_coeff,
+    plm_lbfgs_progress_fn progress, void* instance, plm_lbfgs_result* result)
+{
+    if (!c || !d_x || !result) return plmhip::abi_fail (PLM_HIP_ERR_ARG, "NULL argument");
+    if (metric == PLM_METRIC_MSE && grad_mode == PLM_GRAD_NONE)
+        return plmhip::abi_fail (PLM_HIP_ERR_ARG, "the optimizer needs a gradient");
+    Run r;
+    int device = 0;
+    int64_t knots = 0;
+    int rc = plmhip::ctx_info (c, &device, &r.st, &knots);
+    if (rc) return rc;
+    if (parms) r.p = *parms;
+    else plm_hip_lbfgs_parms_init (&r.p);
+    rc = check_parms (r.p);
+    if (rc) return rc;
+    if (r.hip (hipSetDevice (device), "hipSetDevice")) return r.err;
+    r.c = c;
+    r.n = 3 * knots;
+    r.nb = plmhip::lb_blocks (r.n);
+    r.metric = metric;
+    r.grad_mode = grad_mode;
+    r.reg = reg;
+    r.progress = progress;
+    r.inst = instance;
+    const int m = r.p.m;
+    const size_t n = (size_t) r.n;
+    if (r.alloc (&r.X[0], n, "hipMalloc(L-BFGS x)") || r.alloc (&r.X[1], n, "hipMalloc(L-BFGS x)")
+        || r.alloc (&r.G[0], n, "hipMalloc(L-BFGS g)") || r.alloc (&r.G[1], n, "hipMalloc(L-BFGS g)")
+        || r.alloc (&r.C[0], n, "hipMalloc(L-BFGS coeff)") || r.alloc (&r.C[1], n, "hipMalloc(L-BFGS coeff)")
+        || r.alloc (&r.d, n, "hipMalloc(L-BFGS d)") || r.alloc (&r.S, n * m, "hipMalloc(L-BFGS s)")
+        || r.alloc (&r.Y, n * m, "hipMalloc(L-BFGS y)") || r.alloc (&r.sc, 1, "hipMalloc(L-BFGS scalars)")
+        || r.alloc (&r.part[0], 3 * (size_t) r.nb, "hipMalloc(L-BFGS partials)")
+        || r.alloc (&r.part[1], 3 * (size_t) r.nb, "hipMalloc(L-BFGS partials)")
+        || r.alloc (&r.hb, r.hb_doubles (), "hipMalloc(L-BFGS readback)"))
+        return r.err;
+    if (r.hip (hipHostMalloc ((void**) &r.hh, sizeof (double) * r.hb_doubles (), hipHostMallocDefault),
+            "hipHostMalloc(L-BFGS readback)"))
+        return r.err;
+
+    std::vector<double> pf ((size_t) (r.p.past > 0 ? r.p.past : 1));
+    int ret = 0;
+    double fx = 0.;
+    double step = 0.;
+    int k = 0, end = 0;
+
+    // the start point, evaluated
+    if (r.trial (d_x, nullptr, 0.)) return r.err;
+    r.count ();
+    fx = r.f;
+    pf[0] = fx;
+    if (r.hip (plmhip::lb_launch_neg (r.G[r.gi], r.d, r.n, r.st), "L-BFGS steepest")) return r.err;
+    {
+        double xnorm = std::sqrt (r.xx);
+        const double gnorm = std::sqrt (r.gg);
+        if (xnorm < 1.0) xnorm = 1.0;
+        if (gnorm / xnorm <= r.p.epsilon) {
+            ret = PLM_LBFGS_ALREADY_MINIMIZED;
+            goto done;
+        }
+    }
+    step = 1.0 / std::sqrt (r.gg);      // 1 / |d| with d = -g
+    k = 1;
+    {
+        // d = -g: the first search's slope is -g.g exactly
+        bool slope_known = true;
+        double dginit = -r.gg;
+        for (;;) {
+            std::swap (r.xi, r.xpi);
+            std::swap (r.gi, r.gpi);
+            const int ls = r.search (&fx, &step, slope_known, dginit, k);
+            if (r.err) return r.err;
+            if (ls < 0) {
+                std::swap (r.xi, r.xpi);
+                std::swap (r.gi, r.gpi);
+                ret = ls;
+                goto done;
+            }
+            double xnorm = std::sqrt (r.xx);
+            const double gnorm = std::sqrt (r.gg);
+            if ((ret = r.report (fx, xnorm, gnorm, step, k, ls))) goto done;
+            if (xnorm < 1.0) xnorm = 1.0;
+            if (gnorm / xnorm <= r.p.epsilon) {
+                ret = PLM_LBFGS_SUCCESS;
+                break;
+            }
+            if (r.p.past > 0) {
+                if (r.p.past <= k) {
+                    const double rate = (pf[(size_t) (k % r.p.past)] - fx) / fx;
+                    if (rate < r.p.delta) {
+                        ret = PLM_LBFGS_STOP;
+                        break;
+                    }
+                }
+                pf[(size_t) (k % r.p.past)] = fx;
+            }
+            if (r.p.max_iterations != 0 && r.p.max_iterations < k + 1) {
+                ret = PLM_LBFGSERR_MAXIMUMITERATION;
+                break;
+            }
+            const int jn = end;
+            const int bound = m <= k ? m : k;
+            ++k;
+            end = (end + 1) % m;
+            if (r.direction (jn, end, bound)) return r.err;
+            slope_known = false;
+            dginit = 0.;
+            step = 1.0;
+        }
+    }
+
+done:
+    if (r.hip (hipMemcpyAsync (d_x, r.X[r.xi], sizeof (double) * n, hipMemcpyDeviceToDevice, r.st),
+            "L-BFGS x out"))
+        return r.err;
+    if (d_coeff && r.c_last >= 0
+        && r.hip (hipMemcpyAsync (d_coeff, r.C[r.c_last], sizeof (float) * n, hipMemcpyDeviceToDevice, r.st),
+               "L-BFGS coefficients out"))
+        return r.err;
+    if (r.hip (hipStreamSynchronize (r.st), "L-BFGS end")) return r.err;
+    result->status = ret;
+    result->iterations = r.it;
+    result->fevals = r.feval;
+    result->fx = fx;
+    return PLM_HIP_OK;
+}
+
+} // extern "C"
