@@ -313,7 +313,9 @@ int plm_hip_batch_destroy (plm_hip_batch* b);
  * orthantwise_c = 0, every line-search evaluation reported to the progress
  * rule.  Coefficients, gradients, the search direction and the m history
  * pairs stay in device memory (double); per evaluation the host reads back
- * only the score and a few dot products. */
+ * only the score and a few dot products.  The direction is the two-loop
+ * recursion's d = -H g evaluated in the compact form of the same matrix, so
+ * it agrees with liblbfgs's to rounding, not bit for bit. */
 
 /* Return codes of the optimizer (plm_lbfgs_result::status): liblbfgs's
  * values (libs/liblbfgs-1.9/include/lbfgs.h:72-145). */
@@ -339,7 +341,7 @@ int plm_hip_batch_destroy (plm_hip_batch* b);
  * (progress, src/register/bspline_optimize_liblbfgs.cxx:55-80); a negative
  * limit is no limit. */
 typedef struct plm_lbfgs_parms {
-    int m;                 /* history pairs, 1..32 */
+    int m;                 /* history pairs, 1..16 */
     double epsilon;
     int past;              /* 0, or the period of the delta test, <= 64 */
     double delta;

@@ -281,7 +281,7 @@ class BsplineHip:
             float(default_val), 1 if linear else 0, st))
 
     def optimize_lbfgs(self, d_x, d_coeff=None, metric="mse", grad_mode="ref_nn", reg=None, progress=None,
-                       **parms):
+                       record=True, **parms):
         """L-BFGS on the device (plm_hip_bspline_optimize_lbfgs): minimize the
         score (+ the analytic regularization with weights ``reg``, a dict of
         Regularization_parms names) from the float64 device tensor d_x, which
@@ -290,7 +290,9 @@ class BsplineHip:
         plm_lbfgs_parms fields (liblbfgs names, max_its, max_feval; unset ones
         take liblbfgs's defaults and max_its = max_feval = 10).  ``progress``:
         callable (d_x_ptr, d_g_ptr, fx, xnorm, gnorm, step, n, k, ls) -> int,
-        nonzero stops.  Returns an LbfgsResult."""
+        nonzero stops.  ``record``: keep every progress report in the result
+        (with neither, no callback crosses into Python).  Returns an
+        LbfgsResult."""
         if d_x.numel() != self.bxf.num_coeff:
             raise ValueError("d_x has the wrong length")
         p = _abi.PlmLbfgsParms()
@@ -313,7 +315,7 @@ class BsplineHip:
             reports.append((fx, xnorm, gnorm, step, k, ls))
             return int(progress(px, pg, fx, xnorm, gnorm, step, n, k, ls)) if progress else 0
 
-        cb = _abi.PLM_LBFGS_PROGRESS(_progress)
+        cb = _abi.PLM_LBFGS_PROGRESS(_progress) if (progress or record) else _abi.PLM_LBFGS_PROGRESS()
         res = _abi.PlmLbfgsResult()
         _abi.check(self.lib.plm_hip_bspline_optimize_lbfgs(
             self.ctx, METRICS[metric], GRAD_MODES[grad_mode], ctypes.byref(rp) if rp is not None else None,

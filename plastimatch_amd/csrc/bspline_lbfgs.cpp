@@ -6,15 +6,16 @@
 // recursion over the last m (s, y) pairs with H0 = (ys/yy) I, and the
 // More-Thuente line search with every evaluation reported to the progress
 // rule.  Its vectors live in device memory (lbfgs_kernels.hip); this file
-// holds the scalar logic, which needs per evaluation only the objective
-// and three dot products (g.d, x.x, g.g), read back in one copy.
+// holds the scalar logic.  Per evaluation the host reads back only the
+// objective and three dot products (g.d, x.x, g.g), which the post pass
+// writes straight into pinned host memory.
 //
-// The first trial of each line search is enqueued right behind the
-// direction kernels, before the host knows the initial slope g.d: the
-// slope's partials come back with that trial's results.  Should the
-// slope be positive (liblbfgs returns LBFGSERR_INCREASEGRADIENT before
-// evaluating anything), the trial is dropped uncounted -- its coefficients
-// went to the spare buffer -- so the run's record matches liblbfgs's.
+// The first trial of each line search is formed by the pass that forms
+// the direction, before the host knows the initial slope g.d: the slope's
+// partials come back with that trial's results.  Should the slope be
+// positive (liblbfgs returns LBFGSERR_INCREASEGRADIENT before evaluating
+// anything), the trial is dropped uncounted -- its coefficients went to the
+// spare buffer -- so the run's record matches liblbfgs's.
 
 #include <hip/hip_runtime.h>
 #include <cmath>
@@ -26,7 +27,6 @@
 #include "bspline_internal.h"
 #include "lbfgs_kernels.h"
 
-using plmhip::LbScalars;
 using plmhip::lb_reduce_parts_host;
 
 namespace {
@@ -176,25 +176,31 @@ struct Run {
     double* d = nullptr;
     double* S = nullptr;                    // history slot j at S + j n
     double* Y = nullptr;
-    LbScalars* sc = nullptr;
-    double* part[2] = {nullptr, nullptr};   // partials between the direction kernels
-    double* hb = nullptr;   // device: score, reg score, num_vox, dots [3 nb], slope [nb]
-    double* hh = nullptr;   // its pinned host copy
+    plmhip::LbGram* gram = nullptr;
+    double* part = nullptr;                 // k_lb_post's pair partials
+    double* score = nullptr;                // device: score, reg score, num_vox
+    double* hh = nullptr;                   // pinned, written by the kernels:
+                                            // score, reg score, num_vox, dots [3 nb], slope [nb]
+    double* hh_dev = nullptr;               // its device address
     int xi = 0, xpi = 1, gi = 0, gpi = 1;
     int c_last = -1;        // C buffer of the last counted evaluation
     int c_trial = 0;        // C buffer of the trial in flight
     int feval = 0, it = 0;
     int err = PLM_HIP_OK;   // first C-ABI error; the run stops on it
     double f = 0., dg = 0., xx = 0., gg = 0.;   // the last trial's values
+    // what the trials of the current search do: the first one forms the
+    // direction from dir_b pairs (newest in slot dir_e); each one writes
+    // its would-be pair into slot post_e with post_bp earlier pairs
+    int dir_b = 0, dir_e = 0, post_e = -1, post_bp = 0;
 
     ~Run ()
     {
-        void* bufs[] = {X[0], X[1], G[0], G[1], C[0], C[1], d, S, Y, sc, part[0], part[1], hb};
+        void* bufs[] = {X[0], X[1], G[0], G[1], C[0], C[1], d, S, Y, gram, part, score};
         for (void* b : bufs) if (b) (void) hipFree (b);
         if (hh) (void) hipHostFree (hh);
     }
 
-    size_t hb_doubles () const { return 3 + 4 * (size_t) nb; }
+    size_t hh_doubles () const { return 3 + 4 * (size_t) nb; }
 
     int hip (hipError_t e, const char* what)
     {
@@ -208,25 +214,35 @@ struct Run {
         return hip (hipMalloc ((void**) ptr, sizeof (T) * (count ? count : 1)), what);
     }
 
-    // x = xp + stp d (d null: x = xp), the objective and its gradient at
-    // x, the dot products; wait and read them back.  Not yet counted.
-    int trial (const double* xp, const double* dir, double stp)
+    // One evaluation: its point (kind 0: the start point d_x0; 1: the
+    // search's first trial, formed with the direction; 2: a later trial
+    // xp + stp d), the objective and gradient there, the post pass; wait
+    // and read back.  Not yet counted.
+    int trial (int kind, double stp, const double* x0 = nullptr)
     {
         c_trial = c_last == 0 ? 1 : 0;
         float* cf = C[c_trial];
-        if (hip (plmhip::lb_launch_trial (xp, dir, stp, X[xi], cf, n, st), "L-BFGS trial")) return err;
-        int rc = plm_hip_bspline_score_device (c, metric, grad_mode, cf, hb, G[gi],
-            reinterpret_cast<int64_t*> (hb + 2), nullptr, st);
+        hipError_t e;
+        if (kind == 0) {
+            e = plmhip::lb_launch_trial (x0, nullptr, 0., X[xi], cf, n, st);
+        } else if (kind == 1) {
+            plmhip::LbCombineArgs a {part, dir_b, dir_e, p.m, S, Y, gram, G[gpi], X[xpi], stp, d, X[xi], cf,
+                hh_dev + 3 + 3 * (size_t) nb};
+            e = plmhip::lb_launch_combine (a, n, st);
+        } else {
+            e = plmhip::lb_launch_trial (X[xpi], d, stp, X[xi], cf, n, st);
+        }
+        if (hip (e, "L-BFGS trial point")) return err;
+        int rc = plm_hip_bspline_score_device (c, metric, grad_mode, cf, score, G[gi],
+            reinterpret_cast<int64_t*> (score + 2), nullptr, st);
         if (rc < 0) return err = rc;
         if (reg) {
-            rc = plm_hip_bspline_regularize (c, reg, cf, hb + 1, G[gi], 1, st);
+            rc = plm_hip_bspline_regularize (c, reg, cf, score + 1, G[gi], 1, st);
             if (rc < 0) return err = rc;
         }
-        if (hip (plmhip::lb_launch_dots (G[gi], dir ? dir : G[gi], X[xi], hb + 3, n, st), "L-BFGS dots"))
-            return err;
-        if (hip (hipMemcpyAsync (hh, hb, sizeof (double) * hb_doubles (), hipMemcpyDeviceToHost, st),
-                 "L-BFGS readback"))
-            return err;
+        plmhip::LbPostArgs pa {X[xi], X[xpi], G[gi], G[gpi], kind == 0 ? nullptr : d, S, Y,
+            kind == 0 ? -1 : post_e, post_bp, p.m, score, hh_dev, part};
+        if (hip (plmhip::lb_launch_post (pa, n, st), "L-BFGS post pass")) return err;
         if (hip (hipStreamSynchronize (st), "L-BFGS evaluation")) return err;
         f = reg ? hh[0] + hh[1] : hh[0];
         dg = lb_reduce_parts_host (hh + 3, nb);
@@ -255,20 +271,19 @@ struct Run {
     }
 
     // The More-Thuente search along d from (xp, gp), the current point
-    // being already moved to the xp / gp buffers.  Returns the evaluations
-    // it took or a negative liblbfgs code; err set on a C-ABI failure.
-    // slope_known: dginit is given (the first iteration: -g.g); else it
-    // arrives with the first trial.
-    int search (double* fx, double* stp, bool slope_known, double dginit, int k)
+    // being already moved to the xp / gp buffers; d itself is formed by the
+    // first trial.  Returns the evaluations it took or a negative liblbfgs
+    // code; err set on a C-ABI failure.  The initial slope g.d arrives with
+    // the first trial's results.
+    int search (double* fx, double* stp, int k)
     {
         if (*stp <= 0.) return PLM_LBFGSERR_INVALIDPARAMETERS;
-        if (slope_known && 0 < dginit) return PLM_LBFGSERR_INCREASEGRADIENT;
         int count_ = 0, brackt = 0, stage1 = 1, uinfo = 0;
         const double finit = *fx;
-        double dgtest = p.ftol * dginit;
+        double dginit = 0., dgtest = 0.;
         double width = p.max_step - p.min_step;
         double prev_width = 2.0 * width;
-        Pt bx = {0., finit, dginit}, by = {0., finit, dginit};
+        Pt bx = {0., finit, 0.}, by = {0., finit, 0.};
         double stmin, stmax;
         for (;;) {
             if (brackt) {
@@ -285,9 +300,8 @@ struct Run {
                 || (brackt && (stmax - stmin <= p.xtol * stmax)))
                 *stp = bx.st;
 
-            if (trial (X[xpi], d, *stp)) return PLM_LBFGSERR_INVALIDPARAMETERS;
-            if (!slope_known) {
-                slope_known = true;
+            if (trial (count_ == 0 ? 1 : 2, *stp)) return PLM_LBFGSERR_INVALIDPARAMETERS;
+            if (count_ == 0) {
                 dginit = lb_reduce_parts_host (hh + 3 + 3 * nb, nb);
                 if (0 < dginit) return PLM_LBFGSERR_INCREASEGRADIENT;    // trial dropped
                 dgtest = p.ftol * dginit;
@@ -330,49 +344,13 @@ struct Run {
             }
         }
     }
-
-    // d = -H g from the newest pair (slot jn, just formed) and the bound-1
-    // before it; the last kernel leaves the partials of g.d (the next
-    // search's initial slope) in hb.
-    int direction (int jn, int end, int bound)
-    {
-        const int m = p.m;
-        double* sj_new = S + (size_t) jn * n;
-        double* yj_new = Y + (size_t) jn * n;
-        if (hip (plmhip::lb_launch_pair (X[xi], X[xpi], G[gi], G[gpi], sj_new, yj_new, d, part[0], n, st),
-                 "L-BFGS pair"))
-            return err;
-        int in = 0;
-        int j = end;
-        for (int i = 0; i < bound; ++i) {
-            j = (j + m - 1) % m;
-            const bool last = i == bound - 1;
-            const double* next = last ? Y + (size_t) j * n : S + (size_t) ((j + m - 1) % m) * n;
-            if (hip (plmhip::lb_launch_loop1 (part[in], i == 0, last, j, jn, Y + (size_t) j * n, next, d,
-                         sc, part[1 - in], n, st),
-                     "L-BFGS first loop"))
-                return err;
-            in = 1 - in;
-        }
-        for (int i = 0; i < bound; ++i) {
-            const bool last = i == bound - 1;
-            const double* next = last ? G[gi] : Y + (size_t) ((j + 1) % m) * n;
-            double* out = last ? hb + 3 + 3 * (size_t) nb : part[1 - in];
-            if (hip (plmhip::lb_launch_loop2 (part[in], j, S + (size_t) j * n, next, d, sc, out, n, st),
-                     "L-BFGS second loop"))
-                return err;
-            in = 1 - in;
-            j = (j + 1) % m;
-        }
-        return PLM_HIP_OK;
-    }
 };
 
 int
 check_parms (const plm_lbfgs_parms& p)
 {
     if (p.m < 1 || p.m > plmhip::LB_MAX_M)
-        return plmhip::abi_fail (PLM_HIP_ERR_ARG, "L-BFGS m must be 1..32");
+        return plmhip::abi_fail (PLM_HIP_ERR_ARG, "L-BFGS m must be 1..16");
     if (p.past < 0 || p.past > MAX_PAST)
         return plmhip::abi_fail (PLM_HIP_ERR_ARG, "L-BFGS past must be 0..64");
     if (!(p.epsilon >= 0.) || !(p.delta >= 0.) || !(p.min_step >= 0.) || !(p.max_step >= p.min_step)
@@ -436,14 +414,17 @@ plm_hip_bspline_optimize_lbfgs (plm_hip_ctx* c, int metric, int grad_mode,
         || r.alloc (&r.G[0], n, "hipMalloc(L-BFGS g)") || r.alloc (&r.G[1], n, "hipMalloc(L-BFGS g)")
         || r.alloc (&r.C[0], n, "hipMalloc(L-BFGS coeff)") || r.alloc (&r.C[1], n, "hipMalloc(L-BFGS coeff)")
         || r.alloc (&r.d, n, "hipMalloc(L-BFGS d)") || r.alloc (&r.S, n * m, "hipMalloc(L-BFGS s)")
-        || r.alloc (&r.Y, n * m, "hipMalloc(L-BFGS y)") || r.alloc (&r.sc, 1, "hipMalloc(L-BFGS scalars)")
-        || r.alloc (&r.part[0], 3 * (size_t) r.nb, "hipMalloc(L-BFGS partials)")
-        || r.alloc (&r.part[1], 3 * (size_t) r.nb, "hipMalloc(L-BFGS partials)")
-        || r.alloc (&r.hb, r.hb_doubles (), "hipMalloc(L-BFGS readback)"))
+        || r.alloc (&r.Y, n * m, "hipMalloc(L-BFGS y)") || r.alloc (&r.gram, 1, "hipMalloc(L-BFGS Gram)")
+        || r.alloc (&r.part, (size_t) plmhip::lb_post_components (m - 1) * r.nb, "hipMalloc(L-BFGS partials)")
+        || r.alloc (&r.score, 3, "hipMalloc(L-BFGS score)"))
         return r.err;
-    if (r.hip (hipHostMalloc ((void**) &r.hh, sizeof (double) * r.hb_doubles (), hipHostMallocDefault),
-            "hipHostMalloc(L-BFGS readback)"))
+    if (r.hip (hipHostMalloc ((void**) &r.hh, sizeof (double) * r.hh_doubles (), hipHostMallocMapped),
+            "hipHostMalloc(L-BFGS readback)")
+        || r.hip (hipHostGetDevicePointer ((void**) &r.hh_dev, r.hh, 0), "hipHostGetDevicePointer"))
         return r.err;
+    // the regularizer adds into the gradient and leaves its score; without
+    // it the reg score slot stays zero
+    if (r.hip (hipMemsetAsync (r.score, 0, sizeof (double) * 3, r.st), "L-BFGS score clear")) return r.err;
 
     std::vector<double> pf ((size_t) (r.p.past > 0 ? r.p.past : 1));
     int ret = 0;
@@ -452,11 +433,10 @@ plm_hip_bspline_optimize_lbfgs (plm_hip_ctx* c, int metric, int grad_mode,
     int k = 0, end = 0;
 
     // the start point, evaluated
-    if (r.trial (d_x, nullptr, 0.)) return r.err;
+    if (r.trial (0, 0., d_x)) return r.err;
     r.count ();
     fx = r.f;
     pf[0] = fx;
-    if (r.hip (plmhip::lb_launch_neg (r.G[r.gi], r.d, r.n, r.st), "L-BFGS steepest")) return r.err;
     {
         double xnorm = std::sqrt (r.xx);
         const double gnorm = std::sqrt (r.gg);
@@ -468,52 +448,49 @@ plm_hip_bspline_optimize_lbfgs (plm_hip_ctx* c, int metric, int grad_mode,
     }
     step = 1.0 / std::sqrt (r.gg);      // 1 / |d| with d = -g
     k = 1;
-    {
-        // d = -g: the first search's slope is -g.g exactly
-        bool slope_known = true;
-        double dginit = -r.gg;
-        for (;;) {
+    r.dir_b = 0;                        // d = -g
+    for (;;) {
+        std::swap (r.xi, r.xpi);
+        std::swap (r.gi, r.gpi);
+        r.post_e = end;
+        r.post_bp = (m <= k ? m : k) - 1;
+        const int ls = r.search (&fx, &step, k);
+        if (r.err) return r.err;
+        if (ls < 0) {
             std::swap (r.xi, r.xpi);
             std::swap (r.gi, r.gpi);
-            const int ls = r.search (&fx, &step, slope_known, dginit, k);
-            if (r.err) return r.err;
-            if (ls < 0) {
-                std::swap (r.xi, r.xpi);
-                std::swap (r.gi, r.gpi);
-                ret = ls;
-                goto done;
-            }
-            double xnorm = std::sqrt (r.xx);
-            const double gnorm = std::sqrt (r.gg);
-            if ((ret = r.report (fx, xnorm, gnorm, step, k, ls))) goto done;
-            if (xnorm < 1.0) xnorm = 1.0;
-            if (gnorm / xnorm <= r.p.epsilon) {
-                ret = PLM_LBFGS_SUCCESS;
-                break;
-            }
-            if (r.p.past > 0) {
-                if (r.p.past <= k) {
-                    const double rate = (pf[(size_t) (k % r.p.past)] - fx) / fx;
-                    if (rate < r.p.delta) {
-                        ret = PLM_LBFGS_STOP;
-                        break;
-                    }
-                }
-                pf[(size_t) (k % r.p.past)] = fx;
-            }
-            if (r.p.max_iterations != 0 && r.p.max_iterations < k + 1) {
-                ret = PLM_LBFGSERR_MAXIMUMITERATION;
-                break;
-            }
-            const int jn = end;
-            const int bound = m <= k ? m : k;
-            ++k;
-            end = (end + 1) % m;
-            if (r.direction (jn, end, bound)) return r.err;
-            slope_known = false;
-            dginit = 0.;
-            step = 1.0;
+            ret = ls;
+            goto done;
         }
+        double xnorm = std::sqrt (r.xx);
+        const double gnorm = std::sqrt (r.gg);
+        if ((ret = r.report (fx, xnorm, gnorm, step, k, ls))) goto done;
+        if (xnorm < 1.0) xnorm = 1.0;
+        if (gnorm / xnorm <= r.p.epsilon) {
+            ret = PLM_LBFGS_SUCCESS;
+            break;
+        }
+        if (r.p.past > 0) {
+            if (r.p.past <= k) {
+                const double rate = (pf[(size_t) (k % r.p.past)] - fx) / fx;
+                if (rate < r.p.delta) {
+                    ret = PLM_LBFGS_STOP;
+                    break;
+                }
+            }
+            pf[(size_t) (k % r.p.past)] = fx;
+        }
+        if (r.p.max_iterations != 0 && r.p.max_iterations < k + 1) {
+            ret = PLM_LBFGSERR_MAXIMUMITERATION;
+            break;
+        }
+        // the accepted trial's pair (slot end) is the newest of the next
+        // direction, with the bound - 1 before it
+        r.dir_b = m <= k ? m : k;
+        r.dir_e = end;
+        ++k;
+        end = (end + 1) % m;
+        step = 1.0;
     }
 
 done:
