@@ -305,6 +305,47 @@ int plm_hip_batch_get_timing (plm_hip_batch* b, double* stage_ms, int max_stages
 int plm_hip_batch_destroy (plm_hip_batch* b);
 
 /* ------------------------------------------------------------------------
+ * Stage preprocessing on the device. */
+
+/* Resample kinds of a stage (Stage_parms::resample_type with
+ * Shared_parms::legacy_subsampling, src/register/registration_resample.cxx:
+ * 29-50). */
+#define PLM_RESAMPLE_VOXEL_RATE    0   /* RESAMPLE_AUTO / _VOXEL_RATE: volume_subsample_vox */
+#define PLM_RESAMPLE_VOXEL_LEGACY  1   /* the same with legacy_subsampling: volume_subsample_vox_legacy */
+#define PLM_RESAMPLE_MM            2   /* RESAMPLE_MM: volume_resample_spacing (rate = spacing in mm) */
+#define PLM_RESAMPLE_PCT           3   /* RESAMPLE_PCT: volume_resample_percent (rate = fraction) */
+
+/* The output geometry registration_resample_volume gives a stage
+ * (src/register/registration_resample.cxx:13-58; the geometry functions of
+ * src/base/volume_resample.cxx:298-413).  *identity = 1 when the reference
+ * clones the volume instead (voxel rates of 1, 1, 1). */
+int plm_hip_resample_geometry (int kind, const plm_img_desc* in, const float rate[3],
+    int64_t dim[3], float origin[3], float spacing[3], int* identity);
+
+/* Replaces volume_resample / volume_resample_nn for float volumes
+ * (volume_resample_float_li / _nn, src/base/volume_resample.cxx:26-135,
+ * 227-295): in resampled onto dim / origin / spacing (same direction
+ * cosines), trilinear (nearest = 0) or nearest voxel (1), 0 outside; the
+ * reference's result bit for bit.  in may be on the host or the device; out
+ * holds dim[0]*dim[1]*dim[2] floats at out_location. */
+int plm_hip_volume_resample (const plm_img_desc* in, const int64_t dim[3],
+    const float origin[3], const float spacing[3], int nearest, float* out, int out_location,
+    int gpuid);
+
+/* Replaces registration_resample_volume
+ * (src/register/registration_resample.cxx:13-58): the geometry above, then
+ * the clone or the resample.  Size out with plm_hip_resample_geometry. */
+int plm_hip_stage_resample (int kind, const float rate[3], const plm_img_desc* in, int nearest,
+    float* out, int out_location, int gpuid);
+
+/* Replaces bspline_initialize_mi_hist_eqsp
+ * (src/register/joint_histogram.cxx:115-139): the image's minimum and
+ * maximum by a device scan (exact, order independent), then
+ * delta = (max - min) / (bins - 1), offset = min - delta / 2. */
+int plm_hip_hist_eqsp (const plm_img_desc* vol, int64_t bins, float* offset, float* delta,
+    int gpuid);
+
+/* ------------------------------------------------------------------------
  * The optimizer loop on the device: L-BFGS with the More-Thuente line
  * search, the algorithm of the liblbfgs 1.9 the reference vendors
  * (libs/liblbfgs-1.9/lib/lbfgs.c:256-653, 823-1330) as

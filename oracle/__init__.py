@@ -133,6 +133,9 @@ def lib():
         L.orc_mi_score_f64.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
                                        ctypes.POINTER(OrcVolHdr), P, P, ctypes.POINTER(OrcHist),
                                        ctypes.c_int, P, P, P]
+        L.orc_resample_geometry.restype = ctypes.c_int
+        L.orc_resample_geometry.argtypes = [ctypes.c_int, ctypes.POINTER(OrcVolHdr), P, P, P, P]
+        L.orc_volume_resample.argtypes = [ctypes.POINTER(OrcVolHdr), P, P, P, P, ctypes.c_int, P]
         L.orc_lbfgs_parms_init.argtypes = [ctypes.POINTER(OrcLbfgsParms)]
         L.orc_lbfgs.restype = ctypes.c_int
         L.orc_lbfgs.argtypes = [ctypes.c_int, P, P, LBFGS_EVAL, LBFGS_PROGRESS, P,
@@ -200,6 +203,44 @@ def warp(g, moving, m_spacing, coeff, linear=True, default_val=0.0, m_origin=(0.
     if rc:
         raise ValueError(f"orc_warp failed ({rc})")
     return out
+
+
+RESAMPLE_KINDS = {"voxel_rate": 0, "voxel_legacy": 1, "mm": 2, "pct": 3}
+
+
+def resample_geometry(kind, shape_zyx, spacing, origin, rate):
+    """registration_resample_volume's output geometry: (identity, dim xyz,
+    origin, spacing); identity means the reference clones the volume."""
+    nz, ny, nx = shape_zyx
+    h = OrcVolHdr(c_int64_3(nx, ny, nz), c_float3(*origin), c_float3(*spacing), c_float9(*IDENTITY))
+    r = np.ascontiguousarray(rate, np.float32)
+    dim = np.zeros(3, np.int64)
+    o = np.zeros(3, np.float32)
+    s = np.zeros(3, np.float32)
+    rc = lib().orc_resample_geometry(RESAMPLE_KINDS[kind], ctypes.byref(h), _ptr(r), _ptr(dim), _ptr(o), _ptr(s))
+    if rc < 0:
+        raise ValueError("resample kind or rate the reference cannot use")
+    return bool(rc), tuple(int(v) for v in dim), tuple(float(v) for v in o), tuple(float(v) for v in s)
+
+
+def volume_resample(img, spacing, origin, dim, out_origin, out_spacing, nearest=False):
+    """volume_resample_float_li / _nn of img (z, y, x) onto dim (x, y, z)."""
+    img = np.ascontiguousarray(img, np.float32)
+    nz, ny, nx = img.shape
+    h = OrcVolHdr(c_int64_3(nx, ny, nz), c_float3(*origin), c_float3(*spacing), c_float9(*IDENTITY))
+    d = np.ascontiguousarray(dim, np.int64)
+    out = np.zeros((d[2], d[1], d[0]), np.float32)
+    lib().orc_volume_resample(ctypes.byref(h), _ptr(img), _ptr(d), _ptr(np.asarray(out_origin, np.float32)),
+                              _ptr(np.asarray(out_spacing, np.float32)), 1 if nearest else 0, _ptr(out))
+    return out
+
+
+def stage_resample(img, spacing, origin, kind, rate, nearest=False):
+    """registration_resample_volume: (resampled image, origin, spacing)."""
+    ident, dim, o, s = resample_geometry(kind, img.shape, spacing, origin, rate)
+    if ident:
+        return np.array(img, np.float32, copy=True), o, s
+    return volume_resample(img, spacing, origin, dim, o, s, nearest), o, s
 
 
 def reg_analytic(g, rp, coeff, nthreads=0):

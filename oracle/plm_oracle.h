@@ -113,6 +113,24 @@ int orc_mi_score_f64 (const orc_geom* g,
     const float* coeff, const orc_hist* hp, int nthreads,
     double* score, double* grad, int64_t* num_vox);
 
+/* Stage subsampling (resample_oracle.c).  Kinds follow the stage's
+   resample type and legacy_subsampling flag
+   (src/register/registration_resample.cxx:29-50). */
+enum {
+    ORC_RESAMPLE_VOXEL_RATE = 0,    /* RESAMPLE_AUTO / _VOXEL_RATE: volume_subsample_vox */
+    ORC_RESAMPLE_VOXEL_LEGACY = 1,  /* the same with legacy_subsampling */
+    ORC_RESAMPLE_MM = 2,            /* volume_resample_spacing */
+    ORC_RESAMPLE_PCT = 3            /* volume_resample_percent */
+};
+/* Output geometry; returns 1 when the volume is cloned (rate 1, 1, 1),
+   0 when it is resampled, -1 for a kind or rate the reference cannot use. */
+int orc_resample_geometry (int kind, const orc_vol_hdr* in, const float rate[3],
+    int64_t dim[3], float origin[3], float spacing[3]);
+/* volume_resample_float_li (nearest = 0) or _nn (nearest = 1) onto the
+   given geometry; out holds dim[0]*dim[1]*dim[2] floats. */
+void orc_volume_resample (const orc_vol_hdr* in, const float* img, const int64_t dim[3],
+    const float origin[3], const float spacing[3], int nearest, float* out);
+
 /* L-BFGS (lbfgs_oracle.c): the reference's vendored liblbfgs 1.9 on its
    More-Thuente path.  Return codes keep liblbfgs's values
    (libs/liblbfgs-1.9/include/lbfgs.h:72-145). */

@@ -29,6 +29,7 @@ PLM_MEM_HOST = 0
 PLM_MEM_DEVICE = 1
 PLM_STORE_F32 = 0
 PLM_STORE_BF16 = 1
+PLM_RESAMPLE = {"voxel_rate": 0, "voxel_legacy": 1, "mm": 2, "pct": 3}
 
 c_float3 = ctypes.c_float * 3
 c_int64_3 = ctypes.c_int64 * 3
@@ -148,6 +149,15 @@ SIGNATURES = {
     "plm_hip_batch_get_timing": (ctypes.c_int, [_P, _P, ctypes.c_int,
                                                 ctypes.POINTER(ctypes.c_int64)]),
     "plm_hip_batch_destroy": (ctypes.c_int, [_P]),
+    "plm_hip_resample_geometry": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PlmImgDesc), _P, _P, _P, _P,
+                                                 ctypes.POINTER(ctypes.c_int)]),
+    "plm_hip_volume_resample": (ctypes.c_int, [ctypes.POINTER(PlmImgDesc), _P, _P, _P, ctypes.c_int, _P,
+                                               ctypes.c_int, ctypes.c_int]),
+    "plm_hip_stage_resample": (ctypes.c_int, [ctypes.c_int, _P, ctypes.POINTER(PlmImgDesc), ctypes.c_int, _P,
+                                              ctypes.c_int, ctypes.c_int]),
+    "plm_hip_hist_eqsp": (ctypes.c_int, [ctypes.POINTER(PlmImgDesc), ctypes.c_int64,
+                                         ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                         ctypes.c_int]),
     "plm_hip_lbfgs_parms_init": (None, [ctypes.POINTER(PlmLbfgsParms)]),
     "plm_hip_bspline_optimize_lbfgs": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(PlmRegParms),
                                                       ctypes.POINTER(PlmLbfgsParms), _P, _P, PLM_LBFGS_PROGRESS,

@@ -106,18 +106,11 @@ class Joint_histogram:
         self.fixed_offset = self.fixed_delta = 0.0
         self.moving_offset = self.moving_delta = 0.0
 
-    @staticmethod
-    def _eqsp(img, bins):
-        a = img.detach().cpu().numpy() if _is_torch(img) else img
-        mn = np.float32(a.min())
-        mx = np.float32(a.max())
-        delta = np.float32((mx - mn) / np.float32(bins - 1))
-        offset = np.float32(np.float64(mn) - 0.5 * np.float64(delta))
-        return float(offset), float(delta)
-
-    def initialize(self, fixed, moving):
-        self.fixed_offset, self.fixed_delta = self._eqsp(fixed.img, self.fixed_bins)
-        self.moving_offset, self.moving_delta = self._eqsp(moving.img, self.moving_bins)
+    def initialize(self, fixed, moving, gpuid=0):
+        """Bin offsets and widths from each image's range, scanned on the GPU
+        (bspline_initialize_mi_hist_eqsp)."""
+        self.fixed_offset, self.fixed_delta = hist_eqsp(fixed, self.fixed_bins, gpuid)
+        self.moving_offset, self.moving_delta = hist_eqsp(moving, self.moving_bins, gpuid)
         return self
 
     def desc(self):
@@ -149,6 +142,70 @@ def bf16_round(a):
     u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
     r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
     return (r.astype(np.uint32) << 16).view(np.float32).reshape(np.shape(a))
+
+
+def hist_eqsp(vol, bins, gpuid=0):
+    """Equal-spacing histogram (offset, delta) of a volume: min/max scan on
+    the GPU (src/register/joint_histogram.cxx:115-139)."""
+    lib = _abi.load()
+    d = vol.desc()
+    off, dl = ctypes.c_float(), ctypes.c_float()
+    _abi.check(lib.plm_hip_hist_eqsp(ctypes.byref(d), int(bins), ctypes.byref(off), ctypes.byref(dl), gpuid))
+    return float(off.value), float(dl.value)
+
+
+def resample_geometry(vol, kind, rate):
+    """registration_resample_volume's output geometry for a stage:
+    (identity, dim xyz, origin, spacing)."""
+    lib = _abi.load()
+    d = vol.desc()
+    r = np.ascontiguousarray(rate, np.float32)
+    dim = np.zeros(3, np.int64)
+    o = np.zeros(3, np.float32)
+    sp = np.zeros(3, np.float32)
+    ident = ctypes.c_int()
+    _abi.check(lib.plm_hip_resample_geometry(_abi.PLM_RESAMPLE[kind], ctypes.byref(d), r.ctypes.data,
+                                             dim.ctypes.data, o.ctypes.data, sp.ctypes.data, ctypes.byref(ident)))
+    return bool(ident.value), tuple(int(v) for v in dim), tuple(float(v) for v in o), tuple(float(v) for v in sp)
+
+
+def _new_like(vol, shape):
+    if _is_torch(vol.img):
+        import torch
+        return torch.empty(shape, dtype=torch.float32, device=vol.img.device), _abi.PLM_MEM_DEVICE
+    return np.empty(shape, np.float32), _abi.PLM_MEM_HOST
+
+
+def _ptr_of(a):
+    return ctypes.c_void_p(a.data_ptr() if _is_torch(a) else a.ctypes.data)
+
+
+def volume_resample(vol, dim, origin, spacing, nearest=False, gpuid=0):
+    """vol resampled onto dim (x, y, z) / origin / spacing on the GPU,
+    trilinear or nearest voxel (volume_resample_float_li / _nn)."""
+    lib = _abi.load()
+    d = vol.desc()
+    out, loc = _new_like(vol, (dim[2], dim[1], dim[0]))
+    # named arrays: they must outlive the call that reads them
+    a_dim = np.asarray(dim, np.int64)
+    a_org = np.asarray(origin, np.float32)
+    a_sp = np.asarray(spacing, np.float32)
+    _abi.check(lib.plm_hip_volume_resample(ctypes.byref(d), a_dim.ctypes.data, a_org.ctypes.data,
+                                           a_sp.ctypes.data, 1 if nearest else 0, _ptr_of(out), loc, gpuid))
+    return Volume(out, spacing, origin, vol.direction_cosines)
+
+
+def stage_resample(vol, kind, rate, nearest=False, gpuid=0):
+    """registration_resample_volume on the GPU: the stage's subsampled
+    volume (kind: voxel_rate, voxel_legacy, mm, pct)."""
+    lib = _abi.load()
+    ident, dim, origin, spacing = resample_geometry(vol, kind, rate)
+    d = vol.desc()
+    out, loc = _new_like(vol, (dim[2], dim[1], dim[0]))
+    a_rate = np.asarray(rate, np.float32)
+    _abi.check(lib.plm_hip_stage_resample(_abi.PLM_RESAMPLE[kind], a_rate.ctypes.data, ctypes.byref(d),
+                                          1 if nearest else 0, _ptr_of(out), loc, gpuid))
+    return Volume(out, spacing, origin, vol.direction_cosines)
 
 
 def volume_gradient(vol, gpuid=0):

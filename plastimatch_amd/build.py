@@ -7,7 +7,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libplmregisterhip.so")
 
-SOURCES = ["bspline_kernels.hip", "lbfgs_kernels.hip", "bspline_abi.cpp", "bspline_lbfgs.cpp"]
+SOURCES = ["bspline_kernels.hip", "lbfgs_kernels.hip", "stage_kernels.hip", "bspline_abi.cpp",
+           "bspline_lbfgs.cpp", "stage_abi.cpp"]
 
 # -ffp-contract=off: the score path reproduces the reference's float rounding
 # operation by operation; FMAs are written explicitly where they are wanted.

@@ -930,6 +930,12 @@ abi_hip_fail (hipError_t e, const char* what)
 }
 
 int
+abi_check_img (const plm_img_desc* d, const char* name)
+{
+    return check_img (d, name);
+}
+
+int
 ctx_info (const plm_hip_ctx* c, int* device, hipStream_t* stream, int64_t* num_knots)
 {
     if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");

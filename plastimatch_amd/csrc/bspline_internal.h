@@ -14,6 +14,9 @@ namespace plmhip {
 int abi_fail (int code, const std::string& msg);
 // The same for a failed HIP call.
 int abi_hip_fail (hipError_t e, const char* what);
+// Validate an image descriptor: non-null data, known location, dims >= 2,
+// non-zero spacing, fewer than 2^31 / 3 voxels.
+int abi_check_img (const plm_img_desc* d, const char* name);
 // A context's device, stream and knot count.
 int ctx_info (const plm_hip_ctx* c, int* device, hipStream_t* stream, int64_t* num_knots);
 

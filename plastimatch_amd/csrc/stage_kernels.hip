@@ -1,0 +1,112 @@
+// Stage preprocessing kernels: the float resample of
+// volume_resample_float_li / _nn (src/base/volume_resample.cxx:26-135) over
+// host-made axis tables, and the min/max scan of
+// bspline_initialize_mi_hist_eqsp (src/register/joint_histogram.cxx:
+// 115-139).  Built with -ffp-contract=off: the eight weighted corners are
+// formed and summed exactly as LI_VALUE writes them
+// (src/base/interpolate_macros.h:23-38), so the result is the reference's
+// bit for bit.
+
+#include "stage_kernels.h"
+
+namespace plmhip {
+
+namespace {
+
+constexpr int RS_THREADS = 256;
+
+// One workgroup per (x segment, row j, slice k) of the output.
+template <bool NEAREST>
+__global__ void __launch_bounds__ (RS_THREADS)
+k_resample (const float* __restrict__ in, int id0, int id1, float* __restrict__ out, int od0,
+    int od1, const ResampleAxis* __restrict__ ax, const ResampleAxis* __restrict__ ay,
+    const ResampleAxis* __restrict__ az)
+{
+    const int i = blockIdx.x * RS_THREADS + threadIdx.x;
+    if (i >= od0) return;
+    const int j = blockIdx.y, k = blockIdx.z;
+    const ResampleAxis X = ax[i], Y = ay[j], Z = az[k];
+    const int64_t v = i + (int64_t) od0 * (j + (int64_t) od1 * k);
+    if (!(X.inside & Y.inside & Z.inside)) {
+        out[v] = 0.0f;
+        return;
+    }
+    const int64_t d01 = (int64_t) id0 * id1;
+    const int64_t f = X.idx + (int64_t) id0 * Y.idx + d01 * Z.idx;
+    if (NEAREST) {
+        out[v] = in[f];
+        return;
+    }
+    const float a = X.w1 * Y.w1 * Z.w1 * in[f];
+    const float b = X.w2 * Y.w1 * Z.w1 * in[f + 1];
+    const float c = X.w1 * Y.w2 * Z.w1 * in[f + id0];
+    const float e = X.w2 * Y.w2 * Z.w1 * in[f + id0 + 1];
+    const float g = X.w1 * Y.w1 * Z.w2 * in[f + d01];
+    const float h = X.w2 * Y.w1 * Z.w2 * in[f + d01 + 1];
+    const float p = X.w1 * Y.w2 * Z.w2 * in[f + d01 + id0];
+    const float q = X.w2 * Y.w2 * Z.w2 * in[f + d01 + id0 + 1];
+    out[v] = a + b + c + e + g + h + p + q;
+}
+
+__global__ void __launch_bounds__ (256)
+k_minmax (const float* __restrict__ img, int64_t n, float* __restrict__ part)
+{
+    __shared__ float smin[4], smax[4];
+    float mn = img[0], mx = img[0];
+    const int64_t stride = (int64_t) gridDim.x * 256;
+    for (int64_t e = (int64_t) blockIdx.x * 256 + threadIdx.x; e < n; e += stride) {
+        const float v = img[e];
+        // a NaN fails both tests, as in the reference's scan
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float a = __shfl_xor (mn, o, 64), b = __shfl_xor (mx, o, 64);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        smin[threadIdx.x >> 6] = mn;
+        smax[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads ();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            mn = smin[w] < mn ? smin[w] : mn;
+            mx = smax[w] > mx ? smax[w] : mx;
+        }
+        part[2 * blockIdx.x] = mn;
+        part[2 * blockIdx.x + 1] = mx;
+    }
+}
+
+} // namespace
+
+hipError_t
+launch_resample (const float* in, const int id[3], float* out, const int od[3],
+    const ResampleAxis* ax, const ResampleAxis* ay, const ResampleAxis* az, int nearest,
+    hipStream_t st)
+{
+    const dim3 grid ((od[0] + RS_THREADS - 1) / RS_THREADS, od[1], od[2]);
+    if (nearest)
+        hipLaunchKernelGGL (k_resample<true>, grid, dim3 (RS_THREADS), 0, st, in, id[0], id[1], out,
+            od[0], od[1], ax, ay, az);
+    else
+        hipLaunchKernelGGL (k_resample<false>, grid, dim3 (RS_THREADS), 0, st, in, id[0], id[1], out,
+            od[0], od[1], ax, ay, az);
+    return hipGetLastError ();
+}
+
+hipError_t
+launch_minmax (const float* img, int64_t n, float* part, int* blocks, hipStream_t st)
+{
+    int64_t b = (n + 255) / 256;
+    if (b > MINMAX_BLOCKS) b = MINMAX_BLOCKS;
+    if (b < 1) b = 1;
+    *blocks = (int) b;
+    hipLaunchKernelGGL (k_minmax, dim3 ((unsigned) b), dim3 (256), 0, st, img, n, part);
+    return hipGetLastError ();
+}
+
+} // namespace plmhip

@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 from plastimatch_amd import _abi, synth
-from plastimatch_amd.bspline import Bspline_xform, Joint_histogram, Volume
+from plastimatch_amd.bspline import Bspline_xform, Volume
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "plm_hip_bspline.h")
@@ -75,15 +75,6 @@ def test_host_grid_matches_oracle(name):
     r, c, nk = oracle.header(oracle.geom(cfg["dim"], cfg["spacing"], cfg["vox_per_rgn"]))
     assert bxf.rdims == tuple(r) and bxf.cdims == tuple(c) and bxf.num_knots == nk
     assert bxf.num_coeff == 3 * nk
-
-
-def test_joint_histogram_eqsp_matches_oracle():
-    rng = np.random.default_rng(0)
-    f = rng.normal(0, 300, (10, 11, 12)).astype(np.float32)
-    m = rng.normal(40, 100, (10, 11, 12)).astype(np.float32)
-    jh = Joint_histogram(32, 20).initialize(Volume(f, (1, 1, 1)), Volume(m, (1, 1, 1)))
-    assert (jh.fixed_offset, jh.fixed_delta) == oracle.hist_eqsp(f, 32)
-    assert (jh.moving_offset, jh.moving_delta) == oracle.hist_eqsp(m, 20)
 
 
 def test_jitter_if_zero():
