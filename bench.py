@@ -26,6 +26,8 @@ with their rooflines and the oracle's versions timed beside them.
 "optimizer_loop" runs the device L-BFGS (liblbfgs's algorithm) for 20
 iterations on the headline workload, with and without regularization, and
 the reference's liblbfgs on the oracle's score for 3 iterations beside it.
+"stage_preprocess" times a stage's preprocessing on the device (both
+images subsampled, moving gradient, histogram ranges) against the oracle.
 """
 import argparse
 import json
@@ -457,6 +459,64 @@ def optimizer_leg(args, dev, eval_ms, want_cpu):
     return out
 
 
+STAGE_RATE = (2, 2, 1)
+
+
+def stage_leg(args, dev, want_cpu):
+    """Stage preprocessing of the headline workload on the device: both
+    256x256x128 images subsampled by (2, 2, 1) with legacy subsampling, the
+    moving gradient of the subsampled moving image, and the MI histogram
+    ranges of both -- everything registration_data.cxx:350-384 and
+    joint_histogram.cxx:115-139 do before a stage's first evaluation.  Wall
+    time of the whole sequence from device-resident inputs (each call also
+    uploads its small axis tables); the oracle's restatements timed beside it,
+    and whether the results are the oracle's bit for bit."""
+    import torch
+    from plastimatch_amd import synth
+    from plastimatch_amd.bspline import Volume, hist_eqsp, stage_resample, volume_gradient
+
+    pair = synth.make_config(WORKLOAD, seed=0)
+    fx = Volume(torch.from_numpy(pair.fixed).to(dev), pair.spacing, pair.origin)
+    mv = Volume(torch.from_numpy(pair.moving).to(dev), pair.spacing, pair.origin)
+
+    def run():
+        f_ss = stage_resample(fx, "voxel_legacy", STAGE_RATE)
+        m_ss = stage_resample(mv, "voxel_legacy", STAGE_RATE)
+        mg = volume_gradient(m_ss)
+        h = (hist_eqsp(f_ss, 32), hist_eqsp(m_ss, 32))
+        return f_ss, m_ss, mg, h
+
+    for _ in range(3):
+        run()
+    torch.cuda.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f_ss, m_ss, mg, h = run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    n_in, n_out = pair.fixed.size, f_ss.npix
+    out = {"workload": WORKLOAD, "resample": f"voxel_legacy {STAGE_RATE}, trilinear",
+           "in_dim": list(pair.dim), "out_dim": list(f_ss.dim), "ms_per_stage": dt * 1e3,
+           "resample_alg_bytes_per_image": 4 * (n_in + n_out)}
+    if want_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        rf, _, _ = oracle.stage_resample(pair.fixed, pair.spacing, pair.origin, "voxel_legacy", STAGE_RATE)
+        rm, o, s = oracle.stage_resample(pair.moving, pair.spacing, pair.origin, "voxel_legacy", STAGE_RATE)
+        rg = oracle.volume_gradient(rm, s, o)
+        rh = (oracle.hist_eqsp(rf, 32), oracle.hist_eqsp(rm, 32))
+        t_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": t_cpu * 1e3, "unit": "ms per stage", "cores": 1, "kind": "port",
+                               "sample": "one stage preprocessing (oracle restatements, one thread)"}
+        out["bit_exact_vs_oracle"] = {
+            "fixed": bool(np.array_equal(f_ss.img.cpu().numpy(), rf)),
+            "moving": bool(np.array_equal(m_ss.img.cpu().numpy(), rm)),
+            "moving_gradient": bool(np.array_equal(mg.img.cpu().numpy(), rg)),
+            "hist_eqsp": h == rh}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -467,6 +527,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-path leg")
     ap.add_argument("--no-warp-reg", action="store_true", help="skip the warp and regularization leg")
     ap.add_argument("--no-optimizer", action="store_true", help="skip the optimizer-loop leg")
+    ap.add_argument("--no-stage", action="store_true", help="skip the stage-preprocessing leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy kernel 10 times "
                          "(reference for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -581,6 +642,8 @@ def main():
         out["warp_regularize"] = warp_reg_leg(args, dev, want_cpu=not args.no_cpu)
     if not args.no_optimizer:
         out["optimizer_loop"] = optimizer_leg(args, dev, ms_per_step, want_cpu=not args.no_cpu)
+    if not args.no_stage:
+        out["stage_preprocess"] = stage_leg(args, dev, want_cpu=not args.no_cpu)
     print(json.dumps(out))
 
 

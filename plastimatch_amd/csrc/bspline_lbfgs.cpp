@@ -6,13 +6,14 @@
 // recursion over the last m (s, y) pairs with H0 = (ys/yy) I, and the
 // More-Thuente line search with every evaluation reported to the progress
 // rule.  Its vectors live in device memory (lbfgs_kernels.hip); this file
-// holds the scalar logic.  Per evaluation the host reads back only the
-// objective and three dot products (g.d, x.x, g.g), which the post pass
-// writes straight into pinned host memory.
+// holds the scalar logic -- the line search and the small dense solve of
+// the direction.  Per evaluation the host reads back only a few dozen
+// totals (the objective, g.d, x.x, g.g, the new pair's products), which a
+// one-workgroup pass writes into mapped host memory.
 //
 // The first trial of each line search is formed by the pass that forms
-// the direction, before the host knows the initial slope g.d: the slope's
-// partials come back with that trial's results.  Should the slope be
+// the direction, before the host knows the initial slope g.d: the slope
+// comes back with that trial's results.  Should the slope be
 // positive (liblbfgs returns LBFGSERR_INCREASEGRADIENT before evaluating
 // anything), the trial is dropped uncounted -- its coefficients went to the
 // spare buffer -- so the run's record matches liblbfgs's.
@@ -27,7 +28,6 @@
 #include "bspline_internal.h"
 #include "lbfgs_kernels.h"
 
-using plmhip::lb_reduce_parts_host;
 
 namespace {
 
@@ -176,11 +176,11 @@ struct Run {
     double* d = nullptr;
     double* S = nullptr;                    // history slot j at S + j n
     double* Y = nullptr;
-    plmhip::LbGram* gram = nullptr;
-    double* part = nullptr;                 // k_lb_post's pair partials
     double* score = nullptr;                // device: score, reg score, num_vox
-    double* hh = nullptr;                   // pinned, written by the kernels:
-                                            // score, reg score, num_vox, dots [3 nb], slope [nb]
+    double* part = nullptr;                 // post partials, component c at part + c nb
+    double* slope_part = nullptr;           // combine's partials of g.d
+    double* hh = nullptr;                   // pinned, written by k_lb_finish: score, reg score,
+                                            // num_vox, slope, post component totals
     double* hh_dev = nullptr;               // its device address
     int xi = 0, xpi = 1, gi = 0, gpi = 1;
     int c_last = -1;        // C buffer of the last counted evaluation
@@ -192,15 +192,19 @@ struct Run {
     // direction from dir_b pairs (newest in slot dir_e); each one writes
     // its would-be pair into slot post_e with post_bp earlier pairs
     int dir_b = 0, dir_e = 0, post_e = -1, post_bp = 0;
+    int ncomp = 0;                          // post components held: 7 + 5 (m - 1)
+    // Gram entries of the history by slot: SY[i m + j] = s_i.y_j, YY = y_i.y_j
+    std::vector<double> SY, YY;
+    plmhip::LbCombineArgs dir {};           // the next direction's coefficients
 
     ~Run ()
     {
-        void* bufs[] = {X[0], X[1], G[0], G[1], C[0], C[1], d, S, Y, gram, part, score};
+        void* bufs[] = {X[0], X[1], G[0], G[1], C[0], C[1], d, S, Y, score, part, slope_part};
         for (void* b : bufs) if (b) (void) hipFree (b);
         if (hh) (void) hipHostFree (hh);
     }
 
-    size_t hh_doubles () const { return 3 + 4 * (size_t) nb; }
+    size_t hh_doubles () const { return 4 + (size_t) ncomp; }
 
     int hip (hipError_t e, const char* what)
     {
@@ -226,8 +230,17 @@ struct Run {
         if (kind == 0) {
             e = plmhip::lb_launch_trial (x0, nullptr, 0., X[xi], cf, n, st);
         } else if (kind == 1) {
-            plmhip::LbCombineArgs a {part, dir_b, dir_e, p.m, S, Y, gram, G[gpi], X[xpi], stp, d, X[xi], cf,
-                hh_dev + 3 + 3 * (size_t) nb};
+            plmhip::LbCombineArgs a = dir;
+            a.b = dir_b;
+            a.S = S;
+            a.Y = Y;
+            a.g = G[gpi];
+            a.xp = X[xpi];
+            a.stp = stp;
+            a.d = d;
+            a.x = X[xi];
+            a.coeff = cf;
+            a.slope = slope_part;
             e = plmhip::lb_launch_combine (a, n, st);
         } else {
             e = plmhip::lb_launch_trial (X[xpi], d, stp, X[xi], cf, n, st);
@@ -241,13 +254,16 @@ struct Run {
             if (rc < 0) return err = rc;
         }
         plmhip::LbPostArgs pa {X[xi], X[xpi], G[gi], G[gpi], kind == 0 ? nullptr : d, S, Y,
-            kind == 0 ? -1 : post_e, post_bp, p.m, score, hh_dev, part};
+            kind == 0 ? -1 : post_e, post_bp, p.m, part};
         if (hip (plmhip::lb_launch_post (pa, n, st), "L-BFGS post pass")) return err;
+        const int nc = kind == 0 ? 3 : plmhip::lb_post_components (post_bp);
+        if (hip (plmhip::lb_launch_finish (part, nc, nb, slope_part, score, hh_dev, st), "L-BFGS totals"))
+            return err;
         if (hip (hipStreamSynchronize (st), "L-BFGS evaluation")) return err;
         f = reg ? hh[0] + hh[1] : hh[0];
-        dg = lb_reduce_parts_host (hh + 3, nb);
-        xx = lb_reduce_parts_host (hh + 3 + nb, nb);
-        gg = lb_reduce_parts_host (hh + 3 + 2 * nb, nb);
+        dg = hh[4];
+        xx = hh[5];
+        gg = hh[6];
         return PLM_HIP_OK;
     }
 
@@ -268,6 +284,63 @@ struct Run {
         if (!ret && ((p.max_its >= 0 && it > p.max_its) || (p.max_feval >= 0 && feval > p.max_feval)))
             ret = 1;
         return ret;
+    }
+
+    // The coefficients of the next direction from the accepted trial's post
+    // partials: the newest pair (slot e) and the b - 1 before it.  In age
+    // order (position 0 the oldest), R the upper triangle of S'Y, D its
+    // diagonal, gamma = ys / yy of the newest pair, p = S'g, q = Y'g:
+    //   u = R^-1 p,  a = R^-T ((D + gamma Y'Y) u - gamma q),
+    //   H g = gamma g + S a - gamma Y u     (the compact form; its d = -H g
+    // is the two-loop recursion's).  The newest pair's Gram row and column
+    // are stored for the later iterations.
+    void solve_direction (int b, int e)
+    {
+        const int m = p.m;
+        dir_b = b;
+        dir_e = e;
+        // post components: 3 s_e.y_e, 4 y_e.y_e, 5 s_e.g, 6 y_e.g; for the
+        // pair k older than the newest, 7 + 5 (k - 1) + {s_e.y_j, s_j.y_e,
+        // y_e.y_j, s_j.g, y_j.g}
+        auto red = [&] (int c) { return hh[4 + c]; };
+        SY[e * m + e] = red (3);
+        YY[e * m + e] = red (4);
+        double P[plmhip::LB_MAX_M], Q[plmhip::LB_MAX_M];
+        int o[plmhip::LB_MAX_M];
+        for (int i = 0; i < b; ++i) o[i] = (e - (b - 1) + i + m) % m;
+        P[b - 1] = red (5);
+        Q[b - 1] = red (6);
+        for (int k = 1; k < b; ++k) {
+            const int j = (e - k + m) % m;
+            const int c = 7 + 5 * (k - 1);
+            SY[e * m + j] = red (c);
+            SY[j * m + e] = red (c + 1);
+            YY[e * m + j] = YY[j * m + e] = red (c + 2);
+            P[b - 1 - k] = red (c + 3);
+            Q[b - 1 - k] = red (c + 4);
+        }
+        auto sy = [&] (int i, int j) { return SY[o[i] * m + o[j]]; };
+        auto yy = [&] (int i, int j) { return YY[o[i] * m + o[j]]; };
+        const double gamma = sy (b - 1, b - 1) / yy (b - 1, b - 1);
+        double u[plmhip::LB_MAX_M], t[plmhip::LB_MAX_M];
+        for (int i = b - 1; i >= 0; --i) {
+            double acc = P[i];
+            for (int j = i + 1; j < b; ++j) acc = acc - sy (i, j) * u[j];
+            u[i] = acc / sy (i, i);
+        }
+        for (int i = 0; i < b; ++i) {
+            double w = 0.0;
+            for (int j = 0; j < b; ++j) w = w + yy (i, j) * u[j];
+            t[i] = sy (i, i) * u[i] + gamma * w - gamma * Q[i];
+        }
+        for (int i = 0; i < b; ++i) {
+            double acc = t[i];
+            for (int j = 0; j < i; ++j) acc = acc - sy (j, i) * dir.ca[j];
+            dir.ca[i] = acc / sy (i, i);
+            dir.cu[i] = u[i];
+            dir.slot[i] = o[i];
+        }
+        dir.gamma = gamma;
     }
 
     // The More-Thuente search along d from (xp, gp), the current point
@@ -302,7 +375,7 @@ struct Run {
 
             if (trial (count_ == 0 ? 1 : 2, *stp)) return PLM_LBFGSERR_INVALIDPARAMETERS;
             if (count_ == 0) {
-                dginit = lb_reduce_parts_host (hh + 3 + 3 * nb, nb);
+                dginit = hh[3];
                 if (0 < dginit) return PLM_LBFGSERR_INCREASEGRADIENT;    // trial dropped
                 dgtest = p.ftol * dginit;
                 bx.g = by.g = dginit;
@@ -403,6 +476,9 @@ plm_hip_bspline_optimize_lbfgs (plm_hip_ctx* c, int metric, int grad_mode,
     r.c = c;
     r.n = 3 * knots;
     r.nb = plmhip::lb_blocks (r.n);
+    r.ncomp = plmhip::lb_post_components (r.p.m - 1);
+    r.SY.assign ((size_t) r.p.m * r.p.m, 0.0);
+    r.YY.assign ((size_t) r.p.m * r.p.m, 0.0);
     r.metric = metric;
     r.grad_mode = grad_mode;
     r.reg = reg;
@@ -414,9 +490,9 @@ plm_hip_bspline_optimize_lbfgs (plm_hip_ctx* c, int metric, int grad_mode,
         || r.alloc (&r.G[0], n, "hipMalloc(L-BFGS g)") || r.alloc (&r.G[1], n, "hipMalloc(L-BFGS g)")
         || r.alloc (&r.C[0], n, "hipMalloc(L-BFGS coeff)") || r.alloc (&r.C[1], n, "hipMalloc(L-BFGS coeff)")
         || r.alloc (&r.d, n, "hipMalloc(L-BFGS d)") || r.alloc (&r.S, n * m, "hipMalloc(L-BFGS s)")
-        || r.alloc (&r.Y, n * m, "hipMalloc(L-BFGS y)") || r.alloc (&r.gram, 1, "hipMalloc(L-BFGS Gram)")
-        || r.alloc (&r.part, (size_t) plmhip::lb_post_components (m - 1) * r.nb, "hipMalloc(L-BFGS partials)")
-        || r.alloc (&r.score, 3, "hipMalloc(L-BFGS score)"))
+        || r.alloc (&r.Y, n * m, "hipMalloc(L-BFGS y)") || r.alloc (&r.score, 3, "hipMalloc(L-BFGS score)")
+        || r.alloc (&r.part, (size_t) r.ncomp * r.nb, "hipMalloc(L-BFGS partials)")
+        || r.alloc (&r.slope_part, (size_t) r.nb, "hipMalloc(L-BFGS slope partials)"))
         return r.err;
     if (r.hip (hipHostMalloc ((void**) &r.hh, sizeof (double) * r.hh_doubles (), hipHostMallocMapped),
             "hipHostMalloc(L-BFGS readback)")
@@ -424,7 +500,9 @@ plm_hip_bspline_optimize_lbfgs (plm_hip_ctx* c, int metric, int grad_mode,
         return r.err;
     // the regularizer adds into the gradient and leaves its score; without
     // it the reg score slot stays zero
-    if (r.hip (hipMemsetAsync (r.score, 0, sizeof (double) * 3, r.st), "L-BFGS score clear")) return r.err;
+    if (r.hip (hipMemsetAsync (r.score, 0, sizeof (double) * 3, r.st), "L-BFGS score clear")
+        || r.hip (hipMemsetAsync (r.slope_part, 0, sizeof (double) * r.nb, r.st), "L-BFGS slope clear"))
+        return r.err;
 
     std::vector<double> pf ((size_t) (r.p.past > 0 ? r.p.past : 1));
     int ret = 0;
@@ -486,8 +564,7 @@ plm_hip_bspline_optimize_lbfgs (plm_hip_ctx* c, int metric, int grad_mode,
         }
         // the accepted trial's pair (slot end) is the newest of the next
         // direction, with the bound - 1 before it
-        r.dir_b = m <= k ? m : k;
-        r.dir_e = end;
+        r.solve_direction (m <= k ? m : k, end);
         ++k;
         end = (end + 1) % m;
         step = 1.0;

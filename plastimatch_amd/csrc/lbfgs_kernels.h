@@ -1,19 +1,21 @@
 // Vector kernels of the device-resident L-BFGS (internal).
 //
 // Every vector of the optimizer (x, g, the direction d, the m (s, y)
-// pairs) stays in device memory in double; the host runs only the scalar
-// logic of the More-Thuente line search.  Per evaluation one pass
-// (k_lb_post) forms every dot product the optimizer needs next: the line
-// search's g.d, x.x, g.g, and -- should the trial be accepted -- the new
-// pair s = x - xp, y = g - gp, its products with the pairs kept in the
-// history and with g.  Per iteration one pass (k_lb_combine) turns them
-// into the search direction by the compact form of the L-BFGS matrix
-// (Byrd, Nocedal and Schnabel, Math. Prog. 63, 1994, section 3), which is
-// the two-loop recursion of liblbfgs (libs/liblbfgs-1.9/lib/lbfgs.c:
-// 555-609) regrouped: d = -H g with H0 = (ys/yy) I of the newest pair.
-// Dot products are reduced in two fixed-order levels -- per workgroup into
-// partials, then the partials in lane-then-butterfly order -- so every run
-// gives the same bits.
+// pairs) stays in device memory in double; the host runs only scalar
+// work: the More-Thuente line search and the small dense algebra of the
+// direction.  Per evaluation one pass (k_lb_post) forms every dot product
+// the optimizer needs next -- the line search's g.d, x.x, g.g and, should
+// the trial be accepted, the new pair s = x - xp, y = g - gp with its
+// products against the pairs kept in the history and against g -- as
+// per-workgroup partials; a one-workgroup pass (k_lb_finish) sums them and
+// writes the totals into mapped host memory.  The host keeps the history's
+// Gram entries and solves the compact
+// form of the L-BFGS matrix (Byrd, Nocedal and Schnabel, Math. Prog. 63,
+// 1994, section 3), which is liblbfgs's two-loop recursion
+// (libs/liblbfgs-1.9/lib/lbfgs.c:555-609) regrouped; per iteration one pass
+// (k_lb_combine) then forms d = -H g, the first trial point and the
+// partials of the initial slope.  Every sum has a fixed order, so a run
+// gives the same bits every time.
 #ifndef PLM_LBFGS_KERNELS_H
 #define PLM_LBFGS_KERNELS_H
 
@@ -44,18 +46,6 @@ lb_blocks (int64_t n)
     return (int) (b < 1 ? 1 : b > LB_MAX_BLOCKS ? LB_MAX_BLOCKS : b);
 }
 
-// Inner products of the history by slot: SY[i][j] = s_i.y_j, YY[i][j] =
-// y_i.y_j, kept on the device from one iteration to the next.
-struct LbGram {
-    double SY[LB_MAX_M][LB_MAX_M];
-    double YY[LB_MAX_M][LB_MAX_M];
-};
-
-// The host's reduction of nb partials, in the order the kernels use:
-// lane l sums parts l, l + 64, ... in turn, then a butterfly over the 64
-// lane sums (s_l + s_{l^o}, o = 32 .. 1).
-double lb_reduce_parts_host (const double* parts, int nb);
-
 // What k_lb_post reads and writes.
 struct LbPostArgs {
     const double* x;        // the trial point
@@ -68,9 +58,7 @@ struct LbPostArgs {
     int e;                  // slot of the new pair; -1: no pair (start point)
     int bp;                 // earlier pairs kept with it, 0 .. m - 1
     int m;
-    const double* score;    // objective parts written by the evaluation:
-    double* host;           // copied to host[0..2]; line-search partials at host + 3
-    double* part;           // pair partials: component c >= 3 at part + (c - 3) nb
+    double* part;           // component c's partials at part + c nb
 };
 
 // x = xp + stp * d (x = xp when d is null: the start point); coeff =
@@ -79,23 +67,30 @@ hipError_t lb_launch_trial (const double* xp, const double* d, double stp, doubl
     float* coeff, int64_t n, hipStream_t st);
 // The post-evaluation pass.
 hipError_t lb_launch_post (const LbPostArgs& a, int64_t n, hipStream_t st);
+// Totals for the host: host[0..2] = score[0..2]; host[3] = the slope
+// (nb partials at slope_part); host[4 + c] = component c (nb partials at
+// part + c nb), c < ncomp.  One workgroup, fixed order.
+hipError_t lb_launch_finish (const double* part, int ncomp, int nb, const double* slope_part,
+    const double* score, double* host, hipStream_t st);
 
-// What k_lb_combine reads and writes.
+// What k_lb_combine reads and writes: d = -gamma g - sum_i ca_i s_i +
+// gamma sum_i cu_i y_i over the b pairs in slots slot[0..b) (oldest
+// first); b = 0: d = -g.
 struct LbCombineArgs {
-    const double* part;     // k_lb_post's pair partials (b > 0)
-    int b;                  // pairs in the direction: 0 (d = -g) .. m
-    int e;                  // newest slot
-    int m;
+    int b;
+    int slot[LB_MAX_M];
+    double ca[LB_MAX_M];
+    double cu[LB_MAX_M];
+    double gamma;
     const double* S;
     const double* Y;
-    LbGram* gram;
     const double* g;        // gradient at the current point
     const double* xp;       // the current point
     double stp;             // first trial step
     double* d;              // out: direction
     double* x;              // out: xp + stp d
     float* coeff;           // out: (float) x
-    double* slope;          // out: partials of g.d (host memory)
+    double* slope;          // out: partials of g.d
 };
 
 hipError_t lb_launch_combine (const LbCombineArgs& a, int64_t n, hipStream_t st);
