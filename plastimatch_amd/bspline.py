@@ -90,6 +90,70 @@ class Bspline_xform:
         if not np.any(self.coeff != 0.0):
             self.coeff[:] = np.float32(0.01)
 
+    def save(self, path):
+        """Write the MGH_GPUIT_BSP text file of Bspline_xform::save
+        (src/base/bspline_xform.cxx:101-145): the header, then the
+        coefficients planar (all x, all y, all z), one "%.20f" per line."""
+        lines = ["MGH_GPUIT_BSP <experimental>",
+                 "img_origin = %f %f %f" % self.img_origin,
+                 "img_spacing = %f %f %f" % self.img_spacing,
+                 "img_dim = %u %u %u" % self.img_dim,
+                 "roi_offset = %d %d %d" % self.roi_offset,
+                 "roi_dim = %d %d %d" % self.roi_dim,
+                 "vox_per_rgn = %d %d %d" % self.vox_per_rgn,
+                 "direction_cosines = " + " ".join("%f" % v for v in self.direction_cosines)]
+        c = np.asarray(self.coeff, np.float32).reshape(-1, 3)
+        for d in range(3):
+            lines.extend("%.20f" % v for v in c[:, d].tolist())
+        with open(path, "w") as f:
+            f.write("\n".join(lines) + "\n")
+
+    @classmethod
+    def load(cls, path):
+        """Read an MGH_GPUIT_BSP file as bspline_xform_load does
+        (src/base/bspline_xform.cxx:147-279): header keys in any order, the
+        direction cosines optional (identity), then the planar
+        coefficients.  Raises ValueError where the reference returns 0."""
+        with open(path) as f:
+            lines = f.read().split("\n")
+        if not lines or not lines[0].startswith("MGH_GPUIT_BSP"):
+            raise ValueError(f"{path}: not an MGH_GPUIT_BSP file")
+        h = {"img_origin": (0.0, 0.0, 0.0), "img_spacing": (1.0, 1.0, 1.0), "img_dim": (0, 0, 0),
+             "roi_offset": (0, 0, 0), "roi_dim": (0, 0, 0), "vox_per_rgn": (0, 0, 0),
+             "direction_cosines": IDENTITY}
+        counts = {"img_origin": 3, "img_spacing": 3, "img_dim": 3, "roi_offset": 3, "roi_dim": 3,
+                  "vox_per_rgn": 3, "direction_cosines": 9}
+        i = 1
+        while True:
+            if i >= len(lines) or (i == len(lines) - 1 and lines[i] == ""):
+                raise ValueError(f"{path}: header ends without coefficients")
+            line = lines[i]
+            if "=" not in line:
+                break
+            key, _, val = line.partition("=")
+            key = key.strip()
+            if key not in counts:
+                raise ValueError(f"{path}: unknown header line {line!r}")
+            vals = val.split()
+            if len(vals) < counts[key]:
+                raise ValueError(f"{path}: short header line {line!r}")
+            vals = vals[:counts[key]]
+            if key in ("img_origin", "img_spacing", "direction_cosines"):
+                h[key] = tuple(float(np.float32(v)) for v in vals)
+            else:
+                h[key] = tuple(int(v) for v in vals)
+            i += 1
+        bxf = cls(h["img_origin"], h["img_spacing"], h["img_dim"], h["vox_per_rgn"], h["roi_offset"],
+                  h["roi_dim"], h["direction_cosines"])
+        if bxf.num_coeff < 1:
+            raise ValueError(f"{path}: no coefficients")
+        body = lines[i:i + bxf.num_coeff]
+        if len(body) < bxf.num_coeff or any(not t.strip() for t in body):
+            raise ValueError(f"{path}: {len(body)} coefficient lines for {bxf.num_coeff}")
+        planar = np.array([np.float32(float(t.split()[0])) for t in body], np.float32)
+        bxf.coeff = planar.reshape(3, -1).T.reshape(-1).copy()
+        return bxf
+
     def geom(self):
         return _abi.PlmBspGeom(_abi.c_float3(*self.img_origin), _abi.c_float3(*self.img_spacing),
                                _abi.c_int64_3(*self.img_dim), _abi.c_int64_3(*self.roi_offset),

@@ -28,6 +28,7 @@ usage ()
 {
     fprintf (stderr,
         "usage: bspline_hip_check [options] fixed.mha moving.mha\n"
+        "       bspline_hip_check --xf-roundtrip in.txt out.txt   (MGH_GPUIT_BSP read / write)\n"
         "  -M, --metric mse|mi          similarity metric (mse)\n"
         "  -f, --flavor g|p             MSE dc/dv: g = nearest-voxel gradient, p = exact (g)\n"
         "  -p, --process fwd|ctr|line|score|optimize   (score)\n"
@@ -95,6 +96,22 @@ read_mha (const std::string& fn, Volume** out, std::string* err)
 int
 main (int argc, char** argv)
 {
+    /* --xf-roundtrip in out: read an MGH_GPUIT_BSP file and write it back
+       (no GPU needed) */
+    if (argc == 4 && std::string (argv[1]) == "--xf-roundtrip") {
+        std::string err;
+        Bspline_xform* bxf = bspline_xform_load (argv[2], &err);
+        if (!bxf) {
+            fprintf (stderr, "%s\n", err.c_str ());
+            return 1;
+        }
+        printf ("img_dim %ld %ld %ld vox_per_rgn %ld %ld %ld num_coeff %ld\n", (long) bxf->img_dim[0],
+            (long) bxf->img_dim[1], (long) bxf->img_dim[2], (long) bxf->vox_per_rgn[0],
+            (long) bxf->vox_per_rgn[1], (long) bxf->vox_per_rgn[2], (long) bxf->num_coeff);
+        const bool ok = bxf->save (argv[3]);
+        delete bxf;
+        return ok ? 0 : 1;
+    }
     std::string metric = "mse", flavor = "g", process = "score", output, xform_fn;
     float step = 1e-4f, rlo = 0.f, rhi = 0.f;
     long index = 514;

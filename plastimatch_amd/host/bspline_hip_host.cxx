@@ -159,6 +159,94 @@ Bspline_xform::initialize (const float o[3], const float sp[3], const plm_long d
     coeff.assign ((size_t) num_coeff, 0.f);
 }
 
+bool
+Bspline_xform::save (const char* filename) const
+{
+    FILE* fp = fopen (filename, "wb");
+    if (!fp) return false;
+    fprintf (fp, "MGH_GPUIT_BSP <experimental>\n");
+    fprintf (fp, "img_origin = %f %f %f\n", img_origin[0], img_origin[1], img_origin[2]);
+    fprintf (fp, "img_spacing = %f %f %f\n", img_spacing[0], img_spacing[1], img_spacing[2]);
+    fprintf (fp, "img_dim = %u %u %u\n", (unsigned) img_dim[0], (unsigned) img_dim[1], (unsigned) img_dim[2]);
+    fprintf (fp, "roi_offset = %d %d %d\n", (int) roi_offset[0], (int) roi_offset[1], (int) roi_offset[2]);
+    fprintf (fp, "roi_dim = %d %d %d\n", (int) roi_dim[0], (int) roi_dim[1], (int) roi_dim[2]);
+    fprintf (fp, "vox_per_rgn = %d %d %d\n", (int) vox_per_rgn[0], (int) vox_per_rgn[1], (int) vox_per_rgn[2]);
+    fprintf (fp, "direction_cosines = %f %f %f %f %f %f %f %f %f\n", dc[0], dc[1], dc[2], dc[3], dc[4],
+        dc[5], dc[6], dc[7], dc[8]);
+    /* planar: every knot's x, then y, then z */
+    for (int d = 0; d < 3; d++)
+        for (plm_long k = 0; k < num_knots; k++)
+            fprintf (fp, "%.20f\n", coeff[(size_t) (3 * k + d)]);
+    return fclose (fp) == 0;
+}
+
+Bspline_xform*
+bspline_xform_load (const char* filename, std::string* err)
+{
+    float o[3] = {0.f, 0.f, 0.f}, sp[3] = {1.f, 1.f, 1.f};
+    float dcos[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+    plm_long dim[3] = {0, 0, 0}, roff[3] = {0, 0, 0}, rdim[3] = {0, 0, 0}, vpr[3] = {0, 0, 0};
+    FILE* fp = fopen (filename, "r");
+    if (!fp) {
+        if (err) *err = std::string ("cannot open ") + filename;
+        return nullptr;
+    }
+    auto fail = [&] (const std::string& m) -> Bspline_xform* {
+        fclose (fp);
+        if (err) *err = std::string (filename) + ": " + m;
+        return nullptr;
+    };
+    char line[1024];
+    if (!fgets (line, sizeof line, fp) || strncmp (line, "MGH_GPUIT_BSP", 13) != 0)
+        return fail ("not an MGH_GPUIT_BSP file");
+    /* header lines until the first line without '=' (the first coefficient) */
+    for (;;) {
+        if (!fgets (line, sizeof line, fp)) return fail ("header ends without coefficients");
+        if (!strchr (line, '=')) break;
+        unsigned a, b, c;
+        if (sscanf (line, "img_origin = %f %f %f", &o[0], &o[1], &o[2]) == 3) continue;
+        if (sscanf (line, "img_spacing = %f %f %f", &sp[0], &sp[1], &sp[2]) == 3) continue;
+        if (sscanf (line, "img_dim = %u %u %u", &a, &b, &c) == 3) {
+            dim[0] = a; dim[1] = b; dim[2] = c;
+            continue;
+        }
+        if (sscanf (line, "roi_offset = %u %u %u", &a, &b, &c) == 3) {
+            roff[0] = a; roff[1] = b; roff[2] = c;
+            continue;
+        }
+        if (sscanf (line, "roi_dim = %u %u %u", &a, &b, &c) == 3) {
+            rdim[0] = a; rdim[1] = b; rdim[2] = c;
+            continue;
+        }
+        if (sscanf (line, "vox_per_rgn = %u %u %u", &a, &b, &c) == 3) {
+            vpr[0] = a; vpr[1] = b; vpr[2] = c;
+            continue;
+        }
+        if (sscanf (line, "direction_cosines = %f %f %f %f %f %f %f %f %f", &dcos[0], &dcos[1], &dcos[2],
+                &dcos[3], &dcos[4], &dcos[5], &dcos[6], &dcos[7], &dcos[8]) == 9)
+            continue;
+        return fail (std::string ("bad header line: ") + line);
+    }
+    for (int a = 0; a < 3; a++)
+        if (vpr[a] < 1 || rdim[a] < 1) return fail ("no coefficients");
+    Bspline_xform* bxf = new Bspline_xform;
+    bxf->initialize (o, sp, dim, roff, rdim, vpr, dcos);
+    for (int d = 0; d < 3; d++) {
+        for (plm_long k = 0; k < bxf->num_knots; k++) {
+            if ((d != 0 || k != 0) && !fgets (line, sizeof line, fp)) {
+                delete bxf;
+                return fail ("too few coefficients");
+            }
+            if (sscanf (line, "%f", &bxf->coeff[(size_t) (3 * k + d)]) != 1) {
+                delete bxf;
+                return fail ("bad coefficient line");
+            }
+        }
+    }
+    fclose (fp);
+    return bxf;
+}
+
 void
 Bspline_xform::fill_coefficients (float val)
 {

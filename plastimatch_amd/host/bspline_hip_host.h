@@ -76,7 +76,15 @@ public:
     void fill_coefficients (float val);
     void jitter_if_zero ();
     plm_bsp_geom geom () const;
+    /* MGH_GPUIT_BSP text file (Bspline_xform::save,
+       src/base/bspline_xform.cxx:101-145); false if it cannot be opened. */
+    bool save (const char* filename) const;
 };
+
+/* Read an MGH_GPUIT_BSP file (bspline_xform_load,
+   src/base/bspline_xform.cxx:147-279); nullptr and a message in *err on a
+   file the reference would also refuse. */
+Bspline_xform* bspline_xform_load (const char* filename, std::string* err);
 
 class Histogram {
 public:

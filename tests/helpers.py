@@ -53,3 +53,40 @@ def assert_parity(score, grad, ref, ref64, tol_s=1e-5, tol_g=1e-4, label=""):
     assert rel_scalar(score, s_ref) < bar_s
     assert eg < bar_g, (eg, bar_g)
     assert rel(grad, g_ref) < bar_g, (rel(grad, g_ref), bar_g)
+
+
+def reference_synth(pattern, dim=38, volume_size=50.0, origin=-25.0, background=None, foreground=None,
+                    center=0.0, width=10.0):
+    """The reference's synthetic test images ("plastimatch synth" as the
+    reference's tests call it) on its geometry: --origin given, spacing =
+    volume_size / dim in float (src/cli/pcmd_synth.cxx:338-357), voxel
+    positions origin + spacing * index rounded to float (ITK's
+    TransformIndexToPhysicalPoint into a float point).  pattern "gauss":
+    synth_gauss in double, stored as float (src/util/synthetic_mha.cxx:
+    227-245); "sphere": synth_sphere in float, foreground where the
+    normalized radius is <= 1, stored as uchar values (:304-328).
+    Returns (image zyx, spacing, origin)."""
+    sp = np.float32(volume_size) / np.float32(dim)
+    o = np.float32(origin)
+    phys = (np.float64(o) + np.float64(sp) * np.arange(dim, dtype=np.float64)).astype(np.float32)
+    z, y, x = np.meshgrid(phys, phys, phys, indexing="ij")
+    if pattern == "gauss":
+        bg = -1000.0 if background is None else background
+        fg = 0.0 if foreground is None else foreground
+        f = np.zeros(z.shape, np.float64)
+        for p in (x, y, z):
+            f1 = (p.astype(np.float64) - np.float64(np.float32(center))) / np.float64(np.float32(width))
+            f += f1 * f1
+        f = np.exp(-0.5 * f)
+        img = ((1 - f) * np.float64(np.float32(bg)) + f * np.float64(np.float32(fg))).astype(np.float32)
+    elif pattern == "sphere":
+        bg = 0.0 if background is None else background
+        fg = 255.0 if foreground is None else foreground
+        f = np.zeros(z.shape, np.float32)
+        for p in (x, y, z):
+            f1 = (p - np.float32(center)) / np.float32(width)
+            f = (f + f1 * f1).astype(np.float32)
+        img = np.where(f.astype(np.float64) > 1.0, np.float32(bg), np.float32(fg)).astype(np.float32)
+    else:
+        raise ValueError(pattern)
+    return img, (float(sp),) * 3, (float(o),) * 3

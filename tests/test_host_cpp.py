@@ -89,3 +89,29 @@ def test_host_mi_and_optimizer(tmp_path):
     rc, out, err = run_tool(["-g", "5", "-p", "optimize", "--iterations", "5"], str(tmp_path), pair)
     assert rc == 0, (out, err)
     assert out["final_score"] < out["initial_score"]
+
+
+def test_host_xform_file_round_trip(tmp_path):
+    """The C++ MGH_GPUIT_BSP reader / writer on the reference's own files
+    (Testing/Data/xf-bspline-*.txt): the writer reproduces a file the
+    reference wrote byte for byte; an older file without a direction-cosines
+    line reads with the identity."""
+    subprocess.run(["make", "-s", "-C", HOST], check=True)
+    out = tmp_path / "xf2.txt"
+    r = subprocess.run([TOOL, "--xf-roundtrip", os.path.join(ROOT, "tests/golden/xf-bspline-2.txt"), str(out)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "num_coeff 648" in r.stdout
+    assert out.read_bytes() == open(os.path.join(ROOT, "tests/golden/xf-bspline-2.txt"), "rb").read()
+    out1 = tmp_path / "xf1.txt"
+    r = subprocess.run([TOOL, "--xf-roundtrip", os.path.join(ROOT, "tests/golden/xf-bspline-1.txt"), str(out1)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "num_coeff 192" in r.stdout
+    lines = out1.read_text().splitlines()
+    assert lines[7] == "direction_cosines = 1.000000 0.000000 0.000000 0.000000 1.000000 0.000000 0.000000 0.000000 1.000000"
+    assert lines[8:] == open(os.path.join(ROOT, "tests/golden/xf-bspline-1.txt")).read().splitlines()[7:]
+    bad = tmp_path / "bad.txt"
+    bad.write_text("MGH_GPUIT_BSP <experimental>\nimg_dim = 4 4 4\nnonsense = 1\n0.0\n")
+    r = subprocess.run([TOOL, "--xf-roundtrip", str(bad), str(tmp_path / "x.txt")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "bad header line" in r.stderr
