@@ -29,6 +29,7 @@ usage ()
     fprintf (stderr,
         "usage: bspline_hip_check [options] fixed.mha moving.mha\n"
         "       bspline_hip_check --xf-roundtrip in.txt out.txt   (MGH_GPUIT_BSP read / write)\n"
+        "       bspline_hip_check --mha-roundtrip in.mha out.mha  (MetaImage read / write as float)\n"
         "  -M, --metric mse|mi          similarity metric (mse)\n"
         "  -f, --flavor g|p             MSE dc/dv: g = nearest-voxel gradient, p = exact (g)\n"
         "  -p, --process fwd|ctr|line|score|optimize   (score)\n"
@@ -46,48 +47,17 @@ usage ()
         "  -O, --output file            write the gradient (float64 raw)\n");
 }
 
-/* Minimal MetaImage reader: NDims 3, ElementType MET_FLOAT,
-   ElementDataFile LOCAL or a raw file next to the header. */
+/* The input images: read_mha of the host library (the reference's
+   MetaImage reading, widened to float). */
 static bool
-read_mha (const std::string& fn, Volume** out, std::string* err)
+read_input (const std::string& fn, Volume** out, std::string* err)
 {
-    std::ifstream f (fn, std::ios::binary);
-    if (!f) { *err = "cannot open " + fn; return false; }
-    plm_long dim[3] = {0, 0, 0};
-    float sp[3] = {1, 1, 1}, org[3] = {0, 0, 0};
-    float dc[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    std::string etype, datafile;
-    int planes = 1;
-    std::string line;
-    while (std::getline (f, line)) {
-        size_t eq = line.find ('=');
-        if (eq == std::string::npos) continue;
-        std::string key = line.substr (0, eq), val = line.substr (eq + 1);
-        while (!key.empty () && key.back () == ' ') key.pop_back ();
-        std::istringstream vs (val);
-        if (key == "DimSize") vs >> dim[0] >> dim[1] >> dim[2];
-        else if (key == "ElementSpacing") vs >> sp[0] >> sp[1] >> sp[2];
-        else if (key == "Offset" || key == "Origin" || key == "Position") vs >> org[0] >> org[1] >> org[2];
-        else if (key == "TransformMatrix" || key == "Orientation" || key == "Rotation")
-            for (int i = 0; i < 9; i++) vs >> dc[i];
-        else if (key == "ElementType") vs >> etype;
-        else if (key == "ElementNumberOfChannels") vs >> planes;
-        else if (key == "ElementDataFile") {
-            vs >> datafile;
-            break;
-        }
-    }
-    if (etype != "MET_FLOAT") { *err = fn + ": only MET_FLOAT is read"; return false; }
-    if (dim[0] < 2 || dim[1] < 2 || dim[2] < 2) { *err = fn + ": bad DimSize"; return false; }
-    Volume* v = new Volume (dim, org, sp, dc, planes);
-    size_t bytes = sizeof (float) * v->img.size ();
-    if (datafile == "LOCAL") {
-        f.read ((char*) v->img.data (), bytes);
-    } else {
-        std::string dir = fn.substr (0, fn.rfind ('/') + 1);
-        std::ifstream r (dir + datafile, std::ios::binary);
-        r.read ((char*) v->img.data (), bytes);
-        if (!r) { delete v; *err = "cannot read " + datafile; return false; }
+    Volume* v = read_mha (fn, err);
+    if (!v) return false;
+    if (v->vox_planes != 1 || v->dim[0] < 2 || v->dim[1] < 2 || v->dim[2] < 2) {
+        *err = fn + ": need a scalar image with every dim >= 2";
+        delete v;
+        return false;
     }
     *out = v;
     return true;
@@ -96,6 +66,24 @@ read_mha (const std::string& fn, Volume** out, std::string* err)
 int
 main (int argc, char** argv)
 {
+    /* --mha-roundtrip in out: read a MetaImage and write it back as float
+       (no GPU needed) */
+    if (argc == 4 && std::string (argv[1]) == "--mha-roundtrip") {
+        std::string err;
+        Volume* v = read_mha (argv[2], &err);
+        if (!v) {
+            fprintf (stderr, "%s\n", err.c_str ());
+            return 1;
+        }
+        double sum = 0.0;
+        for (float f : v->img) sum += f;
+        printf ("dim %ld %ld %ld planes %d sum %.6f\n", (long) v->dim[0], (long) v->dim[1], (long) v->dim[2],
+            v->vox_planes, sum);
+        const bool ok = write_mha (argv[3], *v, &err);
+        delete v;
+        if (!ok) fprintf (stderr, "%s\n", err.c_str ());
+        return ok ? 0 : 1;
+    }
     /* --xf-roundtrip in out: read an MGH_GPUIT_BSP file and write it back
        (no GPU needed) */
     if (argc == 4 && std::string (argv[1]) == "--xf-roundtrip") {
@@ -148,7 +136,7 @@ main (int argc, char** argv)
 
     std::string err;
     Volume *fixed = nullptr, *moving = nullptr;
-    if (!read_mha (files[0], &fixed, &err) || !read_mha (files[1], &moving, &err)) {
+    if (!read_input (files[0], &fixed, &err) || !read_input (files[1], &moving, &err)) {
         printf ("{\"status\": %d, \"error\": \"%s\"}\n", PLM_HIP_ERR_ARG, err.c_str ());
         return 1;
     }

@@ -1,6 +1,7 @@
 /* C++ host side of the threading=hip B-spline flavor; see bspline_hip_host.h. */
 #include "bspline_hip_host.h"
 
+#include <algorithm>
 #include <dlfcn.h>
 #include <float.h>
 #include <math.h>
@@ -157,6 +158,132 @@ Bspline_xform::initialize (const float o[3], const float sp[3], const plm_long d
     num_knots = cdims[0] * cdims[1] * cdims[2];
     num_coeff = 3 * num_knots;
     coeff.assign ((size_t) num_coeff, 0.f);
+}
+
+Volume*
+read_mha (const std::string& fn, std::string* err)
+{
+    FILE* fp = fopen (fn.c_str (), "rb");
+    if (!fp) {
+        if (err) *err = "cannot open " + fn;
+        return nullptr;
+    }
+    auto fail = [&] (const std::string& m) -> Volume* {
+        fclose (fp);
+        if (err) *err = fn + ": " + m;
+        return nullptr;
+    };
+    plm_long dim[3] = {0, 0, 0};
+    float sp[3] = {1.f, 1.f, 1.f}, org[3] = {0.f, 0.f, 0.f};
+    float dc[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+    int channels = 1, esize = 0;
+    char etype = 0;                     // 'f', 's', 'u'
+    bool big = false;
+    std::string datafile;
+    char line[512];
+    for (;;) {
+        if (!fgets (line, sizeof line, fp)) return fail ("no ElementDataFile line");
+        size_t len = strlen (line);
+        while (len > 0 && (line[len - 1] == '\n' || line[len - 1] == '\r' || line[len - 1] == ' '))
+            line[--len] = 0;
+        if (strncmp (line, "ElementDataFile = ", 18) == 0) {
+            datafile = line + 18;
+            break;
+        }
+        long a, b, c;
+        if (sscanf (line, "DimSize = %ld %ld %ld", &a, &b, &c) == 3) {
+            dim[0] = a; dim[1] = b; dim[2] = c;
+        } else if (sscanf (line, "Offset = %g %g %g", &org[0], &org[1], &org[2]) == 3) {
+        } else if (sscanf (line, "ElementSpacing = %g %g %g", &sp[0], &sp[1], &sp[2]) == 3) {
+        } else if (sscanf (line, "TransformMatrix = %g %g %g %g %g %g %g %g %g", &dc[0], &dc[3], &dc[6], &dc[1],
+                       &dc[4], &dc[7], &dc[2], &dc[5], &dc[8]) == 9) {
+        } else if (sscanf (line, "ElementNumberOfChannels = %d", &channels) == 1) {
+        } else if (strcmp (line, "ElementType = MET_FLOAT") == 0) {
+            etype = 'f'; esize = 4;
+        } else if (strcmp (line, "ElementType = MET_SHORT") == 0) {
+            etype = 's'; esize = 2;
+        } else if (strcmp (line, "ElementType = MET_UCHAR") == 0) {
+            etype = 'u'; esize = 1;
+        } else if (strcmp (line, "BinaryDataByteOrderMSB = True") == 0) {
+            big = true;
+        }
+    }
+    if (!etype) return fail ("cannot interpret the element type");
+    if (channels != 1 && (channels != 3 || etype != 'f')) return fail ("only float3 vector fields are read");
+    if (dim[0] < 1 || dim[1] < 1 || dim[2] < 1) return fail ("bad DimSize");
+    Volume* v = new Volume (dim, org, sp, dc, channels);
+    const size_t n = v->img.size ();
+    std::vector<unsigned char> raw (n * (size_t) esize);
+    size_t got;
+    if (datafile == "LOCAL") {
+        got = fread (raw.data (), 1, raw.size (), fp);
+    } else {
+        const std::string dir = fn.substr (0, fn.rfind ('/') + 1);
+        FILE* r = fopen ((dir + datafile).c_str (), "rb");
+        if (!r) {
+            delete v;
+            return fail ("cannot open " + datafile);
+        }
+        got = fread (raw.data (), 1, raw.size (), r);
+        fclose (r);
+    }
+    fclose (fp);
+    if (got != raw.size ()) {
+        delete v;
+        if (err) *err = fn + ": short data";
+        return nullptr;
+    }
+    for (size_t i = 0; i < n; i++) {
+        unsigned char* p = raw.data () + i * esize;
+        if (big) std::reverse (p, p + esize);   // data are little-endian on this host
+        if (etype == 'f') {
+            float f;
+            memcpy (&f, p, 4);
+            v->img[i] = f;
+        } else if (etype == 's') {
+            int16_t s16;
+            memcpy (&s16, p, 2);
+            v->img[i] = (float) s16;
+        } else {
+            v->img[i] = (float) *p;
+        }
+    }
+    return v;
+}
+
+bool
+write_mha (const std::string& fn, const Volume& vol, std::string* err)
+{
+    FILE* fp = fopen (fn.c_str (), "wb");
+    if (!fp) {
+        if (err) *err = "cannot open " + fn;
+        return false;
+    }
+    const float* dc = vol.direction_cosines;
+    fprintf (fp,
+        "ObjectType = Image\n"
+        "NDims = 3\n"
+        "BinaryData = True\n"
+        "BinaryDataByteOrderMSB = False\n"
+        "TransformMatrix = %g %g %g %g %g %g %g %g %g\n"
+        "Offset = %g %g %g\n"
+        "CenterOfRotation = 0 0 0\n"
+        "ElementSpacing = %g %g %g\n"
+        "DimSize = %d %d %d\n"
+        "AnatomicalOrientation = RAI\n"
+        "%s"
+        "ElementType = MET_FLOAT\n"
+        "ElementDataFile = LOCAL\n",
+        dc[0], dc[3], dc[6], dc[1], dc[4], dc[7], dc[2], dc[5], dc[8], vol.origin[0], vol.origin[1],
+        vol.origin[2], vol.spacing[0], vol.spacing[1], vol.spacing[2], (int) vol.dim[0], (int) vol.dim[1],
+        (int) vol.dim[2], vol.vox_planes == 3 ? "ElementNumberOfChannels = 3\n" : "");
+    const size_t n = vol.img.size ();
+    const bool ok = fwrite (vol.img.data (), sizeof (float), n, fp) == n;
+    if (fclose (fp) != 0 || !ok) {
+        if (err) *err = "cannot write " + fn;
+        return false;
+    }
+    return true;
 }
 
 bool

@@ -81,6 +81,16 @@ public:
     bool save (const char* filename) const;
 };
 
+/* MetaImage files as the reference reads and writes them (read_mha /
+   write_mha, src/base/mha_io.cxx:26-257): data after the header
+   (ElementDataFile = LOCAL, or a raw file next to it on read), MET_FLOAT,
+   MET_SHORT or MET_UCHAR (widened to float here: the path reads float),
+   float vector fields with ElementNumberOfChannels = 3, TransformMatrix by
+   columns, either byte order.  nullptr and a message in *err on failure. */
+Volume* read_mha (const std::string& filename, std::string* err);
+/* Float volume or float3 field with the reference's header. */
+bool write_mha (const std::string& filename, const Volume& vol, std::string* err);
+
 /* Read an MGH_GPUIT_BSP file (bspline_xform_load,
    src/base/bspline_xform.cxx:147-279); nullptr and a message in *err on a
    file the reference would also refuse. */

@@ -180,18 +180,10 @@ def random_coeff(dim, vox_per_rgn, sigma, seed):
 
 
 def write_mha(path, img, spacing, origin=(0.0, 0.0, 0.0), direction_cosines=IDENTITY):
-    """Uncompressed float MetaImage (.mha, data LOCAL), the format the
-    reference's fixtures use (src/base/mha_io.cxx)."""
-    img = np.ascontiguousarray(img, np.float32)
-    nz, ny, nx = img.shape[:3]
-    hdr = ("ObjectType = Image\nNDims = 3\nBinaryData = True\nBinaryDataByteOrderMSB = False\n"
-           f"TransformMatrix = {' '.join(str(float(v)) for v in direction_cosines)}\n"
-           f"Offset = {origin[0]} {origin[1]} {origin[2]}\n"
-           f"ElementSpacing = {spacing[0]} {spacing[1]} {spacing[2]}\n"
-           f"DimSize = {nx} {ny} {nz}\nElementType = MET_FLOAT\nElementDataFile = LOCAL\n")
-    with open(path, "wb") as f:
-        f.write(hdr.encode())
-        f.write(img.tobytes())
+    """Uncompressed float MetaImage of a synthetic pair, with the geometry
+    written exactly (plastimatch_amd.mha.write_mha, precise)."""
+    from .mha import write_mha as _write
+    _write(path, np.ascontiguousarray(img, np.float32), spacing, origin, direction_cosines, precise=True)
 
 
 def eqsp_hist(img, bins):
