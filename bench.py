@@ -51,24 +51,46 @@ def alg_bytes(pair):
     return 20 * nvox + 4 * pair.true_coeff.size
 
 
-def cpu_baseline(pair, coeffs, gpu_results, budget_s=20.0):
-    """Time the oracle's flavor-g restatement (OpenMP) on a bounded sample of
-    the same workload, and report the GPU results' error against the FP64
-    oracle on the same coefficient vectors."""
-    import oracle
-    threads = min(16, len(os.sched_getaffinity(0)))
-    case = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn)
-    _ = case.m_grad
-    case.mse(coeffs[0], "g", nthreads=threads)          # warm-up
+def cpu_model():
+    """The host CPU's model name, as /proc/cpuinfo gives it."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def time_cpu_evals(case, coeffs, threads, budget_s, warmup=3, evals=10):
+    """Median seconds per evaluation of the oracle's flavor g: `warmup`
+    untimed evaluations, then up to `evals` timed ones (fewer if the budget
+    runs out)."""
+    for i in range(warmup):
+        case.mse(coeffs[i % len(coeffs)], "g", nthreads=threads)
     times = []
     t_start = time.perf_counter()
-    for i in range(10):
+    for i in range(evals):
         t0 = time.perf_counter()
         case.mse(coeffs[i % len(coeffs)], "g", nthreads=threads)
         times.append(time.perf_counter() - t0)
         if time.perf_counter() - t_start > budget_s:
             break
-    med = float(np.median(times))
+    return float(np.median(times)), len(times)
+
+
+def cpu_baseline(pair, coeffs, gpu_results, budget_s=12.0):
+    """Time the oracle's flavor-g restatement (OpenMP) on a bounded sample of
+    the same workload, with every host thread available (up to 16) and with
+    one, and report the GPU results' error against the FP64 oracle on the
+    same coefficient vectors."""
+    import oracle
+    threads = min(16, len(os.sched_getaffinity(0)))
+    case = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn)
+    _ = case.m_grad
+    med, n = time_cpu_evals(case, coeffs, threads, budget_s)
+    med1, n1 = time_cpu_evals(case, coeffs, 1, budget_s)
     errs = []
     for k, (score, grad) in gpu_results.items():
         s64, g64, _ = case.mse_f64(coeffs[k], 0, nthreads=threads)
@@ -78,9 +100,12 @@ def cpu_baseline(pair, coeffs, gpu_results, budget_s=20.0):
         "unit": "evals/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{len(times)} evals of the same workload after 1 warm-up, median "
+        "cpu": cpu_model(),
+        "sample": f"{n} evals of the same workload after 3 warm-ups, median "
                   f"{med * 1e3:.1f} ms/eval, oracle flavor g (OpenMP, float as the reference)",
         "gb_per_s": alg_bytes(pair) / med / 1e9,
+        "one_thread": {"value": 1.0 / med1, "unit": "evals/s", "cores": 1,
+                       "sample": f"{n1} evals after 3 warm-ups, median {med1 * 1e3:.1f} ms/eval"},
     }, errs
 
 

@@ -12,9 +12,12 @@ SOURCES = ["bspline_kernels.hip", "lbfgs_kernels.hip", "stage_kernels.hip", "bsp
 
 # -ffp-contract=off: the score path reproduces the reference's float rounding
 # operation by operation; FMAs are written explicitly where they are wanted.
-# The packed-FP32 feature is switched off for every kernel.
+# The packed-FP32 feature is switched off for every kernel.  The matrix-core
+# chains keep their accumulators in ordinary VGPRs (the AGPR form would add
+# to the unified register count and cost the tile kernel a wave per SIMD).
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-mfma-vgpr-form=1",
          "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
 

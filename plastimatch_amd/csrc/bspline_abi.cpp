@@ -181,6 +181,7 @@ struct plm_hip_ctx {
 
     void* d_fixed = nullptr;
     void* d_moving = nullptr;
+    float* d_mpair = nullptr;       // float storage: the moving image as z-pairs
     void* d_mgrad = nullptr;
     float* d_blut = nullptr;
     float* d_cond = nullptr;
@@ -230,7 +231,7 @@ void
 free_ctx (plm_hip_ctx* c)
 {
     c->timer.destroy ();
-    void* bufs[] = {c->d_fixed, c->d_moving, c->d_mgrad, c->d_blut, c->d_cond,
+    void* bufs[] = {c->d_fixed, c->d_moving, c->d_mpair, c->d_mgrad, c->d_blut, c->d_cond,
         c->d_tile_score, c->d_tile_count, c->d_pair, c->d_hist,
         c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
         c->d_hist_out, c->d_reg_part, c->d_reg_cond};
@@ -485,6 +486,12 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     CTX_TRY (upload_image (c, fixed, nf, &c->d_fixed));
     CTX_TRY (upload_image (c, moving, nm, &c->d_moving));
     if (moving_grad) CTX_TRY (upload_image (c, moving_grad, 3 * nm, &c->d_mgrad));
+    if (c->storage == PLM_STORE_F32) {
+        CTX_TRY (alloc (&c->d_mpair, 2 * (size_t) nm, "hipMalloc(moving pairs)"));
+        e = plmhip::launch_zpair ((const float*) c->d_moving, c->d_mpair, nm,
+            moving->dim[0] * moving->dim[1], c->stream);
+        if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "moving pairs"); }
+    }
 
     std::vector<float> blut (4 * (g.vpr[0] + g.vpr[1] + g.vpr[2]));
     basis_lut (g.vpr[0], &blut[0]);
@@ -530,6 +537,7 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
         pd.fimg = c->d_fixed;
         pd.mimg = c->d_moving;
         pd.mgrad = c->d_mgrad;
+        pd.mpair = c->d_mpair;
         pd.cond = c->d_cond;
         CTX_TRY (alloc (&c->d_pair, 1, "hipMalloc(pair)"));
         e = hipMemcpy (c->d_pair, &pd, sizeof (pd), hipMemcpyHostToDevice);
@@ -1022,6 +1030,7 @@ plm_hip_batch_create (plm_hip_ctx* const* ctxs, int n, plm_hip_batch** out)
         pd.fimg = c->d_fixed;
         pd.mimg = c->d_moving;
         pd.mgrad = c->d_mgrad;
+        pd.mpair = c->d_mpair;
         pd.coeff = c->d_coeff;
         pd.cond = c->d_cond;
         pd.grad = c->d_grad;

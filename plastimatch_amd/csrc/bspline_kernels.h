@@ -49,13 +49,23 @@ tile_slots (int nvox, int vox_per_thread)
     return (nvox + s - 1) / s * s;
 }
 
-// Floats of the tile-kernel LDS region shared by the displacement stages
-// (T1, column terms) and the condense stages (S1, S2).
+// Floats of the tile kernel's column-term planes, which the condense
+// reuses for the tile's dc/dv contracted over x and y.
 __host__ __device__ inline int
-tile_region1_floats (int vx, int vy, int vz)
+tile_col_floats (int vx, int vy, int vz)
 {
-    const int a = 64 * vx + 16 * vx * vy;
-    const int b = 12 * vy * vz + 48 * vz;
+    const int a = 16 * vx * vy;
+    const int b = 48 * vz;
+    return a > b ? a : b;
+}
+
+// Floats of the tile kernel's dc/dv planes (3 x slots), which the first
+// displacement stage (64 floats per x) uses before any dc/dv is written.
+__host__ __device__ inline int
+tile_dv_floats (int vx, int slots)
+{
+    const int a = 3 * slots;
+    const int b = 64 * vx;
     return a > b ? a : b;
 }
 
@@ -88,6 +98,7 @@ struct PairDev {
     const void* fimg;        // fixed, moving, moving gradient (the batch's storage type)
     const void* mimg;
     const void* mgrad;
+    const float* mpair;      // float storage: the moving image as z-pairs, or null
     const float* coeff;      // coefficients of the evaluation (batched launches)
     float* cond;             // slot-major condense buffer, 64 * 3 * num_knots
     double* grad;            // outputs (batched launches): 3 * num_knots
@@ -186,6 +197,10 @@ hipError_t launch_bin_image (const void* img, int storage, uint8_t* out, int64_t
     float offset, float delta, int bins, hipStream_t st);
 
 hipError_t launch_zero (float* p, int64_t n, hipStream_t st);
+// The moving image as z-pairs: out[2v] = m[v], out[2v + 1] = m[v + d01]
+// (0 on the last slice), so the 16 bytes at voxel v hold the trilinear
+// corners (x, z), (x, z + 1), (x + 1, z), (x + 1, z + 1) of one row.
+hipError_t launch_zpair (const float* m, float* out, int64_t n, int64_t d01, hipStream_t st);
 hipError_t launch_volume_gradient (const float* img, float* out, const int dim[3],
     const float sp[3], const Mat3& inv, hipStream_t st);
 hipError_t launch_to_bf16 (const float* src, uint16_t* dst, int64_t n, hipStream_t st);

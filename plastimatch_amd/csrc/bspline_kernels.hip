@@ -284,6 +284,16 @@ template <> struct Img<float> {
         o[1] = __uint_as_float (v[1]);
         o[2] = __uint_as_float (v[2]);
     }
+    // 16 bytes of the z-pair image: (x, z), (x, z + 1), (x + 1, z), (x + 1, z + 1)
+    static __device__ __forceinline__ void quad (float& a, float& b, float& c, float& d, Rsrc r,
+        int off, int soff)
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128 (r, off, soff, 0);
+        a = __uint_as_float (v[0]);
+        b = __uint_as_float (v[1]);
+        c = __uint_as_float (v[2]);
+        d = __uint_as_float (v[3]);
+    }
 };
 
 // BF16: a corner pair (4 bytes) or a gradient sample (6 bytes) starts on an
@@ -371,15 +381,17 @@ struct Shape {
 
 // LDS carve-up of the tile kernel, in floats; every array read as float4
 // starts on a 16-byte boundary (all offsets are multiples of 4 floats).
+// Arrays whose lifetimes within a tile do not overlap share memory: T1
+// (stages 2-3) lies under the dc/dv planes (stages 4-5), the condensed
+// S2 (stage 6) under the column terms (stages 3-4).
 struct TileLds {
     float* bx;      // [x][4] basis LUT along x
     float* by;      // [y][4]
     float* bzz;     // [z][8]: bz[0..3], Z[z][0..2], pad
     float* coef;    // [k][j][i][c]: the tile's 64 knots
-    float* t1;      // [k][j][x][4]: sum_i bx coef
     float* col;     // [4][y][x][4]: planes A[0][k], A[1][k], A[2][k], (C[a], pad)
-    float* s1;      // [z][y][i][c]: dc/dv condensed over x (reuses t1/col)
-    float* s2;      // [z][j][i][c]: then over y (reuses t1/col)
+    float* s2;      // [z][c][j][i]: dc/dv condensed over x and y (reuses col)
+    float* t1;      // [k][j][x][4]: sum_i bx coef (reuses dv)
     float* dv;      // [c][slot]: dc/dv of every voxel slot, one plane per component
     int dvp;        // plane stride (slots)
     double* red;    // block-sum scratch, 16 doubles
@@ -394,14 +406,13 @@ carve (float* sm, int vx, int vy, int vz, int slots)
     L.by = L.bx + 4 * vx;
     L.bzz = L.by + 4 * vy;
     L.coef = L.bzz + 8 * vz;
-    L.t1 = L.coef + 192;
-    L.col = L.t1 + 64 * vx;
-    L.s1 = L.t1;
-    L.s2 = L.s1 + 12 * vy * vz;
-    L.dv = L.t1 + tile_region1_floats (vx, vy, vz);
+    L.col = L.coef + 192;
+    L.s2 = L.col;
+    L.t1 = L.col + tile_col_floats (vx, vy, vz);
+    L.dv = L.t1;
     L.dvp = slots;
-    // slots is a multiple of 256, so red stays 16-byte aligned
-    L.red = (double*) (L.dv + 3 * slots);
+    // both sizes are multiples of 4 floats, so red stays 16-byte aligned
+    L.red = (double*) (L.dv + tile_dv_floats (vx, slots));
     L.tab = (float*) (L.red + 16);
     return L;
 }
@@ -588,100 +599,84 @@ tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int 
 // (src/register/bspline.cxx:377-398); slot-major keeps the reduce's reads
 // coalesced.
 //
+// The three contractions run on the matrix cores as chains of
+// v_mfma_f32_4x4x1_16b_f32: sixteen 4x4 blocks per instruction, block b
+// adding the outer product of lanes 4b..4b+3 of its two operands, D[r][q] +=
+// A[4b + r] * B[4b + q], the result in register r of lane 4b + q.  With one
+// term per instruction each element is a single fused multiply-add, so every
+// sum is the same FMA chain, in the same order, as the scalar form
+// acc = b0 v0, acc = fma (b_n, v_n, acc).
+//   S1 (over x): block = one (c, z) pair, row r = y - 4q for the q-th group
+//      of four rows, column = the x basis index i; A = dv, B = bx.
+//   S2 (over y): the S1 results stay in registers and feed the next chain
+//      directly: register r of lane (b, i) is S1[c][z][4q + r][i], exactly the
+//      A operand of row i of the same block; B = by.  Lane (b, j) ends with
+//      S2[c][z][j][i] in register i.
+//   S3 (over z): one wave, block = (c, j), row = i, column = k; A = S2 from
+//      LDS ([z][c][j][i]: consecutive per lane), B = bz.
+// Every operand read is clamped into its array, so lanes past the tile's
+// pairs or rows (whose results are never used) read in bounds.
+//
 // SCALE: multiply component c by g.inv_msp[c] on the way out (the analytic
 // derivative's 1/spacing, left out per voxel).
+typedef float mfma_f4 __attribute__ ((ext_vector_type (4)));
+
+__device__ __forceinline__ mfma_f4
+mfma_4x4x1 (float a, float b, mfma_f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_4x4x1f32 (a, b, c, 0, 0, 0);
+}
+
 template <int V, bool SCALE>
 __device__ __forceinline__ void
 tile_condense (const TileLds& L, const Shape<V>& S, const KGeom& g, int knot0,
     float* __restrict__ cond, int t)
 {
     const int vx = S.vx, vy = S.vy, vz = S.vz;
-    // S1[z][y][i][c] = sum_x bx[x][i] dv[z][y][x][c], one (z, y) row per thread
-    for (int e = t; e < vy * vz; e += TILE_THREADS) {
-        const float* d0 = L.dv + e * vx;
-        const float* d1 = d0 + L.dvp;
-        const float* d2 = d1 + L.dvp;
-        const float4* bx = reinterpret_cast<const float4*> (L.bx);
-        float acc[4][3];
-        {
-            const float v0 = d0[0], v1 = d1[0], v2 = d2[0];
-            const float4 b = bx[0];
-            acc[0][0] = b.x * v0; acc[0][1] = b.x * v1; acc[0][2] = b.x * v2;
-            acc[1][0] = b.y * v0; acc[1][1] = b.y * v1; acc[1][2] = b.y * v2;
-            acc[2][0] = b.z * v0; acc[2][1] = b.z * v1; acc[2][2] = b.z * v2;
-            acc[3][0] = b.w * v0; acc[3][1] = b.w * v1; acc[3][2] = b.w * v2;
-        }
-#pragma unroll 3
-        for (int x = 1; x < vx; x++) {
-            const float v0 = d0[x], v1 = d1[x], v2 = d2[x];
-            const float4 b = bx[x];
-            const float bb[4] = {b.x, b.y, b.z, b.w};
+    const int lane = t & 63, w = t >> 6, q4 = lane & 3;
+    // S1 and S2: sixteen (c, z) pairs per wave, pair index c * vz + z
+    const int npairs = 3 * vz;
+    for (int h = w; 16 * h < npairs; h += TILE_THREADS / 64) {
+        const int pr = min (16 * h + (lane >> 2), npairs - 1);
+        const int c = pr >= 2 * vz ? 2 : (pr >= vz ? 1 : 0);
+        const int z = pr - c * vz;
+        const float* dpair = L.dv + c * L.dvp + z * vy * vx;
+        mfma_f4 s2 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                acc[i][0] = fmaf (bb[i], v0, acc[i][0]);
-                acc[i][1] = fmaf (bb[i], v1, acc[i][1]);
-                acc[i][2] = fmaf (bb[i], v2, acc[i][2]);
-            }
+        for (int q = 0; q < (vy + 3) / 4; q++) {
+            const float* drow = dpair + min (4 * q + q4, vy - 1) * vx;
+            mfma_f4 s1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int x = 0; x < vx; x++) s1 = mfma_4x4x1 (drow[x], L.bx[4 * x + q4], s1);
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (4 * q + r < vy) s2 = mfma_4x4x1 (s1[r], L.by[4 * (4 * q + r) + q4], s2);
         }
-        float4* o = reinterpret_cast<float4*> (&L.s1[12 * e]);
-        o[0] = make_float4 (acc[0][0], acc[0][1], acc[0][2], acc[1][0]);
-        o[1] = make_float4 (acc[1][1], acc[1][2], acc[2][0], acc[2][1]);
-        o[2] = make_float4 (acc[2][2], acc[3][0], acc[3][1], acc[3][2]);
+        if (16 * h + (lane >> 2) < npairs)
+            reinterpret_cast<float4*> (L.s2)[(z * 3 + c) * 4 + q4] = make_float4 (s2[0], s2[1], s2[2], s2[3]);
     }
     __syncthreads ();
-    // S2[z][j][i][c] = sum_y by[y][j] S1[z][y][i][c], one (z, i, c) per thread
-    for (int e = t; e < 12 * vz; e += TILE_THREADS) {
-        const int z = e / 12, ic = e - 12 * z;
-        const float* s = &L.s1[z * vy * 12 + ic];
-        const float4* by = reinterpret_cast<const float4*> (L.by);
-        float a0, a1, a2, a3;
-        {
-            const float v = s[0];
-            const float4 b = by[0];
-            a0 = b.x * v; a1 = b.y * v; a2 = b.z * v; a3 = b.w * v;
-        }
-        for (int y = 1; y < vy; y++) {
-            const float v = s[12 * y];
-            const float4 b = by[y];
-            a0 = fmaf (b.x, v, a0);
-            a1 = fmaf (b.y, v, a1);
-            a2 = fmaf (b.z, v, a2);
-            a3 = fmaf (b.w, v, a3);
-        }
-        float* o = &L.s2[z * 48 + ic];
-        o[0] = a0; o[12] = a1; o[24] = a2; o[36] = a3;
-    }
-    __syncthreads ();
-    // S3: sum over z into the 4 slots along k, one (j, i, c) per thread
-    if (t < 48) {
-        const int jic = t;                      // (j*4 + i)*3 + c
-        const int ji = jic / 3, c = jic - 3 * ji, i = ji & 3, j = ji >> 2;
-        float a[4];
-        {
-            const float v = L.s2[jic];
-            const float4 b = reinterpret_cast<const float4*> (L.bzz)[0];
-            a[0] = b.x * v; a[1] = b.y * v; a[2] = b.z * v; a[3] = b.w * v;
-        }
-        for (int z = 1; z < vz; z++) {
-            const float v = L.s2[z * 48 + jic];
-            const float4 b = reinterpret_cast<const float4*> (L.bzz)[2 * z];
-            a[0] = fmaf (b.x, v, a[0]);
-            a[1] = fmaf (b.y, v, a[1]);
-            a[2] = fmaf (b.z, v, a[2]);
-            a[3] = fmaf (b.w, v, a[3]);
-        }
-        const int cd0 = g.cdims[0], cd01 = g.cdims[0] * g.cdims[1];
-        const int knot = knot0 + i + cd0 * j;
-        // 32-bit byte offsets: the buffer holds 192 * num_knots floats,
-        // below 2^31 bytes (checked at context creation)
-        const uint32_t plane = 12u * (uint32_t) g.num_knots;
-        const Rsrc cr = make_rsrc (cond, 64u * plane);
-        const float sc = SCALE ? g.inv_msp[c] : 1.f;
-        const uint32_t off = (uint32_t) ji * plane + 4u * (uint32_t) (3 * knot + c);
+    // S3 on the last wave: lane (c, j, k) = 16 c + 4 j + k, register i
+    if (w == TILE_THREADS / 64 - 1) {
+        const int row = min (lane, 47);
+        mfma_f4 s3 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-            __builtin_amdgcn_raw_buffer_store_b32 (__float_as_uint (SCALE ? a[k] * sc : a[k]), cr,
-                (int) (off + (uint32_t) k * (16u * plane + 12u * (uint32_t) cd01)), 0, 0);
+        for (int z = 0; z < vz; z++) s3 = mfma_4x4x1 (L.s2[48 * z + row], L.bzz[8 * z + q4], s3);
+        if (lane < 48) {
+            const int c = lane >> 4, j = (lane >> 2) & 3, k = q4;
+            const int cd0 = g.cdims[0], cd01 = g.cdims[0] * g.cdims[1];
+            // 32-bit byte offsets: the buffer holds 192 * num_knots floats,
+            // below 2^31 bytes (checked at context creation)
+            const uint32_t plane = 12u * (uint32_t) g.num_knots;
+            const Rsrc cr = make_rsrc (cond, 64u * plane);
+            const float sc = SCALE ? g.inv_msp[c] : 1.f;
+            const uint32_t off = (uint32_t) (16 * k + 4 * j) * plane
+                + 4u * (uint32_t) (3 * (knot0 + cd0 * j + cd01 * k) + c);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                __builtin_amdgcn_raw_buffer_store_b32 (__float_as_uint (SCALE ? s3[i] * sc : s3[i]), cr,
+                    (int) (off + (uint32_t) i * (plane + 12u)), 0, 0);
+        }
     }
 }
 
@@ -762,8 +757,8 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
     const int npairs = BATCH ? q.npairs : 1;
 
     extern __shared__ float4 sm4[];
-    __shared__ double s_ps[MAX_BATCH];      // per-pair sums of this workgroup
-    __shared__ int s_pn[MAX_BATCH];
+    __shared__ double s_ps[BATCH ? MAX_BATCH : 1];  // per-pair sums of this workgroup
+    __shared__ int s_pn[BATCH ? MAX_BATCH : 1];
     const int t = threadIdx.x;
     if (SCORE && t < npairs) {
         s_ps[t] = 0.0;
@@ -814,6 +809,17 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
     int np = p;                             // pair of the tile fetched ahead
     TilePos nq = tile_pos (pairs[p].g, tile - pairs[p].tile0);
     float cpre[3] = {0.f, 0.f, 0.f};
+    // Float score kinds with compile-time tile edges stage each tile's fixed
+    // voxels in LDS: the threads with stage index s < vy * vz load row s of
+    // the tile (y = s mod vy, z = s / vy) with ceil(vx / 4) 16-byte loads, a
+    // tile ahead, and write it into the third dc/dv plane at slot s * vx + x.
+    // That plane holds nothing before the voxel pass, where each thread
+    // reads its voxels' values there before writing their dc/dv over them;
+    // a 16-byte load costs the vector memory pipeline no more than a 4-byte
+    // one.
+    constexpr bool STAGE = SCORE && !VF_IN && V > 0 && std::is_same<TI, float>::value;
+    constexpr int NQ = (V + 3) / 4;
+    float frow[STAGE ? 4 * NQ : 1];
     float fpre[G];
     int fbpre[G];
     float dpre[G][3];
@@ -829,14 +835,35 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         for (int u = 0; u < G; u++) {
             const int x = sxyz[u] & 1023, y = (sxyz[u] >> 10) & 1023, z = sxyz[u] >> 20;
             const int fo = fb + x + (int) __umul24 (d0, y) + (int) __umul24 (d01, z);
-            if (SCORE) fpre[u] = Img<TI>::one (fr, fo * ES, 0);
+            if (SCORE && !STAGE) fpre[u] = Img<TI>::one (fr, fo * ES, 0);
             if (MI) fbpre[u] = load_u8 (fr, fo, 0);
             if (VF_IN) Img<float>::vec3 (dpre[u], vr, fo * 12, 0);
         }
     };
+    // row s = z * vy + y of tile position qq of pair pp, into frow, for
+    // the threads with stage index s < vy * vz
+    auto fetch_rows = [&] (int pp, const TilePos& qq, int s) {
+        if (!STAGE || s >= vy * vz) return;
+        const KGeom& gg = pairs[pp].g;
+        const int d0 = gg.fdim[0], d01 = gg.fdim[0] * gg.fdim[1];
+        const Rsrc fr = make_rsrc (pairs[pp].fimg, (uint32_t) (d01 * gg.fdim[2]) * 4u);
+        const int z = S.div_y (s), y = s - z * vy;
+        const int off = 4 * (qq.bx0 + d0 * (qq.by0 + y) + d01 * (qq.bz0 + z));
+#pragma unroll
+        for (int k = 0; k < NQ; k++)
+            Img<float>::quad (frow[4 * k], frow[4 * k + 1], frow[4 * k + 2], frow[4 * k + 3], fr,
+                off + 16 * k, 0);
+    };
+    auto store_rows = [&] (int s) {
+        if (!STAGE || s >= vy * vz) return;
+        float* o = L.dv + 2 * L.dvp + s * vx;
+#pragma unroll
+        for (int x = 0; x < vx; x++) o[x] = frow[x];
+    };
     if (MODE != OFFSETS && t < 64)
         tile_coef_fetch (cpre, pairs[p].g, nq.knot0, t, (coeff0 && p == 0) ? coeff0 : pairs[p].coeff);
     fetch_first (p, nq);
+    fetch_rows (p, nq, t);
 
     // this thread's squared differences and its wave's overlap count over
     // the tiles of the current pair, folded into s_ps/s_pn when the pair
@@ -882,7 +909,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         const int tr = (tt + rot) & (TILE_THREADS - 1);
         rot += 64;
         const int tn = (tt + rot) & (TILE_THREADS - 1);
-        __syncthreads ();       // the previous tile is done with coef, t1, s1, s2, red
+        __syncthreads ();       // the previous tile is done with coef, t1, s2, red
         tile_stage1<V, MODE> (L, S, g, P.bz0, cpre, tr);
         __syncthreads ();
         if (MODE != OFFSETS && tn < 64 && has_next)
@@ -898,6 +925,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         // barrier (red is free here).
         constexpr bool FASTABLE = MODE == PROJECTED && !VF_OUT;
         const bool col_ok = tile_columns<V, MODE> (L, S, g, P.bx0, P.by0, P.bz0, tr);
+        store_rows (tr);
         int* wflag = reinterpret_cast<int*> (L.red + 12);
         if (FASTABLE) {
             const bool w = __all (col_ok);
@@ -929,6 +957,12 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
             KIND == 0 ? img_bytes ((uint32_t) (md01 * g.mdim[2]) * 3, ES) : 0u);
         const Rsrc vr = make_rsrc (vf, VF_IN ? nfix * 12u : 0u);
         const int SX = md0 * ES, SXY = md01 * ES;
+        // Float trilinear kinds read the corners from the z-pair image: two
+        // 16-byte loads a voxel instead of four 8-byte ones (the vector
+        // memory pipeline's cost goes by instructions, not bytes).
+        constexpr bool ZPAIR = std::is_same<TI, float>::value && !MI && !NEAREST;
+        const Rsrc zr = make_rsrc (pd.mpair, ZPAIR ? (uint32_t) (md01 * g.mdim[2]) * 8u : 0u);
+        const int SX8 = md0 * 8, SXY8 = md01 * 8;
         // The moving-index bounds, read once per tile into registers: left
         // in memory, they are reloaded per voxel (each reload a scalar-load
         // round trip).
@@ -958,11 +992,12 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
             for (int u = 0; u < G; u++) {
                 int x, y, z;
                 float dd[3] = {0.f, 0.f, 0.f};
+                if (STAGE) fval[u] = L.dv[2 * L.dvp + base + u * TILE_THREADS + tt];
                 if (first) {
                     x = sxyz[u] & 1023;
                     y = (sxyz[u] >> 10) & 1023;
                     z = sxyz[u] >> 20;
-                    if (SCORE) fval[u] = fpre[u];
+                    if (SCORE && !STAGE) fval[u] = fpre[u];
                     if (MI) fbv[u] = fbpre[u];
                     if (VF_IN) { dd[0] = dpre[u][0]; dd[1] = dpre[u][1]; dd[2] = dpre[u][2]; }
                 } else {
@@ -972,7 +1007,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     y = S.div_x (r);
                     x = r - y * vx;
                     const int fofs = fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
-                    if (SCORE) fval[u] = Img<TI>::one (fr, fofs * ES, 0);
+                    if (SCORE && !STAGE) fval[u] = Img<TI>::one (fr, fofs * ES, 0);
                     if (MI) fbv[u] = load_u8 (fr, fofs, 0);
                     if (VF_IN) Img<float>::vec3 (dd, vr, fofs * 12, 0);
                 }
@@ -1049,6 +1084,11 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     const int roff = (int) __umul24 (f[0] + c[0], ES) + (int) __umul24 (f[1] + c[1], SX)
                         + (int) __umul24 (f[2] + c[2], SXY);
                     nv[u] = Img<TI>::one (mr, roff, 0);
+                } else if (ZPAIR) {
+                    const int off = (int) __umul24 (f[0], 8) + (int) __umul24 (f[1], SX8)
+                        + (int) __umul24 (f[2], SXY8);
+                    Img<float>::quad (a[u][0], a[u][4], a[u][1], a[u][5], zr, off, 0);
+                    Img<float>::quad (a[u][2], a[u][6], a[u][3], a[u][7], zr, off, SX8);
                 } else if (!MI) {
                     const int off = (int) __umul24 (f[0], ES) + (int) __umul24 (f[1], SX)
                         + (int) __umul24 (f[2], SXY);
@@ -1176,6 +1216,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         } else {
             voxels (std::false_type {});
         }
+        if (has_next) fetch_rows (np, nq, tn);
         if (GRAD) {
             __syncthreads ();
             tile_condense<V, DERIV> (L, S, g, P.knot0, pd.cond, tr);
@@ -1790,6 +1831,15 @@ k_bin_image (const TI* __restrict__ img, uint8_t* __restrict__ out, int64_t n, f
         out[i] = (uint8_t) hist_bin (ld (img, (int) i), offset, delta, bins);
 }
 
+// The moving image as z-pairs (see launch_zpair); once per stage.
+__global__ void
+k_zpair (const float* __restrict__ m, float2* __restrict__ out, int64_t n, int64_t d01)
+{
+    for (int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
+         i += (int64_t) gridDim.x * blockDim.x)
+        out[i] = make_float2 (m[i], i + d01 < n ? m[i + d01] : 0.f);
+}
+
 __global__ void
 k_zero_u64 (unsigned long long* __restrict__ p, int n)
 {
@@ -1829,8 +1879,8 @@ tile_lds_bytes (const KGeom& g, int vox_per_thread)
 {
     // the carve-up of the tile kernel
     const int vx = g.vpr[0], vy = g.vpr[1], vz = g.vpr[2];
-    const size_t f = 4 * vx + 4 * vy + 8 * vz + 192 + tile_region1_floats (vx, vy, vz)
-        + 3 * (size_t) tile_slots (vx * vy * vz, vox_per_thread);
+    const size_t f = 4 * vx + 4 * vy + 8 * vz + 192 + tile_col_floats (vx, vy, vz)
+        + tile_dv_floats (vx, tile_slots (vx * vy * vz, vox_per_thread));
     // + the per-pair partial sums, static in the kernel
     return f * sizeof (float) + 16 * sizeof (double) + MAX_BATCH * (sizeof (double) + sizeof (int));
 }
@@ -2060,6 +2110,14 @@ launch_bin_image (const void* img, int storage, uint8_t* out, int64_t n, float o
     else
         hipLaunchKernelGGL ((k_bin_image<uint16_t>), dim3 (1024), dim3 (256), 0, st,
             (const uint16_t*) img, out, n, offset, delta, bins);
+    return hipGetLastError ();
+}
+
+hipError_t
+launch_zpair (const float* m, float* out, int64_t n, int64_t d01, hipStream_t st)
+{
+    hipLaunchKernelGGL (k_zpair, dim3 (1024), dim3 (256), 0, st, m, reinterpret_cast<float2*> (out), n,
+        d01);
     return hipGetLastError ();
 }
 
