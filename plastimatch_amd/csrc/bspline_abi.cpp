@@ -181,7 +181,7 @@ struct plm_hip_ctx {
 
     void* d_fixed = nullptr;
     void* d_moving = nullptr;
-    float* d_mpair = nullptr;       // float storage: the moving image as z-pairs
+    void* d_mpair = nullptr;        // the moving image as z-pairs (storage type)
     void* d_mgrad = nullptr;
     float* d_blut = nullptr;
     float* d_cond = nullptr;
@@ -486,9 +486,11 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     CTX_TRY (upload_image (c, fixed, nf, &c->d_fixed));
     CTX_TRY (upload_image (c, moving, nm, &c->d_moving));
     if (moving_grad) CTX_TRY (upload_image (c, moving_grad, 3 * nm, &c->d_mgrad));
-    if (c->storage == PLM_STORE_F32) {
-        CTX_TRY (alloc (&c->d_mpair, 2 * (size_t) nm, "hipMalloc(moving pairs)"));
-        e = plmhip::launch_zpair ((const float*) c->d_moving, c->d_mpair, nm,
+    {
+        // 2 values a voxel, in the storage type
+        const size_t bytes = 2 * (size_t) nm * (c->storage == PLM_STORE_F32 ? 4 : 2);
+        CTX_TRY (alloc ((uint8_t**) &c->d_mpair, bytes, "hipMalloc(moving pairs)"));
+        e = plmhip::launch_zpair (c->d_moving, c->storage == PLM_STORE_F32 ? 0 : 1, c->d_mpair, nm,
             moving->dim[0] * moving->dim[1], c->stream);
         if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "moving pairs"); }
     }

@@ -98,7 +98,7 @@ struct PairDev {
     const void* fimg;        // fixed, moving, moving gradient (the batch's storage type)
     const void* mimg;
     const void* mgrad;
-    const float* mpair;      // float storage: the moving image as z-pairs, or null
+    const void* mpair;       // the moving image as z-pairs (the batch's storage type)
     const float* coeff;      // coefficients of the evaluation (batched launches)
     float* cond;             // slot-major condense buffer, 64 * 3 * num_knots
     double* grad;            // outputs (batched launches): 3 * num_knots
@@ -198,9 +198,11 @@ hipError_t launch_bin_image (const void* img, int storage, uint8_t* out, int64_t
 
 hipError_t launch_zero (float* p, int64_t n, hipStream_t st);
 // The moving image as z-pairs: out[2v] = m[v], out[2v + 1] = m[v + d01]
-// (0 on the last slice), so the 16 bytes at voxel v hold the trilinear
-// corners (x, z), (x, z + 1), (x + 1, z), (x + 1, z + 1) of one row.
-hipError_t launch_zpair (const float* m, float* out, int64_t n, int64_t d01, hipStream_t st);
+// (0 on the last slice), so the two records at voxel v hold the trilinear
+// corners (x, z), (x, z + 1), (x + 1, z), (x + 1, z + 1) of one row: 16
+// bytes for float storage (storage 0), 8 for BF16 (storage 1).
+hipError_t launch_zpair (const void* m, int storage, void* out, int64_t n, int64_t d01,
+    hipStream_t st);
 hipError_t launch_volume_gradient (const float* img, float* out, const int dim[3],
     const float sp[3], const Mat3& inv, hipStream_t st);
 hipError_t launch_to_bf16 (const float* src, uint16_t* dst, int64_t n, hipStream_t st);

@@ -271,12 +271,6 @@ template <> struct Img<float> {
     {
         return __uint_as_float (__builtin_amdgcn_raw_buffer_load_b32 (r, off, soff, 0));
     }
-    static __device__ __forceinline__ void pair (float& a, float& b, Rsrc r, int off, int soff)
-    {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64 (r, off, soff, 0);
-        a = __uint_as_float (v[0]);
-        b = __uint_as_float (v[1]);
-    }
     static __device__ __forceinline__ void vec3 (float o[3], Rsrc r, int off, int soff)
     {
         const auto v = __builtin_amdgcn_raw_buffer_load_b96 (r, off, soff, 0);
@@ -296,11 +290,11 @@ template <> struct Img<float> {
     }
 };
 
-// BF16: a corner pair (4 bytes) or a gradient sample (6 bytes) starts on an
-// even byte; one 64-bit load from the dword boundary at or below it always
-// covers it, and a funnel shift brings it down.  The 64-bit load may reach
-// up to 4 bytes past the last element, so BF16 images carry 16 bytes of
-// tail padding and their resources count 8 of them (img_bytes).
+// BF16: a gradient sample (6 bytes) starts on an even byte; one 64-bit
+// load from the dword boundary at or below it always covers it, and a
+// funnel shift brings it down.  The 64-bit load may reach up to 4 bytes
+// past the last element, so BF16 images carry 16 bytes of tail padding and
+// their resources count 8 of them (img_bytes).
 template <> struct Img<uint16_t> {
     static constexpr int ES = 2;
     static __device__ __forceinline__ float one (Rsrc r, int off, int soff)
@@ -319,13 +313,6 @@ template <> struct Img<uint16_t> {
         lo = __builtin_amdgcn_alignbit (v[1], v[0], sh);
         hi = v[1] >> sh;
     }
-    static __device__ __forceinline__ void pair (float& a, float& b, Rsrc r, int off, int soff)
-    {
-        uint32_t lo, hi;
-        load_shifted (lo, hi, r, off, soff);
-        a = __uint_as_float (lo << 16);
-        b = __uint_as_float (lo & 0xffff0000u);
-    }
     static __device__ __forceinline__ void vec3 (float o[3], Rsrc r, int off, int soff)
     {
         uint32_t lo, hi;
@@ -333,6 +320,17 @@ template <> struct Img<uint16_t> {
         o[0] = __uint_as_float (lo << 16);
         o[1] = __uint_as_float (lo & 0xffff0000u);
         o[2] = __uint_as_float (hi << 16);
+    }
+    // 8 bytes of the z-pair image (4-byte records, so always aligned):
+    // (x, z), (x, z + 1), (x + 1, z), (x + 1, z + 1)
+    static __device__ __forceinline__ void quad (float& a, float& b, float& c, float& d, Rsrc r,
+        int off, int soff)
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64 (r, off, soff, 0);
+        a = __uint_as_float (v[0] << 16);
+        b = __uint_as_float (v[0] & 0xffff0000u);
+        c = __uint_as_float (v[1] << 16);
+        d = __uint_as_float (v[1] & 0xffff0000u);
     }
 };
 
@@ -960,9 +958,10 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         // Float trilinear kinds read the corners from the z-pair image: two
         // 16-byte loads a voxel instead of four 8-byte ones (the vector
         // memory pipeline's cost goes by instructions, not bytes).
-        constexpr bool ZPAIR = std::is_same<TI, float>::value && !MI && !NEAREST;
-        const Rsrc zr = make_rsrc (pd.mpair, ZPAIR ? (uint32_t) (md01 * g.mdim[2]) * 8u : 0u);
-        const int SX8 = md0 * 8, SXY8 = md01 * 8;
+        constexpr bool ZPAIR = !MI && !NEAREST;
+        constexpr int EZ = 2 * Img<TI>::ES;     // bytes per z-pair record
+        const Rsrc zr = make_rsrc (pd.mpair, ZPAIR ? (uint32_t) (md01 * g.mdim[2]) * EZ : 0u);
+        const int SXZ = md0 * EZ, SXYZ = md01 * EZ;
         // The moving-index bounds, read once per tile into registers: left
         // in memory, they are reloaded per voxel (each reload a scalar-load
         // round trip).
@@ -1085,17 +1084,10 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                         + (int) __umul24 (f[2] + c[2], SXY);
                     nv[u] = Img<TI>::one (mr, roff, 0);
                 } else if (ZPAIR) {
-                    const int off = (int) __umul24 (f[0], 8) + (int) __umul24 (f[1], SX8)
-                        + (int) __umul24 (f[2], SXY8);
-                    Img<float>::quad (a[u][0], a[u][4], a[u][1], a[u][5], zr, off, 0);
-                    Img<float>::quad (a[u][2], a[u][6], a[u][3], a[u][7], zr, off, SX8);
-                } else if (!MI) {
-                    const int off = (int) __umul24 (f[0], ES) + (int) __umul24 (f[1], SX)
-                        + (int) __umul24 (f[2], SXY);
-                    Img<TI>::pair (a[u][0], a[u][1], mr, off, 0);
-                    Img<TI>::pair (a[u][2], a[u][3], mr, off, SX);
-                    Img<TI>::pair (a[u][4], a[u][5], mr, off, SXY);
-                    Img<TI>::pair (a[u][6], a[u][7], mr, off, SX + SXY);
+                    const int off = (int) __umul24 (f[0], EZ) + (int) __umul24 (f[1], SXZ)
+                        + (int) __umul24 (f[2], SXYZ);
+                    Img<TI>::quad (a[u][0], a[u][4], a[u][1], a[u][5], zr, off, 0);
+                    Img<TI>::quad (a[u][2], a[u][6], a[u][3], a[u][7], zr, off, SXZ);
                 } else {
                     const int idx = f[0] + (int) __umul24 (md0, f[1]) + (int) __umul24 (md01, f[2]);
                     bn[u][0] = load_u8 (mr, idx, 0);
@@ -1831,13 +1823,18 @@ k_bin_image (const TI* __restrict__ img, uint8_t* __restrict__ out, int64_t n, f
         out[i] = (uint8_t) hist_bin (ld (img, (int) i), offset, delta, bins);
 }
 
-// The moving image as z-pairs (see launch_zpair); once per stage.
+// The moving image as z-pairs (see launch_zpair); once per stage.  T2 is
+// the record: float2 for float storage, uint32_t (two BF16) for BF16.
+template <typename TI, typename T2>
 __global__ void
-k_zpair (const float* __restrict__ m, float2* __restrict__ out, int64_t n, int64_t d01)
+k_zpair (const TI* __restrict__ m, T2* __restrict__ out, int64_t n, int64_t d01)
 {
     for (int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x * blockDim.x)
-        out[i] = make_float2 (m[i], i + d01 < n ? m[i + d01] : 0.f);
+         i += (int64_t) gridDim.x * blockDim.x) {
+        const TI lo = m[i], hi = i + d01 < n ? m[i + d01] : (TI) 0;
+        if constexpr (std::is_same<TI, float>::value) out[i] = make_float2 (lo, hi);
+        else out[i] = (uint32_t) lo | ((uint32_t) hi << 16);
+    }
 }
 
 __global__ void
@@ -2114,10 +2111,14 @@ launch_bin_image (const void* img, int storage, uint8_t* out, int64_t n, float o
 }
 
 hipError_t
-launch_zpair (const float* m, float* out, int64_t n, int64_t d01, hipStream_t st)
+launch_zpair (const void* m, int storage, void* out, int64_t n, int64_t d01, hipStream_t st)
 {
-    hipLaunchKernelGGL (k_zpair, dim3 (1024), dim3 (256), 0, st, m, reinterpret_cast<float2*> (out), n,
-        d01);
+    if (storage == 0)
+        hipLaunchKernelGGL ((k_zpair<float, float2>), dim3 (1024), dim3 (256), 0, st,
+            (const float*) m, (float2*) out, n, d01);
+    else
+        hipLaunchKernelGGL ((k_zpair<uint16_t, uint32_t>), dim3 (1024), dim3 (256), 0, st,
+            (const uint16_t*) m, (uint32_t*) out, n, d01);
     return hipGetLastError ();
 }
 
