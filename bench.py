@@ -179,7 +179,8 @@ def batched_leg(args, dev, want_errors):
     # (a) one batched evaluation, autotuned, captured once and replayed
     batch = BsplineBatch(stages)
     batch.set_buffers(d_coeffs, d_grads, d_score, d_nvox)
-    vpt, _ = batch.autotune("ref_nn", reps=3)
+    batch.autotune("ref_nn", reps=3)
+    vpt, wgs = batch.get_tuning()
     batch.capture("ref_nn")
     for _ in range(warm):
         batch.launch(stream)
@@ -206,6 +207,7 @@ def batched_leg(args, dev, want_errors):
         "dims": [list(p.dim) for p in pairs],
         "storage": "bf16 (FP32 interpolation, FP64 reductions)",
         "vox_per_thread": vpt,
+        "workgroups_per_cu": wgs,
         "batched_graph_ms_per_eval": t_batch * 1e3,
         "looped_single_ms_per_eval": t_loop * 1e3,
         "batched_pair_evals_per_s": BATCH_PAIRS / t_batch,
@@ -589,7 +591,8 @@ def main():
     d_grad = torch.zeros(N_COEFF_VECTORS, bxf.num_coeff, dtype=torch.float64, device=dev)
     d_nvox = torch.zeros(N_COEFF_VECTORS, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
-    vpt, tuned_ms = st.autotune("mse", "ref_nn", d_coeffs[0], reps=10)
+    st.autotune("mse", "ref_nn", d_coeffs[0], reps=10)
+    vpt, wgs = st.get_tuning()
 
     def step(i):
         k = i % N_COEFF_VECTORS
@@ -640,7 +643,7 @@ def main():
         "config": {"workload": WORKLOAD, "dim": list(pair.dim), "spacing": list(pair.spacing),
                    "vox_per_rgn": list(pair.vox_per_rgn), "grad_mode": "ref_nn",
                    "coeff_vectors": N_COEFF_VECTORS, "num_tiles": st.num_tiles,
-                   "num_coeff": bxf.num_coeff, "vox_per_thread": vpt},
+                   "num_coeff": bxf.num_coeff, "vox_per_thread": vpt, "workgroups_per_cu": wgs},
         "eval_gb_per_s": bytes_per_launch / (ms_per_step * 1e-3) / 1e9,
         "stage_ms": {"k_tiles": stage_ms[0], "k_reduce": stage_ms[1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

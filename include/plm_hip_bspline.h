@@ -198,14 +198,19 @@ int plm_hip_bspline_get_timing (plm_hip_ctx* ctx, double* stage_ms, int max_stag
     int64_t* evals);
 
 /* Launch-shape tuning of the tile kernel: how many voxels each thread keeps
- * in flight (1, 2 or 4).  autotune times every variant on this context's
- * own images (reps evaluations each, after one warm-up, HIP events on the
- * context stream), keeps the fastest whole evaluation and reports it.
- * d_coeff is a device coefficient vector or NULL (zeros).  The counterpart
- * of the reference's launch-configuration search (CUDA_exec_conf_1tpe /
- * _1bpe, src/cuda/cuda_kernel_util.cu:48-188; divideWork,
- * src/opencl/autotune_opencl.cxx:7-53). */
+ * in flight (1, 2 or 4) and how many of its persistent workgroups share a
+ * compute unit (set_occupancy; 0 = as many as fit, the default).  autotune
+ * times every pair of the two (vox_per_thread 1, 2, 4 by workgroups per CU
+ * as many as fit, 6, 5, 4) on this context's own images (reps evaluations
+ * each, after one warm-up, HIP events on the context stream), keeps the
+ * fastest whole evaluation and reports it; get_tuning reads the current
+ * shape.  d_coeff is a device coefficient vector or NULL (zeros).  The
+ * counterpart of the reference's launch-configuration search
+ * (CUDA_exec_conf_1tpe / _1bpe, src/cuda/cuda_kernel_util.cu:48-188;
+ * divideWork, src/opencl/autotune_opencl.cxx:7-53). */
 int plm_hip_bspline_set_tuning (plm_hip_ctx* ctx, int vox_per_thread);
+int plm_hip_bspline_set_occupancy (plm_hip_ctx* ctx, int workgroups_per_cu);
+int plm_hip_bspline_get_tuning (plm_hip_ctx* ctx, int* vox_per_thread, int* workgroups_per_cu);
 int plm_hip_bspline_autotune (plm_hip_ctx* ctx, int metric, int grad_mode,
     const float* d_coeff, int reps, int* chosen_vox_per_thread, double* eval_ms);
 
@@ -288,7 +293,8 @@ int plm_hip_batch_score_device (plm_hip_batch* b, int grad_mode, void* stream);
 
 /* Capture one evaluation into a hipGraph (on the batch's stream) and
  * instantiate it; plm_hip_batch_launch replays it on stream.  Changing the
- * buffers or the tuning drops the graph (launch then returns
+ * buffers or the tuning (vox_per_thread or workgroups per CU) drops the
+ * graph (launch then returns
  * PLM_HIP_ERR_ARG until the next capture). */
 int plm_hip_batch_capture (plm_hip_batch* b, int grad_mode);
 int plm_hip_batch_launch (plm_hip_batch* b, void* stream);
@@ -296,6 +302,8 @@ int plm_hip_batch_launch (plm_hip_batch* b, void* stream);
 /* Tuning and stage timing of the batched tile kernel, as
  * plm_hip_bspline_set_tuning / _autotune / _set_timing / _get_timing. */
 int plm_hip_batch_set_tuning (plm_hip_batch* b, int vox_per_thread);
+int plm_hip_batch_set_occupancy (plm_hip_batch* b, int workgroups_per_cu);
+int plm_hip_batch_get_tuning (plm_hip_batch* b, int* vox_per_thread, int* workgroups_per_cu);
 int plm_hip_batch_autotune (plm_hip_batch* b, int grad_mode, int reps,
     int* chosen_vox_per_thread, double* eval_ms);
 int plm_hip_batch_set_timing (plm_hip_batch* b, int enable);

@@ -124,6 +124,8 @@ SIGNATURES = {
                                                ctypes.c_int]),
     "plm_hip_bspline_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "plm_hip_bspline_set_tuning": (ctypes.c_int, [_P, ctypes.c_int]),
+    "plm_hip_bspline_set_occupancy": (ctypes.c_int, [_P, ctypes.c_int]),
+    "plm_hip_bspline_get_tuning": (ctypes.c_int, [_P, _P, _P]),
     "plm_hip_bspline_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int,
                                                 ctypes.POINTER(ctypes.c_int),
                                                 ctypes.POINTER(ctypes.c_double)]),
@@ -142,6 +144,8 @@ SIGNATURES = {
     "plm_hip_batch_capture": (ctypes.c_int, [_P, ctypes.c_int]),
     "plm_hip_batch_launch": (ctypes.c_int, [_P, _P]),
     "plm_hip_batch_set_tuning": (ctypes.c_int, [_P, ctypes.c_int]),
+    "plm_hip_batch_set_occupancy": (ctypes.c_int, [_P, ctypes.c_int]),
+    "plm_hip_batch_get_tuning": (ctypes.c_int, [_P, _P, _P]),
     "plm_hip_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_double)]),

@@ -454,6 +454,17 @@ class BsplineHip:
     def set_tuning(self, vox_per_thread):
         _abi.check(self.lib.plm_hip_bspline_set_tuning(self.ctx, int(vox_per_thread)))
 
+    def set_occupancy(self, workgroups_per_cu):
+        """Tile-kernel workgroups per compute unit (0: as many as fit)."""
+        _abi.check(self.lib.plm_hip_bspline_set_occupancy(self.ctx, int(workgroups_per_cu)))
+
+    def get_tuning(self):
+        """The current launch shape: (vox_per_thread, workgroups_per_cu)."""
+        v = ctypes.c_int()
+        w = ctypes.c_int()
+        _abi.check(self.lib.plm_hip_bspline_get_tuning(self.ctx, ctypes.byref(v), ctypes.byref(w)))
+        return v.value, w.value
+
     def autotune(self, metric="mse", grad_mode="ref_nn", d_coeff=None, reps=5):
         """Pick the fastest launch shape on this stage's data; returns (choice, ms/eval)."""
         ch = ctypes.c_int()
@@ -548,6 +559,17 @@ class BsplineBatch:
 
     def set_tuning(self, vox_per_thread):
         _abi.check(self.lib.plm_hip_batch_set_tuning(self.b, int(vox_per_thread)))
+
+    def set_occupancy(self, workgroups_per_cu):
+        """Tile-kernel workgroups per compute unit (0: as many as fit)."""
+        _abi.check(self.lib.plm_hip_batch_set_occupancy(self.b, int(workgroups_per_cu)))
+
+    def get_tuning(self):
+        """The current launch shape: (vox_per_thread, workgroups_per_cu)."""
+        v = ctypes.c_int()
+        w = ctypes.c_int()
+        _abi.check(self.lib.plm_hip_batch_get_tuning(self.b, ctypes.byref(v), ctypes.byref(w)))
+        return v.value, w.value
 
     def autotune(self, grad_mode="ref_nn", reps=5):
         ch = ctypes.c_int()

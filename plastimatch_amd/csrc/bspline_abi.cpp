@@ -666,6 +666,24 @@ plm_hip_bspline_set_tuning (plm_hip_ctx* c, int vox_per_thread)
 }
 
 int
+plm_hip_bspline_set_occupancy (plm_hip_ctx* c, int wg_per_cu)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    if (wg_per_cu < 0) return fail (PLM_HIP_ERR_ARG, "workgroups per CU must be >= 0");
+    c->tune.wg_per_cu = wg_per_cu;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_bspline_get_tuning (plm_hip_ctx* c, int* vox_per_thread, int* wg_per_cu)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    if (vox_per_thread) *vox_per_thread = c->tune.vox_per_thread;
+    if (wg_per_cu) *wg_per_cu = c->tune.wg_per_cu;
+    return PLM_HIP_OK;
+}
+
+int
 plm_hip_bspline_autotune (plm_hip_ctx* c, int metric, int grad_mode, const float* d_coeff,
     int reps, int* chosen, double* eval_ms)
 {
@@ -679,10 +697,11 @@ plm_hip_bspline_autotune (plm_hip_ctx* c, int metric, int grad_mode, const float
     if (rc) return rc;
     c->timer.on = true;
     const int cands[3] = {1, 2, 4};
-    int best = c->tune.vox_per_thread;
+    int best = c->tune.vox_per_thread, best_wg = c->tune.wg_per_cu;
     double best_ms = 1e300;
-    for (int k = 0; k < 3; k++) {
-        c->tune.vox_per_thread = cands[k];
+    for (int k = 0; k < 12; k++) {
+        c->tune.vox_per_thread = cands[k % 3];
+        c->tune.wg_per_cu = plmhip::TUNE_WG_CANDS[k / 3];
         for (int r = 0; r < reps + 1; r++) {
             rc = plm_hip_bspline_score_device (c, metric, grad_mode, coeff, c->d_score, c->d_grad,
                 c->d_nvox, nullptr, c->stream);
@@ -698,9 +717,10 @@ plm_hip_bspline_autotune (plm_hip_ctx* c, int metric, int grad_mode, const float
         double total = 0.0;
         for (int i = 0; i < plmhip::MAX_STAGE_EVENTS; i++) total += stage[i];
         total /= (double) (n ? n : 1);
-        if (total < best_ms) { best_ms = total; best = cands[k]; }
+        if (total < best_ms) { best_ms = total; best = cands[k % 3]; best_wg = plmhip::TUNE_WG_CANDS[k / 3]; }
     }
     c->tune.vox_per_thread = best;
+    c->tune.wg_per_cu = best_wg;
     c->timer.on = was_timing;
     if (chosen) *chosen = best;
     if (eval_ms) *eval_ms = best_ms;
@@ -1160,6 +1180,25 @@ plm_hip_batch_set_tuning (plm_hip_batch* b, int vox_per_thread)
 }
 
 int
+plm_hip_batch_set_occupancy (plm_hip_batch* b, int wg_per_cu)
+{
+    if (!b) return fail (PLM_HIP_ERR_ARG, "batch is NULL");
+    if (wg_per_cu < 0) return fail (PLM_HIP_ERR_ARG, "workgroups per CU must be >= 0");
+    if (b->tune.wg_per_cu != wg_per_cu) drop_graph (b);
+    b->tune.wg_per_cu = wg_per_cu;
+    return PLM_HIP_OK;
+}
+
+int
+plm_hip_batch_get_tuning (plm_hip_batch* b, int* vox_per_thread, int* wg_per_cu)
+{
+    if (!b) return fail (PLM_HIP_ERR_ARG, "batch is NULL");
+    if (vox_per_thread) *vox_per_thread = b->tune.vox_per_thread;
+    if (wg_per_cu) *wg_per_cu = b->tune.wg_per_cu;
+    return PLM_HIP_OK;
+}
+
+int
 plm_hip_batch_set_timing (plm_hip_batch* b, int enable)
 {
     if (!b) return fail (PLM_HIP_ERR_ARG, "batch is NULL");
@@ -1190,26 +1229,33 @@ plm_hip_batch_autotune (plm_hip_batch* b, int grad_mode, int reps, int* chosen, 
     if (rc) return rc;
     b->timer.on = true;
     const int cands[3] = {1, 2, 4};
-    const int before = b->tune.vox_per_thread;
-    int best = before;
+    const int before = b->tune.vox_per_thread, before_wg = b->tune.wg_per_cu;
+    int best = before, best_wg = before_wg;
     double best_ms = 1e300;
-    for (int k = 0; k < 3; k++) {
-        b->tune.vox_per_thread = cands[k];
+    for (int k = 0; k < 12; k++) {
+        b->tune.vox_per_thread = cands[k % 3];
+        b->tune.wg_per_cu = plmhip::TUNE_WG_CANDS[k / 3];
         for (int r = 0; r < reps + 1; r++) {
             rc = plm_hip_batch_score_device (b, grad_mode, b->stream);
             if (rc == PLM_HIP_OK && r == 0)     // the first call of each variant is a warm-up
                 rc = plm_hip_batch_get_timing (b, stage, plmhip::MAX_STAGE_EVENTS, &n);
-            if (rc) { b->timer.on = was_timing; b->tune.vox_per_thread = before; return rc; }
+            if (rc) break;
         }
-        rc = plm_hip_batch_get_timing (b, stage, plmhip::MAX_STAGE_EVENTS, &n);
-        if (rc) { b->timer.on = was_timing; b->tune.vox_per_thread = before; return rc; }
+        if (rc == PLM_HIP_OK) rc = plm_hip_batch_get_timing (b, stage, plmhip::MAX_STAGE_EVENTS, &n);
+        if (rc) {
+            b->timer.on = was_timing;
+            b->tune.vox_per_thread = before;
+            b->tune.wg_per_cu = before_wg;
+            return rc;
+        }
         double total = 0.0;
         for (int i = 0; i < plmhip::MAX_STAGE_EVENTS; i++) total += stage[i];
         total /= (double) (n ? n : 1);
-        if (total < best_ms) { best_ms = total; best = cands[k]; }
+        if (total < best_ms) { best_ms = total; best = cands[k % 3]; best_wg = plmhip::TUNE_WG_CANDS[k / 3]; }
     }
     b->tune.vox_per_thread = best;
-    if (best != before) drop_graph (b);
+    b->tune.wg_per_cu = best_wg;
+    if (best != before || best_wg != before_wg) drop_graph (b);
     b->timer.on = was_timing;
     if (chosen) *chosen = best;
     if (eval_ms) *eval_ms = best_ms;

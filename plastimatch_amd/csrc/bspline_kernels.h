@@ -85,7 +85,12 @@ struct Tune {
     int vox_per_thread;      // voxels each tile-kernel thread keeps in flight: 1, 2 or 4
     size_t lds_limit;        // LDS bytes a workgroup may use
     int num_cus;             // compute units of the device (persistent grid size)
+    int wg_per_cu = 0;       // tile-kernel workgroups per CU (0: as many as fit)
 };
+
+// Workgroups-per-CU candidates of the autotuners (0: as many as fit).  Fewer
+// than fit can be faster: the tiles in flight on a CU share its L1.
+constexpr int TUNE_WG_CANDS[4] = {0, 6, 5, 4};
 
 // LDS bytes of the tile kernel at vox_per_thread voxels per batch, static
 // arrays included (the MI table comes on top).

@@ -1905,9 +1905,9 @@ struct TileArgs {
 };
 
 // Persistent grid: as many workgroups as fit on the device at once
-// (occupancy from the kernel's registers and this launch's LDS), a
-// multiple of 8 so every XCD gets the same count, and no more than an XCD
-// has tiles.
+// (occupancy from the kernel's registers and this launch's LDS; at most
+// tune.wg_per_cu per CU when set), a multiple of 8 so every XCD gets the
+// same count, and no more than an XCD has tiles.
 template <int KIND, typename TI, int V, int G, bool BATCH>
 static int
 launch_tiles_vgb (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st,
@@ -1929,6 +1929,7 @@ launch_tiles_vgb (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st
         }
         occ = cached_occ;
     }
+    if (tune.wg_per_cu > 0) occ = std::min (occ, tune.wg_per_cu);
     TileQueue q;
     q.total_tiles = plan.total_tiles;
     q.tiles_per_xcd = (plan.total_tiles + 7) / 8;

@@ -51,6 +51,10 @@ def test_argument_errors_without_device():
     assert lib.plm_hip_batch_score_device(None, 0, None) == _abi.PLM_HIP_ERR_ARG
     assert lib.plm_hip_batch_launch(None, None) == _abi.PLM_HIP_ERR_ARG
     assert lib.plm_hip_batch_set_tuning(None, 2) == _abi.PLM_HIP_ERR_ARG
+    assert lib.plm_hip_batch_set_occupancy(None, 2) == _abi.PLM_HIP_ERR_ARG
+    assert lib.plm_hip_batch_get_tuning(None, None, None) == _abi.PLM_HIP_ERR_ARG
+    assert lib.plm_hip_bspline_set_occupancy(None, 2) == _abi.PLM_HIP_ERR_ARG
+    assert lib.plm_hip_bspline_get_tuning(None, None, None) == _abi.PLM_HIP_ERR_ARG
     assert lib.plm_hip_batch_destroy(None) == _abi.PLM_HIP_OK
 
 

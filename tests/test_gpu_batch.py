@@ -97,6 +97,12 @@ def test_batch_launch_shapes(hip_lib, vpt):
         batch.score_device("ref_nn")
         torch.cuda.synchronize()
         check_same(single, grads, score, nvox, f"vpt {vpt}")
+        # fewer workgroups per CU: the same bits
+        batch.set_occupancy(3)
+        assert batch.get_tuning() == (vpt, 3)
+        batch.score_device("ref_nn")
+        torch.cuda.synchronize()
+        check_same(single, grads, score, nvox, f"vpt {vpt}, 3 workgroups per CU")
     for st in stages:
         st.close()
 
