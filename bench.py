@@ -28,6 +28,9 @@ iterations on the headline workload, with and without regularization, and
 the reference's liblbfgs on the oracle's score for 3 iterations beside it.
 "stage_preprocess" times a stage's preprocessing on the device (both
 images subsampled, moving gradient, histogram ranges) against the oracle.
+"mattes_mi" evaluates Mattes MI on the 512x512x160 pair (12-voxel grid,
+32x32 bins, fixed-point PV histogram) with its roofline, the oracle's
+flavor g timed beside it and the histograms compared bin for bin.
 """
 import argparse
 import json
@@ -544,6 +547,90 @@ def stage_leg(args, dev, want_cpu):
     return out
 
 
+MI_WORKLOAD = "mi_512x512x160_vpr12"
+MI_BINS = 32
+
+
+def mi_leg(args, dev, want_cpu):
+    """Mattes MI (flavor g) on the 512x512x160 CT-like pair with a 12-voxel
+    grid and 32x32 bins: evaluations per second at four coefficient vectors
+    (the true field and three N(0, 4 mm) draws), the stage times, and the
+    HBM roofline of the whole two-pass evaluation (fixed 4 + moving 4 B per
+    voxel and pass, the survey's model; the device reads 1-byte bin images
+    instead).  With the CPU leg: the oracle's flavor g (serial pass 1 as the
+    reference) on one of the vectors, timed, and the device's integer
+    histogram, score and gradient against it and the FP64 oracle."""
+    import torch
+    from plastimatch_amd import synth
+    from plastimatch_amd.bspline import stage_for_pair
+
+    pair = synth.make_config(MI_WORKLOAD, seed=0)
+    nvox = pair.fixed.size
+    st = stage_for_pair(pair, with_grad=False, bins=MI_BINS)
+    coeffs = [pair.true_coeff] + [synth.random_coeff(pair.dim, pair.vox_per_rgn, 4.0, seed=400 + i)
+                                  for i in range(3)]
+    d_coeffs = [torch.from_numpy(np.ascontiguousarray(c, np.float32)).to(dev) for c in coeffs]
+    d_score = torch.zeros(1, dtype=torch.float64, device=dev)
+    d_grad = torch.zeros(st.bxf.num_coeff, dtype=torch.float64, device=dev)
+    d_nvox = torch.zeros(1, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    st.autotune("mi", "ref_nn", d_coeffs[0], reps=3)
+    vpt, wgs = st.get_tuning()
+
+    def step(i):
+        st.score_device(d_coeffs[i % len(d_coeffs)], d_score, d_grad, d_nvox, "mi", stream=stream)
+
+    for i in range(3):
+        step(i)
+    torch.cuda.synchronize()
+    steps = max(8, args.steps // 50)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(i)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st.set_timing(True)
+    for i in range(steps):
+        step(i)
+    ms, n = st.get_timing()
+    st.set_timing(False)
+    stage = [float(x / n) for x in ms[:5]]
+    alg = 16 * nvox + 4 * coeffs[0].size + 8 * coeffs[0].size
+    achieved = alg / dt / 1e9
+    out = {"workload": MI_WORKLOAD, "dim": list(pair.dim), "vox_per_rgn": list(pair.vox_per_rgn),
+           "bins": [MI_BINS, MI_BINS], "coeff_vectors": len(coeffs),
+           "vox_per_thread": vpt, "workgroups_per_cu": wgs,
+           "value": 1.0 / dt, "unit": "evals/s", "ms_per_eval": dt * 1e3,
+           "stage_ms": {"zero": stage[0], "k_mi_hist": stage[1], "k_mi_score": stage[2],
+                        "k_tiles_pass2": stage[3], "k_reduce": stage[4]},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_eval": alg,
+                        "scope": "whole evaluation (both passes)"}}
+    if want_cpu:
+        import oracle
+        threads = min(16, len(os.sched_getaffinity(0)))
+        oc = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn, origin=pair.origin)
+        hp = oc.hist(MI_BINS)
+        t0 = time.perf_counter()
+        o = oc.mi(coeffs[1], hp, nthreads=threads)
+        t_cpu = time.perf_counter() - t0
+        s64, g64, _ = oc.mi_f64(coeffs[1], hp, nthreads=threads)
+        r = st.score(coeffs[1], "mi", want_hist=True)
+        nj = MI_BINS * MI_BINS
+        out["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "evals/s", "cores": threads, "kind": "port",
+                               "sample": "one evaluation of the oracle's flavor g (pass 1 serial as the "
+                                         "reference, pass 2 OpenMP)"}
+        out["identical_histograms"] = bool(np.array_equal(r.hist[:nj], o["fx_j_hist"])
+                                           and np.array_equal(r.hist[nj:], o["fx_f_hist"]))
+        out["num_vox_equal"] = r.num_vox == o["num_vox"]
+        out["rel_err_vs_oracle"] = {"score": abs(r.score - o["score"]) / abs(o["score"]),
+                                    "grad_l2": float(np.linalg.norm(r.grad - o["grad"]) / np.linalg.norm(o["grad"]))}
+        out["rel_err_vs_fp64"] = {"score": abs(r.score - s64) / abs(s64),
+                                  "grad_l2": float(np.linalg.norm(r.grad - g64) / np.linalg.norm(g64))}
+    st.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -555,6 +642,7 @@ def main():
     ap.add_argument("--no-warp-reg", action="store_true", help="skip the warp and regularization leg")
     ap.add_argument("--no-optimizer", action="store_true", help="skip the optimizer-loop leg")
     ap.add_argument("--no-stage", action="store_true", help="skip the stage-preprocessing leg")
+    ap.add_argument("--no-mi", action="store_true", help="skip the Mattes-MI leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy kernel 10 times "
                          "(reference for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -672,6 +760,8 @@ def main():
         out["optimizer_loop"] = optimizer_leg(args, dev, ms_per_step, want_cpu=not args.no_cpu)
     if not args.no_stage:
         out["stage_preprocess"] = stage_leg(args, dev, want_cpu=not args.no_cpu)
+    if not args.no_mi:
+        out["mattes_mi"] = mi_leg(args, dev, want_cpu=not args.no_cpu)
     print(json.dumps(out))
 
 

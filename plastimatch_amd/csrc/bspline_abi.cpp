@@ -196,6 +196,7 @@ struct plm_hip_ctx {
     float* d_nvox_f = nullptr;
     uint8_t* d_fbin = nullptr;              // MI bin images (once per stage)
     uint8_t* d_mbin = nullptr;
+    uint32_t* d_mquad = nullptr;    // moving bins of each voxel's (x, y) quad
     // staging for the host-pointer entry point
     float* d_coeff = nullptr;
     double* d_grad = nullptr;
@@ -233,7 +234,7 @@ free_ctx (plm_hip_ctx* c)
     c->timer.destroy ();
     void* bufs[] = {c->d_fixed, c->d_moving, c->d_mpair, c->d_mgrad, c->d_blut, c->d_cond,
         c->d_tile_score, c->d_tile_count, c->d_pair, c->d_hist,
-        c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
+        c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_mquad, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
         c->d_hist_out, c->d_reg_part, c->d_reg_cond};
     for (void* b : bufs) if (b) (void) hipFree (b);
     if (c->stream) (void) hipStreamDestroy (c->stream);
@@ -472,7 +473,10 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     c->lds_limit = prop.sharedMemPerBlock;
     c->tune.lds_limit = c->lds_limit;
     c->tune.num_cus = prop.multiProcessorCount;
-    c->hist_blocks = (int) std::min<int64_t> (tiles, (int64_t) prop.multiProcessorCount * 2);
+    // MI pass 1: enough workgroups to keep several waves on every SIMD (the
+    // pass is bound by its exact-order FP32 arithmetic), each flushing its
+    // private 8.4 KB histogram once
+    c->hist_blocks = (int) std::min<int64_t> (tiles, (int64_t) prop.multiProcessorCount * 8);
     if (plmhip::tile_lds_bytes (g, 1) + sizeof (float) * h.fbins * h.mbins > c->lds_limit) {
         free_ctx (c);
         return fail (PLM_HIP_ERR_UNSUPPORTED, "tile does not fit in LDS; use a smaller vox_per_rgn");
@@ -523,6 +527,10 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
             e = plmhip::launch_bin_image (c->d_moving, c->storage == PLM_STORE_BF16, c->d_mbin, nm,
                 h.m_offset, h.m_delta, h.mbins, c->stream);
         if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "bin images"); }
+        CTX_TRY (alloc (&c->d_mquad, nm, "hipMalloc(moving bin quads)"));
+        const int mdim[3] = {(int) moving->dim[0], (int) moving->dim[1], (int) moving->dim[2]};
+        e = plmhip::launch_bin_quads (c->d_mbin, c->d_mquad, mdim, c->stream);
+        if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "bin quads"); }
     }
     CTX_TRY (alloc (&c->d_coeff, 3 * knots, "hipMalloc(coeff)"));
     e = hipMemset (c->d_coeff, 0, sizeof (float) * 3 * knots);
@@ -596,7 +604,7 @@ plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
         if (e != hipSuccess) return hip_fail (e, "MSE launch");
     } else {
         const plmhip::EvalOut out {d_grad, d_score, d_num_vox};
-        e = plmhip::launch_mi (c->plan, c->tune, c->h, c->hist_blocks, c->d_fbin, c->d_mbin,
+        e = plmhip::launch_mi (c->plan, c->tune, c->h, c->hist_blocks, c->d_fbin, c->d_mquad,
             d_coeff, c->d_hist, c->d_hist + (size_t) c->h.fbins * c->h.mbins, c->d_table,
             c->d_nvox_f, out, d_hist, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MI launch");

@@ -194,7 +194,7 @@ hipError_t launch_regularize (const KGeom& g, const RegOp& op, const float* coef
     double* cond, double* score, double* grad, int accumulate, hipStream_t st);
 // Mattes MI of a single-pair plan.
 hipError_t launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
-    const uint8_t* fbin, const uint8_t* mbin, const float* coeff, unsigned long long* jhist,
+    const uint8_t* fbin, const uint32_t* mquad, const float* coeff, unsigned long long* jhist,
     unsigned long long* fhist, float* table, float* nvox_f, const EvalOut& out0,
     int64_t* hist_out, hipStream_t st, hipEvent_t* ev);
 
@@ -202,6 +202,9 @@ hipError_t launch_bin_image (const void* img, int storage, uint8_t* out, int64_t
     float offset, float delta, int bins, hipStream_t st);
 
 hipError_t launch_zero (float* p, int64_t n, hipStream_t st);
+// The moving bin-quad image of a bin image of dim voxels: record v holds the
+// bins of v, v + 1, v + dim0, v + dim0 + 1 (low byte first).
+hipError_t launch_bin_quads (const uint8_t* bins, uint32_t* out, const int dim[3], hipStream_t st);
 // The moving image as z-pairs: out[2v] = m[v], out[2v + 1] = m[v + d01]
 // (0 on the last slice), so the two records at voxel v hold the trilinear
 // corners (x, z), (x, z + 1), (x + 1, z), (x + 1, z + 1) of one row: 16

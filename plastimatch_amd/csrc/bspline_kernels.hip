@@ -126,21 +126,6 @@ corners (float a[8], const TI* __restrict__ m, const KGeom& g, int mvf)
     a[7] = ld (m, mvf + sxy + sx + 1);
 }
 
-__device__ __forceinline__ void
-corner_bins (int b[8], const uint8_t* __restrict__ m, const KGeom& g, int mvf)
-{
-    const int sx = g.mdim[0];
-    const int sxy = g.mdim[0] * g.mdim[1];
-    b[0] = m[mvf];
-    b[1] = m[mvf + 1];
-    b[2] = m[mvf + sx];
-    b[3] = m[mvf + sx + 1];
-    b[4] = m[mvf + sxy];
-    b[5] = m[mvf + sxy + 1];
-    b[6] = m[mvf + sxy + sx];
-    b[7] = m[mvf + sxy + sx + 1];
-}
-
 // Linear interpolation t*b + (1-t)*a as two FMAs; exact at t = 0 and
 // t = 1, which the clamped border samples of li_clamp rely on.
 __device__ __forceinline__ float
@@ -333,6 +318,22 @@ template <> struct Img<uint16_t> {
         d = __uint_as_float (v[1] & 0xffff0000u);
     }
 };
+
+// The eight trilinear corner bins from the moving bin-quad image (one
+// 32-bit record per voxel: the bins of (x, y), (x+1, y), (x, y+1),
+// (x+1, y+1), low byte first): the record at byte offset off (slice z) and
+// the one dz bytes on (slice z + 1), in the corner order x, then y, then z.
+__device__ __forceinline__ void
+bin_quads (int b[8], Rsrc r, int off, int dz)
+{
+    const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b32 (r, off, 0, 0);
+    const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b32 (r, off, dz, 0);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        b[n] = (int) ((lo >> (8 * n)) & 0xffu);
+        b[n + 4] = (int) ((hi >> (8 * n)) & 0xffu);
+    }
+}
 
 // Byte size of a resource over n elements of es bytes (BF16 images count
 // part of their tail padding, see Img<uint16_t>).
@@ -720,7 +721,7 @@ struct MiArgs {
     const float* table;        // dS/dP per joint bin (0 where j <= 1e-4)
     const float* nvox_f;       // (float) N
     const uint8_t* fbin;       // per-voxel fixed bin (bin image of the stage)
-    const uint8_t* mbin;       // per-voxel moving bin
+    const uint32_t* mquad;     // per-voxel moving bins of (x, y), (x+1, y), (x, y+1), (x+1, y+1)
 };
 
 // The pair a tile of the flattened queue belongs to; p is the pair of an
@@ -949,8 +950,8 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         const int fbase = P.bx0 + fd0 * P.by0 + fd01 * P.bz0;
         const uint32_t nfix = (uint32_t) (fd01 * g.fdim[2]);
         const Rsrc fr = make_rsrc (MI ? (const void*) mi.fbin : pd.fimg, img_bytes (nfix, ES));
-        const Rsrc mr = make_rsrc (MI ? (const void*) mi.mbin : pd.mimg,
-            img_bytes ((uint32_t) (md01 * g.mdim[2]), ES));
+        const Rsrc mr = MI ? make_rsrc (mi.mquad, (uint32_t) (md01 * g.mdim[2]) * 4u)
+                           : make_rsrc (pd.mimg, img_bytes ((uint32_t) (md01 * g.mdim[2]), ES));
         const Rsrc gr = make_rsrc (pd.mgrad,
             KIND == 0 ? img_bytes ((uint32_t) (md01 * g.mdim[2]) * 3, ES) : 0u);
         const Rsrc vr = make_rsrc (vf, VF_IN ? nfix * 12u : 0u);
@@ -1090,14 +1091,8 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     Img<TI>::quad (a[u][2], a[u][6], a[u][3], a[u][7], zr, off, SXZ);
                 } else {
                     const int idx = f[0] + (int) __umul24 (md0, f[1]) + (int) __umul24 (md01, f[2]);
-                    bn[u][0] = load_u8 (mr, idx, 0);
-                    bn[u][1] = load_u8 (mr, idx + 1, 0);
-                    bn[u][2] = load_u8 (mr, idx, md0);
-                    bn[u][3] = load_u8 (mr, idx + 1, md0);
-                    bn[u][4] = load_u8 (mr, idx, md01);
-                    bn[u][5] = load_u8 (mr, idx + 1, md01);
-                    bn[u][6] = load_u8 (mr, idx, md0 + md01);
-                    bn[u][7] = load_u8 (mr, idx + 1, md0 + md01);
+                    // the bin quads of slices z and z + 1
+                    bin_quads (bn[u], mr, 4 * idx, 4 * md01);
                 }
                 if (KIND == 0) {
                     // nearest voxel: the gradient sample of flavor g
@@ -1386,6 +1381,24 @@ exact_displacement (float out[3], const float* bx, const float* by, const float*
     out[0] = o0; out[1] = o1; out[2] = o2;
 }
 
+// floor(w * 2^32 + 0.5) for a weight 0 <= w <= 1, in integer arithmetic:
+// w = m * 2^(e - 150) with the 24-bit significand m and biased exponent e,
+// so w * 2^32 = m * 2^(e - 118): a left shift when e >= 118, else a
+// right shift by k = 118 - e rounding half up, (m + 2^(k-1)) >> k, which
+// is 0 once k > 24.  The double expression is exact (w * 2^32 + 0.5 needs
+// at most 49 significant bits), so the two agree for every such float.
+__device__ __forceinline__ long long
+pv_fixed (float w)
+{
+    const uint32_t bits = __float_as_uint (w);
+    const int e = (int) ((bits >> 23) & 0xffu);
+    const uint64_t m = (uint64_t) ((bits & 0x7fffffu) | 0x800000u);
+    const int k = 118 - e;
+    if (e == 0 || k > 24) return 0;        // zero, denormal, or below 2^-33
+    if (k <= 0) return (long long) (m << (-k));
+    return (long long) ((m + (1ull << (k - 1))) >> k);
+}
+
 // Fixed-point PV weights (units of 2^-32): q = floor(w*2^32 + 0.5), the
 // remainder to the first largest weight, so each voxel adds exactly 2^32.
 __device__ __forceinline__ void
@@ -1395,7 +1408,7 @@ pv_quantize (const float w[8], long long q[8])
     int amax = 0;
 #pragma unroll
     for (int n = 0; n < 8; n++) {
-        q[n] = (long long) ((double) w[n] * 4294967296.0 + 0.5);
+        q[n] = pv_fixed (w[n]);
         sum += q[n];
         if (w[n] > w[amax]) amax = n;
     }
@@ -1447,20 +1460,23 @@ wave_hist_add (unsigned long long* s_hist, bool act, int key, unsigned long long
 template <bool ALIGNED>
 __global__ __launch_bounds__ (TILE_THREADS)
 void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
-    const uint8_t* __restrict__ mbin, const float* __restrict__ coeff,
+    const uint32_t* __restrict__ mquad, const float* __restrict__ coeff,
     const float* __restrict__ blut, unsigned long long* __restrict__ jhist,
     unsigned long long* __restrict__ fhist)
 {
-    extern __shared__ float sm[];
+    // the histograms first (the dynamic LDS base is 16-byte aligned), then
+    // the LUTs; every pointer stays an LDS pointer, so the atomics are LDS
+    // atomics
+    extern __shared__ unsigned long long sm_u64[];
     const KGeom& G_ = g;
     const int vx = G_.vpr[0], vy = G_.vpr[1], vz = G_.vpr[2];
-    float* s_bx = sm;
+    unsigned long long* s_j = sm_u64;
+    unsigned long long* s_f = s_j + h.fbins * h.mbins;
+    float* s_bx = reinterpret_cast<float*> (s_f + h.fbins);
     float* s_by = s_bx + 4 * vx;
     float* s_bz = s_by + 4 * vy;
-    float* s_coef = s_bz + 4 * vz;
-    uintptr_t hp = ((uintptr_t) (s_coef + 192) + 7) & ~(uintptr_t) 7;
-    unsigned long long* s_j = (unsigned long long*) hp;
-    unsigned long long* s_f = s_j + h.fbins * h.mbins;
+    const Rsrc qr = make_rsrc (mquad, (uint32_t) (G_.mdim[0] * G_.mdim[1] * G_.mdim[2]) * 4u);
+    const int qdz = 4 * G_.mdim[0] * G_.mdim[1];
 
     const int nbins = h.fbins * h.mbins + h.fbins;
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) s_j[i] = 0ull;
@@ -1493,7 +1509,7 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
                     G_.cdims[0], G_.cdims[0] * G_.cdims[1]);
             act = correspond<ALIGNED> (G_, fxyz, d, s) && act;
             int b[8];
-            corner_bins (b, mbin, G_, s.mvf);
+            bin_quads (b, qr, 4 * s.mvf, qdz);
             const int fidx = act ? fi + G_.fdim[0] * (fj + G_.fdim[1] * fk) : 0;
             const int fb = fbin[fidx];
             const float* l1 = s.l1;
@@ -1837,6 +1853,24 @@ k_zpair (const TI* __restrict__ m, T2* __restrict__ out, int64_t n, int64_t d01)
     }
 }
 
+// The moving bin-quad image: record v holds the bins of voxels v, v + 1,
+// v + d0, v + d0 + 1 (low byte first), 0 for a neighbour past the row or
+// the slice (never read: li_clamp keeps the corner below dim - 1).
+__global__ void
+k_bin_quads (const uint8_t* __restrict__ b, uint32_t* __restrict__ out, int64_t n, int d0, int d1)
+{
+    for (int64_t v = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; v < n;
+         v += (int64_t) gridDim.x * blockDim.x) {
+        const int64_t x = v % d0, y = (v / d0) % d1;
+        const bool xr = x + 1 < d0, yr = y + 1 < d1;
+        uint32_t q = b[v];
+        if (xr) q |= (uint32_t) b[v + 1] << 8;
+        if (yr) q |= (uint32_t) b[v + d0] << 16;
+        if (xr && yr) q |= (uint32_t) b[v + d0 + 1] << 24;
+        out[v] = q;
+    }
+}
+
 __global__ void
 k_zero_u64 (unsigned long long* __restrict__ p, int n)
 {
@@ -2060,7 +2094,7 @@ launch_mse_from_vf (const Plan& plan, const Tune& tune, int storage, const float
 
 hipError_t
 launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
-    const uint8_t* fbin, const uint8_t* mbin, const float* coeff, unsigned long long* jhist,
+    const uint8_t* fbin, const uint32_t* mquad, const float* coeff, unsigned long long* jhist,
     unsigned long long* fhist, float* table, float* nvox_f, const EvalOut& out0,
     int64_t* hist_out, hipStream_t st, hipEvent_t* ev)
 {
@@ -2071,23 +2105,23 @@ launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     // jhist and fhist are one allocation: joint bins then fixed bins.
     hipLaunchKernelGGL (k_zero_u64, dim3 ((nbins + 255) / 256), dim3 (256), 0, st, jhist, nbins);
     LAUNCH_CHECK ();
-    const size_t lds1 = ((4 * (g.vpr[0] + g.vpr[1] + g.vpr[2]) + 192) * 4 + 8)
-        + sizeof (unsigned long long) * nbins;
+    const size_t lds1 = sizeof (unsigned long long) * nbins
+        + sizeof (float) * 4 * (g.vpr[0] + g.vpr[1] + g.vpr[2]);
     const size_t table_bytes = sizeof (float) * h.fbins * h.mbins;
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
     if (g.aligned)
         hipLaunchKernelGGL ((k_mi_hist<true>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
-            g, h, fbin, mbin, coeff, plan.blut, jhist, fhist);
+            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist);
     else
         hipLaunchKernelGGL ((k_mi_hist<false>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
-            g, h, fbin, mbin, coeff, plan.blut, jhist, fhist);
+            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 2, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_mi_score, dim3 (1), dim3 (1024), 0, st, h, jhist, fhist, table,
         out0.score, out0.nvox, nvox_f, hist_out);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 3, st)) != hipSuccess) return e;
-    const TileArgs ta {coeff, MiArgs {h, table, nvox_f, fbin, mbin}, nullptr};
+    const TileArgs ta {coeff, MiArgs {h, table, nvox_f, fbin, mquad}, nullptr};
     // pass 2 reads only bin images; the image type parameter is unused
     launch_tiles<2, float> (plan, tune, table_bytes, st, ta);
     LAUNCH_CHECK ();
@@ -2120,6 +2154,14 @@ launch_zpair (const void* m, int storage, void* out, int64_t n, int64_t d01, hip
     else
         hipLaunchKernelGGL ((k_zpair<uint16_t, uint32_t>), dim3 (1024), dim3 (256), 0, st,
             (const uint16_t*) m, (uint32_t*) out, n, d01);
+    return hipGetLastError ();
+}
+
+hipError_t
+launch_bin_quads (const uint8_t* bins, uint32_t* out, const int dim[3], hipStream_t st)
+{
+    const int64_t n = (int64_t) dim[0] * dim[1] * dim[2];
+    hipLaunchKernelGGL (k_bin_quads, dim3 (1024), dim3 (256), 0, st, bins, out, n, dim[0], dim[1]);
     return hipGetLastError ();
 }
 
