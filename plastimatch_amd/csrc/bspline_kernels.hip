@@ -791,9 +791,13 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
 
     // This thread's first-batch voxel slots t + 256u (x fastest), the same
     // in every tile, packed as x | y << 10 | z << 20 (each below 1024).
-    int sxyz[G];
+    // With compile-time tile edges and at most four slots per thread, every
+    // batch's slots are kept (NSK of them), not only the first batch's.
+    constexpr int NS = V > 0 ? (V * V * V + TILE_THREADS * G - 1) / (TILE_THREADS * G) * G : G;
+    constexpr int NSK = NS <= 4 ? NS : G;
+    int sxyz[NSK];
 #pragma unroll
-    for (int u = 0; u < G; u++) {
+    for (int u = 0; u < NSK; u++) {
         const int v = u * TILE_THREADS + t;
         const int z = S.div_xy (v);
         const int r = v - z * (vx * vy);
@@ -899,7 +903,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         int tt = t;
         asm volatile ("" : "+v" (tt));
 #pragma unroll
-        for (int u = 0; u < G; u++) asm volatile ("" : "+v" (sxyz[u]));
+        for (int u = 0; u < NSK; u++) asm volatile ("" : "+v" (sxyz[u]));
         // The per-tile stages keep fewer than 256 threads busy, so which
         // wave leads them rotates from tile to tile (stage index tr = t +
         // 64 * tile count): over a few tiles every wave, and so every SIMD,
@@ -962,7 +966,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         constexpr bool ZPAIR = !MI && !NEAREST;
         constexpr int EZ = 2 * Img<TI>::ES;     // bytes per z-pair record
         const Rsrc zr = make_rsrc (pd.mpair, ZPAIR ? (uint32_t) (md01 * g.mdim[2]) * EZ : 0u);
-        const int SXZ = md0 * EZ, SXYZ = md01 * EZ;
+        const int SXZ = md0 * EZ;
         // The moving-index bounds, read once per tile into registers: left
         // in memory, they are reloaded per voxel (each reload a scalar-load
         // round trip).
@@ -993,19 +997,26 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                 int x, y, z;
                 float dd[3] = {0.f, 0.f, 0.f};
                 if (STAGE) fval[u] = L.dv[2 * L.dvp + base + u * TILE_THREADS + tt];
-                if (first) {
-                    x = sxyz[u] & 1023;
-                    y = (sxyz[u] >> 10) & 1023;
-                    z = sxyz[u] >> 20;
-                    if (SCORE && !STAGE) fval[u] = fpre[u];
-                    if (MI) fbv[u] = fbpre[u];
-                    if (VF_IN) { dd[0] = dpre[u][0]; dd[1] = dpre[u][1]; dd[2] = dpre[u][2]; }
+                // this thread's slot number (a constant of the unrolled loops
+                // for compile-time edges; at runtime edges only the first
+                // batch's slots are kept)
+                const int su = V > 0 ? base / TILE_THREADS + u : u;
+                if (V > 0 ? su < NSK : first) {
+                    x = sxyz[su] & 1023;
+                    y = (sxyz[su] >> 10) & 1023;
+                    z = sxyz[su] >> 20;
                 } else {
                     const int v = base + u * TILE_THREADS + tt;
                     z = S.div_xy (v);
                     const int r = v - z * (vx * vy);
                     y = S.div_x (r);
                     x = r - y * vx;
+                }
+                if (first) {
+                    if (SCORE && !STAGE) fval[u] = fpre[u];
+                    if (MI) fbv[u] = fbpre[u];
+                    if (VF_IN) { dd[0] = dpre[u][0]; dd[1] = dpre[u][1]; dd[2] = dpre[u][2]; }
+                } else if (!STAGE) {
                     const int fofs = fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
                     if (SCORE && !STAGE) fval[u] = Img<TI>::one (fr, fofs * ES, 0);
                     if (MI) fbv[u] = load_u8 (fr, fofs, 0);
@@ -1085,21 +1096,25 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                         + (int) __umul24 (f[2] + c[2], SXY);
                     nv[u] = Img<TI>::one (mr, roff, 0);
                 } else if (ZPAIR) {
-                    const int off = (int) __umul24 (f[0], EZ) + (int) __umul24 (f[1], SXZ)
-                        + (int) __umul24 (f[2], SXYZ);
+                    // the floor voxel's index; the rounded voxel's (the
+                    // gradient sample of flavor g) is one per-axis step on
+                    const int e = f[0] + (int) __umul24 (f[1], md0) + (int) __umul24 (f[2], md01);
+                    const int off = e * EZ;
                     Img<TI>::quad (a[u][0], a[u][4], a[u][1], a[u][5], zr, off, 0);
                     Img<TI>::quad (a[u][2], a[u][6], a[u][3], a[u][7], zr, off, SXZ);
+                    if (KIND == 0) {
+                        // nearest voxel: the gradient sample of flavor g
+                        // (src/register/bspline_mse.cxx:679-681)
+                        // (e can pass 2^24, so a full multiply by the
+                        // constant, not a 24-bit one)
+                        const int goff = e * (3 * ES) + (c[0] ? 3 * ES : 0) + (c[1] ? 3 * SX : 0)
+                            + (c[2] ? 3 * SXY : 0);
+                        Img<TI>::vec3 (mg[u], gr, goff, 0);
+                    }
                 } else {
                     const int idx = f[0] + (int) __umul24 (md0, f[1]) + (int) __umul24 (md01, f[2]);
                     // the bin quads of slices z and z + 1
                     bin_quads (bn[u], mr, 4 * idx, 4 * md01);
-                }
-                if (KIND == 0) {
-                    // nearest voxel: the gradient sample of flavor g
-                    // (src/register/bspline_mse.cxx:679-681)
-                    const int goff = (int) __umul24 (f[0] + c[0], 3 * ES)
-                        + (int) __umul24 (f[1] + c[1], 3 * SX) + (int) __umul24 (f[2] + c[2], 3 * SXY);
-                    Img<TI>::vec3 (mg[u], gr, goff, 0);
                 }
             }
             if (!VF_OUT) {
