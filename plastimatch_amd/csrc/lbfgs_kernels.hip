@@ -132,11 +132,19 @@ k_lb_finish (const double* __restrict__ part, int ncomp, int nb, const double* _
     const int total = ncomp + 1;
     for (int k0 = 0; k0 < total; k0 += 16) {
         const int k = k0 + (t >> 4), l = t & 15;
-        double s = 0.0;
-        if (k < total) {
-            const double* p = k == 0 ? slope_part : part + (int64_t) (k - 1) * nb;
-            for (int q = l; q < nb; q += 16) s += p[q];
+        // every partial of this thread loaded before any is added (nb <=
+        // LB_MAX_BLOCKS), then added in q order; the absent ones add +0.0,
+        // which leaves a sum that started at +0.0 unchanged
+        const double* p = k == 0 ? slope_part : part + (int64_t) (k - 1) * nb;
+        double pv[LB_MAX_BLOCKS / 16];
+#pragma unroll
+        for (int i = 0; i < LB_MAX_BLOCKS / 16; i++) {
+            const int q = l + 16 * i;
+            pv[i] = (k < total && q < nb) ? p[q] : 0.0;
         }
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < LB_MAX_BLOCKS / 16; i++) s += pv[i];
         mid[t >> 4][l] = s;
         __syncthreads ();
         if (t < 16 && k0 + t < total) {
