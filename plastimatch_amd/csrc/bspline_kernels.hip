@@ -1440,7 +1440,7 @@ pv_quantize (const float w[8], long long q[8])
 // come from the stage's bin images.  Because integer addition is
 // associative, contributions to one bin may be merged before the atomic:
 // equal corner bins of a voxel are merged in registers, and lanes of a
-// wavefront that share a joint (or fixed) bin are summed with a wave
+// wavefront that share a joint bin are summed with a wave
 // reduction and added by one lane -- uniform image regions, where every
 // lane hits the same bin, then cost one LDS atomic per wavefront instead
 // of 64 serialized ones.
@@ -1544,8 +1544,10 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
 #pragma unroll
             for (int n = 1; n < 8; n++)
                 if (b[n] == b[0]) { q[0] += q[n]; q[n] = 0; }
+            // (the fixed histogram is not counted here: a voxel adds exactly
+            // 2^32 to its fixed bin's row, so k_mi_score reads the counts off
+            // the row sums)
             const int row = fb * h.mbins;
-            wave_hist_add (s_f, act, fb, 1ull);
             wave_hist_add (s_j, act, row + b[0], (unsigned long long) q[0]);
 #pragma unroll
             for (int n = 1; n < 8; n++)
@@ -1555,8 +1557,6 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
     __syncthreads ();
     for (int i = threadIdx.x; i < h.fbins * h.mbins; i += blockDim.x)
         if (s_j[i]) atomicAdd (&jhist[i], s_j[i]);
-    for (int i = threadIdx.x; i < h.fbins; i += blockDim.x)
-        if (s_f[i]) atomicAdd (&fhist[i], s_f[i]);
 }
 
 // Score from the integer histograms, and the per-bin dS/dP table for
@@ -1565,7 +1565,7 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
 // (src/register/bspline_mi.txx:96-101).  One workgroup of 1024 threads.
 __global__ __launch_bounds__ (1024)
 void k_mi_score (HGeom h, const unsigned long long* __restrict__ jhist,
-    const unsigned long long* __restrict__ fhist, float* __restrict__ table,
+    unsigned long long* __restrict__ fhist, float* __restrict__ table,
     double* __restrict__ out_score, int64_t* __restrict__ out_nvox,
     float* __restrict__ out_nvox_f, int64_t* __restrict__ hist_out)
 {
@@ -1575,17 +1575,25 @@ void k_mi_score (HGeom h, const unsigned long long* __restrict__ jhist,
     __shared__ long long s_n;
     const int fb = h.fbins, mb = h.mbins, nj = fb * mb;
     const double unit = 1.0 / 4294967296.0;
-    if (threadIdx.x == 0) {
-        long long n = 0;
-        for (int i = 0; i < fb; i++) n += (long long) fhist[i];
-        s_n = n;
+    // the fixed histogram from the joint one: every voxel adds exactly 2^32
+    // to the row of its fixed bin, so a row's sum is its count times 2^32
+    for (int a = threadIdx.x; a < fb; a += blockDim.x) {
+        unsigned long long row = 0;
+        for (int b = 0; b < mb; b++) row += jhist[a * mb + b];
+        fhist[a] = row >> 32;
+        s_f[a] = (double) (row >> 32);
     }
     for (int b = threadIdx.x; b < mb; b += blockDim.x) {
         unsigned long long m = 0;
         for (int a = 0; a < fb; a++) m += jhist[a * mb + b];
         s_m[b] = (double) m * unit;
     }
-    for (int a = threadIdx.x; a < fb; a += blockDim.x) s_f[a] = (double) fhist[a];
+    __syncthreads ();
+    if (threadIdx.x == 0) {
+        long long n = 0;
+        for (int i = 0; i < fb; i++) n += (long long) s_f[i];
+        s_n = n;
+    }
     __syncthreads ();
     const long long N = s_n;
     const double fnv = (double) N;
@@ -1614,7 +1622,7 @@ void k_mi_score (HGeom h, const unsigned long long* __restrict__ jhist,
         if (hist_out) hist_out[jb] = (int64_t) jhist[jb];
     }
     if (hist_out)
-        for (int a = threadIdx.x; a < fb; a += blockDim.x) hist_out[nj + a] = (int64_t) fhist[a];
+        for (int a = threadIdx.x; a < fb; a += blockDim.x) hist_out[nj + a] = (int64_t) s_f[a];
     if (threadIdx.x == 0) {
         *out_score = score;
         *out_nvox = N;
