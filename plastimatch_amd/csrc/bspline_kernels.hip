@@ -1,20 +1,24 @@
 // B-spline score/gradient kernels for gfx950 (MI355X).
 //
-// One workgroup owns one B-spline tile (vpr0 x vpr1 x vpr2 fixed voxels
-// sharing the same 4x4x4 control knots).  Inside the tile:
-//   1. the 64x3 tile coefficients and the per-axis basis LUTs go to LDS;
-//   2. the displacement is contracted separably: T1 = sum_i bx*c, then
-//      T2 = sum_j by*T1, so each voxel only does the last 4-term sum in z;
+// A persistent tile kernel walks the B-spline tiles (vpr0 x vpr1 x vpr2
+// fixed voxels sharing the same 4x4x4 control knots).  Per tile:
+//   1. the 64x3 tile coefficients (projected onto the moving index axes)
+//      and the per-axis basis LUTs are in LDS;
+//   2. the displacement is contracted separably: T1 = sum_i bx*c, then the
+//      per-column terms A = sum_j by*T1, so each voxel only does the last
+//      4-term sum in z;
 //   3. each thread takes G voxels at a time (G in {1, 2, 4}, chosen by the
-//      autotuner) and issues all of their global loads (fixed value,
-//      8 moving corners, moving gradient) before using any, so a wavefront
-//      keeps G gathers per lane in flight;
+//      autotuner) and issues all of their loads before using any: the
+//      eight corners as two 16-byte loads from a z-pair copy of the moving
+//      image, the moving gradient sample; the fixed values come from LDS,
+//      where the tile's rows were staged with wide loads;
 //   4. dc/dv is parked in LDS and condensed into the tile's 64 knot slots by
-//      the transposed separable contraction (x, then y, then z);
-//   5. the 192 slot values go to a knot-major buffer cond[knot][slot][c]
+//      the transposed separable contraction (x, then y, then z), run on the
+//      matrix cores as single-term FMA chains;
+//   5. the 192 slot values go to a slot-major buffer cond[slot][knot][c]
 //      (one writer per entry, no atomics on floats), and a reduce kernel
-//      gives each knot one wavefront: lane = slot, FP64 butterfly sum.
-// The overlap count N and the score are fixed-order sums of per-tile
+//      sums each gradient component's 64 slots in FP64 in a fixed order.
+// The overlap count N and the score are fixed-order sums of per-workgroup
 // partials (integer and FP64); there are no atomics on the MSE path, so the
 // results are bitwise reproducible run to run.
 //
@@ -24,8 +28,9 @@
 // fixed-point integers (order independent) with LDS atomics.
 //
 // The file is compiled with -ffp-contract=off: every a*b+c below rounds
-// twice unless fmaf() is written, which is done only where the reference's
-// rounding is not reproduced anyway (the separable contractions).
+// twice unless fmaf() (or a matrix-core FMA) is written, which is done only
+// where the reference's rounding is not reproduced anyway (the separable
+// contractions).
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
