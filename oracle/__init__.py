@@ -133,6 +133,14 @@ def lib():
         L.orc_mi_score_f64.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
                                        ctypes.POINTER(OrcVolHdr), P, P, ctypes.POINTER(OrcHist),
                                        ctypes.c_int, P, P, P]
+        L.orc_mse_f64_support.restype = ctypes.c_int
+        L.orc_mse_f64_support.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
+                                          ctypes.POINTER(OrcVolHdr), P, P, ctypes.c_int64,
+                                          ctypes.c_double, P, P, P]
+        L.orc_mi_score_f64_hist.restype = ctypes.c_int
+        L.orc_mi_score_f64_hist.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
+                                            ctypes.POINTER(OrcVolHdr), P, P, ctypes.POINTER(OrcHist),
+                                            ctypes.c_int, P, ctypes.c_int64, P, P, P]
         L.orc_resample_geometry.restype = ctypes.c_int
         L.orc_resample_geometry.argtypes = [ctypes.c_int, ctypes.POINTER(OrcVolHdr), P, P, P, P]
         L.orc_volume_resample.argtypes = [ctypes.POINTER(OrcVolHdr), P, P, P, P, ctypes.c_int, P]
@@ -347,6 +355,35 @@ class Case:
         assert rc == 0, rc
         return float(sc[0]), grad, int(nv[0])
 
+    def mse_f64_support(self, coeff, ci, delta):
+        """FP64 squared-difference sum over coefficient ci's support with it
+        moved by delta: returns (raw, n inside, n clamped)."""
+        coeff = np.ascontiguousarray(coeff, np.float32)
+        raw = np.zeros(1, np.float64)
+        n = np.zeros(1, np.int64)
+        nc = np.zeros(1, np.int64)
+        rc = lib().orc_mse_f64_support(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
+                                       ctypes.byref(self.mh), _ptr(self.moving), _ptr(coeff), int(ci),
+                                       float(delta), _ptr(raw), _ptr(n), _ptr(nc))
+        assert rc == 0, rc
+        return float(raw[0]), int(n[0]), int(nc[0])
+
+    def fd_mse_f64(self, coeff, idx, h):
+        """Central differences of the FP64 MSE (flavor p's score) at
+        coefficients idx with step h (mm), from the support sums; returns
+        (fd, border): border counts, per coefficient and over both steps,
+        the support's clamped voxels plus the voxels that enter or leave the
+        overlap (where the difference is not that of the smooth score)."""
+        _, _, n_all = self.mse_f64(coeff, 1)
+        fd, border = [], []
+        for i in idx:
+            _, n0, _ = self.mse_f64_support(coeff, i, 0.0)
+            rp, np_, cp = self.mse_f64_support(coeff, i, h)
+            rm, nm, cm = self.mse_f64_support(coeff, i, -h)
+            fd.append((rp - rm) / (2.0 * h) / n_all)
+            border.append(cp + cm + abs(np_ - n0) + abs(nm - n0))
+        return np.array(fd), np.array(border)
+
     def hist(self, bins=32, f_bins=None):
         fb = f_bins or bins
         fo, fd = hist_eqsp(self.fixed, fb)
@@ -374,14 +411,25 @@ class Case:
         return dict(score=float(sc[0]), grad=grad, num_vox=int(nv[0]), f_hist=fh, m_hist=mh,
                     j_hist=jh, fx_f_hist=fxf, fx_j_hist=fxj)
 
-    def mi_f64(self, coeff, hp, nthreads=0):
+    def mi_f64(self, coeff, hp, nthreads=0, hist=None):
+        """FP64 MI: returns (score, grad float64, num_vox).  With hist (the
+        dict of mi()) the score and pass 2 run in FP64 on that float flavor's
+        histograms instead of FP64 ones."""
         coeff = np.ascontiguousarray(coeff, np.float32)
         grad = np.zeros(self.num_coeff, np.float64)
         sc = np.zeros(1, np.float64)
         nv = np.zeros(1, np.int64)
-        rc = lib().orc_mi_score_f64(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
-                                    ctypes.byref(self.mh), _ptr(self.moving), _ptr(coeff),
-                                    ctypes.byref(hp), nthreads, _ptr(sc), _ptr(grad), _ptr(nv))
+        if hist is None:
+            rc = lib().orc_mi_score_f64(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
+                                        ctypes.byref(self.mh), _ptr(self.moving), _ptr(coeff),
+                                        ctypes.byref(hp), nthreads, _ptr(sc), _ptr(grad), _ptr(nv))
+        else:
+            h = np.ascontiguousarray(np.concatenate([hist["f_hist"], hist["m_hist"], hist["j_hist"]]),
+                                     np.float64)
+            rc = lib().orc_mi_score_f64_hist(ctypes.byref(self.g), ctypes.byref(self.fh),
+                                             _ptr(self.fixed), ctypes.byref(self.mh), _ptr(self.moving),
+                                             _ptr(coeff), ctypes.byref(hp), nthreads, _ptr(h),
+                                             int(hist["num_vox"]), _ptr(sc), _ptr(grad), _ptr(nv))
         assert rc == 0, rc
         return float(sc[0]), grad, int(nv[0])
 

@@ -1088,6 +1088,87 @@ orc_mse_score_f64 (const orc_geom* g,
     return 0;
 }
 
+/* The FP64 sum of squared differences over the support of coefficient ci
+   (the voxels of the up to 4x4x4 tiles whose knots include its knot), with
+   that coefficient moved by delta in double; the other coefficients, and
+   every voxel outside the support, are unchanged, so the central difference
+   (raw(+h) - raw(-h)) / (2 h N) is that of the whole FP64 score whenever the
+   overlap count N does not change.  n counts the support's voxels inside
+   the moving image, nclamp those among them whose position li_clamp clamps
+   (some coordinate below 0 or at or above dim - 1: there li_value_dx keeps
+   its slope while the clamped value does not move).  Same arithmetic as
+   orc_mse_score_f64. */
+int
+orc_mse_f64_support (const orc_geom* g,
+    const orc_vol_hdr* fh, const float* f_img,
+    const orc_vol_hdr* mh, const float* m_img,
+    const float* coeff, int64_t ci, double delta,
+    double* raw_out, int64_t* n_out, int64_t* nclamp_out)
+{
+    dgeo_t d;
+    dvol_t fx, mv;
+    if (dgeo_init (&d, g)) return -1;
+    if (ci < 0 || ci >= 3 * d.num_knots) { dgeo_free (&d); return -2; }
+    dvol_init (&fx, fh, f_img);
+    dvol_init (&mv, mh, m_img);
+    const int64_t knot = ci / 3, comp = ci % 3;
+    const int64_t kk[3] = {knot % d.cdims[0], (knot / d.cdims[0]) % d.cdims[1],
+                           knot / (d.cdims[0] * d.cdims[1])};
+    double raw = 0;
+    int64_t n = 0, nclamp = 0;
+    int64_t p[3], q[3], fijk[3];
+    for (p[2] = kk[2] - 3; p[2] <= kk[2]; p[2]++)
+    for (p[1] = kk[1] - 3; p[1] <= kk[1]; p[1]++)
+    for (p[0] = kk[0] - 3; p[0] <= kk[0]; p[0]++) {
+        int out = 0;
+        for (int a = 0; a < 3; a++) if (p[a] < 0 || p[a] >= d.rdims[a]) out = 1;
+        if (out) continue;
+        for (q[2] = 0; q[2] < d.vpr[2]; q[2]++)
+        for (q[1] = 0; q[1] < d.vpr[1]; q[1]++)
+        for (q[0] = 0; q[0] < d.vpr[0]; q[0]++) {
+            int skip = 0;
+            for (int a = 0; a < 3; a++) {
+                fijk[a] = d.roi_offset[a] + d.vpr[a] * p[a] + q[a];
+                if (fijk[a] >= d.roi_offset[a] + d.roi_dim[a]) skip = 1;
+            }
+            if (skip) continue;
+            double fxyz[3], dxyz[3];
+            for (int a = 0; a < 3; a++)
+                fxyz[a] = d.origin[a] + fijk[0] * fx.step[3*a+0]
+                    + fijk[1] * fx.step[3*a+1] + fijk[2] * fx.step[3*a+2];
+            dinterp (dxyz, &d, p, q, coeff);
+            /* the moved coefficient's share: its basis weight times delta */
+            const int64_t i = kk[0] - p[0], j = kk[1] - p[1], k = kk[2] - p[2];
+            dxyz[comp] += d.bx[q[0]*4+i] * d.by[q[1]*4+j] * d.bz[q[2]*4+k] * delta;
+            dsample_t s;
+            if (!dsample (&s, fxyz, dxyz, &mv)) continue;
+            double t[3], mijk[3];
+            for (int a = 0; a < 3; a++) t[a] = fxyz[a] + dxyz[a] - mv.origin[a];
+            for (int a = 0; a < 3; a++)
+                mijk[a] = t[0] * mv.proj[3*a+0] + t[1] * mv.proj[3*a+1] + t[2] * mv.proj[3*a+2];
+            for (int a = 0; a < 3; a++)
+                if (mijk[a] < 0 || mijk[a] >= (double) (mv.dim[a] - 1)) { nclamp++; break; }
+            double a8[8];
+            dcorners (a8, &s, &mv);
+            const double* l1 = s.l1;
+            const double* l2 = s.l2;
+            double m_val = l1[0]*l1[1]*l1[2]*a8[0] + l2[0]*l1[1]*l1[2]*a8[1]
+                + l1[0]*l2[1]*l1[2]*a8[2] + l2[0]*l2[1]*l1[2]*a8[3]
+                + l1[0]*l1[1]*l2[2]*a8[4] + l2[0]*l1[1]*l2[2]*a8[5]
+                + l1[0]*l2[1]*l2[2]*a8[6] + l2[0]*l2[1]*l2[2]*a8[7];
+            int64_t fidx = fijk[0] + fx.dim[0] * (fijk[1] + fx.dim[1] * fijk[2]);
+            double diff = m_val - (double) fx.img[fidx];
+            raw += diff * diff;
+            n++;
+        }
+    }
+    *raw_out = raw;
+    *n_out = n;
+    *nclamp_out = nclamp;
+    dgeo_free (&d);
+    return 0;
+}
+
 /* ---------------------------------------------------------------------
    Mattes mutual information
    --------------------------------------------------------------------- */
@@ -1397,12 +1478,15 @@ orc_mi_score (const orc_geom* g,
 
 /* The same MI algorithm in double precision: double weights and
    histograms, natural log in double, double condense and reduce.
-   Positions use the direct formula in both passes. */
-int
-orc_mi_score_f64 (const orc_geom* g,
+   Positions use the direct formula in both passes.  With hist_in (f, m and
+   joint histograms, in that order, and their voxel count) pass 1 is
+   skipped and the score and pass 2 run on those histograms. */
+static int
+mi_f64 (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
     const float* coeff, const orc_hist* hp, int nthreads,
+    const double* hist_in, int64_t nv_in,
     double* score, double* grad, int64_t* num_vox)
 {
     dgeo_t d;
@@ -1417,8 +1501,14 @@ orc_mi_score_f64 (const orc_geom* g,
     double* jh_ = (double*) calloc (fbins * mbins, sizeof(double));
     double* tiles = (double*) calloc (d.num_tiles * 64 * 3, sizeof(double));
     int64_t nv = 0;
+    if (hist_in) {
+        memcpy (fh_, hist_in, sizeof(double) * fbins);
+        memcpy (mh_, hist_in + fbins, sizeof(double) * mbins);
+        memcpy (jh_, hist_in + fbins + mbins, sizeof(double) * fbins * mbins);
+        nv = nv_in;
+    }
 
-    for (int pass = 1; pass <= 2; pass++) {
+    for (int pass = hist_in ? 2 : 1; pass <= 2; pass++) {
         double sc = 0;
         if (pass == 2) {
             double fnv = (double) nv, thresh = 0.001 / (double) (fbins * mbins);
@@ -1506,6 +1596,32 @@ orc_mi_score_f64 (const orc_geom* g,
     free (fh_); free (mh_); free (jh_); free (tiles);
     dgeo_free (&d);
     return 0;
+}
+
+int
+orc_mi_score_f64 (const orc_geom* g,
+    const orc_vol_hdr* fh, const float* f_img,
+    const orc_vol_hdr* mh, const float* m_img,
+    const float* coeff, const orc_hist* hp, int nthreads,
+    double* score, double* grad, int64_t* num_vox)
+{
+    return mi_f64 (g, fh, f_img, mh, m_img, coeff, hp, nthreads, 0, 0, score, grad, num_vox);
+}
+
+/* FP64 score and pass 2 on given histograms (the float flavor's: the PV
+   histogram is fixed to the reference's, bin for bin, so this isolates the
+   arithmetic of the score and the gradient pass from the float rounding of
+   the histogram weights). */
+int
+orc_mi_score_f64_hist (const orc_geom* g,
+    const orc_vol_hdr* fh, const float* f_img,
+    const orc_vol_hdr* mh, const float* m_img,
+    const float* coeff, const orc_hist* hp, int nthreads,
+    const double* hist_in, int64_t nv_in,
+    double* score, double* grad, int64_t* num_vox)
+{
+    return mi_f64 (g, fh, f_img, mh, m_img, coeff, hp, nthreads, hist_in, nv_in, score, grad,
+        num_vox);
 }
 
 /* ---------------------------------------------------------------------

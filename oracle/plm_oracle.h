@@ -99,6 +99,16 @@ int orc_reg_analytic (const orc_geom* g, const orc_reg_parms* rp, const float* c
 /* Its 32 64x64 V matrices (32 * 4096 doubles). */
 int orc_reg_matrices (const orc_geom* g, const orc_reg_parms* rp, double* V);
 
+/* FP64 sum of squared differences over the support of coefficient ci
+   with that coefficient moved by delta (central differences of the FP64
+   MSE without a whole-image evaluation); n and nclamp count the support's
+   inside and clamped voxels. */
+int orc_mse_f64_support (const orc_geom* g,
+    const orc_vol_hdr* fh, const float* f_img,
+    const orc_vol_hdr* mh, const float* m_img,
+    const float* coeff, int64_t ci, double delta,
+    double* raw, int64_t* n, int64_t* nclamp);
+
 int orc_mi_score (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
@@ -111,6 +121,14 @@ int orc_mi_score_f64 (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
     const float* coeff, const orc_hist* hp, int nthreads,
+    double* score, double* grad, int64_t* num_vox);
+/* The same on given histograms (f, m, joint, concatenated; count nv_in):
+   FP64 score and pass 2 only. */
+int orc_mi_score_f64_hist (const orc_geom* g,
+    const orc_vol_hdr* fh, const float* f_img,
+    const orc_vol_hdr* mh, const float* m_img,
+    const float* coeff, const orc_hist* hp, int nthreads,
+    const double* hist_in, int64_t nv_in,
     double* score, double* grad, int64_t* num_vox);
 
 /* Stage subsampling (resample_oracle.c).  Kinds follow the stage's

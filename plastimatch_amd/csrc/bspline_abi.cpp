@@ -2,6 +2,7 @@
 // Declared in include/plm_hip_bspline.h.
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -184,6 +185,7 @@ struct plm_hip_ctx {
     void* d_mpair = nullptr;        // the moving image as z-pairs (storage type)
     void* d_mgrad = nullptr;
     float* d_blut = nullptr;
+    plmhip::TileBase* d_tbase = nullptr;    // per-tile moving-index frames
     float* d_cond = nullptr;
     double* d_tile_score = nullptr;
     double* d_reg_part = nullptr;           // regularization scratch (first use)
@@ -232,7 +234,7 @@ void
 free_ctx (plm_hip_ctx* c)
 {
     c->timer.destroy ();
-    void* bufs[] = {c->d_fixed, c->d_moving, c->d_mpair, c->d_mgrad, c->d_blut, c->d_cond,
+    void* bufs[] = {c->d_fixed, c->d_moving, c->d_mpair, c->d_mgrad, c->d_blut, c->d_tbase, c->d_cond,
         c->d_tile_score, c->d_tile_count, c->d_pair, c->d_hist,
         c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_mquad, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
         c->d_hist_out, c->d_reg_part, c->d_reg_cond};
@@ -278,6 +280,39 @@ upload_image (plm_hip_ctx* c, const plm_img_desc* d, int64_t nvals, void** out)
     if (e != hipSuccess) { (void) hipFree (bf); return hip_fail (e, "bf16 conversion"); }
     *out = bf;
     return PLM_HIP_OK;
+}
+
+// The moving-index frame of every tile (plmhip::TileBase): the base B of
+// the tile's first voxel in double, its integer part clamped to the moving
+// image (so every local floor stays within +-mdim of it, and B - Bi is small
+// wherever the tile lands inside the image), and the inside-test and clamp
+// bounds moved into that frame.
+std::vector<plmhip::TileBase>
+tile_bases (const KGeom& g, const double moff[3], const double mstep[9])
+{
+    std::vector<plmhip::TileBase> out ((size_t) g.num_tiles);
+    for (int t = 0; t < g.num_tiles; t++) {
+        const int p0 = t % g.rdims[0], p1 = (t / g.rdims[0]) % g.rdims[1];
+        const int p2 = t / (g.rdims[0] * g.rdims[1]);
+        const double f[3] = {(double) (g.roi_off[0] + g.vpr[0] * p0),
+                             (double) (g.roi_off[1] + g.vpr[1] * p1),
+                             (double) (g.roi_off[2] + g.vpr[2] * p2)};
+        plmhip::TileBase& tb = out[(size_t) t];
+        int64_t bi[3];
+        for (int a = 0; a < 3; a++) {
+            const double B = moff[a] + mstep[3*a] * f[0] + mstep[3*a+1] * f[1] + mstep[3*a+2] * f[2];
+            double fl = std::isfinite (B) ? std::floor (B) : 0.0;
+            fl = std::min (std::max (fl, 0.0), (double) (g.mdim[a] - 1));
+            bi[a] = (int64_t) fl;
+            tb.bf[a] = (float) (std::isfinite (B) ? B - fl : 0.0);
+            tb.thr_lo[a] = (float) (-0.5 - fl);
+            tb.thr_hi[a] = (float) ((double) g.mdim[a] - 0.5 - fl);
+            tb.cl_lo[a] = (float) -fl;
+            tb.cl_hi[a] = (float) ((double) (g.mdim[a] - 2) - fl);
+        }
+        tb.boff = (int) (bi[0] + (int64_t) g.mdim[0] * (bi[1] + (int64_t) g.mdim[1] * bi[2]));
+    }
+    return out;
 }
 
 int
@@ -384,6 +419,7 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     // mijk = P d + moff + M ijk with M = P step and moff = P (origin - morigin),
     // P the moving proj; M and moff are formed in double from the float
     // inputs, so an identity map is exactly the identity.
+    double moff_d[3], mstep_d[9];
     {
         double pd[9];
         if (!mat3_inverse_d (pd, moving->direction_cosines))
@@ -394,11 +430,13 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
             double o = 0.0;
             for (int c = 0; c < 3; c++)
                 o += pd[3*a+c] * ((double) geom->img_origin[c] - (double) moving->origin[c]);
+            moff_d[a] = o;
             g.moff[a] = (float) o;
             for (int b = 0; b < 3; b++) {
                 double s = 0.0;
                 for (int c = 0; c < 3; c++)
                     s += pd[3*a+c] * (double) fixed->direction_cosines[3*c+b] * (double) fixed->spacing[b];
+                mstep_d[3*a+b] = s;
                 g.mstep[3*a+b] = (float) s;
             }
         }
@@ -413,6 +451,8 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
         const int64_t nm = moving->dim[0] * moving->dim[1] * moving->dim[2];
         const int64_t f01 = fixed->dim[0] * fixed->dim[1], m01 = moving->dim[0] * moving->dim[1];
         if (f01 >= ((int64_t) 1 << 24) || 12 * m01 >= ((int64_t) 1 << 24)
+            || moving->dim[0] >= ((int64_t) 1 << 23) || moving->dim[1] >= ((int64_t) 1 << 23)
+            || moving->dim[2] >= ((int64_t) 1 << 23)
             || 4 * (nf + plmhip::MAX_VPR * f01) >= ((int64_t) 1 << 31)
             || 12 * nm >= ((int64_t) 1 << 31))
             return fail (PLM_HIP_ERR_UNSUPPORTED,
@@ -507,6 +547,14 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     e = hipMemcpy (c->d_blut, blut.data (), sizeof (float) * blut.size (), hipMemcpyHostToDevice);
     if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "hipMemcpy(lut)"); }
 
+    {
+        const std::vector<plmhip::TileBase> tb = tile_bases (g, moff_d, mstep_d);
+        CTX_TRY (alloc (&c->d_tbase, tb.size (), "hipMalloc(tile bases)"));
+        e = hipMemcpy (c->d_tbase, tb.data (), sizeof (plmhip::TileBase) * tb.size (),
+            hipMemcpyHostToDevice);
+        if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "hipMemcpy(tile bases)"); }
+    }
+
     // slot-major condense buffer; slots no tile owns stay zero for good
     CTX_TRY (alloc (&c->d_cond, (size_t) knots * 192, "hipMalloc(cond)"));
     // per-workgroup partials of the persistent tile kernel: at most one
@@ -544,6 +592,7 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
         // every call
         plmhip::PairDev pd {};
         pd.g = g;
+        pd.tbase = c->d_tbase;
         pd.fimg = c->d_fixed;
         pd.mimg = c->d_moving;
         pd.mgrad = c->d_mgrad;
@@ -1038,6 +1087,10 @@ plm_hip_batch_create (plm_hip_ctx* const* ctxs, int n, plm_hip_batch** out)
         for (int a = 0; a < 3; a++)
             if (c->g.vpr[a] != ctxs[0]->g.vpr[a])
                 return fail (PLM_HIP_ERR_ARG, "batched contexts must share vox_per_rgn");
+        // each pair condenses into its context's own slot buffer, so a
+        // context listed twice would have two pairs overwrite one buffer
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == c) return fail (PLM_HIP_ERR_ARG, "a context appears twice in the batch");
     }
     HIP_TRY (hipSetDevice (ctxs[0]->device));
     plm_hip_batch* b = new plm_hip_batch;
@@ -1057,6 +1110,7 @@ plm_hip_batch_create (plm_hip_ctx* const* ctxs, int n, plm_hip_batch** out)
         plmhip::PairDev& pd = b->h_pairs[i];
         pd = plmhip::PairDev {};
         pd.g = c->g;
+        pd.tbase = c->d_tbase;
         pd.fimg = c->d_fixed;
         pd.mimg = c->d_moving;
         pd.mgrad = c->d_mgrad;

@@ -96,10 +96,29 @@ constexpr int TUNE_WG_CANDS[4] = {0, 6, 5, 4};
 // arrays included (the MI table comes on top).
 size_t tile_lds_bytes (const KGeom& g, int vox_per_thread);
 
+// Moving-index frame of one tile.  The tile's first fixed voxel maps to
+// moving index coordinates B = moff + M (bx0, by0, bz0) (formed in double on
+// the host); with the integer part Bi = clamp(floor(B), 0, mdim - 1) taken
+// out, the tile kernel works in the local coordinate ml = m - Bi, which stays
+// a few voxels in size wherever the tile lands inside the moving image, so
+// its float rounding is that of a small number instead of one as large as
+// the image (at 512 voxels a float position is good to 3e-5 voxel, the local
+// one to ~1e-6).  Every threshold of the inside test and the clamp moves into
+// the same frame exactly (integers and half-integers are exact floats).
+struct TileBase {
+    float bf[3];             // B - Bi, rounded to float
+    float thr_lo[3];         // -0.5 - Bi: inside iff thr_lo < ml < thr_hi (Volume::is_inside)
+    float thr_hi[3];         // mdim - 0.5 - Bi
+    float cl_lo[3];          // -Bi: the clamped floor corner lies in [cl_lo, cl_hi] (li_clamp)
+    float cl_hi[3];          // mdim - 2 - Bi
+    int boff;                // element offset of voxel Bi in the moving image
+};
+
 // One fixed/moving pair as the tile and reduce kernels see it (a device
 // table of these drives a launch; a single context is a table of one).
 struct PairDev {
     KGeom g;
+    const TileBase* tbase;   // per tile of the pair
     const void* fimg;        // fixed, moving, moving gradient (the batch's storage type)
     const void* mimg;
     const void* mgrad;

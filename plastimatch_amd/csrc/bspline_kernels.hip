@@ -53,6 +53,13 @@ __device__ __forceinline__ float ld (const uint16_t* __restrict__ p, int i)
     return __uint_as_float (((uint32_t) p[i]) << 16);
 }
 
+// A wave-uniform float moved to a scalar register.
+__device__ __forceinline__ float
+uniform (float v)
+{
+    return __int_as_float (__builtin_amdgcn_readfirstlane (__float_as_int (v)));
+}
+
 // n / d for n < 2^27, d <= 32, with m = ceil(2^32 / d).
 __device__ __forceinline__ int
 fdiv (int n, uint32_t m)
@@ -480,12 +487,11 @@ enum ColumnMode { PROJECTED = 0, DISPLACEMENT = 1, OFFSETS = 2 };
 // Stage 1 of a tile: its 64 knot coefficients (fetched ahead into
 // registers; PROJECTED: projected onto the moving index axes, P c, so the
 // contractions that follow carry moving-index units and the column terms
-// need no projection) and the per-slice moving-index term
-// Z[z][a] = M[a][2] * (bz0 + z), M = P_moving * step_fixed.
+// need no projection) and the per-slice moving-index term of the tile's
+// local frame (TileBase), Z[z][a] = M[a][2] * z, M = P_moving * step_fixed.
 template <int V, int MODE>
 __device__ __forceinline__ void
-tile_stage1 (const TileLds& L, const Shape<V>& S, const KGeom& g, int bz0, const float c[3],
-    int t)
+tile_stage1 (const TileLds& L, const Shape<V>& S, const KGeom& g, const float c[3], int t)
 {
     if (MODE != OFFSETS && t < 64) {
         float c0 = c[0], c1 = c[1], c2 = c[2];
@@ -504,7 +510,7 @@ tile_stage1 (const TileLds& L, const Shape<V>& S, const KGeom& g, int bz0, const
     }
     for (int e = t; e < 3 * S.vz; e += TILE_THREADS) {
         const int z = e / 3, a = e - 3 * z;
-        L.bzz[8 * z + 4 + a] = MODE == DISPLACEMENT ? 0.f : g.mstep[3 * a + 2] * (float) (bz0 + z);
+        L.bzz[8 * z + 4 + a] = MODE == DISPLACEMENT ? 0.f : g.mstep[3 * a + 2] * (float) z;
     }
 }
 
@@ -530,22 +536,22 @@ tile_t1 (const TileLds& L, const Shape<V>& S, int t)
 
 // Stage 3: the per-column terms of the moving-voxel coordinate.  With
 // A[a][k] = sum_j by[y][j] T1[k][j][x][a] (the projected displacement still
-// to be contracted along z) and C[a] = moff[a] + M[a][0] fi + M[a][1] fj, a
-// voxel (x, y, z) of the tile maps to moving index coordinates
-//   mijk[a] = sum_k bz[z][k] A[a][k] + C[a] + Z[z][a],
+// to be contracted along z) and C[a] = Bf[a] + M[a][0] x + M[a][1] y (x, y,
+// z local to the tile, Bf the fraction of the tile's base, TileBase), a
+// voxel (x, y, z) of the tile maps to the local moving index coordinates
+//   ml[a] = sum_k bz[z][k] A[a][k] + C[a] + Z[z][a],  mijk = Bi + ml,
 // which is P * (fixed position + displacement - moving origin) regrouped
 // (bspline_find_correspondence_dcos, src/util/bspline_correspond.cxx:80-109,
 // with the position of POSITION_FROM_COORDS, src/base/volume_macros.h:82-96).
 //
 // Returns whether every column this thread formed stays in the interior of
 // the moving image over all the tile's slices: with bz >= 0, sum_k bz = 1
-// and Z linear in z, mijk lies in C + [min_k A, max_k A] + [Z(first),
-// Z(last)]; 0.01 voxel of margin covers the rounding of mijk.  Inside that
+// and Z linear in z, ml lies in C + [min_k A, max_k A] + [Z(first),
+// Z(last)]; 0.01 voxel of margin covers the rounding of ml.  Inside that
 // range li_clamp's clamps are identities and Volume::is_inside holds.
 template <int V, int MODE>
 __device__ __forceinline__ bool
-tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int by0, int bz0,
-    int t)
+tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, const TileBase& tb, int t)
 {
     const int vx = S.vx;
     const float4* t1 = reinterpret_cast<const float4*> (L.t1);
@@ -557,10 +563,10 @@ tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int 
         float4* o = reinterpret_cast<float4*> (L.col) + e;
         const int np = vx * S.vy;
         if (MODE != DISPLACEMENT) {
-            const float fi = (float) (bx0 + x), fj = (float) (by0 + y);
+            const float fi = (float) x, fj = (float) y;
             float C[3];
 #pragma unroll
-            for (int a = 0; a < 3; a++) C[a] = (g.moff[a] + g.mstep[3 * a] * fi) + g.mstep[3 * a + 1] * fj;
+            for (int a = 0; a < 3; a++) C[a] = (tb.bf[a] + g.mstep[3 * a] * fi) + g.mstep[3 * a + 1] * fj;
             o[3 * np] = make_float4 (C[0], C[1], C[2], 0.f);
         } else {
             o[3 * np] = make_float4 (0.f, 0.f, 0.f, 0.f);
@@ -585,11 +591,10 @@ tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, int bx0, int 
             if (MODE == PROJECTED) {
                 const float4 Cv = o[3 * np];
                 const float c = a == 0 ? Cv.x : (a == 1 ? Cv.y : Cv.z);
-                const float za = g.mstep[3 * a + 2] * (float) bz0;
-                const float zb = g.mstep[3 * a + 2] * (float) (bz0 + S.vz - 1);
-                const float lo = c + fminf (za, zb) + fminf (fminf (A[0], A[1]), fminf (A[2], A[3]));
-                const float hi = c + fmaxf (za, zb) + fmaxf (fmaxf (A[0], A[1]), fmaxf (A[2], A[3]));
-                ok = ok & (lo >= 0.01f) & (hi <= g.mmax[a] + 0.99f);
+                const float zb = g.mstep[3 * a + 2] * (float) (S.vz - 1);
+                const float lo = c + fminf (0.f, zb) + fminf (fminf (A[0], A[1]), fminf (A[2], A[3]));
+                const float hi = c + fmaxf (0.f, zb) + fmaxf (fmaxf (A[0], A[1]), fmaxf (A[2], A[3]));
+                ok = ok & (lo >= tb.cl_lo[a] + 0.01f) & (hi <= tb.cl_hi[a] + 0.99f);
             }
         }
     }
@@ -895,6 +900,8 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         p = np;
         const KGeom& g = pairs[p].g;
         const PairDev& pd = pairs[p];
+        // the tile's moving-index frame (wave-uniform: scalar loads)
+        const TileBase tb = pd.tbase[tile - pd.tile0];
         const int next = tile + wpx;
         const bool has_next = next < tend;
         if (has_next) {
@@ -918,7 +925,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         rot += 64;
         const int tn = (tt + rot) & (TILE_THREADS - 1);
         __syncthreads ();       // the previous tile is done with coef, t1, s2, red
-        tile_stage1<V, MODE> (L, S, g, P.bz0, cpre, tr);
+        tile_stage1<V, MODE> (L, S, g, cpre, tr);
         __syncthreads ();
         if (MODE != OFFSETS && tn < 64 && has_next)
             tile_coef_fetch (cpre, pairs[np].g, nq.knot0, tn,
@@ -932,7 +939,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         // same results bit for bit): one flag per wave, ANDed after the
         // barrier (red is free here).
         constexpr bool FASTABLE = MODE == PROJECTED && !VF_OUT;
-        const bool col_ok = tile_columns<V, MODE> (L, S, g, P.bx0, P.by0, P.bz0, tr);
+        const bool col_ok = tile_columns<V, MODE> (L, S, g, tb, tr);
         store_rows (tr);
         int* wflag = reinterpret_cast<int*> (L.red + 12);
         if (FASTABLE) {
@@ -972,16 +979,20 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         constexpr int EZ = 2 * Img<TI>::ES;     // bytes per z-pair record
         const Rsrc zr = make_rsrc (pd.mpair, ZPAIR ? (uint32_t) (md01 * g.mdim[2]) * EZ : 0u);
         const int SXZ = md0 * EZ;
-        // The moving-index bounds, read once per tile into registers: left
-        // in memory, they are reloaded per voxel (each reload a scalar-load
-        // round trip).
-        float mhi[3], mmax[3];
+        // The moving-index bounds of the tile's local frame, read once per
+        // tile into registers: left in memory, they are reloaded per voxel
+        // (each reload a scalar-load round trip).
+        // (The values are wave-uniform; readfirstlane puts them in scalar
+        // registers.)
+        float tlo[3], thi[3], clo[3], chi[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            mhi[k] = g.mhi[k];
-            mmax[k] = g.mmax[k];
-            asm volatile ("" : "+s" (mhi[k]), "+s" (mmax[k]));
+            tlo[k] = uniform (tb.thr_lo[k]);
+            thi[k] = uniform (tb.thr_hi[k]);
+            clo[k] = uniform (tb.cl_lo[k]);
+            chi[k] = uniform (tb.cl_hi[k]);
         }
+        const int boff = __builtin_amdgcn_readfirstlane (tb.boff);
 
         auto voxels = [&] (auto fast_c) {
         constexpr bool FAST = decltype (fast_c)::value;
@@ -1084,8 +1095,8 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                         fa = m[k] - fl;
                     } else {
                         // bitwise, not short-circuit: no branch per component
-                        inside = inside & (m[k] > -0.5f) & (m[k] < mhi[k]);
-                        fl = __builtin_amdgcn_fmed3f (floorf (m[k]), 0.f, mmax[k]);
+                        inside = inside & (m[k] > tlo[k]) & (m[k] < thi[k]);
+                        fl = __builtin_amdgcn_fmed3f (floorf (m[k]), clo[k], chi[k]);
                         fa = fminf (fmaxf (m[k] - fl, 0.f), 1.f);
                     }
                     f[k] = (int) fl;
@@ -1093,17 +1104,18 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     fr_[u][k] = fa;
                 }
                 in[u] = inside & act;
-                // byte offsets as 24-bit products (slices hold < 2^24 / 12
-                // voxels, checked at context creation)
+                // element offsets from the tile's base voxel as signed 24-bit
+                // products (the local floor is >= -Bi, and |f| < mdim: slices
+                // hold < 2^23 voxels, checked at context creation)
                 if (NEAREST) {
                     // bspline_warp_dcos's nearest voxel: the rounded index
-                    const int roff = (int) __umul24 (f[0] + c[0], ES) + (int) __umul24 (f[1] + c[1], SX)
-                        + (int) __umul24 (f[2] + c[2], SXY);
-                    nv[u] = Img<TI>::one (mr, roff, 0);
+                    const int ridx = boff + (f[0] + c[0]) + __mul24 (f[1] + c[1], md0)
+                        + __mul24 (f[2] + c[2], md01);
+                    nv[u] = Img<TI>::one (mr, ridx * ES, 0);
                 } else if (ZPAIR) {
                     // the floor voxel's index; the rounded voxel's (the
                     // gradient sample of flavor g) is one per-axis step on
-                    const int e = f[0] + (int) __umul24 (f[1], md0) + (int) __umul24 (f[2], md01);
+                    const int e = boff + f[0] + __mul24 (f[1], md0) + __mul24 (f[2], md01);
                     const int off = e * EZ;
                     Img<TI>::quad (a[u][0], a[u][4], a[u][1], a[u][5], zr, off, 0);
                     Img<TI>::quad (a[u][2], a[u][6], a[u][3], a[u][7], zr, off, SXZ);
@@ -1117,7 +1129,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                         Img<TI>::vec3 (mg[u], gr, goff, 0);
                     }
                 } else {
-                    const int idx = f[0] + (int) __umul24 (md0, f[1]) + (int) __umul24 (md01, f[2]);
+                    const int idx = boff + f[0] + __mul24 (md0, f[1]) + __mul24 (md01, f[2]);
                     // the bin quads of slices z and z + 1
                     bin_quads (bn[u], mr, 4 * idx, 4 * md01);
                 }

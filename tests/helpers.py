@@ -32,27 +32,70 @@ def oracle_case(pair, **kw):
 
 
 def assert_parity(score, grad, ref, ref64, tol_s=1e-5, tol_g=1e-4, label=""):
-    """GPU result against the float reference flavor (ref = (score, grad)) and
-    the FP64 oracle (ref64).  The bars are 1e-5 (score) and 1e-4 (gradient,
-    relative L2).  Where the float flavor is itself farther than the bar from
-    FP64 -- a nearest-voxel rounding flip of flavor g's round(mijk), or
-    cancellation in MI's float PV derivative -- the bar becomes 1.5 times
-    that own error: the GPU must then be as close to FP64 as the reference
-    flavor is."""
+    """GPU result against the FP64 oracle (ref64 = (score, grad)) and the
+    float reference flavor (ref).  Fixed bars: the score within tol_s (1e-5)
+    relative and the gradient within tol_g (1e-4) relative L2 of FP64,
+    always.  Against the float flavor the same bars hold wherever that flavor
+    is itself within them of FP64; where it is not (its own float rounding
+    -- MI's voxel positions at large images, a nearest-voxel flip of flavor
+    g -- puts it farther), the distance is printed and returned, not
+    asserted.  Returns the four distances."""
     s_ref, g_ref = ref
     s64, g64 = ref64
     own_s = rel_scalar(s_ref, s64)
     own_g = rel(g_ref, g64)
-    bar_s = max(tol_s, 1.5 * own_s)
-    bar_g = max(tol_g, 1.5 * own_g)
     es, eg = rel_scalar(score, s64), rel(grad, g64)
+    ers, erg = rel_scalar(score, s_ref), rel(grad, g_ref)
     print(f"{label} score err vs f64 {es:.2e} (reference flavor {own_s:.2e}), "
-          f"grad err vs f64 {eg:.2e} (reference flavor {own_g:.2e}), vs reference "
-          f"{rel(grad, g_ref):.2e}")
-    assert es < bar_s, (es, bar_s)
-    assert rel_scalar(score, s_ref) < bar_s
-    assert eg < bar_g, (eg, bar_g)
-    assert rel(grad, g_ref) < bar_g, (rel(grad, g_ref), bar_g)
+          f"grad err vs f64 {eg:.2e} (reference flavor {own_g:.2e}); vs reference flavor: "
+          f"score {ers:.2e}, grad {erg:.2e}")
+    assert es < tol_s, (es, tol_s)
+    assert eg < tol_g, (eg, tol_g)
+    if own_s < tol_s:
+        assert ers < tol_s, (ers, tol_s)
+    if own_g < tol_g:
+        assert erg < tol_g, (erg, tol_g)
+    return dict(score_f64=es, grad_f64=eg, score_ref=ers, grad_ref=erg, ref_score_f64=own_s,
+                ref_grad_f64=own_g)
+
+
+def assert_parity_mi(score, grad, ref, full64, hist64, tol_s=1e-5, tol_g=1e-4, label=""):
+    """Mattes MI of the GPU against the oracle.  The PV histogram is pinned
+    to the float reference flavor's bin for bin (tested separately), and MI
+    depends on it discontinuously (the score and the gradient drop joint
+    bins at or below fixed thresholds, src/register/bspline_mi.cxx:285-310,
+    bspline_mi.txx:96-101), so the yardstick always asserted is hist64: the
+    FP64 score and gradient pass computed on that float histogram.  The full
+    FP64 oracle (full64, FP64 histogram too) is asserted wherever hist64 is
+    itself within the bars of it, i.e. wherever the histogram's float
+    weights are not what separates them; the float reference flavor (ref)
+    wherever it is within the bars of hist64.  Returns the distances."""
+    s_h, g_h = hist64
+    s64, g64 = full64
+    s_ref, g_ref = ref
+    hist_s, hist_g = rel_scalar(s_h, s64), rel(g_h, g64)
+    print(f"{label} FP64 on the reference histogram vs full FP64: score {hist_s:.2e}, grad {hist_g:.2e}")
+    d = assert_parity(score, grad, ref, hist64, tol_s, tol_g, label=label + " [vs FP64 on its histogram]")
+    es, eg = rel_scalar(score, s64), rel(grad, g64)
+    print(f"{label} vs full FP64: score {es:.2e}, grad {eg:.2e}")
+    if hist_s < tol_s:
+        assert es < tol_s, (es, tol_s)
+    if hist_g < tol_g:
+        assert eg < tol_g, (eg, tol_g)
+    d.update(score_full64=es, grad_full64=eg, hist_score_full64=hist_s, hist_grad_full64=hist_g)
+    return d
+
+
+def assert_count(n, n_ref, n64, label=""):
+    """Overlap count N of the GPU against the FP64 oracle's: equal.  The
+    float reference flavor decides Volume::is_inside on float positions
+    (good to ~1e-5 voxel at these image sizes), so a voxel that close to
+    the (-0.5, dim - 0.5) boundary may fall on the other side there: its
+    count is printed, and may differ by a few such voxels."""
+    if n != n_ref:
+        print(f"{label} overlap count {n} (FP64 {n64}); the float reference flavor counts {n_ref}")
+    assert n == n64, (label, n, n64)
+    assert abs(n - n_ref) <= max(2, n64 // 10000), (label, n, n_ref)
 
 
 def reference_synth(pattern, dim=38, volume_size=50.0, origin=-25.0, background=None, foreground=None,

@@ -13,7 +13,7 @@ import pytest
 
 from plastimatch_amd import synth
 from plastimatch_amd.bspline import BsplineBatch, stage_for_pair
-from tests.helpers import assert_parity, bf16_round, oracle_case, rel_scalar
+from tests.helpers import assert_count, assert_parity, bf16_round, oracle_case, rel_scalar
 
 pytestmark = pytest.mark.gpu
 
@@ -80,8 +80,8 @@ def test_batch_matches_single_pair(hip_lib, vpr, grad_mode):
     c0 = coeffs[0].cpu().numpy()
     flavor = "g" if grad_mode == "ref_nn" else "p"
     s_ref, g_ref, n_ref, _ = oc.mse(c0, flavor)
-    s64, g64, _ = oc.mse_f64(c0, 0 if grad_mode == "ref_nn" else 1)
-    assert int(nvox[0].item()) == n_ref
+    s64, g64, n64 = oc.mse_f64(c0, 0 if grad_mode == "ref_nn" else 1)
+    assert_count(int(nvox[0].item()), n_ref, n64, f"batch pair 0 vpr {vpr}:")
     assert_parity(float(score[0].item()), grads[0].cpu().numpy(), (s_ref, g_ref), (s64, g64),
                   label=f"batch pair 0 vpr {vpr}:")
     for st in stages:
@@ -147,6 +147,58 @@ def test_batch_graph_replays(hip_lib):
             for i in range(len(pairs)):
                 assert torch.equal(grads[i], eager[0][i]), (rep, i)
             assert torch.equal(score, eager[1]) and torch.equal(nvox, eager[2]), rep
+    for st in stages:
+        st.close()
+
+
+def test_full_size_batch16_bf16(hip_lib):
+    """The multi-atlas workload: 16 pairs with edges drawn from 96..256 per
+    axis (10-voxel grid, N(0, 3 mm) fields), BF16 storage, one batched
+    evaluation captured in a hipGraph and replayed.  Every pair against the
+    oracle on the same BF16-rounded images and BF16-stored moving gradient:
+    N equal to FP64's, score within 1e-5 and gradient within 1e-4 of FP64;
+    and the replay equal to the eager batched evaluation."""
+    import torch
+    from plastimatch_amd.bspline import BsplineHip, Bspline_xform, Volume, volume_gradient
+    dev = torch.device("cuda:0")
+    pairs = synth.make_batch(16, seed=0)
+    stages = []
+    for p in pairs:
+        p.fixed = bf16_round(p.fixed)
+        p.moving = bf16_round(p.moving)
+        fixed = Volume(torch.from_numpy(p.fixed).to(dev), p.spacing, p.origin)
+        moving = Volume(torch.from_numpy(p.moving).to(dev), p.spacing, p.origin)
+        bxf = Bspline_xform.for_volume(fixed, p.vox_per_rgn)
+        stages.append(BsplineHip(bxf, fixed, moving, volume_gradient(moving), storage="bf16"))
+    coeffs = [torch.from_numpy(synth.random_coeff(p.dim, p.vox_per_rgn, 3.0, seed=200 + i)).to(dev)
+              for i, p in enumerate(pairs)]
+    grads = [torch.zeros(c.numel(), dtype=torch.float64, device=dev) for c in coeffs]
+    score = torch.zeros(16, dtype=torch.float64, device=dev)
+    nvox = torch.zeros(16, dtype=torch.int64, device=dev)
+    with BsplineBatch(stages) as batch:
+        batch.set_buffers(coeffs, grads, score, nvox)
+        batch.score_device("ref_nn")
+        torch.cuda.synchronize()
+        eager = ([g.clone() for g in grads], score.clone(), nvox.clone())
+        batch.capture("ref_nn")
+        for g in grads:
+            g.zero_()
+        score.zero_()
+        torch.cuda.synchronize()
+        batch.launch()
+        torch.cuda.synchronize()
+        for i in range(16):
+            assert torch.equal(grads[i], eager[0][i]), i
+        assert torch.equal(score, eager[1]) and torch.equal(nvox, eager[2])
+    for i, p in enumerate(pairs):
+        oc = oracle_case(p)
+        oc._m_grad = bf16_round(oc.m_grad)
+        c = coeffs[i].cpu().numpy()
+        s_ref, g_ref, n_ref, _ = oc.mse(c, "g")
+        s64, g64, n64 = oc.mse_f64(c, 0)
+        label = f"batch pair {i} {p.dim}:"
+        assert_count(int(nvox[i].item()), n_ref, n64, label)
+        assert_parity(float(score[i].item()), grads[i].cpu().numpy(), (s_ref, g_ref), (s64, g64), label=label)
     for st in stages:
         st.close()
 

@@ -10,7 +10,7 @@ import pytest
 from plastimatch_amd import synth
 from plastimatch_amd.bspline import (BsplineHip, Bspline_xform, Volume, stage_for_pair,
                                      volume_gradient)
-from tests.helpers import assert_parity, bf16_round, oracle_case, rel, rel_scalar, small_pair
+from tests.helpers import assert_count, assert_parity, bf16_round, oracle_case, rel, rel_scalar, small_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -36,7 +36,7 @@ def test_mse_matches_oracle(hip_lib, case, grad_mode):
             flavor = "g" if grad_mode == "ref_nn" else "p"
             s_ref, g_ref, n_ref, _ = oc.mse(coeff, flavor)
             s64, g64, n64 = oc.mse_f64(coeff, 0 if grad_mode == "ref_nn" else 1)
-            assert r.num_vox == n_ref
+            assert_count(r.num_vox, n_ref, n64, f"case {case} {grad_mode} coeff {k}:")
             assert_parity(r.score, r.grad, (s_ref, g_ref), (s64, g64),
                           label=f"case {case} {grad_mode} coeff {k}:")
 
@@ -100,8 +100,8 @@ def test_bf16_storage_matches_oracle_on_rounded_images(hip_lib):
     st_exact = stage_for_pair(pair, storage="bf16", with_grad=False)
     r = st_exact.score(pair.true_coeff, "mse", "exact")
     s_ref, g_ref, n_ref, _ = oc.mse(pair.true_coeff, "p")
-    s64, g64, _ = oc.mse_f64(pair.true_coeff, 1)
-    assert r.num_vox == n_ref
+    s64, g64, n64 = oc.mse_f64(pair.true_coeff, 1)
+    assert_count(r.num_vox, n_ref, n64, "bf16:")
     assert_parity(r.score, r.grad, (s_ref, g_ref), (s64, g64), label="bf16:")
     st_exact.close()
 
@@ -113,11 +113,29 @@ def test_full_size_mse_256x256x128(hip_lib):
         coeff = synth.random_coeff(pair.dim, pair.vox_per_rgn, 3.0, seed=5)
         r = st.score(coeff, "mse", "ref_nn")
         s_ref, g_ref, n_ref, _ = oc.mse(coeff, "g")
-        s64, g64, _ = oc.mse_f64(coeff, 0)
-        assert r.num_vox == n_ref
+        s64, g64, n64 = oc.mse_f64(coeff, 0)
+        assert_count(r.num_vox, n_ref, n64, "256x256x128:")
         assert_parity(r.score, r.grad, (s_ref, g_ref), (s64, g64), label="256x256x128:")
         r2 = st.score(coeff, "mse", "ref_nn")
         assert np.array_equal(r.grad, r2.grad)
+
+
+@pytest.mark.parametrize("grad_mode", ["ref_nn", "exact"])
+def test_full_size_mse_128cube_vpr15(hip_lib, grad_mode):
+    """The 128^3 workload at a 15-voxel grid (40 Gaussians plus noise, moving
+    warped by a N(0, 2 mm) field) at zero and at the true coefficients,
+    against the oracle's flavors g and p and FP64."""
+    pair = synth.make_config("mse_128cube_vpr15", seed=0)
+    oc = oracle_case(pair)
+    flavor, mode = ("g", 0) if grad_mode == "ref_nn" else ("p", 1)
+    with stage_for_pair(pair, moving_grad=Volume(oc.m_grad, pair.spacing)) as st:
+        for name, coeff in (("zero", np.zeros(oc.num_coeff, np.float32)), ("true", pair.true_coeff)):
+            r = st.score(coeff, "mse", grad_mode)
+            s_ref, g_ref, n_ref, _ = oc.mse(coeff, flavor)
+            s64, g64, n64 = oc.mse_f64(coeff, mode)
+            label = f"128^3 vpr 15 {grad_mode} {name}:"
+            assert_count(r.num_vox, n_ref, n64, label)
+            assert_parity(r.score, r.grad, (s_ref, g_ref), (s64, g64), label=label)
 
 
 @pytest.mark.parametrize("vpr", [(7, 6, 5), (10, 10, 10)])
@@ -140,8 +158,9 @@ def test_launch_shapes_agree_and_autotune(hip_lib, vpr):
             assert np.array_equal(r.grad, ref.grad), key
             # the squared differences of a thread's batch of G voxels are
             # summed in FP32 before the FP64 total, so G moves the score by
-            # rounding only
-            assert rel_scalar(r.score, ref.score) < 1e-9, key
+            # rounding only (a few FP32 roundings of a batch sum, ~1e-9 of
+            # the total here)
+            assert rel_scalar(r.score, ref.score) < 1e-8, key
             assert r.num_vox == ref.num_vox, key
         choice, ms = st.autotune("mse", "ref_nn", reps=3)
         assert choice in (1, 2, 4) and ms > 0

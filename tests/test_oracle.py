@@ -244,6 +244,92 @@ def test_mi_identical_images_is_minus_entropy():
     assert abs(s64 + h) < 1e-9 * h
 
 
+def test_fd_support_sums_match_whole_image_differences():
+    """The support sum of a moved coefficient (orc_mse_f64_support) gives the
+    whole FP64 score's central difference: on coefficients on a 2^-8 grid a
+    2^-10 step is exact in float, so two whole-image evaluations difference
+    the same perturbation (where the overlap count stays the same)."""
+    pair = small_pair(dim=(24, 22, 18), spacing=(1.0, 1.0, 1.5), vpr=(5, 5, 4), seed=12)
+    oc = oracle_case(pair)
+    c = np.round(synth.random_coeff(pair.dim, pair.vox_per_rgn, 1.5, seed=2) * 256) / 256
+    c = c.astype(np.float32)
+    h = 2.0 ** -10
+    idx = np.random.default_rng(5).choice(oc.num_coeff, 12, replace=False)
+    fd, border = oc.fd_mse_f64(c, idx, h)
+    checked = 0
+    scale = np.abs(fd).max()
+    for i, f in zip(idx, fd):
+        a = c.copy(); a[i] += np.float32(h)
+        b = c.copy(); b[i] -= np.float32(h)
+        sa, _, na = oc.mse_f64(a, 1)
+        sb, _, nb = oc.mse_f64(b, 1)
+        n0 = oc.mse_f64(c, 1)[2]
+        if na != n0 or nb != n0:
+            continue
+        whole = (sa - sb) / (2 * h)
+        # (the whole-image scores are OpenMP sums of ~1e4 terms each: their
+        # difference carries summation-order rounding, ~1e-8 of the largest
+        # component)
+        assert abs(f - whole) <= 1e-7 * scale, (i, f, whole)
+        checked += 1
+    assert checked >= 8
+
+
+def test_fd_of_f64_score_vs_flavor_p_gradient():
+    """Central differences of the FP64 score against the FP64 flavor p
+    gradient: equal on coefficients whose support no clamped voxel touches
+    (an interior ROI); on the supports with clamped border voxels they
+    differ, because li_value_dx keeps its slope where li_clamp holds the
+    value (src/base/interpolate.cxx:17-41, 162-244)."""
+    roi_off, roi_dim = (4, 4, 3), (22, 20, 14)
+    pair = small_pair(dim=(30, 28, 20), spacing=(1.0, 1.0, 1.0), vpr=(5, 5, 5), seed=13, ct=False)
+    oc = oracle_case(pair, roi_offset=roi_off, roi_dim=roi_dim)
+    c = synth.random_coeff(roi_dim, pair.vox_per_rgn, 1.0, seed=4)
+    _, g64, _ = oc.mse_f64(c, 1)
+    idx = np.random.default_rng(6).choice(oc.num_coeff, 40, replace=False)
+    fd, border = oc.fd_mse_f64(c, idx, 1e-4)
+    assert np.all(border == 0)
+    assert rel(fd, g64[idx]) < 1e-6
+    # a whole image: supports at the border hold clamped voxels
+    pair = small_pair(dim=(48, 44, 40), spacing=(1.0, 1.0, 1.0), vpr=(5, 5, 5), seed=13, ct=False)
+    oc = oracle_case(pair)
+    c = synth.random_coeff(pair.dim, pair.vox_per_rgn, 1.0, seed=4)
+    _, g64, _ = oc.mse_f64(c, 1)
+    idx = np.random.default_rng(7).choice(oc.num_coeff, 300, replace=False)
+    fd, border = oc.fd_mse_f64(c, idx, 1e-4)
+    g64 = g64[idx]
+    inner, outer = border == 0, border > 0
+    assert inner.sum() >= 8 and outer.sum() > 20
+    assert rel(fd[inner], g64[inner]) < 1e-5
+    assert rel(fd[outer], g64[outer]) > 1e-2
+
+
+def test_mi_f64_on_given_histogram():
+    """The FP64 score and pass 2 on given histograms: at zero coefficients
+    every voxel lands on a moving voxel centre, so the PV weights are exactly
+    0 and 1 in float and double alike, the float flavor's histogram is the
+    FP64 one, and running on it must reproduce the full FP64 result (a
+    mis-ordered or mis-read histogram -- fixed and moving swapped, the joint
+    bins shifted -- changes the score).  Away from integer positions the
+    result moves from the full FP64 one only by the float rounding of the
+    histogram weights."""
+    pair = small_pair(dim=(30, 26, 22), vpr=(5, 6, 4), seed=9)
+    oc = oracle_case(pair)
+    hp = oc.hist(20, f_bins=24)
+    zero = np.zeros(oc.num_coeff, np.float32)
+    o = oc.mi(zero, hp)
+    s64, g64, n64 = oc.mi_f64(zero, hp)
+    sh, gh, nh = oc.mi_f64(zero, hp, hist=o)
+    assert nh == n64 == o["num_vox"]
+    assert rel_scalar(sh, s64) < 1e-13
+    assert rel(gh, g64) < 1e-12
+    o = oc.mi(pair.true_coeff, hp)
+    s64, g64, _ = oc.mi_f64(pair.true_coeff, hp)
+    sh, gh, _ = oc.mi_f64(pair.true_coeff, hp, hist=o)
+    assert rel_scalar(sh, s64) < 1e-6
+    assert rel_scalar(sh, o["score"]) < 1e-6
+
+
 def test_pass1_positions_exact_for_dyadic_geometry():
     for name in ("mi_512x512x160_vpr12",):
         cfg = synth.CONFIGS[name]

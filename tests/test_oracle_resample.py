@@ -85,3 +85,39 @@ def test_hist_eqsp():
     mn, mx = img.min(), img.max()
     assert delta == np.float32((mx - mn) / np.float32(31))
     assert off == np.float32(np.float64(mn) - 0.5 * np.float64(delta))
+
+
+def _bins(img, off, delta):
+    """add_pvi_8's bin of each value, floor((v - offset) / delta) in float
+    (src/register/joint_histogram.cxx:458-482)."""
+    return np.floor((img.astype(np.float32) - np.float32(off)) / np.float32(delta)).astype(np.int64)
+
+
+@pytest.mark.parametrize("bins", [2, 7, 20, 32, 64])
+def test_hist_eqsp_half_bins_at_the_ends(bins):
+    """The equal-spacing bins put the image minimum at bin coordinate 0.5
+    and the maximum at bins - 0.5 -- the top and bottom bins are half full
+    (src/register/joint_histogram.cxx:136-138) -- so every voxel bins into
+    0 .. bins-1; a plain min/max scaling (no half-bin offset) would put the
+    maximum in bin `bins`, and a delta of (max - min) / bins would put it at
+    bins - 1 - 0.5 (bins - 1) / bins."""
+    for seed in range(3):
+        img = vol((9, 8, 7), seed=seed) * np.float32(37.0) - np.float32(600.0)
+        off, delta = oracle.hist_eqsp(img, bins)
+        mn, mx = np.float64(img.min()), np.float64(img.max())
+        lo = (mn - off) / delta
+        hi = (mx - off) / delta
+        assert abs(lo - 0.5) < 1e-4 and abs(hi - (bins - 0.5)) < 1e-4 * bins
+        b = _bins(img, off, delta)
+        assert b.min() == 0 and b.max() == bins - 1
+
+
+def test_hist_eqsp_two_values_fill_the_end_bins():
+    """A two-valued image fills exactly bins 0 and bins - 1."""
+    img = np.where(vol((10, 9, 8), seed=3) > 0, np.float32(1234.5), np.float32(-1000.0)).astype(np.float32)
+    for bins in (2, 16, 32):
+        off, delta = oracle.hist_eqsp(img, bins)
+        counts = np.bincount(_bins(img, off, delta).ravel(), minlength=bins)
+        assert counts.size == bins
+        assert counts[0] == (img < 0).sum() and counts[-1] == (img > 0).sum()
+        assert counts[1:-1].sum() == 0
