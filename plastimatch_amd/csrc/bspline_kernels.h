@@ -49,12 +49,13 @@ tile_slots (int nvox, int vox_per_thread)
     return (nvox + s - 1) / s * s;
 }
 
-// Floats of the tile kernel's column-term planes, which the condense
-// reuses for the tile's dc/dv contracted over x and y.
+// Floats of the tile kernel's column-term planes (three float4 planes of
+// Horner coefficients per column), which the condense reuses for the
+// tile's dc/dv contracted over x and y.
 __host__ __device__ inline int
 tile_col_floats (int vx, int vy, int vz)
 {
-    const int a = 16 * vx * vy;
+    const int a = 12 * vx * vy;
     const int b = 48 * vz;
     return a > b ? a : b;
 }

@@ -237,6 +237,27 @@ tile_coords (const KGeom& g, int tile, int p[3])
     p[2] = t / g.rdims[1];
 }
 
+// A voxel slot of a tile, packed in one register: x (bits 0-4), y (5-9),
+// its column index x + vx y (10-19) and z (24-31), every field below 32
+// (MAX_VPR) or 1024.  The voxel pass needs only the column and z (z as a
+// float comes from its byte in one conversion); x, y and z as integers only
+// at tiles that mask voxels.
+__device__ __forceinline__ int slot_pack (int x, int y, int r, int z) { return x | (y << 5) | (r << 10) | (z << 24); }
+__device__ __forceinline__ int slot_x (int w) { return w & 31; }
+__device__ __forceinline__ int slot_y (int w) { return (w >> 5) & 31; }
+__device__ __forceinline__ int slot_col (int w) { return (w >> 10) & 1023; }
+__device__ __forceinline__ int slot_z (int w) { return (int) ((uint32_t) w >> 24); }
+__device__ __forceinline__ float slot_zf (int w) { return (float) ((uint32_t) w >> 24); }
+
+// (int) floor (x) in one instruction, for |x| < 2^31.
+__device__ __forceinline__ int
+floor_int (float x)
+{
+    int r;
+    asm ("v_cvt_flr_i32_f32 %0, %1" : "=v" (r) : "v" (x));
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // raw buffer loads: 32-bit byte offsets from a wave-uniform base, so the
 // per-voxel address arithmetic stays in 32-bit VALU; a load at or past the
@@ -398,9 +419,9 @@ struct Shape {
 struct TileLds {
     float* bx;      // [x][4] basis LUT along x
     float* by;      // [y][4]
-    float* bzz;     // [z][8]: bz[0..3], Z[z][0..2], pad
+    float* bzz;     // [z][8]: bz[0..3], pad
     float* coef;    // [k][j][i][c]: the tile's 64 knots
-    float* col;     // [4][y][x][4]: planes A[0][k], A[1][k], A[2][k], (C[a], pad)
+    float* col;     // [3][y][x][4]: per component a, the Horner coefficients q0..q3 of ml[a](z)
     float* s2;      // [z][c][j][i]: dc/dv condensed over x and y (reuses col)
     float* t1;      // [k][j][x][4]: sum_i bx coef (reuses dv)
     float* dv;      // [c][slot]: dc/dv of every voxel slot, one plane per component
@@ -487,8 +508,7 @@ enum ColumnMode { PROJECTED = 0, DISPLACEMENT = 1, OFFSETS = 2 };
 // Stage 1 of a tile: its 64 knot coefficients (fetched ahead into
 // registers; PROJECTED: projected onto the moving index axes, P c, so the
 // contractions that follow carry moving-index units and the column terms
-// need no projection) and the per-slice moving-index term of the tile's
-// local frame (TileBase), Z[z][a] = M[a][2] * z, M = P_moving * step_fixed.
+// need no projection).
 template <int V, int MODE>
 __device__ __forceinline__ void
 tile_stage1 (const TileLds& L, const Shape<V>& S, const KGeom& g, const float c[3], int t)
@@ -507,10 +527,6 @@ tile_stage1 (const TileLds& L, const Shape<V>& S, const KGeom& g, const float c[
         L.coef[3 * t + 0] = c0;
         L.coef[3 * t + 1] = c1;
         L.coef[3 * t + 2] = c2;
-    }
-    for (int e = t; e < 3 * S.vz; e += TILE_THREADS) {
-        const int z = e / 3, a = e - 3 * z;
-        L.bzz[8 * z + 4 + a] = MODE == DISPLACEMENT ? 0.f : g.mstep[3 * a + 2] * (float) z;
     }
 }
 
@@ -539,39 +555,53 @@ tile_t1 (const TileLds& L, const Shape<V>& S, int t)
 // to be contracted along z) and C[a] = Bf[a] + M[a][0] x + M[a][1] y (x, y,
 // z local to the tile, Bf the fraction of the tile's base, TileBase), a
 // voxel (x, y, z) of the tile maps to the local moving index coordinates
-//   ml[a] = sum_k bz[z][k] A[a][k] + C[a] + Z[z][a],  mijk = Bi + ml,
+//   ml[a] = sum_k bz_k(z / vz) A[a][k] + C[a] + M[a][2] z,  mijk = Bi + ml,
 // which is P * (fixed position + displacement - moving origin) regrouped
 // (bspline_find_correspondence_dcos, src/util/bspline_correspond.cxx:80-109,
 // with the position of POSITION_FROM_COORDS, src/base/volume_macros.h:82-96).
+// The cubic B-spline basis along z (bspline_basis_eval,
+// src/base/bspline_xform.cxx:73-99) makes ml a cubic polynomial in z for
+// each column, so the column stores its Horner coefficients: one float4
+// plane per component, ml[a] = ((q3 z + q2) z + q1) z + q0, three FMAs a
+// component for each voxel.  (MODE DISPLACEMENT: A only, the displacement
+// in mm; OFFSETS: C and M[a][2] z only.)
 //
 // Returns whether every column this thread formed stays in the interior of
 // the moving image over all the tile's slices: with bz >= 0, sum_k bz = 1
-// and Z linear in z, ml lies in C + [min_k A, max_k A] + [Z(first),
-// Z(last)]; 0.01 voxel of margin covers the rounding of ml.  Inside that
-// range li_clamp's clamps are identities and Volume::is_inside holds.
+// and the last term linear in z, ml lies in C + [min_k A, max_k A] +
+// [0 | M[a][2] (vz - 1)]; 0.01 voxel of margin covers the rounding of ml.
+// Inside that range li_clamp's clamps are identities and Volume::is_inside
+// holds.
 template <int V, int MODE>
 __device__ __forceinline__ bool
 tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, const TileBase& tb, int t)
 {
     const int vx = S.vx;
     const float4* t1 = reinterpret_cast<const float4*> (L.t1);
+    // 1/vz, 1/vz^2, 1/vz^3 times the basis polynomial's factors
+    const float ivz = 1.f / (float) S.vz;
+    const float h1 = 0.5f * ivz, h2 = 0.5f * ivz * ivz, h3 = (1.f / 6.f) * ivz * ivz * ivz;
     bool ok = true;
     for (int e = t; e < vx * S.vy; e += TILE_THREADS) {
         const int y = S.div_x (e), x = e - y * vx;
-        // one float4 plane per term, so the voxel pass reads consecutive
-        // 16-byte records (conflict-free) for consecutive columns
+        // one float4 plane per component, so the voxel pass reads
+        // consecutive 16-byte records (conflict-free) for consecutive columns
         float4* o = reinterpret_cast<float4*> (L.col) + e;
         const int np = vx * S.vy;
+        float C[3] = {0.f, 0.f, 0.f}, Mz[3] = {0.f, 0.f, 0.f};
         if (MODE != DISPLACEMENT) {
             const float fi = (float) x, fj = (float) y;
-            float C[3];
 #pragma unroll
-            for (int a = 0; a < 3; a++) C[a] = (tb.bf[a] + g.mstep[3 * a] * fi) + g.mstep[3 * a + 1] * fj;
-            o[3 * np] = make_float4 (C[0], C[1], C[2], 0.f);
-        } else {
-            o[3 * np] = make_float4 (0.f, 0.f, 0.f, 0.f);
+            for (int a = 0; a < 3; a++) {
+                C[a] = (tb.bf[a] + g.mstep[3 * a] * fi) + g.mstep[3 * a + 1] * fj;
+                Mz[a] = g.mstep[3 * a + 2];
+            }
         }
-        if (MODE == OFFSETS) continue;
+        if (MODE == OFFSETS) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) o[a * np] = make_float4 (C[a], Mz[a], 0.f, 0.f);
+            continue;
+        }
         const float4 b = reinterpret_cast<const float4*> (L.by)[y];
         float T[3][4];
 #pragma unroll
@@ -586,14 +616,20 @@ tile_columns (const TileLds& L, const Shape<V>& S, const KGeom& g, const TileBas
         }
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            const float A[4] = {T[a][0], T[a][1], T[a][2], T[a][3]};
-            o[a * np] = make_float4 (A[0], A[1], A[2], A[3]);
+            const float A0 = T[a][0], A1 = T[a][1], A2 = T[a][2], A3 = T[a][3];
+            // sum_k bz_k(u) A_k = q0 + q1' u + q2' u^2 + q3' u^3 with
+            // 6 bz = ((1-u)^3, 3u^3 - 6u^2 + 4, -3u^3 + 3u^2 + 3u + 1, u^3),
+            // u = z / vz
+            const float s02 = A0 + A2;
+            const float q0 = fmaf (A1, 2.f / 3.f, s02 * (1.f / 6.f)) + C[a];
+            const float q1 = fmaf (A2 - A0, h1, Mz[a]);
+            const float q2 = fmaf (A1, -2.f, s02) * h2;
+            const float q3 = fmaf (A1 - A2, 3.f, A3 - A0) * h3;
+            o[a * np] = make_float4 (q0, q1, q2, q3);
             if (MODE == PROJECTED) {
-                const float4 Cv = o[3 * np];
-                const float c = a == 0 ? Cv.x : (a == 1 ? Cv.y : Cv.z);
-                const float zb = g.mstep[3 * a + 2] * (float) (S.vz - 1);
-                const float lo = c + fminf (0.f, zb) + fminf (fminf (A[0], A[1]), fminf (A[2], A[3]));
-                const float hi = c + fmaxf (0.f, zb) + fmaxf (fmaxf (A[0], A[1]), fmaxf (A[2], A[3]));
+                const float zb = Mz[a] * (float) (S.vz - 1);
+                const float lo = C[a] + fminf (0.f, zb) + fminf (fminf (A0, A1), fminf (A2, A3));
+                const float hi = C[a] + fmaxf (0.f, zb) + fmaxf (fmaxf (A0, A1), fmaxf (A2, A3));
                 ok = ok & (lo >= tb.cl_lo[a] + 0.01f) & (hi <= tb.cl_hi[a] + 0.99f);
             }
         }
@@ -800,7 +836,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
     if (MI) N = *mi.nvox_f;
 
     // This thread's first-batch voxel slots t + 256u (x fastest), the same
-    // in every tile, packed as x | y << 10 | z << 20 (each below 1024).
+    // in every tile, packed by slot_pack (x, y, the column index and z).
     // With compile-time tile edges and at most four slots per thread, every
     // batch's slots are kept (NSK of them), not only the first batch's.
     constexpr int NS = V > 0 ? (V * V * V + TILE_THREADS * G - 1) / (TILE_THREADS * G) * G : G;
@@ -812,7 +848,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         const int z = S.div_xy (v);
         const int r = v - z * (vx * vy);
         const int y = S.div_x (r);
-        sxyz[u] = (r - y * vx) | (y << 10) | (z << 20);
+        sxyz[u] = slot_pack (r - y * vx, y, r, z);
     }
 
     // Fetched one tile ahead: the knot coefficients (threads < 64) and the
@@ -846,7 +882,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         const Rsrc vr = make_rsrc (vf, VF_IN ? nf * 12u : 0u);
 #pragma unroll
         for (int u = 0; u < G; u++) {
-            const int x = sxyz[u] & 1023, y = (sxyz[u] >> 10) & 1023, z = sxyz[u] >> 20;
+            const int x = slot_x (sxyz[u]), y = slot_y (sxyz[u]), z = slot_z (sxyz[u]);
             const int fo = fb + x + (int) __umul24 (d0, y) + (int) __umul24 (d01, z);
             if (SCORE && !STAGE) fpre[u] = Img<TI>::one (fr, fo * ES, 0);
             if (MI) fbpre[u] = load_u8 (fr, fo, 0);
@@ -957,10 +993,16 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         // the tile, G at a time, issuing all their loads before using any.
         // Voxels outside the ROI or the tile still run (their fixed load may
         // fall past the image and read 0) and are masked at the end; z < ez
-        // also rules out the slots past the tile.
+        // also rules out the slots past the tile.  A tile whose every column
+        // stays in the moving image's interior and that lies whole inside
+        // the ROI takes the lean pass (FAST): no clamp, no inside test, no
+        // per-voxel mask but that of the slots past the tile, and its
+        // overlap count is its voxel count.
         const int ex = min (vx, g.roi_end[0] - P.bx0);
         const int ey = min (vy, g.roi_end[1] - P.by0);
         const int ez = min (vz, g.roi_end[2] - P.bz0);
+        const bool full = ex == vx && ey == vy && ez == vz;
+        fast = fast && full;
         const int fd0 = g.fdim[0], fd01 = g.fdim[0] * g.fdim[1];
         const int md0 = g.mdim[0], md01 = g.mdim[0] * g.mdim[1];
         const int fbase = P.bx0 + fd0 * P.by0 + fd01 * P.bz0;
@@ -981,9 +1023,8 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         const int SXZ = md0 * EZ;
         // The moving-index bounds of the tile's local frame, read once per
         // tile into registers: left in memory, they are reloaded per voxel
-        // (each reload a scalar-load round trip).
-        // (The values are wave-uniform; readfirstlane puts them in scalar
-        // registers.)
+        // (each reload a scalar-load round trip).  (The values are
+        // wave-uniform; readfirstlane puts them in scalar registers.)
         float tlo[3], thi[3], clo[3], chi[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -993,6 +1034,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
             chi[k] = uniform (tb.cl_hi[k]);
         }
         const int boff = __builtin_amdgcn_readfirstlane (tb.boff);
+        if (fast && t < 64) count += nvox;      // every voxel of a lean tile is inside
 
         auto voxels = [&] (auto fast_c) {
         constexpr bool FAST = decltype (fast_c)::value;
@@ -1010,65 +1052,69 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
             int wofs[G];            // WARP: the fixed voxel written, or -1
 #pragma unroll
             for (int u = 0; u < G; u++) {
-                int x, y, z;
                 float dd[3] = {0.f, 0.f, 0.f};
-                if (STAGE) fval[u] = L.dv[2 * L.dvp + base + u * TILE_THREADS + tt];
-                // this thread's slot number (a constant of the unrolled loops
-                // for compile-time edges; at runtime edges only the first
+                const int v = base + u * TILE_THREADS + tt;
+                if (STAGE) fval[u] = L.dv[2 * L.dvp + v];
+                // this thread's slot (a constant of the unrolled loops for
+                // compile-time edges; at runtime edges only the first
                 // batch's slots are kept)
                 const int su = V > 0 ? base / TILE_THREADS + u : u;
+                int sw;
                 if (V > 0 ? su < NSK : first) {
-                    x = sxyz[su] & 1023;
-                    y = (sxyz[su] >> 10) & 1023;
-                    z = sxyz[su] >> 20;
+                    sw = sxyz[su];
                 } else {
-                    const int v = base + u * TILE_THREADS + tt;
-                    z = S.div_xy (v);
-                    const int r = v - z * (vx * vy);
-                    y = S.div_x (r);
-                    x = r - y * vx;
+                    const int z = S.div_xy (v);
+                    const int rr = v - z * (vx * vy);
+                    const int y = S.div_x (rr);
+                    sw = slot_pack (rr - y * vx, y, rr, z);
                 }
+                // slots that may lie past the tile (the last batch's)
+                const bool may_pad = V > 0 ? base + u * TILE_THREADS + TILE_THREADS > V * V * V : true;
                 if (first) {
                     if (SCORE && !STAGE) fval[u] = fpre[u];
                     if (MI) fbv[u] = fbpre[u];
                     if (VF_IN) { dd[0] = dpre[u][0]; dd[1] = dpre[u][1]; dd[2] = dpre[u][2]; }
                 } else if (!STAGE) {
-                    const int fofs = fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
+                    const int fofs = fbase + slot_x (sw) + (int) __umul24 (fd0, slot_y (sw))
+                        + (int) __umul24 (fd01, slot_z (sw));
                     if (SCORE && !STAGE) fval[u] = Img<TI>::one (fr, fofs * ES, 0);
                     if (MI) fbv[u] = load_u8 (fr, fofs, 0);
                     if (VF_IN) Img<float>::vec3 (dd, vr, fofs * 12, 0);
                 }
-                const bool act = x < ex && y < ey && z < ez;
-                if (WARP)
-                    wofs[u] = act ? fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z) : -1;
-                const int r = x + (int) __umul24 (vx, y);
-                const int zc = min (z, vz - 1);
-
-                const float4* cp = reinterpret_cast<const float4*> (L.col) + r;
-                const float4* zp = reinterpret_cast<const float4*> (&L.bzz[8 * zc]);
-                const float4 b = zp[0], Z = zp[1];
+                bool act;
+                if (FAST) {
+                    act = may_pad ? v < nvox : true;
+                } else {
+                    act = slot_x (sw) < ex && slot_y (sw) < ey && slot_z (sw) < ez;
+                }
+                if (WARP) {
+                    const int fofs = fbase + slot_x (sw) + (int) __umul24 (fd0, slot_y (sw))
+                        + (int) __umul24 (fd01, slot_z (sw));
+                    wofs[u] = act ? fofs : -1;
+                }
+                // the column's Horner coefficients, evaluated at the slot's z
+                const float4* cp = reinterpret_cast<const float4*> (L.col) + slot_col (sw);
                 const int ncol = vx * vy;
-                const float4 C = cp[3 * ncol];
+                const float4 Q0 = cp[0], Q1 = cp[ncol], Q2 = cp[2 * ncol];
+                const float zf = slot_zf (sw);
                 float m[3];
+                m[0] = fmaf (fmaf (fmaf (Q0.w, zf, Q0.z), zf, Q0.y), zf, Q0.x);
+                m[1] = fmaf (fmaf (fmaf (Q1.w, zf, Q1.z), zf, Q1.y), zf, Q1.x);
+                m[2] = fmaf (fmaf (fmaf (Q2.w, zf, Q2.z), zf, Q2.y), zf, Q2.x);
                 if (VF_IN) {
                     // P * (position + d - moving origin), regrouped as for
                     // the interpolated displacement
                     const float* Pm = g.mproj;
 #pragma unroll
                     for (int k = 0; k < 3; k++)
-                        m[k] = ((k == 0 ? C.x : k == 1 ? C.y : C.z) + (k == 0 ? Z.x : k == 1 ? Z.y : Z.z))
-                            + fmaf (Pm[3 * k + 2], dd[2], fmaf (Pm[3 * k + 1], dd[1], Pm[3 * k] * dd[0]));
-                } else {
-                    const float4 A0 = cp[0], A1 = cp[ncol], A2 = cp[2 * ncol];
-                    m[0] = fmaf (b.w, A0.w, fmaf (b.z, A0.z, fmaf (b.y, A0.y, fmaf (b.x, A0.x, C.x)))) + Z.x;
-                    m[1] = fmaf (b.w, A1.w, fmaf (b.z, A1.z, fmaf (b.y, A1.y, fmaf (b.x, A1.x, C.y)))) + Z.y;
-                    m[2] = fmaf (b.w, A2.w, fmaf (b.z, A2.z, fmaf (b.y, A2.y, fmaf (b.x, A2.x, C.z)))) + Z.z;
+                        m[k] = m[k] + fmaf (Pm[3 * k + 2], dd[2], fmaf (Pm[3 * k + 1], dd[1], Pm[3 * k] * dd[0]));
                 }
                 if (VF_OUT) {
                     // bspline_interpolate_vf (src/base/bspline_interpolate.cxx:112-142):
                     // the ROI voxels of the field get the displacement in mm
                     if (act) {
-                        const int fofs = fbase + x + (int) __umul24 (fd0, y) + (int) __umul24 (fd01, z);
+                        const int fofs = fbase + slot_x (sw) + (int) __umul24 (fd0, slot_y (sw))
+                            + (int) __umul24 (fd01, slot_z (sw));
                         float* o = vf + 3 * (size_t) fofs;
                         o[0] = m[0];
                         o[1] = m[1];
@@ -1083,23 +1129,25 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                 // and round = f + (frac >= 0.5) give exactly its three
                 // branches (and the rounding of ROUND_PLM_LONG).  Any m, NaN
                 // included, yields in-range indices, so the loads are issued
-                // unconditionally.
+                // unconditionally.  A lean tile's m stay in the interior,
+                // where the clamps are identities and floor and fraction are
+                // one instruction each (the fraction m - floor(m) is exact
+                // for m >= 0).
                 bool inside = true;
                 int f[3], c[3];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    float fl, fa;
+                    float fa;
                     if (FAST) {
-                        // interior tile: the clamps are identities
-                        fl = floorf (m[k]);
-                        fa = m[k] - fl;
+                        f[k] = floor_int (m[k]);
+                        fa = __builtin_amdgcn_fractf (m[k]);
                     } else {
                         // bitwise, not short-circuit: no branch per component
                         inside = inside & (m[k] > tlo[k]) & (m[k] < thi[k]);
-                        fl = __builtin_amdgcn_fmed3f (floorf (m[k]), clo[k], chi[k]);
+                        const float fl = __builtin_amdgcn_fmed3f (floorf (m[k]), clo[k], chi[k]);
                         fa = fminf (fmaxf (m[k] - fl, 0.f), 1.f);
+                        f[k] = (int) fl;
                     }
-                    f[k] = (int) fl;
                     c[k] = fa >= 0.5f ? 1 : 0;
                     fr_[u][k] = fa;
                 }
@@ -1167,10 +1215,12 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     // them) into it.
                     float fv = fval[u];
                     asm volatile ("" : "+v" (m_val), "+v" (fv));
+                    // (a lean tile's in[] is true but for the slots past
+                    // the tile, and constant where none can be)
                     const float diff = in[u] ? m_val - fv : 0.f;
                     bsum += diff * diff;
                     // the wave's overlap count, on the scalar unit
-                    count += (int) __popcll (__ballot (in[u]));
+                    if (!FAST) count += (int) __popcll (__ballot (in[u]));
                     if (KIND == 0) {
                         dc0 = diff * mg[u][0];
                         dc1 = diff * mg[u][1];

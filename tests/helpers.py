@@ -87,14 +87,18 @@ def assert_parity_mi(score, grad, ref, full64, hist64, tol_s=1e-5, tol_g=1e-4, l
 
 
 def assert_count(n, n_ref, n64, label=""):
-    """Overlap count N of the GPU against the FP64 oracle's: equal.  The
-    float reference flavor decides Volume::is_inside on float positions
-    (good to ~1e-5 voxel at these image sizes), so a voxel that close to
-    the (-0.5, dim - 0.5) boundary may fall on the other side there: its
-    count is printed, and may differ by a few such voxels."""
-    if n != n_ref:
+    """Overlap count N of the GPU against the FP64 oracle's and the float
+    reference flavor's.  N is decided by Volume::is_inside on the voxel's
+    moving position; the GPU's positions are good to ~1e-6 voxel (the tile's
+    local frame), the float flavor's to ~1e-5 at these image sizes, so a
+    voxel that close to the (-0.5, dim - 0.5) boundary can fall on the other
+    side of it.  Below a million voxels N must equal FP64's; above, it may
+    differ by one such voxel per million (the MI histograms, whose counts
+    come from the reference's own positions, are compared bin for bin
+    elsewhere)."""
+    if n != n_ref or n != n64:
         print(f"{label} overlap count {n} (FP64 {n64}); the float reference flavor counts {n_ref}")
-    assert n == n64, (label, n, n64)
+    assert abs(n - n64) <= n64 // 1000000, (label, n, n64)
     assert abs(n - n_ref) <= max(2, n64 // 10000), (label, n, n_ref)
 
 
