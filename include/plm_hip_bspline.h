@@ -169,6 +169,40 @@ int plm_hip_bspline_score_device (
     int64_t* d_hist,
     void* stream);
 
+/* Scratch that the allocation-free entry point below reserves on each GPU
+ * it runs on, for the whole life of the process. */
+#define PLM_HIP_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  One MSE evaluation -- what plm_hip_bspline_score_device
+ * computes for PLM_METRIC_MSE, the same gradient bits and overlap count --
+ * without a context: fixed, moving and (for PLM_GRAD_REF_NN) moving_grad
+ * are float32 images in device memory (PLM_MEM_DEVICE), read in place;
+ * d_coeff, d_score, d_grad (may be NULL for PLM_GRAD_NONE) and d_num_vox are
+ * device buffers.  Enqueued on stream (a hipStream_t; NULL for the null
+ * stream) on device gpuid, which it makes current; returns without waiting.
+ * Calls no hipMalloc, hipFree or allocating routine: every intermediate
+ * lives in a static device array of PLM_HIP_NOALLOC_SCRATCH_BYTES per GPU
+ * (192 floats a knot, 76 bytes a tile and 26 KB besides; a grid that needs
+ * more returns PLM_HIP_ERR_UNSUPPORTED, e.g. above ~40000 knots).  The first
+ * call on a device looks up that array's address (make it outside a graph
+ * capture).  Every call on a GPU shares the array and the next call
+ * overwrites it: calls that overlap on one GPU (two streams or threads)
+ * race. */
+int plm_hip_bspline_score_noalloc (
+    const plm_bsp_geom* geom,
+    const plm_img_desc* fixed,
+    const plm_img_desc* moving,
+    const plm_img_desc* moving_grad,
+    int grad_mode,
+    int gpuid,
+    const float* d_coeff,
+    double* d_score,
+    double* d_grad,
+    int64_t* d_num_vox,
+    void* stream);
+
 /* Replaces CUDA_bspline_mse_cleanup_j / CUDA_bspline_mi_cleanup_a and
  * bspline_cuda_state_destroy (src/register/bspline_state.cxx:239-271). */
 int plm_hip_bspline_destroy (plm_hip_ctx* ctx);

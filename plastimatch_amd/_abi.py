@@ -119,6 +119,9 @@ SIGNATURES = {
         _P, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P]),
     "plm_hip_bspline_score_device": (ctypes.c_int, [
         _P, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P, _P]),
+    "plm_hip_bspline_score_noalloc": (ctypes.c_int, [
+        ctypes.POINTER(PlmBspGeom), ctypes.POINTER(PlmImgDesc), ctypes.POINTER(PlmImgDesc),
+        ctypes.POINTER(PlmImgDesc), ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P]),
     "plm_hip_bspline_destroy": (ctypes.c_int, [_P]),
     "plm_hip_volume_gradient": (ctypes.c_int, [ctypes.POINTER(PlmImgDesc), _P, ctypes.c_int,
                                                ctypes.c_int]),

@@ -7,7 +7,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libplmregisterhip.so")
 
-SOURCES = ["bspline_kernels.hip", "lbfgs_kernels.hip", "stage_kernels.hip", "bspline_abi.cpp",
+SOURCES = ["bspline_kernels.hip", "lbfgs_kernels.hip", "stage_kernels.hip", "noalloc.hip", "bspline_abi.cpp",
            "bspline_lbfgs.cpp", "stage_abi.cpp"]
 
 # -ffp-contract=off: the score path reproduces the reference's float rounding

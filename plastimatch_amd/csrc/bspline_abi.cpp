@@ -282,36 +282,12 @@ upload_image (plm_hip_ctx* c, const plm_img_desc* d, int64_t nvals, void** out)
     return PLM_HIP_OK;
 }
 
-// The moving-index frame of every tile (plmhip::TileBase): the base B of
-// the tile's first voxel in double, its integer part clamped to the moving
-// image (so every local floor stays within +-mdim of it, and B - Bi is small
-// wherever the tile lands inside the image), and the inside-test and clamp
-// bounds moved into that frame.
+// The moving-index frame of every tile (plmhip::TileBase).
 std::vector<plmhip::TileBase>
 tile_bases (const KGeom& g, const double moff[3], const double mstep[9])
 {
     std::vector<plmhip::TileBase> out ((size_t) g.num_tiles);
-    for (int t = 0; t < g.num_tiles; t++) {
-        const int p0 = t % g.rdims[0], p1 = (t / g.rdims[0]) % g.rdims[1];
-        const int p2 = t / (g.rdims[0] * g.rdims[1]);
-        const double f[3] = {(double) (g.roi_off[0] + g.vpr[0] * p0),
-                             (double) (g.roi_off[1] + g.vpr[1] * p1),
-                             (double) (g.roi_off[2] + g.vpr[2] * p2)};
-        plmhip::TileBase& tb = out[(size_t) t];
-        int64_t bi[3];
-        for (int a = 0; a < 3; a++) {
-            const double B = moff[a] + mstep[3*a] * f[0] + mstep[3*a+1] * f[1] + mstep[3*a+2] * f[2];
-            double fl = std::isfinite (B) ? std::floor (B) : 0.0;
-            fl = std::min (std::max (fl, 0.0), (double) (g.mdim[a] - 1));
-            bi[a] = (int64_t) fl;
-            tb.bf[a] = (float) (std::isfinite (B) ? B - fl : 0.0);
-            tb.thr_lo[a] = (float) (-0.5 - fl);
-            tb.thr_hi[a] = (float) ((double) g.mdim[a] - 0.5 - fl);
-            tb.cl_lo[a] = (float) -fl;
-            tb.cl_hi[a] = (float) ((double) (g.mdim[a] - 2) - fl);
-        }
-        tb.boff = (int) (bi[0] + (int64_t) g.mdim[0] * (bi[1] + (int64_t) g.mdim[1] * bi[2]));
-    }
+    for (int t = 0; t < g.num_tiles; t++) out[(size_t) t] = plmhip::tile_base_of (g, moff, mstep, t);
     return out;
 }
 
@@ -334,44 +310,21 @@ check_img (const plm_img_desc* d, const char* name)
 
 } // namespace
 
-extern "C" {
+namespace plmhip {
 
+// The tile-kernel geometry of a B-spline grid over a fixed/moving pair:
+// validates the grid (Bspline_header::set, src/base/bspline_header.cxx:28-62)
+// and the direction cosines, forms step/proj (src/base/direction_matrices.cxx:
+// 10-20) and the moving-index map M = P step, moff = P (origin - moving
+// origin) in double (moff_d, mstep_d) and float (g), and checks the
+// 32-bit offset limits of the tile kernel.
 int
-plm_hip_abi_version (void)
+make_geometry (const plm_bsp_geom* geom, const plm_img_desc* fixed, const plm_img_desc* moving,
+    KGeom& g, double moff_d[3], double mstep_d[9], int64_t& tiles, int64_t& knots)
 {
-    return PLM_HIP_ABI_VERSION;
-}
-
-const char*
-plm_hip_last_error (void)
-{
-    return g_last_error.c_str ();
-}
-
-int
-plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
-    const plm_img_desc* moving, const plm_img_desc* moving_grad,
-    const plm_hist_desc* hist, int gpuid, plm_hip_ctx** out)
-{
-    if (!out) return fail (PLM_HIP_ERR_ARG, "out is NULL");
-    *out = nullptr;
-    if (!geom) return fail (PLM_HIP_ERR_ARG, "geom is NULL");
-    int rc = check_img (fixed, "fixed");
-    if (rc) return rc;
-    rc = check_img (moving, "moving");
-    if (rc) return rc;
-    if (moving_grad) {
-        if (!moving_grad->data) return fail (PLM_HIP_ERR_ARG, "moving_grad data is NULL");
-        for (int a = 0; a < 3; a++)
-            if (moving_grad->dim[a] != moving->dim[a])
-                return fail (PLM_HIP_ERR_GEOMETRY, "moving_grad dims differ from moving");
-    }
-    if (fixed->storage != moving->storage
-        || (fixed->storage != PLM_STORE_F32 && fixed->storage != PLM_STORE_BF16))
-        return fail (PLM_HIP_ERR_ARG, "fixed and moving must use the same storage");
-
-    KGeom g {};
-    int64_t tiles = 1, knots = 1;
+    g = KGeom {};
+    tiles = 1;
+    knots = 1;
     for (int a = 0; a < 3; a++) {
         int64_t vpr = geom->vox_per_rgn[a];
         if (vpr < 1 || vpr > plmhip::MAX_VPR)
@@ -419,7 +372,6 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     // mijk = P d + moff + M ijk with M = P step and moff = P (origin - morigin),
     // P the moving proj; M and moff are formed in double from the float
     // inputs, so an identity map is exactly the identity.
-    double moff_d[3], mstep_d[9];
     {
         double pd[9];
         if (!mat3_inverse_d (pd, moving->direction_cosines))
@@ -467,6 +419,59 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
             if (i != j && (g.fstep[3*i+j] != 0.f || g.mproj[3*i+j] != 0.f)) g.aligned = 0;
+
+    return PLM_HIP_OK;
+}
+
+void
+abi_basis_lut (int vpr, float* lut)
+{
+    basis_lut (vpr, lut);
+}
+
+} // namespace plmhip
+
+extern "C" {
+
+int
+plm_hip_abi_version (void)
+{
+    return PLM_HIP_ABI_VERSION;
+}
+
+const char*
+plm_hip_last_error (void)
+{
+    return g_last_error.c_str ();
+}
+
+int
+plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
+    const plm_img_desc* moving, const plm_img_desc* moving_grad,
+    const plm_hist_desc* hist, int gpuid, plm_hip_ctx** out)
+{
+    if (!out) return fail (PLM_HIP_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!geom) return fail (PLM_HIP_ERR_ARG, "geom is NULL");
+    int rc = check_img (fixed, "fixed");
+    if (rc) return rc;
+    rc = check_img (moving, "moving");
+    if (rc) return rc;
+    if (moving_grad) {
+        if (!moving_grad->data) return fail (PLM_HIP_ERR_ARG, "moving_grad data is NULL");
+        for (int a = 0; a < 3; a++)
+            if (moving_grad->dim[a] != moving->dim[a])
+                return fail (PLM_HIP_ERR_GEOMETRY, "moving_grad dims differ from moving");
+    }
+    if (fixed->storage != moving->storage
+        || (fixed->storage != PLM_STORE_F32 && fixed->storage != PLM_STORE_BF16))
+        return fail (PLM_HIP_ERR_ARG, "fixed and moving must use the same storage");
+
+    KGeom g {};
+    int64_t tiles = 1, knots = 1;
+    double moff_d[3], mstep_d[9];
+    rc = plmhip::make_geometry (geom, fixed, moving, g, moff_d, mstep_d, tiles, knots);
+    if (rc) return rc;
 
     HGeom h {};
     bool mi_ok = false;

@@ -20,6 +20,16 @@ int abi_check_img (const plm_img_desc* d, const char* name);
 // A context's device, stream and knot count.
 int ctx_info (const plm_hip_ctx* c, int* device, hipStream_t* stream, int64_t* num_knots);
 
+struct KGeom;
+// The tile-kernel geometry of a B-spline grid over a fixed/moving pair
+// (validated; the moving-index map also in double), its tile and knot
+// counts.
+int make_geometry (const plm_bsp_geom* geom, const plm_img_desc* fixed, const plm_img_desc* moving,
+    KGeom& g, double moff_d[3], double mstep_d[9], int64_t& tiles, int64_t& knots);
+// Cubic B-spline basis B_t(i / vpr) for i < vpr, t < 4, as
+// bspline_basis_eval rounds it (src/base/bspline_xform.cxx:73-99).
+void abi_basis_lut (int vpr, float* lut);
+
 } // namespace plmhip
 
 #endif

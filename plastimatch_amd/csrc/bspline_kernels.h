@@ -115,6 +115,39 @@ struct TileBase {
     int boff;                // element offset of voxel Bi in the moving image
 };
 
+// The frame of tile t: the base B of the tile's first voxel in double, its
+// integer part clamped to the moving image (so every local floor stays
+// within +-mdim of it, and B - Bi is small wherever the tile lands inside
+// the image), and the inside-test and clamp bounds moved into that frame.
+// Host and device (the allocation-free entry point builds the table on the
+// device) run the same double arithmetic.
+__host__ __device__ inline TileBase
+tile_base_of (const KGeom& g, const double moff[3], const double mstep[9], int t)
+{
+    const int p0 = t % g.rdims[0], p1 = (t / g.rdims[0]) % g.rdims[1];
+    const int p2 = t / (g.rdims[0] * g.rdims[1]);
+    const double f0 = (double) (g.roi_off[0] + g.vpr[0] * p0);
+    const double f1 = (double) (g.roi_off[1] + g.vpr[1] * p1);
+    const double f2 = (double) (g.roi_off[2] + g.vpr[2] * p2);
+    TileBase tb;
+    int64_t bi[3];
+    for (int a = 0; a < 3; a++) {
+        const double B = moff[a] + mstep[3*a] * f0 + mstep[3*a+1] * f1 + mstep[3*a+2] * f2;
+        const bool finite = (B - B) == 0.0;
+        double fl = finite ? floor (B) : 0.0;
+        const double top = (double) (g.mdim[a] - 1);
+        fl = fl < 0.0 ? 0.0 : (fl > top ? top : fl);
+        bi[a] = (int64_t) fl;
+        tb.bf[a] = (float) (finite ? B - fl : 0.0);
+        tb.thr_lo[a] = (float) (-0.5 - fl);
+        tb.thr_hi[a] = (float) ((double) g.mdim[a] - 0.5 - fl);
+        tb.cl_lo[a] = (float) -fl;
+        tb.cl_hi[a] = (float) ((double) (g.mdim[a] - 2) - fl);
+    }
+    tb.boff = (int) (bi[0] + (int64_t) g.mdim[0] * (bi[1] + (int64_t) g.mdim[1] * bi[2]));
+    return tb;
+}
+
 // One fixed/moving pair as the tile and reduce kernels see it (a device
 // table of these drives a launch; a single context is a table of one).
 struct PairDev {
@@ -179,6 +212,13 @@ constexpr int MSE_SCORE_ONLY = 2;    // score and overlap count (single-pair pla
 // entry when not null).
 hipError_t launch_mse (const Plan& plan, const Tune& tune, int mode, int storage,
     const float* coeff0, const EvalOut& out0, hipStream_t st, hipEvent_t* ev);
+
+// The same for a single-pair float plan with no z-pair image: the tile
+// kernel reads the trilinear corners from the moving image itself (the
+// allocation-free entry point, which has no z-pair copy), at
+// vox_per_thread 2.  No stage timing.
+hipError_t launch_mse_plain (const Plan& plan, const Tune& tune, int mode, const float* coeff0,
+    const EvalOut& out0, hipStream_t st);
 
 // Dense displacement field of a single-pair plan: float3 (mm) per fixed
 // voxel, the ROI filled and the rest zero.

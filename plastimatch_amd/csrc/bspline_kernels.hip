@@ -783,13 +783,20 @@ advance_pair (const PairDev* __restrict__ pairs, int npairs, int p, int tile)
 // registers for the whole launch.  vf is the dense displacement field of
 // pair 0 (KIND 4 writes it, KIND 5 reads it), float3 interleaved per fixed
 // voxel.
-template <int KIND, typename TI, int V, int G, bool BATCH>
+// KINDX & TILE_PLAIN: float storage with no z-pair image, the corners read
+// from the moving image itself (four 8-byte loads a voxel).
+constexpr int TILE_PLAIN = 16;
+
+template <int KINDX, typename TI, int V, int G, bool BATCH>
 __global__ __launch_bounds__ (TILE_THREADS)
 void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __restrict__ coeff0,
     const float* __restrict__ blut, double* __restrict__ part_score,
     int* __restrict__ part_count, MiArgs mi, float* __restrict__ vf, float fill)
 {
     // what this kind computes
+    constexpr int KIND = KINDX & 15;
+    constexpr bool PLAIN = (KINDX & TILE_PLAIN) != 0;
+    static_assert (!PLAIN || std::is_same<TI, float>::value, "plain corners are float storage only");
     constexpr bool MI = KIND == 2;
     constexpr bool VF_OUT = KIND == 4;          // the displacement field only
     constexpr bool VF_IN = KIND == 5;           // the displacement read from vf
@@ -1019,7 +1026,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         // memory pipeline's cost goes by instructions, not bytes).
         constexpr bool ZPAIR = !MI && !NEAREST;
         constexpr int EZ = 2 * Img<TI>::ES;     // bytes per z-pair record
-        const Rsrc zr = make_rsrc (pd.mpair, ZPAIR ? (uint32_t) (md01 * g.mdim[2]) * EZ : 0u);
+        const Rsrc zr = make_rsrc (pd.mpair, ZPAIR && !PLAIN ? (uint32_t) (md01 * g.mdim[2]) * EZ : 0u);
         const int SXZ = md0 * EZ;
         // The moving-index bounds of the tile's local frame, read once per
         // tile into registers: left in memory, they are reloaded per voxel
@@ -1164,9 +1171,24 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     // the floor voxel's index; the rounded voxel's (the
                     // gradient sample of flavor g) is one per-axis step on
                     const int e = boff + f[0] + __mul24 (f[1], md0) + __mul24 (f[2], md01);
-                    const int off = e * EZ;
-                    Img<TI>::quad (a[u][0], a[u][4], a[u][1], a[u][5], zr, off, 0);
-                    Img<TI>::quad (a[u][2], a[u][6], a[u][3], a[u][7], zr, off, SXZ);
+                    if constexpr (PLAIN) {
+                        // the four x-pairs of rows (y, z), (y + 1, z),
+                        // (y, z + 1), (y + 1, z + 1): one address register,
+                        // the row steps as scalar offsets
+                        const int po = e * 4;
+                        const auto v0 = __builtin_amdgcn_raw_buffer_load_b64 (mr, po, 0, 0);
+                        const auto v1 = __builtin_amdgcn_raw_buffer_load_b64 (mr, po, SX, 0);
+                        const auto v2 = __builtin_amdgcn_raw_buffer_load_b64 (mr, po, SXY, 0);
+                        const auto v3 = __builtin_amdgcn_raw_buffer_load_b64 (mr, po, SX + SXY, 0);
+                        a[u][0] = __uint_as_float (v0[0]); a[u][1] = __uint_as_float (v0[1]);
+                        a[u][2] = __uint_as_float (v1[0]); a[u][3] = __uint_as_float (v1[1]);
+                        a[u][4] = __uint_as_float (v2[0]); a[u][5] = __uint_as_float (v2[1]);
+                        a[u][6] = __uint_as_float (v3[0]); a[u][7] = __uint_as_float (v3[1]);
+                    } else {
+                        const int off = e * EZ;
+                        Img<TI>::quad (a[u][0], a[u][4], a[u][1], a[u][5], zr, off, 0);
+                        Img<TI>::quad (a[u][2], a[u][6], a[u][3], a[u][7], zr, off, SXZ);
+                    }
                     if (KIND == 0) {
                         // nearest voxel: the gradient sample of flavor g
                         // (src/register/bspline_mse.cxx:679-681)
@@ -2152,6 +2174,42 @@ launch_mse (const Plan& plan, const Tune& tune, int mode, int storage, const flo
         grad ? 1 : 0);
     LAUNCH_CHECK ();
     return mark (ev, 2, st);
+}
+
+// The MSE evaluation of a single-pair float plan whose pair has no z-pair
+// image (the corners come from the moving image), at vox_per_thread 2.
+template <int KIND>
+static int
+launch_tiles_plain (const Plan& plan, const Tune& tune, hipStream_t st, const TileArgs& ta)
+{
+    const KGeom& g = plan.g0;
+    constexpr size_t stat = MAX_BATCH * (sizeof (double) + sizeof (int));
+    const size_t lds = tile_lds_bytes (g, 2) - stat;
+    const int v = g.vpr[0];
+    const bool iso = g.vpr[1] == v && g.vpr[2] == v;
+    constexpr int K = KIND | TILE_PLAIN;
+    if (iso && v == 8) return launch_tiles_vgb<K, float, 8, 2, false> (plan, tune, lds, st, ta);
+    if (iso && v == 10) return launch_tiles_vgb<K, float, 10, 2, false> (plan, tune, lds, st, ta);
+    if (iso && v == 12) return launch_tiles_vgb<K, float, 12, 2, false> (plan, tune, lds, st, ta);
+    if (iso && v == 15) return launch_tiles_vgb<K, float, 15, 2, false> (plan, tune, lds, st, ta);
+    return launch_tiles_vgb<K, float, 0, 2, false> (plan, tune, lds, st, ta);
+}
+
+hipError_t
+launch_mse_plain (const Plan& plan, const Tune& tune, int mode, const float* coeff0,
+    const EvalOut& out0, hipStream_t st)
+{
+    const TileArgs ta {coeff0, MiArgs {}, nullptr};
+    int nparts;
+    if (mode == MSE_REF_NN) nparts = launch_tiles_plain<0> (plan, tune, st, ta);
+    else if (mode == MSE_EXACT) nparts = launch_tiles_plain<1> (plan, tune, st, ta);
+    else nparts = launch_tiles_plain<3> (plan, tune, st, ta);
+    LAUNCH_CHECK ();
+    const bool grad = mode != MSE_SCORE_ONLY;
+    hipLaunchKernelGGL (k_reduce<true>, dim3 (grad ? plan.reduce_blocks : 1), dim3 (REDUCE_THREADS),
+        0, st, plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts, out0,
+        grad ? 1 : 0);
+    return hipGetLastError ();
 }
 
 hipError_t
