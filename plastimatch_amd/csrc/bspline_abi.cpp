@@ -199,6 +199,7 @@ struct plm_hip_ctx {
     uint8_t* d_fbin = nullptr;              // MI bin images (once per stage)
     uint8_t* d_mbin = nullptr;
     uint32_t* d_mquad = nullptr;    // moving bins of each voxel's (x, y) quad
+    float* d_ptab = nullptr;        // MI pass-1 walk positions (only where they differ from direct)
     // staging for the host-pointer entry point
     float* d_coeff = nullptr;
     double* d_grad = nullptr;
@@ -236,7 +237,8 @@ free_ctx (plm_hip_ctx* c)
     c->timer.destroy ();
     void* bufs[] = {c->d_fixed, c->d_moving, c->d_mpair, c->d_mgrad, c->d_blut, c->d_tbase, c->d_cond,
         c->d_tile_score, c->d_tile_count, c->d_pair, c->d_hist,
-        c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_mquad, c->d_coeff, c->d_grad, c->d_score, c->d_nvox,
+        c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_mquad, c->d_ptab, c->d_coeff, c->d_grad,
+        c->d_score, c->d_nvox,
         c->d_hist_out, c->d_reg_part, c->d_reg_cond};
     for (void* b : bufs) if (b) (void) hipFree (b);
     if (c->stream) (void) hipStreamDestroy (c->stream);
@@ -584,6 +586,27 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
         const int mdim[3] = {(int) moving->dim[0], (int) moving->dim[1], (int) moving->dim[2]};
         e = plmhip::launch_bin_quads (c->d_mbin, c->d_mquad, mdim, c->stream);
         if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "bin quads"); }
+        // pass 1 walks the fixed image as the reference's LOOP_Z/Y/X do;
+        // keep those positions where they differ from the direct formula
+        // (not on dyadic axis-aligned grids)
+        unsigned int* d_bad = nullptr;
+        CTX_TRY (alloc (&d_bad, 1, "hipMalloc(mismatch)"));
+        unsigned int bad = 0;
+        e = hipMemcpy (d_bad, &bad, sizeof bad, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = plmhip::launch_pass1_positions (g, nullptr, d_bad, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize (c->stream);
+        if (e == hipSuccess) e = hipMemcpy (&bad, d_bad, sizeof bad, hipMemcpyDeviceToHost);
+        (void) hipFree (d_bad);
+        if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "pass-1 positions"); }
+        if (bad > 0) {
+            CTX_TRY (alloc (&c->d_ptab, 3 * (size_t) nf, "hipMalloc(pass-1 positions)"));
+            unsigned int* d_b2 = nullptr;
+            CTX_TRY (alloc (&d_b2, 1, "hipMalloc(mismatch)"));
+            e = plmhip::launch_pass1_positions (g, c->d_ptab, d_b2, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize (c->stream);
+            (void) hipFree (d_b2);
+            if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "pass-1 positions"); }
+        }
     }
     CTX_TRY (alloc (&c->d_coeff, 3 * knots, "hipMalloc(coeff)"));
     e = hipMemset (c->d_coeff, 0, sizeof (float) * 3 * knots);
@@ -660,7 +683,7 @@ plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
         const plmhip::EvalOut out {d_grad, d_score, d_num_vox};
         e = plmhip::launch_mi (c->plan, c->tune, c->h, c->hist_blocks, c->d_fbin, c->d_mquad,
             d_coeff, c->d_hist, c->d_hist + (size_t) c->h.fbins * c->h.mbins, c->d_table,
-            c->d_nvox_f, out, d_hist, st, ev);
+            c->d_nvox_f, out, d_hist, c->d_ptab, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MI launch");
     }
     if (ev) c->timer.last_stages = metric == PLM_METRIC_MSE ? plmhip::MSE_STAGES : plmhip::MI_STAGES;

@@ -252,11 +252,17 @@ struct RegOp {
 // scratch; score = the score, grad = (accumulate ? grad : 0) + gradient.
 hipError_t launch_regularize (const KGeom& g, const RegOp& op, const float* coeff, double* part,
     double* cond, double* score, double* grad, int accumulate, hipStream_t st);
-// Mattes MI of a single-pair plan.
+// Mattes MI of a single-pair plan.  ptab: the fixed positions of pass 1's
+// walk (launch_pass1_positions), or null where they equal the direct ones.
 hipError_t launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     const uint8_t* fbin, const uint32_t* mquad, const float* coeff, unsigned long long* jhist,
     unsigned long long* fhist, float* table, float* nvox_f, const EvalOut& out0,
-    int64_t* hist_out, hipStream_t st, hipEvent_t* ev);
+    int64_t* hist_out, const float* ptab, hipStream_t st, hipEvent_t* ev);
+// The reference's pass-1 walk of the fixed image (LOOP_Z/Y/X): adds to
+// *mismatch the voxels whose walk position differs from origin + step ijk,
+// and with out (float3 per fixed voxel) not null writes the walk positions.
+hipError_t launch_pass1_positions (const KGeom& g, float* out, unsigned int* mismatch,
+    hipStream_t st);
 
 hipError_t launch_bin_image (const void* img, int storage, uint8_t* out, int64_t n,
     float offset, float delta, int bins, hipStream_t st);

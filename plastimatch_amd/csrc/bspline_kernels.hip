@@ -1566,7 +1566,7 @@ __global__ __launch_bounds__ (TILE_THREADS)
 void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
     const uint32_t* __restrict__ mquad, const float* __restrict__ coeff,
     const float* __restrict__ blut, unsigned long long* __restrict__ jhist,
-    unsigned long long* __restrict__ fhist)
+    unsigned long long* __restrict__ fhist, const float* __restrict__ ptab)
 {
     // the histograms first (the dynamic LDS base is 16-byte aligned), then
     // the LUTs; every pointer stays an LDS pointer, so the atomics are LDS
@@ -1607,7 +1607,17 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
             Sample s;
             float d[3] = {0.f, 0.f, 0.f};
             float fxyz[3];
-            position<ALIGNED> (fxyz, G_, fi, fj, fk);
+            if (ptab) {
+                // the fixed positions of the reference's LOOP_Z/Y/X walk,
+                // where they differ from the direct formula (see
+                // k_pass1_positions)
+                const int fidx0 = act ? fi + G_.fdim[0] * (fj + G_.fdim[1] * fk) : 0;
+                fxyz[0] = ptab[3 * fidx0 + 0];
+                fxyz[1] = ptab[3 * fidx0 + 1];
+                fxyz[2] = ptab[3 * fidx0 + 2];
+            } else {
+                position<ALIGNED> (fxyz, G_, fi, fj, fk);
+            }
             if (act)
                 exact_displacement (d, &s_bx[4 * qx], &s_by[4 * qy], &s_bz[4 * qz], coeff, knot0,
                     G_.cdims[0], G_.cdims[0] * G_.cdims[1]);
@@ -1646,6 +1656,60 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
     __syncthreads ();
     for (int i = threadIdx.x; i < h.fbins * h.mbins; i += blockDim.x)
         if (s_j[i]) atomicAdd (&jhist[i], s_j[i]);
+}
+
+// Pass 1 of the reference walks the fixed image with the LOOP_Z / LOOP_Y /
+// LOOP_X macros (src/base/volume_macros.h:38-75, used at
+// src/register/bspline_mi.cxx:1567-1572): each row (j, k) starts at
+// (origin + k step[:,2]) + j step[:,1] and every x step adds step[:,0] to the
+// float position, so voxel i of a row carries i rounded additions; the tile
+// kernels form origin + step ijk directly (POSITION_FROM_COORDS, as pass 2
+// does).  The two agree wherever the products and sums are exact (dyadic
+// spacings on axis-aligned images); elsewhere the PV weights, and so the
+// histogram, would differ.  One thread per row replays the walk: counts the
+// voxels whose walk position differs from the direct one (mismatch), and
+// with out writes the walk positions (float3 per voxel) for pass 1.
+template <bool ALIGNED>
+__global__ __launch_bounds__ (256)
+void k_pass1_positions (KGeom g, float* __restrict__ out, unsigned int* __restrict__ mismatch)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= g.fdim[1] * g.fdim[2]) return;
+    const int j = row % g.fdim[1], k = row / g.fdim[1];
+    const float fj = (float) j, fk = (float) k;
+    const float* s = g.fstep;
+    float w[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) w[a] = (g.img_origin[a] + fk * s[3 * a + 2]) + fj * s[3 * a + 1];
+    unsigned int bad = 0;
+    const size_t base = (size_t) g.fdim[0] * row;
+    for (int i = 0; i < g.fdim[0]; i++) {
+        float d[3];
+        position<ALIGNED> (d, g, i, j, k);
+        bad += (d[0] != w[0] || d[1] != w[1] || d[2] != w[2]) ? 1u : 0u;
+        if (out) {
+            out[3 * (base + i) + 0] = w[0];
+            out[3 * (base + i) + 1] = w[1];
+            out[3 * (base + i) + 2] = w[2];
+        }
+        w[0] += s[0];
+        w[1] += s[3];
+        w[2] += s[6];
+    }
+    if (bad) atomicAdd (mismatch, bad);
+}
+
+hipError_t
+launch_pass1_positions (const KGeom& g, float* out, unsigned int* mismatch, hipStream_t st)
+{
+    const int rows = g.fdim[1] * g.fdim[2];
+    if (g.aligned)
+        hipLaunchKernelGGL ((k_pass1_positions<true>), dim3 ((rows + 255) / 256), dim3 (256), 0, st, g,
+            out, mismatch);
+    else
+        hipLaunchKernelGGL ((k_pass1_positions<false>), dim3 ((rows + 255) / 256), dim3 (256), 0, st, g,
+            out, mismatch);
+    return hipGetLastError ();
 }
 
 // Score from the integer histograms, and the per-bin dS/dP table for
@@ -2244,7 +2308,7 @@ hipError_t
 launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     const uint8_t* fbin, const uint32_t* mquad, const float* coeff, unsigned long long* jhist,
     unsigned long long* fhist, float* table, float* nvox_f, const EvalOut& out0,
-    int64_t* hist_out, hipStream_t st, hipEvent_t* ev)
+    int64_t* hist_out, const float* ptab, hipStream_t st, hipEvent_t* ev)
 {
     const KGeom& g = plan.g0;
     const int nbins = h.fbins * h.mbins + h.fbins;
@@ -2259,10 +2323,10 @@ launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
     if (g.aligned)
         hipLaunchKernelGGL ((k_mi_hist<true>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
-            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist);
+            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist, ptab);
     else
         hipLaunchKernelGGL ((k_mi_hist<false>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
-            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist);
+            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist, ptab);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 2, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_mi_score, dim3 (1), dim3 (1024), 0, st, h, jhist, fhist, table,
