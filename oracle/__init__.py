@@ -23,6 +23,12 @@ class OrcVolHdr(ctypes.Structure):
     _fields_ = [("dim", c_int64_3), ("origin", c_float3), ("spacing", c_float3), ("dc", c_float9)]
 
 
+class OrcRois(ctypes.Structure):
+    """Optional fixed / moving regions of interest (uchar volumes)."""
+    _fields_ = [("fixed_hdr", ctypes.POINTER(OrcVolHdr)), ("fixed", ctypes.c_void_p),
+                ("moving_hdr", ctypes.POINTER(OrcVolHdr)), ("moving", ctypes.c_void_p)]
+
+
 class OrcGeom(ctypes.Structure):
     _fields_ = [("img_origin", c_float3), ("img_spacing", c_float3), ("img_dim", c_int64_3),
                 ("roi_offset", c_int64_3), ("roi_dim", c_int64_3), ("vox_per_rgn", c_int64_3),
@@ -114,11 +120,11 @@ def lib():
         L.orc_mse_score.restype = ctypes.c_int
         L.orc_mse_score.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
                                     ctypes.POINTER(OrcVolHdr), P, P, P, ctypes.c_int, ctypes.c_int,
-                                    P, P, P, P]
+                                    ctypes.POINTER(OrcRois), P, P, P, P]
         L.orc_mse_score_f64.restype = ctypes.c_int
         L.orc_mse_score_f64.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
                                         ctypes.POINTER(OrcVolHdr), P, P, P, ctypes.c_int, ctypes.c_int,
-                                        P, P, P]
+                                        ctypes.POINTER(OrcRois), P, P, P]
         L.orc_pv_quantize.argtypes = [P, P]
         L.orc_reg_analytic.restype = ctypes.c_int
         L.orc_reg_analytic.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcRegParms), P, ctypes.c_int,
@@ -128,19 +134,20 @@ def lib():
         L.orc_mi_score.restype = ctypes.c_int
         L.orc_mi_score.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
                                    ctypes.POINTER(OrcVolHdr), P, P, ctypes.POINTER(OrcHist),
-                                   ctypes.c_int, P, P, P, P, P, P, P, P]
+                                   ctypes.c_int, ctypes.POINTER(OrcRois), P, P, P, P, P, P, P, P]
         L.orc_mi_score_f64.restype = ctypes.c_int
         L.orc_mi_score_f64.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
                                        ctypes.POINTER(OrcVolHdr), P, P, ctypes.POINTER(OrcHist),
-                                       ctypes.c_int, P, P, P]
+                                       ctypes.c_int, ctypes.POINTER(OrcRois), P, P, P]
         L.orc_mse_f64_support.restype = ctypes.c_int
         L.orc_mse_f64_support.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
                                           ctypes.POINTER(OrcVolHdr), P, P, ctypes.c_int64,
-                                          ctypes.c_double, P, P, P]
+                                          ctypes.c_double, ctypes.POINTER(OrcRois), P, P, P]
         L.orc_mi_score_f64_hist.restype = ctypes.c_int
         L.orc_mi_score_f64_hist.argtypes = [ctypes.POINTER(OrcGeom), ctypes.POINTER(OrcVolHdr), P,
                                             ctypes.POINTER(OrcVolHdr), P, P, ctypes.POINTER(OrcHist),
-                                            ctypes.c_int, P, ctypes.c_int64, P, P, P]
+                                            ctypes.c_int, ctypes.POINTER(OrcRois), P, ctypes.c_int64, P,
+                                            P, P]
         L.orc_resample_geometry.restype = ctypes.c_int
         L.orc_resample_geometry.argtypes = [ctypes.c_int, ctypes.POINTER(OrcVolHdr), P, P, P, P]
         L.orc_volume_resample.argtypes = [ctypes.POINTER(OrcVolHdr), P, P, P, P, ctypes.c_int, P]
@@ -304,7 +311,9 @@ class Case:
 
     def __init__(self, fixed, moving, spacing, vox_per_rgn, origin=(0.0, 0.0, 0.0),
                  roi_offset=(0, 0, 0), roi_dim=None, dc=IDENTITY, m_spacing=None, m_origin=None,
-                 m_dc=None):
+                 m_dc=None, fixed_roi=None, moving_roi=None):
+        """fixed_roi / moving_roi: optional uchar region-of-interest volumes,
+        each (image zyx, spacing, origin, direction cosines)."""
         self.fixed = np.ascontiguousarray(fixed, np.float32)
         self.moving = np.ascontiguousarray(moving, np.float32)
         nz, ny, nx = self.fixed.shape
@@ -321,6 +330,22 @@ class Case:
         self.rdims, self.cdims, self.num_knots = header(self.g)
         self.num_coeff = 3 * self.num_knots
         self._m_grad = None
+        self._roi_keep = []
+        self.rois = None
+        if fixed_roi is not None or moving_roi is not None:
+            r = OrcRois()
+            for kind, roi in (("fixed", fixed_roi), ("moving", moving_roi)):
+                if roi is None:
+                    continue
+                img = np.ascontiguousarray(roi[0], np.uint8)
+                hdr = vol_hdr(img, *roi[1:])
+                self._roi_keep += [img, hdr]
+                setattr(r, kind + "_hdr", ctypes.pointer(hdr))
+                setattr(r, kind, img.ctypes.data)
+            self.rois = r
+
+    def _rois(self):
+        return ctypes.byref(self.rois) if self.rois is not None else None
 
     @property
     def m_grad(self):
@@ -338,7 +363,7 @@ class Case:
         mg = self.m_grad if flavor in ("g", "c") else None
         rc = lib().orc_mse_score(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
                                  ctypes.byref(self.mh), _ptr(self.moving), _ptr(mg), _ptr(coeff),
-                                 ord(flavor), nthreads, _ptr(raw), _ptr(sc), _ptr(grad), _ptr(nv))
+                                 ord(flavor), nthreads, self._rois(), _ptr(raw), _ptr(sc), _ptr(grad), _ptr(nv))
         assert rc == 0, rc
         return float(sc[0]), grad, int(nv[0]), float(raw[0])
 
@@ -351,7 +376,7 @@ class Case:
         mg = self.m_grad if grad_mode == 0 else None
         rc = lib().orc_mse_score_f64(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
                                      ctypes.byref(self.mh), _ptr(self.moving), _ptr(mg), _ptr(coeff),
-                                     grad_mode, nthreads, _ptr(sc), _ptr(grad), _ptr(nv))
+                                     grad_mode, nthreads, self._rois(), _ptr(sc), _ptr(grad), _ptr(nv))
         assert rc == 0, rc
         return float(sc[0]), grad, int(nv[0])
 
@@ -364,7 +389,7 @@ class Case:
         nc = np.zeros(1, np.int64)
         rc = lib().orc_mse_f64_support(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
                                        ctypes.byref(self.mh), _ptr(self.moving), _ptr(coeff), int(ci),
-                                       float(delta), _ptr(raw), _ptr(n), _ptr(nc))
+                                       float(delta), self._rois(), _ptr(raw), _ptr(n), _ptr(nc))
         assert rc == 0, rc
         return float(raw[0]), int(n[0]), int(nc[0])
 
@@ -405,7 +430,8 @@ class Case:
         fxj = np.zeros(fb * mb, np.int64)
         rc = lib().orc_mi_score(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
                                 ctypes.byref(self.mh), _ptr(self.moving), _ptr(coeff), ctypes.byref(hp),
-                                nthreads, _ptr(sc), _ptr(grad), _ptr(nv), _ptr(fh), _ptr(mh), _ptr(jh),
+                                nthreads, self._rois(), _ptr(sc), _ptr(grad), _ptr(nv), _ptr(fh), _ptr(mh),
+                                _ptr(jh),
                                 _ptr(fxf), _ptr(fxj))
         assert rc == 0, rc
         return dict(score=float(sc[0]), grad=grad, num_vox=int(nv[0]), f_hist=fh, m_hist=mh,
@@ -422,13 +448,14 @@ class Case:
         if hist is None:
             rc = lib().orc_mi_score_f64(ctypes.byref(self.g), ctypes.byref(self.fh), _ptr(self.fixed),
                                         ctypes.byref(self.mh), _ptr(self.moving), _ptr(coeff),
-                                        ctypes.byref(hp), nthreads, _ptr(sc), _ptr(grad), _ptr(nv))
+                                        ctypes.byref(hp), nthreads, self._rois(), _ptr(sc), _ptr(grad),
+                                        _ptr(nv))
         else:
             h = np.ascontiguousarray(np.concatenate([hist["f_hist"], hist["m_hist"], hist["j_hist"]]),
                                      np.float64)
             rc = lib().orc_mi_score_f64_hist(ctypes.byref(self.g), ctypes.byref(self.fh),
                                              _ptr(self.fixed), ctypes.byref(self.mh), _ptr(self.moving),
-                                             _ptr(coeff), ctypes.byref(hp), nthreads, _ptr(h),
+                                             _ptr(coeff), ctypes.byref(hp), nthreads, self._rois(), _ptr(h),
                                              int(hist["num_vox"]), _ptr(sc), _ptr(grad), _ptr(nv))
         assert rc == 0, rc
         return float(sc[0]), grad, int(nv[0])

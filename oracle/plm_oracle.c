@@ -227,6 +227,76 @@ bxf_free (bxf_t* b)
     free (b->q_lut); free (b->c_lut);
 }
 
+/* A uchar region-of-interest volume (Bspline_state::fixed_roi /
+   moving_roi), with its own geometry. */
+typedef struct roi_t {
+    int64_t dim[3];
+    float origin[3];
+    float proj[9];
+    double dorigin[3];
+    double dproj[9];
+    const uint8_t* img;
+} roi_t;
+
+static void dproj_of (double* proj, const orc_vol_hdr* h);
+
+static const roi_t*
+roi_init (roi_t* r, const orc_vol_hdr* h, const uint8_t* img)
+{
+    if (!h || !img) return 0;
+    vol_t v;
+    vol_init (&v, h, 0);
+    for (int d = 0; d < 3; d++) {
+        r->dim[d] = v.dim[d];
+        r->origin[d] = v.origin[d];
+        r->dorigin[d] = h->origin[d];
+    }
+    for (int i = 0; i < 9; i++) r->proj[i] = v.proj[i];
+    dproj_of (r->dproj, h);
+    r->img = img;
+    return r;
+}
+
+/* 1 if xyz (mm) falls on a non-zero ROI voxel: index = floor of the
+   projected position, clamped into the volume (below 0 -> 0, at or past dim
+   -> dim - 1).  Follows inside_roi and float_to_plm_long_clamp,
+   src/util/bspline_correspond.cxx:16-74, with PROJECT_X/Y/Z,
+   src/base/volume_macros.h:75-80. */
+static inline int
+inside_roi (const float xyz[3], const roi_t* r)
+{
+    float tmp[3], pf[3];
+    int64_t p[3];
+    tmp[0] = xyz[0] - r->origin[0];
+    tmp[1] = xyz[1] - r->origin[1];
+    tmp[2] = xyz[2] - r->origin[2];
+    for (int d = 0; d < 3; d++)
+        pf[d] = tmp[0] * r->proj[3*d+0] + tmp[1] * r->proj[3*d+1] + tmp[2] * r->proj[3*d+2];
+    for (int d = 0; d < 3; d++) {
+        if (pf[d] < 0.f) p[d] = 0;
+        else if (pf[d] >= (float) r->dim[d]) p[d] = r->dim[d] - 1;
+        else p[d] = (int64_t) floor (pf[d]);
+    }
+    return (int) r->img[p[0] + r->dim[0] * (p[1] + r->dim[1] * p[2])];
+}
+
+/* The same test in double (the FP64 yardstick's positions). */
+static inline int
+inside_roi_d (const double xyz[3], const roi_t* r)
+{
+    double tmp[3], pf[3];
+    int64_t p[3];
+    for (int d = 0; d < 3; d++) tmp[d] = xyz[d] - r->dorigin[d];
+    for (int d = 0; d < 3; d++)
+        pf[d] = tmp[0] * r->dproj[3*d+0] + tmp[1] * r->dproj[3*d+1] + tmp[2] * r->dproj[3*d+2];
+    for (int d = 0; d < 3; d++) {
+        if (pf[d] < 0.0) p[d] = 0;
+        else if (pf[d] >= (double) r->dim[d]) p[d] = r->dim[d] - 1;
+        else p[d] = (int64_t) floor (pf[d]);
+    }
+    return (int) r->img[p[0] + r->dim[0] * (p[1] + r->dim[1] * p[2])];
+}
+
 /* ---------------------------------------------------------------------
    Per-voxel primitives (float)
    --------------------------------------------------------------------- */
@@ -633,8 +703,8 @@ mse_normalize (double raw, int64_t n, float* score, float* grad, int64_t nc)
    derivative), src/register/bspline_mse.cxx:1149-1384. */
 static void
 mse_tiles (const bxf_t* b, const vol_t* fx, const vol_t* mv,
-    const float* m_grad, int analytic, double* raw, int64_t* nvox,
-    float* grad)
+    const float* m_grad, int analytic, const roi_t* froi, const roi_t* mroi,
+    double* raw, int64_t* nvox, float* grad)
 {
     int64_t csz = 64 * b->num_knots;
     float* cx = (float*) calloc (csz, sizeof(float));
@@ -667,10 +737,19 @@ mse_tiles (const bxf_t* b, const vol_t* fx, const vol_t* mv,
             for (int d = 0; d < 3; d++)
                 fxyz[d] = b->img_origin[d] + fijk[0] * fx->step[3*d+0]
                     + fijk[1] * fx->step[3*d+1] + fijk[2] * fx->step[3*d+2];
+            /* flavors i and r: the fixed ROI at the fixed position, the
+               moving ROI at the moving one (src/register/bspline_mse.cxx:
+               172-196, bspline_find_correspondence_dcos_roi,
+               src/util/bspline_correspond.cxx:113-145) */
+            if (froi && !inside_roi (fxyz, froi)) continue;
             int64_t fidx = fijk[0] + fx->dim[0] * (fijk[1] + fx->dim[1] * fijk[2]);
             float dxyz[3], mijk[3];
             interp_pix_c (dxyz, b, pidx, q);
             if (!find_correspondence (mijk, fxyz, dxyz, mv)) continue;
+            if (mroi) {
+                float mxyz[3] = {fxyz[0] + dxyz[0], fxyz[1] + dxyz[1], fxyz[2] + dxyz[2]};
+                if (!inside_roi (mxyz, mroi)) continue;
+            }
             int64_t mf[3], mr[3];
             float l1[3], l2[3];
             li_clamp_3d (mijk, mf, mr, l1, l2, mv);
@@ -782,11 +861,15 @@ int
 orc_mse_score (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img, const float* m_grad,
-    const float* coeff, int flavor, int nthreads,
+    const float* coeff, int flavor, int nthreads, const orc_rois* rois,
     double* raw_score, float* score, float* grad, int64_t* num_vox)
 {
     bxf_t b;
     vol_t fx, mv;
+    roi_t fr_, mr_;
+    const roi_t* froi = rois ? roi_init (&fr_, rois->fixed_hdr, rois->fixed) : 0;
+    const roi_t* mroi = rois ? roi_init (&mr_, rois->moving_hdr, rois->moving) : 0;
+    if (flavor == 'c' && (froi || mroi)) return -4;
     if (bxf_init (&b, g, coeff)) return -1;
     vol_init (&fx, fh, f_img);
     vol_init (&mv, mh, m_img);
@@ -796,9 +879,9 @@ orc_mse_score (const orc_geom* g,
     int64_t n = 0;
     if (flavor == 'g') {
         if (!m_grad) { bxf_free (&b); return -2; }
-        mse_tiles (&b, &fx, &mv, m_grad, 0, &raw, &n, grad);
+        mse_tiles (&b, &fx, &mv, m_grad, 0, froi, mroi, &raw, &n, grad);
     } else if (flavor == 'p') {
-        mse_tiles (&b, &fx, &mv, 0, 1, &raw, &n, grad);
+        mse_tiles (&b, &fx, &mv, 0, 1, froi, mroi, &raw, &n, grad);
     } else if (flavor == 'c') {
         if (!m_grad) { bxf_free (&b); return -2; }
         mse_voxel_order (&b, &fx, &mv, m_grad, &raw, &n, grad);
@@ -837,6 +920,29 @@ typedef struct dvol_t {
     double inv_sp[3];
     const float* img;
 } dvol_t;
+
+/* proj = inv(dc) / spacing in double, as dvol_init forms it. */
+static void
+dproj_of (double* proj, const orc_vol_hdr* h)
+{
+    const float* m = h->dc;
+    double a[9], inv[9];
+    for (int i = 0; i < 9; i++) a[i] = m[i];
+    double det = a[0]*(a[4]*a[8]-a[5]*a[7]) - a[1]*(a[3]*a[8]-a[5]*a[6])
+        + a[2]*(a[3]*a[7]-a[4]*a[6]);
+    inv[0] =  (a[4]*a[8]-a[5]*a[7]) / det;
+    inv[1] = -(a[1]*a[8]-a[2]*a[7]) / det;
+    inv[2] =  (a[1]*a[5]-a[2]*a[4]) / det;
+    inv[3] = -(a[3]*a[8]-a[5]*a[6]) / det;
+    inv[4] =  (a[0]*a[8]-a[2]*a[6]) / det;
+    inv[5] = -(a[0]*a[5]-a[2]*a[3]) / det;
+    inv[6] =  (a[3]*a[7]-a[4]*a[6]) / det;
+    inv[7] = -(a[0]*a[7]-a[1]*a[6]) / det;
+    inv[8] =  (a[0]*a[4]-a[1]*a[3]) / det;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            proj[3*i+j] = inv[3*i+j] / (double) h->spacing[i];
+}
 
 static void
 dvol_init (dvol_t* v, const orc_vol_hdr* h, const float* img)
@@ -1007,11 +1113,14 @@ int
 orc_mse_score_f64 (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img, const float* m_grad,
-    const float* coeff, int grad_mode, int nthreads,
+    const float* coeff, int grad_mode, int nthreads, const orc_rois* rois,
     double* score, double* grad, int64_t* num_vox)
 {
     dgeo_t d;
     dvol_t fx, mv;
+    roi_t fr_, mr_;
+    const roi_t* froi = rois ? roi_init (&fr_, rois->fixed_hdr, rois->fixed) : 0;
+    const roi_t* mroi = rois ? roi_init (&mr_, rois->moving_hdr, rois->moving) : 0;
     if (dgeo_init (&d, g)) return -1;
     if (grad_mode == 0 && !m_grad) { dgeo_free (&d); return -2; }
     dvol_init (&fx, fh, f_img);
@@ -1041,9 +1150,14 @@ orc_mse_score_f64 (const orc_geom* g,
             for (int a = 0; a < 3; a++)
                 fxyz[a] = d.origin[a] + fijk[0] * fx.step[3*a+0]
                     + fijk[1] * fx.step[3*a+1] + fijk[2] * fx.step[3*a+2];
+            if (froi && !inside_roi_d (fxyz, froi)) continue;
             dinterp (dxyz, &d, p, q, coeff);
             dsample_t s;
             if (!dsample (&s, fxyz, dxyz, &mv)) continue;
+            if (mroi) {
+                double mxyz[3] = {fxyz[0] + dxyz[0], fxyz[1] + dxyz[1], fxyz[2] + dxyz[2]};
+                if (!inside_roi_d (mxyz, mroi)) continue;
+            }
             double a8[8];
             dcorners (a8, &s, &mv);
             const double* l1 = s.l1;
@@ -1102,11 +1216,14 @@ int
 orc_mse_f64_support (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
-    const float* coeff, int64_t ci, double delta,
+    const float* coeff, int64_t ci, double delta, const orc_rois* rois,
     double* raw_out, int64_t* n_out, int64_t* nclamp_out)
 {
     dgeo_t d;
     dvol_t fx, mv;
+    roi_t fr_, mr_;
+    const roi_t* froi = rois ? roi_init (&fr_, rois->fixed_hdr, rois->fixed) : 0;
+    const roi_t* mroi = rois ? roi_init (&mr_, rois->moving_hdr, rois->moving) : 0;
     if (dgeo_init (&d, g)) return -1;
     if (ci < 0 || ci >= 3 * d.num_knots) { dgeo_free (&d); return -2; }
     dvol_init (&fx, fh, f_img);
@@ -1140,8 +1257,13 @@ orc_mse_f64_support (const orc_geom* g,
             /* the moved coefficient's share: its basis weight times delta */
             const int64_t i = kk[0] - p[0], j = kk[1] - p[1], k = kk[2] - p[2];
             dxyz[comp] += d.bx[q[0]*4+i] * d.by[q[1]*4+j] * d.bz[q[2]*4+k] * delta;
+            if (froi && !inside_roi_d (fxyz, froi)) continue;
             dsample_t s;
             if (!dsample (&s, fxyz, dxyz, &mv)) continue;
+            if (mroi) {
+                double mxyz[3] = {fxyz[0] + dxyz[0], fxyz[1] + dxyz[1], fxyz[2] + dxyz[2]};
+                if (!inside_roi_d (mxyz, mroi)) continue;
+            }
             double t[3], mijk[3];
             for (int a = 0; a < 3; a++) t[a] = fxyz[a] + dxyz[a] - mv.origin[a];
             for (int a = 0; a < 3; a++)
@@ -1350,13 +1472,16 @@ int
 orc_mi_score (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
-    const float* coeff, const orc_hist* hp, int nthreads,
+    const float* coeff, const orc_hist* hp, int nthreads, const orc_rois* rois,
     float* score, float* grad, int64_t* num_vox,
     double* f_hist, double* m_hist, double* j_hist,
     int64_t* fx_f_hist, int64_t* fx_j_hist)
 {
     bxf_t b;
     vol_t fx, mv;
+    roi_t fr_, mr_;
+    const roi_t* froi = rois ? roi_init (&fr_, rois->fixed_hdr, rois->fixed) : 0;
+    const roi_t* mroi = rois ? roi_init (&mr_, rois->moving_hdr, rois->moving) : 0;
     if (bxf_init (&b, g, coeff)) return -1;
     vol_init (&fx, fh, f_img);
     vol_init (&mv, mh, m_img);
@@ -1387,8 +1512,16 @@ orc_mi_score (const orc_geom* g,
                     p[0] = fijk[0] / b.vpr[0]; q[0] = fijk[0] % b.vpr[0];
                     int64_t pidx = p[0] + b.rdims[0] * (p[1] + b.rdims[1] * p[2]);
                     float dxyz[3], mijk[3];
-                    interp_pix_c (dxyz, &b, pidx, q);
-                    if (find_correspondence (mijk, fxyz, dxyz, &mv)) {
+                    /* flavor h: the ROIs of bspline_score_h_mi,
+                       src/register/bspline_mi.cxx:1304-1326 */
+                    int keep = !froi || inside_roi (fxyz, froi);
+                    if (keep) interp_pix_c (dxyz, &b, pidx, q);
+                    if (keep) keep = find_correspondence (mijk, fxyz, dxyz, &mv);
+                    if (keep && mroi) {
+                        float mxyz[3] = {fxyz[0] + dxyz[0], fxyz[1] + dxyz[1], fxyz[2] + dxyz[2]};
+                        keep = inside_roi (mxyz, mroi);
+                    }
+                    if (keep) {
                         int64_t mf[3], mr[3];
                         float l1[3], l2[3];
                         li_clamp_3d (mijk, mf, mr, l1, l2, &mv);
@@ -1451,9 +1584,14 @@ orc_mi_score (const orc_geom* g,
             for (int d = 0; d < 3; d++)
                 fxyz[d] = b.img_origin[d] + fijk[0] * fx.step[3*d+0]
                     + fijk[1] * fx.step[3*d+1] + fijk[2] * fx.step[3*d+2];
+            if (froi && !inside_roi (fxyz, froi)) continue;
             float dxyz[3], mijk[3];
             interp_pix_c (dxyz, &b, pidx, q);
             if (!find_correspondence (mijk, fxyz, dxyz, &mv)) continue;
+            if (mroi) {
+                float mxyz[3] = {fxyz[0] + dxyz[0], fxyz[1] + dxyz[1], fxyz[2] + dxyz[2]};
+                if (!inside_roi (mxyz, mroi)) continue;
+            }
             int64_t mf[3], mr[3];
             float l1[3], l2[3];
             li_clamp_3d (mijk, mf, mr, l1, l2, &mv);
@@ -1485,12 +1623,15 @@ static int
 mi_f64 (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
-    const float* coeff, const orc_hist* hp, int nthreads,
+    const float* coeff, const orc_hist* hp, int nthreads, const orc_rois* rois,
     const double* hist_in, int64_t nv_in,
     double* score, double* grad, int64_t* num_vox)
 {
     dgeo_t d;
     dvol_t fx, mv;
+    roi_t fr_, mr_;
+    const roi_t* froi = rois ? roi_init (&fr_, rois->fixed_hdr, rois->fixed) : 0;
+    const roi_t* mroi = rois ? roi_init (&mr_, rois->moving_hdr, rois->moving) : 0;
     if (dgeo_init (&d, g)) return -1;
     dvol_init (&fx, fh, f_img);
     dvol_init (&mv, mh, m_img);
@@ -1539,9 +1680,14 @@ mi_f64 (const orc_geom* g,
                 for (int a = 0; a < 3; a++)
                     fxyz[a] = d.origin[a] + fijk[0] * fx.step[3*a+0]
                         + fijk[1] * fx.step[3*a+1] + fijk[2] * fx.step[3*a+2];
+                if (froi && !inside_roi_d (fxyz, froi)) continue;
                 dinterp (dxyz, &d, p, q, coeff);
                 dsample_t s;
                 if (!dsample (&s, fxyz, dxyz, &mv)) continue;
+                if (mroi) {
+                    double mxyz[3] = {fxyz[0] + dxyz[0], fxyz[1] + dxyz[1], fxyz[2] + dxyz[2]};
+                    if (!inside_roi_d (mxyz, mroi)) continue;
+                }
                 int64_t fidx = fijk[0] + fx.dim[0] * (fijk[1] + fx.dim[1] * fijk[2]);
                 const double* l1 = s.l1;
                 const double* l2 = s.l2;
@@ -1602,10 +1748,10 @@ int
 orc_mi_score_f64 (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
-    const float* coeff, const orc_hist* hp, int nthreads,
+    const float* coeff, const orc_hist* hp, int nthreads, const orc_rois* rois,
     double* score, double* grad, int64_t* num_vox)
 {
-    return mi_f64 (g, fh, f_img, mh, m_img, coeff, hp, nthreads, 0, 0, score, grad, num_vox);
+    return mi_f64 (g, fh, f_img, mh, m_img, coeff, hp, nthreads, rois, 0, 0, score, grad, num_vox);
 }
 
 /* FP64 score and pass 2 on given histograms (the float flavor's: the PV
@@ -1616,11 +1762,11 @@ int
 orc_mi_score_f64_hist (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
-    const float* coeff, const orc_hist* hp, int nthreads,
+    const float* coeff, const orc_hist* hp, int nthreads, const orc_rois* rois,
     const double* hist_in, int64_t nv_in,
     double* score, double* grad, int64_t* num_vox)
 {
-    return mi_f64 (g, fh, f_img, mh, m_img, coeff, hp, nthreads, hist_in, nv_in, score, grad,
+    return mi_f64 (g, fh, f_img, mh, m_img, coeff, hp, nthreads, rois, hist_in, nv_in, score, grad,
         num_vox);
 }
 

@@ -29,6 +29,16 @@ typedef struct orc_vol_hdr {
     float dc[9];            /* direction cosines, row-major */
 } orc_vol_hdr;
 
+/* Optional regions of interest: uchar volumes with their own geometry
+   (Bspline_state::fixed_roi / moving_roi, src/register/bspline_state.h).
+   A NULL header or image means no ROI of that kind. */
+typedef struct orc_rois {
+    const orc_vol_hdr* fixed_hdr;
+    const uint8_t* fixed;
+    const orc_vol_hdr* moving_hdr;
+    const uint8_t* moving;
+} orc_rois;
+
 /* B-spline grid: the arguments of Bspline_header::set
    (src/base/bspline_header.cxx:28-62). */
 typedef struct orc_geom {
@@ -68,14 +78,14 @@ int64_t orc_pass1_position_mismatch (const orc_vol_hdr* fh, const orc_geom* g);
 int orc_mse_score (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img, const float* m_grad,
-    const float* coeff, int flavor, int nthreads,
+    const float* coeff, int flavor, int nthreads, const orc_rois* rois,
     double* raw_score, float* score, float* grad, int64_t* num_vox);
 
 /* grad_mode 0 = nearest-voxel m_grad (flavor g), 1 = analytic (flavor p). */
 int orc_mse_score_f64 (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img, const float* m_grad,
-    const float* coeff, int grad_mode, int nthreads,
+    const float* coeff, int grad_mode, int nthreads, const orc_rois* rois,
     double* score, double* grad, int64_t* num_vox);
 
 void orc_pv_quantize (const float w[8], int64_t q[8]);
@@ -106,13 +116,13 @@ int orc_reg_matrices (const orc_geom* g, const orc_reg_parms* rp, double* V);
 int orc_mse_f64_support (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
-    const float* coeff, int64_t ci, double delta,
+    const float* coeff, int64_t ci, double delta, const orc_rois* rois,
     double* raw, int64_t* n, int64_t* nclamp);
 
 int orc_mi_score (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
-    const float* coeff, const orc_hist* hp, int nthreads,
+    const float* coeff, const orc_hist* hp, int nthreads, const orc_rois* rois,
     float* score, float* grad, int64_t* num_vox,
     double* f_hist, double* m_hist, double* j_hist,
     int64_t* fx_f_hist, int64_t* fx_j_hist);
@@ -120,14 +130,14 @@ int orc_mi_score (const orc_geom* g,
 int orc_mi_score_f64 (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
-    const float* coeff, const orc_hist* hp, int nthreads,
+    const float* coeff, const orc_hist* hp, int nthreads, const orc_rois* rois,
     double* score, double* grad, int64_t* num_vox);
 /* The same on given histograms (f, m, joint, concatenated; count nv_in):
    FP64 score and pass 2 only. */
 int orc_mi_score_f64_hist (const orc_geom* g,
     const orc_vol_hdr* fh, const float* f_img,
     const orc_vol_hdr* mh, const float* m_img,
-    const float* coeff, const orc_hist* hp, int nthreads,
+    const float* coeff, const orc_hist* hp, int nthreads, const orc_rois* rois,
     const double* hist_in, int64_t nv_in,
     double* score, double* grad, int64_t* num_vox);
 

@@ -85,9 +85,23 @@ def _dweights(l1, l2):
                      for n in range(8)])
 
 
-def mse(fixed, moving, spacing, vpr, coeff, grad_mode, m_grad=None):
+def _roi_at(mask, pos):
+    """inside_roi of a mask on an index grid: the floor of each coordinate,
+    below 0 taken as 0 and at or past the edge as the last voxel."""
+    nz, ny, nx = mask.shape
+    p = []
+    for a, n in enumerate((nx, ny, nz)):
+        v = pos[a]
+        p.append(0 if v < 0 else (n - 1 if v >= n else int(math.floor(v))))
+    return bool(mask[p[2], p[1], p[0]])
+
+
+def mse(fixed, moving, spacing, vpr, coeff, grad_mode, m_grad=None, fmask=None, mmask=None):
     """FP64 MSE score, gradient (2/N normalized) and N; identity direction
-    cosines, zero origins.  grad_mode 0: diff * m_grad[round], 1: analytic."""
+    cosines, zero origins.  grad_mode 0: diff * m_grad[round], 1: analytic.
+    fmask / mmask: optional regions of interest on the fixed and moving
+    grids (a voxel counts only where the fixed mask holds at its position
+    and the moving mask at its displaced position)."""
     nz, ny, nx = fixed.shape
     dim = (nx, ny, nz)
     _, cd = grid(dim, vpr)
@@ -97,11 +111,15 @@ def mse(fixed, moving, spacing, vpr, coeff, grad_mode, m_grad=None):
     for k in range(nz):
         for j in range(ny):
             for i in range(nx):
+                if fmask is not None and not fmask[k, j, i]:
+                    continue
                 ws = _voxel_weights((i, j, k), vpr, cd)
                 d = sum(w * c[kn] for kn, w in ws)
                 mijk = [(v * spacing[a] + d[a]) / spacing[a] for a, v in enumerate((i, j, k))]
                 smp = _sample(moving, mijk)
                 if smp is None:
+                    continue
+                if mmask is not None and not _roi_at(mmask, mijk):
                     continue
                 f, r, l2 = smp
                 a8 = _corners(moving, f)
@@ -120,9 +138,11 @@ def mse(fixed, moving, spacing, vpr, coeff, grad_mode, m_grad=None):
     return raw / n, (2.0 * grad / n).reshape(-1), n
 
 
-def mi(fixed, moving, spacing, vpr, coeff, bins, f_off, f_delta, m_off, m_delta):
+def mi(fixed, moving, spacing, vpr, coeff, bins, f_off, f_delta, m_off, m_delta, fmask=None,
+       mmask=None):
     """FP64 PV-8 Mattes MI: score -MI, gradient (thresholds 0.001/B^2 for the
-    score, 1e-4 for the gradient) and N; identity direction cosines."""
+    score, 1e-4 for the gradient) and N; identity direction cosines.
+    fmask / mmask as for mse."""
     nz, ny, nx = fixed.shape
     dim = (nx, ny, nz)
     _, cd = grid(dim, vpr)
@@ -134,11 +154,15 @@ def mi(fixed, moving, spacing, vpr, coeff, bins, f_off, f_delta, m_off, m_delta)
     for k in range(nz):
         for j in range(ny):
             for i in range(nx):
+                if fmask is not None and not fmask[k, j, i]:
+                    continue
                 ws = _voxel_weights((i, j, k), vpr, cd)
                 d = sum(w * c[kn] for kn, w in ws)
                 mijk = [(v * spacing[a] + d[a]) / spacing[a] for a, v in enumerate((i, j, k))]
                 smp = _sample(moving, mijk)
                 if smp is None:
+                    continue
+                if mmask is not None and not _roi_at(mmask, mijk):
                     continue
                 f, r, l2 = smp
                 a8 = _corners(moving, f)

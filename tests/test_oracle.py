@@ -549,3 +549,71 @@ def test_reg_assembly_matches_dense_forms():
         G[kn, 2] += 2 * (sym + D[2]) @ pz + Exz @ px + Eyz @ py
     assert abs(S - S_ref) < 1e-10 * abs(S_ref)
     assert np.abs(grad - G.ravel()).max() < 1e-10 * np.abs(G).max()
+
+
+# ---------------------------------------------------------------- regions of interest
+
+def _roi_case(seed=0, dim=(14, 12, 10)):
+    """Tiny dyadic case with random fixed and moving masks on the images'
+    own grids (zero origin, identity direction cosines)."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import bruteforce
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dim
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    fixed = (100 * np.exp(-((x - 6.3) ** 2 + (y - 5.1) ** 2 + (z - 4.2) ** 2) / 18.0)).astype(np.float32)
+    moving = (100 * np.exp(-((x - 6.9) ** 2 + (y - 4.6) ** 2 + (z - 4.9) ** 2) / 15.0)
+              + rng.normal(0, 1.0, fixed.shape)).astype(np.float32)
+    sp, vpr = (1.0, 1.0, 2.0), (4, 4, 3)
+    fmask = (rng.uniform(size=fixed.shape) < 0.7).astype(np.uint8)
+    mmask = (rng.uniform(size=fixed.shape) < 0.8).astype(np.uint8)
+    coeff = (np.round(rng.normal(0, 0.8, size=synth.num_coeff(dim, vpr)) * 64) / 64).astype(np.float32)
+    return bruteforce, fixed, moving, sp, vpr, fmask, mmask, coeff
+
+
+def test_roi_oracle_matches_bruteforce():
+    """MSE flavors g/p with ROIs (the reference's flavors i and r) and MI with
+    ROIs (flavor h) against the independent FP64 brute force, each mask
+    alone and both together: the fixed mask excludes fixed voxels, the
+    moving mask excludes voxels whose displaced position lands on a zero
+    moving-mask voxel (inside_roi, src/util/bspline_correspond.cxx:52-74)."""
+    bf, fixed, moving, sp, vpr, fmask, mmask, coeff = _roi_case()
+    m_grad = oracle.volume_gradient(moving, sp)
+    for fm, mm in ((fmask, None), (None, mmask), (fmask, mmask)):
+        oc = oracle.Case(fixed, moving, sp, vpr, fixed_roi=(fm, sp) if fm is not None else None,
+                         moving_roi=(mm, sp) if mm is not None else None)
+        for mode, flavor in ((0, "g"), (1, "p")):
+            s_b, g_b, n_b = bf.mse(fixed, moving, sp, vpr, coeff, mode, m_grad=m_grad, fmask=fm, mmask=mm)
+            s64, g64, n64 = oc.mse_f64(coeff, mode)
+            assert n64 == n_b and rel_scalar(s64, s_b) < 1e-12 and rel(g64, g_b) < 1e-10
+            s, g, n, _ = oc.mse(coeff, flavor)
+            assert n == n_b and rel_scalar(s, s_b) < 1e-5 and rel(g, g_b) < 1e-4
+        hp = oc.hist(12)
+        s_b, g_b, n_b = bf.mi(fixed, moving, sp, vpr, coeff, 12, hp.f_offset, hp.f_delta, hp.m_offset,
+                              hp.m_delta, fmask=fm, mmask=mm)
+        s64, g64, n64 = oc.mi_f64(coeff, hp)
+        assert n64 == n_b and rel_scalar(s64, s_b) < 1e-10 and rel(g64, g_b) < 1e-8
+        r = oc.mi(coeff, hp)
+        assert r["num_vox"] == n_b and r["fx_j_hist"].sum() == n_b << 32
+        assert rel_scalar(r["score"], s_b) < 1e-5 and rel(r["grad"], g_b) < 1e-4
+
+
+def test_roi_all_ones_and_all_zeros():
+    """An all-ones ROI is no ROI, bit for bit; an empty fixed ROI leaves no
+    voxel (score FLT_MAX, zero gradient)."""
+    _, fixed, moving, sp, vpr, fmask, mmask, coeff = _roi_case(1)
+    ones = np.ones(fixed.shape, np.uint8)
+    plain = oracle.Case(fixed, moving, sp, vpr)
+    # ROIs on a coarser grid of their own
+    both = oracle.Case(fixed, moving, sp, vpr, fixed_roi=(ones[::2, ::2, ::2], (2.0, 2.0, 4.0), (-0.5, -0.5, -1.0)),
+                       moving_roi=(ones, sp))
+    for flavor in ("g", "p"):
+        a, b = plain.mse(coeff, flavor), both.mse(coeff, flavor)
+        assert a[0] == b[0] and np.array_equal(a[1], b[1]) and a[2] == b[2]
+    hp = plain.hist(12)
+    a, b = plain.mi(coeff, hp), both.mi(coeff, hp)
+    assert a["score"] == b["score"] and np.array_equal(a["grad"], b["grad"])
+    empty = oracle.Case(fixed, moving, sp, vpr, fixed_roi=(np.zeros_like(ones), sp))
+    s, g, n, _ = empty.mse(coeff, "g")
+    assert n == 0 and s == np.finfo(np.float32).max and not np.any(g)
