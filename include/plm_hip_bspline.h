@@ -64,6 +64,7 @@ extern "C" {
 /* how the library keeps an image in HBM (arithmetic is always FP32) */
 #define PLM_STORE_F32  0
 #define PLM_STORE_BF16 1
+#define PLM_STORE_U8   2   /* region-of-interest volumes: one byte a voxel, non-zero inside */
 
 /* B-spline grid: the arguments of Bspline_header::set
  * (src/base/bspline_header.cxx:28-62). rdims/cdims are derived. */
@@ -205,6 +206,25 @@ int plm_hip_bspline_score_noalloc (
 
 /* Replaces CUDA_bspline_mse_cleanup_j / CUDA_bspline_mi_cleanup_a and
  * bspline_cuda_state_destroy (src/register/bspline_state.cxx:239-271). */
+/* Regions of interest of a context: the fixed and moving ROI volumes of
+ * Bspline_state (fixed_roi / moving_roi, src/register/bspline_state.h:44-45,
+ * set by Bspline_stage, src/register/bspline_stage.cxx:129-205), uchar
+ * images (storage PLM_STORE_U8) with geometries of their own, on the host
+ * or the device; either may be NULL, and NULL for both clears them.  With a
+ * ROI set, MSE follows flavor i (src/register/bspline_mse.cxx:82-290) for
+ * PLM_GRAD_REF_NN and PLM_GRAD_NONE and flavor r (:1628-1862) for
+ * PLM_GRAD_EXACT: a fixed voxel counts only where the fixed ROI holds at
+ * its position and the moving ROI at its moving position (inside_roi,
+ * src/util/bspline_correspond.cxx:52-74, 113-145).  MI follows flavor h
+ * (src/register/bspline_mi.cxx:1232-1496): pass 1 tests both ROIs at its
+ * walk positions, pass 2 at the tiles' positions.  The histogram ranges
+ * stay what the context was created with (Joint_histogram::initialize
+ * spans the whole images whatever the ROIs).  No evaluation of the context
+ * may be in flight; a batch takes its contexts' ROIs when it is created.
+ * plm_hip_bspline_score_vf_device refuses a context with a ROI. */
+int plm_hip_bspline_set_roi (plm_hip_ctx* ctx, const plm_img_desc* fixed_roi,
+    const plm_img_desc* moving_roi);
+
 int plm_hip_bspline_destroy (plm_hip_ctx* ctx);
 
 /* Moving-image gradient for PLM_GRAD_REF_NN, computed on the GPU:

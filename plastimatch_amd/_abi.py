@@ -29,6 +29,7 @@ PLM_MEM_HOST = 0
 PLM_MEM_DEVICE = 1
 PLM_STORE_F32 = 0
 PLM_STORE_BF16 = 1
+PLM_STORE_U8 = 2
 PLM_RESAMPLE = {"voxel_rate": 0, "voxel_legacy": 1, "mm": 2, "pct": 3}
 
 c_float3 = ctypes.c_float * 3
@@ -123,6 +124,7 @@ SIGNATURES = {
         ctypes.POINTER(PlmBspGeom), ctypes.POINTER(PlmImgDesc), ctypes.POINTER(PlmImgDesc),
         ctypes.POINTER(PlmImgDesc), ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P]),
     "plm_hip_bspline_destroy": (ctypes.c_int, [_P]),
+    "plm_hip_bspline_set_roi": (ctypes.c_int, [_P, ctypes.POINTER(PlmImgDesc), ctypes.POINTER(PlmImgDesc)]),
     "plm_hip_volume_gradient": (ctypes.c_int, [ctypes.POINTER(PlmImgDesc), _P, ctypes.c_int,
                                                ctypes.c_int]),
     "plm_hip_bspline_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),

@@ -9,6 +9,8 @@ Mirrors the reference objects the score path reads and writes:
 * ``Joint_histogram`` -- src/register/joint_histogram.h:14-46 (EQSP bins);
 * ``Bspline_score``   -- src/register/bspline_score.h:33-95 (curr_smetric,
                          curr_smetric_grad, curr_num_vox);
+* ``update_roi`` / ``stage_rois`` -- the stage's region-of-interest
+                         preparation (src/register/bspline_stage.cxx:71-205);
 * ``BsplineHip``      -- the device state the reference keeps behind
                          ``Bspline_state::dev_ptrs`` (bspline_state.cxx:165-271),
                          created once per stage, scored once per evaluation.
@@ -27,7 +29,7 @@ IDENTITY = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
 
 METRICS = {"mse": _abi.PLM_METRIC_MSE, "mi": _abi.PLM_METRIC_MI}
 GRAD_MODES = {"ref_nn": _abi.PLM_GRAD_REF_NN, "exact": _abi.PLM_GRAD_EXACT, "none": _abi.PLM_GRAD_NONE}
-STORAGE = {"f32": _abi.PLM_STORE_F32, "bf16": _abi.PLM_STORE_BF16}
+STORAGE = {"f32": _abi.PLM_STORE_F32, "bf16": _abi.PLM_STORE_BF16, "u8": _abi.PLM_STORE_U8}
 
 
 def _is_torch(x):
@@ -35,10 +37,16 @@ def _is_torch(x):
 
 
 class Volume:
-    """A float image (numpy array or CUDA torch tensor), shape (nz, ny, nx[, 3])."""
+    """A float image (numpy array or CUDA torch tensor), shape (nz, ny, nx[, 3]);
+    a uint8 array is a region of interest (PT_UCHAR)."""
 
     def __init__(self, img, spacing, origin=(0.0, 0.0, 0.0), direction_cosines=IDENTITY):
-        self.img = img if _is_torch(img) else np.ascontiguousarray(img, np.float32)
+        if _is_torch(img):
+            self.img = img
+        elif np.asarray(img).dtype == np.uint8:
+            self.img = np.ascontiguousarray(img)
+        else:
+            self.img = np.ascontiguousarray(img, np.float32)
         nz, ny, nx = img.shape[:3]
         self.dim = (int(nx), int(ny), int(nz))
         self.spacing = tuple(float(s) for s in spacing)
@@ -49,7 +57,13 @@ class Volume:
     def npix(self):
         return self.dim[0] * self.dim[1] * self.dim[2]
 
+    @property
+    def is_uchar(self):
+        return str(self.img.dtype).endswith("uint8")
+
     def desc(self, storage="f32"):
+        if self.is_uchar:
+            storage = "u8"
         if _is_torch(self.img):
             if not self.img.is_cuda or not self.img.is_contiguous():
                 raise ValueError("device images must be contiguous CUDA tensors")
@@ -361,6 +375,20 @@ class BsplineHip:
             hist.ctypes.data_as(ctypes.c_void_p) if hist is not None else None))
         return Bspline_score(sc.value, grad, nv.value, hist, rc)
 
+    def set_roi(self, fixed_roi=None, moving_roi=None):
+        """Set (or, with neither, clear) the fixed and moving regions of
+        interest, uint8 Volumes with geometries of their own
+        (Bspline_state::fixed_roi / moving_roi): MSE then follows flavor i
+        (r for grad_mode "exact"), MI flavor h."""
+        for r in (fixed_roi, moving_roi):
+            if r is not None and not r.is_uchar:
+                raise ValueError("a region of interest is a uint8 Volume")
+        fd = fixed_roi.desc() if fixed_roi is not None else None
+        md = moving_roi.desc() if moving_roi is not None else None
+        _abi.check(self.lib.plm_hip_bspline_set_roi(self.ctx, ctypes.byref(fd) if fd is not None else None,
+                                                   ctypes.byref(md) if md is not None else None))
+        self.fixed_roi, self.moving_roi = fixed_roi, moving_roi
+
     def score_device(self, d_coeff, d_score, d_grad, d_num_vox, metric="mse", grad_mode="ref_nn",
                      d_hist=None, stream=None):
         """Enqueue one evaluation with device tensors; returns immediately."""
@@ -501,6 +529,39 @@ class BsplineHip:
             self.close()
         except Exception:
             pass
+
+
+def update_roi(roi, image, min_val, max_val):
+    """update_roi (src/register/bspline_stage.cxx:71-93): with no ROI, a new
+    one on the image's grid that holds 1 where min_val <= value <= max_val;
+    otherwise the ROI with its voxels outside that range set to 0 (the ROI
+    shares the image's grid, voxel for voxel)."""
+    img = np.asarray(image.img)
+    in_range = (img >= min_val) & (img <= max_val)
+    if roi is None:
+        return Volume(in_range.astype(np.uint8), image.spacing, image.origin, image.direction_cosines)
+    r = np.asarray(roi.img)
+    if r.shape != img.shape:
+        raise ValueError("update_roi walks the ROI and the image voxel for voxel: dims must match")
+    out = np.where(in_range, r, np.uint8(0)).astype(np.uint8)
+    return Volume(out, roi.spacing, roi.origin, roi.direction_cosines)
+
+
+def stage_rois(fixed, moving, fixed_roi=None, moving_roi=None, mi_fixed_image_minVal=0.0,
+               mi_fixed_image_maxVal=0.0, mi_moving_image_minVal=0.0, mi_moving_image_maxVal=0.0):
+    """The ROIs a stage hands the score path (src/register/bspline_stage.cxx:
+    129-192): each image's user intensity range, when set (min or max
+    non-zero, min < max), narrows that image's ROI, or makes one; returns
+    (fixed_roi, moving_roi), either possibly None.  (The stage then
+    subsamples them by the image's rate, nearest voxel: stage_resample with
+    nearest=True.)"""
+    def ranged(lo, hi):
+        return (lo != 0 or hi != 0) and lo < hi
+    if ranged(mi_moving_image_minVal, mi_moving_image_maxVal):
+        moving_roi = update_roi(moving_roi, moving, mi_moving_image_minVal, mi_moving_image_maxVal)
+    if ranged(mi_fixed_image_minVal, mi_fixed_image_maxVal):
+        fixed_roi = update_roi(fixed_roi, fixed, mi_fixed_image_minVal, mi_fixed_image_maxVal)
+    return fixed_roi, moving_roi
 
 
 def stage_for_pair(pair, gpuid=0, storage="f32", with_grad=True, bins=None, moving_grad=None):

@@ -200,6 +200,13 @@ struct plm_hip_ctx {
     uint8_t* d_mbin = nullptr;
     uint32_t* d_mquad = nullptr;    // moving bins of each voxel's (x, y) quad
     float* d_ptab = nullptr;        // MI pass-1 walk positions (only where they differ from direct)
+    // regions of interest (plm_hip_bspline_set_roi)
+    float m_dc[9] = {};             // the moving image's direction cosines and spacing
+    float m_sp[3] = {};
+    uint8_t* d_fmask = nullptr;     // the fixed ROI at every fixed voxel's position
+    uint8_t* d_fmask1 = nullptr;    // the same at MI pass 1's walk positions (with d_ptab)
+    uint8_t* d_mroi = nullptr;      // the moving ROI volume
+    plmhip::RoiVol mroi {};
     // staging for the host-pointer entry point
     float* d_coeff = nullptr;
     double* d_grad = nullptr;
@@ -239,10 +246,28 @@ free_ctx (plm_hip_ctx* c)
         c->d_tile_score, c->d_tile_count, c->d_pair, c->d_hist,
         c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_mquad, c->d_ptab, c->d_coeff, c->d_grad,
         c->d_score, c->d_nvox,
-        c->d_hist_out, c->d_reg_part, c->d_reg_cond};
+        c->d_hist_out, c->d_reg_part, c->d_reg_cond, c->d_fmask, c->d_fmask1, c->d_mroi};
     for (void* b : bufs) if (b) (void) hipFree (b);
     if (c->stream) (void) hipStreamDestroy (c->stream);
     delete c;
+}
+
+// The context as the tile kernels see it; the evaluation's buffers come
+// with each call, or from a batch's table.
+plmhip::PairDev
+pair_of (const plm_hip_ctx* c)
+{
+    plmhip::PairDev pd {};
+    pd.g = c->g;
+    pd.tbase = c->d_tbase;
+    pd.fimg = c->d_fixed;
+    pd.mimg = c->d_moving;
+    pd.mgrad = c->d_mgrad;
+    pd.mpair = c->d_mpair;
+    pd.cond = c->d_cond;
+    pd.fmask = c->d_fmask;
+    pd.mroi = c->mroi;
+    return pd;
 }
 
 template <typename T>
@@ -513,6 +538,8 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     c->mi_ok = mi_ok;
     c->have_mgrad = moving_grad != nullptr;
     c->storage = fixed->storage;
+    for (int i = 0; i < 9; i++) c->m_dc[i] = moving->direction_cosines[i];
+    for (int a = 0; a < 3; a++) c->m_sp[a] = moving->spacing[a];
     c->num_knots = knots;
     c->num_tiles = tiles;
     for (int a = 0; a < 3; a++)
@@ -618,14 +645,7 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     {
         // the context as a table of one pair; evaluation buffers come with
         // every call
-        plmhip::PairDev pd {};
-        pd.g = g;
-        pd.tbase = c->d_tbase;
-        pd.fimg = c->d_fixed;
-        pd.mimg = c->d_moving;
-        pd.mgrad = c->d_mgrad;
-        pd.mpair = c->d_mpair;
-        pd.cond = c->d_cond;
+        const plmhip::PairDev pd = pair_of (c);
         CTX_TRY (alloc (&c->d_pair, 1, "hipMalloc(pair)"));
         e = hipMemcpy (c->d_pair, &pd, sizeof (pd), hipMemcpyHostToDevice);
         if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "hipMemcpy(pair)"); }
@@ -683,7 +703,7 @@ plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
         const plmhip::EvalOut out {d_grad, d_score, d_num_vox};
         e = plmhip::launch_mi (c->plan, c->tune, c->h, c->hist_blocks, c->d_fbin, c->d_mquad,
             d_coeff, c->d_hist, c->d_hist + (size_t) c->h.fbins * c->h.mbins, c->d_table,
-            c->d_nvox_f, out, d_hist, c->d_ptab, st, ev);
+            c->d_nvox_f, out, d_hist, c->d_ptab, c->d_ptab ? c->d_fmask1 : c->d_fmask, c->mroi, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MI launch");
     }
     if (ev) c->timer.last_stages = metric == PLM_METRIC_MSE ? plmhip::MSE_STAGES : plmhip::MI_STAGES;
@@ -810,6 +830,148 @@ plm_hip_bspline_autotune (plm_hip_ctx* c, int metric, int grad_mode, const float
     if (chosen) *chosen = best;
     if (eval_ms) *eval_ms = best_ms;
     return PLM_HIP_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+// A ROI descriptor: one byte a voxel, dims >= 1, non-zero spacing, fewer
+// than 2^31 voxels.
+int
+check_roi (const plm_img_desc* d, const char* name)
+{
+    if (!d->data) return fail (PLM_HIP_ERR_ARG, std::string (name) + " data is NULL");
+    if (d->location != PLM_MEM_HOST && d->location != PLM_MEM_DEVICE)
+        return fail (PLM_HIP_ERR_ARG, std::string (name) + ": bad location");
+    if (d->storage != PLM_STORE_U8)
+        return fail (PLM_HIP_ERR_ARG, std::string (name) + " must be PLM_STORE_U8");
+    int64_t n = 1;
+    for (int a = 0; a < 3; a++) {
+        if (d->dim[a] < 1) return fail (PLM_HIP_ERR_GEOMETRY, std::string (name) + ": dim < 1");
+        if (!(d->spacing[a] != 0.f)) return fail (PLM_HIP_ERR_GEOMETRY, std::string (name) + ": zero spacing");
+        n *= d->dim[a];
+    }
+    if (n >= ((int64_t) 1 << 31))
+        return fail (PLM_HIP_ERR_GEOMETRY, std::string (name) + ": too many voxels for 32-bit indexing");
+    return PLM_HIP_OK;
+}
+
+// The ROI's bytes in a new device buffer.
+int
+upload_roi (const plm_img_desc* d, uint8_t** out)
+{
+    const size_t n = (size_t) (d->dim[0] * d->dim[1] * d->dim[2]);
+    int rc = alloc (out, n, "hipMalloc(roi)");
+    if (rc) return rc;
+    hipError_t e = hipMemcpy (*out, d->data, n, d->location == PLM_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                                              : hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void) hipFree (*out);
+        *out = nullptr;
+        return hip_fail (e, "hipMemcpy(roi)");
+    }
+    return PLM_HIP_OK;
+}
+
+// The kernels' view of ROI d (its bytes at dimg): the float proj the
+// reference forms for a volume (inv(dc) / spacing, src/base/
+// direction_matrices.cxx:10-20), and for a moving ROI the map from the
+// moving image's index coordinates to the ROI's, R = P_roi S_m and
+// r0 = P_roi (moving origin - ROI origin), S_m = dc * spacing of the moving
+// image, in double.
+int
+roi_vol (const plm_img_desc* d, const uint8_t* dimg, const plm_hip_ctx* c, bool moving, plmhip::RoiVol& r)
+{
+    r = plmhip::RoiVol {};
+    r.img = dimg;
+    float inv[9];
+    double pd[9];
+    if (!mat3_inverse (inv, d->direction_cosines) || !mat3_inverse_d (pd, d->direction_cosines))
+        return fail (PLM_HIP_ERR_GEOMETRY, "singular ROI direction cosines");
+    for (int a = 0; a < 3; a++) {
+        r.dim[a] = (int) d->dim[a];
+        r.origin[a] = d->origin[a];
+        for (int b = 0; b < 3; b++) {
+            r.proj[3*a+b] = inv[3*a+b] / d->spacing[a];
+            pd[3*a+b] /= (double) d->spacing[a];
+        }
+    }
+    if (!moving) return PLM_HIP_OK;
+    for (int a = 0; a < 3; a++) {
+        double o = 0.0;
+        for (int k = 0; k < 3; k++) o += pd[3*a+k] * ((double) c->g.morigin[k] - (double) d->origin[k]);
+        r.r0[a] = o;
+        for (int b = 0; b < 3; b++) {
+            double v = 0.0;
+            for (int k = 0; k < 3; k++) v += pd[3*a+k] * (double) c->m_dc[3*k+b] * (double) c->m_sp[b];
+            r.Rd[3*a+b] = v;
+            r.R[3*a+b] = (float) v;
+        }
+    }
+    return PLM_HIP_OK;
+}
+
+void
+clear_roi (plm_hip_ctx* c)
+{
+    for (uint8_t** p : {&c->d_fmask, &c->d_fmask1, &c->d_mroi}) {
+        if (*p) (void) hipFree (*p);
+        *p = nullptr;
+    }
+    c->mroi = plmhip::RoiVol {};
+}
+
+int
+build_roi (plm_hip_ctx* c, const plm_img_desc* fixed_roi, const plm_img_desc* moving_roi)
+{
+    if (fixed_roi) {
+        uint8_t* tmp = nullptr;
+        int rc = upload_roi (fixed_roi, &tmp);
+        if (rc) return rc;
+        plmhip::RoiVol fr;
+        rc = roi_vol (fixed_roi, tmp, c, false, fr);
+        const size_t nf = (size_t) c->g.fdim[0] * c->g.fdim[1] * c->g.fdim[2];
+        if (!rc) rc = alloc (&c->d_fmask, nf, "hipMalloc(fixed ROI mask)");
+        if (!rc && c->d_ptab) rc = alloc (&c->d_fmask1, nf, "hipMalloc(fixed ROI mask)");
+        hipError_t e = hipSuccess;
+        if (!rc) e = plmhip::launch_roi_fixed (c->g, fr, nullptr, c->d_fmask, c->stream);
+        if (!rc && e == hipSuccess && c->d_ptab)
+            e = plmhip::launch_roi_fixed (c->g, fr, c->d_ptab, c->d_fmask1, c->stream);
+        if (!rc && e == hipSuccess) e = hipStreamSynchronize (c->stream);
+        (void) hipFree (tmp);
+        if (rc) return rc;
+        if (e != hipSuccess) return hip_fail (e, "fixed ROI mask");
+    }
+    if (moving_roi) {
+        int rc = upload_roi (moving_roi, &c->d_mroi);
+        if (!rc) rc = roi_vol (moving_roi, c->d_mroi, c, true, c->mroi);
+        if (rc) return rc;
+    }
+    return PLM_HIP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int
+plm_hip_bspline_set_roi (plm_hip_ctx* c, const plm_img_desc* fixed_roi, const plm_img_desc* moving_roi)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    int rc = fixed_roi ? check_roi (fixed_roi, "fixed_roi") : PLM_HIP_OK;
+    if (!rc && moving_roi) rc = check_roi (moving_roi, "moving_roi");
+    if (rc) return rc;
+    HIP_TRY (hipSetDevice (c->device));
+    HIP_TRY (hipStreamSynchronize (c->stream));
+    clear_roi (c);
+    rc = build_roi (c, fixed_roi, moving_roi);
+    if (rc) clear_roi (c);
+    // the one-pair table the single-context launches read
+    const plmhip::PairDev pd = pair_of (c);
+    hipError_t e = hipMemcpy (c->d_pair, &pd, sizeof (pd), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail (e, "hipMemcpy(pair)");
+    return rc;
 }
 
 int
@@ -1000,6 +1162,8 @@ plm_hip_bspline_score_vf_device (plm_hip_ctx* c, const float* d_vf, double* d_sc
 {
     if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
     if (!d_vf || !d_score || !d_grad || !d_num_vox) return fail (PLM_HIP_ERR_ARG, "NULL argument");
+    if (c->d_fmask || c->d_mroi)
+        return fail (PLM_HIP_ERR_UNSUPPORTED, "the field-driven MSE takes no region of interest");
     HIP_TRY (hipSetDevice (c->device));
     hipStream_t st = stream ? (hipStream_t) stream : c->stream;
     hipEvent_t* ev = nullptr;
@@ -1136,15 +1300,8 @@ plm_hip_batch_create (plm_hip_ctx* const* ctxs, int n, plm_hip_batch** out)
     for (int i = 0; i < n; i++) {
         plm_hip_ctx* c = ctxs[i];
         plmhip::PairDev& pd = b->h_pairs[i];
-        pd = plmhip::PairDev {};
-        pd.g = c->g;
-        pd.tbase = c->d_tbase;
-        pd.fimg = c->d_fixed;
-        pd.mimg = c->d_moving;
-        pd.mgrad = c->d_mgrad;
-        pd.mpair = c->d_mpair;
+        pd = pair_of (c);
         pd.coeff = c->d_coeff;
-        pd.cond = c->d_cond;
         pd.grad = c->d_grad;
         pd.score = c->d_score;
         pd.nvox = c->d_nvox;

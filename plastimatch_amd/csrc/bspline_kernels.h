@@ -148,11 +148,28 @@ tile_base_of (const KGeom& g, const double moff[3], const double mstep[9], int t
     return tb;
 }
 
+// A region-of-interest volume (one byte a voxel, a geometry of its own) as
+// inside_roi tests it (src/util/bspline_correspond.cxx:52-74): the byte at
+// the floor of proj (xyz - origin), each index clamped into the volume.
+struct RoiVol {
+    const uint8_t* img;      // null: no ROI
+    int dim[3];
+    float origin[3];
+    float proj[9];           // inv(dc) / spacing, formed as for the images
+    // The tile kernels' map from the moving image's index coordinates to
+    // the ROI's: roi = R mijk + r0 (a moving ROI only; double on the host).
+    float R[9];
+    double Rd[9];
+    double r0[3];
+};
+
 // One fixed/moving pair as the tile and reduce kernels see it (a device
 // table of these drives a launch; a single context is a table of one).
 struct PairDev {
     KGeom g;
     const TileBase* tbase;   // per tile of the pair
+    const uint8_t* fmask;    // the fixed ROI's byte at every fixed voxel (null: none)
+    RoiVol mroi;             // the moving ROI (img null: none)
     const void* fimg;        // fixed, moving, moving gradient (the batch's storage type)
     const void* mimg;
     const void* mgrad;
@@ -254,10 +271,18 @@ hipError_t launch_regularize (const KGeom& g, const RegOp& op, const float* coef
     double* cond, double* score, double* grad, int accumulate, hipStream_t st);
 // Mattes MI of a single-pair plan.  ptab: the fixed positions of pass 1's
 // walk (launch_pass1_positions), or null where they equal the direct ones.
+// fmask1: the fixed ROI at pass 1's positions (null: none); mroi: the
+// moving ROI, tested in pass 1 at the reference's float moving position.
 hipError_t launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     const uint8_t* fbin, const uint32_t* mquad, const float* coeff, unsigned long long* jhist,
     unsigned long long* fhist, float* table, float* nvox_f, const EvalOut& out0,
-    int64_t* hist_out, const float* ptab, hipStream_t st, hipEvent_t* ev);
+    int64_t* hist_out, const float* ptab, const uint8_t* fmask1, const RoiVol& mroi,
+    hipStream_t st, hipEvent_t* ev);
+// The fixed ROI r at every voxel of g's fixed image: out[v] = 1 where
+// inside_roi holds at the voxel's position (ptab's where given, else
+// origin + step ijk), else 0.
+hipError_t launch_roi_fixed (const KGeom& g, const RoiVol& r, const float* ptab, uint8_t* out,
+    hipStream_t st);
 // The reference's pass-1 walk of the fixed image (LOOP_Z/Y/X): adds to
 // *mismatch the voxels whose walk position differs from origin + step ijk,
 // and with out (float3 per fixed voxel) not null writes the walk positions.

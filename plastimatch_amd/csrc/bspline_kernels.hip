@@ -216,6 +216,23 @@ position (float fxyz[3], const KGeom& g, int fi, int fj, int fk)
     }
 }
 
+// inside_roi (src/util/bspline_correspond.cxx:52-74): the byte offset in r
+// of the voxel holding world position xyz, tmp = xyz - origin projected in
+// the reference's order, each index the floor, below 0 taken as 0 and at or
+// past dim as dim - 1 (float_to_plm_long_clamp, :16-27).
+__device__ __forceinline__ int
+roi_offset (const RoiVol& r, const float xyz[3])
+{
+    const float t0 = xyz[0] - r.origin[0], t1 = xyz[1] - r.origin[1], t2 = xyz[2] - r.origin[2];
+    int i[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float pf = t0 * r.proj[3*a] + t1 * r.proj[3*a+1] + t2 * r.proj[3*a+2];
+        i[a] = pf < 0.f ? 0 : (pf >= (float) r.dim[a] ? r.dim[a] - 1 : (int) floorf (pf));
+    }
+    return i[0] + r.dim[0] * (i[1] + r.dim[1] * i[2]);
+}
+
 __device__ __forceinline__ int
 hist_bin (float v, float offset, float delta, int bins)
 {
@@ -979,8 +996,9 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         }
         // MSE and MI tiles whose every column stays in the moving image's
         // interior take the voxel pass without clamps or inside tests (the
-        // same results bit for bit): one flag per wave, ANDed after the
-        // barrier (red is free here).
+        // same floors, fractions and decisions; the trilinear lerps take
+        // their shared-difference form): one flag per wave, ANDed after
+        // the barrier (red is free here).
         constexpr bool FASTABLE = MODE == PROJECTED && !VF_OUT;
         const bool col_ok = tile_columns<V, MODE> (L, S, g, tb, tr);
         store_rows (tr);
@@ -1010,6 +1028,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         const int ez = min (vz, g.roi_end[2] - P.bz0);
         const bool full = ex == vx && ey == vy && ez == vz;
         fast = fast && full;
+        constexpr bool ROIABLE = KIND <= 3;     // the score and gradient kinds
         const int fd0 = g.fdim[0], fd01 = g.fdim[0] * g.fdim[1];
         const int md0 = g.mdim[0], md01 = g.mdim[0] * g.mdim[1];
         const int fbase = P.bx0 + fd0 * P.by0 + fd01 * P.bz0;
@@ -1041,6 +1060,36 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
             chi[k] = uniform (tb.cl_hi[k]);
         }
         const int boff = __builtin_amdgcn_readfirstlane (tb.boff);
+        // Regions of interest (MSE flavor i, src/register/bspline_mse.cxx:
+        // 172-204; MI flavor h pass 2, src/register/bspline_mi.cxx:
+        // 1416-1430): the fixed ROI's byte at the voxel, and the moving ROI's
+        // at the moving position, whose ROI index coordinates are
+        // R (m + Bi) + r0 = R m + rt (rt formed per tile in double).  Tiles
+        // of a pair with a ROI take the full voxel pass.
+        const uint8_t* fmask = ROIABLE ? pd.fmask : nullptr;
+        const uint8_t* mroi = ROIABLE ? pd.mroi.img : nullptr;
+        const bool use_roi = ROIABLE && (fmask != nullptr || mroi != nullptr);
+        fast = fast && !use_roi;
+        float rR[9], rt[3], rtop[3];
+        int rd0 = 0, rd01 = 0;
+        const Rsrc fmr = make_rsrc (fmask, ROIABLE ? nfix : 0u);
+        const Rsrc rmr = make_rsrc (mroi, ROIABLE && mroi
+            ? (uint32_t) (pd.mroi.dim[0] * pd.mroi.dim[1] * pd.mroi.dim[2]) : 0u);
+        if (ROIABLE && mroi) {
+            const int b2 = boff / md01, rem = boff - b2 * md01;
+            const int b1 = rem / md0, b0 = rem - b1 * md0;
+            const RoiVol& rv = pd.mroi;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                rt[k] = uniform ((float) (rv.r0[k] + rv.Rd[3*k] * b0 + rv.Rd[3*k+1] * b1
+                    + rv.Rd[3*k+2] * b2));
+                rtop[k] = (float) (rv.dim[k] - 1);
+            }
+#pragma unroll
+            for (int k = 0; k < 9; k++) rR[k] = uniform (rv.R[k]);
+            rd0 = rv.dim[0];
+            rd01 = rv.dim[0] * rv.dim[1];
+        }
         if (fast && t < 64) count += nvox;      // every voxel of a lean tile is inside
 
         auto voxels = [&] (auto fast_c) {
@@ -1057,6 +1106,7 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
             int bn[G][8];
             float nv[G];            // WARP, NEAREST: the nearest moving voxel
             int wofs[G];            // WARP: the fixed voxel written, or -1
+            int rk[G];              // ROIs: 0 where a ROI excludes the voxel
 #pragma unroll
             for (int u = 0; u < G; u++) {
                 float dd[3] = {0.f, 0.f, 0.f};
@@ -1203,12 +1253,34 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                     // the bin quads of slices z and z + 1
                     bin_quads (bn[u], mr, 4 * idx, 4 * md01);
                 }
+                if constexpr (ROIABLE && !FAST) {
+                    rk[u] = 1;
+                    if (use_roi) {
+                        if (fmask) {
+                            const int fofs = fbase + slot_x (sw) + (int) __umul24 (fd0, slot_y (sw))
+                                + (int) __umul24 (fd01, slot_z (sw));
+                            rk[u] = load_u8 (fmr, fofs, 0);
+                        }
+                        if (mroi) {
+                            int ro = 0;
+#pragma unroll
+                            for (int k = 0; k < 3; k++) {
+                                const float pf = fmaf (rR[3*k], m[0], fmaf (rR[3*k+1], m[1],
+                                    fmaf (rR[3*k+2], m[2], rt[k])));
+                                const int ik = (int) __builtin_amdgcn_fmed3f (floorf (pf), 0.f, rtop[k]);
+                                ro += k == 0 ? ik : (k == 1 ? ik * rd0 : ik * rd01);
+                            }
+                            rk[u] = rk[u] & load_u8 (rmr, ro, 0);
+                        }
+                    }
+                }
             }
             if (!VF_OUT) {
             float bsum = 0.f;       // this batch's squared differences (<= G terms)
 #pragma unroll
             for (int u = 0; u < G; u++) {
                 const int v = base + u * TILE_THREADS + tt;
+                if constexpr (ROIABLE && !FAST) in[u] = in[u] & (rk[u] != 0);
                 if (WARP) {
                     // bspline_warp_dcos (src/base/bspline_warp.cxx:27-167):
                     // the moving value where the voxel lands, the fill
@@ -1566,7 +1638,8 @@ __global__ __launch_bounds__ (TILE_THREADS)
 void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
     const uint32_t* __restrict__ mquad, const float* __restrict__ coeff,
     const float* __restrict__ blut, unsigned long long* __restrict__ jhist,
-    unsigned long long* __restrict__ fhist, const float* __restrict__ ptab)
+    unsigned long long* __restrict__ fhist, const float* __restrict__ ptab,
+    const uint8_t* __restrict__ fmask, RoiVol mroi)
 {
     // the histograms first (the dynamic LDS base is 16-byte aligned), then
     // the LUTs; every pointer stays an LDS pointer, so the atomics are LDS
@@ -1604,6 +1677,9 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
             const int qx = rr - qy * vx;
             const int fi = vx * p[0] + qx, fj = vy * p[1] + qy, fk = vz * p[2] + qz;
             bool act = v < nvox && fi < G_.fdim[0] && fj < G_.fdim[1] && fk < G_.fdim[2];
+            // flavor h: the fixed ROI at the walk position
+            // (src/register/bspline_mi.cxx:1314-1318)
+            if (fmask && act) act = fmask[fi + G_.fdim[0] * (fj + G_.fdim[1] * fk)] != 0;
             Sample s;
             float d[3] = {0.f, 0.f, 0.f};
             float fxyz[3];
@@ -1622,6 +1698,12 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
                 exact_displacement (d, &s_bx[4 * qx], &s_by[4 * qy], &s_bz[4 * qz], coeff, knot0,
                     G_.cdims[0], G_.cdims[0] * G_.cdims[1]);
             act = correspond<ALIGNED> (G_, fxyz, d, s) && act;
+            // and the moving ROI at mxyz = fxyz + d
+            // (bspline_find_correspondence_dcos_roi, src/util/bspline_correspond.cxx:113-145)
+            if (mroi.img && act) {
+                const float mxyz[3] = {fxyz[0] + d[0], fxyz[1] + d[1], fxyz[2] + d[2]};
+                act = mroi.img[roi_offset (mroi, mxyz)] != 0;
+            }
             int b[8];
             bin_quads (b, qr, 4 * s.mvf, qdz);
             const int fidx = act ? fi + G_.fdim[0] * (fj + G_.fdim[1] * fk) : 0;
@@ -1697,6 +1779,38 @@ void k_pass1_positions (KGeom g, float* __restrict__ out, unsigned int* __restri
         w[2] += s[6];
     }
     if (bad) atomicAdd (mismatch, bad);
+}
+
+template <bool ALIGNED>
+__global__ __launch_bounds__ (256)
+void k_roi_fixed (KGeom g, RoiVol r, const float* __restrict__ ptab, uint8_t* __restrict__ out)
+{
+    const int64_t n = (int64_t) g.fdim[0] * g.fdim[1] * g.fdim[2];
+    for (int64_t v = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; v < n;
+         v += (int64_t) gridDim.x * blockDim.x) {
+        const int i = (int) (v % g.fdim[0]);
+        const int64_t jk = v / g.fdim[0];
+        const int j = (int) (jk % g.fdim[1]), k = (int) (jk / g.fdim[1]);
+        float xyz[3];
+        if (ptab) {
+            xyz[0] = ptab[3 * v];
+            xyz[1] = ptab[3 * v + 1];
+            xyz[2] = ptab[3 * v + 2];
+        } else {
+            position<ALIGNED> (xyz, g, i, j, k);
+        }
+        out[v] = r.img[roi_offset (r, xyz)] != 0 ? 1 : 0;
+    }
+}
+
+hipError_t
+launch_roi_fixed (const KGeom& g, const RoiVol& r, const float* ptab, uint8_t* out, hipStream_t st)
+{
+    if (g.aligned)
+        hipLaunchKernelGGL ((k_roi_fixed<true>), dim3 (2048), dim3 (256), 0, st, g, r, ptab, out);
+    else
+        hipLaunchKernelGGL ((k_roi_fixed<false>), dim3 (2048), dim3 (256), 0, st, g, r, ptab, out);
+    return hipGetLastError ();
 }
 
 hipError_t
@@ -2308,7 +2422,8 @@ hipError_t
 launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     const uint8_t* fbin, const uint32_t* mquad, const float* coeff, unsigned long long* jhist,
     unsigned long long* fhist, float* table, float* nvox_f, const EvalOut& out0,
-    int64_t* hist_out, const float* ptab, hipStream_t st, hipEvent_t* ev)
+    int64_t* hist_out, const float* ptab, const uint8_t* fmask1, const RoiVol& mroi,
+    hipStream_t st, hipEvent_t* ev)
 {
     const KGeom& g = plan.g0;
     const int nbins = h.fbins * h.mbins + h.fbins;
@@ -2323,10 +2438,10 @@ launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
     if (g.aligned)
         hipLaunchKernelGGL ((k_mi_hist<true>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
-            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist, ptab);
+            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist, ptab, fmask1, mroi);
     else
         hipLaunchKernelGGL ((k_mi_hist<false>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
-            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist, ptab);
+            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist, ptab, fmask1, mroi);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 2, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_mi_score, dim3 (1), dim3 (1024), 0, st, h, jhist, fhist, table,
