@@ -28,6 +28,8 @@ iterations on the headline workload, with and without regularization, and
 the reference's liblbfgs on the oracle's score for 3 iterations beside it.
 "stage_preprocess" times a stage's preprocessing on the device (both
 images subsampled, moving gradient, histogram ranges) against the oracle.
+"mse_128cube" evaluates the 128^3 pair (15-voxel grid) at zero and at the
+true coefficients, single-pair HIP beside the oracle's flavor g.
 "mattes_mi" evaluates Mattes MI on the 512x512x160 pair (12-voxel grid,
 32x32 bins, fixed-point PV histogram) with its roofline, the oracle's
 flavor g timed beside it and the histograms compared bin for bin.
@@ -83,13 +85,27 @@ def time_cpu_evals(case, coeffs, threads, budget_s, warmup=3, evals=10):
     return float(np.median(times)), len(times)
 
 
+def cpu_threads():
+    """Host threads for the CPU legs: OMP_NUM_THREADS where the environment
+    sets it (the host's share of cores), else every core of this process's
+    affinity mask; never more than the mask holds."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(aff, int(env))) if env.isdigit() and int(env) > 0 else aff
+
+
+def cpu_host_info():
+    return {"threads": cpu_threads(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "os_cpu_count": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(pair, coeffs, gpu_results, budget_s=12.0):
     """Time the oracle's flavor-g restatement (OpenMP) on a bounded sample of
-    the same workload, with every host thread available (up to 16) and with
-    one, and report the GPU results' error against the FP64 oracle on the
+    the same workload, with the host's threads (cpu_threads) and with one,
+    and report the GPU results' error against the FP64 oracle on the
     same coefficient vectors."""
     import oracle
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads = cpu_threads()
     case = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn)
     _ = case.m_grad
     med, n = time_cpu_evals(case, coeffs, threads, budget_s)
@@ -104,6 +120,7 @@ def cpu_baseline(pair, coeffs, gpu_results, budget_s=12.0):
         "cores": threads,
         "kind": "port",
         "cpu": cpu_model(),
+        "host": cpu_host_info(),
         "sample": f"{n} evals of the same workload after 3 warm-ups, median "
                   f"{med * 1e3:.1f} ms/eval, oracle flavor g (OpenMP, float as the reference)",
         "gb_per_s": alg_bytes(pair) / med / 1e9,
@@ -122,7 +139,7 @@ def batch_oracle_errors(pairs, coeffs, results):
     moving gradient."""
     import oracle
     from plastimatch_amd.bspline import bf16_round
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads = cpu_threads()
     errs = []
     for p, c, (score, grad) in zip(pairs, coeffs, results):
         case = oracle.Case(p.fixed, p.moving, p.spacing, p.vox_per_rgn, origin=p.origin)
@@ -298,7 +315,7 @@ def adjoint_leg(args, dev, want_errors):
                        "median_abs_rel": float(np.median(np.abs(g[idx] - fd) / np.maximum(np.abs(fd), 1e-30)))}
     if want_errors:
         import oracle
-        threads = min(16, len(os.sched_getaffinity(0)))
+        threads = cpu_threads()
         case = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn)
         s_ref, g_ref, _, _ = case.mse(c, "p", nthreads=threads)
         s64, g64, _ = case.mse_f64(c, 1, nthreads=threads)
@@ -385,7 +402,7 @@ def warp_reg_leg(args, dev, want_cpu):
     st.close()
     if want_cpu:
         import oracle
-        threads = min(16, len(os.sched_getaffinity(0)))
+        threads = cpu_threads()
         g = oracle.geom(pair.dim, pair.spacing, pair.vox_per_rgn, origin=pair.origin)
         t0 = time.perf_counter()
         ref_img = oracle.warp(g, pair.moving, pair.spacing, c, True, -1000.0, m_origin=pair.origin,
@@ -462,7 +479,7 @@ def optimizer_leg(args, dev, eval_ms, want_cpu):
     st.close()
     if want_cpu:
         import oracle
-        threads = min(16, len(os.sched_getaffinity(0)))
+        threads = cpu_threads()
         case = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn)
         _ = case.m_grad
 
@@ -608,7 +625,7 @@ def mi_leg(args, dev, want_cpu):
                         "scope": "whole evaluation (both passes)"}}
     if want_cpu:
         import oracle
-        threads = min(16, len(os.sched_getaffinity(0)))
+        threads = cpu_threads()
         oc = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn, origin=pair.origin)
         hp = oc.hist(MI_BINS)
         t0 = time.perf_counter()
@@ -637,6 +654,94 @@ def mi_leg(args, dev, want_cpu):
     return out
 
 
+CUBE_WORKLOAD = "mse_128cube_vpr15"
+
+
+def cube_leg(args, dev, want_cpu):
+    """MSE flavor g on the 128^3 pair (1 mm, 15-voxel grid, N(0, 2 mm)
+    field) at zero and at the true coefficients: evaluations per second of
+    the single-pair HIP path per vector, its tile kernel's roofline, and --
+    with the CPU leg -- the oracle's flavor g timed on the host's threads
+    and on one, with the device results' errors against the FP64 oracle
+    and the float flavor, and the overlap counts."""
+    import torch
+    from plastimatch_amd import synth
+    from plastimatch_amd.bspline import stage_for_pair
+
+    pair = synth.make_config(CUBE_WORKLOAD, seed=0)
+    st = stage_for_pair(pair)
+    vecs = {"zero": np.zeros(pair.true_coeff.size, np.float32), "true": pair.true_coeff}
+    d_score = torch.zeros(1, dtype=torch.float64, device=dev)
+    d_grad = torch.zeros(st.bxf.num_coeff, dtype=torch.float64, device=dev)
+    d_nvox = torch.zeros(1, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    d0 = torch.from_numpy(vecs["true"]).to(dev)
+    st.autotune("mse", "ref_nn", d0, reps=5)
+    vpt, wgs = st.get_tuning()
+    steps = max(50, args.steps // 5)
+    out = {"workload": CUBE_WORKLOAD, "dim": list(pair.dim), "vox_per_rgn": list(pair.vox_per_rgn),
+           "grad_mode": "ref_nn", "vox_per_thread": vpt, "workgroups_per_cu": wgs, "per_vector": {}}
+    alg = alg_bytes(pair)
+    for name, c in vecs.items():
+        dc = torch.from_numpy(c).to(dev)
+        for _ in range(5):
+            st.score_device(dc, d_score, d_grad, d_nvox, "mse", "ref_nn", stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st.score_device(dc, d_score, d_grad, d_nvox, "mse", "ref_nn", stream=stream)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        st.set_timing(True)
+        for _ in range(steps):
+            st.score_device(dc, d_score, d_grad, d_nvox, "mse", "ref_nn", stream=stream)
+        ms, n = st.get_timing()
+        st.set_timing(False)
+        tile_ms = float(ms[0] / n)
+        r = st.score(c, "mse", "ref_nn")
+        v = {"value": 1.0 / dt, "unit": "evals/s", "ms_per_eval": dt * 1e3,
+             "stage_ms": {"k_tiles": tile_ms, "k_reduce": float(ms[1] / n)},
+             "roofline": {"bound": "hbm", "achieved": alg / (tile_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": alg / (tile_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "alg_bytes_per_launch": alg, "kernel": "k_tiles"},
+             "score": r.score, "num_vox": r.num_vox, "_grad": r.grad}
+        out["per_vector"][name] = v
+    if want_cpu:
+        import oracle
+        threads = cpu_threads()
+        case = oracle.Case(pair.fixed, pair.moving, pair.spacing, pair.vox_per_rgn)
+        mg = case.m_grad
+        for name, c in vecs.items():
+            v = out["per_vector"][name]
+            times = []
+            for k in range(4):
+                t0 = time.perf_counter()
+                s_ref, g_ref, n_ref, _ = case.mse(c, "g", nthreads=threads)
+                times.append(time.perf_counter() - t0)
+            t1 = time.perf_counter()
+            case.mse(c, "g", nthreads=1)
+            t_one = time.perf_counter() - t1
+            s64, g64, n64 = case.mse_f64(c, 0, nthreads=threads)
+            g = v["_grad"]
+            v["cpu_baseline"] = {"value": 1.0 / float(np.median(times[1:])), "unit": "evals/s",
+                                 "cores": threads, "kind": "port", "host": cpu_host_info(),
+                                 "sample": "median of 3 evals after 1 warm-up, oracle flavor g (OpenMP)",
+                                 "one_thread": {"value": 1.0 / t_one, "unit": "evals/s", "cores": 1,
+                                                "sample": "one eval"}}
+            v["num_vox_oracle"] = n_ref
+            v["num_vox_fp64"] = n64
+            v["rel_err_vs_fp64"] = {"score": abs(v["score"] - s64) / abs(s64) if s64 else abs(v["score"]),
+                                    "grad_l2": float(np.linalg.norm(g - g64) / max(np.linalg.norm(g64), 1e-300))}
+            v["rel_err_vs_reference_flavor"] = {
+                "score": abs(v["score"] - s_ref) / abs(s_ref) if s_ref else abs(v["score"]),
+                "grad_l2": float(np.linalg.norm(g - g_ref) / max(np.linalg.norm(g_ref), 1e-300))}
+        del mg
+    for v in out["per_vector"].values():
+        del v["_grad"]
+    st.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -649,6 +754,7 @@ def main():
     ap.add_argument("--no-optimizer", action="store_true", help="skip the optimizer-loop leg")
     ap.add_argument("--no-stage", action="store_true", help="skip the stage-preprocessing leg")
     ap.add_argument("--no-mi", action="store_true", help="skip the Mattes-MI leg")
+    ap.add_argument("--no-cube", action="store_true", help="skip the 128^3 MSE leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy kernel 10 times "
                          "(reference for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -756,6 +862,8 @@ def main():
         base, errs = cpu_baseline(pair, coeffs, check)
         out["cpu_baseline"] = base
         out["rel_err_vs_fp64"] = {"score": max(e[0] for e in errs), "grad_l2": max(e[1] for e in errs)}
+    if not args.no_cube:
+        out["mse_128cube"] = cube_leg(args, dev, want_cpu=not args.no_cpu)
     if not args.no_batch:
         out["batched"] = batched_leg(args, dev, want_errors=not args.no_cpu)
     if not args.no_adjoint:

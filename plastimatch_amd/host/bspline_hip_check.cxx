@@ -41,6 +41,10 @@ usage ()
         "  -X, --input-xform file       coefficients, float32 raw, interleaved\n"
         "  -g, --grid-spac v            vox_per_rgn in all three axes (15)\n"
         "      --bins n                 MI histogram bins (32)\n"
+        "      --fixed-roi f.mha        fixed region of interest (uchar MetaImage)\n"
+        "      --moving-roi m.mha       moving region of interest (uchar MetaImage)\n"
+        "      --fixed-range lo hi      MI fixed intensity range (mi_fixed_image_minVal/maxVal)\n"
+        "      --moving-range lo hi     MI moving intensity range\n"
         "      --bf16                   keep the images in HBM as bfloat16\n"
         "      --iterations n           optimize: max iterations (10)\n"
         "      --gpuid n                device (0)\n"
@@ -61,6 +65,23 @@ read_input (const std::string& fn, Volume** out, std::string* err)
     }
     *out = v;
     return true;
+}
+
+/* A region of interest: a MetaImage read as float, one byte a voxel. */
+static Volume_uchar*
+read_roi (const std::string& fn, std::string* err)
+{
+    Volume* v = read_mha (fn, err);
+    if (!v) return nullptr;
+    Volume_uchar* r = nullptr;
+    if (v->vox_planes != 1) {
+        *err = fn + ": a region of interest is a scalar image";
+    } else {
+        r = new Volume_uchar (v->dim, v->origin, v->spacing, v->direction_cosines);
+        for (plm_long p = 0; p < r->npix; p++) r->img[(size_t) p] = (uint8_t) v->img[(size_t) p];
+    }
+    delete v;
+    return r;
 }
 
 int
@@ -100,7 +121,8 @@ main (int argc, char** argv)
         delete bxf;
         return ok ? 0 : 1;
     }
-    std::string metric = "mse", flavor = "g", process = "score", output, xform_fn;
+    std::string metric = "mse", flavor = "g", process = "score", output, xform_fn, froi_fn, mroi_fn;
+    float frange[2] = {0.f, 0.f}, mrange[2] = {0.f, 0.f};
     float step = 1e-4f, rlo = 0.f, rhi = 0.f;
     long index = 514;
     int line_a = 0, line_b = 30, seed = 0, vpr = 15, bins = 32, gpuid = 0, iters = 10;
@@ -124,6 +146,10 @@ main (int argc, char** argv)
         else if (a == "-X" || a == "--input-xform") xform_fn = next ();
         else if (a == "-g" || a == "--grid-spac") vpr = atoi (next ());
         else if (a == "--bins") bins = atoi (next ());
+        else if (a == "--fixed-roi") froi_fn = next ();
+        else if (a == "--moving-roi") mroi_fn = next ();
+        else if (a == "--fixed-range") { frange[0] = (float) atof (next ()); frange[1] = (float) atof (next ()); }
+        else if (a == "--moving-range") { mrange[0] = (float) atof (next ()); mrange[1] = (float) atof (next ()); }
         else if (a == "--bf16") bf16 = true;
         else if (a == "--iterations") iters = atoi (next ());
         else if (a == "--gpuid") gpuid = atoi (next ());
@@ -148,6 +174,22 @@ main (int argc, char** argv)
     parms.max_its = iters;
     parms.max_feval = 3 * iters + 3;
     parms.mi_hist_fixed_bins = parms.mi_hist_moving_bins = bins;
+    parms.mi_fixed_image_minVal = frange[0];
+    parms.mi_fixed_image_maxVal = frange[1];
+    parms.mi_moving_image_minVal = mrange[0];
+    parms.mi_moving_image_maxVal = mrange[1];
+    /* the stage's regions of interest (src/register/bspline_stage.cxx:129-192) */
+    Volume_uchar *froi = nullptr, *mroi = nullptr;
+    std::vector<Volume_uchar*> made;
+    if ((!froi_fn.empty () && !(froi = read_roi (froi_fn, &err)))
+        || (!mroi_fn.empty () && !(mroi = read_roi (mroi_fn, &err)))) {
+        printf ("{\"status\": %d, \"error\": \"%s\"}\n", PLM_HIP_ERR_ARG, err.c_str ());
+        return 1;
+    }
+    if (!bspline_stage_rois (&parms, fixed, moving, &froi, &mroi, &made)) {
+        printf ("{\"status\": %d, \"error\": \"a ROI and its image differ in dims\"}\n", PLM_HIP_ERR_ARG);
+        return 1;
+    }
 
     Bspline_xform bxf;
     const plm_long zero[3] = {0, 0, 0};
@@ -173,6 +215,8 @@ main (int argc, char** argv)
     Bspline_state bst;
     bst.fixed = fixed;
     bst.moving = moving;
+    bst.fixed_roi = froi;
+    bst.moving_roi = mroi;
     Volume* mgrad = nullptr;
     Joint_histogram* jh = nullptr;
     if (metric == "mi") {
@@ -255,6 +299,8 @@ main (int argc, char** argv)
         }
     }
     printf (", \"status\": %d}\n", rc);
+    delete froi;        /* (what bspline_stage_rois made is one of the two) */
+    delete mroi;
     delete mgrad;
     delete jh;
     delete fixed;

@@ -29,6 +29,7 @@ struct Hip_plugin {
     PLM_SYM (plm_hip_bspline_destroy)
     PLM_SYM (plm_hip_volume_gradient)
     PLM_SYM (plm_hip_bspline_num_knots)
+    PLM_SYM (plm_hip_bspline_set_roi)
 #undef PLM_SYM
 };
 
@@ -90,6 +91,7 @@ hip_plugin_load (std::string* err)
     PLM_LOAD (plm_hip_bspline_destroy)
     PLM_LOAD (plm_hip_volume_gradient)
     PLM_LOAD (plm_hip_bspline_num_knots)
+    PLM_LOAD (plm_hip_bspline_set_roi)
 #undef PLM_LOAD
     if (g_plugin.plm_hip_abi_version () != PLM_HIP_ABI_VERSION) {
         if (err) *err = "libplmregisterhip.so ABI version mismatch";
@@ -134,6 +136,69 @@ Volume::desc (int storage) const
     }
     memcpy (d.direction_cosines, direction_cosines, sizeof d.direction_cosines);
     return d;
+}
+
+Volume_uchar::Volume_uchar (const plm_long d[3], const float o[3], const float sp[3], const float* dc)
+{
+    for (int a = 0; a < 3; a++) {
+        dim[a] = d[a];
+        origin[a] = o[a];
+        spacing[a] = sp[a];
+    }
+    memcpy (direction_cosines, dc ? dc : IDENTITY, sizeof direction_cosines);
+    npix = dim[0] * dim[1] * dim[2];
+    img.assign ((size_t) npix, 0);
+}
+
+plm_img_desc
+Volume_uchar::desc () const
+{
+    plm_img_desc d;
+    d.data = img.data ();
+    d.location = PLM_MEM_HOST;
+    d.storage = PLM_STORE_U8;
+    for (int a = 0; a < 3; a++) {
+        d.dim[a] = dim[a];
+        d.origin[a] = origin[a];
+        d.spacing[a] = spacing[a];
+    }
+    memcpy (d.direction_cosines, direction_cosines, sizeof d.direction_cosines);
+    return d;
+}
+
+bool
+update_roi (Volume_uchar* roi, const Volume* image, float min_val, float max_val, bool fill_empty_roi)
+{
+    for (int a = 0; a < 3; a++)
+        if (roi->dim[a] != image->dim[a]) return false;
+    for (plm_long p = 0; p < roi->npix; p++) {
+        const float v = image->img[(size_t) p];
+        const bool in_range = v >= min_val && v <= max_val;
+        if (fill_empty_roi && in_range) roi->img[(size_t) p] = 1;
+        else if (!in_range && roi->img[(size_t) p] > 0) roi->img[(size_t) p] = 0;
+    }
+    return true;
+}
+
+bool
+bspline_stage_rois (const Bspline_parms* parms, const Volume* fixed, const Volume* moving,
+    Volume_uchar** fixed_roi, Volume_uchar** moving_roi, std::vector<Volume_uchar*>* made)
+{
+    struct Side { float lo, hi; const Volume* img; Volume_uchar** roi; };
+    const Side sides[2] = {
+        {parms->mi_moving_image_minVal, parms->mi_moving_image_maxVal, moving, moving_roi},
+        {parms->mi_fixed_image_minVal, parms->mi_fixed_image_maxVal, fixed, fixed_roi}};
+    for (const Side& sd : sides) {
+        if (!((sd.lo != 0 || sd.hi != 0) && sd.lo < sd.hi)) continue;
+        const bool fill = *sd.roi == nullptr;
+        if (fill) {
+            *sd.roi = new Volume_uchar (sd.img->dim, sd.img->origin, sd.img->spacing,
+                sd.img->direction_cosines);
+            made->push_back (*sd.roi);
+        }
+        if (!update_roi (*sd.roi, sd.img, sd.lo, sd.hi, fill)) return false;
+    }
+    return true;
 }
 
 /* rdims = 1 + (roi_dim-1)/vpr, cdims = rdims + 3, as Bspline_header::set
@@ -451,7 +516,8 @@ Joint_histogram::desc () const
 Bspline_parms::Bspline_parms ()
     : threading (BTHR_HIP), implementation ('g'), gpuid (0), storage (PLM_STORE_F32),
       grad_mode (PLM_GRAD_REF_NN), mi_hist_fixed_bins (32), mi_hist_moving_bins (32),
-      max_its (10), max_feval (10)
+      mi_fixed_image_minVal (0), mi_fixed_image_maxVal (0), mi_moving_image_minVal (0),
+      mi_moving_image_maxVal (0), max_its (10), max_feval (10)
 {
 }
 
@@ -497,7 +563,7 @@ Bspline_score::accumulate (float lambda)
 Bspline_state::Bspline_state ()
     : it (0), feval (0), dev_ptrs (nullptr), metric_type (SIMILARITY_METRIC_MSE),
       metric_lambda (1.0f), fixed (nullptr), moving (nullptr), moving_grad (nullptr),
-      mi_hist (nullptr)
+      mi_hist (nullptr), fixed_roi (nullptr), moving_roi (nullptr)
 {
 }
 
@@ -535,6 +601,17 @@ bspline_state_initialize_hip (Bspline_state* bst, Bspline_xform* bxf, Bspline_pa
         return rc;
     }
     bst->dev_ptrs = ctx;
+    if (bst->fixed_roi || bst->moving_roi) {
+        plm_img_desc fr, mr;
+        if (bst->fixed_roi) fr = bst->fixed_roi->desc ();
+        if (bst->moving_roi) mr = bst->moving_roi->desc ();
+        rc = g_plugin.plm_hip_bspline_set_roi (ctx, bst->fixed_roi ? &fr : nullptr,
+            bst->moving_roi ? &mr : nullptr);
+        if (rc < 0) {
+            bst->last_error = hip_error ();
+            return rc;
+        }
+    }
     return PLM_HIP_OK;
 }
 

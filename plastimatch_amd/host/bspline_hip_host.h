@@ -53,6 +53,29 @@ public:
     plm_img_desc desc (int storage) const;
 };
 
+/* A uchar volume (PT_UCHAR, src/base/volume.h:39-57): the fixed and moving
+   regions of interest. */
+class Volume_uchar {
+public:
+    plm_long dim[3];
+    plm_long npix;
+    float origin[3];
+    float spacing[3];
+    float direction_cosines[9];
+    std::vector<uint8_t> img;
+public:
+    Volume_uchar (const plm_long dim[3], const float origin[3], const float spacing[3],
+        const float* dc);
+    plm_img_desc desc () const;
+};
+
+/* update_roi (src/register/bspline_stage.cxx:71-93): voxels of image
+   outside [min_val, max_val] leave roi; with fill_empty_roi, voxels inside
+   the range join it.  roi and image share one grid, voxel for voxel
+   (false, and nothing changed, if their dims differ). */
+bool update_roi (Volume_uchar* roi, const Volume* image, float min_val, float max_val,
+    bool fill_empty_roi);
+
 class Bspline_xform {
 public:
     float img_origin[3];
@@ -126,6 +149,12 @@ public:
     int grad_mode;                /* PLM_GRAD_REF_NN (flavor g) or PLM_GRAD_EXACT (p) */
     plm_long mi_hist_fixed_bins;
     plm_long mi_hist_moving_bins;
+    /* user intensity ranges of MI (Bspline_parms, src/register/
+       bspline_parms.h:34-92); a set range becomes a ROI (bspline_stage_rois) */
+    float mi_fixed_image_minVal;
+    float mi_fixed_image_maxVal;
+    float mi_moving_image_minVal;
+    float mi_moving_image_maxVal;
     int max_its;
     int max_feval;
 public:
@@ -170,6 +199,8 @@ public:
     Volume* moving;
     Volume* moving_grad;
     Joint_histogram* mi_hist;
+    Volume_uchar* fixed_roi;      /* not owned; either may be null */
+    Volume_uchar* moving_roi;
     std::string last_error;
 public:
     Bspline_state ();
@@ -196,7 +227,16 @@ public:
    false with a message when the library or a symbol is missing. */
 bool hip_plugin_load (std::string* err);
 
-/* Creates the device state for the stage (images uploaded once). */
+/* The stage's ROI preparation (src/register/bspline_stage.cxx:150-192):
+   each image's intensity range, when set (min or max non-zero, min < max),
+   narrows that image's ROI, or makes one on the image's grid, which *made
+   then holds (owned by the caller).  Returns false if a given ROI's dims
+   differ from its image's. */
+bool bspline_stage_rois (const Bspline_parms* parms, const Volume* fixed, const Volume* moving,
+    Volume_uchar** fixed_roi, Volume_uchar** moving_roi, std::vector<Volume_uchar*>* made);
+
+/* Creates the device state for the stage (images uploaded once, and the
+   state's ROIs, if any, handed to the library). */
 int bspline_state_initialize_hip (Bspline_state* bst, Bspline_xform* bxf,
     Bspline_parms* parms);
 

@@ -115,3 +115,35 @@ def test_host_xform_file_round_trip(tmp_path):
     r = subprocess.run([TOOL, "--xf-roundtrip", str(bad), str(tmp_path / "x.txt")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1 and "bad header line" in r.stderr
+
+
+@pytest.mark.gpu
+def test_host_mi_rois_and_ranges(tmp_path):
+    """MI flavor h through the C++ host: a fixed ROI file narrowed by the
+    fixed intensity range, and a moving ROI made from the moving range
+    (bspline_stage_rois / update_roi), against the oracle with the ROIs the
+    stage's rules give."""
+    from plastimatch_amd.bspline import Volume, stage_rois
+    pair = small_pair(dim=(40, 36, 28), spacing=(1.0, 1.0, 2.5), vpr=(5, 5, 5), seed=35)
+    pair.moving = synth.ct_remap(pair.moving.astype(np.float64)).astype(np.float32)
+    nz, ny, nx = pair.fixed.shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    box = ((i > 4) & (i < 34) & (j > 3) & (j < 30) & (k > 2) & (k < 25)).astype(np.uint8)
+    froi_fn = os.path.join(str(tmp_path), "froi.mha")
+    synth.write_mha(froi_fn, box.astype(np.float32), pair.spacing, pair.origin)
+    flo, fhi = (float(v) for v in np.percentile(pair.fixed, [15, 97]))
+    mlo, mhi = (float(v) for v in np.percentile(pair.moving, [10, 92]))
+    gfile = os.path.join(str(tmp_path), "grad.raw")
+    rc, out, err = run_tool(["-g", "5", "-M", "mi", "--bins", "24", "--fixed-roi", froi_fn,
+                             "--fixed-range", repr(flo), repr(fhi), "--moving-range", repr(mlo), repr(mhi),
+                             "-O", gfile], str(tmp_path), pair)
+    assert rc == 0, (out, err)
+    fixed = Volume(pair.fixed, pair.spacing, pair.origin)
+    moving = Volume(pair.moving, pair.spacing, pair.origin)
+    fr, mr = stage_rois(fixed, moving, Volume(box, pair.spacing, pair.origin), None,
+                        np.float32(flo), np.float32(fhi), np.float32(mlo), np.float32(mhi))
+    oc = oracle_case(pair, fixed_roi=(fr.img, fr.spacing, fr.origin), moving_roi=(mr.img, mr.spacing, mr.origin))
+    o = oc.mi(np.full(oc.num_coeff, 0.01, np.float32), oc.hist(24))
+    assert out["num_vox"] == o["num_vox"] and o["num_vox"] < 0.6 * pair.fixed.size
+    assert rel_scalar(out["score"], o["score"]) < 1e-5
+    assert rel(np.fromfile(gfile, np.float64), o["grad"]) < 1e-4
