@@ -204,8 +204,6 @@ int plm_hip_bspline_score_noalloc (
     int64_t* d_num_vox,
     void* stream);
 
-/* Replaces CUDA_bspline_mse_cleanup_j / CUDA_bspline_mi_cleanup_a and
- * bspline_cuda_state_destroy (src/register/bspline_state.cxx:239-271). */
 /* Regions of interest of a context: the fixed and moving ROI volumes of
  * Bspline_state (fixed_roi / moving_roi, src/register/bspline_state.h:44-45,
  * set by Bspline_stage, src/register/bspline_stage.cxx:129-205), uchar
@@ -225,6 +223,21 @@ int plm_hip_bspline_score_noalloc (
 int plm_hip_bspline_set_roi (plm_hip_ctx* ctx, const plm_img_desc* fixed_roi,
     const plm_img_desc* moving_roi);
 
+/* V-optimal bins for a context's MI (Histogram::type HIST_VOPT and its
+ * key_lut, src/register/histogram.h:14-35): fixed_key_lut / moving_key_lut
+ * (PLM_HIP_VOPT_KEYS ints each, host memory, as plm_hip_hist_vopt writes
+ * them; NULL keeps that side's equal-spacing bins) map the key
+ * floor((v - offset) / delta) of a voxel to its bin, as add_pvi_8 and the
+ * pass-2 derivative do (src/register/joint_histogram.cxx:458-481,
+ * bspline_mi.txx:82-94); offset and delta are the hist the context was
+ * created with, which must then be the key grid's (plm_hip_hist_vopt's).
+ * Every entry must lie in [0, bins - 1].  The bin images are rebuilt; no
+ * evaluation of the context may be in flight. */
+int plm_hip_bspline_set_hist_keys (plm_hip_ctx* ctx, const int32_t* fixed_key_lut,
+    const int32_t* moving_key_lut);
+
+/* Replaces CUDA_bspline_mse_cleanup_j / CUDA_bspline_mi_cleanup_a and
+ * bspline_cuda_state_destroy (src/register/bspline_state.cxx:239-271). */
 int plm_hip_bspline_destroy (plm_hip_ctx* ctx);
 
 /* Moving-image gradient for PLM_GRAD_REF_NN, computed on the GPU:
@@ -406,6 +419,25 @@ int plm_hip_stage_resample (int kind, const float rate[3], const plm_img_desc* i
  * delta = (max - min) / (bins - 1), offset = min - delta / 2. */
 int plm_hip_hist_eqsp (const plm_img_desc* vol, int64_t bins, float* offset, float* delta,
     int gpuid);
+
+/* Keys of the V-optimal histogram (VOPT_RES, src/register/histogram.h:12). */
+#define PLM_HIP_VOPT_KEYS 1000
+
+/* Replaces bspline_initialize_mi_hist_vopt
+ * (src/register/joint_histogram.cxx:195-324), the histogram_type=vopt
+ * initialization (registration_parms.cxx:478-479): the image's range split
+ * into PLM_HIP_VOPT_KEYS equal keys (offset and delta of that grid, the
+ * equal-spacing formulas with the keys for bins), then the grouping of
+ * consecutive keys into bins that the reference's dynamic programme picks;
+ * key_lut (PLM_HIP_VOPT_KEYS ints, host memory) receives the bin of every
+ * key.  The same offset, delta and table as the reference, bit for bit: the
+ * range scan, the key counts and sums (exact in any order) and the
+ * programme run on the device, the running sums over the keys on the host.
+ * bins must be in [2, PLM_HIP_VOPT_KEYS] (the reference falls back to equal
+ * spacing above, which its caller does here); a constant image is refused.
+ * Use the result with plm_hip_bspline_set_hist_keys. */
+int plm_hip_hist_vopt (const plm_img_desc* vol, int64_t bins, float* offset, float* delta,
+    int32_t* key_lut, int gpuid);
 
 /* ------------------------------------------------------------------------
  * The optimizer loop on the device: L-BFGS with the More-Thuente line

@@ -55,10 +55,23 @@ def reg_parms(**kw):
     return OrcRegParms(**d)
 
 
+VOPT_KEYS = 1000
+
+
 class OrcHist(ctypes.Structure):
+    """orc_hist; f_keys / m_keys point at a side's V-optimal key table (its
+    numpy array is kept on the instance) or are NULL (equal spacing)."""
     _fields_ = [("fixed_bins", ctypes.c_int64), ("moving_bins", ctypes.c_int64),
                 ("f_offset", ctypes.c_float), ("f_delta", ctypes.c_float),
-                ("m_offset", ctypes.c_float), ("m_delta", ctypes.c_float)]
+                ("m_offset", ctypes.c_float), ("m_delta", ctypes.c_float),
+                ("f_keys", ctypes.c_void_p), ("m_keys", ctypes.c_void_p)]
+
+    def set_keys(self, f_lut=None, m_lut=None):
+        self._luts = (None if f_lut is None else np.ascontiguousarray(f_lut, np.int32),
+                      None if m_lut is None else np.ascontiguousarray(m_lut, np.int32))
+        self.f_keys = None if f_lut is None else self._luts[0].ctypes.data
+        self.m_keys = None if m_lut is None else self._luts[1].ctypes.data
+        return self
 
 
 class OrcLbfgsParms(ctypes.Structure):
@@ -115,6 +128,9 @@ def lib():
                                ctypes.c_float, ctypes.c_int, P]
         L.orc_volume_gradient.argtypes = [ctypes.POINTER(OrcVolHdr), P, P]
         L.orc_hist_eqsp.argtypes = [P, ctypes.c_int64, ctypes.c_int64, P, P]
+        L.orc_hist_vopt.restype = ctypes.c_int
+        L.orc_hist_vopt.argtypes = [P, ctypes.c_int64, ctypes.c_int64, P, P, P, P, P]
+        L.orc_vopt_partition.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64, P]
         L.orc_pass1_position_mismatch.restype = ctypes.c_int64
         L.orc_pass1_position_mismatch.argtypes = [ctypes.POINTER(OrcVolHdr), ctypes.POINTER(OrcGeom)]
         L.orc_mse_score.restype = ctypes.c_int
@@ -294,6 +310,34 @@ def hist_eqsp(img, bins):
     return float(off[0]), float(dl[0])
 
 
+def hist_vopt(img, bins, with_keys=False):
+    """V-optimal histogram of an image: (offset, delta, key table) of the
+    VOPT_KEYS-key grid (and, with_keys, the per-key counts and sums of
+    v - offset); None where the reference would not build one (bins
+    outside [2, VOPT_KEYS], constant image)."""
+    img = np.ascontiguousarray(img, np.float32)
+    off = np.zeros(1, np.float32)
+    dl = np.zeros(1, np.float32)
+    lut = np.zeros(VOPT_KEYS, np.int32)
+    cnt = np.zeros(VOPT_KEYS, np.float64)
+    sm = np.zeros(VOPT_KEYS, np.float64)
+    if lib().orc_hist_vopt(_ptr(img), img.size, bins, _ptr(off), _ptr(dl), _ptr(lut), _ptr(cnt), _ptr(sm)):
+        return None
+    if with_keys:
+        return float(off[0]), float(dl[0]), lut, cnt, sm
+    return float(off[0]), float(dl[0]), lut
+
+
+def vopt_partition(s, ssq, cnt, bins):
+    """The partition step of the V-optimal histogram on given prefix tables."""
+    s = np.ascontiguousarray(s, np.float64)
+    ssq = np.ascontiguousarray(ssq, np.float64)
+    cnt = np.ascontiguousarray(cnt, np.float64)
+    lut = np.zeros(s.size, np.int32)
+    lib().orc_vopt_partition(_ptr(s), _ptr(ssq), _ptr(cnt), s.size, bins, _ptr(lut))
+    return lut
+
+
 def pass1_position_mismatch(fixed, spacing, g, origin=(0.0, 0.0, 0.0), dc=IDENTITY):
     h = vol_hdr(fixed, spacing, origin, dc)
     return lib().orc_pass1_position_mismatch(ctypes.byref(h), ctypes.byref(g))
@@ -409,8 +453,15 @@ class Case:
             border.append(cp + cm + abs(np_ - n0) + abs(nm - n0))
         return np.array(fd), np.array(border)
 
-    def hist(self, bins=32, f_bins=None):
+    def hist(self, bins=32, f_bins=None, vopt=False):
+        """Histogram parameters of the pair: equal spacing, or V-optimal on
+        both sides (vopt=True; Joint_histogram::initialize with
+        HIST_VOPT)."""
         fb = f_bins or bins
+        if vopt:
+            fo, fd, fl = hist_vopt(self.fixed, fb)
+            mo, md, ml = hist_vopt(self.moving, bins)
+            return OrcHist(fb, bins, fo, fd, mo, md).set_keys(fl, ml)
         fo, fd = hist_eqsp(self.fixed, fb)
         mo, md = hist_eqsp(self.moving, bins)
         return OrcHist(fb, bins, fo, fd, mo, md)

@@ -1310,6 +1310,113 @@ orc_hist_eqsp (const float* img, int64_t npix, int64_t bins, float* offset,
     *offset = (float) (mn - 0.5 * *delta);
 }
 
+/* V-optimal histogram.  Follows bspline_initialize_mi_hist_vopt and
+   vopt_bin_error (src/register/joint_histogram.cxx:62-82, 195-324): the
+   image range split into ORC_VOPT_KEYS equal keys (the grid of
+   bspline_initialize_mi_hist_eqsp with keys for bins), each key's voxel
+   count and average of (v - offset) (a key with no voxels takes the
+   previous key's average), running sums of the averages, of their squares
+   and of the counts, then a dynamic programme over (bins, keys) for the
+   contiguous grouping of keys into bins of least total error, a bin's
+   error being ssq - s^2 / n over its prefix-sum differences (and DBL_MAX
+   when it holds less than one voxel), ties going to the last split
+   examined; finally the key table by walking the splits back. */
+static double
+vopt_err (int64_t start, int64_t end, const double* s, const double* ssq, const double* cnt)
+{
+    const double sq = ssq[end] - ssq[start];
+    const double d = s[end] - s[start];
+    const double nk = (double) end - (double) start + 1.0;
+    const double n = cnt[end] - cnt[start];
+    const double v = sq - (d * d) / nk;
+    return n < 1.0 ? DBL_MAX : v;
+}
+
+void
+orc_vopt_partition (const double* s, const double* ssq, const double* cnt, int64_t keys,
+    int64_t bins, int32_t* key_lut)
+{
+    double* err = (double*) malloc (sizeof (double) * bins * keys);
+    int64_t* split = (int64_t*) calloc ((size_t) (bins * keys), sizeof (int64_t));
+    for (int64_t i = 0; i < keys; i++) err[i] = vopt_err (0, i, s, ssq, cnt);
+    for (int64_t j = 1; j < bins; j++) {
+        for (int64_t i = 0; i < keys; i++) {
+            double best = DBL_MAX;
+            int64_t arg = 0;
+            for (int64_t k = 0; k < i; k++) {
+                const double c = err[(j - 1) * keys + k] + vopt_err (k + 1, i, s, ssq, cnt);
+                if (c <= best) {
+                    best = c;
+                    arg = k;
+                }
+            }
+            err[j * keys + i] = best;
+            split[j * keys + i] = arg;
+        }
+    }
+    for (int64_t i = 0; i < keys; i++) key_lut[i] = 0;
+    int64_t hi = keys - 1;
+    for (int64_t j = bins - 1; j >= 0; j--) {
+        const int64_t nx = split[j * keys + hi];
+        const int64_t lo = j == 0 ? 0 : nx + 1;
+        for (int64_t i = lo; i <= hi; i++) key_lut[i] = (int32_t) j;
+        hi = nx;
+    }
+    free (err);
+    free (split);
+}
+
+int
+orc_hist_vopt (const float* img, int64_t npix, int64_t bins, float* offset, float* delta,
+    int32_t* key_lut, double* key_count, double* key_sum)
+{
+    const int64_t keys = ORC_VOPT_KEYS;
+    if (bins < 2 || bins > keys || npix < 1) return -1;
+    float mn = img[0], mx = img[0];
+    for (int64_t i = 1; i < npix; i++) {
+        if (img[i] < mn) mn = img[i];
+        else if (img[i] > mx) mx = img[i];
+    }
+    if (!(mx > mn)) return -1;
+    const float dl = (mx - mn) / (float) (keys - 1);
+    const float off = (float) (mn - 0.5 * dl);
+    double* cnt1 = (double*) calloc ((size_t) keys, sizeof (double));
+    double* avg = (double*) calloc ((size_t) keys, sizeof (double));
+    for (int64_t i = 0; i < npix; i++) {
+        const float x = img[i] - off;
+        int64_t k = (int64_t) floorf (x / dl);
+        k = k < 0 ? 0 : (k >= keys ? keys - 1 : k);
+        cnt1[k] += 1.0;
+        avg[k] += x;
+    }
+    if (key_count) memcpy (key_count, cnt1, sizeof (double) * keys);
+    if (key_sum) memcpy (key_sum, avg, sizeof (double) * keys);
+    for (int64_t i = 0; i < keys; i++) {
+        if (cnt1[i] > 0) avg[i] = avg[i] / cnt1[i];
+        else if (i > 0) avg[i] = avg[i - 1];
+    }
+    double* s = (double*) malloc (sizeof (double) * keys);
+    double* ssq = (double*) malloc (sizeof (double) * keys);
+    double* cnt = (double*) malloc (sizeof (double) * keys);
+    s[0] = avg[0];
+    ssq[0] = avg[0] * avg[0];
+    cnt[0] = cnt1[0];
+    for (int64_t i = 1; i < keys; i++) {
+        s[i] = s[i - 1] + avg[i];
+        ssq[i] = ssq[i - 1] + avg[i] * avg[i];
+        cnt[i] = cnt[i - 1] + cnt1[i];
+    }
+    orc_vopt_partition (s, ssq, cnt, keys, bins, key_lut);
+    *offset = off;
+    *delta = dl;
+    free (cnt1);
+    free (avg);
+    free (s);
+    free (ssq);
+    free (cnt);
+    return 0;
+}
+
 /* Fixed-point partial-volume weights: q = floor(w*2^32 + 0.5), and the
    remainder 2^32 - sum(q) is added to the first largest weight, so every
    voxel adds exactly 2^32 to the moving and joint histograms.  This
@@ -1332,6 +1439,27 @@ static inline int
 hist_bin (float v, float offset, float delta)
 {
     return (int) floorf ((v - offset) / delta);
+}
+
+/* The bin of a fixed / moving value: its equal-spacing bin, or with a key
+   table the bin of its key (src/register/joint_histogram.cxx:458-481). */
+static inline int
+key_bin (int b, const int32_t* keys)
+{
+    if (!keys) return b;
+    return keys[b < 0 ? 0 : (b >= ORC_VOPT_KEYS ? ORC_VOPT_KEYS - 1 : b)];
+}
+
+static inline int
+fbin_of (const orc_hist* hp, float v)
+{
+    return key_bin (hist_bin (v, hp->f_offset, hp->f_delta), hp->f_keys);
+}
+
+static inline int
+mbin_of (const orc_hist* hp, float v)
+{
+    return key_bin (hist_bin (v, hp->m_offset, hp->m_delta), hp->m_keys);
 }
 
 /* PV-8 weights w[n], n in x,y,z bit order.  Follows
@@ -1401,10 +1529,10 @@ mi_dc_dv (float dc_dv[3], const double* f_hist, const double* m_hist,
     dw[15] = (+1) * l1[1] * l2[2]; dw[16] = l2[0] * (-1) * l2[2]; dw[17] = l2[0] * l1[1] * (+1);
     dw[18] = (-1) * l2[1] * l2[2]; dw[19] = l1[0] * (+1) * l2[2]; dw[20] = l1[0] * l2[1] * (+1);
     dw[21] = (+1) * l2[1] * l2[2]; dw[22] = l2[0] * (+1) * l2[2]; dw[23] = l2[0] * l2[1] * (+1);
-    int fb = hist_bin (f_val, hp->f_offset, hp->f_delta);
+    int fb = fbin_of (hp, f_val);
     int64_t off = (int64_t) fb * hp->moving_bins;
     for (int pv = 0; pv < 8; pv++) {
-        int mb = hist_bin (m_img[n[pv]], hp->m_offset, hp->m_delta);
+        int mb = mbin_of (hp, m_img[n[pv]]);
         int64_t jb = off + mb;
         if (j_hist[jb] > 0.0001) {
             float ds = logf ((float) (((double) num_vox_f * j_hist[jb])
@@ -1532,12 +1660,12 @@ orc_mi_score (const orc_geom* g,
                         pv_weights (w, l1, l2);
                         pv_neighbors (n, mvf, mv.dim);
                         orc_pv_quantize (w, qw);
-                        int fb = hist_bin (f_img[fidx], hp->f_offset, hp->f_delta);
+                        int fb = fbin_of (hp, f_img[fidx]);
                         f_hist[fb]++;
                         if (fx_f_hist) fx_f_hist[fb]++;
                         int64_t off = (int64_t) fb * mbins;
                         for (int pv = 0; pv < 8; pv++) {
-                            int mb = hist_bin (m_img[n[pv]], hp->m_offset, hp->m_delta);
+                            int mb = mbin_of (hp, m_img[n[pv]]);
                             m_hist[mb] += w[pv];
                             j_hist[off + mb] += w[pv];
                             if (fx_j_hist) fx_j_hist[off + mb] += qw[pv];
@@ -1696,12 +1824,12 @@ mi_f64 (const orc_geom* g,
                     l1[0]*l1[1]*l2[2], l2[0]*l1[1]*l2[2], l1[0]*l2[1]*l2[2], l2[0]*l2[1]*l2[2]};
                 int64_t n[8];
                 pv_neighbors (n, s.mvf, mv.dim);
-                int fb = hist_bin (f_img[fidx], hp->f_offset, hp->f_delta);
+                int fb = fbin_of (hp, f_img[fidx]);
                 int64_t off = (int64_t) fb * mbins;
                 if (pass == 1) {
                     fh_[fb] += 1;
                     for (int pv = 0; pv < 8; pv++) {
-                        int mb = hist_bin (m_img[n[pv]], hp->m_offset, hp->m_delta);
+                        int mb = mbin_of (hp, m_img[n[pv]]);
                         mh_[mb] += w[pv];
                         jh_[off + mb] += w[pv];
                     }
@@ -1718,7 +1846,7 @@ mi_f64 (const orc_geom* g,
                         +l2[1]*l2[2], +l2[0]*l2[2], +l2[0]*l2[1]};
                     double dc[3] = {0, 0, 0};
                     for (int pv = 0; pv < 8; pv++) {
-                        int mb = hist_bin (m_img[n[pv]], hp->m_offset, hp->m_delta);
+                        int mb = mbin_of (hp, m_img[n[pv]]);
                         int64_t jb = off + mb;
                         if (jh_[jb] > 0.0001) {
                             double ds = log ((double) nv * jh_[jb] / (fh_[fb] * mh_[mb])) - *score;

@@ -51,12 +51,18 @@ typedef struct orc_geom {
     float dc[9];
 } orc_geom;
 
-/* Equal-spacing histogram parameters (src/register/histogram.h:19-35). */
+/* Histogram parameters (src/register/histogram.h:19-35): equal spacing,
+   or V-optimal where a side has a key table (f_keys / m_keys non-NULL:
+   ORC_VOPT_KEYS entries, the bin of each equal-spacing key, with offset
+   and delta those of the key grid). */
+#define ORC_VOPT_KEYS 1000
 typedef struct orc_hist {
     int64_t fixed_bins;
     int64_t moving_bins;
     float f_offset, f_delta;
     float m_offset, m_delta;
+    const int32_t* f_keys;
+    const int32_t* m_keys;
 } orc_hist;
 
 int64_t orc_header (const orc_geom* g, int64_t rdims[3], int64_t cdims[3]);
@@ -71,6 +77,18 @@ int orc_warp (const orc_geom* g, const orc_vol_hdr* mh, const float* m_img,
 void orc_volume_gradient (const orc_vol_hdr* h, const float* img, float* out);
 void orc_hist_eqsp (const float* img, int64_t npix, int64_t bins,
     float* offset, float* delta);
+/* V-optimal histogram (bspline_initialize_mi_hist_vopt): offset and delta
+   of the ORC_VOPT_KEYS-key grid and the key -> bin table; key_count and
+   key_sum (optional, ORC_VOPT_KEYS each) receive the per-key voxel counts
+   and sums of (v - offset).  Returns -1 for bins outside [2, keys] or a
+   constant image. */
+int orc_hist_vopt (const float* img, int64_t npix, int64_t bins, float* offset,
+    float* delta, int32_t* key_lut, double* key_count, double* key_sum);
+/* Its partition step alone: bins contiguous groups of keys from the
+   prefix tables s (running sum of key averages), ssq (of their squares)
+   and cnt (of key counts). */
+void orc_vopt_partition (const double* s, const double* ssq, const double* cnt,
+    int64_t keys, int64_t bins, int32_t* key_lut);
 int64_t orc_pass1_position_mismatch (const orc_vol_hdr* fh, const orc_geom* g);
 
 /* flavor: 'g' (tile condense, nearest-voxel m_grad), 'p' (tile condense,

@@ -10,6 +10,7 @@ import ctypes
 import os
 
 LIB_NAME = "libplmregisterhip.so"
+VOPT_KEYS = 1000      # PLM_HIP_VOPT_KEYS
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 PLM_HIP_OK = 0
@@ -167,6 +168,10 @@ SIGNATURES = {
     "plm_hip_hist_eqsp": (ctypes.c_int, [ctypes.POINTER(PlmImgDesc), ctypes.c_int64,
                                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                          ctypes.c_int]),
+    "plm_hip_hist_vopt": (ctypes.c_int, [ctypes.POINTER(PlmImgDesc), ctypes.c_int64,
+                                         ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _P,
+                                         ctypes.c_int]),
+    "plm_hip_bspline_set_hist_keys": (ctypes.c_int, [_P, _P, _P]),
     "plm_hip_lbfgs_parms_init": (None, [ctypes.POINTER(PlmLbfgsParms)]),
     "plm_hip_bspline_optimize_lbfgs": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(PlmRegParms),
                                                       ctypes.POINTER(PlmLbfgsParms), _P, _P, PLM_LBFGS_PROGRESS,

@@ -176,19 +176,38 @@ class Bspline_xform:
 
 
 class Joint_histogram:
-    """Equal-spacing joint histogram parameters (src/register/joint_histogram.cxx:115-139)."""
+    """Joint histogram parameters (src/register/histogram.h:14-35,
+    joint_histogram.cxx:115-139, 195-346): equal spacing ("eqsp") or
+    V-optimal ("vopt", the stage's histogram_type), per side offset and
+    delta and, for V-optimal bins, the key table."""
 
-    def __init__(self, fixed_bins=32, moving_bins=32):
+    def __init__(self, fixed_bins=32, moving_bins=32, hist_type="eqsp"):
+        if hist_type not in ("eqsp", "vopt"):
+            raise ValueError("hist_type is 'eqsp' or 'vopt'")
         self.fixed_bins = int(fixed_bins)
         self.moving_bins = int(moving_bins)
+        self.fixed_type = self.moving_type = hist_type
         self.fixed_offset = self.fixed_delta = 0.0
         self.moving_offset = self.moving_delta = 0.0
+        self.fixed_keys = self.moving_keys = None
 
     def initialize(self, fixed, moving, gpuid=0):
-        """Bin offsets and widths from each image's range, scanned on the GPU
-        (bspline_initialize_mi_hist_eqsp)."""
-        self.fixed_offset, self.fixed_delta = hist_eqsp(fixed, self.fixed_bins, gpuid)
-        self.moving_offset, self.moving_delta = hist_eqsp(moving, self.moving_bins, gpuid)
+        """Bin offsets and widths (and V-optimal key tables) from each image,
+        computed on the GPU (bspline_initialize_mi_hist).  As the reference
+        does, a side asking for more V-optimal bins than there are keys falls
+        back to equal spacing."""
+        out = []
+        for vol, bins, kind in ((fixed, self.fixed_bins, self.fixed_type),
+                                (moving, self.moving_bins, self.moving_type)):
+            if kind == "vopt" and bins > _abi.VOPT_KEYS:
+                print(f"WARNING: Falling back to EQSP histograms.\n         (Reason: # bins > {_abi.VOPT_KEYS})")
+                kind = "eqsp"
+            if kind == "vopt":
+                out.append((kind,) + hist_vopt(vol, bins, gpuid))
+            else:
+                out.append((kind,) + hist_eqsp(vol, bins, gpuid) + (None,))
+        (self.fixed_type, self.fixed_offset, self.fixed_delta, self.fixed_keys), \
+            (self.moving_type, self.moving_offset, self.moving_delta, self.moving_keys) = out
         return self
 
     def desc(self):
@@ -230,6 +249,18 @@ def hist_eqsp(vol, bins, gpuid=0):
     off, dl = ctypes.c_float(), ctypes.c_float()
     _abi.check(lib.plm_hip_hist_eqsp(ctypes.byref(d), int(bins), ctypes.byref(off), ctypes.byref(dl), gpuid))
     return float(off.value), float(dl.value)
+
+
+def hist_vopt(vol, bins, gpuid=0):
+    """V-optimal histogram of a volume (src/register/joint_histogram.cxx:
+    195-324): (offset, delta, key table) on the GPU."""
+    lib = _abi.load()
+    d = vol.desc()
+    off, dl = ctypes.c_float(), ctypes.c_float()
+    lut = np.zeros(_abi.VOPT_KEYS, np.int32)
+    _abi.check(lib.plm_hip_hist_vopt(ctypes.byref(d), int(bins), ctypes.byref(off), ctypes.byref(dl),
+                                     lut.ctypes.data_as(ctypes.c_void_p), gpuid))
+    return float(off.value), float(dl.value), lut
 
 
 def resample_geometry(vol, kind, rate):
@@ -346,6 +377,11 @@ class BsplineHip:
             raise RuntimeError("library and host disagree on the knot count")
         _abi.check(self.lib.plm_hip_bspline_num_tiles(ctx, ctypes.byref(n)))
         self.num_tiles = n.value
+        if hist is not None and (hist.fixed_keys is not None or hist.moving_keys is not None):
+            keys = [None if k is None else np.ascontiguousarray(k, np.int32)
+                    for k in (hist.fixed_keys, hist.moving_keys)]
+            _abi.check(self.lib.plm_hip_bspline_set_hist_keys(
+                ctx, *[None if k is None else k.ctypes.data_as(ctypes.c_void_p) for k in keys]))
 
     @property
     def fixed_voxels(self):
@@ -564,15 +600,17 @@ def stage_rois(fixed, moving, fixed_roi=None, moving_roi=None, mi_fixed_image_mi
     return fixed_roi, moving_roi
 
 
-def stage_for_pair(pair, gpuid=0, storage="f32", with_grad=True, bins=None, moving_grad=None):
-    """Build Volumes, grid, optional histogram and the device state for a synth.Pair."""
+def stage_for_pair(pair, gpuid=0, storage="f32", with_grad=True, bins=None, moving_grad=None,
+                   hist_type="eqsp"):
+    """Build Volumes, grid, optional histogram (hist_type "eqsp" or "vopt")
+    and the device state for a synth.Pair."""
     fixed = Volume(pair.fixed, pair.spacing, pair.origin, pair.direction_cosines)
     moving = Volume(pair.moving, pair.spacing, pair.origin, pair.direction_cosines)
     bxf = Bspline_xform.for_volume(fixed, pair.vox_per_rgn)
     mg = None
     if with_grad:
         mg = moving_grad if moving_grad is not None else volume_gradient(moving, gpuid)
-    hist = Joint_histogram(bins, bins).initialize(fixed, moving) if bins else None
+    hist = Joint_histogram(bins, bins, hist_type).initialize(fixed, moving) if bins else None
     return BsplineHip(bxf, fixed, moving, mg, hist, gpuid, storage)
 
 

@@ -975,6 +975,49 @@ plm_hip_bspline_set_roi (plm_hip_ctx* c, const plm_img_desc* fixed_roi, const pl
 }
 
 int
+plm_hip_bspline_set_hist_keys (plm_hip_ctx* c, const int32_t* fixed_key_lut, const int32_t* moving_key_lut)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    if (!c->have_hist) return fail (PLM_HIP_ERR_ARG, "the context has no histogram");
+    constexpr int K = PLM_HIP_VOPT_KEYS;
+    const int32_t* luts[2] = {fixed_key_lut, moving_key_lut};
+    const int nbins[2] = {c->h.fbins, c->h.mbins};
+    for (int s = 0; s < 2; s++) {
+        if (!luts[s]) continue;
+        for (int k = 0; k < K; k++)
+            if (luts[s][k] < 0 || luts[s][k] >= nbins[s])
+                return fail (PLM_HIP_ERR_ARG, s ? "moving_key_lut entry out of range" : "fixed_key_lut entry out of range");
+    }
+    if (!fixed_key_lut && !moving_key_lut) return PLM_HIP_OK;
+    HIP_TRY (hipSetDevice (c->device));
+    HIP_TRY (hipStreamSynchronize (c->stream));
+    int* d_lut = nullptr;
+    int rc = alloc (&d_lut, 2 * (size_t) K, "hipMalloc(key tables)");
+    if (rc) return rc;
+    const int64_t nf = (int64_t) c->g.fdim[0] * c->g.fdim[1] * c->g.fdim[2];
+    const int64_t nm = (int64_t) c->g.mdim[0] * c->g.mdim[1] * c->g.mdim[2];
+    const int bf16 = c->storage == PLM_STORE_BF16;
+    hipError_t e = hipSuccess;
+    if (fixed_key_lut) {
+        e = hipMemcpy (d_lut, fixed_key_lut, sizeof (int) * K, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = plmhip::launch_bin_image (c->d_fixed, bf16, c->d_fbin, nf, c->h.f_offset, c->h.f_delta,
+                c->h.fbins, c->stream, d_lut, K);
+    }
+    if (e == hipSuccess && moving_key_lut) {
+        e = hipMemcpy (d_lut + K, moving_key_lut, sizeof (int) * K, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = plmhip::launch_bin_image (c->d_moving, bf16, c->d_mbin, nm, c->h.m_offset, c->h.m_delta,
+                c->h.mbins, c->stream, d_lut + K, K);
+        if (e == hipSuccess) e = plmhip::launch_bin_quads (c->d_mbin, c->d_mquad, c->g.mdim, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize (c->stream);
+    (void) hipFree (d_lut);
+    if (e != hipSuccess) return hip_fail (e, "V-optimal bin images");
+    return PLM_HIP_OK;
+}
+
+int
 plm_hip_bspline_destroy (plm_hip_ctx* c)
 {
     if (!c) return PLM_HIP_OK;

@@ -289,8 +289,11 @@ hipError_t launch_roi_fixed (const KGeom& g, const RoiVol& r, const float* ptab,
 hipError_t launch_pass1_positions (const KGeom& g, float* out, unsigned int* mismatch,
     hipStream_t st);
 
+// Bin image of an image (storage 0 float, 1 BF16): the equal-spacing bin
+// floor((v - offset) / delta) clamped to bins, or with lut (device, keys
+// entries) the bin lut[key] of that key clamped to keys.
 hipError_t launch_bin_image (const void* img, int storage, uint8_t* out, int64_t n,
-    float offset, float delta, int bins, hipStream_t st);
+    float offset, float delta, int bins, hipStream_t st, const int* lut = nullptr, int keys = 0);
 
 hipError_t launch_zero (float* p, int64_t n, hipStream_t st);
 // The moving bin-quad image of a bin image of dim voxels: record v holds the

@@ -2118,15 +2118,18 @@ void k_volume_gradient (const float* __restrict__ img, float* __restrict__ out,
 
 // Bin image: floor((v - offset) / delta) with the IEEE division, as
 // Joint_histogram::add_pvi_8 (src/register/joint_histogram.cxx:458-482)
-// computes it per use; done once per stage.
+// computes it per use; with a V-optimal key table (keys entries), the bin
+// of that key; done once per stage.
 template <typename TI>
 __global__ void
 k_bin_image (const TI* __restrict__ img, uint8_t* __restrict__ out, int64_t n, float offset,
-    float delta, int bins)
+    float delta, int bins, const int* __restrict__ lut, int keys)
 {
     for (int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x * blockDim.x)
-        out[i] = (uint8_t) hist_bin (ld (img, (int) i), offset, delta, bins);
+         i += (int64_t) gridDim.x * blockDim.x) {
+        const float v = ld (img, (int) i);
+        out[i] = (uint8_t) (lut ? lut[hist_bin (v, offset, delta, keys)] : hist_bin (v, offset, delta, bins));
+    }
 }
 
 // The moving image as z-pairs (see launch_zpair); once per stage.  T2 is
@@ -2461,14 +2464,14 @@ launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
 
 hipError_t
 launch_bin_image (const void* img, int storage, uint8_t* out, int64_t n, float offset, float delta,
-    int bins, hipStream_t st)
+    int bins, hipStream_t st, const int* lut, int keys)
 {
     if (storage == 0)
         hipLaunchKernelGGL ((k_bin_image<float>), dim3 (1024), dim3 (256), 0, st,
-            (const float*) img, out, n, offset, delta, bins);
+            (const float*) img, out, n, offset, delta, bins, lut, keys);
     else
         hipLaunchKernelGGL ((k_bin_image<uint16_t>), dim3 (1024), dim3 (256), 0, st,
-            (const uint16_t*) img, out, n, offset, delta, bins);
+            (const uint16_t*) img, out, n, offset, delta, bins, lut, keys);
     return hipGetLastError ();
 }
 

@@ -1,5 +1,6 @@
 // C ABI of the stage preprocessing: plm_hip_resample_geometry,
-// plm_hip_volume_resample, plm_hip_stage_resample, plm_hip_hist_eqsp.
+// plm_hip_volume_resample, plm_hip_stage_resample, plm_hip_hist_eqsp,
+// plm_hip_hist_vopt.
 // Declared in include/plm_hip_bspline.h.
 
 #include <hip/hip_runtime.h>
@@ -121,6 +122,31 @@ device_input (const plm_img_desc* in, Bufs& bufs, const float** src)
     hipError_t e = hipMemcpy (d, in->data, sizeof (float) * n, hipMemcpyHostToDevice);
     if (e != hipSuccess) return plmhip::abi_hip_fail (e, "hipMemcpy(resample input)");
     *src = d;
+    return PLM_HIP_OK;
+}
+
+// Minimum and maximum of a device image (the scan of
+// bspline_initialize_mi_hist_eqsp / _vopt; a NaN fails both comparisons,
+// as there).
+int
+image_range (const float* src, const plm_img_desc* vol, Bufs& bufs, float* mn, float* mx)
+{
+    float* d_part = nullptr;
+    int rc = bufs.get (&d_part, 2 * (size_t) plmhip::MINMAX_BLOCKS, "hipMalloc(min/max partials)");
+    if (rc) return rc;
+    const int64_t n = vol->dim[0] * vol->dim[1] * vol->dim[2];
+    int nb = 0;
+    hipError_t e = plmhip::launch_minmax (src, n, d_part, &nb, nullptr);
+    std::vector<float> part (2 * (size_t) nb);
+    if (e == hipSuccess) e = hipMemcpy (part.data (), d_part, sizeof (float) * part.size (), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return plmhip::abi_hip_fail (e, "min/max scan");
+    float a = part[0], b = part[1];
+    for (int k = 1; k < nb; k++) {
+        a = part[2 * k] < a ? part[2 * k] : a;
+        b = part[2 * k + 1] > b ? part[2 * k + 1] : b;
+    }
+    *mn = a;
+    *mx = b;
     return PLM_HIP_OK;
 }
 
@@ -263,25 +289,96 @@ plm_hip_hist_eqsp (const plm_img_desc* vol, int64_t bins, float* offset, float* 
     if ((rc = select_device (gpuid))) return rc;
     Bufs bufs;
     const float* src = nullptr;
-    float* d_part = nullptr;
-    if ((rc = device_input (vol, bufs, &src))
-        || (rc = bufs.get (&d_part, 2 * (size_t) plmhip::MINMAX_BLOCKS, "hipMalloc(min/max partials)")))
-        return rc;
-    const int64_t n = vol->dim[0] * vol->dim[1] * vol->dim[2];
-    int nb = 0;
-    hipError_t e = plmhip::launch_minmax (src, n, d_part, &nb, nullptr);
-    std::vector<float> part (2 * (size_t) nb);
-    if (e == hipSuccess) e = hipMemcpy (part.data (), d_part, sizeof (float) * part.size (), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return plmhip::abi_hip_fail (e, "min/max scan");
-    float mn = part[0], mx = part[1];
-    for (int b = 1; b < nb; b++) {
-        mn = part[2 * b] < mn ? part[2 * b] : mn;
-        mx = part[2 * b + 1] > mx ? part[2 * b + 1] : mx;
-    }
+    float mn = 0.f, mx = 0.f;
+    if ((rc = device_input (vol, bufs, &src)) || (rc = image_range (src, vol, bufs, &mn, &mx))) return rc;
     // bspline_initialize_mi_hist_eqsp (src/register/joint_histogram.cxx:
     // 131-133): the top and bottom bins are half full
     *delta = (mx - mn) / (float) (bins - 1);
     *offset = (float) (mn - 0.5 * *delta);
+    return PLM_HIP_OK;
+}
+
+// bspline_initialize_mi_hist_vopt (src/register/joint_histogram.cxx:
+// 195-324).  The key sums are the one place where the reference's order of
+// additions could matter (it adds v - offset key by key in voxel order, in
+// double); it does not: every term of key k is a float in
+// [k delta, (k + 1) delta) -- key 0 starts at delta / 2, the image minimum
+// -- so the terms of one key lie within a factor of two of each other and
+// are multiples of the smallest one's ulp, 2^(e - 23); a sum of fewer than
+// 2^28 of them stays below 2^(e + 30) and is itself such a multiple,
+// exact in double's 53 bits.  Every order then gives the exact sum, and the
+// device adds them with atomics.
+int
+plm_hip_hist_vopt (const plm_img_desc* vol, int64_t bins, float* offset, float* delta,
+    int32_t* key_lut, int gpuid)
+{
+    constexpr int K = plmhip::VOPT_KEYS;
+    int rc = plmhip::abi_check_img (vol, "vol");
+    if (rc) return rc;
+    if (!offset || !delta || !key_lut) return plmhip::abi_fail (PLM_HIP_ERR_ARG, "NULL argument");
+    if (bins < 2 || bins > K)
+        return plmhip::abi_fail (PLM_HIP_ERR_ARG, "bins must be in [2, PLM_HIP_VOPT_KEYS]");
+    if (vol->storage != PLM_STORE_F32) return plmhip::abi_fail (PLM_HIP_ERR_ARG, "vol must be float32");
+    const int64_t n = vol->dim[0] * vol->dim[1] * vol->dim[2];
+    if (n >= ((int64_t) 1 << 28))
+        return plmhip::abi_fail (PLM_HIP_ERR_UNSUPPORTED, "image too large for exact key sums (2^28 voxels)");
+    if ((rc = select_device (gpuid))) return rc;
+    Bufs bufs;
+    const float* src = nullptr;
+    float mn = 0.f, mx = 0.f;
+    if ((rc = device_input (vol, bufs, &src)) || (rc = image_range (src, vol, bufs, &mn, &mx))) return rc;
+    if (!(mx > mn)) return plmhip::abi_fail (PLM_HIP_ERR_ARG, "constant image: no histogram range");
+    // the equal-spacing grid of K keys (src/register/joint_histogram.cxx:242-244)
+    const float dl = (mx - mn) / (float) (K - 1);
+    const float off = (float) (mn - 0.5 * dl);
+    unsigned long long* d_cnt = nullptr;
+    double* d_tab = nullptr;     // key sums, then the three prefix tables
+    int* d_split = nullptr;
+    if ((rc = bufs.get (&d_cnt, K, "hipMalloc(key counts)")) || (rc = bufs.get (&d_tab, 3 * K, "hipMalloc(key tables)"))
+        || (rc = bufs.get (&d_split, (size_t) bins * K, "hipMalloc(key splits)")))
+        return rc;
+    std::vector<unsigned long long> cnt1 (K);
+    std::vector<double> sum (K);
+    hipError_t e = hipMemset (d_cnt, 0, sizeof (unsigned long long) * K);
+    if (e == hipSuccess) e = hipMemset (d_tab, 0, sizeof (double) * K);
+    if (e == hipSuccess) e = plmhip::launch_vopt_keys (src, n, off, dl, d_cnt, d_tab, nullptr);
+    if (e == hipSuccess) e = hipMemcpy (cnt1.data (), d_cnt, sizeof (unsigned long long) * K, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy (sum.data (), d_tab, sizeof (double) * K, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return plmhip::abi_hip_fail (e, "VOPT key sums");
+    // key averages (an empty key repeats the previous one's) and their
+    // running sums, in key order (:253-272)
+    std::vector<double> avg (K), tab (3 * K);
+    for (int i = 0; i < K; i++) {
+        if (cnt1[i] > 0) avg[i] = sum[i] / (double) cnt1[i];
+        else avg[i] = i > 0 ? avg[i - 1] : 0.0;
+    }
+    double* ps = tab.data ();
+    double* pssq = ps + K;
+    double* pcnt = pssq + K;
+    ps[0] = avg[0];
+    pssq[0] = avg[0] * avg[0];
+    pcnt[0] = (double) cnt1[0];
+    for (int i = 1; i < K; i++) {
+        ps[i] = ps[i - 1] + avg[i];
+        pssq[i] = pssq[i - 1] + avg[i] * avg[i];
+        pcnt[i] = pcnt[i - 1] + (double) cnt1[i];
+    }
+    std::vector<int> split ((size_t) bins * K);
+    e = hipMemcpy (d_tab, tab.data (), sizeof (double) * 3 * K, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = plmhip::launch_vopt_dp (d_tab, d_tab + K, d_tab + 2 * K, (int) bins, d_split, nullptr);
+    if (e == hipSuccess) e = hipMemcpy (split.data (), d_split, sizeof (int) * split.size (), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return plmhip::abi_hip_fail (e, "VOPT partition");
+    // the key table, walking the best splits back from the last key (:304-321)
+    for (int i = 0; i < K; i++) key_lut[i] = 0;
+    int hi = K - 1;
+    for (int j = (int) bins - 1; j >= 0; j--) {
+        const int nx = j > 0 ? split[(size_t) j * K + hi] : 0;
+        const int lo = j == 0 ? 0 : nx + 1;
+        for (int i = lo; i <= hi; i++) key_lut[i] = j;
+        hi = nx;
+    }
+    *offset = off;
+    *delta = dl;
     return PLM_HIP_OK;
 }
 

@@ -31,6 +31,25 @@ constexpr int MINMAX_BLOCKS = 1024;
 // part[2 b + 1] = max, b < the returned block count.
 hipError_t launch_minmax (const float* img, int64_t n, float* part, int* blocks, hipStream_t st);
 
+// Keys of the V-optimal histogram (the equal-spacing grid of VOPT_KEYS keys).
+constexpr int VOPT_KEYS = 1000;
+
+// Voxel count and sum of (v - offset) per key, key = floor((v - offset) /
+// delta) clamped to the grid, added into count[VOPT_KEYS] and
+// sum[VOPT_KEYS] (zeroed by the caller).  Every partial sum of one key is
+// exact (see stage_abi.cpp), so the order of the additions does not change
+// a bit.
+hipError_t launch_vopt_keys (const float* img, int64_t n, float offset, float delta,
+    unsigned long long* count, double* sum, hipStream_t st);
+
+// The V-optimal grouping of the keys into bins: the dynamic programme of
+// bspline_initialize_mi_hist_vopt over the prefix tables s, ssq, cnt
+// (VOPT_KEYS each); split[j * VOPT_KEYS + i] receives the last key of bin
+// j - 1 in the best grouping of keys 0..i into j + 1 bins (row 0 unused).
+// One workgroup.
+hipError_t launch_vopt_dp (const double* s, const double* ssq, const double* cnt, int bins,
+    int* split, hipStream_t st);
+
 } // namespace plmhip
 
 #endif

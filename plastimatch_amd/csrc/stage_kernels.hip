@@ -7,6 +7,8 @@
 // (src/base/interpolate_macros.h:23-38), so the result is the reference's
 // bit for bit.
 
+#include <float.h>
+
 #include "stage_kernels.h"
 
 namespace plmhip {
@@ -81,6 +83,88 @@ k_minmax (const float* __restrict__ img, int64_t n, float* __restrict__ part)
     }
 }
 
+// Key counts and sums of the V-optimal histogram: per workgroup in LDS
+// (integer counts, double sums), then added to the device totals.
+__global__ void __launch_bounds__ (256)
+k_vopt_keys (const float* __restrict__ img, int64_t n, float offset, float delta,
+    unsigned long long* __restrict__ count, double* __restrict__ sum)
+{
+    __shared__ unsigned int c[VOPT_KEYS];
+    __shared__ double s[VOPT_KEYS];
+    for (int k = threadIdx.x; k < VOPT_KEYS; k += 256) {
+        c[k] = 0u;
+        s[k] = 0.0;
+    }
+    __syncthreads ();
+    const int64_t stride = (int64_t) gridDim.x * 256;
+    for (int64_t e = (int64_t) blockIdx.x * 256 + threadIdx.x; e < n; e += stride) {
+        // (v - offset) and its quotient in float, as the reference bins it
+        const float x = img[e] - offset;
+        int k = (int) floorf (x / delta);
+        k = k < 0 ? 0 : (k >= VOPT_KEYS ? VOPT_KEYS - 1 : k);
+        atomicAdd (&c[k], 1u);
+        atomicAdd (&s[k], (double) x);
+    }
+    __syncthreads ();
+    for (int k = threadIdx.x; k < VOPT_KEYS; k += 256) {
+        if (c[k]) {
+            atomicAdd (&count[k], (unsigned long long) c[k]);
+            atomicAdd (&sum[k], s[k]);
+        }
+    }
+}
+
+// vopt_bin_error (src/register/joint_histogram.cxx:62-82) on the prefix
+// tables: the keys start + 1 .. end as one bin, DBL_MAX below one voxel.
+__device__ __forceinline__ double
+vopt_err (int start, int end, const double* s, const double* ssq, const double* cnt)
+{
+    const double sq = ssq[end] - ssq[start];
+    const double d = s[end] - s[start];
+    const double nk = (double) end - (double) start + 1.0;
+    const double nv = cnt[end] - cnt[start];
+    const double v = sq - (d * d) / nk;
+    return nv < 1.0 ? DBL_MAX : v;
+}
+
+// Thread i owns key i: row j of the programme is, for every i, the least
+// err[j - 1][k] + vopt_err(k + 1, i) over k < i in increasing k, a tie
+// going to the later k (bspline_initialize_mi_hist_vopt,
+// src/register/joint_histogram.cxx:282-301); the rows live in LDS, the
+// argmins go to split.  Same double operations in the same order as the
+// sequential form, so the same bits.
+__global__ void __launch_bounds__ (1024)
+k_vopt_dp (const double* __restrict__ gs, const double* __restrict__ gssq,
+    const double* __restrict__ gcnt, int bins, int* __restrict__ split)
+{
+    __shared__ double s[VOPT_KEYS], ssq[VOPT_KEYS], cnt[VOPT_KEYS], err[2][VOPT_KEYS];
+    const int i = threadIdx.x;
+    if (i < VOPT_KEYS) {
+        s[i] = gs[i];
+        ssq[i] = gssq[i];
+        cnt[i] = gcnt[i];
+    }
+    __syncthreads ();
+    if (i < VOPT_KEYS) err[0][i] = vopt_err (0, i, s, ssq, cnt);
+    for (int j = 1; j < bins; j++) {
+        __syncthreads ();
+        const double* prev = err[(j - 1) & 1];
+        if (i < VOPT_KEYS) {
+            double best = DBL_MAX;
+            int arg = 0;
+            for (int k = 0; k < i; k++) {
+                const double c = prev[k] + vopt_err (k + 1, i, s, ssq, cnt);
+                if (c <= best) {
+                    best = c;
+                    arg = k;
+                }
+            }
+            err[j & 1][i] = best;
+            split[j * VOPT_KEYS + i] = arg;
+        }
+    }
+}
+
 } // namespace
 
 hipError_t
@@ -106,6 +190,26 @@ launch_minmax (const float* img, int64_t n, float* part, int* blocks, hipStream_
     if (b < 1) b = 1;
     *blocks = (int) b;
     hipLaunchKernelGGL (k_minmax, dim3 ((unsigned) b), dim3 (256), 0, st, img, n, part);
+    return hipGetLastError ();
+}
+
+hipError_t
+launch_vopt_keys (const float* img, int64_t n, float offset, float delta, unsigned long long* count,
+    double* sum, hipStream_t st)
+{
+    int64_t b = (n + 255) / 256;
+    if (b > 1024) b = 1024;
+    if (b < 1) b = 1;
+    hipLaunchKernelGGL (k_vopt_keys, dim3 ((unsigned) b), dim3 (256), 0, st, img, n, offset, delta,
+        count, sum);
+    return hipGetLastError ();
+}
+
+hipError_t
+launch_vopt_dp (const double* s, const double* ssq, const double* cnt, int bins, int* split,
+    hipStream_t st)
+{
+    hipLaunchKernelGGL (k_vopt_dp, dim3 (1), dim3 (1024), 0, st, s, ssq, cnt, bins, split);
     return hipGetLastError ();
 }
 

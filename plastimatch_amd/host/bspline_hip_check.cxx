@@ -41,6 +41,7 @@ usage ()
         "  -X, --input-xform file       coefficients, float32 raw, interleaved\n"
         "  -g, --grid-spac v            vox_per_rgn in all three axes (15)\n"
         "      --bins n                 MI histogram bins (32)\n"
+        "      --hist eqsp|vopt         MI histogram type (eqsp)\n"
         "      --fixed-roi f.mha        fixed region of interest (uchar MetaImage)\n"
         "      --moving-roi m.mha       moving region of interest (uchar MetaImage)\n"
         "      --fixed-range lo hi      MI fixed intensity range (mi_fixed_image_minVal/maxVal)\n"
@@ -123,6 +124,7 @@ main (int argc, char** argv)
     }
     std::string metric = "mse", flavor = "g", process = "score", output, xform_fn, froi_fn, mroi_fn;
     float frange[2] = {0.f, 0.f}, mrange[2] = {0.f, 0.f};
+    std::string hist_type = "eqsp";
     float step = 1e-4f, rlo = 0.f, rhi = 0.f;
     long index = 514;
     int line_a = 0, line_b = 30, seed = 0, vpr = 15, bins = 32, gpuid = 0, iters = 10;
@@ -148,6 +150,7 @@ main (int argc, char** argv)
         else if (a == "--bins") bins = atoi (next ());
         else if (a == "--fixed-roi") froi_fn = next ();
         else if (a == "--moving-roi") mroi_fn = next ();
+        else if (a == "--hist") hist_type = next ();
         else if (a == "--fixed-range") { frange[0] = (float) atof (next ()); frange[1] = (float) atof (next ()); }
         else if (a == "--moving-range") { mrange[0] = (float) atof (next ()); mrange[1] = (float) atof (next ()); }
         else if (a == "--bf16") bf16 = true;
@@ -221,8 +224,11 @@ main (int argc, char** argv)
     Joint_histogram* jh = nullptr;
     if (metric == "mi") {
         bst.metric_type = SIMILARITY_METRIC_MI_MATTES;
-        jh = new Joint_histogram (bins, bins);
-        jh->initialize (fixed, moving);
+        jh = new Joint_histogram (hist_type == "vopt" ? HIST_VOPT : HIST_EQSP, bins, bins);
+        if (!jh->initialize (fixed, moving, gpuid)) {
+            printf ("{\"status\": -1, \"error\": \"histogram: %s\"}\n", jh->last_error.c_str ());
+            return 1;
+        }
         bst.mi_hist = jh;
         bxf.jitter_if_zero ();          /* src/register/bspline_optimize.cxx:105-107 */
     } else if (parms.grad_mode == PLM_GRAD_REF_NN) {

@@ -30,6 +30,8 @@ struct Hip_plugin {
     PLM_SYM (plm_hip_volume_gradient)
     PLM_SYM (plm_hip_bspline_num_knots)
     PLM_SYM (plm_hip_bspline_set_roi)
+    PLM_SYM (plm_hip_hist_vopt)
+    PLM_SYM (plm_hip_bspline_set_hist_keys)
 #undef PLM_SYM
 };
 
@@ -92,6 +94,8 @@ hip_plugin_load (std::string* err)
     PLM_LOAD (plm_hip_volume_gradient)
     PLM_LOAD (plm_hip_bspline_num_knots)
     PLM_LOAD (plm_hip_bspline_set_roi)
+    PLM_LOAD (plm_hip_hist_vopt)
+    PLM_LOAD (plm_hip_bspline_set_hist_keys)
 #undef PLM_LOAD
     if (g_plugin.plm_hip_abi_version () != PLM_HIP_ABI_VERSION) {
         if (err) *err = "libplmregisterhip.so ABI version mismatch";
@@ -471,7 +475,12 @@ Bspline_xform::geom () const
 }
 
 Joint_histogram::Joint_histogram (plm_long fb, plm_long mb)
-    : moving (mb), fixed (fb), f_hist (fb, 0.0), m_hist (mb, 0.0), j_hist (fb * mb, 0.0)
+    : Joint_histogram (HIST_EQSP, fb, mb)
+{
+}
+
+Joint_histogram::Joint_histogram (Mi_hist_type type, plm_long fb, plm_long mb)
+    : moving (type, mb), fixed (type, fb), f_hist (fb, 0.0), m_hist (mb, 0.0), j_hist (fb * mb, 0.0)
 {
 }
 
@@ -489,11 +498,34 @@ hist_eqsp (Histogram* h, const Volume* v)
     h->offset = (float) (mn - 0.5 * h->delta);
 }
 
-void
-Joint_histogram::initialize (const Volume* f, const Volume* m)
+/* bspline_initialize_mi_hist (src/register/joint_histogram.cxx:327-346):
+   more V-optimal bins than keys fall back to equal spacing. */
+bool
+Joint_histogram::initialize (const Volume* f, const Volume* m, int gpuid)
 {
-    hist_eqsp (&fixed, f);
-    hist_eqsp (&moving, m);
+    Histogram* hs[2] = {&fixed, &moving};
+    const Volume* vs[2] = {f, m};
+    for (int k = 0; k < 2; k++) {
+        Histogram* h = hs[k];
+        if (h->type == HIST_VOPT && h->bins > PLM_HIP_VOPT_KEYS) {
+            printf ("WARNING: Falling back to EQSP histograms.\n"
+                    "         (Reason: # bins > %i)\n", PLM_HIP_VOPT_KEYS);
+            h->type = HIST_EQSP;
+        }
+        if (h->type == HIST_EQSP) {
+            hist_eqsp (h, vs[k]);
+            h->key_lut.clear ();
+            continue;
+        }
+        if (!hip_plugin_load (&last_error)) return false;
+        const plm_img_desc d = vs[k]->desc (PLM_STORE_F32);
+        h->key_lut.assign (PLM_HIP_VOPT_KEYS, 0);
+        if (g_plugin.plm_hip_hist_vopt (&d, h->bins, &h->offset, &h->delta, h->key_lut.data (), gpuid) < 0) {
+            last_error = hip_error ();
+            return false;
+        }
+    }
+    return true;
 }
 
 plm_hist_desc
@@ -601,6 +633,16 @@ bspline_state_initialize_hip (Bspline_state* bst, Bspline_xform* bxf, Bspline_pa
         return rc;
     }
     bst->dev_ptrs = ctx;
+    if (bst->mi_hist && (!bst->mi_hist->fixed.key_lut.empty () || !bst->mi_hist->moving.key_lut.empty ())) {
+        const Joint_histogram* jh = bst->mi_hist;
+        rc = g_plugin.plm_hip_bspline_set_hist_keys (ctx,
+            jh->fixed.key_lut.empty () ? nullptr : jh->fixed.key_lut.data (),
+            jh->moving.key_lut.empty () ? nullptr : jh->moving.key_lut.data ());
+        if (rc < 0) {
+            bst->last_error = hip_error ();
+            return rc;
+        }
+    }
     if (bst->fixed_roi || bst->moving_roi) {
         plm_img_desc fr, mr;
         if (bst->fixed_roi) fr = bst->fixed_roi->desc ();

@@ -119,12 +119,18 @@ bool write_mha (const std::string& filename, const Volume& vol, std::string* err
    file the reference would also refuse. */
 Bspline_xform* bspline_xform_load (const char* filename, std::string* err);
 
+/* Mi_hist_type (src/register/histogram.h:14-17) */
+enum Mi_hist_type { HIST_EQSP, HIST_VOPT };
+
 class Histogram {
 public:
+    Mi_hist_type type;
     plm_long bins;
     float offset;
     float delta;
-    Histogram (plm_long bins = 32) : bins (bins), offset (0.f), delta (0.f) {}
+    std::vector<int32_t> key_lut;       /* HIST_VOPT: bin of each key */
+    Histogram (Mi_hist_type type = HIST_EQSP, plm_long bins = 32)
+        : type (type), bins (bins), offset (0.f), delta (0.f) {}
 };
 
 class Joint_histogram {
@@ -134,9 +140,14 @@ public:
     std::vector<double> f_hist;
     std::vector<double> m_hist;
     std::vector<double> j_hist;
+    std::string last_error;
 public:
     Joint_histogram (plm_long fixed_bins, plm_long moving_bins);
-    void initialize (const Volume* fixed, const Volume* moving);
+    Joint_histogram (Mi_hist_type type, plm_long fixed_bins, plm_long moving_bins);
+    /* Equal spacing on the host; V-optimal bins through the plugin
+       (plm_hip_hist_vopt, device gpuid).  Returns false with last_error
+       set when the plugin cannot build them. */
+    bool initialize (const Volume* fixed, const Volume* moving, int gpuid = 0);
     plm_hist_desc desc () const;
 };
 

@@ -139,10 +139,15 @@ def mse(fixed, moving, spacing, vpr, coeff, grad_mode, m_grad=None, fmask=None, 
 
 
 def mi(fixed, moving, spacing, vpr, coeff, bins, f_off, f_delta, m_off, m_delta, fmask=None,
-       mmask=None):
+       mmask=None, f_keys=None, m_keys=None):
     """FP64 PV-8 Mattes MI: score -MI, gradient (thresholds 0.001/B^2 for the
     score, 1e-4 for the gradient) and N; identity direction cosines.
-    fmask / mmask as for mse."""
+    fmask / mmask as for mse.  f_keys / m_keys: V-optimal key tables (the
+    bin of key floor((v - offset) / delta)), or None for equal spacing."""
+    def to_bin(v, off, dl, keys):
+        b = int(math.floor((float(v) - off) / dl))
+        return b if keys is None else int(keys[min(max(b, 0), len(keys) - 1)])
+
     nz, ny, nx = fixed.shape
     dim = (nx, ny, nz)
     _, cd = grid(dim, vpr)
@@ -167,8 +172,8 @@ def mi(fixed, moving, spacing, vpr, coeff, bins, f_off, f_delta, m_off, m_delta,
                 f, r, l2 = smp
                 a8 = _corners(moving, f)
                 w8, l1 = _weights(l2)
-                fb = int(math.floor((float(fixed[k, j, i]) - f_off) / f_delta))
-                mbs = [int(math.floor((float(v) - m_off) / m_delta)) for v in a8]
+                fb = to_bin(fixed[k, j, i], f_off, f_delta, f_keys)
+                mbs = [to_bin(v, m_off, m_delta, m_keys) for v in a8]
                 fh[fb] += 1
                 for nn in range(8):
                     mh[mbs[nn]] += w8[nn]

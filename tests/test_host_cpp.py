@@ -147,3 +147,25 @@ def test_host_mi_rois_and_ranges(tmp_path):
     assert out["num_vox"] == o["num_vox"] and o["num_vox"] < 0.6 * pair.fixed.size
     assert rel_scalar(out["score"], o["score"]) < 1e-5
     assert rel(np.fromfile(gfile, np.float64), o["grad"]) < 1e-4
+
+
+@pytest.mark.gpu
+def test_host_mi_vopt(tmp_path):
+    """histogram_type=vopt through the C++ host: the V-optimal key tables come
+    from the plugin (plm_hip_hist_vopt) and reach the context
+    (plm_hip_bspline_set_hist_keys); MI against the oracle on the oracle's
+    own V-optimal bins."""
+    pair = small_pair(dim=(40, 36, 28), spacing=(1.0, 1.0, 2.5), vpr=(5, 5, 5), seed=36)
+    pair.moving = synth.ct_remap(pair.moving.astype(np.float64)).astype(np.float32)
+    gfile = os.path.join(str(tmp_path), "grad.raw")
+    rc, out, err = run_tool(["-g", "5", "-M", "mi", "--bins", "16", "--hist", "vopt", "-O", gfile],
+                            str(tmp_path), pair)
+    assert rc == 0, (out, err)
+    oc = oracle_case(pair)
+    o = oc.mi(np.full(oc.num_coeff, 0.01, np.float32), oc.hist(16, vopt=True))
+    e = oc.mi(np.full(oc.num_coeff, 0.01, np.float32), oc.hist(16))
+    assert out["num_vox"] == o["num_vox"]
+    assert rel_scalar(out["score"], o["score"]) < 1e-5
+    assert rel(np.fromfile(gfile, np.float64), o["grad"]) < 1e-4
+    # and not the equal-spacing result
+    assert rel_scalar(out["score"], e["score"]) > 1e-3
