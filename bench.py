@@ -24,8 +24,10 @@ finite-difference check of the adjoint.  "warp_regularize" times the
 warped moving image and the analytic regularizer on the headline workload,
 with their rooflines and the oracle's versions timed beside them.
 "optimizer_loop" runs the device L-BFGS (liblbfgs's algorithm) for 20
-iterations on the headline workload, with and without regularization, and
-the reference's liblbfgs on the oracle's score for 3 iterations beside it.
+iterations on the headline workload, with and without regularization
+(the device-resident loop), both loops' cost per evaluation (resident and
+host-driven), and the reference's liblbfgs on the oracle's score for 3
+iterations beside it.
 "stage_preprocess" times a stage's preprocessing on the device (both
 images subsampled, moving gradient, histogram ranges) against the oracle.
 "mse_128cube" evaluates the 128^3 pair (15-voxel grid) at zero and at the
@@ -455,9 +457,12 @@ def optimizer_leg(args, dev, eval_ms, want_cpu):
     st.autotune("mse", "ref_nn", None, reps=5)
     d_x = torch.zeros(bxf.num_coeff, dtype=torch.float64, device=dev)
     d_c = torch.zeros(bxf.num_coeff, dtype=torch.float32, device=dev)
-    st.optimize_lbfgs(d_x, d_c, max_its=2, max_feval=-1)          # warm-up
+    st.optimize_lbfgs(d_x, d_c, max_its=2, max_feval=-1)                  # warm-ups
+    st.optimize_lbfgs(d_x, d_c, max_its=2, max_feval=-1, resident=False)
     out = {"workload": WORKLOAD, "optimizer": "L-BFGS, More-Thuente (liblbfgs 1.9 defaults)",
-           "start": "identity (zero coefficients)"}
+           "start": "identity (zero coefficients)",
+           "loop": "resident: scalar logic on the device, evaluation chains replayed from a hipGraph "
+                   "(4 chains a replay); host_loop: the same logic on the host between evaluations"}
     runs = {}
     for label, reg in (("mse", None), ("mse_plus_regularization", OPT_REG_WEIGHTS)):
         d_x.zero_()
@@ -473,9 +478,29 @@ def optimizer_leg(args, dev, eval_ms, want_cpu):
                       "ms_per_iteration": dt * 1e3 / max(r.iterations, 1),
                       "ms_per_evaluation_in_loop": dt * 1e3 / r.fevals,
                       "fx_end": r.fx}
+    # the host-driven loop on the same run, and both loops' cost per
+    # evaluation from the difference of a long and a short run (the fixed
+    # cost of a call cancels)
+    per_eval = {}
+    for resident in (True, False):
+        tt = []
+        for its in (OPT_ITS, 3 * OPT_ITS):
+            d_x.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = st.optimize_lbfgs(d_x, d_c, "mse", "ref_nn", None, record=False, resident=resident,
+                                  max_its=its, max_feval=-1)
+            tt.append((time.perf_counter() - t0, r.fevals))
+        per_eval["resident" if resident else "host_loop"] = {
+            "ms_per_evaluation_in_loop": tt[0][0] * 1e3 / tt[0][1],
+            "marginal_ms_per_evaluation": (tt[1][0] - tt[0][0]) * 1e3 / (tt[1][1] - tt[0][1]),
+            "fevals": [tt[0][1], tt[1][1]]}
+    out["mse_loops"] = per_eval
     out["mse_plus_regularization"]["weights"] = OPT_REG_WEIGHTS
     out["mse"]["bare_evaluation_ms"] = eval_ms
     out["mse"]["loop_overhead_ms_per_evaluation"] = out["mse"]["ms_per_evaluation_in_loop"] - eval_ms
+    for v in per_eval.values():
+        v["marginal_overhead_ms_per_evaluation"] = v["marginal_ms_per_evaluation"] - eval_ms
     st.close()
     if want_cpu:
         import oracle

@@ -521,6 +521,33 @@ int plm_hip_bspline_optimize_lbfgs (plm_hip_ctx* ctx, int metric, int grad_mode,
     const plm_reg_parms* reg, const plm_lbfgs_parms* parms, double* d_x, float* d_coeff,
     plm_lbfgs_progress_fn progress, void* instance, plm_lbfgs_result* result);
 
+/* One progress report of a run: the arguments liblbfgs's progress
+ * callback receives (fx, xnorm, gnorm, step, k, ls). */
+typedef struct plm_lbfgs_report {
+    double fx;
+    double xnorm;
+    double gnorm;
+    double step;
+    int k;
+    int ls;
+} plm_lbfgs_report;
+
+/* The same run with the whole loop on the device: after every evaluation
+ * one GPU thread advances the line search and the direction solve (the
+ * same double arithmetic as the host loop, so the same run bit for bit),
+ * and the evaluation chain -- trial point, score, regularization, post
+ * pass, that step -- reads everything it needs from device memory, so it
+ * is captured once in a hipGraph and replayed back to back with no host
+ * work between evaluations; the reference's max_its / max_feval rule
+ * stops it.  plm_hip_bspline_optimize_lbfgs takes this path when progress
+ * is NULL.  No counterpart in the reference, whose liblbfgs calls back into
+ * the host every evaluation (bspline_optimize_liblbfgs.cxx:83-109).
+ * reports (may be NULL) receives the first max_reports progress reports in
+ * order; n_reports (may be NULL) the number made. */
+int plm_hip_bspline_optimize_lbfgs_resident (plm_hip_ctx* ctx, int metric, int grad_mode,
+    const plm_reg_parms* reg, const plm_lbfgs_parms* parms, double* d_x, float* d_coeff,
+    plm_lbfgs_report* reports, int max_reports, int* n_reports, plm_lbfgs_result* result);
+
 #ifdef __cplusplus
 }
 #endif

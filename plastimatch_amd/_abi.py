@@ -87,6 +87,11 @@ class PlmLbfgsResult(ctypes.Structure):
 
 
 # progress callback: (instance, d_x, d_g, fx, xnorm, gnorm, step, n, k, ls) -> int
+class PlmLbfgsReport(ctypes.Structure):
+    _fields_ = [("fx", ctypes.c_double), ("xnorm", ctypes.c_double), ("gnorm", ctypes.c_double),
+                ("step", ctypes.c_double), ("k", ctypes.c_int), ("ls", ctypes.c_int)]
+
+
 PLM_LBFGS_PROGRESS = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_int)
@@ -176,6 +181,11 @@ SIGNATURES = {
     "plm_hip_bspline_optimize_lbfgs": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(PlmRegParms),
                                                       ctypes.POINTER(PlmLbfgsParms), _P, _P, PLM_LBFGS_PROGRESS,
                                                       _P, ctypes.POINTER(PlmLbfgsResult)]),
+    "plm_hip_bspline_optimize_lbfgs_resident": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int,
+                                                               ctypes.POINTER(PlmRegParms),
+                                                               ctypes.POINTER(PlmLbfgsParms), _P, _P, _P,
+                                                               ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                               ctypes.POINTER(PlmLbfgsResult)]),
 }
 
 _lib = None

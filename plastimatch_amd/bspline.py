@@ -466,18 +466,21 @@ class BsplineHip:
             float(default_val), 1 if linear else 0, st))
 
     def optimize_lbfgs(self, d_x, d_coeff=None, metric="mse", grad_mode="ref_nn", reg=None, progress=None,
-                       record=True, **parms):
-        """L-BFGS on the device (plm_hip_bspline_optimize_lbfgs): minimize the
-        score (+ the analytic regularization with weights ``reg``, a dict of
-        Regularization_parms names) from the float64 device tensor d_x, which
-        holds the x liblbfgs returns at exit; d_coeff (float32 device tensor,
-        optional) gets the coefficients of the last evaluation.  ``parms``:
-        plm_lbfgs_parms fields (liblbfgs names, max_its, max_feval; unset ones
-        take liblbfgs's defaults and max_its = max_feval = 10).  ``progress``:
-        callable (d_x_ptr, d_g_ptr, fx, xnorm, gnorm, step, n, k, ls) -> int,
-        nonzero stops.  ``record``: keep every progress report in the result
-        (with neither, no callback crosses into Python).  Returns an
-        LbfgsResult."""
+                       record=True, resident=None, max_reports=4096, **parms):
+        """L-BFGS on the device: minimize the score (+ the analytic
+        regularization with weights ``reg``, a dict of Regularization_parms
+        names) from the float64 device tensor d_x, which holds the x liblbfgs
+        returns at exit; d_coeff (float32 device tensor, optional) gets the
+        coefficients of the last evaluation.  ``parms``: plm_lbfgs_parms
+        fields (liblbfgs names, max_its, max_feval; unset ones take liblbfgs's
+        defaults and max_its = max_feval = 10).  ``progress``: callable
+        (d_x_ptr, d_g_ptr, fx, xnorm, gnorm, step, n, k, ls) -> int, nonzero
+        stops; it needs the host loop (plm_hip_bspline_optimize_lbfgs with a
+        callback).  ``resident`` (default: no progress callable): the whole
+        loop on the device, the evaluation chain replayed from a hipGraph
+        (plm_hip_bspline_optimize_lbfgs_resident), its reports read back
+        afterwards (the first ``max_reports``).  ``record``: keep every
+        progress report in the result.  Returns an LbfgsResult."""
         if d_x.numel() != self.bxf.num_coeff:
             raise ValueError("d_x has the wrong length")
         p = _abi.PlmLbfgsParms()
@@ -495,6 +498,22 @@ class BsplineHip:
             w.update(reg)
             rp = _abi.PlmRegParms(**w)
         reports = []
+        if resident is None:
+            resident = progress is None
+        if resident:
+            if progress is not None:
+                raise ValueError("a progress callable needs the host loop (resident=False)")
+            cap = int(max_reports) if record else 0
+            buf = (_abi.PlmLbfgsReport * max(cap, 1))()
+            nrep = ctypes.c_int()
+            res = _abi.PlmLbfgsResult()
+            _abi.check(self.lib.plm_hip_bspline_optimize_lbfgs_resident(
+                self.ctx, METRICS[metric], GRAD_MODES[grad_mode], ctypes.byref(rp) if rp is not None else None,
+                ctypes.byref(p), ctypes.c_void_p(d_x.data_ptr()),
+                ctypes.c_void_p(d_coeff.data_ptr()) if d_coeff is not None else None,
+                ctypes.cast(buf, ctypes.c_void_p) if cap else None, cap, ctypes.byref(nrep), ctypes.byref(res)))
+            reports = [(r.fx, r.xnorm, r.gnorm, r.step, r.k, r.ls) for r in buf[:min(cap, nrep.value)]]
+            return LbfgsResult(res.status, res.iterations, res.fevals, res.fx, reports)
 
         def _progress(inst, px, pg, fx, xnorm, gnorm, step, n, k, ls):
             reports.append((fx, xnorm, gnorm, step, k, ls))

@@ -215,6 +215,7 @@ struct plm_hip_ctx {
     int64_t* d_hist_out = nullptr;
 
     StageTimer timer;       // plm_hip_bspline_set_timing
+    plmhip::LbWorkspace* lb_ws = nullptr;   // the optimizer's buffers (first run)
 };
 
 // A batch of contexts evaluated together (plm_hip_batch_*).
@@ -242,6 +243,7 @@ void
 free_ctx (plm_hip_ctx* c)
 {
     c->timer.destroy ();
+    plmhip::lb_workspace_free (c->lb_ws);
     void* bufs[] = {c->d_fixed, c->d_moving, c->d_mpair, c->d_mgrad, c->d_blut, c->d_tbase, c->d_cond,
         c->d_tile_score, c->d_tile_count, c->d_pair, c->d_hist,
         c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_mquad, c->d_ptab, c->d_coeff, c->d_grad,
@@ -667,16 +669,19 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
     return PLM_HIP_OK;
 }
 
+} // extern "C"
+
+namespace {
+
+// The checks and launches of one evaluation (ev: stage timing events or
+// null; skip: see ctx_eval).
 int
-plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
-    const float* d_coeff, double* d_score, double* d_grad, int64_t* d_num_vox,
-    int64_t* d_hist, void* stream)
+eval_launch (plm_hip_ctx* c, int metric, int grad_mode, const float* d_coeff, double* d_score,
+    double* d_grad, int64_t* d_num_vox, int64_t* d_hist, hipStream_t st, hipEvent_t* ev,
+    const int* skip)
 {
-    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
     if (!d_coeff || !d_score || !d_num_vox || (!d_grad && grad_mode != PLM_GRAD_NONE))
         return fail (PLM_HIP_ERR_ARG, "NULL output or coefficient pointer");
-    hipStream_t st = stream ? (hipStream_t) stream : c->stream;
-    HIP_TRY (hipSetDevice (c->device));
     if (metric != PLM_METRIC_MSE && metric != PLM_METRIC_MI)
         return fail (PLM_HIP_ERR_ARG, "unknown metric");
     if (metric == PLM_METRIC_MSE) {
@@ -689,23 +694,55 @@ plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
         if (!c->have_hist) return fail (PLM_HIP_ERR_ARG, "MI needs a histogram description");
         if (!c->mi_ok) return fail (PLM_HIP_ERR_UNSUPPORTED, "MI flavor g does not take a ROI");
     }
-    hipEvent_t* ev = nullptr;
-    HIP_TRY (c->timer.next (&ev));
     hipError_t e;
+    plmhip::EvalOut out {d_grad, d_score, d_num_vox};
+    out.skip = skip;
     if (metric == PLM_METRIC_MSE) {
-        const plmhip::EvalOut out {d_grad, d_score, d_num_vox};
         const int mode = grad_mode == PLM_GRAD_REF_NN ? plmhip::MSE_REF_NN
             : grad_mode == PLM_GRAD_EXACT ? plmhip::MSE_EXACT : plmhip::MSE_SCORE_ONLY;
         e = plmhip::launch_mse (c->plan, c->tune, mode, c->storage == PLM_STORE_BF16,
             d_coeff, out, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MSE launch");
     } else {
-        const plmhip::EvalOut out {d_grad, d_score, d_num_vox};
         e = plmhip::launch_mi (c->plan, c->tune, c->h, c->hist_blocks, c->d_fbin, c->d_mquad,
             d_coeff, c->d_hist, c->d_hist + (size_t) c->h.fbins * c->h.mbins, c->d_table,
             c->d_nvox_f, out, d_hist, c->d_ptab, c->d_ptab ? c->d_fmask1 : c->d_fmask, c->mroi, st, ev);
         if (e != hipSuccess) return hip_fail (e, "MI launch");
     }
+    return PLM_HIP_OK;
+}
+
+} // namespace
+
+namespace plmhip {
+
+int
+ctx_eval (plm_hip_ctx* c, int metric, int grad_mode, const float* d_coeff, double* d_score,
+    double* d_grad, int64_t* d_num_vox, hipStream_t st, const int* skip)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    HIP_TRY (hipSetDevice (c->device));
+    return eval_launch (c, metric, grad_mode, d_coeff, d_score, d_grad, d_num_vox, nullptr, st, nullptr,
+        skip);
+}
+
+} // namespace plmhip
+
+extern "C" {
+
+int
+plm_hip_bspline_score_device (plm_hip_ctx* c, int metric, int grad_mode,
+    const float* d_coeff, double* d_score, double* d_grad, int64_t* d_num_vox,
+    int64_t* d_hist, void* stream)
+{
+    if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
+    hipStream_t st = stream ? (hipStream_t) stream : c->stream;
+    HIP_TRY (hipSetDevice (c->device));
+    hipEvent_t* ev = nullptr;
+    HIP_TRY (c->timer.next (&ev));
+    const int rc = eval_launch (c, metric, grad_mode, d_coeff, d_score, d_grad, d_num_vox, d_hist, st, ev,
+        nullptr);
+    if (rc) return rc;
     if (ev) c->timer.last_stages = metric == PLM_METRIC_MSE ? plmhip::MSE_STAGES : plmhip::MI_STAGES;
     return PLM_HIP_OK;
 }
@@ -1168,20 +1205,50 @@ plm_hip_bspline_regularize (plm_hip_ctx* c, const plm_reg_parms* parms, const fl
     if (accumulate != 0 && accumulate != 1) return fail (PLM_HIP_ERR_ARG, "accumulate must be 0 or 1");
     HIP_TRY (hipSetDevice (c->device));
     hipStream_t st = stream ? (hipStream_t) stream : c->stream;
-    if (!c->d_reg_part) {
-        int rc = alloc (&c->d_reg_part, (size_t) c->num_tiles, "hipMalloc(reg partials)");
-        if (rc) return rc;
-        rc = alloc (&c->d_reg_cond, (size_t) 192 * c->num_knots, "hipMalloc(reg slots)");
-        if (rc) return rc;
-        // slots no region owns (knots near the grid's edge) stay zero for good
-        HIP_TRY (hipMemset (c->d_reg_cond, 0, sizeof (double) * 192 * (size_t) c->num_knots));
-    }
+    const int rc = plmhip::ctx_reg_prepare (c);
+    if (rc) return rc;
     const plmhip::RegOp op = reg_operator (c->grid_spac, *parms);
     hipError_t e = plmhip::launch_regularize (c->g, op, d_coeff, c->d_reg_part, c->d_reg_cond, d_score,
         d_grad, accumulate, st);
     if (e != hipSuccess) return hip_fail (e, "regularization launch");
     return PLM_HIP_OK;
 }
+
+} // extern "C"
+
+namespace plmhip {
+
+int
+ctx_reg_prepare (plm_hip_ctx* c)
+{
+    if (c->d_reg_part) return PLM_HIP_OK;
+    HIP_TRY (hipSetDevice (c->device));
+    int rc = alloc (&c->d_reg_part, (size_t) c->num_tiles, "hipMalloc(reg partials)");
+    if (rc) return rc;
+    rc = alloc (&c->d_reg_cond, (size_t) 192 * c->num_knots, "hipMalloc(reg slots)");
+    if (rc) return rc;
+    // slots no region owns (knots near the grid's edge) stay zero for good
+    HIP_TRY (hipMemset (c->d_reg_cond, 0, sizeof (double) * 192 * (size_t) c->num_knots));
+    return PLM_HIP_OK;
+}
+
+int
+ctx_regularize (plm_hip_ctx* c, const plm_reg_parms* parms, const float* d_coeff, double* d_score,
+    double* d_grad, hipStream_t st, const int* skip)
+{
+    HIP_TRY (hipSetDevice (c->device));
+    const int rc = ctx_reg_prepare (c);
+    if (rc) return rc;
+    const RegOp op = reg_operator (c->grid_spac, *parms);
+    hipError_t e = launch_regularize (c->g, op, d_coeff, c->d_reg_part, c->d_reg_cond, d_score, d_grad, 1,
+        st, skip);
+    if (e != hipSuccess) return hip_fail (e, "regularization launch");
+    return PLM_HIP_OK;
+}
+
+} // namespace plmhip
+
+extern "C" {
 
 int
 plm_hip_bspline_warp (plm_hip_ctx* c, const float* d_coeff, float* d_out, float default_val,
@@ -1255,6 +1322,12 @@ int
 abi_check_img (const plm_img_desc* d, const char* name)
 {
     return check_img (d, name);
+}
+
+LbWorkspace**
+ctx_lb_workspace (plm_hip_ctx* c)
+{
+    return &c->lb_ws;
 }
 
 int

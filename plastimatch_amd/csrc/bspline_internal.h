@@ -19,6 +19,23 @@ int abi_hip_fail (hipError_t e, const char* what);
 int abi_check_img (const plm_img_desc* d, const char* name);
 // A context's device, stream and knot count.
 int ctx_info (const plm_hip_ctx* c, int* device, hipStream_t* stream, int64_t* num_knots);
+// plm_hip_bspline_score_device and plm_hip_bspline_regularize (accumulate
+// 1) on stream st, without stage timing, every kernel returning at once
+// when *skip (a device int, may be null) is nonzero at its launch: the
+// evaluation steps of an optimizer chain that may be replayed after its
+// run ended.  ctx_reg_prepare allocates the regularizer's scratch ahead of
+// a graph capture.
+int ctx_eval (plm_hip_ctx* c, int metric, int grad_mode, const float* d_coeff, double* d_score,
+    double* d_grad, int64_t* d_num_vox, hipStream_t st, const int* skip);
+int ctx_reg_prepare (plm_hip_ctx* c);
+
+// The optimizer's device workspace, kept by a context between runs (freed
+// with it): the slot, and how the context frees what it holds.
+struct LbWorkspace;
+LbWorkspace** ctx_lb_workspace (plm_hip_ctx* c);
+void lb_workspace_free (LbWorkspace* w);
+int ctx_regularize (plm_hip_ctx* c, const plm_reg_parms* parms, const float* d_coeff, double* d_score,
+    double* d_grad, hipStream_t st, const int* skip);
 
 struct KGeom;
 // The tile-kernel geometry of a B-spline grid over a fixed/moving pair

@@ -197,6 +197,10 @@ struct EvalOut {
     double* grad;
     double* score;
     int64_t* nvox;
+    // when not null and nonzero at launch time, every kernel of the
+    // evaluation returns at once (an optimizer chain enqueued after its run
+    // ended)
+    const int* skip = nullptr;
 };
 
 // Everything a launch needs about its set of pairs.
@@ -268,7 +272,8 @@ struct RegOp {
 // part (num_tiles doubles) and cond (64 * 3 * num_knots doubles) are
 // scratch; score = the score, grad = (accumulate ? grad : 0) + gradient.
 hipError_t launch_regularize (const KGeom& g, const RegOp& op, const float* coeff, double* part,
-    double* cond, double* score, double* grad, int accumulate, hipStream_t st);
+    double* cond, double* score, double* grad, int accumulate, hipStream_t st,
+    const int* skip = nullptr);
 // Mattes MI of a single-pair plan.  ptab: the fixed positions of pass 1's
 // walk (launch_pass1_positions), or null where they equal the direct ones.
 // fmask1: the fixed ROI at pass 1's positions (null: none); mroi: the

@@ -808,8 +808,10 @@ template <int KINDX, typename TI, int V, int G, bool BATCH>
 __global__ __launch_bounds__ (TILE_THREADS)
 void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __restrict__ coeff0,
     const float* __restrict__ blut, double* __restrict__ part_score,
-    int* __restrict__ part_count, MiArgs mi, float* __restrict__ vf, float fill)
+    int* __restrict__ part_count, MiArgs mi, float* __restrict__ vf, float fill,
+    const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     // what this kind computes
     constexpr int KIND = KINDX & 15;
     constexpr bool PLAIN = (KINDX & TILE_PLAIN) != 0;
@@ -1444,6 +1446,7 @@ __global__ __launch_bounds__ (REDUCE_THREADS)
 void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __restrict__ part_score,
     const int* __restrict__ part_count, int nparts, EvalOut out0, int with_grad)
 {
+    if (out0.skip && *out0.skip) return;
     __shared__ long long s_n[REDUCE_THREADS / 64];
     __shared__ double s_s[REDUCE_THREADS / 64];
     __shared__ double s_q[4][REDUCE_BLOCK];
@@ -1557,40 +1560,42 @@ exact_displacement (float out[3], const float* bx, const float* by, const float*
     out[0] = o0; out[1] = o1; out[2] = o2;
 }
 
-// floor(w * 2^32 + 0.5) for a weight 0 <= w <= 1, in integer arithmetic:
-// w = m * 2^(e - 150) with the 24-bit significand m and biased exponent e,
-// so w * 2^32 = m * 2^(e - 118): a left shift when e >= 118, else a
-// right shift by k = 118 - e rounding half up, (m + 2^(k-1)) >> k, which
-// is 0 once k > 24.  The double expression is exact (w * 2^32 + 0.5 needs
-// at most 49 significant bits), so the two agree for every such float.
-__device__ __forceinline__ long long
-pv_fixed (float w)
-{
-    const uint32_t bits = __float_as_uint (w);
-    const int e = (int) ((bits >> 23) & 0xffu);
-    const uint64_t m = (uint64_t) ((bits & 0x7fffffu) | 0x800000u);
-    const int k = 118 - e;
-    if (e == 0 || k > 24) return 0;        // zero, denormal, or below 2^-33
-    if (k <= 0) return (long long) (m << (-k));
-    return (long long) ((m + (1ull << (k - 1))) >> k);
-}
-
 // Fixed-point PV weights (units of 2^-32): q = floor(w*2^32 + 0.5), the
 // remainder to the first largest weight, so each voxel adds exactly 2^32.
+// For a weight 0 <= w < 1, x = w * 2^32 is exact in float and below 2^32,
+// floor(x) and x - floor(x) are exact, so q = floor(x) + (frac >= 1/2) is
+// the double expression's value and fits 32 bits; w = 1 (a corner on a
+// voxel centre: the other seven weights are 0) is 2^32.  The eight q then
+// sum to 2^32 - r with |r| <= 4, so the 32-bit sum, negated, is r.
 __device__ __forceinline__ void
 pv_quantize (const float w[8], long long q[8])
 {
-    long long sum = 0;
+    uint32_t q32[8];
+    uint32_t sum = 0u;
     int amax = 0;
+    float wmax = w[0];
+    bool one = false;
 #pragma unroll
     for (int n = 0; n < 8; n++) {
-        q[n] = pv_fixed (w[n]);
-        sum += q[n];
-        if (w[n] > w[amax]) amax = n;
+        const float x = w[n] * 4294967296.0f;
+        const float fl = floorf (x);
+        // (the clamp keeps w = 1's conversion in range; its q is set below)
+        const uint32_t v = (uint32_t) fminf (fl, 4294967040.0f) + (x - fl >= 0.5f ? 1u : 0u);
+        q32[n] = w[n] == 1.0f ? 0u : v;
+        one = one || w[n] == 1.0f;
+        sum += q32[n];
+        if (n > 0 && w[n] > wmax) {
+            wmax = w[n];
+            amax = n;
+        }
     }
-    const long long r = (1ll << 32) - sum;
+    const int r = (int) (0u - sum);
 #pragma unroll
-    for (int n = 0; n < 8; n++) if (n == amax) q[n] += r;
+    for (int n = 0; n < 8; n++) {
+        long long v = (long long) q32[n];
+        if (n == amax) v = one ? (1ll << 32) : v + r;
+        q[n] = v;
+    }
 }
 
 // Pass 1: PV-8 joint histogram.  Follows the pass-1 loop of
@@ -1600,38 +1605,11 @@ pv_quantize (const float w[8], long long q[8])
 // private int64 histogram in LDS, flushed once with integer atomics.  Bins
 // come from the stage's bin images.  Because integer addition is
 // associative, contributions to one bin may be merged before the atomic:
-// equal corner bins of a voxel are merged in registers, and lanes of a
-// wavefront that share a joint bin are summed with a wave
-// reduction and added by one lane -- uniform image regions, where every
-// lane hits the same bin, then cost one LDS atomic per wavefront instead
-// of 64 serialized ones.
-
-__device__ __forceinline__ unsigned long long
-wave_sum_u64 (unsigned long long v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor (v, off);
-    return v;
-}
-
-// Adds val at s_hist[key] for every lane with act set, merging lanes that
-// share a key (up to 4 distinct keys per wavefront; the rest go directly).
-__device__ __forceinline__ void
-wave_hist_add (unsigned long long* s_hist, bool act, int key, unsigned long long val)
-{
-    const int lane = threadIdx.x & 63;
-    unsigned long long pending = __ballot (act);
-    for (int it = 0; it < 4 && pending; it++) {
-        const int leader = __ffsll ((long long) pending) - 1;
-        const int lkey = __shfl (key, leader);
-        const bool match = act && key == lkey;
-        const unsigned long long tot = wave_sum_u64 (match ? val : 0ull);
-        if (lane == leader) atomicAdd (&s_hist[lkey], tot);
-        pending &= ~__ballot (match);
-        act = act && !match;
-    }
-    if (act) atomicAdd (&s_hist[key], val);
-}
+// the corners of a voxel that share corner 0's bin are merged in
+// registers, so a voxel in a uniform region costs one LDS atomic.  (The
+// lanes of a wavefront are not merged: the conflicts of same-address LDS
+// atomics cost the LDS pipe less than the shuffles of a wave reduction
+// cost the VALU, 0.73 against 0.82 ms at 512x512x160.)
 
 template <bool ALIGNED>
 __global__ __launch_bounds__ (TILE_THREADS)
@@ -1639,8 +1617,9 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
     const uint32_t* __restrict__ mquad, const float* __restrict__ coeff,
     const float* __restrict__ blut, unsigned long long* __restrict__ jhist,
     unsigned long long* __restrict__ fhist, const float* __restrict__ ptab,
-    const uint8_t* __restrict__ fmask, RoiVol mroi)
+    const uint8_t* __restrict__ fmask, RoiVol mroi, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     // the histograms first (the dynamic LDS base is 16-byte aligned), then
     // the LUTs; every pointer stays an LDS pointer, so the atomics are LDS
     // atomics
@@ -1729,7 +1708,7 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
             // 2^32 to its fixed bin's row, so k_mi_score reads the counts off
             // the row sums)
             const int row = fb * h.mbins;
-            wave_hist_add (s_j, act, row + b[0], (unsigned long long) q[0]);
+            if (act) atomicAdd (&s_j[row + b[0]], (unsigned long long) q[0]);
 #pragma unroll
             for (int n = 1; n < 8; n++)
                 if (act && q[n] != 0) atomicAdd (&s_j[row + b[n]], (unsigned long long) q[n]);
@@ -1834,8 +1813,9 @@ __global__ __launch_bounds__ (1024)
 void k_mi_score (HGeom h, const unsigned long long* __restrict__ jhist,
     unsigned long long* __restrict__ fhist, float* __restrict__ table,
     double* __restrict__ out_score, int64_t* __restrict__ out_nvox,
-    float* __restrict__ out_nvox_f, int64_t* __restrict__ hist_out)
+    float* __restrict__ out_nvox_f, int64_t* __restrict__ hist_out, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     __shared__ double s_m[MAX_BINS_1D];
     __shared__ double s_f[MAX_BINS_1D];
     __shared__ double s_red[1024];
@@ -1929,8 +1909,9 @@ reg_contract (const double* M, const double* in, int t)
 
 __global__ __launch_bounds__ (64)
 void k_regularize (KGeom g, RegOp op, const float* __restrict__ coeff, double* __restrict__ part,
-    double* __restrict__ cond)
+    double* __restrict__ cond, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     // per component a: p, the x-digit contractions Z0..Z3, Z01, Z10, and
     // the staged sums of the y step
     __shared__ double s_p[3][64];
@@ -2045,8 +2026,10 @@ void k_regularize (KGeom g, RegOp op, const float* __restrict__ coeff, double* _
 // order; workgroup 0 also sums the region scores in region order.
 __global__ __launch_bounds__ (256)
 void k_reg_reduce (const double* __restrict__ cond, const double* __restrict__ part, int nregions,
-    int nc, double* __restrict__ score, double* __restrict__ grad, int accumulate)
+    int nc, double* __restrict__ score, double* __restrict__ grad, int accumulate,
+    const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     __shared__ double s_q[4][64];
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
@@ -2165,8 +2148,9 @@ k_bin_quads (const uint8_t* __restrict__ b, uint32_t* __restrict__ out, int64_t 
 }
 
 __global__ void
-k_zero_u64 (unsigned long long* __restrict__ p, int n)
+k_zero_u64 (unsigned long long* __restrict__ p, int n, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         p[i] = 0ull;
 }
@@ -2229,6 +2213,7 @@ struct TileArgs {
     float* vf;              // pair 0's displacement field (KIND 4 out, KIND 5 in), or
                             // the warped image (KIND 6, 7 out)
     float fill = 0.f;       // KIND 6, 7: the value where a voxel leaves the moving image
+    const int* skip = nullptr;  // EvalOut::skip
 };
 
 // Persistent grid: as many workgroups as fit on the device at once
@@ -2266,7 +2251,7 @@ launch_tiles_vgb (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st
     const int per_xcd = std::max (1, std::min (q.tiles_per_xcd, occ * cus / 8));
     hipLaunchKernelGGL ((k_tiles<KIND, TI, V, G, BATCH>), dim3 (8 * per_xcd), dim3 (TILE_THREADS),
         lds, st, plan.d_pairs, q, ta.coeff0, plan.blut, plan.part_score, plan.part_count, ta.mi,
-        ta.vf, ta.fill);
+        ta.vf, ta.fill, ta.skip);
     return 8 * per_xcd;
 }
 
@@ -2339,7 +2324,8 @@ hipError_t
 launch_mse (const Plan& plan, const Tune& tune, int mode, int storage, const float* coeff0,
     const EvalOut& out0, hipStream_t st, hipEvent_t* ev)
 {
-    const TileArgs ta {coeff0, MiArgs {}, nullptr};
+    TileArgs ta {coeff0, MiArgs {}, nullptr};
+    ta.skip = out0.skip;
     hipError_t e = mark (ev, 0, st);
     if (e != hipSuccess) return e;
     int nparts;
@@ -2433,7 +2419,8 @@ launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     hipError_t e = mark (ev, 0, st);
     if (e != hipSuccess) return e;
     // jhist and fhist are one allocation: joint bins then fixed bins.
-    hipLaunchKernelGGL (k_zero_u64, dim3 ((nbins + 255) / 256), dim3 (256), 0, st, jhist, nbins);
+    hipLaunchKernelGGL (k_zero_u64, dim3 ((nbins + 255) / 256), dim3 (256), 0, st, jhist, nbins,
+        out0.skip);
     LAUNCH_CHECK ();
     const size_t lds1 = sizeof (unsigned long long) * nbins
         + sizeof (float) * 4 * (g.vpr[0] + g.vpr[1] + g.vpr[2]);
@@ -2441,17 +2428,18 @@ launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
     if (g.aligned)
         hipLaunchKernelGGL ((k_mi_hist<true>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
-            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist, ptab, fmask1, mroi);
+            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist, ptab, fmask1, mroi, out0.skip);
     else
         hipLaunchKernelGGL ((k_mi_hist<false>), dim3 (hist_blocks), dim3 (TILE_THREADS), lds1, st,
-            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist, ptab, fmask1, mroi);
+            g, h, fbin, mquad, coeff, plan.blut, jhist, fhist, ptab, fmask1, mroi, out0.skip);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 2, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_mi_score, dim3 (1), dim3 (1024), 0, st, h, jhist, fhist, table,
-        out0.score, out0.nvox, nvox_f, hist_out);
+        out0.score, out0.nvox, nvox_f, hist_out, out0.skip);
     LAUNCH_CHECK ();
     if ((e = mark (ev, 3, st)) != hipSuccess) return e;
-    const TileArgs ta {coeff, MiArgs {h, table, nvox_f, fbin, mquad}, nullptr};
+    TileArgs ta {coeff, MiArgs {h, table, nvox_f, fbin, mquad}, nullptr};
+    ta.skip = out0.skip;
     // pass 2 reads only bin images; the image type parameter is unused
     launch_tiles<2, float> (plan, tune, table_bytes, st, ta);
     LAUNCH_CHECK ();
@@ -2504,13 +2492,14 @@ launch_zero (float* p, int64_t n, hipStream_t st)
 
 hipError_t
 launch_regularize (const KGeom& g, const RegOp& op, const float* coeff, double* part, double* cond,
-    double* score, double* grad, int accumulate, hipStream_t st)
+    double* score, double* grad, int accumulate, hipStream_t st, const int* skip)
 {
-    hipLaunchKernelGGL (k_regularize, dim3 (g.num_tiles), dim3 (64), 0, st, g, op, coeff, part, cond);
+    hipLaunchKernelGGL (k_regularize, dim3 (g.num_tiles), dim3 (64), 0, st, g, op, coeff, part, cond,
+        skip);
     LAUNCH_CHECK ();
     const int nc = 3 * g.num_knots;
     hipLaunchKernelGGL (k_reg_reduce, dim3 ((nc + 63) / 64), dim3 (256), 0, st, cond, part,
-        g.num_tiles, nc, score, grad, accumulate);
+        g.num_tiles, nc, score, grad, accumulate, skip);
     return hipGetLastError ();
 }
 

@@ -1,12 +1,13 @@
 // Vector kernels of the device-resident L-BFGS: the vector work of
 // liblbfgs's lbfgs() (libs/liblbfgs-1.9/lib/lbfgs.c:449-609) and of its
-// More-Thuente trials (:903-906), in double: one pass after each
-// evaluation, one per iteration to form the direction.
+// More-Thuente trials (:903-906), in double, and the pass that runs the
+// scalar logic on the device.
 //
 // Built with -ffp-contract=off: each element update rounds its product and
 // its sum separately, as liblbfgs's vecadd / vecdiff do.
 
 #include "lbfgs_kernels.h"
+#include "lbfgs_logic.h"
 
 namespace plmhip {
 
@@ -62,80 +63,153 @@ block_partials (const double (&v)[NC], double* __restrict__ out, int nb)
     }
 }
 
+// The point of the next evaluation (LbAct::kind): 0 the start point x0; 1
+// a new direction d = -H g from the coefficients the scalar logic solved
+// for, its first trial xp + stp d and the partials of g.d; 2 a trial
+// xp + stp d along the current d.  x and the float coefficients are
+// written.
 __global__ void __launch_bounds__ (LB_THREADS)
-k_lb_trial (const double* __restrict__ xp, const double* __restrict__ d, double stp,
-    double* __restrict__ x, float* __restrict__ coeff, int64_t n)
+k_lb_next (const LbAct* __restrict__ act, LbBufs b)
 {
+    if (act->done) return;
+    const int kind = act->kind;
+    const double stp = act->stp;
+    double* __restrict__ x = b.X[act->xi];
+    const double* __restrict__ xp = b.X[act->xpi];
+    const double* __restrict__ g = b.G[act->gpi];
+    const int nd = act->dir_b;
+    const double gamma = act->gamma;
+    const int64_t n = b.n;
+    double v = 0.0;
     const int64_t stride = (int64_t) gridDim.x * LB_THREADS;
     for (int64_t e = (int64_t) blockIdx.x * LB_THREADS + threadIdx.x; e < n; e += stride) {
-        double v = xp[e];
-        if (d) v = v + stp * d[e];
-        x[e] = v;
-        coeff[e] = (float) v;
+        double xe;
+        if (kind == 0) {
+            xe = b.x0[e];
+        } else if (kind == 1) {
+            const double ge = g[e];
+            double de;
+            if (nd == 0) {
+                de = -ge;
+            } else {
+                double t1 = 0.0, t2 = 0.0;
+                for (int i = 0; i < nd; ++i) {
+                    const int64_t off = (int64_t) act->slot[i] * n + e;
+                    t1 = t1 + act->ca[i] * b.S[off];
+                    t2 = t2 + act->cu[i] * b.Y[off];
+                }
+                de = -gamma * ge - t1 + gamma * t2;
+            }
+            b.d[e] = de;
+            xe = xp[e] + stp * de;
+            v += ge * de;
+        } else {
+            xe = xp[e] + stp * b.d[e];
+        }
+        x[e] = xe;
+        b.coeff[e] = (float) xe;
+    }
+    if (kind == 1) {
+        const double s = block_sum1 (v);
+        if (threadIdx.x == 0) b.slope[blockIdx.x] = s;
     }
 }
 
-// BP < 0: the start point (line-search dots only); else the new pair into
-// slot a.e with BP earlier pairs.
-template <int BP>
+// After the evaluation: the gradient it wrote kept in the trial's g buffer,
+// and the partials of every post component (LbAct::post_bp <= BPMAX earlier
+// pairs; the start point has no pair and takes g for d).
+template <int BPMAX>
 __global__ void __launch_bounds__ (LB_THREADS)
-k_lb_post (LbPostArgs a, int64_t n)
+k_lb_post (const LbAct* __restrict__ act, LbBufs b)
 {
-    constexpr int NC = BP < 0 ? 3 : 7 + 5 * BP;
+    if (act->done) return;
+    constexpr int NC = 7 + 5 * BPMAX;
+    const int pe = act->post_e, bp = act->post_bp, m = b.m;
+    const bool pair = pe >= 0;
+    const double* __restrict__ x = b.X[act->xi];
+    const double* __restrict__ xp = b.X[act->xpi];
+    const double* __restrict__ g = b.geval;
+    double* __restrict__ gk = b.G[act->gi];
+    const double* __restrict__ gp = b.G[act->gpi];
+    const double* __restrict__ d = act->kind == 0 ? nullptr : b.d;
+    const int64_t n = b.n;
     double v[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) v[c] = 0.0;
-    int slot[BP > 0 ? BP : 1];
+    int slot[BPMAX > 0 ? BPMAX : 1];
 #pragma unroll
-    for (int i = 1; i <= BP; ++i) slot[i - 1] = (a.e - i + a.m) % a.m;
+    for (int i = 1; i <= BPMAX; ++i) slot[i - 1] = (pe - i + m) % m;
     const int64_t stride = (int64_t) gridDim.x * LB_THREADS;
     for (int64_t e = (int64_t) blockIdx.x * LB_THREADS + threadIdx.x; e < n; e += stride) {
-        const double xe = a.x[e], ge = a.g[e];
-        const double de = a.d ? a.d[e] : ge;
+        const double xe = x[e], ge = g[e];
+        gk[e] = ge;
+        const double de = d ? d[e] : ge;
         v[0] += ge * de;
         v[1] += xe * xe;
         v[2] += ge * ge;
-        if constexpr (BP >= 0) {
-            const double se = xe - a.xp[e];
-            const double ye = ge - a.gp[e];
-            a.S[(int64_t) a.e * n + e] = se;
-            a.Y[(int64_t) a.e * n + e] = ye;
+        if (pair) {
+            const double se = xe - xp[e];
+            const double ye = ge - gp[e];
+            b.S[(int64_t) pe * n + e] = se;
+            b.Y[(int64_t) pe * n + e] = ye;
             v[3] += se * ye;
             v[4] += ye * ye;
             v[5] += se * ge;
             v[6] += ye * ge;
 #pragma unroll
-            for (int i = 0; i < BP; ++i) {
-                const double sj = a.S[(int64_t) slot[i] * n + e];
-                const double yj = a.Y[(int64_t) slot[i] * n + e];
-                v[7 + 5 * i + 0] += se * yj;
-                v[7 + 5 * i + 1] += sj * ye;
-                v[7 + 5 * i + 2] += ye * yj;
-                v[7 + 5 * i + 3] += sj * ge;
-                v[7 + 5 * i + 4] += yj * ge;
+            for (int i = 0; i < BPMAX; ++i) {
+                if (i < bp) {
+                    const double sj = b.S[(int64_t) slot[i] * n + e];
+                    const double yj = b.Y[(int64_t) slot[i] * n + e];
+                    v[7 + 5 * i + 0] += se * yj;
+                    v[7 + 5 * i + 1] += sj * ye;
+                    v[7 + 5 * i + 2] += ye * yj;
+                    v[7 + 5 * i + 3] += sj * ge;
+                    v[7 + 5 * i + 4] += yj * ge;
+                }
             }
         }
     }
-    block_partials<NC> (v, a.part, gridDim.x);
+    block_partials<NC> (v, b.part, gridDim.x);
 }
 
-// One workgroup: the slope's total into host[3], component c's into
-// host[4 + c]; each is the sum of its nb partials, thread (k, l) of a group
-// of sixteen totals adding partials l, l + 16, ... in turn, then the
-// sixteen in turn.
+// One workgroup: the totals -- the score slots, the slope (sum of the nb
+// partials of k_lb_next) and every component LbAct::ncomp (sums of the nb
+// partials of k_lb_post), thread (k, l) of a group of sixteen totals adding
+// partials l, l + 16, ... in turn, then the sixteen in turn.  Without a
+// machine the totals go to host; with one, its state is staged in LDS,
+// thread 0 advances it over the totals, and the next action (or the run's
+// end) goes back to device memory, with a note to the polling host.
 __global__ void __launch_bounds__ (LB_THREADS)
-k_lb_finish (const double* __restrict__ part, int ncomp, int nb, const double* __restrict__ slope_part,
-    const double* __restrict__ score, double* __restrict__ host)
+k_lb_finish (const LbAct* __restrict__ act, LbMachine* __restrict__ machine, LbBufs b,
+    double* __restrict__ host, LbFlag* __restrict__ flag)
 {
+    if (act->done) return;
     __shared__ double mid[16][16];
+    __shared__ double tot[4 + 7 + 5 * (LB_MAX_M - 1)];
+    __shared__ LbMachine ms;
     const int t = threadIdx.x;
-    const int total = ncomp + 1;
+    const int nb = b.nb;
+    const int total = act->ncomp + 1;
+    // the machine's words are fetched first, so their latency hides under
+    // the sums (it is 8-byte aligned)
+    constexpr int W = (int) (sizeof (LbMachine) / 8);
+    constexpr int WT = (W + LB_THREADS - 1) / LB_THREADS;
+    static_assert (sizeof (LbMachine) % 8 == 0, "LbMachine size");
+    uint64_t* ws = reinterpret_cast<uint64_t*> (&ms);
+    uint64_t* wg = reinterpret_cast<uint64_t*> (machine);
+    uint64_t mw[WT];
+    if (machine) {
+#pragma unroll
+        for (int i = 0; i < WT; i++)
+            if (t + i * LB_THREADS < W) mw[i] = wg[t + i * LB_THREADS];
+    }
     for (int k0 = 0; k0 < total; k0 += 16) {
         const int k = k0 + (t >> 4), l = t & 15;
         // every partial of this thread loaded before any is added (nb <=
         // LB_MAX_BLOCKS), then added in q order; the absent ones add +0.0,
         // which leaves a sum that started at +0.0 unchanged
-        const double* p = k == 0 ? slope_part : part + (int64_t) (k - 1) * nb;
+        const double* p = k == 0 ? b.slope : b.part + (int64_t) (k - 1) * nb;
         double pv[LB_MAX_BLOCKS / 16];
 #pragma unroll
         for (int i = 0; i < LB_MAX_BLOCKS / 16; i++) {
@@ -151,107 +225,94 @@ k_lb_finish (const double* __restrict__ part, int ncomp, int nb, const double* _
             double u = 0.0;
 #pragma unroll
             for (int j = 0; j < 16; ++j) u += mid[t][j];
-            host[3 + k0 + t] = u;
+            tot[3 + k0 + t] = u;
         }
         __syncthreads ();
     }
+    if (t < 3) tot[t] = b.score[t];
+    __syncthreads ();
+    if (!machine) {
+        for (int i = t; i < 3 + total; i += LB_THREADS) host[i] = tot[i];
+        return;
+    }
+    // the machine through LDS
+#pragma unroll
+    for (int i = 0; i < WT; i++)
+        if (t + i * LB_THREADS < W) ws[t + i * LB_THREADS] = mw[i];
+    __syncthreads ();
+    if (t == 0) ms.advance (tot);
+    __syncthreads ();
+    for (int i = t; i < W; i += LB_THREADS) wg[i] = ws[i];
     if (t == 0) {
-        host[0] = score[0];
-        host[1] = score[1];
-        host[2] = score[2];
+        // evaluations and the end in one 8-byte store, so the host never
+        // sees one without the other
+        LbFlag f {ms.evals, ms.a.done, ms.nrep, 0};
+        *reinterpret_cast<volatile unsigned long long*> (flag) =
+            *reinterpret_cast<const unsigned long long*> (&f);
     }
 }
 
-// The direction d = -H g from the coefficients the host solved for, the
-// first trial point xp + stp d, its coefficients, and the partials of g.d.
 __global__ void __launch_bounds__ (LB_THREADS)
-k_lb_combine (LbCombineArgs a, int64_t n)
+k_lb_convert (const double* __restrict__ x, float* __restrict__ coeff, int64_t n)
 {
-    double v = 0.0;
-    const int b = a.b;
     const int64_t stride = (int64_t) gridDim.x * LB_THREADS;
-    for (int64_t e = (int64_t) blockIdx.x * LB_THREADS + threadIdx.x; e < n; e += stride) {
-        const double ge = a.g[e];
-        double de;
-        if (b == 0) {
-            de = -ge;
-        } else {
-            double t1 = 0.0, t2 = 0.0;
-            for (int i = 0; i < b; ++i) {
-                const int64_t off = (int64_t) a.slot[i] * n + e;
-                t1 = t1 + a.ca[i] * a.S[off];
-                t2 = t2 + a.cu[i] * a.Y[off];
-            }
-            de = -a.gamma * ge - t1 + a.gamma * t2;
-        }
-        a.d[e] = de;
-        const double xe = a.xp[e] + a.stp * de;
-        a.x[e] = xe;
-        a.coeff[e] = (float) xe;
-        v += ge * de;
-    }
-    const double s = block_sum1 (v);
-    if (threadIdx.x == 0) a.slope[blockIdx.x] = s;
+    for (int64_t e = (int64_t) blockIdx.x * LB_THREADS + threadIdx.x; e < n; e += stride)
+        coeff[e] = (float) x[e];
 }
 
-template <int BP>
+template <int BPMAX>
 hipError_t
-launch_post_bp (const LbPostArgs& a, int64_t n, hipStream_t st)
+launch_post_m (const LbAct* act, const LbBufs& b, hipStream_t st)
 {
-    hipLaunchKernelGGL (k_lb_post<BP>, dim3 (lb_blocks (n)), dim3 (LB_THREADS), 0, st, a, n);
+    hipLaunchKernelGGL (k_lb_post<BPMAX>, dim3 (b.nb), dim3 (LB_THREADS), 0, st, act, b);
     return hipGetLastError ();
 }
 
 } // namespace
 
 hipError_t
-lb_launch_finish (const double* part, int ncomp, int nb, const double* slope_part, const double* score,
-    double* host, hipStream_t st)
+lb_launch_next (const LbAct* act, const LbBufs& b, hipStream_t st)
 {
-    hipLaunchKernelGGL (k_lb_finish, dim3 (1), dim3 (LB_THREADS), 0, st, part, ncomp, nb, slope_part, score,
-        host);
+    hipLaunchKernelGGL (k_lb_next, dim3 (b.nb), dim3 (LB_THREADS), 0, st, act, b);
     return hipGetLastError ();
 }
 
 hipError_t
-lb_launch_trial (const double* xp, const double* d, double stp, double* x, float* coeff,
-    int64_t n, hipStream_t st)
+lb_launch_post (const LbAct* act, const LbBufs& b, hipStream_t st)
 {
-    hipLaunchKernelGGL (k_lb_trial, dim3 (lb_blocks (n)), dim3 (LB_THREADS), 0, st, xp, d, stp, x,
-        coeff, n);
-    return hipGetLastError ();
-}
-
-hipError_t
-lb_launch_post (const LbPostArgs& a, int64_t n, hipStream_t st)
-{
-    if (a.e < 0) return launch_post_bp<-1> (a, n, st);
-    switch (a.bp) {
-    case 0: return launch_post_bp<0> (a, n, st);
-    case 1: return launch_post_bp<1> (a, n, st);
-    case 2: return launch_post_bp<2> (a, n, st);
-    case 3: return launch_post_bp<3> (a, n, st);
-    case 4: return launch_post_bp<4> (a, n, st);
-    case 5: return launch_post_bp<5> (a, n, st);
-    case 6: return launch_post_bp<6> (a, n, st);
-    case 7: return launch_post_bp<7> (a, n, st);
-    case 8: return launch_post_bp<8> (a, n, st);
-    case 9: return launch_post_bp<9> (a, n, st);
-    case 10: return launch_post_bp<10> (a, n, st);
-    case 11: return launch_post_bp<11> (a, n, st);
-    case 12: return launch_post_bp<12> (a, n, st);
-    case 13: return launch_post_bp<13> (a, n, st);
-    case 14: return launch_post_bp<14> (a, n, st);
-    case 15: return launch_post_bp<15> (a, n, st);
+    switch (b.m - 1) {
+    case 0: return launch_post_m<0> (act, b, st);
+    case 1: return launch_post_m<1> (act, b, st);
+    case 2: return launch_post_m<2> (act, b, st);
+    case 3: return launch_post_m<3> (act, b, st);
+    case 4: return launch_post_m<4> (act, b, st);
+    case 5: return launch_post_m<5> (act, b, st);
+    case 6: return launch_post_m<6> (act, b, st);
+    case 7: return launch_post_m<7> (act, b, st);
+    case 8: return launch_post_m<8> (act, b, st);
+    case 9: return launch_post_m<9> (act, b, st);
+    case 10: return launch_post_m<10> (act, b, st);
+    case 11: return launch_post_m<11> (act, b, st);
+    case 12: return launch_post_m<12> (act, b, st);
+    case 13: return launch_post_m<13> (act, b, st);
+    case 14: return launch_post_m<14> (act, b, st);
+    case 15: return launch_post_m<15> (act, b, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t
-lb_launch_combine (const LbCombineArgs& a, int64_t n, hipStream_t st)
+lb_launch_finish (const LbAct* act, LbMachine* machine, const LbBufs& b, double* host, LbFlag* flag,
+    hipStream_t st)
 {
-    if (a.b < 0 || a.b > LB_MAX_M) return hipErrorInvalidValue;
-    hipLaunchKernelGGL (k_lb_combine, dim3 (lb_blocks (n)), dim3 (LB_THREADS), 0, st, a, n);
+    hipLaunchKernelGGL (k_lb_finish, dim3 (1), dim3 (LB_THREADS), 0, st, act, machine, b, host, flag);
+    return hipGetLastError ();
+}
+
+hipError_t
+lb_launch_convert (const double* x, float* coeff, int64_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL (k_lb_convert, dim3 (lb_blocks (n)), dim3 (LB_THREADS), 0, st, x, coeff, n);
     return hipGetLastError ();
 }
 

@@ -146,3 +146,30 @@ def test_device_lbfgs_against_reference_objective(hip_lib):
     assert abs(dev.fx - ora.fx) <= 1e-4 * abs(ora.fx)
     assert rel(x, ora.x) < 1e-3
     st.close()
+
+
+@pytest.mark.parametrize("variant", ["mse", "exact_reg", "mi"])
+def test_resident_loop_equals_host_loop(hip_lib, variant):
+    """The loop run on the device (its scalar logic on one GPU thread, the
+    evaluation chain replayed from a hipGraph) and the host-driven loop
+    (the same logic on the host between evaluations) make the same run bit
+    for bit: every report, the status, the counts, x and the coefficients."""
+    metric, grad_mode, reg, bins = "mse", "ref_nn", None, None
+    if variant == "exact_reg":
+        grad_mode, reg = "exact", REG
+    elif variant == "mi":
+        metric, bins = "mi", 32
+    pair, bxf, st = stage(bins=bins, seed=8)
+    runs = []
+    for resident in (True, False):
+        import torch
+        d_x = torch.zeros(bxf.num_coeff, dtype=torch.float64, device="cuda")
+        d_c = torch.full((bxf.num_coeff,), 7.0, dtype=torch.float32, device="cuda")
+        r = st.optimize_lbfgs(d_x, d_c, metric, grad_mode, reg, resident=resident, max_its=6, max_feval=25)
+        runs.append((r, d_x.cpu().numpy(), d_c.cpu().numpy()))
+    (a, xa, ca), (b, xb, cb) = runs
+    print(f"{variant}: status {a.status} it {a.iterations} feval {a.fevals} reports {len(a.reports)}")
+    assert (a.status, a.iterations, a.fevals, a.fx) == (b.status, b.iterations, b.fevals, b.fx)
+    assert a.reports == b.reports and len(a.reports) > 3
+    assert np.array_equal(xa, xb) and np.array_equal(ca, cb)
+    st.close()

@@ -1,59 +1,72 @@
 """The MI histogram kernel quantizes each partial-volume weight w in [0, 1]
-with integer operations on the float's bits (a shift of its 24-bit
-significand, rounding half up: plastimatch_amd/csrc/bspline_kernels.hip,
-pv_fixed) instead of the double expression floor(w * 2^32 + 0.5) of the
-oracle's orc_pv_quantize.  The two agree because the double expression is
-exact; this checks the integer rule against the oracle on random weight
-vectors and on every exponent a weight can have, on the CPU."""
+with float operations and 32-bit integers (plastimatch_amd/csrc/
+bspline_kernels.hip, pv_quantize): x = w * 2^32 in float, q =
+floor(x) + (x - floor(x) >= 1/2), w = 1 as 2^32, and the remainder r =
+2^32 - sum(q) from the eight q's 32-bit sum, negated -- instead of the
+double expression floor(w * 2^32 + 0.5) and the 64-bit sum of the oracle's
+orc_pv_quantize.  This restates the kernel's steps in numpy (float32 and
+uint32 arithmetic, as the kernel's registers hold them) and checks them
+against the oracle on random trilinear weight vectors and on every
+exponent a weight can have, on the CPU."""
 import numpy as np
 
 import oracle
 
 
-def pv_fixed(w):
-    """The kernel's rule, vectorized: w = m 2^(e-150), w 2^32 = m 2^(e-118)."""
-    b = np.asarray(w, np.float32).view(np.uint32).astype(np.uint64)
-    e = ((b >> np.uint64(23)) & np.uint64(0xff)).astype(np.int64)
-    m = (b & np.uint64(0x7fffff)) | np.uint64(0x800000)
-    k = 118 - e
-    out = np.zeros(b.shape, np.int64)
-    left = (e != 0) & (k <= 0)
-    out[left] = (m[left] << (-k[left]).astype(np.uint64)).astype(np.int64)
-    right = (e != 0) & (k > 0) & (k <= 24)
-    kk = k[right].astype(np.uint64)
-    out[right] = ((m[right] + (np.uint64(1) << (kk - np.uint64(1)))) >> kk).astype(np.int64)
-    return out
+def pv_fixed32(w):
+    """The kernel's per-weight rule: q in 32 bits (0 for w = 1)."""
+    w = np.asarray(w, np.float32)
+    x = (w * np.float32(4294967296.0)).astype(np.float32)
+    fl = np.floor(x).astype(np.float32)
+    frac = (x - fl).astype(np.float32)
+    v = np.minimum(fl, np.float32(4294967040.0)).astype(np.uint64) + (frac >= np.float32(0.5))
+    return np.where(w == np.float32(1.0), 0, v).astype(np.uint32)
 
 
-def test_integer_rule_matches_oracle_weights():
+def pv_quantize_kernel(w):
+    """The kernel's pv_quantize: 32-bit weights, the negated 32-bit sum as
+    the remainder, given to the first largest weight (2^32 where w = 1)."""
+    w = np.asarray(w, np.float32)
+    q32 = pv_fixed32(w)
+    s = np.uint32(int(q32.astype(np.uint64).sum()) & 0xffffffff)
+    r = np.int64(np.int32(np.uint32((0 - int(s)) & 0xffffffff)))
+    q = q32.astype(np.int64)
+    amax = int(np.argmax(w))                            # the first largest
+    q[amax] = (1 << 32) if np.any(w == np.float32(1.0)) else q[amax] + r
+    return q
+
+
+def test_kernel_rule_matches_oracle_weights():
     """On trilinear weight vectors (products of li fractions, as the kernel
-    forms them) the first seven quantized weights equal the oracle's; the
-    eighth differs only by the remainder the oracle gives the largest."""
+    forms them), a corner on a voxel centre and clamped axes included, the
+    quantized vector equals the oracle's."""
     rng = np.random.default_rng(3)
-    for _ in range(2000):
+    for t in range(3000):
         f = rng.random(3).astype(np.float32)
         if rng.random() < 0.3:
             f[rng.integers(3)] = np.float32(0.0)         # a clamped axis
+        if t % 500 == 0:
+            f[:] = np.float32(0.0)                        # every weight on corner 0
         l2 = f
         l1 = (np.float32(1) - f).astype(np.float32)
         w = np.array([l1[0] * l1[1] * l1[2], l2[0] * l1[1] * l1[2], l1[0] * l2[1] * l1[2],
                       l2[0] * l2[1] * l1[2], l1[0] * l1[1] * l2[2], l2[0] * l1[1] * l2[2],
                       l1[0] * l2[1] * l2[2], l2[0] * l2[1] * l2[2]], np.float32)
-        q = pv_fixed(w)
-        amax = int(np.argmax(w))                          # the first largest
-        q[amax] += (1 << 32) - int(q.sum())
+        q = pv_quantize_kernel(w)
         assert np.array_equal(q, oracle.pv_quantize(w)), w
+        assert int(q.sum()) == 1 << 32
 
 
-def test_integer_rule_every_exponent():
+def test_kernel_rule_every_exponent():
     """floor(w 2^32 + 1/2) in double, for w drawn across every exponent from
     below 2^-33 up to 1, zero, denormals and the rounding ties."""
     rng = np.random.default_rng(4)
     bits = (np.arange(1, 128, dtype=np.uint32)[:, None] << np.uint32(23)) \
         | rng.integers(0, 1 << 23, (127, 2000), dtype=np.uint32)
     w = np.concatenate([bits.ravel().view(np.float32),
-                        np.array([0.0, 1.0, 0.5, 2.0 ** -33, 2.0 ** -34, 3 * 2.0 ** -34, 2.0 ** -32,
-                                  1e-45, 2.0 ** -126], np.float32)])
-    w = w[(w >= 0) & (w <= 1)]
+                        np.array([0.0, 0.5, 2.0 ** -33, 2.0 ** -34, 3 * 2.0 ** -34, 2.0 ** -32,
+                                  1e-45, 2.0 ** -126, np.nextafter(np.float32(1), np.float32(0))],
+                                 np.float32)])
+    w = w[(w >= 0) & (w < 1)]
     ref = np.floor(w.astype(np.float64) * 4294967296.0 + 0.5).astype(np.int64)
-    assert np.array_equal(pv_fixed(w), ref)
+    assert np.array_equal(pv_fixed32(w).astype(np.int64), ref)
