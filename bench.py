@@ -24,10 +24,9 @@ finite-difference check of the adjoint.  "warp_regularize" times the
 warped moving image and the analytic regularizer on the headline workload,
 with their rooflines and the oracle's versions timed beside them.
 "optimizer_loop" runs the device L-BFGS (liblbfgs's algorithm) for 20
-iterations on the headline workload, with and without regularization
-(the device-resident loop), both loops' cost per evaluation (resident and
-host-driven), and the reference's liblbfgs on the oracle's score for 3
-iterations beside it.
+iterations on the headline workload, with and without regularization,
+both loops' cost per evaluation (host-driven and device-resident), and the
+reference's liblbfgs on the oracle's score for 3 iterations beside it.
 "stage_preprocess" times a stage's preprocessing on the device (both
 images subsampled, moving gradient, histogram ranges) against the oracle.
 "mse_128cube" evaluates the 128^3 pair (15-voxel grid) at zero and at the
@@ -461,8 +460,8 @@ def optimizer_leg(args, dev, eval_ms, want_cpu):
     st.optimize_lbfgs(d_x, d_c, max_its=2, max_feval=-1, resident=False)
     out = {"workload": WORKLOAD, "optimizer": "L-BFGS, More-Thuente (liblbfgs 1.9 defaults)",
            "start": "identity (zero coefficients)",
-           "loop": "resident: scalar logic on the device, evaluation chains replayed from a hipGraph "
-                   "(4 chains a replay); host_loop: the same logic on the host between evaluations"}
+           "loop": "host-driven (the default); mse_loops also times the device-resident loop "
+                   "(scalar logic on the device, evaluation chains replayed from a hipGraph, 4 a replay)"}
     runs = {}
     for label, reg in (("mse", None), ("mse_plus_regularization", OPT_REG_WEIGHTS)):
         d_x.zero_()

@@ -539,9 +539,11 @@ typedef struct plm_lbfgs_report {
  * pass, that step -- reads everything it needs from device memory, so it
  * is captured once in a hipGraph and replayed back to back with no host
  * work between evaluations; the reference's max_its / max_feval rule
- * stops it.  plm_hip_bspline_optimize_lbfgs takes this path when progress
- * is NULL.  No counterpart in the reference, whose liblbfgs calls back into
- * the host every evaluation (bspline_optimize_liblbfgs.cxx:83-109).
+ * stops it.  For callers that cannot keep a host thread in the loop: on
+ * one MI355X the host-driven loop above is as fast or faster (the scalar
+ * logic is latency bound on one device thread).  No counterpart in the
+ * reference, whose liblbfgs calls back into the host every evaluation
+ * (bspline_optimize_liblbfgs.cxx:83-109).
  * reports (may be NULL) receives the first max_reports progress reports in
  * order; n_reports (may be NULL) the number made. */
 int plm_hip_bspline_optimize_lbfgs_resident (plm_hip_ctx* ctx, int metric, int grad_mode,

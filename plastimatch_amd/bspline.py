@@ -466,7 +466,7 @@ class BsplineHip:
             float(default_val), 1 if linear else 0, st))
 
     def optimize_lbfgs(self, d_x, d_coeff=None, metric="mse", grad_mode="ref_nn", reg=None, progress=None,
-                       record=True, resident=None, max_reports=4096, **parms):
+                       record=True, resident=False, max_reports=4096, **parms):
         """L-BFGS on the device: minimize the score (+ the analytic
         regularization with weights ``reg``, a dict of Regularization_parms
         names) from the float64 device tensor d_x, which holds the x liblbfgs
@@ -475,12 +475,12 @@ class BsplineHip:
         fields (liblbfgs names, max_its, max_feval; unset ones take liblbfgs's
         defaults and max_its = max_feval = 10).  ``progress``: callable
         (d_x_ptr, d_g_ptr, fx, xnorm, gnorm, step, n, k, ls) -> int, nonzero
-        stops; it needs the host loop (plm_hip_bspline_optimize_lbfgs with a
-        callback).  ``resident`` (default: no progress callable): the whole
-        loop on the device, the evaluation chain replayed from a hipGraph
-        (plm_hip_bspline_optimize_lbfgs_resident), its reports read back
-        afterwards (the first ``max_reports``).  ``record``: keep every
-        progress report in the result.  Returns an LbfgsResult."""
+        stops (the host-driven loop, plm_hip_bspline_optimize_lbfgs).
+        ``resident``: the whole loop on the device, the evaluation chain
+        replayed from a hipGraph (plm_hip_bspline_optimize_lbfgs_resident),
+        its reports read back afterwards (the first ``max_reports``; no
+        progress callable).  ``record``: keep every progress report in the
+        result.  Returns an LbfgsResult."""
         if d_x.numel() != self.bxf.num_coeff:
             raise ValueError("d_x has the wrong length")
         p = _abi.PlmLbfgsParms()
@@ -498,8 +498,6 @@ class BsplineHip:
             w.update(reg)
             rp = _abi.PlmRegParms(**w)
         reports = []
-        if resident is None:
-            resident = progress is None
         if resident:
             if progress is not None:
                 raise ValueError("a progress callable needs the host loop (resident=False)")

@@ -8,13 +8,14 @@
 // More-Thuente line search with every evaluation reported to the progress
 // rule.  Its vectors live in device memory (lbfgs_kernels.hip); its scalar
 // logic is the state machine of lbfgs_logic.h, advanced once per
-// evaluation.  With a progress callback the machine runs here, on the
-// host: after each evaluation chain the host reads a few dozen totals from
-// mapped memory, advances, and hands the next action to the device.
-// Without one the machine runs on the device (k_lb_finish): the chain,
-// whose kernels read the action from device memory, is captured once in a
-// hipGraph and replayed back to back, a few replays ahead, while the host
-// only watches for the end.
+// evaluation.  plm_hip_bspline_optimize_lbfgs runs the machine here, on
+// the host: after each evaluation chain the host reads a few dozen totals
+// from mapped memory, advances (calling the progress callback at every
+// report), and passes the next action to the chain's kernels by value.
+// plm_hip_bspline_optimize_lbfgs_resident runs it on the device
+// (k_lb_finish): the chain, whose kernels then read the action from device
+// memory, is captured once in a hipGraph and replayed back to back, a few
+// replays ahead, while the host only watches for the end.
 //
 // The first trial of each line search is formed by the pass that forms
 // the direction, before the initial slope g.d is known: the slope comes
@@ -49,8 +50,6 @@ struct LbWorkspace {
     int m = 0;
     int cap = 0;                            // reports the log holds
     LbBufs b {};
-    LbAct* d_act = nullptr;                 // the action (host-driven runs)
-    LbAct* h_act = nullptr;                 // its pinned staging copy
     LbMachine* d_mach = nullptr;            // the machine (device-resident runs)
     plm_lbfgs_report* d_rep = nullptr;      // their report log
     double* hh = nullptr;                   // pinned totals (host-driven runs)
@@ -61,9 +60,9 @@ struct LbWorkspace {
     ~LbWorkspace ()
     {
         void* bufs[] = {b.X[0], b.X[1], b.G[0], b.G[1], b.geval, b.coeff, b.d, b.S, b.Y, b.slope, b.part,
-            b.score, d_act, d_mach, d_rep};
+            b.score, d_mach, d_rep};
         for (void* q : bufs) if (q) (void) hipFree (q);
-        void* hbufs[] = {h_act, hh, flag};
+        void* hbufs[] = {hh, flag};
         for (void* q : hbufs) if (q) (void) hipHostFree (q);
     }
 };
@@ -176,9 +175,6 @@ struct Run {
                 || alloc (&wb.score, 3, "hipMalloc(L-BFGS score)")
                 || alloc (&wb.part, (size_t) ncomp * wb.nb, "hipMalloc(L-BFGS partials)")
                 || alloc (&wb.slope, (size_t) wb.nb, "hipMalloc(L-BFGS slope partials)")
-                || alloc (&w->d_act, 1, "hipMalloc(L-BFGS action)")
-                || hip (hipHostMalloc ((void**) &w->h_act, sizeof (LbAct), hipHostMallocDefault),
-                       "hipHostMalloc(L-BFGS action)")
                 || pinned (&w->hh, &w->hh_dev, 4 + (size_t) ncomp, "hipHostMalloc(L-BFGS readback)")
                 || alloc (&w->d_mach, 1, "hipMalloc(L-BFGS state)")
                 || pinned (&w->flag, &w->flag_dev, 1, "hipHostMalloc(L-BFGS note)")) {
@@ -199,18 +195,19 @@ struct Run {
         return PLM_HIP_OK;
     }
 
-    // One evaluation chain reading its action from act.
-    int chain (const LbAct* act, LbMachine* mach)
+    // One evaluation chain reading its action from act (device memory),
+    // or with act null from av.
+    int chain (const LbAct* act, const LbAct& av, LbMachine* mach)
     {
-        const int* skip = &act->done;
-        if (hip (plmhip::lb_launch_next (act, b, st), "L-BFGS trial point")) return err;
+        const int* skip = act ? &act->done : nullptr;
+        if (hip (plmhip::lb_launch_next (act, av, b, st), "L-BFGS trial point")) return err;
         int rc = plmhip::ctx_eval (c, metric, grad_mode, b.coeff, b.score, b.geval,
             reinterpret_cast<int64_t*> (b.score + 2), st, skip);
         if (rc < 0) return err = rc;
         if (reg && (rc = plmhip::ctx_regularize (c, reg, b.coeff, b.score + 1, b.geval, st, skip)) < 0)
             return err = rc;
-        if (hip (plmhip::lb_launch_post (act, b, st), "L-BFGS post pass")
-            || hip (plmhip::lb_launch_finish (act, mach, b, w->hh_dev, w->flag_dev, st), "L-BFGS totals"))
+        if (hip (plmhip::lb_launch_post (act, av, b, st), "L-BFGS post pass")
+            || hip (plmhip::lb_launch_finish (act, av, mach, b, w->hh_dev, w->flag_dev, st), "L-BFGS totals"))
             return err;
         return PLM_HIP_OK;
     }
@@ -225,10 +222,7 @@ struct Run {
         M.cb_g[0] = b.G[0];
         M.cb_g[1] = b.G[1];
         while (M.phase != 2) {
-            *w->h_act = M.a;
-            if (hip (hipMemcpyAsync (w->d_act, w->h_act, sizeof (LbAct), hipMemcpyHostToDevice, st),
-                    "L-BFGS action")
-                || chain (w->d_act, nullptr) || hip (hipStreamSynchronize (st), "L-BFGS evaluation"))
+            if (chain (nullptr, M.a, nullptr) || hip (hipStreamSynchronize (st), "L-BFGS evaluation"))
                 return err;
             M.advance (w->hh);
         }
@@ -256,7 +250,7 @@ struct Run {
             return err;
         if (hip (hipStreamBeginCapture (st, hipStreamCaptureModeRelaxed), "L-BFGS capture")) return err;
         int rc = PLM_HIP_OK;
-        for (int i = 0; i < RESIDENT_CHAINS && !rc; i++) rc = chain (&w->d_mach->a, w->d_mach);
+        for (int i = 0; i < RESIDENT_CHAINS && !rc; i++) rc = chain (&w->d_mach->a, M.a, w->d_mach);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture (st, &g);
         graph = g;
@@ -355,9 +349,10 @@ plm_hip_bspline_optimize_lbfgs (plm_hip_ctx* c, int metric, int grad_mode,
     const plm_reg_parms* reg, const plm_lbfgs_parms* parms, double* d_x, float* d_coeff,
     plm_lbfgs_progress_fn progress, void* instance, plm_lbfgs_result* result)
 {
-    // without a callback nothing needs the host between evaluations
-    return optimize (c, metric, grad_mode, reg, parms, d_x, d_coeff, progress, instance, progress == nullptr,
-        nullptr, 0, nullptr, result);
+    // (the host-driven loop: its per-evaluation round trip costs less than
+    // the scalar logic on one device thread)
+    return optimize (c, metric, grad_mode, reg, parms, d_x, d_coeff, progress, instance, false, nullptr, 0,
+        nullptr, result);
 }
 
 int

@@ -81,16 +81,19 @@ struct LbFlag {
     int pad;
 };
 
+// The chain's kernels read the action from act in device memory (a
+// device-resident run) or, with act null, from av passed by value (a
+// host-driven run, which hands each chain its action).
 // The point of the next evaluation, its coefficients, and for a new
 // direction d itself and the partials of g.d.
-hipError_t lb_launch_next (const LbAct* act, const LbBufs& b, hipStream_t st);
+hipError_t lb_launch_next (const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t st);
 // The post-evaluation pass (instantiated for the run's m).
-hipError_t lb_launch_post (const LbAct* act, const LbBufs& b, hipStream_t st);
+hipError_t lb_launch_post (const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t st);
 // The totals: with machine null, written to host (mapped host memory:
 // score, reg score, count, slope, then the components); else the machine
 // advances on the device (act must be &machine->a) and flag is updated.
-hipError_t lb_launch_finish (const LbAct* act, LbMachine* machine, const LbBufs& b, double* host,
-    LbFlag* flag, hipStream_t st);
+hipError_t lb_launch_finish (const LbAct* act, const LbAct& av, LbMachine* machine, const LbBufs& b,
+    double* host, LbFlag* flag, hipStream_t st);
 // coeff = (float) x.
 hipError_t lb_launch_convert (const double* x, float* coeff, int64_t n, hipStream_t st);
 

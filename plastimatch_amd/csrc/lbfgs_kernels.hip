@@ -69,8 +69,9 @@ block_partials (const double (&v)[NC], double* __restrict__ out, int nb)
 // xp + stp d along the current d.  x and the float coefficients are
 // written.
 __global__ void __launch_bounds__ (LB_THREADS)
-k_lb_next (const LbAct* __restrict__ act, LbBufs b)
+k_lb_next (const LbAct* __restrict__ ap, LbAct av, LbBufs b)
 {
+    const LbAct* act = ap ? ap : &av;
     if (act->done) return;
     const int kind = act->kind;
     const double stp = act->stp;
@@ -120,8 +121,9 @@ k_lb_next (const LbAct* __restrict__ act, LbBufs b)
 // pairs; the start point has no pair and takes g for d).
 template <int BPMAX>
 __global__ void __launch_bounds__ (LB_THREADS)
-k_lb_post (const LbAct* __restrict__ act, LbBufs b)
+k_lb_post (const LbAct* __restrict__ ap, LbAct av, LbBufs b)
 {
+    const LbAct* act = ap ? ap : &av;
     if (act->done) return;
     constexpr int NC = 7 + 5 * BPMAX;
     const int pe = act->post_e, bp = act->post_bp, m = b.m;
@@ -181,9 +183,10 @@ k_lb_post (const LbAct* __restrict__ act, LbBufs b)
 // thread 0 advances it over the totals, and the next action (or the run's
 // end) goes back to device memory, with a note to the polling host.
 __global__ void __launch_bounds__ (LB_THREADS)
-k_lb_finish (const LbAct* __restrict__ act, LbMachine* __restrict__ machine, LbBufs b,
+k_lb_finish (const LbAct* __restrict__ ap, LbAct av, LbMachine* __restrict__ machine, LbBufs b,
     double* __restrict__ host, LbFlag* __restrict__ flag)
 {
+    const LbAct* act = ap ? ap : &av;
     if (act->done) return;
     __shared__ double mid[16][16];
     __shared__ double tot[4 + 7 + 5 * (LB_MAX_M - 1)];
@@ -262,50 +265,50 @@ k_lb_convert (const double* __restrict__ x, float* __restrict__ coeff, int64_t n
 
 template <int BPMAX>
 hipError_t
-launch_post_m (const LbAct* act, const LbBufs& b, hipStream_t st)
+launch_post_m (const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t st)
 {
-    hipLaunchKernelGGL (k_lb_post<BPMAX>, dim3 (b.nb), dim3 (LB_THREADS), 0, st, act, b);
+    hipLaunchKernelGGL (k_lb_post<BPMAX>, dim3 (b.nb), dim3 (LB_THREADS), 0, st, act, av, b);
     return hipGetLastError ();
 }
 
 } // namespace
 
 hipError_t
-lb_launch_next (const LbAct* act, const LbBufs& b, hipStream_t st)
+lb_launch_next (const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t st)
 {
-    hipLaunchKernelGGL (k_lb_next, dim3 (b.nb), dim3 (LB_THREADS), 0, st, act, b);
+    hipLaunchKernelGGL (k_lb_next, dim3 (b.nb), dim3 (LB_THREADS), 0, st, act, av, b);
     return hipGetLastError ();
 }
 
 hipError_t
-lb_launch_post (const LbAct* act, const LbBufs& b, hipStream_t st)
+lb_launch_post (const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t st)
 {
     switch (b.m - 1) {
-    case 0: return launch_post_m<0> (act, b, st);
-    case 1: return launch_post_m<1> (act, b, st);
-    case 2: return launch_post_m<2> (act, b, st);
-    case 3: return launch_post_m<3> (act, b, st);
-    case 4: return launch_post_m<4> (act, b, st);
-    case 5: return launch_post_m<5> (act, b, st);
-    case 6: return launch_post_m<6> (act, b, st);
-    case 7: return launch_post_m<7> (act, b, st);
-    case 8: return launch_post_m<8> (act, b, st);
-    case 9: return launch_post_m<9> (act, b, st);
-    case 10: return launch_post_m<10> (act, b, st);
-    case 11: return launch_post_m<11> (act, b, st);
-    case 12: return launch_post_m<12> (act, b, st);
-    case 13: return launch_post_m<13> (act, b, st);
-    case 14: return launch_post_m<14> (act, b, st);
-    case 15: return launch_post_m<15> (act, b, st);
+    case 0: return launch_post_m<0> (act, av, b, st);
+    case 1: return launch_post_m<1> (act, av, b, st);
+    case 2: return launch_post_m<2> (act, av, b, st);
+    case 3: return launch_post_m<3> (act, av, b, st);
+    case 4: return launch_post_m<4> (act, av, b, st);
+    case 5: return launch_post_m<5> (act, av, b, st);
+    case 6: return launch_post_m<6> (act, av, b, st);
+    case 7: return launch_post_m<7> (act, av, b, st);
+    case 8: return launch_post_m<8> (act, av, b, st);
+    case 9: return launch_post_m<9> (act, av, b, st);
+    case 10: return launch_post_m<10> (act, av, b, st);
+    case 11: return launch_post_m<11> (act, av, b, st);
+    case 12: return launch_post_m<12> (act, av, b, st);
+    case 13: return launch_post_m<13> (act, av, b, st);
+    case 14: return launch_post_m<14> (act, av, b, st);
+    case 15: return launch_post_m<15> (act, av, b, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t
-lb_launch_finish (const LbAct* act, LbMachine* machine, const LbBufs& b, double* host, LbFlag* flag,
-    hipStream_t st)
+lb_launch_finish (const LbAct* act, const LbAct& av, LbMachine* machine, const LbBufs& b, double* host,
+    LbFlag* flag, hipStream_t st)
 {
-    hipLaunchKernelGGL (k_lb_finish, dim3 (1), dim3 (LB_THREADS), 0, st, act, machine, b, host, flag);
+    hipLaunchKernelGGL (k_lb_finish, dim3 (1), dim3 (LB_THREADS), 0, st, act, av, machine, b, host, flag);
     return hipGetLastError ();
 }
 

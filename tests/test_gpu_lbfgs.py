@@ -64,12 +64,12 @@ def gpu_objective(st, n, metric, grad_mode, reg):
     return fun
 
 
-def device_run(st, bxf, metric, grad_mode, reg, x0=None, progress=None, **parms):
+def device_run(st, bxf, metric, grad_mode, reg, x0=None, progress=None, resident=False, **parms):
     import torch
     x0 = np.zeros(bxf.num_coeff) if x0 is None else x0
     d_x = torch.from_numpy(np.array(x0, np.float64)).cuda()
     d_c = torch.full((bxf.num_coeff,), 7.0, dtype=torch.float32, device="cuda")
-    r = st.optimize_lbfgs(d_x, d_c, metric, grad_mode, reg, progress=progress, **parms)
+    r = st.optimize_lbfgs(d_x, d_c, metric, grad_mode, reg, progress=progress, resident=resident, **parms)
     return r, d_x.cpu().numpy(), d_c.cpu().numpy()
 
 
@@ -84,8 +84,9 @@ def compare(dev, ora, tol_f=1e-6):
             assert abs(u - v) <= tol_f * max(abs(v), 1e-30), (a, b)
 
 
+@pytest.mark.parametrize("resident", [False, True])
 @pytest.mark.parametrize("variant", ["ref_nn", "exact_reg_roi", "mi", "max_feval"])
-def test_device_lbfgs_matches_oracle(hip_lib, variant):
+def test_device_lbfgs_matches_oracle(hip_lib, variant, resident):
     metric, grad_mode, reg, roi, bins, lim = "mse", "ref_nn", None, None, None, dict(max_its=8, max_feval=30)
     if variant == "exact_reg_roi":
         grad_mode, reg, roi = "exact", REG, ((3, 4, 2), (31, 27, 22))
@@ -96,7 +97,7 @@ def test_device_lbfgs_matches_oracle(hip_lib, variant):
     pair, bxf, st = stage(roi=roi, bins=bins)
     fun = gpu_objective(st, bxf.num_coeff, metric, grad_mode, reg)
     ora = oracle.lbfgs(fun, np.zeros(bxf.num_coeff), **lim)
-    dev, x, c = device_run(st, bxf, metric, grad_mode, reg, **lim)
+    dev, x, c = device_run(st, bxf, metric, grad_mode, reg, resident=resident, **lim)
     print(f"{variant}: status {dev.status} it {dev.iterations} feval {dev.fevals} "
           f"fx {ora.evals[0][0]:.6g} -> {dev.fx:.6g} (oracle {ora.fx:.6g}), x rel {rel(x, ora.x):.2e}")
     compare(dev, ora)
