@@ -28,7 +28,8 @@ iterations on the headline workload, with and without regularization,
 both loops' cost per evaluation (host-driven and device-resident), and the
 reference's liblbfgs on the oracle's score for 3 iterations beside it.
 "stage_preprocess" times a stage's preprocessing on the device (both
-images subsampled, moving gradient, histogram ranges) against the oracle.
+images subsampled, moving gradient, histogram ranges) against the oracle,
+and the V-optimal histograms of both stage images.
 "mse_128cube" evaluates the 128^3 pair (15-voxel grid) at zero and at the
 true coefficients, single-pair HIP beside the oracle's flavor g.
 "mattes_mi" evaluates Mattes MI on the 512x512x160 pair (12-voxel grid,
@@ -544,7 +545,8 @@ def stage_leg(args, dev, want_cpu):
     and whether the results are the oracle's bit for bit."""
     import torch
     from plastimatch_amd import synth
-    from plastimatch_amd.bspline import Volume, hist_eqsp, stage_resample, volume_gradient
+    from plastimatch_amd.bspline import Volume, hist_eqsp, hist_vopt, stage_resample, volume_gradient
+    from plastimatch_amd._abi import VOPT_KEYS
 
     pair = synth.make_config(WORKLOAD, seed=0)
     fx = Volume(torch.from_numpy(pair.fixed).to(dev), pair.spacing, pair.origin)
@@ -570,6 +572,16 @@ def stage_leg(args, dev, want_cpu):
     out = {"workload": WORKLOAD, "resample": f"voxel_legacy {STAGE_RATE}, trilinear",
            "in_dim": list(pair.dim), "out_dim": list(f_ss.dim), "ms_per_stage": dt * 1e3,
            "resample_alg_bytes_per_image": 4 * (n_in + n_out)}
+    # the histogram_type=vopt alternative to the equal-spacing ranges:
+    # V-optimal bins of both stage images
+    for _ in range(2):
+        hist_vopt(f_ss, 32), hist_vopt(m_ss, 32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hv = (hist_vopt(f_ss, 32), hist_vopt(m_ss, 32))
+    torch.cuda.synchronize()
+    out["hist_vopt"] = {"bins": 32, "keys": VOPT_KEYS, "ms_both_images": (time.perf_counter() - t0) / reps * 1e3}
     if want_cpu:
         import oracle
         t0 = time.perf_counter()
@@ -585,6 +597,13 @@ def stage_leg(args, dev, want_cpu):
             "moving": bool(np.array_equal(m_ss.img.cpu().numpy(), rm)),
             "moving_gradient": bool(np.array_equal(mg.img.cpu().numpy(), rg)),
             "hist_eqsp": h == rh}
+        t0 = time.perf_counter()
+        rv = (oracle.hist_vopt(rf, 32), oracle.hist_vopt(rm, 32))
+        t_cpu = time.perf_counter() - t0
+        out["hist_vopt"]["cpu_baseline"] = {"value": t_cpu * 1e3, "unit": "ms both images", "cores": 1,
+                                            "kind": "port", "sample": "oracle orc_hist_vopt on both stage images"}
+        out["hist_vopt"]["bit_exact_vs_oracle"] = all(
+            a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2]) for a, b in zip(hv, rv))
     return out
 
 
