@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -331,19 +332,26 @@ plm_hip_hist_vopt (const plm_img_desc* vol, int64_t bins, float* offset, float* 
     // the equal-spacing grid of K keys (src/register/joint_histogram.cxx:242-244)
     const float dl = (mx - mn) / (float) (K - 1);
     const float off = (float) (mn - 0.5 * dl);
-    unsigned long long* d_cnt = nullptr;
-    double* d_tab = nullptr;     // key sums, then the three prefix tables
-    int* d_split = nullptr;
-    if ((rc = bufs.get (&d_cnt, K, "hipMalloc(key counts)")) || (rc = bufs.get (&d_tab, 3 * K, "hipMalloc(key tables)"))
-        || (rc = bufs.get (&d_split, (size_t) bins * K, "hipMalloc(key splits)")))
+    // one allocation: key counts, key sums (then the three prefix tables),
+    // the programme's last two rows, the splits of every row
+    char* blob = nullptr;
+    if ((rc = bufs.get (&blob, sizeof (double) * 6 * K + sizeof (int) * (size_t) bins * K, "hipMalloc(VOPT tables)")))
         return rc;
+    unsigned long long* d_cnt = reinterpret_cast<unsigned long long*> (blob);
+    double* d_tab = reinterpret_cast<double*> (blob) + K;
+    double* d_err = d_tab + 3 * K;
+    int* d_split = reinterpret_cast<int*> (d_err + 2 * K);
     std::vector<unsigned long long> cnt1 (K);
     std::vector<double> sum (K);
-    hipError_t e = hipMemset (d_cnt, 0, sizeof (unsigned long long) * K);
-    if (e == hipSuccess) e = hipMemset (d_tab, 0, sizeof (double) * K);
+    // counts and sums are adjacent: one clear, one read-back
+    std::vector<unsigned long long> ks (2 * (size_t) K);
+    hipError_t e = hipMemsetAsync (blob, 0, sizeof (double) * 2 * K, nullptr);
     if (e == hipSuccess) e = plmhip::launch_vopt_keys (src, n, off, dl, d_cnt, d_tab, nullptr);
-    if (e == hipSuccess) e = hipMemcpy (cnt1.data (), d_cnt, sizeof (unsigned long long) * K, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy (sum.data (), d_tab, sizeof (double) * K, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy (ks.data (), blob, sizeof (double) * 2 * K, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) {
+        std::memcpy (cnt1.data (), ks.data (), sizeof (unsigned long long) * K);
+        std::memcpy (sum.data (), ks.data () + K, sizeof (double) * K);
+    }
     if (e != hipSuccess) return plmhip::abi_hip_fail (e, "VOPT key sums");
     // key averages (an empty key repeats the previous one's) and their
     // running sums, in key order (:253-272)
@@ -365,7 +373,8 @@ plm_hip_hist_vopt (const plm_img_desc* vol, int64_t bins, float* offset, float* 
     }
     std::vector<int> split ((size_t) bins * K);
     e = hipMemcpy (d_tab, tab.data (), sizeof (double) * 3 * K, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = plmhip::launch_vopt_dp (d_tab, d_tab + K, d_tab + 2 * K, (int) bins, d_split, nullptr);
+    if (e == hipSuccess)
+        e = plmhip::launch_vopt_dp (d_tab, d_tab + K, d_tab + 2 * K, (int) bins, d_err, d_split, nullptr);
     if (e == hipSuccess) e = hipMemcpy (split.data (), d_split, sizeof (int) * split.size (), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return plmhip::abi_hip_fail (e, "VOPT partition");
     // the key table, walking the best splits back from the last key (:304-321)

@@ -46,9 +46,9 @@ hipError_t launch_vopt_keys (const float* img, int64_t n, float offset, float de
 // bspline_initialize_mi_hist_vopt over the prefix tables s, ssq, cnt
 // (VOPT_KEYS each); split[j * VOPT_KEYS + i] receives the last key of bin
 // j - 1 in the best grouping of keys 0..i into j + 1 bins (row 0 unused).
-// One workgroup.
+// One launch a row; err (2 VOPT_KEYS doubles) holds the last two rows.
 hipError_t launch_vopt_dp (const double* s, const double* ssq, const double* cnt, int bins,
-    int* split, hipStream_t st);
+    double* err, int* split, hipStream_t st);
 
 } // namespace plmhip
 

@@ -127,41 +127,48 @@ vopt_err (int start, int end, const double* s, const double* ssq, const double* 
     return nv < 1.0 ? DBL_MAX : v;
 }
 
-// Thread i owns key i: row j of the programme is, for every i, the least
-// err[j - 1][k] + vopt_err(k + 1, i) over k < i in increasing k, a tie
-// going to the later k (bspline_initialize_mi_hist_vopt,
-// src/register/joint_histogram.cxx:282-301); the rows live in LDS, the
-// argmins go to split.  Same double operations in the same order as the
-// sequential form, so the same bits.
-__global__ void __launch_bounds__ (1024)
-k_vopt_dp (const double* __restrict__ gs, const double* __restrict__ gssq,
-    const double* __restrict__ gcnt, int bins, int* __restrict__ split)
+// Row j of the programme (bspline_initialize_mi_hist_vopt,
+// src/register/joint_histogram.cxx:277-301): for every key i, the least
+// prev[k] + vopt_err(k + 1, i) over k < i, a tie going to the later k, with
+// DBL_MAX and split 0 where no candidate is <= DBL_MAX; row 0 (prev null)
+// is vopt_err(0, i).  One wave per key: lane l takes k = l, l + 64, ... in
+// increasing order with the sequential rule, then the lanes' (value, k)
+// pairs are combined by the least value, equal values by the larger k --
+// the pair the sequential loop ends with, since every candidate is formed
+// by the same double operations.
+__global__ void __launch_bounds__ (256)
+k_vopt_row (const double* __restrict__ s, const double* __restrict__ ssq,
+    const double* __restrict__ cnt, const double* __restrict__ prev, double* __restrict__ cur,
+    int* __restrict__ split)
 {
-    __shared__ double s[VOPT_KEYS], ssq[VOPT_KEYS], cnt[VOPT_KEYS], err[2][VOPT_KEYS];
-    const int i = threadIdx.x;
-    if (i < VOPT_KEYS) {
-        s[i] = gs[i];
-        ssq[i] = gssq[i];
-        cnt[i] = gcnt[i];
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= VOPT_KEYS) return;
+    if (!prev) {
+        if (lane == 0) cur[i] = vopt_err (0, i, s, ssq, cnt);
+        return;
     }
-    __syncthreads ();
-    if (i < VOPT_KEYS) err[0][i] = vopt_err (0, i, s, ssq, cnt);
-    for (int j = 1; j < bins; j++) {
-        __syncthreads ();
-        const double* prev = err[(j - 1) & 1];
-        if (i < VOPT_KEYS) {
-            double best = DBL_MAX;
-            int arg = 0;
-            for (int k = 0; k < i; k++) {
-                const double c = prev[k] + vopt_err (k + 1, i, s, ssq, cnt);
-                if (c <= best) {
-                    best = c;
-                    arg = k;
-                }
-            }
-            err[j & 1][i] = best;
-            split[j * VOPT_KEYS + i] = arg;
+    double best = DBL_MAX;
+    int arg = -1;
+    for (int k = lane; k < i; k += 64) {
+        const double c = prev[k] + vopt_err (k + 1, i, s, ssq, cnt);
+        if (c <= best) {
+            best = c;
+            arg = k;
         }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double ob = __shfl_xor (best, o);
+        const int oa = __shfl_xor (arg, o);
+        if (ob < best || (ob == best && oa > arg)) {
+            best = ob;
+            arg = oa;
+        }
+    }
+    if (lane == 0) {
+        cur[i] = best;
+        split[i] = arg < 0 ? 0 : arg;
     }
 }
 
@@ -206,10 +213,17 @@ launch_vopt_keys (const float* img, int64_t n, float offset, float delta, unsign
 }
 
 hipError_t
-launch_vopt_dp (const double* s, const double* ssq, const double* cnt, int bins, int* split,
-    hipStream_t st)
+launch_vopt_dp (const double* s, const double* ssq, const double* cnt, int bins, double* err,
+    int* split, hipStream_t st)
 {
-    hipLaunchKernelGGL (k_vopt_dp, dim3 (1), dim3 (1024), 0, st, s, ssq, cnt, bins, split);
+    const dim3 grid ((VOPT_KEYS + 3) / 4);
+    hipLaunchKernelGGL (k_vopt_row, grid, dim3 (256), 0, st, s, ssq, cnt, nullptr, err, split);
+    for (int j = 1; j < bins; j++) {
+        const double* prev = err + (size_t) ((j - 1) & 1) * VOPT_KEYS;
+        double* cur = err + (size_t) (j & 1) * VOPT_KEYS;
+        hipLaunchKernelGGL (k_vopt_row, grid, dim3 (256), 0, st, s, ssq, cnt, prev, cur,
+            split + (size_t) j * VOPT_KEYS);
+    }
     return hipGetLastError ();
 }
 
