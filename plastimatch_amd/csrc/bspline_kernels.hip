@@ -1563,39 +1563,53 @@ exact_displacement (float out[3], const float* bx, const float* by, const float*
 // Fixed-point PV weights (units of 2^-32): q = floor(w*2^32 + 0.5), the
 // remainder to the first largest weight, so each voxel adds exactly 2^32.
 // For a weight 0 <= w < 1, x = w * 2^32 is exact in float and below 2^32,
-// floor(x) and x - floor(x) are exact, so q = floor(x) + (frac >= 1/2) is
-// the double expression's value and fits 32 bits; w = 1 (a corner on a
-// voxel centre: the other seven weights are 0) is 2^32.  The eight q then
-// sum to 2^32 - r with |r| <= 4, so the 32-bit sum, negated, is r.
-__device__ __forceinline__ void
-pv_quantize (const float w[8], long long q[8])
+// its truncation is floor(x) and its fraction x - floor(x) (v_fract) is
+// exact, so q = floor(x) + (frac >= 1/2) is the double expression's value
+// and fits 32 bits.  The eight q then sum to
+// 2^32 - r with |r| <= 4, so the 32-bit sum, negated, is r, and the first
+// largest q plus r still fits 32 bits.  A weight of 1 (a corner on a voxel
+// centre; the other seven are then 0) is 2^32: returned as true, with that
+// corner in amax and the q left meaningless.
+__device__ __forceinline__ bool
+pv_quantize (const float w[8], uint32_t q[8], int& amax)
 {
-    uint32_t q32[8];
     uint32_t sum = 0u;
-    int amax = 0;
+    amax = 0;
     float wmax = w[0];
-    bool one = false;
 #pragma unroll
     for (int n = 0; n < 8; n++) {
         const float x = w[n] * 4294967296.0f;
-        const float fl = floorf (x);
-        // (the clamp keeps w = 1's conversion in range; its q is set below)
-        const uint32_t v = (uint32_t) fminf (fl, 4294967040.0f) + (x - fl >= 0.5f ? 1u : 0u);
-        q32[n] = w[n] == 1.0f ? 0u : v;
-        one = one || w[n] == 1.0f;
-        sum += q32[n];
+        // (the clamp keeps w = 1's conversion in range)
+        q[n] = (uint32_t) fminf (x, 4294967040.0f) + (__builtin_amdgcn_fractf (x) >= 0.5f ? 1u : 0u);
+        sum += q[n];
         if (n > 0 && w[n] > wmax) {
             wmax = w[n];
             amax = n;
         }
     }
-    const int r = (int) (0u - sum);
+    const uint32_t r = 0u - sum;
 #pragma unroll
-    for (int n = 0; n < 8; n++) {
-        long long v = (long long) q32[n];
-        if (n == amax) v = one ? (1ll << 32) : v + r;
-        q[n] = v;
+    for (int n = 0; n < 8; n++)
+        if (n == amax) q[n] += r;
+    return wmax == 1.0f;
+}
+
+// The corners that fall in corner 0's bin merged into q[0] (their own q
+// set to 0), in 32 bits: the merged value M lies in [0, 2^32], and the
+// 32-bit sum m is M mod 2^32, so M = 2^32 exactly where m is 0 and some
+// merged q is not (the eight q sum to 2^32).  Returns M.
+__device__ __forceinline__ unsigned long long
+pv_merge (uint32_t q[8], const int b[8])
+{
+    uint32_t m = q[0], any = q[0];
+#pragma unroll
+    for (int n = 1; n < 8; n++) {
+        const uint32_t v = b[n] == b[0] ? q[n] : 0u;
+        m += v;
+        any |= v;
+        q[n] -= v;
     }
+    return (m == 0u && any != 0u) ? (1ull << 32) : (unsigned long long) m;
 }
 
 // Pass 1: PV-8 joint histogram.  Follows the pass-1 loop of
@@ -1698,20 +1712,27 @@ void k_mi_hist (KGeom g, HGeom h, const uint8_t* __restrict__ fbin,
             w[5] = l2[0] * l1[1] * l2[2];
             w[6] = l1[0] * l2[1] * l2[2];
             w[7] = l2[0] * l2[1] * l2[2];
-            long long q[8];
-            pv_quantize (w, q);
-            // merge corners that fall in the bin of corner 0
-#pragma unroll
-            for (int n = 1; n < 8; n++)
-                if (b[n] == b[0]) { q[0] += q[n]; q[n] = 0; }
+            uint32_t q[8];
+            int amax;
+            const bool one = pv_quantize (w, q, amax);
             // (the fixed histogram is not counted here: a voxel adds exactly
             // 2^32 to its fixed bin's row, so k_mi_score reads the counts off
             // the row sums)
             const int row = fb * h.mbins;
-            if (act) atomicAdd (&s_j[row + b[0]], (unsigned long long) q[0]);
+            if (one) {
+                // a corner on a voxel centre takes the whole voxel
+                int bo = b[0];
+#pragma unroll
+                for (int n = 1; n < 8; n++)
+                    if (n == amax) bo = b[n];
+                if (act) atomicAdd (&s_j[row + bo], 1ull << 32);
+            }
+            act = act && !one;
+            const unsigned long long q0 = pv_merge (q, b);
+            if (act) atomicAdd (&s_j[row + b[0]], q0);
 #pragma unroll
             for (int n = 1; n < 8; n++)
-                if (act && q[n] != 0) atomicAdd (&s_j[row + b[n]], (unsigned long long) q[n]);
+                if (act && q[n] != 0u) atomicAdd (&s_j[row + b[n]], (unsigned long long) q[n]);
         }
     }
     __syncthreads ();

@@ -1,39 +1,79 @@
 """The MI histogram kernel quantizes each partial-volume weight w in [0, 1]
 with float operations and 32-bit integers (plastimatch_amd/csrc/
-bspline_kernels.hip, pv_quantize): x = w * 2^32 in float, q =
-floor(x) + (x - floor(x) >= 1/2), w = 1 as 2^32, and the remainder r =
-2^32 - sum(q) from the eight q's 32-bit sum, negated -- instead of the
-double expression floor(w * 2^32 + 0.5) and the 64-bit sum of the oracle's
-orc_pv_quantize.  This restates the kernel's steps in numpy (float32 and
-uint32 arithmetic, as the kernel's registers hold them) and checks them
-against the oracle on random trilinear weight vectors and on every
-exponent a weight can have, on the CPU."""
+bspline_kernels.hip, pv_quantize): x = w * 2^32 in float, q = trunc(x) +
+(fract(x) >= 1/2), the remainder r = 2^32 - sum(q) from the eight q's 32-bit
+sum, negated, given to the first largest weight, and a weight of 1 as the
+whole voxel on that corner -- instead of the double expression
+floor(w * 2^32 + 0.5) and the 64-bit sum of the oracle's orc_pv_quantize.
+It then merges the corners that share corner 0's bin in 32 bits
+(pv_merge), recovering a merged value of exactly 2^32 from the wrapped sum.
+This restates the kernel's steps in numpy (float32 and uint32 arithmetic,
+as the kernel's registers hold them) and checks them against the oracle on
+random trilinear weight vectors, on every exponent a weight can have, and
+bin for bin on the contributions a voxel adds, on the CPU."""
 import numpy as np
 
 import oracle
 
 
 def pv_fixed32(w):
-    """The kernel's per-weight rule: q in 32 bits (0 for w = 1)."""
+    """The kernel's per-weight rule: q in 32 bits (meaningless for w = 1)."""
     w = np.asarray(w, np.float32)
     x = (w * np.float32(4294967296.0)).astype(np.float32)
-    fl = np.floor(x).astype(np.float32)
-    frac = (x - fl).astype(np.float32)
-    v = np.minimum(fl, np.float32(4294967040.0)).astype(np.uint64) + (frac >= np.float32(0.5))
-    return np.where(w == np.float32(1.0), 0, v).astype(np.uint32)
+    frac = (x - np.floor(x)).astype(np.float32)
+    v = np.minimum(x, np.float32(4294967040.0)).astype(np.uint64) + (frac >= np.float32(0.5))
+    return v.astype(np.uint32)
 
 
 def pv_quantize_kernel(w):
-    """The kernel's pv_quantize: 32-bit weights, the negated 32-bit sum as
-    the remainder, given to the first largest weight (2^32 where w = 1)."""
+    """The kernel's pv_quantize: (one, amax, q) with q the 32-bit weights,
+    the negated 32-bit sum added to the first largest."""
     w = np.asarray(w, np.float32)
-    q32 = pv_fixed32(w)
-    s = np.uint32(int(q32.astype(np.uint64).sum()) & 0xffffffff)
-    r = np.int64(np.int32(np.uint32((0 - int(s)) & 0xffffffff)))
-    q = q32.astype(np.int64)
+    q = pv_fixed32(w)
+    s = int(q.astype(np.uint64).sum()) & 0xffffffff
+    r = (0 - s) & 0xffffffff
     amax = int(np.argmax(w))                            # the first largest
-    q[amax] = (1 << 32) if np.any(w == np.float32(1.0)) else q[amax] + r
-    return q
+    q = q.copy()
+    q[amax] = np.uint32((int(q[amax]) + r) & 0xffffffff)
+    return bool(w[amax] == np.float32(1.0)), amax, q
+
+
+def kernel_contributions(w, b):
+    """What the kernel adds to each bin for one voxel with corner bins b:
+    pv_quantize, the one-corner case, pv_merge in 32 bits and the atomics."""
+    one, amax, q = pv_quantize_kernel(w)
+    add = {}
+    if one:
+        add[b[amax]] = add.get(b[amax], 0) + (1 << 32)
+        return add
+    q = [int(v) for v in q]
+    m, anyb = q[0], q[0]
+    for n in range(1, 8):
+        v = q[n] if b[n] == b[0] else 0
+        m = (m + v) & 0xffffffff
+        anyb |= v
+        q[n] -= v
+    add[b[0]] = add.get(b[0], 0) + ((1 << 32) if (m == 0 and anyb != 0) else m)
+    for n in range(1, 8):
+        if q[n] != 0:
+            add[b[n]] = add.get(b[n], 0) + q[n]
+    return add
+
+
+def oracle_contributions(w, b):
+    """The same from the oracle's 64-bit weights, corner by corner."""
+    add = {}
+    for n, v in enumerate(oracle.pv_quantize(w)):
+        add[b[n]] = add.get(b[n], 0) + int(v)
+    return {k: v for k, v in add.items() if v != 0}
+
+
+def trilinear_weights(f):
+    l2 = np.asarray(f, np.float32)
+    l1 = (np.float32(1) - l2).astype(np.float32)
+    return np.array([l1[0] * l1[1] * l1[2], l2[0] * l1[1] * l1[2], l1[0] * l2[1] * l1[2],
+                     l2[0] * l2[1] * l1[2], l1[0] * l1[1] * l2[2], l2[0] * l1[1] * l2[2],
+                     l1[0] * l2[1] * l2[2], l2[0] * l2[1] * l2[2]], np.float32)
 
 
 def test_kernel_rule_matches_oracle_weights():
@@ -47,14 +87,47 @@ def test_kernel_rule_matches_oracle_weights():
             f[rng.integers(3)] = np.float32(0.0)         # a clamped axis
         if t % 500 == 0:
             f[:] = np.float32(0.0)                        # every weight on corner 0
-        l2 = f
-        l1 = (np.float32(1) - f).astype(np.float32)
-        w = np.array([l1[0] * l1[1] * l1[2], l2[0] * l1[1] * l1[2], l1[0] * l2[1] * l1[2],
-                      l2[0] * l2[1] * l1[2], l1[0] * l1[1] * l2[2], l2[0] * l1[1] * l2[2],
-                      l1[0] * l2[1] * l2[2], l2[0] * l2[1] * l2[2]], np.float32)
-        q = pv_quantize_kernel(w)
-        assert np.array_equal(q, oracle.pv_quantize(w)), w
-        assert int(q.sum()) == 1 << 32
+        w = trilinear_weights(f)
+        one, amax, q = pv_quantize_kernel(w)
+        ref = oracle.pv_quantize(w)
+        if one:
+            assert ref[amax] == 1 << 32 and int(ref.sum()) == 1 << 32
+        else:
+            assert np.array_equal(q.astype(np.int64), ref), w
+            assert int(q.astype(np.int64).sum()) == 1 << 32
+
+
+def test_kernel_merge_adds_the_oracle_weights():
+    """Bin for bin, a voxel's contributions through the kernel's 32-bit
+    quantize and merge equal the oracle's 64-bit weights summed per bin:
+    random corner bins from a few values (so corners collide), a corner on
+    a voxel centre, one or two axes on a voxel centre (four or six zero
+    weights) with the other corners in corner 0's bin -- the merged 2^32
+    that wraps to 0 in 32 bits -- and all eight corners in one bin."""
+    rng = np.random.default_rng(8)
+    cases = 0
+    for t in range(6000):
+        f = rng.random(3).astype(np.float32)
+        kind = t % 6
+        if kind == 1:
+            f[rng.integers(3)] = np.float32(0.0)
+        elif kind == 2:
+            f[rng.choice(3, 2, replace=False)] = np.float32(0.0)
+        elif kind == 3:
+            f[:] = np.float32(0.0)
+        w = trilinear_weights(f)
+        b = list(rng.integers(0, 3, 8))
+        if kind in (1, 2) or t % 7 == 0:
+            # the corners that carry weight all in corner 0's bin
+            nz = [n for n in range(8) if w[n] != 0]
+            for n in nz:
+                b[n] = b[0]
+        if kind == 4:
+            b = [b[0]] * 8
+        got = {k: v for k, v in kernel_contributions(w, b).items() if v != 0}
+        assert got == oracle_contributions(w, b), (w, b)
+        cases += 1
+    assert cases == 6000
 
 
 def test_kernel_rule_every_exponent():
