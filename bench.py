@@ -16,8 +16,9 @@ The same line carries the batched multi-atlas comparison ("batched"): 16
 pairs with edges drawn from 96..256 per axis (10-voxel grid, N(0, 3 mm)
 fields), images and moving gradient stored as BF16, evaluated (a) as one
 batched evaluation replayed from a hipGraph and (b) by looping the
-single-pair entry point over the 16 pairs on one stream; both report
-aggregate pair-evaluations per second.  The "adjoint" object covers the
+single-pair entry point over the 16 pairs on the caller's stream (and, as
+a second figure, on each context's own stream, where the pairs' launches
+may overlap); both report aggregate pair-evaluations per second.  The "adjoint" object covers the
 differentiable path (torch.autograd.Function) on 192x192x96 with an 8-voxel
 grid: forward+backward time and memory of its two modes and a central
 finite-difference check of the adjoint.  "warp_regularize" times the
@@ -176,27 +177,37 @@ def batched_leg(args, dev, want_errors):
     d_grads = [torch.zeros(c.size, dtype=torch.float64, device=dev) for c in coeffs]
     d_score = torch.zeros(BATCH_PAIRS, dtype=torch.float64, device=dev)
     d_nvox = torch.zeros(BATCH_PAIRS, dtype=torch.int64, device=dev)
-    stream = torch.cuda.current_stream().cuda_stream
+    # the caller's stream (a stream handle of its own: the null handle would
+    # select each context's own stream)
+    caller = torch.cuda.Stream()
+    stream = caller.cuda_stream
     steps = max(10, args.steps // 10)
     warm = max(2, args.warmup // 10)
 
-    # (b) looping the single-pair entry point, each stage autotuned on its own
+    # (b) looping the single-pair entry point, each stage autotuned on its
+    # own: on the caller's stream (the loop runs the pairs one after the
+    # other), and on each context's own stream (the pairs' launches may
+    # overlap, up to the hardware queues)
     for st, c in zip(stages, d_coeffs):
         st.autotune("mse", "ref_nn", c, reps=3)
 
-    def loop_once():
+    def loop_once(own_streams=False):
         for i, st in enumerate(stages):
             st.score_device(d_coeffs[i], d_score[i:i + 1], d_grads[i], d_nvox[i:i + 1], "mse", "ref_nn",
-                            stream=stream)
+                            stream=None if own_streams else stream)
 
-    for _ in range(warm):
-        loop_once()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        loop_once()
-    torch.cuda.synchronize()
-    t_loop = (time.perf_counter() - t0) / steps
+    def time_loop(own_streams):
+        for _ in range(warm):
+            loop_once(own_streams)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            loop_once(own_streams)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_loop = time_loop(False)
+    t_loop_ctx = time_loop(True)
 
     # (a) one batched evaluation, autotuned, captured once and replayed
     batch = BsplineBatch(stages)
@@ -232,9 +243,11 @@ def batched_leg(args, dev, want_errors):
         "workgroups_per_cu": wgs,
         "batched_graph_ms_per_eval": t_batch * 1e3,
         "looped_single_ms_per_eval": t_loop * 1e3,
+        "looped_single_context_streams_ms_per_eval": t_loop_ctx * 1e3,
         "batched_pair_evals_per_s": BATCH_PAIRS / t_batch,
         "looped_pair_evals_per_s": BATCH_PAIRS / t_loop,
         "speedup_batched_vs_looped": t_loop / t_batch,
+        "speedup_batched_vs_looped_context_streams": t_loop_ctx / t_batch,
         "batched_tile_kernel_ms": tile_ms,
         "batched_gb_per_s": bytes_per_eval / t_batch / 1e9,
         "tile_kernel_gb_per_s": 10 * nvox / (tile_ms * 1e-3) / 1e9,

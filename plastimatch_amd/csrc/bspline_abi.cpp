@@ -191,6 +191,7 @@ struct plm_hip_ctx {
     double* d_reg_part = nullptr;           // regularization scratch (first use)
     double* d_reg_cond = nullptr;
     int* d_tile_count = nullptr;
+    unsigned long long* d_tile_ctr = nullptr;   // the tile kernel's per-XCD counters
     plmhip::PairDev* d_pair = nullptr;      // this context as a one-pair table
     plmhip::Plan plan {};
     unsigned long long* d_hist = nullptr;   // joint bins then fixed bins
@@ -231,6 +232,7 @@ struct plm_hip_batch {
     plmhip::Tune tune {2, 0, 0};
     double* d_part_score = nullptr;
     int* d_part_count = nullptr;
+    unsigned long long* d_tile_ctr = nullptr;
     bool buffers_set = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -245,7 +247,7 @@ free_ctx (plm_hip_ctx* c)
     c->timer.destroy ();
     plmhip::lb_workspace_free (c->lb_ws);
     void* bufs[] = {c->d_fixed, c->d_moving, c->d_mpair, c->d_mgrad, c->d_blut, c->d_tbase, c->d_cond,
-        c->d_tile_score, c->d_tile_count, c->d_pair, c->d_hist,
+        c->d_tile_score, c->d_tile_count, c->d_tile_ctr, c->d_pair, c->d_hist,
         c->d_table, c->d_nvox_f, c->d_fbin, c->d_mbin, c->d_mquad, c->d_ptab, c->d_coeff, c->d_grad,
         c->d_score, c->d_nvox,
         c->d_hist_out, c->d_reg_part, c->d_reg_cond, c->d_fmask, c->d_fmask1, c->d_mroi};
@@ -593,10 +595,14 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
 
     // slot-major condense buffer; slots no tile owns stay zero for good
     CTX_TRY (alloc (&c->d_cond, (size_t) knots * 192, "hipMalloc(cond)"));
-    // per-workgroup partials of the persistent tile kernel: at most one
-    // workgroup per tile, rounded up to a multiple of 8
+    // partials of the persistent tile kernel: the score per tile, the
+    // overlap count per workgroup (at most one workgroup per tile, rounded
+    // up to a multiple of 8); its tile counters, zero from here on
     CTX_TRY (alloc (&c->d_tile_score, tiles + 8, "hipMalloc(tile_score)"));
     CTX_TRY (alloc (&c->d_tile_count, tiles + 8, "hipMalloc(tile_count)"));
+    CTX_TRY (alloc (&c->d_tile_ctr, plmhip::TILE_CTR_WORDS, "hipMalloc(tile counters)"));
+    e = hipMemset (c->d_tile_ctr, 0, sizeof (unsigned long long) * plmhip::TILE_CTR_WORDS);
+    if (e != hipSuccess) { free_ctx (c); return hip_fail (e, "hipMemset(tile counters)"); }
     const size_t nbins = (size_t) h.fbins * h.mbins + h.fbins;
     CTX_TRY (alloc (&c->d_hist, nbins, "hipMalloc(hist)"));
     CTX_TRY (alloc (&c->d_table, (size_t) h.fbins * h.mbins, "hipMalloc(table)"));
@@ -660,6 +666,7 @@ plm_hip_bspline_create (const plm_bsp_geom* geom, const plm_img_desc* fixed,
         pl.blut = c->d_blut;
         pl.part_score = c->d_tile_score;
         pl.part_count = c->d_tile_count;
+        pl.tile_ctr = c->d_tile_ctr;
     }
 #undef CTX_TRY
     e = plmhip::launch_zero (c->d_cond, (int64_t) knots * 192, c->stream);
@@ -1358,7 +1365,7 @@ free_batch (plm_hip_batch* b)
 {
     drop_graph (b);
     b->timer.destroy ();
-    void* bufs[] = {b->d_pairs, b->d_part_score, b->d_part_count};
+    void* bufs[] = {b->d_pairs, b->d_part_score, b->d_part_count, b->d_tile_ctr};
     for (void* p : bufs) if (p) (void) hipFree (p);
     if (b->stream) (void) hipStreamDestroy (b->stream);
     delete b;
@@ -1435,7 +1442,10 @@ plm_hip_batch_create (plm_hip_ctx* const* ctxs, int n, plm_hip_batch** out)
     BATCH_TRY (alloc (&b->d_pairs, (size_t) n, "hipMalloc(pairs)"));
     BATCH_TRY (alloc (&b->d_part_score, (size_t) n * (tiles + 8), "hipMalloc(partials)"));
     BATCH_TRY (alloc (&b->d_part_count, (size_t) n * (tiles + 8), "hipMalloc(partials)"));
+    BATCH_TRY (alloc (&b->d_tile_ctr, plmhip::TILE_CTR_WORDS, "hipMalloc(tile counters)"));
 #undef BATCH_TRY
+    e = hipMemset (b->d_tile_ctr, 0, sizeof (unsigned long long) * plmhip::TILE_CTR_WORDS);
+    if (e != hipSuccess) { free_batch (b); return hip_fail (e, "hipMemset(tile counters)"); }
     e = hipMemcpy (b->d_pairs, b->h_pairs.data (), sizeof (plmhip::PairDev) * n,
         hipMemcpyHostToDevice);
     if (e != hipSuccess) { free_batch (b); return hip_fail (e, "hipMemcpy(pairs)"); }
@@ -1448,6 +1458,7 @@ plm_hip_batch_create (plm_hip_ctx* const* ctxs, int n, plm_hip_batch** out)
     pl.blut = ctxs[0]->d_blut;      // equal tile edges, equal LUTs
     pl.part_score = b->d_part_score;
     pl.part_count = b->d_part_count;
+    pl.tile_ctr = b->d_tile_ctr;
     *out = b;
     return PLM_HIP_OK;
 }

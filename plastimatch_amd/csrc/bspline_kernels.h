@@ -183,12 +183,25 @@ struct PairDev {
     int blk0;                // first reduce workgroup of the pair
 };
 
+// The tile kernel's per-XCD tile counters: XCD x's 64-bit word at
+// ctr[TILE_CTR_STRIDE * x] (each on a 128-byte line of its own), zero
+// between launches (every launch leaves them zero).  One set per plan;
+// launches that share a set must not overlap.
+constexpr int TILE_CTR_STRIDE = 16;
+constexpr int TILE_CTR_WORDS = 8 * TILE_CTR_STRIDE;
+
 struct TileQueue {
     int total_tiles;
     int tiles_per_xcd;       // ceil(total_tiles / 8)
     int npairs;
     int pad;
+    unsigned long long* ctr; // TILE_CTR_WORDS counters (launches with counted tiles)
 };
+
+// A batched launch or MI pass 2 takes its tiles from the counters when its
+// workgroups would otherwise walk at least this many tiles each at a
+// fixed stride.
+constexpr int TILE_DYNAMIC_ROUNDS = 12;
 
 // Per-evaluation outputs of pair 0 that override its table entry (the
 // single-pair entry points pass them with every call); null fields fall
@@ -211,8 +224,9 @@ struct Plan {
     int total_tiles;
     int reduce_blocks;       // sum over pairs of ceil(3 * num_knots / REDUCE_BLOCK)
     const float* blut;       // basis LUTs of the shared tile edges
-    double* part_score;      // per (pair, workgroup) partials, npairs * (total_tiles + 8)
-    int* part_count;
+    double* part_score;      // per tile of the flattened queue (score kinds)
+    int* part_count;         // per (pair, workgroup) overlap counts, npairs * (total_tiles + 8)
+    unsigned long long* tile_ctr;   // TILE_CTR_WORDS, zeroed once at allocation
 };
 
 constexpr int REDUCE_BLOCK = 64;     // gradient components per reduce workgroup

@@ -18,9 +18,11 @@
 //   5. the 192 slot values go to a slot-major buffer cond[slot][knot][c]
 //      (one writer per entry, no atomics on floats), and a reduce kernel
 //      sums each gradient component's 64 slots in FP64 in a fixed order.
-// The overlap count N and the score are fixed-order sums of per-workgroup
-// partials (integer and FP64); there are no atomics on the MSE path, so the
-// results are bitwise reproducible run to run.
+// The score is a fixed-order sum of per-tile FP64 partials and the overlap
+// count N a sum of per-workgroup integer partials; the only atomics on the
+// MSE path are integer ones (the tile counters, the counts), so the
+// results are bitwise reproducible run to run and do not depend on which
+// workgroup took which tile.
 //
 // The MI histogram pass instead evaluates the displacement in the
 // reference's k-j-i order without FMA so its partial-volume weights are
@@ -770,6 +772,26 @@ block_sum (const TileLds& L, double& s, int& n)
     }
 }
 
+// Sums over a wavefront by an xor butterfly: every lane ends with the same
+// value (each step adds a pair of equal partner sums, and addition is
+// commutative), formed in an order that depends on nothing but the lanes'
+// values.
+__device__ __forceinline__ double
+wave_sum (double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor (v, off);
+    return v;
+}
+
+__device__ __forceinline__ long long
+wave_sum_ll (long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor (v, off);
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // Tile kernel.  KIND 0: MSE with the nearest-voxel moving gradient
 // (bspline_score_g_mse, src/register/bspline_mse.cxx:519-720); KIND 1: MSE
@@ -804,7 +826,12 @@ advance_pair (const PairDev* __restrict__ pairs, int npairs, int p, int tile)
 // from the moving image itself (four 8-byte loads a voxel).
 constexpr int TILE_PLAIN = 16;
 
-template <int KINDX, typename TI, int V, int G, bool BATCH>
+// The tile kernel's static LDS at most: per-pair score sums and overlap
+// counts, the per-wave tile scores, the next tile (rounded up).
+constexpr size_t TILE_STATIC_LDS = MAX_BATCH * (sizeof (double) + sizeof (int))
+    + (TILE_THREADS / 64) * sizeof (double) + 16;
+
+template <int KINDX, typename TI, int V, int G, bool BATCH, bool DYN>
 __global__ __launch_bounds__ (TILE_THREADS)
 void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __restrict__ coeff0,
     const float* __restrict__ blut, double* __restrict__ part_score,
@@ -830,19 +857,47 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
     extern __shared__ float4 sm4[];
     __shared__ double s_ps[BATCH ? MAX_BATCH : 1];  // per-pair sums of this workgroup
     __shared__ int s_pn[BATCH ? MAX_BATCH : 1];
+    __shared__ double s_ws[TILE_THREADS / 64];      // counted tiles: the last tile's score, per wave
+    __shared__ int s_next;                          // counted tiles: the workgroup's next tile
     const int t = threadIdx.x;
     if (SCORE && t < npairs) {
         s_ps[t] = 0.0;
         s_pn[t] = 0;
     }
-    // Persistent workgroups: those on XCD x = blockIdx % 8 walk that XCD's
-    // contiguous run of the (flattened) tile queue together, so
-    // neighbouring tiles, which share image lines and knots, meet in the
-    // same L2.
+    // Persistent workgroups: those on XCD x = blockIdx % 8 take the tiles
+    // of that XCD's contiguous run of the (flattened) tile queue, so
+    // neighbouring tiles, which share image lines and knots, meet in its
+    // L2.  A workgroup's first tile is run start + its index on the XCD.
+    // On a short run (a few tiles per workgroup) the later ones follow at a
+    // fixed stride.  On a long run (a batch of large pairs, MI pass 2 at
+    // 512 x 512 x 160: tens of tiles per workgroup) a fixed stride lets the
+    // workgroups drift apart, and the L2 then no longer holds their
+    // scattered footprints (at 16 pairs of 232 x 198 x 178, 1.5 x the HBM
+    // fetch of single-pair launches): there (DYN, built for batched
+    // launches and MI pass 2) the later tiles come from the XCD's counter in
+    // queue order, taken one tile ahead (its low 32 bits
+    // count the tiles taken past the first wpx, its high 32 bits the
+    // workgroups that have finished; the last to finish clears it).  The
+    // counter's device-scope atomic, whose return the tile's loads wait
+    // behind, costs a short run more than the drift it saves.  Which
+    // workgroup takes a counted tile matters to no result: the score goes
+    // out per tile (at a fixed stride, per (pair, workgroup)) and the
+    // overlap counts are integers.
     const int wpx = gridDim.x >> 3;
     const int tbeg = (blockIdx.x & 7) * q.tiles_per_xcd;
     const int tend = min (tbeg + q.tiles_per_xcd, q.total_tiles);
+    unsigned long long* const ctr = q.ctr + TILE_CTR_STRIDE * (blockIdx.x & 7);
     int tile = tbeg + (blockIdx.x >> 3);
+    // the next counted tile, clamped to the run (past it: tend)
+    const unsigned counted = tend - tbeg > wpx ? (unsigned) (tend - tbeg - wpx) : 0u;
+    auto take = [&] () -> int {
+        const unsigned k = (unsigned) (uint32_t) atomicAdd (ctr, 1ull);
+        return k < counted ? tbeg + wpx + (int) k : tend;
+    };
+    constexpr bool dyn = DYN;
+    int fut = tend;     // (thread 0) the tile after the current one, in flight
+    if (dyn && t == 0 && tile < tend) fut = take ();
+    int ptile = -1;     // the last tile whose score partial is pending
 
     // every pair shares the tile edges, so the LDS layout and the LUTs
     // are those of pair 0
@@ -942,11 +997,12 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
 
     // this thread's squared differences and its wave's overlap count over
     // the tiles of the current pair, folded into s_ps/s_pn when the pair
-    // changes
+    // changes (counted tiles: the squared differences over the current
+    // tile only, sent out per tile)
     double score = 0.0;
     int count = 0;
     int rot = 0;
-    for (; tile < tend; tile += wpx) {
+    while (tile < tend) {
         const TilePos P = nq;
         if (SCORE && np != p) {
             // the workgroup moved on to the next pair: fold the last one
@@ -964,9 +1020,13 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         const PairDev& pd = pairs[p];
         // the tile's moving-index frame (wave-uniform: scalar loads)
         const TileBase tb = pd.tbase[tile - pd.tile0];
-        const int next = tile + wpx;
-        const bool has_next = next < tend;
-        if (has_next) {
+        int next = tile + wpx;
+        if (dyn) {
+            if (t == 0) {
+                s_next = fut;
+                if (fut < tend) fut = take ();
+            }
+        } else if (next < tend) {
             np = advance_pair (pairs, npairs, p, next);
             nq = tile_pos (pairs[np].g, next - pairs[np].tile0);
         }
@@ -987,6 +1047,17 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         rot += 64;
         const int tn = (tt + rot) & (TILE_THREADS - 1);
         __syncthreads ();       // the previous tile is done with coef, t1, s2, red
+        if (dyn) {
+            if (SCORE && t == 0 && ptile >= 0)
+                part_score[ptile] = ((s_ws[0] + s_ws[1]) + s_ws[2]) + s_ws[3];
+            // (the same in every lane: kept scalar, as the stride's is)
+            next = __builtin_amdgcn_readfirstlane (s_next);
+            if (next < tend) {
+                np = advance_pair (pairs, npairs, p, next);
+                nq = tile_pos (pairs[np].g, next - pairs[np].tile0);
+            }
+        }
+        const bool has_next = next < tend;
         tile_stage1<V, MODE> (L, S, g, cpre, tr);
         __syncthreads ();
         if (MODE != OFFSETS && tn < 64 && has_next)
@@ -1382,10 +1453,20 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
             voxels (std::false_type {});
         }
         if (has_next) fetch_rows (np, nq, tn);
+        if (SCORE && dyn) {
+            // a counted tile's score: each thread's voxels in their fixed
+            // order, the lanes by the butterfly, the waves in order (thread
+            // 0 writes it after the next barrier)
+            const double ws = wave_sum (score);
+            if ((t & 63) == 0) s_ws[t >> 6] = ws;
+            score = 0.0;
+            ptile = tile;
+        }
         if (GRAD) {
             __syncthreads ();
             tile_condense<V, DERIV> (L, S, g, P.knot0, pd.cond, tr);
         }
+        tile = next;
     }
     if (SCORE) {
         // fold the last pair: each thread's tiles in their fixed order, then
@@ -1398,14 +1479,24 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         }
     }
     }
-    // one partial per (pair, workgroup); pairs this workgroup did not touch
-    // get zeros
+    // one partial per (pair, workgroup), zeros for the pairs this workgroup
+    // did not touch (counted tiles: the last tile's score partial, and the
+    // counts only)
     if (SCORE) {
         __syncthreads ();
-        if (t < npairs) {
+        if (dyn) {
+            if (t == 0 && ptile >= 0) part_score[ptile] = ((s_ws[0] + s_ws[1]) + s_ws[2]) + s_ws[3];
+        } else if (t < npairs) {
             part_score[t * gridDim.x + blockIdx.x] = s_ps[t];
-            part_count[t * gridDim.x + blockIdx.x] = s_pn[t];
         }
+        if (t < npairs) part_count[t * gridDim.x + blockIdx.x] = s_pn[t];
+    }
+    // done with the counter: the XCD's last workgroup to finish clears it
+    // (every workgroup took its last tile before adding its finish, on the
+    // same word)
+    if (dyn && t == 0) {
+        const unsigned long long old = atomicAdd (ctr, 1ull << 32);
+        if ((int) (old >> 32) == wpx - 1) atomicExch (ctr, 0ull);
     }
 }
 
@@ -1420,31 +1511,16 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
 // For MSE every workgroup also sums its pair's per-workgroup overlap counts
 // (integers, so every workgroup gets the same N with no cross-workgroup
 // traffic), fetched with the slot loads; the pair's first workgroup also
-// sums the score partials in a fixed order and writes score = raw/N
-// (FLT_MAX if N < 1) and N.
+// sums the score partials in a fixed order -- per (pair, workgroup), or
+// with tile_scores (the tile kernel took counted tiles) per tile in tile
+// order -- and writes score = raw/N (FLT_MAX if N < 1) and N.
 
 constexpr int REDUCE_THREADS = 4 * REDUCE_BLOCK;
-
-__device__ __forceinline__ double
-wave_sum (double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor (v, off);
-    return v;
-}
-
-__device__ __forceinline__ long long
-wave_sum_ll (long long v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor (v, off);
-    return v;
-}
 
 template <bool MSE>
 __global__ __launch_bounds__ (REDUCE_THREADS)
 void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __restrict__ part_score,
-    const int* __restrict__ part_count, int nparts, EvalOut out0, int with_grad)
+    const int* __restrict__ part_count, int nparts, int tile_scores, EvalOut out0, int with_grad)
 {
     if (out0.skip && *out0.skip) return;
     __shared__ long long s_n[REDUCE_THREADS / 64];
@@ -1465,7 +1541,7 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
     const float* c = pd.cond + (size_t) (16 * q) * nc + (i < nc ? i : 0);
     double* grad = (out0.grad && p == 0) ? out0.grad : pd.grad;
     // all 16 slot loads are issued before any is consumed, then summed in
-    // slot order; the pair's per-workgroup partials are fetched meanwhile.
+    // slot order; the pair's partials are fetched meanwhile.
     // The loads are unconditional (c stays in range) so nothing serializes
     // them.
     float v[16];
@@ -1484,6 +1560,7 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
         constexpr int CH = 8;
         const int* pc = part_count + (size_t) p * nparts;
         const double* ps = part_score + (size_t) p * nparts;
+        const bool wg_scores = lead && !tile_scores;
         for (int t0 = 0; t0 < nparts; t0 += CH * REDUCE_THREADS) {
             int cnt[CH];
             double s[CH];
@@ -1491,12 +1568,28 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
             for (int k = 0; k < CH; k++) {
                 const int t = t0 + k * REDUCE_THREADS + threadIdx.x;
                 cnt[k] = t < nparts ? pc[t] : 0;
-                s[k] = lead && t < nparts ? ps[t] : 0.0;
+                s[k] = wg_scores && t < nparts ? ps[t] : 0.0;
             }
 #pragma unroll
             for (int k = 0; k < CH; k++) {
                 n += cnt[k];
                 sc += s[k];
+            }
+        }
+        if (lead && tile_scores) {
+            // per-tile score partials (counted tiles), in tile order
+            constexpr int CT = 32;
+            const double* pt = part_score + pd.tile0;
+            const int nt = pd.g.num_tiles;
+            for (int t0 = 0; t0 < nt; t0 += CT * REDUCE_THREADS) {
+                double s[CT];
+#pragma unroll
+                for (int k = 0; k < CT; k++) {
+                    const int t = t0 + k * REDUCE_THREADS + threadIdx.x;
+                    s[k] = t < nt ? pt[t] : 0.0;
+                }
+#pragma unroll
+                for (int k = 0; k < CT; k++) sc += s[k];
             }
         }
     }
@@ -2210,8 +2303,8 @@ tile_lds_bytes (const KGeom& g, int vox_per_thread)
     const int vx = g.vpr[0], vy = g.vpr[1], vz = g.vpr[2];
     const size_t f = 4 * vx + 4 * vy + 8 * vz + 192 + tile_col_floats (vx, vy, vz)
         + tile_dv_floats (vx, tile_slots (vx * vy * vz, vox_per_thread));
-    // + the per-pair partial sums, static in the kernel
-    return f * sizeof (float) + 16 * sizeof (double) + MAX_BATCH * (sizeof (double) + sizeof (int));
+    // + the kernel's static arrays
+    return f * sizeof (float) + 16 * sizeof (double) + TILE_STATIC_LDS;
 }
 
 // Records ev[i] on st when stage timing is on (ev != nullptr).
@@ -2254,7 +2347,7 @@ launch_tiles_vgb (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st
         std::lock_guard<std::mutex> lock (mu);
         if (cached_lds != lds) {
             int n = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor (&n, k_tiles<KIND, TI, V, G, BATCH>,
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor (&n, k_tiles<KIND, TI, V, G, BATCH, false>,
                     TILE_THREADS, lds) != hipSuccess || n < 1)
                 n = 1;
             cached_lds = lds;
@@ -2268,9 +2361,22 @@ launch_tiles_vgb (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st
     q.tiles_per_xcd = (plan.total_tiles + 7) / 8;
     q.npairs = plan.npairs;
     q.pad = 0;
+    q.ctr = plan.tile_ctr;
     const int cus = tune.num_cus > 0 ? tune.num_cus : 8;
     const int per_xcd = std::max (1, std::min (q.tiles_per_xcd, occ * cus / 8));
-    hipLaunchKernelGGL ((k_tiles<KIND, TI, V, G, BATCH>), dim3 (8 * per_xcd), dim3 (TILE_THREADS),
+    // counted tiles on long runs, for the launches built with them (a batch,
+    // MI pass 2); the occupancy above is the fixed-stride build's
+    constexpr bool DYN_BUILT = BATCH || (KIND & 15) == 2;
+    const bool dyn = DYN_BUILT && plan.tile_ctr && q.tiles_per_xcd >= TILE_DYNAMIC_ROUNDS * per_xcd;
+    if constexpr (DYN_BUILT) {
+        if (dyn) {
+            hipLaunchKernelGGL ((k_tiles<KIND, TI, V, G, BATCH, true>), dim3 (8 * per_xcd), dim3 (TILE_THREADS),
+                lds, st, plan.d_pairs, q, ta.coeff0, plan.blut, plan.part_score, plan.part_count, ta.mi,
+                ta.vf, ta.fill, ta.skip);
+            return -8 * per_xcd;
+        }
+    }
+    hipLaunchKernelGGL ((k_tiles<KIND, TI, V, G, BATCH, false>), dim3 (8 * per_xcd), dim3 (TILE_THREADS),
         lds, st, plan.d_pairs, q, ta.coeff0, plan.blut, plan.part_score, plan.part_count, ta.mi,
         ta.vf, ta.fill, ta.skip);
     return 8 * per_xcd;
@@ -2295,7 +2401,7 @@ launch_tiles_v (const Plan& plan, const Tune& tune, int vpt, size_t extra, hipSt
 {
     const KGeom& g = plan.g0;
     // dynamic LDS: the carve-up (+ the MI table), without the static arrays
-    constexpr size_t stat = MAX_BATCH * (sizeof (double) + sizeof (int));
+    constexpr size_t stat = TILE_STATIC_LDS;
     if (vpt == 1)
         return launch_tiles_vg<KIND, TI, V, 1> (plan, tune, tile_lds_bytes (g, 1) - stat + extra,
             st, ta);
@@ -2310,7 +2416,8 @@ launch_tiles_v (const Plan& plan, const Tune& tune, int vpt, size_t extra, hipSt
 // edges 8, 10, 12 and 15 voxels; other vox_per_rgn run the generic build.
 // vox_per_thread falls back to the largest value whose dc/dv slots fit in
 // LDS (1 always fits: the context checked it at creation).  Returns the
-// grid size, the number of per-workgroup score/count partials per pair.
+// grid size, the number of per-workgroup partials per pair, negated when
+// the launch took counted tiles (its score partials are then per tile).
 // (The dynamic LDS passed at launch excludes the kernel's static arrays,
 // which tile_lds_bytes counts for the fit.)
 template <int KIND, typename TI>
@@ -2358,8 +2465,8 @@ launch_mse (const Plan& plan, const Tune& tune, int mode, int storage, const flo
     // the score alone needs one reduce workgroup per pair (single-pair plans)
     const bool grad = mode != MSE_SCORE_ONLY;
     hipLaunchKernelGGL (k_reduce<true>, dim3 (grad ? plan.reduce_blocks : 1), dim3 (REDUCE_THREADS),
-        0, st, plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts, out0,
-        grad ? 1 : 0);
+        0, st, plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts < 0 ? -nparts : nparts,
+        nparts < 0 ? 1 : 0, out0, grad ? 1 : 0);
     LAUNCH_CHECK ();
     return mark (ev, 2, st);
 }
@@ -2371,7 +2478,7 @@ static int
 launch_tiles_plain (const Plan& plan, const Tune& tune, hipStream_t st, const TileArgs& ta)
 {
     const KGeom& g = plan.g0;
-    constexpr size_t stat = MAX_BATCH * (sizeof (double) + sizeof (int));
+    constexpr size_t stat = TILE_STATIC_LDS;
     const size_t lds = tile_lds_bytes (g, 2) - stat;
     const int v = g.vpr[0];
     const bool iso = g.vpr[1] == v && g.vpr[2] == v;
@@ -2395,8 +2502,8 @@ launch_mse_plain (const Plan& plan, const Tune& tune, int mode, const float* coe
     LAUNCH_CHECK ();
     const bool grad = mode != MSE_SCORE_ONLY;
     hipLaunchKernelGGL (k_reduce<true>, dim3 (grad ? plan.reduce_blocks : 1), dim3 (REDUCE_THREADS),
-        0, st, plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts, out0,
-        grad ? 1 : 0);
+        0, st, plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts < 0 ? -nparts : nparts,
+        nparts < 0 ? 1 : 0, out0, grad ? 1 : 0);
     return hipGetLastError ();
 }
 
@@ -2423,7 +2530,8 @@ launch_mse_from_vf (const Plan& plan, const Tune& tune, int storage, const float
     LAUNCH_CHECK ();
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_reduce<true>, dim3 (plan.reduce_blocks), dim3 (REDUCE_THREADS), 0, st,
-        plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts, out0, 1);
+        plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts < 0 ? -nparts : nparts,
+        nparts < 0 ? 1 : 0, out0, 1);
     LAUNCH_CHECK ();
     return mark (ev, 2, st);
 }
@@ -2466,7 +2574,7 @@ launch_mi (const Plan& plan, const Tune& tune, const HGeom& h, int hist_blocks,
     LAUNCH_CHECK ();
     if ((e = mark (ev, 4, st)) != hipSuccess) return e;
     hipLaunchKernelGGL (k_reduce<false>, dim3 (plan.reduce_blocks), dim3 (REDUCE_THREADS), 0, st,
-        plan.d_pairs, plan.npairs, nullptr, nullptr, 0, out0, 1);
+        plan.d_pairs, plan.npairs, nullptr, nullptr, 0, 0, out0, 1);
     LAUNCH_CHECK ();
     return mark (ev, 5, st);
 }

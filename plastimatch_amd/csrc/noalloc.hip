@@ -34,7 +34,7 @@ constexpr int MAX_DEVICES = 64;
 // Byte offsets of the evaluation's buffers in the scratch array (each
 // 256-aligned).
 struct Layout {
-    size_t pair, lut, part_score, part_count, tbase, cond, total;
+    size_t pair, lut, part_score, part_count, ctr, tbase, cond, total;
 };
 
 size_t
@@ -50,10 +50,12 @@ layout (int64_t tiles, int64_t knots)
     size_t o = 0;
     L.pair = o;       o += up256 (sizeof (PairDev));
     L.lut = o;        o += up256 (sizeof (float) * 12 * MAX_VPR);
-    // one partial per workgroup of the persistent grid (at most one per tile,
-    // rounded up to a multiple of 8)
+    // the score per tile; one overlap count per workgroup of the
+    // persistent grid (at most one per tile, rounded up to a multiple of 8);
+    // the tile kernel's counters
     L.part_score = o; o += up256 (sizeof (double) * (size_t) (tiles + 8));
     L.part_count = o; o += up256 (sizeof (int) * (size_t) (tiles + 8));
+    L.ctr = o;        o += up256 (sizeof (unsigned long long) * TILE_CTR_WORDS);
     L.tbase = o;      o += up256 (sizeof (TileBase) * (size_t) tiles);
     L.cond = o;       o += up256 (sizeof (float) * 192 * (size_t) knots);
     L.total = o;
@@ -79,6 +81,9 @@ void k_noalloc_setup (SetupArgs a, unsigned char* base, Layout L)
     TileBase* tb = reinterpret_cast<TileBase*> (base + L.tbase);
     if (i < a.g.num_tiles) tb[i] = tile_base_of (a.g, a.moff, a.mstep, i);
     if (i < a.nlut) reinterpret_cast<float*> (base + L.lut)[i] = a.lut[i];
+    // (each tile launch leaves its counters zero; cleared here all the
+    // same, should an earlier call have stopped short)
+    if (i < TILE_CTR_WORDS) reinterpret_cast<unsigned long long*> (base + L.ctr)[i] = 0ull;
     if (i == 0) {
         PairDev pd {};
         pd.g = a.g;
@@ -201,7 +206,7 @@ plm_hip_bspline_score_noalloc (const plm_bsp_geom* geom, const plm_img_desc* fix
     a.fimg = fixed->data;
     a.mimg = moving->data;
     a.mgrad = grad_mode == PLM_GRAD_REF_NN ? moving_grad->data : nullptr;
-    const int n = (int) std::max<int64_t> (tiles, a.nlut);
+    const int n = (int) std::max<int64_t> (std::max<int64_t> (tiles, a.nlut), TILE_CTR_WORDS);
     hipLaunchKernelGGL (k_noalloc_setup, dim3 ((n + 255) / 256), dim3 (256), 0, st, a, ds.ptr, L);
     e = hipGetLastError ();
     if (e != hipSuccess) return abi_hip_fail (e, "noalloc setup");
@@ -219,6 +224,7 @@ plm_hip_bspline_score_noalloc (const plm_bsp_geom* geom, const plm_img_desc* fix
     plan.blut = reinterpret_cast<const float*> (ds.ptr + L.lut);
     plan.part_score = reinterpret_cast<double*> (ds.ptr + L.part_score);
     plan.part_count = reinterpret_cast<int*> (ds.ptr + L.part_count);
+    plan.tile_ctr = reinterpret_cast<unsigned long long*> (ds.ptr + L.ctr);
     Tune tune {2, ds.lds_limit, ds.num_cus, 0};
     const EvalOut out {d_grad, d_score, d_num_vox};
     const int mode = grad_mode == PLM_GRAD_REF_NN ? MSE_REF_NN

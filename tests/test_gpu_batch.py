@@ -151,6 +151,45 @@ def test_batch_graph_replays(hip_lib):
         st.close()
 
 
+def test_batch_counted_tiles(hip_lib):
+    """Four pairs of about 64^3 at a 4-voxel grid: at one workgroup per CU
+    each workgroup walks more than 12 tiles, so the batched launch takes its
+    tiles from the XCD counters; with as many workgroups as fit it walks
+    them at a fixed stride.  Both give every pair's single-pair gradient
+    bits and overlap count and its score to summation order; the counted
+    launch gives the same bits on every call and from a captured graph
+    replayed three times (each launch leaves the counters zero)."""
+    import torch
+    dims = [(64, 60, 56), (58, 66, 62), (70, 52, 60), (60, 64, 58)]
+    pairs = [synth.make_pair(f"c{i}", d, (1.0, 1.0, 1.0), (4, 4, 4), 2.0, 12, seed=700 + i, ct=True)
+             for i, d in enumerate(dims)]
+    stages, batch, coeffs, grads, score, nvox, single = single_and_batch(pairs, "ref_nn", vpt=1)
+    with batch:
+        runs = []
+        for w in (0, 1, 1):
+            batch.set_occupancy(w)
+            batch.score_device("ref_nn")
+            torch.cuda.synchronize()
+            check_same(single, grads, score, nvox, f"{w} workgroups per CU")
+            runs.append((score.clone(), [g.clone() for g in grads]))
+        assert torch.equal(runs[1][0], runs[2][0])
+        batch.capture("ref_nn")
+        for rep in range(3):
+            for g in grads:
+                g.zero_()
+            score.zero_()
+            nvox.zero_()
+            torch.cuda.synchronize()
+            batch.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(score, runs[1][0]), rep
+            for i in range(len(pairs)):
+                assert torch.equal(grads[i], runs[1][1][i]), (rep, i)
+            assert [int(v) for v in nvox.cpu()] == [r[2] for r in single], rep
+    for st in stages:
+        st.close()
+
+
 def test_full_size_batch16_bf16(hip_lib):
     """The multi-atlas workload: 16 pairs with edges drawn from 96..256 per
     axis (10-voxel grid, N(0, 3 mm) fields), BF16 storage, one batched

@@ -168,3 +168,25 @@ def test_launch_shapes_agree_and_autotune(hip_lib, vpr):
         assert v == choice and w in (0, 4, 5, 6)
         r = st.score(pair.true_coeff, "mse", "ref_nn")
         assert np.array_equal(r.grad, ref.grad)
+
+
+def test_tile_counters_match_fixed_stride(hip_lib):
+    """At one workgroup per CU a 128x128x96 pair with 8-voxel tiles gives
+    each workgroup at least 12 tiles, so the tile kernel takes its tiles
+    from the XCD counters; with as many workgroups as fit each walks one or
+    two at a fixed stride.  Both give the same score, count and gradient
+    bits, call after call (each launch leaves its counters zero), for MSE
+    with either derivative, the score alone and MI pass 2."""
+    pair = small_pair(dim=(128, 128, 96), vpr=(8, 8, 8), seed=43, n_blobs=20)
+    with stage_for_pair(pair, bins=24) as st:
+        st.set_tuning(1)
+        for metric, mode in (("mse", "ref_nn"), ("mse", "exact"), ("mse", "none"), ("mi", "ref_nn")):
+            res = []
+            for w in (0, 1, 1, 0, 1):
+                st.set_occupancy(w)
+                res.append((w, st.score(pair.true_coeff, metric, mode)))
+            ref = res[0][1]
+            for w, r in res:
+                assert r.score == ref.score, (metric, mode, w)
+                assert r.num_vox == ref.num_vox, (metric, mode, w)
+                assert np.array_equal(r.grad, ref.grad), (metric, mode, w)
