@@ -2363,6 +2363,10 @@ launch_tiles_vgb (const Plan& plan, const Tune& tune, size_t lds, hipStream_t st
     q.pad = 0;
     q.ctr = plan.tile_ctr;
     const int cus = tune.num_cus > 0 ? tune.num_cus : 8;
+    // as many workgroups as fit (evening out the rounds instead -- 184
+    // workgroups an XCD for 1099 tiles at 256x256x128 rather than 192, of
+    // which 54 sit out the last of six rounds -- was slower: 51.3 against
+    // 48.4 us; the concurrency of the full rounds is worth more)
     const int per_xcd = std::max (1, std::min (q.tiles_per_xcd, occ * cus / 8));
     // counted tiles on long runs, for the launches built with them (a batch,
     // MI pass 2); the occupancy above is the fixed-stride build's
