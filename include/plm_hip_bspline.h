@@ -158,7 +158,10 @@ int plm_hip_bspline_score (
 /* The same evaluation with every pointer in device memory, enqueued on
  * stream (a hipStream_t, or NULL for the context's own stream); returns
  * without waiting.  No counterpart in the reference: it is what an
- * optimizer loop that keeps coefficients and gradient resident calls. */
+ * optimizer loop that keeps coefficients and gradient resident calls.
+ * The evaluations of one context share its intermediate buffers: they must
+ * not overlap (enqueue them on one stream, or order the streams); those of
+ * different contexts may. */
 int plm_hip_bspline_score_device (
     plm_hip_ctx* ctx,
     int metric,
