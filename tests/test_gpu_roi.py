@@ -158,6 +158,35 @@ def test_mi_roi_matches_oracle(hip_lib, which):
             assert_parity_mi(r.score, r.grad, (o["score"], o["grad"]), (s64, g64), (sh, gh), label=label)
 
 
+def test_mi_roi_on_vopt_bins(hip_lib):
+    """Flavor h on V-optimal bins: both ROIs and the key tables together
+    (the bins are set up over the whole images, as Joint_histogram::
+    initialize does whatever the ROIs): the joint and fixed histograms bin
+    for bin, N, and the score and gradient against the oracle."""
+    pair = small_pair(dim=(40, 36, 28), spacing=(1.0, 1.0, 2.5), vpr=(5, 5, 5), seed=36)
+    pair.moving = synth.ct_remap(pair.moving.astype(np.float64)).astype(np.float32)
+    fr, mr = rois_for(pair)
+    oc = oracle_case(pair, fixed_roi=fr, moving_roi=mr)
+    bins = 20
+    hp = oc.hist(bins, vopt=True)
+    nj = bins * bins
+    with stage_for_pair(pair, with_grad=False, bins=bins, hist_type="vopt") as st:
+        assert np.array_equal(st.hist.fixed_keys, hp._luts[0])
+        assert np.array_equal(st.hist.moving_keys, hp._luts[1])
+        st.set_roi(roi_volume(fr), roi_volume(mr))
+        for k, coeff in enumerate([np.full(oc.num_coeff, 0.01, np.float32),
+                                   synth.random_coeff(pair.dim, pair.vox_per_rgn, 2.0, seed=44)]):
+            r = st.score(coeff, "mi", want_hist=True)
+            o = oc.mi(coeff, hp)
+            s64, g64, _ = oc.mi_f64(coeff, hp)
+            sh, gh, _ = oc.mi_f64(coeff, hp, hist=o)
+            assert r.num_vox == o["num_vox"]
+            assert np.array_equal(r.hist[:nj], o["fx_j_hist"])
+            assert np.array_equal(r.hist[nj:], o["fx_f_hist"])
+            assert_parity_mi(r.score, r.grad, (o["score"], o["grad"]), (s64, g64), (sh, gh),
+                             label=f"MI ROI on V-optimal bins, coeff {k}:")
+
+
 ROT = dict(f_sp=(1.0, 1.1, 2.3), f_org=(-3.0, 5.0, 1.5),
            f_dc=(0.9927, -0.1197, 0.0147, 0.1192, 0.9922, 0.0346, -0.0187, -0.0326, 0.9993),
            m_sp=(0.95, 1.05, 2.2), m_org=(-2.0, 4.0, 0.5))
