@@ -629,8 +629,8 @@ def mi_leg(args, dev, want_cpu):
     grid and 32x32 bins: evaluations per second at four coefficient vectors
     (the true field and three N(0, 4 mm) draws), the stage times, and the
     HBM roofline of the whole two-pass evaluation (fixed 4 + moving 4 B per
-    voxel and pass, the survey's model; the device reads 1-byte bin images
-    instead).  With the CPU leg: the oracle's flavor g (serial pass 1 as the
+    voxel and pass, the float-image byte model; the device reads 1-byte bin
+    images instead).  With the CPU leg: the oracle's flavor g (serial pass 1 as the
     reference) on one of the vectors, timed, and the device's integer
     histogram, score and gradient against it and the FP64 oracle."""
     import torch
