@@ -913,8 +913,6 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
         for (int i = t; i < nj; i += TILE_THREADS) L.tab[i] = mi.table[i];
     }
     constexpr int ES = MI ? 1 : Img<TI>::ES;    // bytes per voxel of what is loaded
-    float N = 1.f;
-    if (MI) N = *mi.nvox_f;
 
     // This thread's first-batch voxel slots t + 256u (x fastest), the same
     // in every tile, packed by slot_pack (x, y, the column index and z).
@@ -1422,11 +1420,11 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
                         dc1 -= dw[3 * n + 1] * ds;
                         dc2 -= dw[3 * n + 2] * ds;
                     }
-                    dc0 = dc0 / N;
-                    dc1 = dc1 / N;
-                    dc2 = dc2 / N;
                     // projected in place, component by component, as the
-                    // reference does (src/register/bspline_mi.txx:107-109)
+                    // reference does (src/register/bspline_mi.txx:107-109);
+                    // its division by N before the projection (:104-106), a
+                    // linear map's common factor, is the reduce's FP64 1/N
+                    // instead of three float divisions a voxel
                     const float* Pm = g.mproj;
                     dc0 = dc0 * Pm[0] + dc1 * Pm[1] + dc2 * Pm[2];
                     dc1 = dc0 * Pm[3] + dc1 * Pm[4] + dc2 * Pm[5];
@@ -1507,7 +1505,8 @@ void k_tiles (const PairDev* __restrict__ pairs, TileQueue q, const float* __res
 // a fixed order, so the result is bitwise reproducible (the slot sum of
 // bspline_condense_smetric_grad, src/register/bspline.cxx:400-413, in
 // double) -- then scaled by 2/N for MSE (src/register/bspline_mse.cxx:43-73)
-// or 1 for MI.  Workgroups are dealt to the pairs of the launch in order.
+// or 1/N for MI (the per-voxel division of src/register/bspline_mi.txx:
+// 104-106, moved here: the condense and the projection are linear).  Workgroups are dealt to the pairs of the launch in order.
 // For MSE every workgroup also sums its pair's per-workgroup overlap counts
 // (integers, so every workgroup gets the same N with no cross-workgroup
 // traffic), fetched with the slot loads; the pair's first workgroup also
@@ -1602,6 +1601,12 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
         n = wave_sum_ll (n);
         sc = wave_sum (sc);
         if (lane == 0) { s_n[wave] = n; s_s[wave] = sc; }
+    }
+    if (!MSE) {
+        // MI: dc/dv's factor 1/N (N written by k_mi_score), in FP64 here
+        const int64_t* nv = (out0.nvox && p == 0) ? out0.nvox : pd.nvox;
+        const long long Nv = *nv;
+        scale = Nv >= 1 ? 1.0 / (double) Nv : 0.0;
     }
     __syncthreads ();
     if (MSE) {
