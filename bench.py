@@ -265,6 +265,9 @@ def batched_leg(args, dev, want_errors):
 ADJOINT_WORKLOAD = "adjoint_192x192x96_vpr8"
 FD_COMPONENTS = 200
 FD_STEP_MM = 1e-3
+# the FP64 score resolves a much smaller step, where the piecewise-smooth
+# trilinear score is differentiable across nearly every voxel
+FD64_STEP_MM = 1e-5
 
 
 def adjoint_leg(args, dev, want_errors):
@@ -272,7 +275,8 @@ def adjoint_leg(args, dev, want_errors):
     forward+backward time and extra memory of the recompute-in-backward and
     stored-field modes, and the adjoint against central finite differences
     of the GPU score on 200 random coefficients (and, in the CPU leg, against
-    the reference flavor's gradient)."""
+    the reference flavor's gradient, FP64, and central differences of the
+    FP64 score on 200 coefficients whose support holds no clamped voxel)."""
     import torch
     from plastimatch_amd import synth
     from plastimatch_amd.autograd import mse
@@ -340,6 +344,18 @@ def adjoint_leg(args, dev, want_errors):
             "grad_l2": float(np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref))}
         out["rel_err_vs_fp64"] = {"score": abs(sc - s64) / abs(s64),
                                   "grad_l2": float(np.linalg.norm(g - g64) / np.linalg.norm(g64))}
+        # central differences of the FP64 score (flavor p) on coefficients
+        # whose support holds no clamped voxel
+        inner = []
+        for i in np.random.default_rng(302).permutation(c.size):
+            if case.mse_f64_support(c, int(i), 0.0)[2] == 0:
+                inner.append(int(i))
+                if len(inner) == FD_COMPONENTS:
+                    break
+        fd64, crossed = case.fd_mse_f64(c, np.array(inner), FD64_STEP_MM)
+        out["fd64_check"] = {"components": len(inner), "h_mm": FD64_STEP_MM, "unclamped_support": True,
+                             "border_voxels_in_steps": int(np.sum(crossed)),
+                             "rel_l2": float(np.linalg.norm(g[inner] - fd64) / np.linalg.norm(fd64))}
     st.close()
     return out
 

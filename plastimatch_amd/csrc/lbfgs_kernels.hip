@@ -188,7 +188,7 @@ k_lb_finish (const LbAct* __restrict__ ap, LbAct av, LbMachine* __restrict__ mac
 {
     const LbAct* act = ap ? ap : &av;
     if (act->done) return;
-    __shared__ double mid[16][16];
+    __shared__ double mid[48][16];
     __shared__ double tot[4 + 7 + 5 * (LB_MAX_M - 1)];
     __shared__ LbMachine ms;
     const int t = threadIdx.x;
@@ -207,24 +207,35 @@ k_lb_finish (const LbAct* __restrict__ ap, LbAct av, LbMachine* __restrict__ mac
         for (int i = 0; i < WT; i++)
             if (t + i * LB_THREADS < W) mw[i] = wg[t + i * LB_THREADS];
     }
-    for (int k0 = 0; k0 < total; k0 += 16) {
-        const int k = k0 + (t >> 4), l = t & 15;
+    if (t < 3) tot[t] = b.score[t];
+    // RG groups of sixteen totals a round (48: every total of a history of
+    // up to 9 pairs in one round), all of a round's loads in flight together
+    constexpr int RG = 3;
+    for (int k0 = 0; k0 < total; k0 += 16 * RG) {
+        const int l = t & 15;
         // every partial of this thread loaded before any is added (nb <=
         // LB_MAX_BLOCKS), then added in q order; the absent ones add +0.0,
         // which leaves a sum that started at +0.0 unchanged
-        const double* p = k == 0 ? b.slope : b.part + (int64_t) (k - 1) * nb;
-        double pv[LB_MAX_BLOCKS / 16];
+        double pv[RG][LB_MAX_BLOCKS / 16];
 #pragma unroll
-        for (int i = 0; i < LB_MAX_BLOCKS / 16; i++) {
-            const int q = l + 16 * i;
-            pv[i] = (k < total && q < nb) ? p[q] : 0.0;
+        for (int r = 0; r < RG; r++) {
+            const int k = k0 + 16 * r + (t >> 4);
+            const double* p = k == 0 ? b.slope : b.part + (int64_t) (k - 1) * nb;
+#pragma unroll
+            for (int i = 0; i < LB_MAX_BLOCKS / 16; i++) {
+                const int q = l + 16 * i;
+                pv[r][i] = (k < total && q < nb) ? p[q] : 0.0;
+            }
         }
-        double s = 0.0;
 #pragma unroll
-        for (int i = 0; i < LB_MAX_BLOCKS / 16; i++) s += pv[i];
-        mid[t >> 4][l] = s;
+        for (int r = 0; r < RG; r++) {
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < LB_MAX_BLOCKS / 16; i++) s += pv[r][i];
+            mid[16 * r + (t >> 4)][l] = s;
+        }
         __syncthreads ();
-        if (t < 16 && k0 + t < total) {
+        if (t < 16 * RG && k0 + t < total) {
             double u = 0.0;
 #pragma unroll
             for (int j = 0; j < 16; ++j) u += mid[t][j];
@@ -232,8 +243,6 @@ k_lb_finish (const LbAct* __restrict__ ap, LbAct av, LbMachine* __restrict__ mac
         }
         __syncthreads ();
     }
-    if (t < 3) tot[t] = b.score[t];
-    __syncthreads ();
     if (!machine) {
         for (int i = t; i < 3 + total; i += LB_THREADS) host[i] = tot[i];
         return;

@@ -198,10 +198,6 @@ struct LbMachine {
     // Gram entries of the history by slot: SY[i m + j] = s_i.y_j, YY = y_i.y_j
     double SY[LB_MAX_M * LB_MAX_M];
     double YY[LB_MAX_M * LB_MAX_M];
-    // the direction solve's work arrays (members, so a device run keeps
-    // them with the rest of the state in LDS rather than in scratch)
-    double wP[LB_MAX_M], wQ[LB_MAX_M], wu[LB_MAX_M], wt[LB_MAX_M];
-    int wo[LB_MAX_M];
     // progress reports: every one is counted in nrep, the first cap kept
     LbReport* rep;
     int cap, nrep;
@@ -277,49 +273,126 @@ struct LbMachine {
     // (the compact form of Byrd, Nocedal and Schnabel, Math. Prog. 63,
     // 1994; its d = -H g is liblbfgs's two-loop recursion, lbfgs.c:555-609).
     // The newest pair's Gram row and column are kept for later iterations.
+    // B = b is a template argument, so the slot order and the work arrays
+    // are registers and every Gram entry's address is known before the
+    // arithmetic starts (on the device the machine lives in LDS, where a
+    // loop of dependent index-then-value reads is a chain of LDS latencies).
+    template <int B>
     __host__ __device__ void
-    solve_direction (const double* red, int b, int e)
+    solve_b (const double* red, int e)
     {
         const int m = p.m;
-        a.dir_b = b;
+        int o[B];
+        double P[B], Q[B];
+        // o[i] = (e - (B - 1) + i) mod m, the slots in age order
+        o[0] = (e - (B - 1) + m) % m;
+#pragma unroll
+        for (int i = 1; i < B; ++i) o[i] = o[i - 1] + 1 == m ? 0 : o[i - 1] + 1;
         SY[e * m + e] = red[3];
         YY[e * m + e] = red[4];
-        double* P = wP;
-        double* Q = wQ;
-        int* o = wo;
-        for (int i = 0; i < b; ++i) o[i] = (e - (b - 1) + i + m) % m;
-        P[b - 1] = red[5];
-        Q[b - 1] = red[6];
-        for (int kk = 1; kk < b; ++kk) {
-            const int j = (e - kk + m) % m;
+        P[B - 1] = red[5];
+        Q[B - 1] = red[6];
+#pragma unroll
+        for (int kk = 1; kk < B; ++kk) {
+            const int j = o[B - 1 - kk];
             const int c = 7 + 5 * (kk - 1);
             SY[e * m + j] = red[c];
             SY[j * m + e] = red[c + 1];
             YY[e * m + j] = YY[j * m + e] = red[c + 2];
-            P[b - 1 - kk] = red[c + 3];
-            Q[b - 1 - kk] = red[c + 4];
+            P[B - 1 - kk] = red[c + 3];
+            Q[B - 1 - kk] = red[c + 4];
         }
-        const double gamma = SY[o[b - 1] * m + o[b - 1]] / YY[o[b - 1] * m + o[b - 1]];
-        double* u = wu;
-        double* t = wt;
-        for (int i = b - 1; i >= 0; --i) {
+        // s_o(i).y_o(j) and y_o(i).y_o(j) in age order: no store falls
+        // between these reads, so they are free to be issued early
+        const double* sy = SY;
+        const double* yy = YY;
+        const double gamma = sy[o[B - 1] * m + o[B - 1]] / yy[o[B - 1] * m + o[B - 1]];
+        double u[B], t[B], ca[B];
+#pragma unroll
+        for (int i = B - 1; i >= 0; --i) {
             double acc = P[i];
-            for (int j = i + 1; j < b; ++j) acc = acc - SY[o[i] * m + o[j]] * u[j];
-            u[i] = acc / SY[o[i] * m + o[i]];
+#pragma unroll
+            for (int j = i + 1; j < B; ++j) acc = acc - sy[o[i] * m + o[j]] * u[j];
+            u[i] = acc / sy[o[i] * m + o[i]];
         }
-        for (int i = 0; i < b; ++i) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
             double w = 0.0;
-            for (int j = 0; j < b; ++j) w = w + YY[o[i] * m + o[j]] * u[j];
-            t[i] = SY[o[i] * m + o[i]] * u[i] + gamma * w - gamma * Q[i];
+#pragma unroll
+            for (int j = 0; j < B; ++j) w = w + yy[o[i] * m + o[j]] * u[j];
+            t[i] = sy[o[i] * m + o[i]] * u[i] + gamma * w - gamma * Q[i];
         }
-        for (int i = 0; i < b; ++i) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
             double acc = t[i];
-            for (int j = 0; j < i; ++j) acc = acc - SY[o[j] * m + o[i]] * a.ca[j];
-            a.ca[i] = acc / SY[o[i] * m + o[i]];
+#pragma unroll
+            for (int j = 0; j < i; ++j) acc = acc - sy[o[j] * m + o[i]] * ca[j];
+            ca[i] = acc / sy[o[i] * m + o[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            a.ca[i] = ca[i];
             a.cu[i] = u[i];
             a.slot[i] = o[i];
         }
         a.gamma = gamma;
+    }
+
+    // The same solve for any b, in the same order of operations: u in
+    // a.cu, t then ca in place in a.ca (ca[i] is formed from t[i] and the
+    // ca[j], j < i, already written), P and Q read from the components.
+    __host__ __device__ void
+    solve_any (const double* red, int b, int e)
+    {
+        const int m = p.m;
+        const int o0 = (e - (b - 1) + m) % m;
+        auto o = [&] (int i) { const int s = o0 + i; return s >= m ? s - m : s; };
+        auto P = [&] (int i) { return i == b - 1 ? red[5] : red[7 + 5 * (b - 2 - i) + 3]; };
+        auto Q = [&] (int i) { return i == b - 1 ? red[6] : red[7 + 5 * (b - 2 - i) + 4]; };
+        SY[e * m + e] = red[3];
+        YY[e * m + e] = red[4];
+        for (int kk = 1; kk < b; ++kk) {
+            const int j = o (b - 1 - kk);
+            const int c = 7 + 5 * (kk - 1);
+            SY[e * m + j] = red[c];
+            SY[j * m + e] = red[c + 1];
+            YY[e * m + j] = YY[j * m + e] = red[c + 2];
+        }
+        const double gamma = SY[e * m + e] / YY[e * m + e];
+        for (int i = b - 1; i >= 0; --i) {
+            double acc = P (i);
+            for (int j = i + 1; j < b; ++j) acc = acc - SY[o (i) * m + o (j)] * a.cu[j];
+            a.cu[i] = acc / SY[o (i) * m + o (i)];
+        }
+        for (int i = 0; i < b; ++i) {
+            double w = 0.0;
+            for (int j = 0; j < b; ++j) w = w + YY[o (i) * m + o (j)] * a.cu[j];
+            a.ca[i] = SY[o (i) * m + o (i)] * a.cu[i] + gamma * w - gamma * Q (i);
+        }
+        for (int i = 0; i < b; ++i) {
+            double acc = a.ca[i];
+            for (int j = 0; j < i; ++j) acc = acc - SY[o (j) * m + o (i)] * a.ca[j];
+            a.ca[i] = acc / SY[o (i) * m + o (i)];
+            a.slot[i] = o (i);
+        }
+        a.gamma = gamma;
+    }
+
+    // b <= 6 (liblbfgs's default m is 6) in registers; the register form
+    // of longer histories spills
+    __host__ __device__ void
+    solve_direction (const double* red, int b, int e)
+    {
+        a.dir_b = b;
+        switch (b) {
+        case 1: solve_b<1> (red, e); break;
+        case 2: solve_b<2> (red, e); break;
+        case 3: solve_b<3> (red, e); break;
+        case 4: solve_b<4> (red, e); break;
+        case 5: solve_b<5> (red, e); break;
+        case 6: solve_b<6> (red, e); break;
+        default: solve_any (red, b, e); break;
+        }
     }
 
     // The top of the search loop (lbfgs.c:960-985): the interval, the step
