@@ -1554,27 +1554,30 @@ void k_reduce (const PairDev* __restrict__ pairs, int npairs, const double* __re
     long long n = 0;
     double sc = 0.0;
     if (MSE) {
-        // CH partials per thread with all loads in flight together (one
-        // round for up to 2048 partials), summed in a fixed order
-        constexpr int CH = 8;
+        // the overlap counts (integers: any order gives the same N), 16
+        // bytes a lane where the pair's counts are 16-byte aligned (the
+        // tile grid is a multiple of eight workgroups), two loads a lane in
+        // flight together for up to 2048 partials
         const int* pc = part_count + (size_t) p * nparts;
         const double* ps = part_score + (size_t) p * nparts;
-        const bool wg_scores = lead && !tile_scores;
-        for (int t0 = 0; t0 < nparts; t0 += CH * REDUCE_THREADS) {
-            int cnt[CH];
-            double s[CH];
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-                const int t = t0 + k * REDUCE_THREADS + threadIdx.x;
-                cnt[k] = t < nparts ? pc[t] : 0;
-                s[k] = wg_scores && t < nparts ? ps[t] : 0.0;
+        int t4 = 0;
+        if ((reinterpret_cast<uintptr_t> (pc) & 15) == 0) {
+            const int4* pc4 = reinterpret_cast<const int4*> (pc);
+            const int n4 = nparts >> 2;
+            for (int t0 = 0; t0 < n4; t0 += 2 * REDUCE_THREADS) {
+                const int ta = t0 + threadIdx.x, tb = ta + REDUCE_THREADS;
+                const int4 a = ta < n4 ? pc4[ta] : make_int4 (0, 0, 0, 0);
+                const int4 b = tb < n4 ? pc4[tb] : make_int4 (0, 0, 0, 0);
+                n += ((long long) a.x + a.y) + ((long long) a.z + a.w)
+                    + ((long long) b.x + b.y) + ((long long) b.z + b.w);
             }
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-                n += cnt[k];
-                sc += s[k];
-            }
+            t4 = 4 * n4;
         }
+        for (int t = t4 + threadIdx.x; t < nparts; t += REDUCE_THREADS) n += pc[t];
+        // the lead's per-workgroup score partials, thread t adding t,
+        // t + 256, ... in turn
+        if (lead && !tile_scores)
+            for (int t = threadIdx.x; t < nparts; t += REDUCE_THREADS) sc += ps[t];
         if (lead && tile_scores) {
             // per-tile score partials (counted tiles), in tile order
             constexpr int CT = 32;
