@@ -112,6 +112,28 @@ def test_device_lbfgs_matches_oracle(hip_lib, variant, resident):
     st.close()
 
 
+@pytest.mark.parametrize("resident", [False, True])
+@pytest.mark.parametrize("m", [3, 11])
+def test_device_lbfgs_history_lengths(hip_lib, m, resident):
+    """Histories shorter and longer than the six pairs whose direction solve
+    runs on registers: m = 3 overwrites its oldest pair from the fourth
+    iteration on, and m = 11 takes the runtime-length solve from the
+    seventh iteration and overwrites its oldest pair after the eleventh;
+    both against the oracle's liblbfgs restatement."""
+    pair, bxf, st = stage(seed=7)
+    fun = gpu_objective(st, bxf.num_coeff, "mse", "ref_nn", None)
+    lim = dict(max_its=16, max_feval=60)
+    ora = oracle.lbfgs(fun, np.zeros(bxf.num_coeff), m=m, **lim)
+    dev, x, c = device_run(st, bxf, "mse", "ref_nn", None, resident=resident, m=m, **lim)
+    print(f"m {m}: status {dev.status} it {dev.iterations} feval {dev.fevals} fx {dev.fx:.6g} "
+          f"(oracle {ora.fx:.6g}), x rel {rel(x, ora.x):.2e}")
+    assert dev.iterations > 12
+    compare(dev, ora)
+    assert abs(dev.fx - ora.fx) <= 1e-6 * abs(ora.fx)
+    assert rel(x, ora.x) < 1e-5
+    st.close()
+
+
 def test_device_lbfgs_deterministic_and_progress_stop(hip_lib):
     pair, bxf, st = stage()
     runs = [device_run(st, bxf, "mse", "ref_nn", REG, max_its=5, max_feval=20) for _ in range(2)]
