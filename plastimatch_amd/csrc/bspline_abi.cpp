@@ -685,7 +685,7 @@ namespace {
 int
 eval_launch (plm_hip_ctx* c, int metric, int grad_mode, const float* d_coeff, double* d_score,
     double* d_grad, int64_t* d_num_vox, int64_t* d_hist, hipStream_t st, hipEvent_t* ev,
-    const int* skip)
+    const int* skip, const plmhip::ReduceHook* hook = nullptr)
 {
     if (!d_coeff || !d_score || !d_num_vox || (!d_grad && grad_mode != PLM_GRAD_NONE))
         return fail (PLM_HIP_ERR_ARG, "NULL output or coefficient pointer");
@@ -708,7 +708,7 @@ eval_launch (plm_hip_ctx* c, int metric, int grad_mode, const float* d_coeff, do
         const int mode = grad_mode == PLM_GRAD_REF_NN ? plmhip::MSE_REF_NN
             : grad_mode == PLM_GRAD_EXACT ? plmhip::MSE_EXACT : plmhip::MSE_SCORE_ONLY;
         e = plmhip::launch_mse (c->plan, c->tune, mode, c->storage == PLM_STORE_BF16,
-            d_coeff, out, st, ev);
+            d_coeff, out, st, ev, hook);
         if (e != hipSuccess) return hip_fail (e, "MSE launch");
     } else {
         e = plmhip::launch_mi (c->plan, c->tune, c->h, c->hist_blocks, c->d_fbin, c->d_mquad,
@@ -725,12 +725,12 @@ namespace plmhip {
 
 int
 ctx_eval (plm_hip_ctx* c, int metric, int grad_mode, const float* d_coeff, double* d_score,
-    double* d_grad, int64_t* d_num_vox, hipStream_t st, const int* skip)
+    double* d_grad, int64_t* d_num_vox, hipStream_t st, const int* skip, const ReduceHook* hook)
 {
     if (!c) return fail (PLM_HIP_ERR_ARG, "ctx is NULL");
     HIP_TRY (hipSetDevice (c->device));
     return eval_launch (c, metric, grad_mode, d_coeff, d_score, d_grad, d_num_vox, nullptr, st, nullptr,
-        skip);
+        skip, hook);
 }
 
 } // namespace plmhip

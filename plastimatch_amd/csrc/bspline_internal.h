@@ -23,10 +23,12 @@ int ctx_info (const plm_hip_ctx* c, int* device, hipStream_t* stream, int64_t* n
 // 1) on stream st, without stage timing, every kernel returning at once
 // when *skip (a device int, may be null) is nonzero at its launch: the
 // evaluation steps of an optimizer chain that may be replayed after its
-// run ended.  ctx_reg_prepare allocates the regularizer's scratch ahead of
-// a graph capture.
+// run ended.  With hook (MSE with a gradient), the hook launches in place
+// of the gradient reduce and d_grad is not written.  ctx_reg_prepare
+// allocates the regularizer's scratch ahead of a graph capture.
+struct ReduceHook;
 int ctx_eval (plm_hip_ctx* c, int metric, int grad_mode, const float* d_coeff, double* d_score,
-    double* d_grad, int64_t* d_num_vox, hipStream_t st, const int* skip);
+    double* d_grad, int64_t* d_num_vox, hipStream_t st, const int* skip, const ReduceHook* hook = nullptr);
 int ctx_reg_prepare (plm_hip_ctx* c);
 
 // The optimizer's device workspace, kept by a context between runs (freed

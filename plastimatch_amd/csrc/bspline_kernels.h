@@ -243,10 +243,21 @@ constexpr int MSE_REF_NN = 0;        // score + gradient, nearest-voxel moving g
 constexpr int MSE_EXACT = 1;         // score + gradient, analytic trilinear derivative
 constexpr int MSE_SCORE_ONLY = 2;    // score and overlap count (single-pair plans)
 
+// A launch that takes the place of a single-pair evaluation's gradient
+// reduce (the optimizer's chain folds its post pass into it): called with
+// the tile kernel's partial count and whether it wrote per-tile scores.
+struct ReduceHook {
+    hipError_t (*launch) (void* user, const Plan& plan, int nparts, int tile_scores, const EvalOut& out0,
+        hipStream_t st);
+    void* user;
+};
+
 // MSE over every pair of the plan (coeff0/out0 override pair 0's table
-// entry when not null).
+// entry when not null); with hook (gradient modes only) the hook launches
+// in place of k_reduce.
 hipError_t launch_mse (const Plan& plan, const Tune& tune, int mode, int storage,
-    const float* coeff0, const EvalOut& out0, hipStream_t st, hipEvent_t* ev);
+    const float* coeff0, const EvalOut& out0, hipStream_t st, hipEvent_t* ev,
+    const ReduceHook* hook = nullptr);
 
 // The same for a single-pair float plan with no z-pair image: the tile
 // kernel reads the trilinear corners from the moving image itself (the

@@ -2462,7 +2462,7 @@ launch_tiles_storage (const Plan& plan, const Tune& tune, int storage, hipStream
 
 hipError_t
 launch_mse (const Plan& plan, const Tune& tune, int mode, int storage, const float* coeff0,
-    const EvalOut& out0, hipStream_t st, hipEvent_t* ev)
+    const EvalOut& out0, hipStream_t st, hipEvent_t* ev, const ReduceHook* hook)
 {
     TileArgs ta {coeff0, MiArgs {}, nullptr};
     ta.skip = out0.skip;
@@ -2476,10 +2476,16 @@ launch_mse (const Plan& plan, const Tune& tune, int mode, int storage, const flo
     if ((e = mark (ev, 1, st)) != hipSuccess) return e;
     // the score alone needs one reduce workgroup per pair (single-pair plans)
     const bool grad = mode != MSE_SCORE_ONLY;
-    hipLaunchKernelGGL (k_reduce<true>, dim3 (grad ? plan.reduce_blocks : 1), dim3 (REDUCE_THREADS),
-        0, st, plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts < 0 ? -nparts : nparts,
-        nparts < 0 ? 1 : 0, out0, grad ? 1 : 0);
-    LAUNCH_CHECK ();
+    if (hook && grad && plan.npairs == 1) {
+        if ((e = hook->launch (hook->user, plan, nparts < 0 ? -nparts : nparts, nparts < 0 ? 1 : 0, out0, st))
+            != hipSuccess)
+            return e;
+    } else {
+        hipLaunchKernelGGL (k_reduce<true>, dim3 (grad ? plan.reduce_blocks : 1), dim3 (REDUCE_THREADS),
+            0, st, plan.d_pairs, plan.npairs, plan.part_score, plan.part_count, nparts < 0 ? -nparts : nparts,
+            nparts < 0 ? 1 : 0, out0, grad ? 1 : 0);
+        LAUNCH_CHECK ();
+    }
     return mark (ev, 2, st);
 }
 

@@ -33,6 +33,7 @@
 
 #include "plm_hip_bspline.h"
 #include "bspline_internal.h"
+#include "bspline_kernels.h"
 #include "lbfgs_kernels.h"
 #include "lbfgs_logic.h"
 
@@ -195,18 +196,43 @@ struct Run {
         return PLM_HIP_OK;
     }
 
+    // The gradient reduce of an MSE evaluation with the post pass folded in
+    // (lb_launch_reduce_post), launched by the evaluation in place of its
+    // own reduce.
+    struct Fused {
+        const LbAct* act;
+        const LbAct* av;
+        const plmhip::LbBufs* b;
+        bool launched;
+    };
+    static hipError_t
+    reduce_post (void* user, const plmhip::Plan& plan, int nparts, int tile_scores, const plmhip::EvalOut& out0,
+        hipStream_t s)
+    {
+        Fused* f = static_cast<Fused*> (user);
+        f->launched = true;
+        return plmhip::lb_launch_reduce_post (plan.d_pairs, plan.part_score, plan.part_count, nparts,
+            tile_scores, out0, f->act, *f->av, *f->b, s);
+    }
+
     // One evaluation chain reading its action from act (device memory),
-    // or with act null from av.
+    // or with act null from av.  An MSE objective without regularization
+    // (m <= 6, n within one element a thread of the post grid) takes the
+    // fused reduce and post pass: the same bits, one launch fewer.
     int chain (const LbAct* act, const LbAct& av, LbMachine* mach)
     {
         const int* skip = act ? &act->done : nullptr;
         if (hip (plmhip::lb_launch_next (act, av, b, st), "L-BFGS trial point")) return err;
+        const bool fuse = metric == PLM_METRIC_MSE && !reg && grad_mode != PLM_GRAD_NONE && b.m <= 6
+            && (int64_t) b.nb * plmhip::LB_THREADS >= b.n;
+        Fused f {act, &av, &b, false};
+        const plmhip::ReduceHook hook {&reduce_post, &f};
         int rc = plmhip::ctx_eval (c, metric, grad_mode, b.coeff, b.score, b.geval,
-            reinterpret_cast<int64_t*> (b.score + 2), st, skip);
+            reinterpret_cast<int64_t*> (b.score + 2), st, skip, fuse ? &hook : nullptr);
         if (rc < 0) return err = rc;
         if (reg && (rc = plmhip::ctx_regularize (c, reg, b.coeff, b.score + 1, b.geval, st, skip)) < 0)
             return err = rc;
-        if (hip (plmhip::lb_launch_post (act, av, b, st), "L-BFGS post pass")
+        if ((!f.launched && hip (plmhip::lb_launch_post (act, av, b, st), "L-BFGS post pass"))
             || hip (plmhip::lb_launch_finish (act, av, mach, b, w->hh_dev, w->flag_dev, st), "L-BFGS totals"))
             return err;
         return PLM_HIP_OK;

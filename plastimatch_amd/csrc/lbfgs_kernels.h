@@ -89,6 +89,14 @@ struct LbFlag {
 hipError_t lb_launch_next (const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t st);
 // The post-evaluation pass (instantiated for the run's m).
 hipError_t lb_launch_post (const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t st);
+// The MSE gradient reduce of a single-pair evaluation and the post pass in
+// one launch (m <= 6, one element a thread: n <= nb * 256; out0.score and
+// out0.nvox set): the gradient goes straight into the trial's g buffer.
+struct PairDev;
+struct EvalOut;
+hipError_t lb_launch_reduce_post (const PairDev* pairs, const double* part_score, const int* part_count,
+    int nparts, int tile_scores, const EvalOut& out0, const LbAct* act, const LbAct& av, const LbBufs& b,
+    hipStream_t st);
 // The totals: with machine null, written to host (mapped host memory:
 // score, reg score, count, slope, then the components); else the machine
 // advances on the device (act must be &machine->a) and flag is updated.

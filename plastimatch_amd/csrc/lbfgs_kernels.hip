@@ -6,6 +6,10 @@
 // Built with -ffp-contract=off: each element update rounds its product and
 // its sum separately, as liblbfgs's vecadd / vecdiff do.
 
+#include <float.h>
+#include <stdint.h>
+
+#include "bspline_kernels.h"
 #include "lbfgs_kernels.h"
 #include "lbfgs_logic.h"
 
@@ -116,42 +120,79 @@ k_lb_next (const LbAct* __restrict__ ap, LbAct av, LbBufs b)
     }
 }
 
-// After the evaluation: the gradient it wrote kept in the trial's g buffer,
-// and the partials of every post component (LbAct::post_bp <= BPMAX earlier
-// pairs; the start point has no pair and takes g for d).
+// The post pass's work on element e of gradient value ge: ge kept in the
+// trial's g buffer, s = x - xp and y = g - gp written into the would-be
+// pair's slot pe, and the element's terms added to every post component
+// (LbAct::post_bp <= BPMAX earlier pairs; the start point has no pair and
+// takes g for d).
 template <int BPMAX>
-__global__ void __launch_bounds__ (LB_THREADS)
-k_lb_post (const LbAct* __restrict__ ap, LbAct av, LbBufs b)
-{
-    const LbAct* act = ap ? ap : &av;
-    if (act->done) return;
-    constexpr int NC = 7 + 5 * BPMAX;
-    const int pe = act->post_e, bp = act->post_bp, m = b.m;
-    const bool pair = pe >= 0;
-    const double* __restrict__ x = b.X[act->xi];
-    const double* __restrict__ xp = b.X[act->xpi];
-    const double* __restrict__ g = b.geval;
-    double* __restrict__ gk = b.G[act->gi];
-    const double* __restrict__ gp = b.G[act->gpi];
-    const double* __restrict__ d = act->kind == 0 ? nullptr : b.d;
-    const int64_t n = b.n;
-    double v[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = 0.0;
+struct PostElem {
+    static constexpr int NC = 7 + 5 * BPMAX;
+    const double* __restrict__ x;
+    const double* __restrict__ xp;
+    double* __restrict__ gk;
+    const double* __restrict__ gp;
+    const double* __restrict__ d;
+    int pe, bp;
+    bool pair;
     int slot[BPMAX > 0 ? BPMAX : 1];
+
+    __device__ __forceinline__
+    PostElem (const LbAct* act, const LbBufs& b)
+    {
+        pe = act->post_e;
+        bp = act->post_bp;
+        pair = pe >= 0;
+        x = b.X[act->xi];
+        xp = b.X[act->xpi];
+        gk = b.G[act->gi];
+        gp = b.G[act->gpi];
+        d = act->kind == 0 ? nullptr : b.d;
+        const int m = b.m;
 #pragma unroll
-    for (int i = 1; i <= BPMAX; ++i) slot[i - 1] = (pe - i + m) % m;
-    const int64_t stride = (int64_t) gridDim.x * LB_THREADS;
-    for (int64_t e = (int64_t) blockIdx.x * LB_THREADS + threadIdx.x; e < n; e += stride) {
-        const double xe = x[e], ge = g[e];
+        for (int i = 1; i <= BPMAX; ++i) slot[i - 1] = (pe - i + m) % m;
+    }
+
+    // The element's inputs, none of which the pass writes (the earlier
+    // pairs' slots are never pe), so they may be fetched before its
+    // gradient value is known.
+    struct In {
+        double xe, xpe, gpe, de;
+        double sj[BPMAX > 0 ? BPMAX : 1], yj[BPMAX > 0 ? BPMAX : 1];
+    };
+
+    __device__ __forceinline__ void
+    load (const LbBufs& b, int64_t e, In& w) const
+    {
+        const int64_t n = b.n;
+        w.xe = x[e];
+        w.de = d ? d[e] : 0.0;
+        if (pair) {
+            w.xpe = xp[e];
+            w.gpe = gp[e];
+#pragma unroll
+            for (int i = 0; i < BPMAX; ++i) {
+                if (i < bp) {
+                    w.sj[i] = b.S[(int64_t) slot[i] * n + e];
+                    w.yj[i] = b.Y[(int64_t) slot[i] * n + e];
+                }
+            }
+        }
+    }
+
+    __device__ __forceinline__ void
+    add (const LbBufs& b, int64_t e, double ge, const In& w, double (&v)[NC]) const
+    {
+        const int64_t n = b.n;
+        const double xe = w.xe;
         gk[e] = ge;
-        const double de = d ? d[e] : ge;
+        const double de = d ? w.de : ge;
         v[0] += ge * de;
         v[1] += xe * xe;
         v[2] += ge * ge;
         if (pair) {
-            const double se = xe - xp[e];
-            const double ye = ge - gp[e];
+            const double se = xe - w.xpe;
+            const double ye = ge - w.gpe;
             b.S[(int64_t) pe * n + e] = se;
             b.Y[(int64_t) pe * n + e] = ye;
             v[3] += se * ye;
@@ -161,8 +202,8 @@ k_lb_post (const LbAct* __restrict__ ap, LbAct av, LbBufs b)
 #pragma unroll
             for (int i = 0; i < BPMAX; ++i) {
                 if (i < bp) {
-                    const double sj = b.S[(int64_t) slot[i] * n + e];
-                    const double yj = b.Y[(int64_t) slot[i] * n + e];
+                    const double sj = w.sj[i];
+                    const double yj = w.yj[i];
                     v[7 + 5 * i + 0] += se * yj;
                     v[7 + 5 * i + 1] += sj * ye;
                     v[7 + 5 * i + 2] += ye * yj;
@@ -172,7 +213,131 @@ k_lb_post (const LbAct* __restrict__ ap, LbAct av, LbBufs b)
             }
         }
     }
-    block_partials<NC> (v, b.part, gridDim.x);
+};
+
+// After the evaluation: the gradient it wrote (geval) through PostElem,
+// then the workgroup's partial of every post component.
+template <int BPMAX>
+__global__ void __launch_bounds__ (LB_THREADS)
+k_lb_post (const LbAct* __restrict__ ap, LbAct av, LbBufs b)
+{
+    const LbAct* act = ap ? ap : &av;
+    if (act->done) return;
+    using PE = PostElem<BPMAX>;
+    const PE pe (act, b);
+    double v[PE::NC];
+#pragma unroll
+    for (int c = 0; c < PE::NC; ++c) v[c] = 0.0;
+    const int64_t n = b.n;
+    const int64_t stride = (int64_t) gridDim.x * LB_THREADS;
+    for (int64_t e = (int64_t) blockIdx.x * LB_THREADS + threadIdx.x; e < n; e += stride) {
+        typename PE::In w;
+        pe.load (b, e, w);
+        pe.add (b, e, b.geval[e], w, v);
+    }
+    block_partials<PE::NC> (v, b.part, gridDim.x);
+}
+
+__device__ __forceinline__ double
+wave_sum_d (double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor (v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ long long
+wave_sum_i64 (long long v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor (v, o, 64);
+    return v;
+}
+
+// The MSE gradient reduce and the post pass in one launch, for a single-pair
+// MSE objective without regularization whose n elements take one thread
+// each in k_lb_post's grid (n <= lb_blocks (n) * 256).  Thread e forms
+// gradient component e as k_reduce<true> does -- its 64 condense slots as
+// four runs of 16 summed in FP64 in slot order, the runs added in order,
+// times 2/N -- and hands it to PostElem as k_lb_post would from geval; N
+// and the score are k_reduce's too (every workgroup sums the overlap
+// counts, the first one the score partials: per tile in tile order when
+// the tile kernel took counted tiles, else per workgroup, thread t adding
+// t, t + 256, ... and the four wave sums in order).  The gradient, score,
+// N and every post partial are the bits of k_reduce followed by k_lb_post.
+template <int BPMAX>
+__global__ void __launch_bounds__ (LB_THREADS)
+k_lb_reduce_post (const PairDev* __restrict__ pairs, const double* __restrict__ part_score,
+    const int* __restrict__ part_count, int nparts, int tile_scores, EvalOut out0,
+    const LbAct* __restrict__ ap, LbAct av, LbBufs b)
+{
+    const LbAct* act = ap ? ap : &av;
+    if (act->done) return;
+    __shared__ long long s_n[LB_THREADS / 64];
+    __shared__ double s_s[LB_THREADS / 64];
+    using PE = PostElem<BPMAX>;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const PairDev& pd = pairs[0];
+    const int64_t n = b.n;
+    const int64_t e = (int64_t) blockIdx.x * LB_THREADS + t;
+    const bool live = e < n;
+    // the 64 slot loads first, unconditional (the index stays in range)
+    const float* c = pd.cond + (live ? e : 0);
+    float sv[64];
+#pragma unroll
+    for (int m = 0; m < 64; m++) sv[m] = c[(int64_t) m * n];
+    // and the post pass's inputs of the element, in the same round
+    const PE pe (act, b);
+    typename PE::In w;
+    pe.load (b, live ? e : 0, w);
+    // N: the overlap counts, 16 bytes a lane where aligned
+    long long cnt = 0;
+    int t4 = 0;
+    if ((reinterpret_cast<uintptr_t> (part_count) & 15) == 0) {
+        const int4* pc4 = reinterpret_cast<const int4*> (part_count);
+        const int n4 = nparts >> 2;
+        for (int t0 = 0; t0 < n4; t0 += 2 * LB_THREADS) {
+            const int ta = t0 + t, tb = ta + LB_THREADS;
+            const int4 a = ta < n4 ? pc4[ta] : make_int4 (0, 0, 0, 0);
+            const int4 q = tb < n4 ? pc4[tb] : make_int4 (0, 0, 0, 0);
+            cnt += ((long long) a.x + a.y) + ((long long) a.z + a.w)
+                + ((long long) q.x + q.y) + ((long long) q.z + q.w);
+        }
+        t4 = 4 * n4;
+    }
+    for (int k = t4 + t; k < nparts; k += LB_THREADS) cnt += part_count[k];
+    double sc = 0.0;
+    if (blockIdx.x == 0) {
+        const double* ps = tile_scores ? part_score + pd.tile0 : part_score;
+        const int np = tile_scores ? pd.g.num_tiles : nparts;
+        for (int k = t; k < np; k += LB_THREADS) sc += ps[k];
+    }
+    cnt = wave_sum_i64 (cnt);
+    sc = wave_sum_d (sc);
+    if (lane == 0) { s_n[wave] = cnt; s_s[wave] = sc; }
+    __syncthreads ();
+    long long Nv = 0;
+    double Sv = 0.0;
+    for (int w = 0; w < LB_THREADS / 64; w++) { Nv += s_n[w]; Sv += s_s[w]; }
+    const double scale = Nv >= 1 ? 2.0 / (double) Nv : 0.0;
+    if (blockIdx.x == 0 && t == 0) {
+        *out0.score = Nv >= 1 ? Sv / (double) Nv : (double) FLT_MAX;
+        *out0.nvox = Nv;
+    }
+    double a[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 0; m < 16; m++) acc += (double) sv[16 * q + m];
+        a[q] = acc;
+    }
+    const double ge = (((a[0] + a[1]) + a[2]) + a[3]) * scale;
+    double v[PE::NC];
+#pragma unroll
+    for (int k = 0; k < PE::NC; ++k) v[k] = 0.0;
+    if (live) pe.add (b, e, ge, w, v);
+    block_partials<PE::NC> (v, b.part, gridDim.x);
 }
 
 // One workgroup: the totals -- the score slots, the slope (sum of the nb
@@ -309,6 +474,32 @@ lb_launch_post (const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t 
     case 13: return launch_post_m<13> (act, av, b, st);
     case 14: return launch_post_m<14> (act, av, b, st);
     case 15: return launch_post_m<15> (act, av, b, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int BPMAX>
+hipError_t
+launch_reduce_post_m (const PairDev* pairs, const double* part_score, const int* part_count, int nparts,
+    int tile_scores, const EvalOut& out0, const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t st)
+{
+    hipLaunchKernelGGL (k_lb_reduce_post<BPMAX>, dim3 (b.nb), dim3 (LB_THREADS), 0, st, pairs, part_score,
+        part_count, nparts, tile_scores, out0, act, av, b);
+    return hipGetLastError ();
+}
+
+hipError_t
+lb_launch_reduce_post (const PairDev* pairs, const double* part_score, const int* part_count, int nparts,
+    int tile_scores, const EvalOut& out0, const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t st)
+{
+    if ((int64_t) b.nb * LB_THREADS < b.n || !out0.score || !out0.nvox) return hipErrorInvalidValue;
+    switch (b.m - 1) {
+    case 0: return launch_reduce_post_m<0> (pairs, part_score, part_count, nparts, tile_scores, out0, act, av, b, st);
+    case 1: return launch_reduce_post_m<1> (pairs, part_score, part_count, nparts, tile_scores, out0, act, av, b, st);
+    case 2: return launch_reduce_post_m<2> (pairs, part_score, part_count, nparts, tile_scores, out0, act, av, b, st);
+    case 3: return launch_reduce_post_m<3> (pairs, part_score, part_count, nparts, tile_scores, out0, act, av, b, st);
+    case 4: return launch_reduce_post_m<4> (pairs, part_score, part_count, nparts, tile_scores, out0, act, av, b, st);
+    case 5: return launch_reduce_post_m<5> (pairs, part_score, part_count, nparts, tile_scores, out0, act, av, b, st);
     default: return hipErrorInvalidValue;
     }
 }
