@@ -1,7 +1,8 @@
 // Vector kernels of the device-resident L-BFGS: the vector work of
 // liblbfgs's lbfgs() (libs/liblbfgs-1.9/lib/lbfgs.c:449-609) and of its
-// More-Thuente trials (:903-906), in double, and the pass that runs the
-// scalar logic on the device.
+// More-Thuente trials (:903-906), in double, the MSE gradient reduce with
+// that work folded in, and the pass that runs the scalar logic on the
+// device.
 //
 // Built with -ffp-contract=off: each element update rounds its product and
 // its sum separately, as liblbfgs's vecadd / vecdiff do.
