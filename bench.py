@@ -833,7 +833,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU split of this path is described in DESIGN.md)", file=sys.stderr)
+              "(across GPUs this path would split the tiles into slabs of z-planes, one slab a GPU, "
+              "and add one FP64 all-reduce of the gradient per evaluation)", file=sys.stderr)
         sys.exit(2)
 
     import torch
