@@ -73,6 +73,7 @@ block_partials (const double (&v)[NC], double* __restrict__ out, int nb)
 // for, its first trial xp + stp d and the partials of g.d; 2 a trial
 // xp + stp d along the current d.  x and the float coefficients are
 // written.
+template <int MMAX>
 __global__ void __launch_bounds__ (LB_THREADS)
 k_lb_next (const LbAct* __restrict__ ap, LbAct av, LbBufs b)
 {
@@ -98,11 +99,17 @@ k_lb_next (const LbAct* __restrict__ ap, LbAct av, LbBufs b)
             if (nd == 0) {
                 de = -ge;
             } else {
+                // nd <= MMAX (the run's m): the bound is a constant, so the
+                // action's slots and coefficients and every history load
+                // are issued before the sums start
                 double t1 = 0.0, t2 = 0.0;
-                for (int i = 0; i < nd; ++i) {
-                    const int64_t off = (int64_t) act->slot[i] * n + e;
-                    t1 = t1 + act->ca[i] * b.S[off];
-                    t2 = t2 + act->cu[i] * b.Y[off];
+#pragma unroll
+                for (int i = 0; i < MMAX; ++i) {
+                    if (i < nd) {
+                        const int64_t off = (int64_t) act->slot[i] * n + e;
+                        t1 = t1 + act->ca[i] * b.S[off];
+                        t2 = t2 + act->cu[i] * b.Y[off];
+                    }
                 }
                 de = -gamma * ge - t1 + gamma * t2;
             }
@@ -448,11 +455,36 @@ launch_post_m (const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t s
 
 } // namespace
 
+template <int MMAX>
+hipError_t
+launch_next_m (const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t st)
+{
+    hipLaunchKernelGGL (k_lb_next<MMAX>, dim3 (b.nb), dim3 (LB_THREADS), 0, st, act, av, b);
+    return hipGetLastError ();
+}
+
 hipError_t
 lb_launch_next (const LbAct* act, const LbAct& av, const LbBufs& b, hipStream_t st)
 {
-    hipLaunchKernelGGL (k_lb_next, dim3 (b.nb), dim3 (LB_THREADS), 0, st, act, av, b);
-    return hipGetLastError ();
+    switch (b.m) {
+    case 1: return launch_next_m<1> (act, av, b, st);
+    case 2: return launch_next_m<2> (act, av, b, st);
+    case 3: return launch_next_m<3> (act, av, b, st);
+    case 4: return launch_next_m<4> (act, av, b, st);
+    case 5: return launch_next_m<5> (act, av, b, st);
+    case 6: return launch_next_m<6> (act, av, b, st);
+    case 7: return launch_next_m<7> (act, av, b, st);
+    case 8: return launch_next_m<8> (act, av, b, st);
+    case 9: return launch_next_m<9> (act, av, b, st);
+    case 10: return launch_next_m<10> (act, av, b, st);
+    case 11: return launch_next_m<11> (act, av, b, st);
+    case 12: return launch_next_m<12> (act, av, b, st);
+    case 13: return launch_next_m<13> (act, av, b, st);
+    case 14: return launch_next_m<14> (act, av, b, st);
+    case 15: return launch_next_m<15> (act, av, b, st);
+    case 16: return launch_next_m<16> (act, av, b, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t
